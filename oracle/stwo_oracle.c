@@ -1,0 +1,1349 @@
+/*
+ * stwo_oracle.c -- TEST INFRASTRUCTURE ONLY (the checker, never the product).
+ *
+ * Plain-C restatement of the reference CPU backend's wide-Fibonacci proving
+ * path (welttowelt/stwo-zig, pinned upstream Stwo a8fcf4bd).  Loaded only by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ *
+ * Pinning (see tests/test_oracle_*.py):
+ *   - every function below is exercised by the end-to-end byte equality of the
+ *     six committed golden proofs (5x16 pow0, 10x8, 14x32, 16x64, 18x100, 20x100);
+ *   - fields: closed forms / Python big-int brute force;
+ *   - Blake2s: hashlib.blake2s and the four Merkle KATs of
+ *     src/core/vcs_lifted/blake2_merkle.zig:340-384;
+ *   - FFT: interpolate(evaluate(x)) == x and equality with direct O(n^2)
+ *     evaluation of the circle basis at domain points;
+ *   - FRI folds: a folded low-degree evaluation stays low degree.
+ * Threading: OpenMP over columns / rows; results are order independent.
+ */
+#include "stwo_oracle.h"
+
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define P ORA_P
+
+/* ======================================================================
+ * M31 (src/core/fields/m31.zig:10-171)
+ * ==================================================================== */
+static inline uint32_t m_add(uint32_t a, uint32_t b) {
+    uint32_t s = a + b;
+    return s >= P ? s - P : s;
+}
+static inline uint32_t m_sub(uint32_t a, uint32_t b) { return a >= b ? a - b : a + P - b; }
+static inline uint32_t m_neg(uint32_t a) { return a == 0 ? 0 : P - a; }
+/* reduceProduct (m31.zig:164-171): one Mersenne fold + conditional subtract. */
+static inline uint32_t m_mul(uint32_t a, uint32_t b) {
+    uint64_t x = (uint64_t)a * b;
+    uint32_t r = (uint32_t)((x & P) + (x >> 31));
+    return r >= P ? r - P : r;
+}
+/* reduce64 (m31.zig:150-158). */
+static inline uint32_t m_from_u64(uint64_t x) {
+    uint64_t t = (x & P) + (x >> 31);
+    t = (t & P) + (t >> 31);
+    uint32_t r = (uint32_t)t;
+    uint32_t r2 = r - P;
+    return r < r2 ? r : r2;
+}
+static uint32_t m_pow(uint32_t a, uint64_t e) {
+    uint32_t acc = 1;
+    while (e) {
+        if (e & 1) acc = m_mul(acc, a);
+        a = m_mul(a, a);
+        e >>= 1;
+    }
+    return acc;
+}
+/* invUncheckedNonZero = a^(p-2) (m31.zig:97-100). */
+static inline uint32_t m_inv(uint32_t a) { return m_pow(a, P - 2); }
+
+uint32_t ora_m31_mul(uint32_t a, uint32_t b) { return m_mul(a, b); }
+uint32_t ora_m31_inv(uint32_t a) { return m_inv(a); }
+
+/* ======================================================================
+ * CM31 = M31[i]/(i^2+1) (src/core/fields/cm31.zig:97-129)
+ * ==================================================================== */
+typedef struct { uint32_t a, b; } cm31;
+static inline cm31 c_add(cm31 x, cm31 y) { return (cm31){m_add(x.a, y.a), m_add(x.b, y.b)}; }
+static inline cm31 c_sub(cm31 x, cm31 y) { return (cm31){m_sub(x.a, y.a), m_sub(x.b, y.b)}; }
+static inline cm31 c_neg(cm31 x) { return (cm31){m_neg(x.a), m_neg(x.b)}; }
+static inline cm31 c_mul(cm31 x, cm31 y) {
+    uint32_t ac = m_mul(x.a, y.a), bd = m_mul(x.b, y.b);
+    uint32_t cross = m_mul(m_add(x.a, x.b), m_add(y.a, y.b));
+    return (cm31){m_sub(ac, bd), m_sub(m_sub(cross, ac), bd)};
+}
+static inline cm31 c_mul_m(cm31 x, uint32_t m) { return (cm31){m_mul(x.a, m), m_mul(x.b, m)}; }
+static inline cm31 c_square(cm31 x) { return c_mul(x, x); }
+static inline int c_is_zero(cm31 x) { return x.a == 0 && x.b == 0; }
+/* (a+bi)^-1 = (a-bi)/(a^2+b^2) (cm31.zig inv). */
+static inline cm31 c_inv(cm31 x) {
+    uint32_t d = m_inv(m_add(m_mul(x.a, x.a), m_mul(x.b, x.b)));
+    return (cm31){m_mul(x.a, d), m_mul(m_neg(x.b), d)};
+}
+/* mulByR: (a+bi)(2+i) = (2a-b) + (a+2b)i (qm31.zig:269-277). */
+static inline cm31 c_mul_r(cm31 v) {
+    return (cm31){m_sub(m_add(v.a, v.a), v.b), m_add(v.a, m_add(v.b, v.b))};
+}
+
+/* ======================================================================
+ * QM31 = CM31[u]/(u^2 - (2+i)) (src/core/fields/qm31.zig:13-222)
+ * words (c0.a, c0.b, c1.a, c1.b) (qm31.zig:64-70)
+ * ==================================================================== */
+typedef struct { cm31 c0, c1; } qm31;
+static const qm31 Q_ZERO = {{0, 0}, {0, 0}};
+static const qm31 Q_ONE = {{1, 0}, {0, 0}};
+static inline qm31 q_from_m(uint32_t m) { return (qm31){{m, 0}, {0, 0}}; }
+static inline qm31 q_add(qm31 x, qm31 y) { return (qm31){c_add(x.c0, y.c0), c_add(x.c1, y.c1)}; }
+static inline qm31 q_sub(qm31 x, qm31 y) { return (qm31){c_sub(x.c0, y.c0), c_sub(x.c1, y.c1)}; }
+static inline qm31 q_neg(qm31 x) { return (qm31){c_neg(x.c0), c_neg(x.c1)}; }
+static inline qm31 q_mul(qm31 x, qm31 y) {
+    cm31 ac = c_mul(x.c0, y.c0), bd = c_mul(x.c1, y.c1);
+    cm31 cross = c_sub(c_sub(c_mul(c_add(x.c0, x.c1), c_add(y.c0, y.c1)), ac), bd);
+    return (qm31){c_add(ac, c_mul_r(bd)), cross};
+}
+static inline qm31 q_mul_m(qm31 x, uint32_t m) { return (qm31){c_mul_m(x.c0, m), c_mul_m(x.c1, m)}; }
+static inline qm31 q_mul_c(qm31 x, cm31 c) { return (qm31){c_mul(x.c0, c), c_mul(x.c1, c)}; }
+static inline qm31 q_square(qm31 x) { return q_mul(x, x); }
+static inline int q_eq(qm31 x, qm31 y) {
+    return x.c0.a == y.c0.a && x.c0.b == y.c0.b && x.c1.a == y.c1.a && x.c1.b == y.c1.b;
+}
+static inline int q_is_zero(qm31 x) { return c_is_zero(x.c0) && c_is_zero(x.c1); }
+/* (a+bu)^-1 = (a-bu)/(a^2-(2+i)b^2) (qm31.zig:182-199). */
+static qm31 q_inv(qm31 x) {
+    cm31 b2 = c_square(x.c1);
+    cm31 rb2 = c_add(c_add(b2, b2), (cm31){m_neg(b2.b), b2.a});
+    cm31 denom = c_sub(c_square(x.c0), rb2);
+    cm31 di = c_inv(denom);
+    return (qm31){c_mul(x.c0, di), c_mul(c_neg(x.c1), di)};
+}
+/* complexConjugate: (c0, -c1) (qm31.zig:205-210). */
+static inline qm31 q_conj(qm31 x) { return (qm31){x.c0, c_neg(x.c1)}; }
+static inline void q_store(qm31 x, uint32_t *w) {
+    w[0] = x.c0.a; w[1] = x.c0.b; w[2] = x.c1.a; w[3] = x.c1.b;
+}
+static inline qm31 q_load(const uint32_t *w) { return (qm31){{w[0], w[1]}, {w[2], w[3]}}; }
+
+void ora_qm31_mul(const uint32_t a[4], const uint32_t b[4], uint32_t out[4]) {
+    q_store(q_mul(q_load(a), q_load(b)), out);
+}
+void ora_qm31_inv(const uint32_t a[4], uint32_t out[4]) { q_store(q_inv(q_load(a)), out); }
+
+/* ======================================================================
+ * Circle group (src/core/circle.zig:8-374)
+ * ==================================================================== */
+typedef struct { uint32_t x, y; } pt;
+typedef struct { qm31 x, y; } qpt;
+#define LOG_ORDER 31
+#define ORDER_MASK 0x7fffffffu
+
+static inline pt p_add(pt a, pt b) {
+    return (pt){m_sub(m_mul(a.x, b.x), m_mul(a.y, b.y)), m_add(m_mul(a.x, b.y), m_mul(a.y, b.x))};
+}
+static inline pt p_conj(pt a) { return (pt){a.x, m_neg(a.y)}; }
+static inline qpt qp_add(qpt a, qpt b) {
+    return (qpt){q_sub(q_mul(a.x, b.x), q_mul(a.y, b.y)), q_add(q_mul(a.x, b.y), q_mul(a.y, b.x))};
+}
+static inline qm31 q_double_x(qm31 x) {
+    qm31 s = q_square(x);
+    return q_sub(q_add(s, s), Q_ONE);
+}
+static inline uint32_t m_double_x(uint32_t x) {
+    uint32_t s = m_mul(x, x);
+    return m_sub(m_add(s, s), 1);
+}
+
+static pt GEN_DOUBLES[LOG_ORDER];
+static int g_init_done = 0;
+static void init_gen_doubles(void) {
+    if (g_init_done) return;
+    /* M31_CIRCLE_GEN = (2, 1268011823), log order 31 (circle.zig:137-143). */
+    pt cur = {2, 1268011823u};
+    for (int i = 0; i < LOG_ORDER; i++) {
+        GEN_DOUBLES[i] = cur;
+        cur = p_add(cur, cur);
+    }
+    g_init_done = 1;
+}
+/* CirclePointIndex.toPoint (circle.zig:187-196). */
+static pt idx_to_point(uint32_t idx) {
+    init_gen_doubles();
+    pt acc = {1, 0};
+    uint32_t bits = idx & ORDER_MASK;
+    while (bits) {
+        int b = __builtin_ctz(bits);
+        acc = p_add(acc, GEN_DOUBLES[b]);
+        bits &= bits - 1;
+    }
+    return acc;
+}
+static inline uint32_t idx_neg(uint32_t i) { return (0x80000000u - (i & ORDER_MASK)) & ORDER_MASK; }
+static inline uint32_t subgroup_gen(uint32_t log_size) { return 1u << (LOG_ORDER - log_size); }
+
+/* Coset (circle.zig:245-374). */
+typedef struct { uint32_t initial_index, step_size, log_size; } coset;
+static inline coset coset_new(uint32_t initial, uint32_t log_size) {
+    return (coset){initial & ORDER_MASK, subgroup_gen(log_size), log_size};
+}
+static inline coset coset_half_odds(uint32_t log_size) { return coset_new(subgroup_gen(log_size + 2), log_size); }
+static inline coset coset_odds(uint32_t log_size) { return coset_new(subgroup_gen(log_size + 1), log_size); }
+static inline coset coset_double(coset c) {
+    return (coset){(c.initial_index * 2u) & ORDER_MASK, (c.step_size * 2u) & ORDER_MASK, c.log_size - 1};
+}
+static inline uint32_t coset_index_at(coset c, uint64_t i) {
+    return (uint32_t)((c.initial_index + (uint64_t)c.step_size * i) & ORDER_MASK);
+}
+static inline pt coset_at(coset c, uint64_t i) { return idx_to_point(coset_index_at(c, i)); }
+
+/* CanonicCoset(log).circleDomain() = CircleDomain(halfOdds(log-1)) (canonic.zig:9-28);
+ * CircleDomain.indexAt (domain.zig:58-62). */
+static inline uint32_t domain_index_at(uint32_t log_size, uint64_t i) {
+    coset hc = coset_half_odds(log_size - 1);
+    uint64_t half = 1ull << (log_size - 1);
+    if (i < half) return coset_index_at(hc, i);
+    return idx_neg(coset_index_at(hc, i - half));
+}
+static inline pt domain_at(uint32_t log_size, uint64_t i) { return idx_to_point(domain_index_at(log_size, i)); }
+
+void ora_circle_point_at_index(uint32_t idx, uint32_t out[2]) {
+    pt p = idx_to_point(idx);
+    out[0] = p.x; out[1] = p.y;
+}
+void ora_circle_domain_point(uint32_t log_size, uint64_t i, uint32_t out[2]) {
+    pt p = domain_at(log_size, i);
+    out[0] = p.x; out[1] = p.y;
+}
+
+/* bitReverseIndex (core/utils.zig:14-18). */
+static inline uint64_t bitrev(uint64_t i, uint32_t log_size) {
+    if (log_size == 0) return i;
+    uint64_t r = 0;
+    for (uint32_t k = 0; k < log_size; k++) r |= ((i >> k) & 1ull) << (log_size - 1 - k);
+    return r;
+}
+static void bitrev_q(qm31 *v, uint64_t n) {
+    uint32_t lg = 0;
+    while ((1ull << lg) < n) lg++;
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t j = bitrev(i, lg);
+        if (j > i) { qm31 t = v[i]; v[i] = v[j]; v[j] = t; }
+    }
+}
+
+/* ======================================================================
+ * Twiddles (src/prover/poly/twiddles.zig:37-91)
+ * ==================================================================== */
+void ora_twiddles(uint32_t circle_log, uint32_t *tw, uint32_t *itw) {
+    coset c = coset_half_odds(circle_log - 1); /* CanonicCoset(log).circleDomain().half_coset */
+    uint64_t at = 0;
+    for (uint32_t i = 0; i < circle_log - 1; i++) {
+        uint64_t layer_len = (1ull << c.log_size) / 2;
+        uint32_t lg = c.log_size - 1;
+        pt cur = idx_to_point(c.initial_index), step = idx_to_point(c.step_size);
+        for (uint64_t j = 0; j < layer_len; j++) {
+            tw[at + bitrev(j, lg)] = cur.x;
+            cur = p_add(cur, step);
+        }
+        at += layer_len;
+        c = coset_double(c);
+    }
+    tw[at] = 1;
+    if (itw)
+        for (uint64_t j = 0; j <= at; j++) itw[j] = m_inv(tw[j]);
+}
+
+/* ======================================================================
+ * Circle FFT (src/prover/poly/circle/transforms.zig:149-360; core/fft.zig:8-18)
+ * Values are stored in bit-reversed circle-domain order.
+ * ==================================================================== */
+static inline void bfly(uint32_t *v0, uint32_t *v1, uint32_t t) {
+    uint32_t tmp = m_mul(*v1, t);
+    *v1 = m_sub(*v0, tmp);
+    *v0 = m_add(*v0, tmp);
+}
+static inline void ibfly(uint32_t *v0, uint32_t *v1, uint32_t t) {
+    uint32_t tmp = *v0;
+    *v0 = m_add(tmp, *v1);
+    *v1 = m_mul(m_sub(tmp, *v1), t);
+}
+
+typedef struct { uint32_t log; uint32_t *tw, *itw; } twiddle_tree;
+static twiddle_tree tw_cache[32];
+static twiddle_tree *get_twiddles(uint32_t log) {
+    twiddle_tree *t = &tw_cache[log];
+    if (!t->tw) {
+        uint64_t n = 1ull << (log - 1);
+        uint32_t *tw = malloc(n * 4), *itw = malloc(n * 4);
+        ora_twiddles(log, tw, itw);
+        t->log = log;
+        t->itw = itw;
+        __atomic_store_n(&t->tw, tw, __ATOMIC_RELEASE);
+    }
+    return t;
+}
+/* Layer j (stage j+1) slice: twiddles[len_total - 2^(m-1-j) .. ] of length 2^(m-2-j). */
+static inline const uint32_t *tw_layer(const uint32_t *tw, uint32_t m, uint32_t j) {
+    return tw + ((1ull << (m - 1)) - (1ull << (m - 1 - j)));
+}
+
+/* evaluateBufferWithTwiddles (transforms.zig:149-246). */
+static void circle_evaluate(uint32_t *v, uint32_t m) {
+    if (m == 1) {
+        pt p0 = idx_to_point(coset_half_odds(0).initial_index);
+        bfly(&v[0], &v[1], p0.y);
+        return;
+    }
+    if (m == 2) {
+        pt p0 = idx_to_point(coset_half_odds(1).initial_index);
+        bfly(&v[0], &v[2], p0.x);
+        bfly(&v[1], &v[3], p0.x);
+        bfly(&v[0], &v[1], p0.y);
+        bfly(&v[2], &v[3], m_neg(p0.y));
+        return;
+    }
+    const uint32_t *tw = get_twiddles(m)->tw;
+    for (uint32_t s = m - 1; s >= 1; s--) {
+        const uint32_t *T = tw_layer(tw, m, s - 1);
+        uint64_t nblocks = 1ull << (m - 1 - s), half = 1ull << s;
+        for (uint64_t h = 0; h < nblocks; h++) {
+            uint32_t t = T[h];
+            uint32_t *b = v + (h << (s + 1));
+            for (uint64_t l = 0; l < half; l++) bfly(&b[l], &b[l + half], t);
+        }
+    }
+    const uint32_t *T0 = tw_layer(tw, m, 0);
+    for (uint64_t k = 0; k < (1ull << (m - 3)); k++) {
+        uint32_t x = T0[2 * k], y = T0[2 * k + 1];
+        uint32_t *b = v + 8 * k;
+        bfly(&b[0], &b[1], y);
+        bfly(&b[2], &b[3], m_neg(y));
+        bfly(&b[4], &b[5], m_neg(x));
+        bfly(&b[6], &b[7], x);
+    }
+}
+
+/* interpolateIntoBufferWithTwiddles (transforms.zig:249-360). */
+static void circle_interpolate(uint32_t *v, uint32_t m) {
+    if (m == 1) {
+        pt p0 = idx_to_point(coset_half_odds(0).initial_index);
+        uint32_t y = p0.y;
+        uint32_t yn_inv = m_inv(m_mul(y, 2));
+        uint32_t y_inv = m_mul(yn_inv, 2), n_inv = m_mul(yn_inv, y);
+        ibfly(&v[0], &v[1], y_inv);
+        v[0] = m_mul(v[0], n_inv);
+        v[1] = m_mul(v[1], n_inv);
+        return;
+    }
+    if (m == 2) {
+        pt p0 = idx_to_point(coset_half_odds(1).initial_index);
+        uint32_t x = p0.x, y = p0.y;
+        uint32_t xyn_inv = m_inv(m_mul(m_mul(x, y), 4));
+        uint32_t x_inv = m_mul(m_mul(xyn_inv, y), 4), y_inv = m_mul(m_mul(xyn_inv, x), 4);
+        uint32_t n_inv = m_mul(m_mul(xyn_inv, x), y);
+        ibfly(&v[0], &v[1], y_inv);
+        ibfly(&v[2], &v[3], m_neg(y_inv));
+        ibfly(&v[0], &v[2], x_inv);
+        ibfly(&v[1], &v[3], x_inv);
+        for (int i = 0; i < 4; i++) v[i] = m_mul(v[i], n_inv);
+        return;
+    }
+    const uint32_t *itw = get_twiddles(m)->itw;
+    const uint32_t *T0 = tw_layer(itw, m, 0);
+    for (uint64_t k = 0; k < (1ull << (m - 3)); k++) {
+        uint32_t x = T0[2 * k], y = T0[2 * k + 1];
+        uint32_t *b = v + 8 * k;
+        ibfly(&b[0], &b[1], y);
+        ibfly(&b[2], &b[3], m_neg(y));
+        ibfly(&b[4], &b[5], m_neg(x));
+        ibfly(&b[6], &b[7], x);
+    }
+    for (uint32_t s = 1; s < m; s++) {
+        const uint32_t *T = tw_layer(itw, m, s - 1);
+        uint64_t nblocks = 1ull << (m - 1 - s), half = 1ull << s;
+        for (uint64_t h = 0; h < nblocks; h++) {
+            uint32_t t = T[h];
+            uint32_t *b = v + (h << (s + 1));
+            for (uint64_t l = 0; l < half; l++) ibfly(&b[l], &b[l + half], t);
+        }
+    }
+    uint32_t n_inv = m_inv((uint32_t)((1ull << m) % P));
+    for (uint64_t i = 0; i < (1ull << m); i++) v[i] = m_mul(v[i], n_inv);
+}
+
+void ora_circle_evaluate(uint32_t *values, uint32_t log_size) { circle_evaluate(values, log_size); }
+void ora_circle_interpolate(uint32_t *values, uint32_t log_size) { circle_interpolate(values, log_size); }
+/* evaluateExtensionBufferWithTwiddles (transforms.zig:132-147): zero padding. */
+void ora_circle_lde(const uint32_t *coeffs, uint32_t log_size, uint32_t log_blowup, uint32_t *out) {
+    uint64_t n = 1ull << log_size, N = n << log_blowup;
+    memcpy(out, coeffs, n * 4);
+    memset(out + n, 0, (N - n) * 4);
+    circle_evaluate(out, log_size + log_blowup);
+}
+
+/* ======================================================================
+ * Point evaluation (point_evaluation.zig:14-38,469-488)
+ * p(P) = sum_i c_i * prod_{bit j of i} f_j, f = [P.y, P.x, 2x^2-1, ...].
+ * Evaluated through the weight vector w_i = prod f_j^{bit_j(i)}.
+ * ==================================================================== */
+static qm31 *eval_weights(qpt point, uint32_t m) {
+    qm31 f[32];
+    f[0] = point.y;
+    if (m > 1) {
+        f[1] = point.x;
+        for (uint32_t k = 2; k < m; k++) f[k] = q_double_x(f[k - 1]);
+    }
+    uint64_t n = 1ull << m;
+    qm31 *w = malloc(n * sizeof(qm31));
+    w[0] = Q_ONE;
+    for (uint32_t k = 0; k < m; k++) {
+        uint64_t half = 1ull << k;
+        for (uint64_t i = 0; i < half; i++) w[half + i] = q_mul(w[i], f[k]);
+    }
+    return w;
+}
+static qm31 dot_weights(const uint32_t *c, const qm31 *w, uint64_t n) {
+    uint64_t acc[4] = {0, 0, 0, 0};
+    qm31 r = Q_ZERO;
+    for (uint64_t i = 0; i < n; i++) {
+        acc[0] += (uint64_t)c[i] * w[i].c0.a;
+        acc[1] += (uint64_t)c[i] * w[i].c0.b;
+        acc[2] += (uint64_t)c[i] * w[i].c1.a;
+        acc[3] += (uint64_t)c[i] * w[i].c1.b;
+        if ((i & 3) == 3) {
+            r = q_add(r, (qm31){{m_from_u64(acc[0]), m_from_u64(acc[1])}, {m_from_u64(acc[2]), m_from_u64(acc[3])}});
+            acc[0] = acc[1] = acc[2] = acc[3] = 0;
+        }
+    }
+    r = q_add(r, (qm31){{m_from_u64(acc[0]), m_from_u64(acc[1])}, {m_from_u64(acc[2]), m_from_u64(acc[3])}});
+    return r;
+}
+void ora_eval_at_point(const uint32_t *coeffs, uint32_t log_size, const uint32_t point[8], uint32_t out[4]) {
+    qpt P_ = {q_load(point), q_load(point + 4)};
+    if (log_size == 0) {
+        q_store(q_from_m(coeffs[0]), out);
+        return;
+    }
+    qm31 *w = eval_weights(P_, log_size);
+    q_store(dot_weights(coeffs, w, 1ull << log_size), out);
+    free(w);
+}
+
+/* ======================================================================
+ * Blake2s-256, unkeyed (RFC 7693; src/core/crypto/blake2s_backend.zig)
+ * ==================================================================== */
+static const uint32_t B2_IV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                  0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+static const uint8_t B2_SIGMA[10][16] = {
+    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15}, {14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3},
+    {11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4}, {7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8},
+    {9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13}, {2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9},
+    {12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11}, {13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10},
+    {6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5}, {10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0}};
+static inline uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+#define B2G(a, b, c, d, x, y)            \
+    do {                                 \
+        a = a + b + x; d = rotr(d ^ a, 16); \
+        c = c + d; b = rotr(b ^ c, 12);  \
+        a = a + b + y; d = rotr(d ^ a, 8);  \
+        c = c + d; b = rotr(b ^ c, 7);   \
+    } while (0)
+static void b2_compress(uint32_t h[8], const uint32_t m[16], uint64_t t, int last) {
+    uint32_t v[16];
+    for (int i = 0; i < 8; i++) { v[i] = h[i]; v[i + 8] = B2_IV[i]; }
+    v[12] ^= (uint32_t)t;
+    v[13] ^= (uint32_t)(t >> 32);
+    if (last) v[14] = ~v[14];
+    for (int r = 0; r < 10; r++) {
+        const uint8_t *s = B2_SIGMA[r];
+        B2G(v[0], v[4], v[8], v[12], m[s[0]], m[s[1]]);
+        B2G(v[1], v[5], v[9], v[13], m[s[2]], m[s[3]]);
+        B2G(v[2], v[6], v[10], v[14], m[s[4]], m[s[5]]);
+        B2G(v[3], v[7], v[11], v[15], m[s[6]], m[s[7]]);
+        B2G(v[0], v[5], v[10], v[15], m[s[8]], m[s[9]]);
+        B2G(v[1], v[6], v[11], v[12], m[s[10]], m[s[11]]);
+        B2G(v[2], v[7], v[8], v[13], m[s[12]], m[s[13]]);
+        B2G(v[3], v[4], v[9], v[14], m[s[14]], m[s[15]]);
+    }
+    for (int i = 0; i < 8; i++) h[i] ^= v[i] ^ v[i + 8];
+}
+typedef struct { uint32_t h[8]; uint8_t buf[64]; uint32_t buflen; uint64_t t; } b2s;
+static void b2_init(b2s *s) {
+    memcpy(s->h, B2_IV, 32);
+    s->h[0] ^= 0x01010020u; /* digest 32, no key, fanout 1, depth 1 */
+    s->buflen = 0;
+    s->t = 0;
+}
+static void b2_update(b2s *s, const uint8_t *d, size_t n) {
+    while (n > 0) {
+        if (s->buflen == 64) {
+            s->t += 64;
+            uint32_t m[16];
+            memcpy(m, s->buf, 64);
+            b2_compress(s->h, m, s->t, 0);
+            s->buflen = 0;
+        }
+        size_t take = 64 - s->buflen;
+        if (take > n) take = n;
+        memcpy(s->buf + s->buflen, d, take);
+        s->buflen += (uint32_t)take;
+        d += take;
+        n -= take;
+    }
+}
+static void b2_final(b2s *s, uint8_t out[32]) {
+    s->t += s->buflen;
+    memset(s->buf + s->buflen, 0, 64 - s->buflen);
+    uint32_t m[16];
+    memcpy(m, s->buf, 64);
+    b2_compress(s->h, m, s->t, 1);
+    memcpy(out, s->h, 32);
+}
+void ora_blake2s(const uint8_t *data, size_t len, uint8_t out[32]) {
+    b2s s;
+    b2_init(&s);
+    b2_update(&s, data, len);
+    b2_final(&s, out);
+}
+
+/* ======================================================================
+ * Lifted Blake2s Merkle (src/prover/vcs_lifted/{prover,columns,leaves,layers}.zig,
+ * src/core/vcs_lifted/blake2_merkle.zig:9-262)
+ * leaf[p] = H("leaf"+0^60 || LE32(col[lift(p)]) for cols sorted by (log asc, index))
+ * node    = H("node"+0^60 || left || right)
+ * ==================================================================== */
+static uint8_t LEAF_PREFIX[64] = {'l', 'e', 'a', 'f'};
+static uint8_t NODE_PREFIX[64] = {'n', 'o', 'd', 'e'};
+
+struct ora_tree {
+    uint32_t max_log;
+    uint8_t **layers; /* layers[k]: 2^k hashes, root first (prover.zig:23) */
+};
+
+typedef struct { const uint32_t *values; uint32_t log_size; uint32_t index; } colref;
+static int colref_cmp(const void *a, const void *b) {
+    const colref *x = a, *y = b;
+    if (x->log_size != y->log_size) return x->log_size < y->log_size ? -1 : 1;
+    return x->index < y->index ? -1 : (x->index > y->index);
+}
+/* Lifting index (leaves.zig:105-111; quotient_column_geometry.zig:38-42). */
+static inline uint64_t lift_index(uint64_t pos, uint32_t max_log, uint32_t log_size) {
+    uint32_t sh = max_log - log_size + 1;
+    return ((pos >> sh) << 1) + (pos & 1);
+}
+
+ora_tree *ora_merkle_commit(const uint32_t *const *cols, const uint32_t *log_sizes, uint32_t n_cols) {
+    colref *sorted = malloc(sizeof(colref) * (n_cols ? n_cols : 1));
+    for (uint32_t i = 0; i < n_cols; i++) sorted[i] = (colref){cols[i], log_sizes[i], i};
+    qsort(sorted, n_cols, sizeof(colref), colref_cmp); /* stable: ties broken by index */
+    uint32_t max_log = n_cols ? sorted[n_cols - 1].log_size : 0;
+    ora_tree *t = malloc(sizeof(ora_tree));
+    t->max_log = max_log;
+    t->layers = calloc(max_log + 1, sizeof(uint8_t *));
+    uint64_t n_leaves = 1ull << max_log;
+    uint8_t *leaves = malloc(n_leaves * 32);
+    b2s seed;
+    b2_init(&seed);
+    b2_update(&seed, LEAF_PREFIX, 64);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < (int64_t)n_leaves; p++) {
+        b2s s = seed;
+        for (uint32_t c = 0; c < n_cols; c++) {
+            uint32_t v = sorted[c].values[lift_index((uint64_t)p, max_log, sorted[c].log_size)];
+            b2_update(&s, (const uint8_t *)&v, 4);
+        }
+        b2_final(&s, leaves + 32 * p);
+    }
+    t->layers[max_log] = leaves;
+    b2s nseed;
+    b2_init(&nseed);
+    b2_update(&nseed, NODE_PREFIX, 64);
+    for (int32_t k = (int32_t)max_log - 1; k >= 0; k--) {
+        uint64_t n = 1ull << k;
+        uint8_t *layer = malloc(n * 32);
+        const uint8_t *child = t->layers[k + 1];
+#pragma omp parallel for schedule(static) if (n > 4096)
+        for (int64_t i = 0; i < (int64_t)n; i++) {
+            b2s s = nseed;
+            b2_update(&s, child + 64 * i, 64);
+            b2_final(&s, layer + 32 * i);
+        }
+        t->layers[k] = layer;
+    }
+    free(sorted);
+    return t;
+}
+void ora_tree_root(const ora_tree *t, uint8_t out[32]) { memcpy(out, t->layers[0], 32); }
+uint32_t ora_tree_max_log(const ora_tree *t) { return t->max_log; }
+void ora_tree_layer(const ora_tree *t, uint32_t layer_log, uint8_t *out) {
+    memcpy(out, t->layers[layer_log], 32ull << layer_log);
+}
+void ora_tree_free(ora_tree *t) {
+    if (!t) return;
+    for (uint32_t k = 0; k <= t->max_log; k++) free(t->layers[k]);
+    free(t->layers);
+    free(t);
+}
+
+/* ======================================================================
+ * Growable byte/word buffers for proof assembly.
+ * ==================================================================== */
+typedef struct { uint8_t *d; size_t n, cap; } bytes;
+static void by_push(bytes *b, const void *src, size_t n) {
+    if (b->n + n > b->cap) {
+        b->cap = (b->n + n) * 2 + 64;
+        b->d = realloc(b->d, b->cap);
+    }
+    memcpy(b->d + b->n, src, n);
+    b->n += n;
+}
+static void by_str(bytes *b, const char *s) { by_push(b, s, strlen(s)); }
+static void by_u64(bytes *b, uint64_t v) {
+    char tmp[32];
+    snprintf(tmp, sizeof tmp, "%llu", (unsigned long long)v);
+    by_str(b, tmp);
+}
+
+/* Decommitment (src/prover/vcs_lifted/decommit.zig:56-237): hash witness for
+ * sorted unique positions, queried values per column at lifted positions. */
+typedef struct { bytes witness; /* 32-byte hashes */ } decommitment;
+static void merkle_decommit_witness(const ora_tree *t, const uint64_t *pos, size_t npos, bytes *witness) {
+    uint64_t *prev = malloc(sizeof(uint64_t) * (npos + 1)), *cur = malloc(sizeof(uint64_t) * (npos + 1));
+    size_t nprev = 0;
+    for (size_t i = 0; i < npos; i++)
+        if (i == 0 || pos[i - 1] != pos[i]) prev[nprev++] = pos[i];
+    for (int32_t layer = (int32_t)t->max_log - 1; layer >= 0; layer--) {
+        const uint8_t *child = t->layers[layer + 1];
+        size_t ncur = 0, q = 0;
+        while (q < nprev) {
+            uint64_t first = prev[q];
+            int has_sib = q + 1 < nprev && (first ^ 1) == prev[q + 1];
+            if (!has_sib) by_push(witness, child + 32 * (first ^ 1), 32);
+            cur[ncur++] = first >> 1;
+            q += has_sib ? 2 : 1;
+        }
+        uint64_t *tmp = prev; prev = cur; cur = tmp;
+        nprev = ncur;
+    }
+    free(prev);
+    free(cur);
+}
+
+/* ======================================================================
+ * Blake2s channel (src/core/channel/blake2s.zig:32-287; mixRoot blake2_merkle.zig:237-249)
+ * ==================================================================== */
+typedef struct { uint8_t digest[32]; uint32_t n_draws; } channel;
+static void ch_update(channel *ch, const uint8_t d[32]) {
+    memcpy(ch->digest, d, 32);
+    ch->n_draws = 0;
+}
+static void ch_mix_bytes(channel *ch, const void *data, size_t n) {
+    b2s s;
+    b2_init(&s);
+    b2_update(&s, ch->digest, 32);
+    b2_update(&s, data, n);
+    uint8_t out[32];
+    b2_final(&s, out);
+    ch_update(ch, out);
+}
+static void ch_mix_felts(channel *ch, const qm31 *f, size_t n) {
+    uint32_t *w = malloc(16 * (n ? n : 1));
+    for (size_t i = 0; i < n; i++) q_store(f[i], w + 4 * i);
+    ch_mix_bytes(ch, w, 16 * n);
+    free(w);
+}
+static void ch_mix_u32s(channel *ch, const uint32_t *d, size_t n) { ch_mix_bytes(ch, d, 4 * n); }
+static void ch_mix_u64(channel *ch, uint64_t v) {
+    uint32_t w[2] = {(uint32_t)v, (uint32_t)(v >> 32)};
+    ch_mix_u32s(ch, w, 2);
+}
+static void ch_mix_root(channel *ch, const uint8_t root[32]) { ch_mix_bytes(ch, root, 32); }
+static void ch_draw_u32s(channel *ch, uint32_t out[8]) {
+    uint8_t in[37];
+    memcpy(in, ch->digest, 32);
+    memcpy(in + 32, &ch->n_draws, 4);
+    in[36] = 0;
+    ch->n_draws++;
+    uint8_t h[32];
+    ora_blake2s(in, 37, h);
+    memcpy(out, h, 32);
+}
+static void ch_draw_base_felts(channel *ch, uint32_t out[8]) {
+    for (;;) {
+        uint32_t w[8];
+        ch_draw_u32s(ch, w);
+        int ok = 1;
+        for (int i = 0; i < 8; i++)
+            if (w[i] >= 2 * P) ok = 0;
+        if (!ok) continue;
+        for (int i = 0; i < 8; i++) out[i] = m_from_u64(w[i]);
+        return;
+    }
+}
+static qm31 ch_draw_secure_felt(channel *ch) {
+    uint32_t f[8];
+    ch_draw_base_felts(ch, f);
+    return q_load(f);
+}
+/* PoW (blake2s.zig:135-166,289-300). */
+static int pow_check(const uint8_t prefix[32], uint64_t nonce, uint32_t bits) {
+    uint8_t in[40], h[32];
+    memcpy(in, prefix, 32);
+    memcpy(in + 32, &nonce, 8);
+    ora_blake2s(in, 40, h);
+    uint32_t count = 0;
+    for (int i = 0; i < 16; i++) {
+        if (h[i] == 0) { count += 8; continue; }
+        count += __builtin_ctz(h[i]);
+        break;
+    }
+    return count >= bits;
+}
+uint64_t ora_pow_grind(const uint8_t digest[32], uint32_t pow_bits) {
+    if (pow_bits == 0) return 0;
+    uint8_t in[52], prefix[32];
+    uint32_t pp = 0x12345678u;
+    memcpy(in, &pp, 4);
+    memset(in + 4, 0, 12);
+    memcpy(in + 16, digest, 32);
+    memcpy(in + 48, &pow_bits, 4);
+    ora_blake2s(in, 52, prefix);
+    /* Lowest valid nonce: search ascending chunks in parallel (result is
+     * scheduling independent, as in grindWorker's fetchMin). */
+    const uint64_t chunk = 1u << 16;
+    for (uint64_t base = 0;; base += chunk) {
+        uint64_t best = UINT64_MAX;
+#pragma omp parallel for schedule(static) reduction(min : best)
+        for (int64_t i = 0; i < (int64_t)chunk; i++)
+            if (pow_check(prefix, base + (uint64_t)i, pow_bits) && base + (uint64_t)i < best) best = base + (uint64_t)i;
+        if (best != UINT64_MAX) return best;
+    }
+}
+
+/* ======================================================================
+ * FRI folds (src/core/fri/folding.zig:316-380,558-602)
+ * ==================================================================== */
+void ora_fold_line(const uint32_t *src, uint32_t log_src, uint32_t coset_initial_index, const uint32_t alpha[4],
+                   uint32_t *dst) {
+    coset c = coset_new(coset_initial_index, log_src);
+    qm31 a = q_load(alpha);
+    uint64_t n = 1ull << (log_src - 1);
+    uint32_t lg = log_src - 1;
+#pragma omp parallel for schedule(static) if (n > 4096)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        uint32_t x = coset_at(c, bitrev((uint64_t)i, lg)).x;
+        qm31 f0 = q_load(src + 8 * i), f1 = q_load(src + 8 * i + 4);
+        qm31 s = q_add(f0, f1), d = q_mul_m(q_sub(f0, f1), m_inv(x));
+        q_store(q_add(s, q_mul(a, d)), dst + 4 * i);
+    }
+}
+void ora_fold_circle_into_line(const uint32_t *const src[4], uint32_t log_src, const uint32_t alpha[4],
+                               uint32_t *dst) {
+    coset hc = coset_half_odds(log_src - 1);
+    qm31 a = q_load(alpha), a2 = q_square(q_load(alpha));
+    uint64_t n = 1ull << (log_src - 1);
+    uint32_t lg = log_src - 1;
+#pragma omp parallel for schedule(static) if (n > 4096)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        uint32_t y = coset_at(hc, bitrev((uint64_t)i, lg)).y;
+        qm31 f0 = {{src[0][2 * i], src[1][2 * i]}, {src[2][2 * i], src[3][2 * i]}};
+        qm31 f1 = {{src[0][2 * i + 1], src[1][2 * i + 1]}, {src[2][2 * i + 1], src[3][2 * i + 1]}};
+        qm31 s = q_add(f0, f1), d = q_mul_m(q_sub(f0, f1), m_inv(y));
+        qm31 fp = q_add(s, q_mul(a, d));
+        qm31 cur = q_load(dst + 4 * i);
+        q_store(q_add(q_mul(cur, a2), fp), dst + 4 * i);
+    }
+}
+
+/* ======================================================================
+ * Wide Fibonacci trace (src/examples/wide_fibonacci/trace.zig:40-162)
+ * ==================================================================== */
+static inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+/* Seeded uniform M31 word k of a trace (mode 1): splitmix64(seed ^ k*golden) mod p. */
+static inline uint32_t seeded_m31(uint64_t seed, uint64_t k) {
+    return (uint32_t)(splitmix64(seed ^ (k * 0xD1B54A32D192ED03ull)) % P);
+}
+void ora_widefib_trace(uint32_t log_n, uint32_t seq_len, uint32_t mode, uint64_t seed, uint32_t *out) {
+    uint64_t n = 1ull << log_n;
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < (int64_t)n; s++) {
+        uint32_t a, b;
+        if (mode == 0) {
+            /* logicalRow = circleDomainIndexToCosetIndex(bitReverse(s)) (trace.zig:159-162; utils.zig) */
+            uint64_t ci = bitrev((uint64_t)s, log_n);
+            uint64_t row = ci < n / 2 ? ci * 2 : (n - 1 - ci) * 2 + 1;
+            a = 1;
+            b = (uint32_t)row;
+        } else {
+            a = seeded_m31(seed, 2 * (uint64_t)s);
+            b = seeded_m31(seed, 2 * (uint64_t)s + 1);
+        }
+        out[s] = a;
+        out[n + s] = b;
+        for (uint32_t c = 2; c < seq_len; c++) {
+            uint32_t nx = m_add(m_mul(a, a), m_mul(b, b));
+            out[(uint64_t)c * n + s] = nx;
+            a = b;
+            b = nx;
+        }
+    }
+}
+
+/* ======================================================================
+ * Proof assembly
+ * ==================================================================== */
+static double g_timings[8];
+void ora_last_timings(double out[8]) { memcpy(out, g_timings, sizeof g_timings); }
+static double now_ms(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec / 1e6;
+}
+
+typedef struct {
+    uint32_t n_cols;
+    uint32_t *log_sizes;  /* LDE log sizes */
+    uint32_t **lde;       /* evaluations on CanonicCoset(log).circleDomain(), bitrev */
+    uint32_t **coeffs;    /* retained coefficients (log - blowup) */
+    uint32_t *coeff_logs;
+    ora_tree *tree;
+} ctree;
+
+static void ctree_free(ctree *t) {
+    for (uint32_t i = 0; i < t->n_cols; i++) {
+        free(t->lde[i]);
+        free(t->coeffs[i]);
+    }
+    free(t->lde);
+    free(t->coeffs);
+    free(t->log_sizes);
+    free(t->coeff_logs);
+    ora_tree_free(t->tree);
+}
+
+/* commitPolys / commit (pcs/scheme.zig:115-140; commit_polys.zig:10-67): coefficients ->
+ * LDE (blowup) -> Merkle -> mixRoot.  Takes ownership of coeffs. */
+static void commit_coeffs(ctree *t, uint32_t n_cols, uint32_t **coeffs, const uint32_t *coeff_logs, uint32_t blowup,
+                          channel *ch) {
+    t->n_cols = n_cols;
+    t->coeffs = coeffs;
+    t->coeff_logs = malloc(4 * (n_cols ? n_cols : 1));
+    t->log_sizes = malloc(4 * (n_cols ? n_cols : 1));
+    t->lde = malloc(sizeof(uint32_t *) * (n_cols ? n_cols : 1));
+    for (uint32_t i = 0; i < n_cols; i++) {
+        t->coeff_logs[i] = coeff_logs[i];
+        t->log_sizes[i] = coeff_logs[i] + blowup;
+        t->lde[i] = malloc(4ull << t->log_sizes[i]);
+    }
+    for (uint32_t i = 0; i < n_cols; i++) get_twiddles(t->log_sizes[i] > 2 ? t->log_sizes[i] : 3);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = 0; i < (int64_t)n_cols; i++) ora_circle_lde(coeffs[i], coeff_logs[i], blowup, t->lde[i]);
+    t->tree = ora_merkle_commit((const uint32_t *const *)t->lde, t->log_sizes, n_cols);
+    uint8_t root[32];
+    ora_tree_root(t->tree, root);
+    ch_mix_root(ch, root);
+}
+
+static void json_hash(bytes *b, const uint8_t *h) {
+    by_str(b, "[");
+    for (int i = 0; i < 32; i++) {
+        if (i) by_str(b, ",");
+        by_u64(b, h[i]);
+    }
+    by_str(b, "]");
+}
+static void json_qm31(bytes *b, qm31 v) {
+    uint32_t w[4];
+    q_store(v, w);
+    by_str(b, "[");
+    for (int i = 0; i < 4; i++) {
+        if (i) by_str(b, ",");
+        by_u64(b, w[i]);
+    }
+    by_str(b, "]");
+}
+static void json_hash_list(bytes *b, const bytes *w) {
+    by_str(b, "[");
+    for (size_t i = 0; i < w->n / 32; i++) {
+        if (i) by_str(b, ",");
+        json_hash(b, w->d + 32 * i);
+    }
+    by_str(b, "]");
+}
+
+typedef struct {
+    bytes json_layer; /* encoded FriLayerWire */
+} fri_layer_out;
+
+/* decommitLayerExtended for fold_step 1 (prover/fri.zig:626-740): pairs of
+ * positions, witness = non-queried partner, Merkle decommit at the pairs. */
+static void fri_layer_json(bytes *out, const ora_tree *tree, const qm31 *col, const uint64_t *q, size_t nq) {
+    uint64_t *dpos = malloc(sizeof(uint64_t) * (2 * nq + 1));
+    size_t nd = 0;
+    qm31 *wit = malloc(sizeof(qm31) * (2 * nq + 1));
+    size_t nw = 0;
+    size_t i = 0;
+    while (i < nq) {
+        uint64_t key = q[i] >> 1;
+        size_t j = i + 1;
+        while (j < nq && (q[j] >> 1) == key) j++;
+        size_t at = i;
+        for (uint64_t p = key << 1; p < (key << 1) + 2; p++) {
+            dpos[nd++] = p;
+            if (at < j && q[at] == p) at++;
+            else wit[nw++] = col[p];
+        }
+        i = j;
+    }
+    bytes hw = {0};
+    merkle_decommit_witness(tree, dpos, nd, &hw);
+    uint8_t root[32];
+    ora_tree_root(tree, root);
+    by_str(out, "{\"fri_witness\":[");
+    for (size_t k = 0; k < nw; k++) {
+        if (k) by_str(out, ",");
+        json_qm31(out, wit[k]);
+    }
+    by_str(out, "],\"decommitment\":{\"hash_witness\":");
+    json_hash_list(out, &hw);
+    by_str(out, "},\"commitment\":");
+    json_hash(out, root);
+    by_str(out, "}");
+    free(hw.d);
+    free(dpos);
+    free(wit);
+}
+
+static ora_tree *commit_qm31_column(const qm31 *v, uint32_t log_size, uint32_t **coords_out) {
+    uint64_t n = 1ull << log_size;
+    uint32_t *c[4];
+    for (int k = 0; k < 4; k++) c[k] = malloc(4 * n);
+    for (uint64_t i = 0; i < n; i++) {
+        c[0][i] = v[i].c0.a; c[1][i] = v[i].c0.b; c[2][i] = v[i].c1.a; c[3][i] = v[i].c1.b;
+    }
+    uint32_t logs[4] = {log_size, log_size, log_size, log_size};
+    ora_tree *t = ora_merkle_commit((const uint32_t *const *)c, logs, 4);
+    if (coords_out)
+        for (int k = 0; k < 4; k++) coords_out[k] = c[k];
+    else
+        for (int k = 0; k < 4; k++) free(c[k]);
+    return t;
+}
+
+static int cmp_u64(const void *a, const void *b) {
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : (x > y);
+}
+
+int ora_prove_wide_fibonacci(const ora_widefib_params *prm, uint8_t *out, size_t cap, size_t *out_len) {
+    const uint32_t n = prm->log_n_rows, W = prm->sequence_len, B = prm->log_blowup_factor;
+    if (n < 1 || W < 3 || B < 1) return 1;
+    double t0 = now_ms();
+    memset(g_timings, 0, sizeof g_timings);
+    init_gen_doubles();
+    channel ch;
+    memset(&ch, 0, sizeof ch);
+    /* PcsConfig.mixInto (core/pcs/mod.zig:32-52), fold_step 1, no lifting override. */
+    qm31 cfg = {{prm->pow_bits, B}, {prm->n_queries, prm->log_last_layer_degree_bound}};
+    ch_mix_felts(&ch, &cfg, 1);
+
+    /* Tree 0: preprocessed, no columns (examples/wide_fibonacci/trace.zig:184-186). */
+    ctree trees[3];
+    memset(trees, 0, sizeof trees);
+    commit_coeffs(&trees[0], 0, NULL, NULL, B, &ch);
+
+    /* Tree 1: main trace: generate -> interpolate -> LDE -> Merkle (preparation.zig:113-210). */
+    uint64_t N = 1ull << n;
+    uint32_t *trace = malloc(4ull * N * W);
+    ora_widefib_trace(n, W, prm->trace_mode, prm->trace_seed, trace);
+    double t1 = now_ms();
+    uint32_t **tc = malloc(sizeof(uint32_t *) * W);
+    uint32_t *tlogs = malloc(4 * W);
+    get_twiddles(n > 2 ? n : 3);
+    for (uint32_t c = 0; c < W; c++) {
+        tc[c] = malloc(4 * N);
+        memcpy(tc[c], trace + (uint64_t)c * N, 4 * N);
+        tlogs[c] = n;
+    }
+    free(trace);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t c = 0; c < (int64_t)W; c++) circle_interpolate(tc[c], n);
+    commit_coeffs(&trees[1], W, tc, tlogs, B, &ch);
+    free(tlogs);
+    double t2 = now_ms();
+
+    /* mixStatement (wide_fibonacci.zig:496-498). */
+    uint32_t st[2] = {n, W};
+    ch_mix_u32s(&ch, st, 2);
+
+    /* prove.zig:209-318: composition. */
+    qm31 alpha = ch_draw_secure_felt(&ch);
+    const uint32_t NC = W - 2, CL = n + 1; /* composition log size */
+    uint64_t CN = 1ull << CL;
+    qm31 *pw = malloc(sizeof(qm31) * NC);
+    pw[0] = Q_ONE;
+    for (uint32_t k = 1; k < NC; k++) pw[k] = q_mul(pw[k - 1], alpha);
+    /* Trace evaluations on CanonicCoset(n+1): the LDE itself when blowup is 1. */
+    uint32_t **ev = malloc(sizeof(uint32_t *) * W);
+    int own_ev = (B != 1);
+    for (uint32_t c = 0; c < W; c++) {
+        if (!own_ev) ev[c] = trees[1].lde[c];
+        else ev[c] = malloc(4 * CN);
+    }
+    if (own_ev) {
+        get_twiddles(CL);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t c = 0; c < (int64_t)W; c++) ora_circle_lde(trees[1].coeffs[c], n, 1, ev[c]);
+    }
+    /* cosetVanishing(CanonicCoset(n).coset(), domain.at(0/1)) (constraints.zig:16-27). */
+    uint32_t den_inv[2];
+    for (int r = 0; r < 2; r++) {
+        coset tc_ = coset_odds(n);
+        pt dp = domain_at(CL, (uint64_t)r);
+        pt shifted = p_add(p_add(dp, p_conj(idx_to_point(tc_.initial_index))),
+                           idx_to_point(tc_.step_size / 2)); /* half_step */
+        uint32_t x = shifted.x;
+        for (uint32_t i = 1; i < n; i++) x = m_double_x(x);
+        den_inv[r] = m_inv(x);
+    }
+    uint32_t *comp[4];
+    for (int k = 0; k < 4; k++) comp[k] = malloc(4 * CN);
+#pragma omp parallel for schedule(static)
+    for (int64_t row = 0; row < (int64_t)CN; row++) {
+        uint32_t a = ev[0][row], b = ev[1][row];
+        qm31 acc = Q_ZERO;
+        for (uint32_t i = 0; i < NC; i++) {
+            uint32_t c = ev[i + 2][row];
+            uint32_t cons = m_sub(c, m_add(m_mul(a, a), m_mul(b, b)));
+            acc = q_add(acc, q_mul_m(pw[NC - 1 - i], cons));
+            a = b;
+            b = c;
+        }
+        acc = q_mul_m(acc, den_inv[(uint64_t)row >> n]);
+        comp[0][row] = acc.c0.a; comp[1][row] = acc.c0.b; comp[2][row] = acc.c1.a; comp[3][row] = acc.c1.b;
+    }
+    if (own_ev)
+        for (uint32_t c = 0; c < W; c++) free(ev[c]);
+    free(ev);
+    free(pw);
+    /* interpolate + splitAtMid (secure_poly.zig:164-215; poly.zig:302-318) -> commitPolys. */
+    get_twiddles(CL);
+#pragma omp parallel for
+    for (int k = 0; k < 4; k++) circle_interpolate(comp[k], CL);
+    uint32_t **cc = malloc(sizeof(uint32_t *) * 8);
+    uint32_t clogs[8];
+    for (int k = 0; k < 4; k++) {
+        cc[k] = malloc(4 * N);
+        cc[4 + k] = malloc(4 * N);
+        memcpy(cc[k], comp[k], 4 * N);
+        memcpy(cc[4 + k], comp[k] + N, 4 * N);
+        clogs[k] = clogs[4 + k] = n;
+        free(comp[k]);
+    }
+    commit_coeffs(&trees[2], 8, cc, clogs, B, &ch);
+    double t3 = now_ms();
+
+    /* OODS point (circle.zig:159-167). */
+    qm31 tt = ch_draw_secure_felt(&ch);
+    qm31 t2_ = q_square(tt);
+    qm31 inv1 = q_inv(q_add(t2_, Q_ONE));
+    qpt oods = {q_mul(q_sub(Q_ONE, t2_), inv1), q_mul(q_add(tt, tt), inv1)};
+
+    /* Sampled values (sampled_values.zig): every column sampled once at the OODS
+     * point folded (lifting - log) times; here all trees share one log size. */
+    uint32_t L = 0;
+    for (int tr = 0; tr < 3; tr++)
+        for (uint32_t c = 0; c < trees[tr].n_cols; c++)
+            if (trees[tr].log_sizes[c] > L) L = trees[tr].log_sizes[c];
+    uint32_t total_cols = trees[1].n_cols + trees[2].n_cols;
+    qm31 *sv = malloc(sizeof(qm31) * total_cols);
+    qpt *spt = malloc(sizeof(qpt) * total_cols);
+    {
+        size_t at = 0;
+        for (int tr = 1; tr < 3; tr++)
+            for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
+                qpt p = oods;
+                for (uint32_t f = trees[tr].log_sizes[c]; f < L; f++) { /* repeatedDouble */
+                    qm31 xx = q_square(p.x), xy = q_mul(p.x, p.y);
+                    p.x = q_sub(q_add(xx, xx), Q_ONE);
+                    p.y = q_add(xy, xy);
+                }
+                spt[at++] = p;
+            }
+        /* All wide-Fib columns share one point and one coefficient log size. */
+        qm31 *w = eval_weights(spt[0], n);
+        at = 0;
+        for (int tr = 1; tr < 3; tr++) {
+            uint32_t nc = trees[tr].n_cols;
+#pragma omp parallel for schedule(dynamic, 1)
+            for (int64_t c = 0; c < (int64_t)nc; c++) sv[at + c] = dot_weights(trees[tr].coeffs[c], w, N);
+            at += nc;
+        }
+        free(w);
+    }
+    /* sampled_value_transcript.mixIntoChannel: tree, column, sample order. */
+    ch_mix_felts(&ch, sv, total_cols);
+    double t4 = now_ms();
+
+    /* Quotients (core/pcs/quotients/{samples,row_evaluation}.zig; constraints.zig:87-97). */
+    qm31 qalpha = ch_draw_secure_felt(&ch);
+    uint64_t LN = 1ull << L;
+    qm31 *lc_c = malloc(sizeof(qm31) * total_cols);
+    qm31 sum_a = Q_ZERO, sum_b = Q_ZERO;
+    {
+        qm31 rp = Q_ONE;
+        for (uint32_t i = 0; i < total_cols; i++) {
+            qm31 v = sv[i];
+            qm31 a = q_sub(q_conj(v), v);
+            qm31 c = q_sub(q_conj(spt[i].y), spt[i].y);
+            qm31 b = q_sub(q_mul(v, c), q_mul(a, spt[i].y));
+            lc_c[i] = q_mul(rp, c);
+            sum_a = q_add(sum_a, q_mul(rp, a));
+            sum_b = q_add(sum_b, q_mul(rp, b));
+            rp = q_mul(rp, qalpha);
+        }
+    }
+    const uint32_t **qcols = malloc(sizeof(uint32_t *) * total_cols);
+    uint32_t *qlogs = malloc(4 * total_cols);
+    {
+        size_t at = 0;
+        for (int tr = 1; tr < 3; tr++)
+            for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
+                qcols[at] = trees[tr].lde[c];
+                qlogs[at++] = trees[tr].log_sizes[c];
+            }
+    }
+    cm31 prx = oods.x.c0, pry = oods.y.c0, pix = oods.x.c1, piy = oods.y.c1;
+    cm31 det = c_sub(c_mul(prx, piy), c_mul(pry, pix));
+    qm31 *quot = malloc(sizeof(qm31) * LN);
+#pragma omp parallel for schedule(static)
+    for (int64_t row = 0; row < (int64_t)LN; row++) {
+        pt dp = domain_at(L, bitrev((uint64_t)row, L));
+        qm31 num = Q_ZERO;
+        for (uint32_t i = 0; i < total_cols; i++) {
+            uint32_t v = qcols[i][lift_index((uint64_t)row, L, qlogs[i])];
+            num = q_add(num, q_mul_m(lc_c[i], v));
+        }
+        num = q_sub(num, q_add(q_mul_m(sum_a, dp.y), sum_b));
+        cm31 den = c_add(c_sub(det, c_mul_m(piy, dp.x)), c_mul_m(pix, dp.y));
+        quot[row] = q_mul_c(num, c_inv(den));
+    }
+    free(qcols);
+    free(qlogs);
+    free(lc_c);
+    double t5 = now_ms();
+
+    /* FRI commit (prover/fri.zig:163-621). */
+    uint32_t nlayers_max = L + 1;
+    ora_tree **ftrees = calloc(nlayers_max, sizeof(ora_tree *));
+    qm31 **fcols = calloc(nlayers_max, sizeof(qm31 *));
+    uint32_t *flogs = calloc(nlayers_max, 4);
+    uint32_t *first_coords[4];
+    ftrees[0] = commit_qm31_column(quot, L, first_coords);
+    fcols[0] = quot;
+    flogs[0] = L;
+    {
+        uint8_t r[32];
+        ora_tree_root(ftrees[0], r);
+        ch_mix_root(&ch, r);
+    }
+    qm31 fa = ch_draw_secure_felt(&ch);
+    uint64_t ln = LN / 2;
+    qm31 *line = calloc(ln, sizeof(qm31));
+    ora_fold_circle_into_line((const uint32_t *const *)first_coords, L, (const uint32_t *)&fa, (uint32_t *)line);
+    for (int k = 0; k < 4; k++) free(first_coords[k]);
+    coset ldom = coset_half_odds(L - 1);
+    uint32_t llog = L - 1;
+    const uint32_t last_log = prm->log_last_layer_degree_bound + B;
+    uint32_t ninner = 0;
+    while (llog > last_log) {
+        ftrees[1 + ninner] = commit_qm31_column(line, llog, NULL);
+        fcols[1 + ninner] = line;
+        flogs[1 + ninner] = llog;
+        uint8_t r[32];
+        ora_tree_root(ftrees[1 + ninner], r);
+        ch_mix_root(&ch, r);
+        qm31 la = ch_draw_secure_felt(&ch);
+        qm31 *next = malloc(sizeof(qm31) * (1ull << (llog - 1)));
+        ora_fold_line((const uint32_t *)line, llog, ldom.initial_index, (const uint32_t *)&la, (uint32_t *)next);
+        ldom = coset_double(ldom);
+        llog--;
+        line = next;
+        ninner++;
+    }
+    /* Last layer (fri.zig:594-621; line_evaluation.zig:97-151; line.zig:134-142). */
+    uint64_t lsz = 1ull << llog;
+    bitrev_q(line, lsz);
+    {
+        coset cd = ldom;
+        while ((1ull << cd.log_size) > 1) {
+            uint64_t csz = 1ull << cd.log_size, half = csz / 2;
+            for (uint64_t st0 = 0; st0 < lsz; st0 += csz) {
+                pt cur = idx_to_point(cd.initial_index), step = idx_to_point(cd.step_size);
+                for (uint64_t i = 0; i < half; i++) {
+                    uint32_t ix = m_inv(cur.x);
+                    qm31 v0 = line[st0 + i], v1 = line[st0 + half + i];
+                    line[st0 + i] = q_add(v0, v1);
+                    line[st0 + half + i] = q_mul_m(q_sub(v0, v1), ix);
+                    cur = p_add(cur, step);
+                }
+            }
+            cd = coset_double(cd);
+        }
+        uint32_t li = m_inv((uint32_t)(lsz % P));
+        for (uint64_t i = 0; i < lsz; i++) line[i] = q_mul_m(line[i], li);
+    }
+    bitrev_q(line, lsz); /* ordered coefficients */
+    uint64_t bound = 1ull << prm->log_last_layer_degree_bound;
+    for (uint64_t i = bound; i < lsz; i++)
+        if (!q_is_zero(line[i])) return 3; /* InvalidLastLayerDegree */
+    qm31 *lastp = malloc(sizeof(qm31) * bound);
+    memcpy(lastp, line, sizeof(qm31) * bound);
+    bitrev_q(lastp, bound); /* fromOrderedCoefficients */
+    free(line);
+    ch_mix_felts(&ch, lastp, bound);
+    double t6 = now_ms();
+
+    /* PoW (scheme.zig:733-735). */
+    uint64_t nonce = ora_pow_grind(ch.digest, prm->pow_bits);
+    ch_mix_u64(&ch, nonce);
+    double t7 = now_ms();
+
+    /* Queries (core/queries.zig:8-60). */
+    uint32_t nq = prm->n_queries;
+    uint64_t *qs = malloc(sizeof(uint64_t) * (nq + 8));
+    {
+        uint32_t got = 0;
+        while (got < nq) {
+            uint32_t w[8];
+            ch_draw_u32s(&ch, w);
+            for (int i = 0; i < 8 && got < nq; i++) qs[got++] = w[i] & ((1u << L) - 1);
+        }
+    }
+    qsort(qs, nq, sizeof(uint64_t), cmp_u64);
+    size_t nuq = 0;
+    for (uint32_t i = 0; i < nq; i++)
+        if (nuq == 0 || qs[nuq - 1] != qs[i]) qs[nuq++] = qs[i];
+
+    /* Assemble the proof wire (interop/proof_wire.zig:19-58). */
+    bytes j = {0};
+    by_str(&j, "{\"config\":{\"pow_bits\":");
+    by_u64(&j, prm->pow_bits);
+    by_str(&j, ",\"fri_config\":{\"log_blowup_factor\":");
+    by_u64(&j, B);
+    by_str(&j, ",\"log_last_layer_degree_bound\":");
+    by_u64(&j, prm->log_last_layer_degree_bound);
+    by_str(&j, ",\"n_queries\":");
+    by_u64(&j, nq);
+    by_str(&j, ",\"fold_step\":1},\"lifting_log_size\":null},\"commitments\":[");
+    for (int tr = 0; tr < 3; tr++) {
+        uint8_t r[32];
+        ora_tree_root(trees[tr].tree, r);
+        if (tr) by_str(&j, ",");
+        json_hash(&j, r);
+    }
+    by_str(&j, "],\"sampled_values\":[[]");
+    {
+        size_t at = 0;
+        for (int tr = 1; tr < 3; tr++) {
+            by_str(&j, ",[");
+            for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
+                if (c) by_str(&j, ",");
+                by_str(&j, "[");
+                json_qm31(&j, sv[at++]);
+                by_str(&j, "]");
+            }
+            by_str(&j, "]");
+        }
+    }
+    by_str(&j, "],\"decommitments\":[");
+    for (int tr = 0; tr < 3; tr++) {
+        bytes hw = {0};
+        if (trees[tr].n_cols) merkle_decommit_witness(trees[tr].tree, qs, nuq, &hw);
+        if (tr) by_str(&j, ",");
+        by_str(&j, "{\"hash_witness\":");
+        json_hash_list(&j, &hw);
+        by_str(&j, "}");
+        free(hw.d);
+    }
+    by_str(&j, "],\"queried_values\":[");
+    for (int tr = 0; tr < 3; tr++) {
+        if (tr) by_str(&j, ",");
+        by_str(&j, "[");
+        for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
+            if (c) by_str(&j, ",");
+            by_str(&j, "[");
+            for (size_t q = 0; q < nuq; q++) {
+                if (q) by_str(&j, ",");
+                by_u64(&j, trees[tr].lde[c][lift_index(qs[q], trees[tr].tree->max_log, trees[tr].log_sizes[c])]);
+            }
+            by_str(&j, "]");
+        }
+        by_str(&j, "]");
+    }
+    by_str(&j, "],\"proof_of_work\":");
+    by_u64(&j, nonce);
+    by_str(&j, ",\"fri_proof\":{\"first_layer\":");
+    /* First layer column is the quotient (AoS view). */
+    fri_layer_json(&j, ftrees[0], fcols[0], qs, nuq);
+    by_str(&j, ",\"inner_layers\":[");
+    uint64_t *lq = malloc(sizeof(uint64_t) * (nuq + 1));
+    size_t nlq = 0;
+    for (size_t i = 0; i < nuq; i++) {
+        uint64_t v = qs[i] >> 1;
+        if (nlq == 0 || lq[nlq - 1] != v) lq[nlq++] = v;
+    }
+    for (uint32_t k = 0; k < ninner; k++) {
+        if (k) by_str(&j, ",");
+        fri_layer_json(&j, ftrees[1 + k], fcols[1 + k], lq, nlq);
+        size_t m2 = 0;
+        for (size_t i = 0; i < nlq; i++) {
+            uint64_t v = lq[i] >> 1;
+            if (m2 == 0 || lq[m2 - 1] != v) lq[m2++] = v;
+        }
+        nlq = m2;
+    }
+    by_str(&j, "],\"last_layer_poly\":[");
+    for (uint64_t i = 0; i < bound; i++) {
+        if (i) by_str(&j, ",");
+        json_qm31(&j, lastp[i]);
+    }
+    by_str(&j, "]}}");
+    double t8 = now_ms();
+
+    g_timings[0] = t1 - t0; /* trace generation */
+    g_timings[1] = t2 - t1; /* main commit */
+    g_timings[2] = t3 - t2; /* composition + commit */
+    g_timings[3] = t4 - t3; /* sampled values */
+    g_timings[4] = t5 - t4; /* quotients */
+    g_timings[5] = t6 - t5; /* FRI commit */
+    g_timings[6] = t7 - t6; /* PoW */
+    g_timings[7] = t8 - t1; /* prove total (excludes trace generation) */
+
+    int rc = 0;
+    *out_len = j.n;
+    if (j.n > cap) rc = 2;
+    else memcpy(out, j.d, j.n);
+    free(j.d);
+    free(lq);
+    free(qs);
+    free(lastp);
+    for (uint32_t k = 0; k <= ninner; k++) {
+        ora_tree_free(ftrees[k]);
+        free(fcols[k]);
+    }
+    free(ftrees);
+    free(fcols);
+    free(flogs);
+    free(sv);
+    free(spt);
+    for (int tr = 0; tr < 3; tr++) ctree_free(&trees[tr]);
+    return rc;
+}

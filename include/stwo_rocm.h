@@ -1,0 +1,190 @@
+/*
+ * stwo_rocm.h -- C-ABI of the MI355X-native (gfx950) proving backend for stwo-zig.
+ *
+ * This is the drop-in boundary a Zig `RocmBackend` binds with `extern fn`, in the
+ * place of the Metal runtime ABI (src/backends/metal/runtime/bindings.zig:11-654)
+ * and of the external CUDA library's FFI (src/backends/cuda/ffi.zig:27-280).
+ *
+ * Conventions (modelled on the Metal ABI, not on the CUDA one, which returns void):
+ *   - every entry point returns an int status: STWO_ROCM_OK (0) or an error code;
+ *     the message of the last failure is available from stwo_rocm_last_error();
+ *   - sizes are uint64_t / uint32_t log sizes; no C++ or torch types cross the ABI;
+ *   - "device pointers" are HBM addresses obtained from stwo_rocm_buffer_alloc;
+ *     arrays *of* device pointers (`cols`) are host arrays, as in ffi.zig
+ *     (`values: [*][*]u32`);
+ *   - M31 values are canonical u32 words in [0, 2^31-1); a QM31 is 4 words
+ *     (c0.a, c0.b, c1.a, c1.b) (src/core/fields/qm31.zig:64-70); a circle point over
+ *     QM31 is 8 words (x then y); hashes are 32 bytes;
+ *   - columns are stored in bit-reversed circle-domain order, as in the reference;
+ *   - calls on one context are serialised by an internal mutex and execute on the
+ *     context's stream; results are complete when a call returns unless stated.
+ */
+#ifndef STWO_ROCM_H
+#define STWO_ROCM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STWO_ROCM_OK 0
+#define STWO_ROCM_EINVAL 1     /* invalid argument / shape */
+#define STWO_ROCM_ENOMEM 2     /* device allocation failed */
+#define STWO_ROCM_EDEVICE 3    /* HIP runtime error */
+#define STWO_ROCM_EBUDGET 4    /* HBM budget of the context exceeded */
+#define STWO_ROCM_EINTERNAL 5  /* anything else (e.g. constraints not satisfied) */
+
+typedef struct stwo_rocm_ctx stwo_rocm_ctx;
+typedef struct stwo_rocm_tree stwo_rocm_tree;
+
+/* ------------------------------------------------------------------ runtime */
+/* Replaces the Metal shared runtime init (src/backends/metal/shared_runtime.zig:1-40;
+ * MetalCommitBackend.initializeRuntime, metal/commit_backend.zig). hbm_budget_bytes = 0:
+ * unlimited.  On failure *out is NULL and err (if non-NULL) receives the message. */
+int stwo_rocm_ctx_create(int device, uint64_t hbm_budget_bytes, stwo_rocm_ctx** out, char* err, size_t err_len);
+int stwo_rocm_ctx_destroy(stwo_rocm_ctx* ctx);
+const char* stwo_rocm_last_error(stwo_rocm_ctx* ctx);
+/* runtimePlatformIdentityAlloc (metal/commit_backend.zig): "name;arch;CUs;HBM bytes". */
+int stwo_rocm_ctx_identity(stwo_rocm_ctx* ctx, char* out, size_t out_len);
+int stwo_rocm_synchronize(stwo_rocm_ctx* ctx);
+/* telemetrySnapshot (metal/telemetry.zig): arena bytes live / cached / peak. */
+int stwo_rocm_memory_info(stwo_rocm_ctx* ctx, uint64_t* live, uint64_t* cached, uint64_t* peak);
+int stwo_rocm_release_cached(stwo_rocm_ctx* ctx);
+
+/* ------------------------------------------------------------------ buffers */
+/* cuda_malloc_uint32_t / cuda_free_memory / copy_uint32_t_vec_* (cuda/ffi.zig:27-37,82). */
+int stwo_rocm_buffer_alloc(stwo_rocm_ctx* ctx, uint64_t bytes, void** dptr);
+int stwo_rocm_buffer_free(stwo_rocm_ctx* ctx, void* dptr);
+int stwo_rocm_buffer_upload(stwo_rocm_ctx* ctx, void* dst_dev, const void* src_host, uint64_t bytes);
+int stwo_rocm_buffer_download(stwo_rocm_ctx* ctx, void* dst_host, const void* src_dev, uint64_t bytes);
+
+/* ------------------------------------------------------------------ twiddles */
+/* precompute_twiddles (cuda/ffi.zig:63); M31TwiddleTower.init (prover/poly/twiddle_tower.zig:29-64).
+ * Makes the resident tower cover circle domains up to max_circle_log. */
+int stwo_rocm_twiddles_prepare(stwo_rocm_ctx* ctx, uint32_t max_circle_log);
+/* M31TwiddleTower.view (twiddle_tower.zig:66-87): copies the 2^(log-1) twiddles and
+ * inverse twiddles of CanonicCoset(circle_log) to host memory. */
+int stwo_rocm_twiddles_read(stwo_rocm_ctx* ctx, uint32_t circle_log, uint32_t* tw_host, uint32_t* itw_host);
+
+/* ------------------------------------------------------------------ field */
+/* batch_inverse_base_field (cuda/ffi.zig:43); FieldOps.batchInverse (backend/field_ops.zig). */
+int stwo_rocm_batch_inverse(stwo_rocm_ctx* ctx, const uint32_t* d_in, uint64_t n, uint32_t* d_out);
+/* bit_reverse_base_field / bit_reverse_secure_field (cuda/ffi.zig:44-45): words_per_elem 1 or 4. */
+int stwo_rocm_bit_reverse(stwo_rocm_ctx* ctx, uint32_t* d_vals, uint32_t log_size, uint32_t words_per_elem);
+/* accumulate (cuda/ffi.zig:76; backend/accumulation_ops.zig): acc[i] += src[i] mod p. */
+int stwo_rocm_accumulate(stwo_rocm_ctx* ctx, uint32_t* d_acc, const uint32_t* d_src, uint64_t n_words);
+
+/* ------------------------------------------------------------------ circle polynomials */
+/* interpolateCircleBuffers (prover/pcs/columns/circle_transforms.zig:248-250;
+ * prover/poly/circle/transforms.zig:249-360); ntt_b2n (cuda/ffi.zig:52).
+ * In place: evaluations on CanonicCoset(log_size).circleDomain() -> coefficients. */
+int stwo_rocm_circle_interpolate(stwo_rocm_ctx* ctx, uint32_t* const* cols, uint32_t n_cols, uint32_t log_size,
+                                 double* gpu_ms);
+/* evaluateCircleBuffers (circle_transforms.zig:443-451; transforms.zig:132-246); ntt_n2b (cuda/ffi.zig:51).
+ * coeffs (2^log_size each) -> evaluations on CanonicCoset(log_size + log_blowup), written to out. */
+int stwo_rocm_circle_evaluate(stwo_rocm_ctx* ctx, const uint32_t* const* coeffs, uint32_t n_cols, uint32_t log_size,
+                              uint32_t log_blowup, uint32_t* const* out, double* gpu_ms);
+/* batch_eval_at_points (cuda/ffi.zig:88); evaluateCoefficientPlans (prover/pcs/sampled_values.zig:67-80).
+ * out_host[4*i..] = poly_i(point) for coefficient columns of 2^log_size. */
+int stwo_rocm_eval_at_point(stwo_rocm_ctx* ctx, const uint32_t* const* coeffs, uint32_t n_cols, uint32_t log_size,
+                            const uint32_t point[8], uint32_t* out_host);
+
+/* ------------------------------------------------------------------ Merkle */
+/* commitMerkle (backend/merkle_ops.zig:20-56; prover/vcs_lifted/prover.zig:58-68,276-340);
+ * commit_on_first_layer_lifted (cuda/ffi.zig:272-278).  Lifted Blake2s Merkle with the
+ * "leaf"/"node" 64-byte domain prefixes over columns of any power-of-two lengths >= 2.
+ * The tree stays resident; root_out (32 bytes) may be NULL. */
+int stwo_rocm_merkle_commit(stwo_rocm_ctx* ctx, const uint32_t* const* cols, const uint32_t* log_sizes,
+                            uint32_t n_cols, stwo_rocm_tree** out, uint8_t* root_out, double* gpu_ms);
+/* commit_on_layer_in_gpu (cuda/ffi.zig:70): one parent layer (2^log_size nodes) from 2^(log_size+1) children. */
+int stwo_rocm_merkle_parent_layer(stwo_rocm_ctx* ctx, const uint8_t* d_children, uint32_t log_size,
+                                  uint8_t* d_parents);
+/* MerkleTree(H).root / maxLogSize / readHashes / deinit (merkle_ops.zig:27-40). */
+int stwo_rocm_tree_root(stwo_rocm_ctx* ctx, stwo_rocm_tree* tree, uint8_t root_out[32]);
+int stwo_rocm_tree_max_log(stwo_rocm_tree* tree, uint32_t* out);
+int stwo_rocm_tree_read_hashes(stwo_rocm_ctx* ctx, stwo_rocm_tree* tree, uint32_t layer_log, const uint32_t* indices,
+                               uint32_t n, uint8_t* out_host);
+/* MerkleTree(H).decommit (prover/vcs_lifted/decommit.zig:56-155): hash witness for sorted
+ * query positions. witness_out receives up to witness_cap hashes; *n_witness the count. */
+int stwo_rocm_tree_decommit(stwo_rocm_ctx* ctx, stwo_rocm_tree* tree, const uint64_t* positions, uint32_t n_positions,
+                            uint8_t* witness_out, uint32_t witness_cap, uint32_t* n_witness);
+int stwo_rocm_tree_free(stwo_rocm_ctx* ctx, stwo_rocm_tree* tree);
+
+/* ------------------------------------------------------------------ FRI */
+/* foldLine / foldLineN (backend/fri_ops.zig:8-17; core/fri/folding.zig:352-380); fold_line (cuda/ffi.zig:61).
+ * d_src: 2^log_src QM31 (AoS) on LineDomain(Coset.halfOdds(log_src)); d_dst: 2^(log_src-1) QM31. */
+int stwo_rocm_fri_fold_line(stwo_rocm_ctx* ctx, const uint32_t* d_src, uint32_t log_src, const uint32_t alpha[4],
+                            uint32_t* d_dst);
+/* foldCircleIntoLine (backend/fri_ops.zig; core/fri/folding.zig:558-602); fold_circle_into_line (cuda/ffi.zig:62).
+ * src_coords: 4 coordinate columns of 2^log_src on CanonicCoset(log_src).circleDomain();
+ * d_dst (in/out): 2^(log_src-1) QM31, dst = dst * alpha^2 + fold. */
+int stwo_rocm_fri_fold_circle_into_line(stwo_rocm_ctx* ctx, const uint32_t* const* src_coords, uint32_t log_src,
+                                        const uint32_t alpha[4], uint32_t* d_dst);
+
+/* ------------------------------------------------------------------ quotients */
+/* computeLazyQuotients / commitLazyMerkle (prover/fri.zig:341-363; prover/pcs/quotients/lazy_provider.zig:316-395);
+ * accumulate_quotients (cuda/ffi.zig:94-108).
+ * Columns (device, lengths 2^column_log_sizes[i]) are lifted to 2^lifting_log_size.
+ * Samples are listed in tree/column/sample order: (column index, point (8 words), value (4 words)).
+ * Random coefficient powers and point batches follow core/pcs/quotients/samples.zig:245-312.
+ * out_coords: 4 device columns of 2^lifting_log_size.  When tree_out is non-NULL the 4
+ * coordinate columns are also committed (fused leaf hashing) and the tree returned. */
+int stwo_rocm_compute_quotients(stwo_rocm_ctx* ctx, uint32_t lifting_log_size, const uint32_t* const* columns,
+                                const uint32_t* column_log_sizes, uint32_t n_columns, const uint32_t* sample_columns,
+                                const uint32_t* sample_points, const uint32_t* sample_values, uint32_t n_samples,
+                                const uint32_t random_coeff[4], uint32_t* const* out_coords,
+                                stwo_rocm_tree** tree_out);
+
+/* ------------------------------------------------------------------ wide Fibonacci */
+/* fillQuadraticRecurrenceTrace (examples/wide_fibonacci/trace.zig:63-72); generate_wide_fibonacci_trace
+ * (cuda/ffi.zig:263).  d_out: seq_len columns of 2^log_n_rows, column-major.
+ * mode 0: seeds (1, logicalRow(i)); mode 1: seeded uniform M31 pair per row (seed). */
+int stwo_rocm_widefib_trace(stwo_rocm_ctx* ctx, uint32_t log_n_rows, uint32_t seq_len, uint32_t mode, uint64_t seed,
+                            uint32_t* d_out);
+/* computeCompositionEvaluation (cpu_scalar/secure_composition.zig; wide_fibonacci/component.zig:146-228):
+ * eval_cols = trace evaluations on CanonicCoset(log_n_rows + 1); out_coords = 4 device columns. */
+int stwo_rocm_widefib_composition(stwo_rocm_ctx* ctx, const uint32_t* const* eval_cols, uint32_t n_cols,
+                                  uint32_t log_n_rows, const uint32_t random_coeff[4], uint32_t* const* out_coords);
+
+/* ------------------------------------------------------------------ proof of work */
+/* Blake2sChannel.grind (core/channel/blake2s.zig:168-258): lowest nonce with >= pow_bits
+ * trailing zeros of H(H(0x12345678 || 0^12 || digest || bits) || nonce). */
+int stwo_rocm_pow_grind(stwo_rocm_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits, uint64_t* nonce);
+
+/* ------------------------------------------------------------------ whole proof */
+typedef struct {
+    uint32_t log_n_rows;
+    uint32_t sequence_len;
+    uint32_t pow_bits;
+    uint32_t log_blowup_factor;
+    uint32_t log_last_layer_degree_bound;
+    uint32_t n_queries;
+    uint32_t trace_mode; /* 0: reference seeds (1, i); 1: seeded uniform pair */
+    uint64_t trace_seed;
+} stwo_rocm_widefib_params;
+
+typedef struct {
+    /* main_commit, composition, sampled_values, quotients, fri_commit, pow, decommit, encode */
+    double stage_gpu_ms[8];
+    double stage_host_ms[8];
+    double total_gpu_ms;
+    double total_host_ms;
+    uint64_t peak_hbm_bytes;
+} stwo_rocm_prove_stats;
+
+/* ProverEngine.prove for the wide-Fibonacci example (examples/wide_fibonacci.zig:181-343 ->
+ * examples/common/prover_transaction.zig:222-339 -> prover/prove.zig:209-389) with
+ * PcsConfig{pow_bits, FriConfig{log_last_layer_degree_bound, log_blowup_factor, n_queries}}.
+ * d_trace: optional resident trace (sequence_len columns of 2^log_n_rows, column-major);
+ * NULL generates it on the device inside the call.  Writes the canonical proof bytes
+ * (compact JSON wire, src/interop/proof_wire.zig:15-76) to proof_out; when cap is too
+ * small returns STWO_ROCM_EINVAL with *proof_len set to the required size. */
+int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params,
+                                   const uint32_t* d_trace, uint8_t* proof_out, uint64_t cap, uint64_t* proof_len,
+                                   stwo_rocm_prove_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

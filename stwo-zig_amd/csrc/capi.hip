@@ -1,0 +1,590 @@
+// C-ABI exports of the rocm backend (declared in include/stwo_rocm.h).
+// Every entry point: lock the context, translate C++ exceptions into status
+// codes, keep the message for stwo_rocm_last_error().
+#include <string.h>
+
+#include <algorithm>
+
+#include "../../include/stwo_rocm.h"
+#include "prover.hpp"
+
+using namespace sr;
+
+struct stwo_rocm_ctx {
+    Ctx c;
+};
+struct stwo_rocm_tree {
+    DeviceTree* t;
+};
+
+template <class F>
+static int guarded(stwo_rocm_ctx* ctx, F&& f) {
+    if (!ctx) return STWO_ROCM_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->c.mu);
+    try {
+        SR_HIP(hipSetDevice(ctx->c.device));
+        f(ctx->c);
+        return STWO_ROCM_OK;
+    } catch (const Error& e) {
+        ctx->c.last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->c.last_error = e.what();
+        return STWO_ROCM_EINTERNAL;
+    }
+}
+
+static qm31 qin(const uint32_t* w) {
+    for (int i = 0; i < 4; i++) SR_CHECK(w[i] < P, "non-canonical M31 word");
+    return qload(w);
+}
+
+extern "C" {
+
+int stwo_rocm_ctx_create(int device, uint64_t hbm_budget_bytes, stwo_rocm_ctx** out, char* err, size_t err_len) {
+    if (!out) return STWO_ROCM_EINVAL;
+    *out = nullptr;
+    stwo_rocm_ctx* ctx = new stwo_rocm_ctx();
+    try {
+        int n = 0;
+        SR_HIP(hipGetDeviceCount(&n));
+        SR_CHECK(device >= 0 && device < n, "no such HIP device");
+        ctx->c.device = device;
+        SR_HIP(hipSetDevice(device));
+        SR_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+        ctx->c.hbm_budget = hbm_budget_bytes;
+        ctx->c.leaf_seed = b2_prefix_seed("leaf");
+        ctx->c.node_seed = b2_prefix_seed("node");
+        ctx->c.gen = make_gen_doubles();
+        *out = ctx;
+        return STWO_ROCM_OK;
+    } catch (const Error& e) {
+        if (err && err_len) {
+            strncpy(err, e.what(), err_len - 1);
+            err[err_len - 1] = 0;
+        }
+        if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+        delete ctx;
+        return e.code;
+    }
+}
+
+int stwo_rocm_ctx_destroy(stwo_rocm_ctx* ctx) {
+    if (!ctx) return STWO_ROCM_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        Ctx& c = ctx->c;
+        (void)hipSetDevice(c.device);
+        (void)hipStreamSynchronize(c.stream);
+        for (auto& kv : c.live_blocks) (void)hipFree(kv.first);
+        for (auto& kv : c.free_blocks) (void)hipFree(kv.second);
+        c.live_blocks.clear();
+        c.free_blocks.clear();
+        if (c.h_pinned) (void)hipHostFree(c.h_pinned);
+        if (c.h_ring) (void)hipHostFree(c.h_ring);
+        (void)hipStreamDestroy(c.stream);
+    }
+    delete ctx;
+    return STWO_ROCM_OK;
+}
+
+const char* stwo_rocm_last_error(stwo_rocm_ctx* ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
+
+int stwo_rocm_ctx_identity(stwo_rocm_ctx* ctx, char* out, size_t out_len) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(out && out_len, "null output");
+        hipDeviceProp_t p;
+        SR_HIP(hipGetDeviceProperties(&p, c.device));
+        std::string s = std::string(p.name) + ";" + p.gcnArchName + ";" + std::to_string(p.multiProcessorCount) +
+                        ";" + std::to_string((unsigned long long)p.totalGlobalMem);
+        strncpy(out, s.c_str(), out_len - 1);
+        out[out_len - 1] = 0;
+    });
+}
+
+int stwo_rocm_synchronize(stwo_rocm_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) { c.sync(); });
+}
+
+int stwo_rocm_memory_info(stwo_rocm_ctx* ctx, uint64_t* live, uint64_t* cached, uint64_t* peak) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (live) *live = c.bytes_live;
+        if (cached) *cached = c.bytes_cached;
+        if (peak) *peak = c.peak_live;
+    });
+}
+
+int stwo_rocm_release_cached(stwo_rocm_ctx* ctx) {
+    return guarded(ctx, [&](Ctx& c) { c.trim(); });
+}
+
+int stwo_rocm_buffer_alloc(stwo_rocm_ctx* ctx, uint64_t bytes, void** dptr) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(dptr, "null output");
+        *dptr = c.alloc(bytes);
+    });
+}
+
+int stwo_rocm_buffer_free(stwo_rocm_ctx* ctx, void* dptr) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(c.live_blocks.count(dptr), "pointer not owned by this context");
+        c.free(dptr);
+    });
+}
+
+int stwo_rocm_buffer_upload(stwo_rocm_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (!bytes) return;
+        SR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
+        c.sync();
+    });
+}
+
+int stwo_rocm_buffer_download(stwo_rocm_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (!bytes) return;
+        SR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+int stwo_rocm_twiddles_prepare(stwo_rocm_ctx* ctx, uint32_t max_circle_log) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(max_circle_log >= 1 && max_circle_log <= 30, "circle log out of range");
+        c.ensure_tower(max_circle_log);
+        c.sync();
+    });
+}
+
+int stwo_rocm_twiddles_read(stwo_rocm_ctx* ctx, uint32_t log, uint32_t* tw, uint32_t* itw) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(log >= 1 && log <= 30, "circle log out of range");
+        c.ensure_tower(log);
+        uint64_t n = 1ull << (log - 1);
+        uint64_t off = (1ull << (c.tower_log - 1)) - n;  // suffix view (twiddle_tower.zig:66-87)
+        if (tw) SR_HIP(hipMemcpyAsync(tw, c.d_tw + off, n * 4, hipMemcpyDeviceToHost, c.stream));
+        if (itw) SR_HIP(hipMemcpyAsync(itw, c.d_itw + off, n * 4, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+int stwo_rocm_batch_inverse(stwo_rocm_ctx* ctx, const uint32_t* d_in, uint64_t n, uint32_t* d_out) {
+    return guarded(ctx, [&](Ctx& c) {
+        launch_batch_inverse(c, d_in, n, d_out);
+        c.sync();
+    });
+}
+
+int stwo_rocm_bit_reverse(stwo_rocm_ctx* ctx, uint32_t* d_vals, uint32_t log_size, uint32_t wpe) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(log_size <= 30 && (wpe == 1 || wpe == 4), "bit_reverse: bad shape");
+        launch_bit_reverse(c, d_vals, log_size, wpe);
+        c.sync();
+    });
+}
+
+int stwo_rocm_accumulate(stwo_rocm_ctx* ctx, uint32_t* d_acc, const uint32_t* d_src, uint64_t n_words) {
+    return guarded(ctx, [&](Ctx& c) {
+        launch_accumulate(c, d_acc, d_src, n_words);
+        c.sync();
+    });
+}
+
+static double timed(Ctx& c, const std::function<void()>& f) {
+    if (!c.ev0) {
+        SR_HIP(hipEventCreate(&c.ev0));
+        SR_HIP(hipEventCreate(&c.ev1));
+    }
+    SR_HIP(hipEventRecord(c.ev0, c.stream));
+    f();
+    SR_HIP(hipEventRecord(c.ev1, c.stream));
+    c.sync();
+    float ms = 0;
+    SR_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    return ms;
+}
+
+int stwo_rocm_circle_interpolate(stwo_rocm_ctx* ctx, uint32_t* const* cols, uint32_t n_cols, uint32_t log_size,
+                                 double* gpu_ms) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(cols || !n_cols, "null columns");
+        SR_CHECK(log_size >= 1 && log_size <= 30, "log size out of range");
+        if (log_size >= 3) c.ensure_tower(log_size);
+        DBuf d = upload(c, cols, n_cols);
+        double ms = timed(c, [&] {
+            launch_circle_fft(c, true, d.as<const uint32_t* const>(), d.as<uint32_t* const>(), n_cols, log_size,
+                              log_size);
+        });
+        if (gpu_ms) *gpu_ms = ms;
+    });
+}
+
+int stwo_rocm_circle_evaluate(stwo_rocm_ctx* ctx, const uint32_t* const* coeffs, uint32_t n_cols, uint32_t log_size,
+                              uint32_t log_blowup, uint32_t* const* out, double* gpu_ms) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK((coeffs && out) || !n_cols, "null columns");
+        uint32_t m = log_size + log_blowup;
+        SR_CHECK(m >= 1 && m <= 30, "log size out of range");
+        if (m >= 3) c.ensure_tower(m);
+        DBuf ds = upload(c, coeffs, n_cols), dd = upload(c, out, n_cols);
+        double ms = timed(c, [&] {
+            launch_circle_fft(c, false, ds.as<const uint32_t* const>(), dd.as<uint32_t* const>(), n_cols, m,
+                              log_size);
+        });
+        if (gpu_ms) *gpu_ms = ms;
+    });
+}
+
+int stwo_rocm_eval_at_point(stwo_rocm_ctx* ctx, const uint32_t* const* coeffs, uint32_t n_cols, uint32_t log_size,
+                            const uint32_t point[8], uint32_t* out_host) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(point && out_host && (coeffs || !n_cols), "null argument");
+        SR_CHECK(log_size <= 30, "log size out of range");
+        if (!n_cols) return;
+        qm31 px = qin(point), py = qin(point + 4);
+        qm31 f[32];
+        if (log_size >= 1) f[0] = py;
+        if (log_size >= 2) f[1] = px;
+        for (uint32_t k = 2; k < log_size; k++) {
+            qm31 s = qsqr(f[k - 1]);
+            f[k] = qsub(qadd(s, s), qone());
+        }
+        DBuf w(&c, (1ull << log_size) * 16);
+        launch_eval_weights(c, f, log_size, w.as<uint32_t>());
+        DBuf d = upload(c, coeffs, n_cols);
+        DBuf o(&c, (size_t)n_cols * 16);
+        launch_eval_columns(c, d.as<const uint32_t* const>(), n_cols, log_size, w.as<uint32_t>(), o.as<uint32_t>());
+        SR_HIP(hipMemcpyAsync(out_host, o.p, (size_t)n_cols * 16, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+int stwo_rocm_merkle_commit(stwo_rocm_ctx* ctx, const uint32_t* const* cols, const uint32_t* log_sizes,
+                            uint32_t n_cols, stwo_rocm_tree** out, uint8_t* root_out, double* gpu_ms) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(out, "null output");
+        SR_CHECK((cols && log_sizes) || !n_cols, "null columns");
+        std::vector<const uint32_t*> cv(cols, cols + n_cols);
+        std::vector<uint32_t> lv(log_sizes, log_sizes + n_cols);
+        DeviceTree* t = nullptr;
+        double ms = timed(c, [&] { t = merkle_commit(c, cv, lv); });
+        if (n_cols) tree_read_root(c, *t);
+        if (root_out) memcpy(root_out, t->root, 32);
+        *out = new stwo_rocm_tree{t};
+        if (gpu_ms) *gpu_ms = ms;
+    });
+}
+
+int stwo_rocm_merkle_parent_layer(stwo_rocm_ctx* ctx, const uint8_t* d_children, uint32_t log_size,
+                                  uint8_t* d_parents) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(log_size <= 30, "log size out of range");
+        launch_merkle_parent_layer(c, d_children, log_size, d_parents);
+        c.sync();
+    });
+}
+
+int stwo_rocm_tree_root(stwo_rocm_ctx* ctx, stwo_rocm_tree* tree, uint8_t root_out[32]) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(tree && root_out, "null argument");
+        memcpy(root_out, tree->t->root, 32);
+    });
+}
+
+int stwo_rocm_tree_max_log(stwo_rocm_tree* tree, uint32_t* out) {
+    if (!tree || !out) return STWO_ROCM_EINVAL;
+    *out = tree->t->max_log;
+    return STWO_ROCM_OK;
+}
+
+int stwo_rocm_tree_read_hashes(stwo_rocm_ctx* ctx, stwo_rocm_tree* tree, uint32_t layer_log, const uint32_t* indices,
+                               uint32_t n, uint8_t* out_host) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(tree && (indices || !n) && (out_host || !n), "null argument");
+        SR_CHECK(layer_log <= tree->t->max_log, "layer out of range");
+        if (!n) return;
+        std::vector<uint64_t> addr;
+        for (uint32_t i = 0; i < n; i++) {
+            SR_CHECK(indices[i] < (1ull << layer_log), "hash index out of range");
+            const uint32_t* p = (const uint32_t*)(tree->t->layer(layer_log) + (uint64_t)indices[i] * 32);
+            for (int k = 0; k < 8; k++) addr.push_back((uint64_t)(uintptr_t)(p + k));
+        }
+        DBuf da = upload(c, addr.data(), addr.size());
+        DBuf o(&c, addr.size() * 4);
+        launch_gather(c, da.as<uint64_t>(), addr.size(), o.as<uint32_t>());
+        SR_HIP(hipMemcpyAsync(out_host, o.p, addr.size() * 4, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+int stwo_rocm_tree_decommit(stwo_rocm_ctx* ctx, stwo_rocm_tree* tree, const uint64_t* positions, uint32_t n_positions,
+                            uint8_t* witness_out, uint32_t witness_cap, uint32_t* n_witness) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(tree && n_witness && (positions || !n_positions), "null argument");
+        const DeviceTree& t = *tree->t;
+        std::vector<uint64_t> prev;
+        for (uint32_t i = 0; i < n_positions; i++) {
+            SR_CHECK(positions[i] < (1ull << t.max_log), "position out of range");
+            SR_CHECK(i == 0 || positions[i - 1] <= positions[i], "positions must be sorted");
+            if (i == 0 || positions[i - 1] != positions[i]) prev.push_back(positions[i]);
+        }
+        std::vector<uint64_t> addr;
+        for (int32_t layer = (int32_t)t.max_log - 1; layer >= 0; layer--) {
+            std::vector<uint64_t> cur;
+            size_t q = 0;
+            while (q < prev.size()) {
+                uint64_t first = prev[q];
+                bool sib = q + 1 < prev.size() && (first ^ 1) == prev[q + 1];
+                if (!sib) {
+                    const uint32_t* p = (const uint32_t*)(t.layer(layer + 1) + (first ^ 1) * 32);
+                    for (int k = 0; k < 8; k++) addr.push_back((uint64_t)(uintptr_t)(p + k));
+                }
+                cur.push_back(first >> 1);
+                q += sib ? 2 : 1;
+            }
+            prev.swap(cur);
+        }
+        *n_witness = (uint32_t)(addr.size() / 8);
+        SR_CHECK(*n_witness <= witness_cap, "witness buffer too small");
+        if (addr.empty()) return;
+        SR_CHECK(witness_out, "null witness buffer");
+        DBuf da = upload(c, addr.data(), addr.size());
+        DBuf o(&c, addr.size() * 4);
+        launch_gather(c, da.as<uint64_t>(), addr.size(), o.as<uint32_t>());
+        SR_HIP(hipMemcpyAsync(witness_out, o.p, addr.size() * 4, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+int stwo_rocm_tree_free(stwo_rocm_ctx* ctx, stwo_rocm_tree* tree) {
+    return guarded(ctx, [&](Ctx& c) {
+        if (!tree) return;
+        c.sync();
+        tree_free(c, tree->t);
+        delete tree;
+    });
+}
+
+int stwo_rocm_fri_fold_line(stwo_rocm_ctx* ctx, const uint32_t* d_src, uint32_t log_src, const uint32_t alpha[4],
+                            uint32_t* d_dst) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(d_src && d_dst && alpha, "null argument");
+        SR_CHECK(log_src >= 1 && log_src <= 29, "log size out of range");
+        launch_fold_line(c, d_src, log_src, qin(alpha), d_dst, nullptr);
+        c.sync();
+    });
+}
+
+int stwo_rocm_fri_fold_circle_into_line(stwo_rocm_ctx* ctx, const uint32_t* const* src_coords, uint32_t log_src,
+                                        const uint32_t alpha[4], uint32_t* d_dst) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(src_coords && d_dst && alpha, "null argument");
+        SR_CHECK(log_src >= 1 && log_src <= 30, "log size out of range");
+        launch_fold_circle_into_line(c, src_coords, log_src, qin(alpha), d_dst, true, nullptr);
+        c.sync();
+    });
+}
+
+int stwo_rocm_compute_quotients(stwo_rocm_ctx* ctx, uint32_t L, const uint32_t* const* columns,
+                                const uint32_t* column_log_sizes, uint32_t n_columns, const uint32_t* sample_columns,
+                                const uint32_t* sample_points, const uint32_t* sample_values, uint32_t n_samples,
+                                const uint32_t random_coeff[4], uint32_t* const* out_coords,
+                                stwo_rocm_tree** tree_out) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(L >= 1 && L <= 30, "lifting log size out of range");
+        SR_CHECK(out_coords && random_coeff, "null argument");
+        SR_CHECK((columns && column_log_sizes) || !n_columns, "null columns");
+        SR_CHECK((sample_columns && sample_points && sample_values) || !n_samples, "null samples");
+        for (uint32_t i = 0; i < n_columns; i++)
+            SR_CHECK(column_log_sizes[i] >= 1 && column_log_sizes[i] <= L, "column log size out of range");
+        // Samples per column in input order (tree/column/sample order).
+        std::vector<std::vector<uint32_t>> per_col(n_columns);
+        for (uint32_t s = 0; s < n_samples; s++) {
+            SR_CHECK(sample_columns[s] < n_columns, "sample column out of range");
+            SR_CHECK(s == 0 || sample_columns[s - 1] <= sample_columns[s], "samples must be in column order");
+            per_col[sample_columns[s]].push_back(s);
+        }
+        struct Pt8 {
+            qm31 x, y;
+        };
+        std::vector<Pt8> bpts;
+        std::vector<std::vector<std::pair<uint32_t, qm31>>> bent;
+        std::vector<qm31> sa, sb;
+        qm31 alpha = qin(random_coeff), rp = qone();
+        uint32_t lift_gen = subgroup_gen(L);
+        auto add_sample = [&](uint32_t col, Pt8 p, qm31 v) {
+            size_t bi = 0;
+            for (; bi < bpts.size(); bi++)
+                if (qeq(bpts[bi].x, p.x) && qeq(bpts[bi].y, p.y)) break;
+            if (bi == bpts.size()) {
+                bpts.push_back(p);
+                bent.emplace_back();
+                sa.push_back(qzero());
+                sb.push_back(qzero());
+            }
+            // complexConjugateLineCoeffs (core/constraints.zig:87-97)
+            SR_CHECK(!qeq(p.y, qconj(p.y)), "degenerate sample point (y in the base field)");
+            qm31 a = qsub(qconj(v), v);
+            qm31 cc = qsub(qconj(p.y), p.y);
+            qm31 b = qsub(qmul(v, cc), qmul(a, p.y));
+            bent[bi].push_back({col, qmul(rp, cc)});
+            sa[bi] = qadd(sa[bi], qmul(rp, a));
+            sb[bi] = qadd(sb[bi], qmul(rp, b));
+            rp = qmul(rp, alpha);
+        };
+        for (uint32_t col = 0; col < n_columns; col++) {
+            auto& ss = per_col[col];
+            if (ss.size() == 2) {
+                // Periodicity sample (core/pcs/quotients/samples.zig:270-285).
+                uint32_t s = ss[1];
+                pt g = idx_to_point(c.gen, (uint32_t)(((uint64_t)lift_gen << column_log_sizes[col]) & ORDER_MASK));
+                Pt8 p = {qin(sample_points + 8 * s), qin(sample_points + 8 * s + 4)};
+                Pt8 q = {qsub(qmul(p.x, qfromm(g.x)), qmul(p.y, qfromm(g.y))),
+                         qadd(qmul(p.x, qfromm(g.y)), qmul(p.y, qfromm(g.x)))};
+                add_sample(col, q, qin(sample_values + 4 * s));
+            }
+            for (uint32_t s : ss) add_sample(col, {qin(sample_points + 8 * s), qin(sample_points + 8 * s + 4)},
+                                             qin(sample_values + 4 * s));
+        }
+        std::vector<QuotientBatchDev> batches;
+        std::vector<uint32_t> ecols, ecf;
+        for (size_t bi = 0; bi < bpts.size(); bi++) {
+            QuotientBatchDev bd;
+            cm31 prx = bpts[bi].x.c0, pry = bpts[bi].y.c0, pix = bpts[bi].x.c1, piy = bpts[bi].y.c1;
+            bd.det = csub(cmul(prx, piy), cmul(pry, pix));
+            bd.pix = pix;
+            bd.piy = piy;
+            bd.sum_a = sa[bi];
+            bd.sum_b = sb[bi];
+            bd.first = (uint32_t)ecols.size();
+            bd.count = (uint32_t)bent[bi].size();
+            for (auto& e : bent[bi]) {
+                ecols.push_back(e.first);
+                uint32_t w4[4];
+                qstore(e.second, w4);
+                ecf.insert(ecf.end(), w4, w4 + 4);
+            }
+            batches.push_back(bd);
+        }
+        DeviceTree* t = nullptr;
+        try {
+            if (tree_out) {
+                t = new DeviceTree();
+                t->max_log = L;
+                t->layers = (uint8_t*)c.alloc(((2ull << L) - 1) * 32);
+            }
+            DBuf dc = upload(c, columns, n_columns), dl = upload(c, column_log_sizes, n_columns);
+            DBuf de = upload(c, ecols.data(), ecols.size()), df = upload(c, ecf.data(), ecf.size());
+            DBuf db = upload(c, batches.data(), batches.size());
+            launch_quotients(c, dc.as<const uint32_t* const>(), dl.as<uint32_t>(), de.as<uint32_t>(),
+                             df.as<uint32_t>(), db.as<QuotientBatchDev>(), (uint32_t)batches.size(), L, out_coords,
+                             t ? t->layer(L) : nullptr);
+            if (t) {
+                launch_merkle_parents(c, *t);
+                tree_read_root(c, *t);
+                *tree_out = new stwo_rocm_tree{t};
+            }
+            c.sync();
+        } catch (...) {
+            if (t) tree_free(c, t);
+            throw;
+        }
+    });
+}
+
+int stwo_rocm_widefib_trace(stwo_rocm_ctx* ctx, uint32_t log_n_rows, uint32_t seq_len, uint32_t mode, uint64_t seed,
+                            uint32_t* d_out) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(d_out, "null output");
+        SR_CHECK(log_n_rows >= 1 && log_n_rows <= 30 && seq_len >= 2 && mode <= 1, "bad trace shape");
+        launch_widefib_trace(c, log_n_rows, seq_len, mode, seed, d_out, 1ull << log_n_rows);
+        c.sync();
+    });
+}
+
+int stwo_rocm_widefib_composition(stwo_rocm_ctx* ctx, const uint32_t* const* eval_cols, uint32_t n_cols,
+                                  uint32_t log_n_rows, const uint32_t random_coeff[4], uint32_t* const* out_coords) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(eval_cols && out_coords && random_coeff, "null argument");
+        SR_CHECK(n_cols >= 3 && log_n_rows >= 1 && log_n_rows <= 29, "bad shape");
+        qm31 a = qin(random_coeff);
+        uint32_t NC = n_cols - 2, CL = log_n_rows + 1;
+        std::vector<uint32_t> pw(4 * NC);
+        qm31 cur = qone();
+        for (uint32_t k = 0; k < NC; k++) {
+            qstore(cur, &pw[4 * k]);
+            cur = qmul(cur, a);
+        }
+        uint32_t den[2];
+        for (int r = 0; r < 2; r++) {
+            pt dp = idx_to_point(c.gen, domain_index_at(CL, (uint64_t)r));
+            pt init = idx_to_point(c.gen, subgroup_gen(log_n_rows + 1));
+            pt s = padd(padd(dp, pconj(init)), init);
+            uint32_t x = s.x;
+            for (uint32_t i = 1; i < log_n_rows; i++) x = mdouble_x(x);
+            den[r] = minv(x);
+        }
+        DBuf dp = upload(c, pw.data(), pw.size());
+        DBuf dc = upload(c, eval_cols, n_cols);
+        launch_widefib_composition(c, dc.as<const uint32_t* const>(), n_cols, CL, log_n_rows, dp.as<uint32_t>(),
+                                   den[0], den[1], out_coords);
+        c.sync();
+    });
+}
+
+int stwo_rocm_pow_grind(stwo_rocm_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits, uint64_t* nonce) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(digest && nonce, "null argument");
+        SR_CHECK(pow_bits <= 40, "pow_bits out of range");
+        Channel ch;
+        memcpy(ch.digest, digest, 32);
+        *nonce = 0;
+        if (pow_bits == 0) return;
+        uint8_t prefix[32];
+        ch.pow_prefix(pow_bits, prefix);
+        uint64_t chunk = pow_bits <= 16 ? (1ull << 20) : (1ull << 24);
+        for (uint64_t base = 0;; base += chunk) {
+            uint64_t r = launch_pow_grind(c, prefix, pow_bits, base, chunk);
+            if (r != UINT64_MAX) {
+                *nonce = r;
+                return;
+            }
+        }
+    });
+}
+
+int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params,
+                                   const uint32_t* d_trace, uint8_t* proof_out, uint64_t cap, uint64_t* proof_len,
+                                   stwo_rocm_prove_stats* stats) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(params && proof_len, "null argument");
+        ProveParams p;
+        p.log_n_rows = params->log_n_rows;
+        p.sequence_len = params->sequence_len;
+        p.pow_bits = params->pow_bits;
+        p.log_blowup_factor = params->log_blowup_factor;
+        p.log_last_layer_degree_bound = params->log_last_layer_degree_bound;
+        p.n_queries = params->n_queries;
+        p.trace_mode = params->trace_mode;
+        p.trace_seed = params->trace_seed;
+        SR_CHECK(p.trace_mode <= 1, "trace_mode must be 0 or 1");
+        std::string proof;
+        ProveStats st;
+        prove_wide_fibonacci(c, p, d_trace, proof, &st);
+        *proof_len = proof.size();
+        if (stats) {
+            for (int i = 0; i < STAGES; i++) {
+                stats->stage_gpu_ms[i] = st.stage_gpu_ms[i];
+                stats->stage_host_ms[i] = st.stage_host_ms[i];
+            }
+            stats->total_gpu_ms = st.total_gpu_ms;
+            stats->total_host_ms = st.total_host_ms;
+            stats->peak_hbm_bytes = st.peak_hbm_bytes;
+        }
+        SR_CHECK(proof.size() <= cap, "proof buffer too small");
+        SR_CHECK(proof_out, "null proof buffer");
+        memcpy(proof_out, proof.data(), proof.size());
+    });
+}
+
+}  // extern "C"

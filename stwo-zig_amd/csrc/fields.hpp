@@ -1,0 +1,173 @@
+// Exact M31 / CM31 / QM31 arithmetic and the M31 circle group, usable from
+// host and device code.
+//
+// Field laws follow src/core/fields/m31.zig:49-171 (canonical [0,p) words,
+// one-fold product reduction), cm31.zig:97-129 (Karatsuba) and
+// qm31.zig:13,153-222,269-284 (u^2 = 2+i, mulByR = (2a-b, a+2b)).  The circle
+// group follows src/core/circle.zig:8-374.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SR_HD __host__ __device__ __forceinline__
+
+namespace sr {
+
+constexpr uint32_t P = 0x7fffffffu;
+
+SR_HD uint32_t madd(uint32_t a, uint32_t b) {
+    uint32_t s = a + b;
+    return s >= P ? s - P : s;
+}
+SR_HD uint32_t msub(uint32_t a, uint32_t b) { return a >= b ? a - b : a + P - b; }
+SR_HD uint32_t mneg(uint32_t a) { return a == 0 ? 0 : P - a; }
+// Product of two canonical words is < p^2 < 2^62: one Mersenne fold lands in
+// [0, 2p) and a conditional subtraction canonicalises (m31.zig:164-171).
+SR_HD uint32_t mmul(uint32_t a, uint32_t b) {
+    uint64_t x = (uint64_t)a * b;
+    uint32_t r = (uint32_t)(x & P) + (uint32_t)(x >> 31);
+    return r >= P ? r - P : r;
+}
+// Fold a 64-bit accumulator of folded products to a canonical word (m31.zig:150-158).
+SR_HD uint32_t mred64(uint64_t x) {
+    uint64_t t = (x & P) + (x >> 31);
+    t = (t & P) + (t >> 31);
+    uint32_t r = (uint32_t)t;
+    return r >= P ? r - P : r;
+}
+// One Mersenne fold of a product, < 2^32; 2^32 such terms fit a u64 accumulator.
+SR_HD uint64_t mfold(uint64_t x) { return (x & P) + (x >> 31); }
+SR_HD uint32_t msqr(uint32_t a) { return mmul(a, a); }
+SR_HD uint32_t mpow(uint32_t a, uint32_t e) {
+    uint32_t acc = 1;
+    while (e) {
+        if (e & 1) acc = mmul(acc, a);
+        a = mmul(a, a);
+        e >>= 1;
+    }
+    return acc;
+}
+// a^(p-2); p-2 = 2^31 - 3.  Inverse is unique, so any exponentiation chain is exact.
+SR_HD uint32_t minv(uint32_t a) {
+    // a^(2^k - 1) ladder: t = a^(2^29-1), result = t^4 * a  (2^31-3 = (2^29-1)*4 + 1)
+    uint32_t t = a;  // a^(2^1-1)
+    for (int i = 1; i < 29; i++) t = mmul(msqr(t), a);  // a^(2^(i+1)-1)
+    return mmul(msqr(msqr(t)), a);
+}
+
+struct cm31 {
+    uint32_t a, b;
+};
+SR_HD cm31 cadd(cm31 x, cm31 y) { return {madd(x.a, y.a), madd(x.b, y.b)}; }
+SR_HD cm31 csub(cm31 x, cm31 y) { return {msub(x.a, y.a), msub(x.b, y.b)}; }
+SR_HD cm31 cneg(cm31 x) { return {mneg(x.a), mneg(x.b)}; }
+SR_HD cm31 cmul(cm31 x, cm31 y) {
+    uint32_t ac = mmul(x.a, y.a), bd = mmul(x.b, y.b);
+    uint32_t cr = mmul(madd(x.a, x.b), madd(y.a, y.b));
+    return {msub(ac, bd), msub(msub(cr, ac), bd)};
+}
+SR_HD cm31 cmulm(cm31 x, uint32_t m) { return {mmul(x.a, m), mmul(x.b, m)}; }
+SR_HD cm31 cinv(cm31 x) {
+    uint32_t d = minv(madd(mmul(x.a, x.a), mmul(x.b, x.b)));
+    return {mmul(x.a, d), mmul(mneg(x.b), d)};
+}
+SR_HD cm31 cmul_r(cm31 v) { return {msub(madd(v.a, v.a), v.b), madd(v.a, madd(v.b, v.b))}; }
+
+struct qm31 {
+    cm31 c0, c1;
+};
+SR_HD qm31 qzero() { return {{0, 0}, {0, 0}}; }
+SR_HD qm31 qone() { return {{1, 0}, {0, 0}}; }
+SR_HD qm31 qfromm(uint32_t m) { return {{m, 0}, {0, 0}}; }
+SR_HD qm31 qadd(qm31 x, qm31 y) { return {cadd(x.c0, y.c0), cadd(x.c1, y.c1)}; }
+SR_HD qm31 qsub(qm31 x, qm31 y) { return {csub(x.c0, y.c0), csub(x.c1, y.c1)}; }
+SR_HD qm31 qneg(qm31 x) { return {cneg(x.c0), cneg(x.c1)}; }
+SR_HD qm31 qmul(qm31 x, qm31 y) {
+    cm31 ac = cmul(x.c0, y.c0), bd = cmul(x.c1, y.c1);
+    cm31 cr = csub(csub(cmul(cadd(x.c0, x.c1), cadd(y.c0, y.c1)), ac), bd);
+    return {cadd(ac, cmul_r(bd)), cr};
+}
+SR_HD qm31 qmulm(qm31 x, uint32_t m) { return {cmulm(x.c0, m), cmulm(x.c1, m)}; }
+SR_HD qm31 qmulc(qm31 x, cm31 c) { return {cmul(x.c0, c), cmul(x.c1, c)}; }
+SR_HD qm31 qsqr(qm31 x) { return qmul(x, x); }
+SR_HD qm31 qconj(qm31 x) { return {x.c0, cneg(x.c1)}; }
+SR_HD bool qeq(qm31 x, qm31 y) {
+    return x.c0.a == y.c0.a && x.c0.b == y.c0.b && x.c1.a == y.c1.a && x.c1.b == y.c1.b;
+}
+SR_HD bool qiszero(qm31 x) { return (x.c0.a | x.c0.b | x.c1.a | x.c1.b) == 0; }
+SR_HD qm31 qinv(qm31 x) {
+    cm31 b2 = cmul(x.c1, x.c1);
+    cm31 rb2 = cadd(cadd(b2, b2), cm31{mneg(b2.b), b2.a});
+    cm31 den = csub(cmul(x.c0, x.c0), rb2);
+    cm31 di = cinv(den);
+    return {cmul(x.c0, di), cmul(cneg(x.c1), di)};
+}
+SR_HD qm31 qload(const uint32_t* w) { return {{w[0], w[1]}, {w[2], w[3]}}; }
+SR_HD void qstore(qm31 x, uint32_t* w) {
+    w[0] = x.c0.a;
+    w[1] = x.c0.b;
+    w[2] = x.c1.a;
+    w[3] = x.c1.b;
+}
+
+// ---- circle group over M31 (circle.zig) ----
+struct pt {
+    uint32_t x, y;
+};
+SR_HD pt padd(pt a, pt b) {
+    return {msub(mmul(a.x, b.x), mmul(a.y, b.y)), madd(mmul(a.x, b.y), mmul(a.y, b.x))};
+}
+SR_HD pt pconj(pt a) { return {a.x, mneg(a.y)}; }
+SR_HD uint32_t mdouble_x(uint32_t x) {
+    uint32_t s = msqr(x);
+    return msub(madd(s, s), 1);
+}
+constexpr uint32_t ORDER_MASK = 0x7fffffffu;
+SR_HD uint32_t idx_neg(uint32_t i) { return (0x80000000u - (i & ORDER_MASK)) & ORDER_MASK; }
+SR_HD uint32_t subgroup_gen(uint32_t log_size) { return 1u << (31 - log_size); }
+
+// Generator doubles G_k = 2^k * (2, 1268011823) (circle.zig:137-143,226-237).
+struct GenDoubles {
+    pt d[31];
+};
+SR_HD GenDoubles make_gen_doubles() {
+    GenDoubles g;
+    pt cur = {2u, 1268011823u};
+    for (int i = 0; i < 31; i++) {
+        g.d[i] = cur;
+        cur = padd(cur, cur);
+    }
+    return g;
+}
+// CirclePointIndex.toPoint (circle.zig:187-196).
+SR_HD pt idx_to_point(const GenDoubles& g, uint32_t idx) {
+    pt acc = {1, 0};
+    uint32_t bits = idx & ORDER_MASK;
+    while (bits) {
+        int b = __builtin_ctz(bits);
+        acc = padd(acc, g.d[b]);
+        bits &= bits - 1;
+    }
+    return acc;
+}
+// Coset.halfOdds(log) = (subgroupGen(log+2), step subgroupGen(log)) (circle.zig:280-282).
+SR_HD uint32_t half_odds_initial(uint32_t log_size) { return subgroup_gen(log_size + 2); }
+// CanonicCoset(log).circleDomain().indexAt(i) (canonic.zig:24-26; domain.zig:58-62).
+SR_HD uint32_t domain_index_at(uint32_t log_size, uint64_t i) {
+    uint32_t init = half_odds_initial(log_size - 1), step = subgroup_gen(log_size - 1);
+    uint64_t half = 1ull << (log_size - 1);
+    if (i < half) return (uint32_t)((init + (uint64_t)step * i) & ORDER_MASK);
+    return idx_neg((uint32_t)((init + (uint64_t)step * (i - half)) & ORDER_MASK));
+}
+SR_HD uint32_t bitrev32(uint32_t i, uint32_t log_size) {
+    if (log_size == 0) return i;
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_bitreverse32(i) >> (32 - log_size);
+#else
+    uint32_t r = 0;
+    for (uint32_t k = 0; k < log_size; k++) r |= ((i >> k) & 1u) << (log_size - 1 - k);
+    return r;
+#endif
+}
+
+}  // namespace sr

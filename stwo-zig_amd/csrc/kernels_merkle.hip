@@ -1,0 +1,231 @@
+// Lifted Blake2s Merkle commitment on gfx950.
+//
+// Leaves (src/prover/vcs_lifted/leaves.zig:24-200; blake2_merkle.zig:193-215):
+//   leaf[p] = Blake2s("leaf"+0^60 || LE32(col[lift(p)]) for columns sorted by
+//   (log size asc, original index)), lift(p) = ((p >> (L-l+1)) << 1) + (p & 1).
+// One leaf per lane; the 64-byte prefix is pre-absorbed into a seed state, then
+// 16 column words per compression are read with coalesced loads (consecutive
+// lanes read consecutive rows of the same column).
+//
+// Parents (layers.zig:99-140; blake2_merkle.zig:112-154): node = Blake2s(
+// "node"+0^60 || left || right), one compression from the node seed.  One
+// launch builds up to 9 levels: each block hashes 256 parents from HBM and
+// then halves them in LDS, writing every level, so a 2^21-leaf tree takes three
+// launches instead of 21.
+#include "runtime.hpp"
+
+namespace sr {
+
+constexpr int MK_THREADS = 256;
+
+__device__ __forceinline__ void store_hash(uint8_t* __restrict__ out, uint64_t i, const uint32_t h[8]) {
+    uint4* o = (uint4*)(out + i * 32);
+    o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+    o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+}
+
+__global__ __launch_bounds__(MK_THREADS) void merkle_leaves_kernel(const uint32_t* const* __restrict__ cols,
+                                                                   const uint32_t* __restrict__ logs, uint32_t n_cols,
+                                                                   uint32_t max_log, B2State seed,
+                                                                   uint8_t* __restrict__ leaves) {
+    uint64_t p = (uint64_t)blockIdx.x * MK_THREADS + threadIdx.x;
+    if (p >= (1ull << max_log)) return;
+    uint32_t h[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) h[i] = seed.h[i];
+    const uint32_t nblocks = (n_cols + 15) >> 4;
+    for (uint32_t blk = 0; blk < nblocks; blk++) {
+        uint32_t m[16];
+#pragma unroll
+        for (int w = 0; w < 16; w++) {
+            uint32_t c = blk * 16 + w;
+            uint32_t v = 0;
+            if (c < n_cols) {
+                uint32_t sh = max_log - logs[c] + 1;
+                uint64_t idx = ((p >> sh) << 1) + (p & 1);
+                v = cols[c][idx];
+            }
+            m[w] = v;
+        }
+        const bool last = blk + 1 == nblocks;
+        const uint32_t t = 64 + (last ? 4 * n_cols : 64 * (blk + 1));
+        b2_compress(h, m, t, last);
+    }
+    store_hash(leaves, p, h);
+}
+
+// Leaves of a QM31 column in AoS form: 4 coordinate words per leaf (FRI layers).
+__global__ __launch_bounds__(MK_THREADS) void merkle_leaves_qm31_kernel(const uint4* __restrict__ vals,
+                                                                        uint32_t log_size, B2State seed,
+                                                                        uint8_t* __restrict__ leaves) {
+    uint64_t p = (uint64_t)blockIdx.x * MK_THREADS + threadIdx.x;
+    if (p >= (1ull << log_size)) return;
+    uint4 v = vals[p];
+    uint32_t m[16] = {v.x, v.y, v.z, v.w, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t h[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) h[i] = seed.h[i];
+    b2_compress(h, m, 80, true);
+    store_hash(leaves, p, h);
+}
+
+// Builds `levels` parent levels below layer k.  Block b covers parents
+// [b*P0, (b+1)*P0) of level k-1 with P0 = 2^(levels-1).
+__global__ __launch_bounds__(MK_THREADS) void merkle_parents_kernel(uint8_t* __restrict__ tree, uint32_t k,
+                                                                    uint32_t levels, B2State seed) {
+    __shared__ uint32_t sh[MK_THREADS * 8];
+    const uint32_t P0 = 1u << (levels - 1);
+    const uint32_t t = threadIdx.x;
+    const uint8_t* child = tree + (((uint64_t)1 << k) - 1) * 32;
+    uint32_t count = P0;
+    uint32_t level = k - 1;
+    uint64_t first = (uint64_t)blockIdx.x * P0;
+    // level k-1 from HBM children
+    if (t < count) {
+        const uint4* c4 = (const uint4*)(child + (first + t) * 64);
+        uint4 a = c4[0], b = c4[1], c = c4[2], d = c4[3];
+        uint32_t m[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
+        uint32_t h[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) h[i] = seed.h[i];
+        b2_compress(h, m, 128, true);
+        store_hash(tree + (((uint64_t)1 << level) - 1) * 32, first + t, h);
+#pragma unroll
+        for (int i = 0; i < 8; i++) sh[t * 8 + i] = h[i];
+    }
+    for (uint32_t lv = 1; lv < levels; lv++) {
+        __syncthreads();
+        count >>= 1;
+        level--;
+        first >>= 1;
+        uint32_t h[8];
+        if (t < count) {
+            uint32_t m[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) m[i] = sh[t * 16 + i];
+#pragma unroll
+            for (int i = 0; i < 8; i++) h[i] = seed.h[i];
+            b2_compress(h, m, 128, true);
+            store_hash(tree + (((uint64_t)1 << level) - 1) * 32, first + t, h);
+        }
+        __syncthreads();
+        if (t < count) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) sh[t * 8 + i] = h[i];
+        }
+    }
+}
+
+// Single parent layer from a separate children array (commit_on_layer shape).
+__global__ __launch_bounds__(MK_THREADS) void merkle_parent_layer_kernel(const uint8_t* __restrict__ children,
+                                                                         uint32_t log_size, B2State seed,
+                                                                         uint8_t* __restrict__ parents) {
+    uint64_t i = (uint64_t)blockIdx.x * MK_THREADS + threadIdx.x;
+    if (i >= (1ull << log_size)) return;
+    const uint4* c4 = (const uint4*)(children + i * 64);
+    uint4 a = c4[0], b = c4[1], c = c4[2], d = c4[3];
+    uint32_t m[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
+    uint32_t h[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) h[k] = seed.h[k];
+    b2_compress(h, m, 128, true);
+    store_hash(parents, i, h);
+}
+
+void launch_merkle_parent_layer(Ctx& c, const uint8_t* d_children, uint32_t log_size, uint8_t* d_parents) {
+    uint64_t n = 1ull << log_size;
+    hipLaunchKernelGGL(merkle_parent_layer_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)),
+                       dim3(MK_THREADS), 0, c.stream, d_children, log_size, c.node_seed, d_parents);
+    SR_HIP(hipGetLastError());
+}
+
+void launch_merkle_leaves(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_logs, uint32_t n_cols,
+                          uint32_t max_log, uint8_t* d_leaves) {
+    uint64_t n = 1ull << max_log;
+    hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)), dim3(MK_THREADS), 0,
+                       c.stream, d_cols, d_logs, n_cols, max_log, c.leaf_seed, d_leaves);
+    SR_HIP(hipGetLastError());
+}
+
+void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size, uint8_t* d_leaves) {
+    uint64_t n = 1ull << log_size;
+    hipLaunchKernelGGL(merkle_leaves_qm31_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)),
+                       dim3(MK_THREADS), 0, c.stream, (const uint4*)d_qm31, log_size, c.leaf_seed, d_leaves);
+    SR_HIP(hipGetLastError());
+}
+
+void launch_merkle_parents(Ctx& c, DeviceTree& t) {
+    uint32_t k = t.max_log;
+    while (k > 0) {
+        uint32_t levels = k < 9 ? k : 9;
+        uint32_t blocks = 1u << (k - levels);
+        hipLaunchKernelGGL(merkle_parents_kernel, dim3(blocks), dim3(MK_THREADS), 0, c.stream, t.layers, k, levels,
+                           c.node_seed);
+        SR_HIP(hipGetLastError());
+        k -= levels;
+    }
+}
+
+static DeviceTree* new_tree(Ctx& c, uint32_t max_log) {
+    SR_CHECK(max_log <= 30, "merkle: log size too large");
+    DeviceTree* t = new DeviceTree();
+    t->max_log = max_log;
+    t->layers = (uint8_t*)c.alloc(((2ull << max_log) - 1) * 32);
+    return t;
+}
+
+DeviceTree* merkle_commit(Ctx& c, const std::vector<const uint32_t*>& cols, const std::vector<uint32_t>& logs) {
+    SR_CHECK(cols.size() == logs.size(), "merkle: column/log count mismatch");
+    std::vector<uint32_t> order(cols.size());
+    for (size_t i = 0; i < order.size(); i++) {
+        SR_CHECK(logs[i] >= 1 && logs[i] <= 30, "merkle: column length must be a power of two >= 2");
+        order[i] = (uint32_t)i;
+    }
+    // sortByLogSizeAsc, stable on the original index (columns.zig:12-46).
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return logs[a] < logs[b]; });
+    uint32_t max_log = order.empty() ? 0 : logs[order.back()];
+    DeviceTree* t = new_tree(c, max_log);
+    if (cols.empty()) {
+        // No columns: a single leaf Blake2s(prefix) (leaves.zig:30-34).
+        uint8_t prefix[64] = {'l', 'e', 'a', 'f'};
+        uint8_t h[32];
+        b2_hash(prefix, 64, h);
+        SR_HIP(hipMemcpyAsync(t->layers, c.stage(h, 32), 32, hipMemcpyHostToDevice, c.stream));
+        memcpy(t->root, h, 32);
+        return t;
+    }
+    std::vector<const uint32_t*> sc(cols.size());
+    std::vector<uint32_t> sl(cols.size());
+    for (size_t i = 0; i < order.size(); i++) {
+        sc[i] = cols[order[i]];
+        sl[i] = logs[order[i]];
+    }
+    DBuf d_cols = upload(c, sc.data(), sc.size());
+    DBuf d_logs = upload(c, sl.data(), sl.size());
+    launch_merkle_leaves(c, d_cols.as<const uint32_t* const>(), d_logs.as<uint32_t>(), (uint32_t)sc.size(), max_log,
+                         t->layer(max_log));
+    launch_merkle_parents(c, *t);
+    return t;
+}
+
+DeviceTree* merkle_commit_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size) {
+    DeviceTree* t = new_tree(c, log_size);
+    launch_merkle_leaves_qm31(c, d_qm31, log_size, t->layer(log_size));
+    launch_merkle_parents(c, *t);
+    return t;
+}
+
+void tree_read_root(Ctx& c, DeviceTree& t) {
+    uint8_t* h = c.pinned(32);
+    SR_HIP(hipMemcpyAsync(h, t.layers, 32, hipMemcpyDeviceToHost, c.stream));
+    SR_HIP(hipStreamSynchronize(c.stream));
+    memcpy(t.root, h, 32);
+}
+
+void tree_free(Ctx& c, DeviceTree* t) {
+    if (!t) return;
+    if (t->layers) c.free(t->layers);
+    delete t;
+}
+
+}  // namespace sr

@@ -1,0 +1,249 @@
+// Row-parallel proving kernels on gfx950: quotient accumulation (+ fused FRI
+// first-layer leaf hash), wide-Fibonacci composition and trace generation,
+// and sampled-value point evaluation.
+//
+// All are HBM-streaming kernels: each reads its input columns once with
+// coalesced loads (consecutive lanes = consecutive rows of one column) and
+// accumulates M31 products in 64-bit registers (one Mersenne fold per product,
+// one canonicalisation per sum), which is exact and order independent.
+#include "runtime.hpp"
+
+namespace sr {
+
+__device__ __forceinline__ uint32_t lifted_row(uint64_t row, uint32_t log_size, uint32_t col_log) {
+    uint32_t sh = log_size - col_log + 1;
+    return (uint32_t)(((row >> sh) << 1) + (row & 1));
+}
+
+// Quotients (core/pcs/quotients/row_evaluation.zig:402-521): for row p with
+// domain point (x, y) = domain.at(bitrev(p)):
+//   q(p) = sum_b ( sum_{e in b} c_e * v_e(p) - (sum_a_b * y + sum_b_b) ) / den_b(p),
+//   den_b = det_b - piy_b * x + pix_b * y  (CM31).
+__global__ __launch_bounds__(256) void quotients_kernel(const uint32_t* const* __restrict__ cols,
+                                                        const uint32_t* __restrict__ col_logs,
+                                                        const uint32_t* __restrict__ entry_cols,
+                                                        const uint4* __restrict__ entry_coeffs,
+                                                        const QuotientBatchDev* __restrict__ batches,
+                                                        uint32_t n_batches, uint32_t log_size,
+                                                        const uint32_t* __restrict__ domain_xy,
+                                                        uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
+                                                        uint32_t* __restrict__ o2, uint32_t* __restrict__ o3,
+                                                        uint8_t* __restrict__ leaves, B2State seed) {
+    uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = 1u << log_size;
+    if (row >= n) return;
+    uint32_t x = domain_xy[row], y = domain_xy[n + row];
+    qm31 acc = qzero();
+    for (uint32_t bi = 0; bi < n_batches; bi++) {
+        QuotientBatchDev bt = batches[bi];
+        uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        for (uint32_t e = bt.first; e < bt.first + bt.count; e++) {
+            uint32_t col = entry_cols[e];
+            uint32_t v = cols[col][lifted_row(row, log_size, col_logs[col])];
+            uint4 cf = entry_coeffs[e];
+            s0 += mfold((uint64_t)cf.x * v);
+            s1 += mfold((uint64_t)cf.y * v);
+            s2 += mfold((uint64_t)cf.z * v);
+            s3 += mfold((uint64_t)cf.w * v);
+        }
+        qm31 num = {{mred64(s0), mred64(s1)}, {mred64(s2), mred64(s3)}};
+        num = qsub(num, qadd(qmulm(bt.sum_a, y), bt.sum_b));
+        cm31 den = cadd(csub(bt.det, cmulm(bt.piy, x)), cmulm(bt.pix, y));
+        acc = qadd(acc, qmulc(num, cinv(den)));
+    }
+    o0[row] = acc.c0.a;
+    o1[row] = acc.c0.b;
+    o2[row] = acc.c1.a;
+    o3[row] = acc.c1.b;
+    if (leaves) {
+        uint32_t m[16] = {acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t h[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) h[k] = seed.h[k];
+        b2_compress(h, m, 80, true);
+        uint4* o = (uint4*)(leaves + (uint64_t)row * 32);
+        o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+        o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+    }
+}
+
+void launch_quotients(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_col_logs, const uint32_t* d_entry_cols,
+                      const uint32_t* d_entry_coeffs, const QuotientBatchDev* d_batches, uint32_t n_batches,
+                      uint32_t log_size, uint32_t* const* out, uint8_t* d_leaf_out) {
+    const uint32_t* xy = c.domain_points(log_size);
+    uint32_t n = 1u << log_size;
+    hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_cols, d_col_logs,
+                       d_entry_cols, (const uint4*)d_entry_coeffs, d_batches, n_batches, log_size, xy, out[0], out[1],
+                       out[2], out[3], d_leaf_out, c.leaf_seed);
+    SR_HIP(hipGetLastError());
+}
+
+// Wide-Fibonacci composition (examples/wide_fibonacci/component.zig:146-228):
+//   row r: sum_{i} alpha^(N-1-i) * (c_{i+2} - c_i^2 - c_{i+1}^2) * Z^-1[r >> log_n]
+// on CanonicCoset(log_n + 1), N = n_cols - 2 constraints.
+__global__ __launch_bounds__(256) void widefib_composition_kernel(const uint32_t* const* __restrict__ cols,
+                                                                  uint32_t n_cols, uint32_t log_eval,
+                                                                  uint32_t log_n, const uint4* __restrict__ powers,
+                                                                  uint32_t den0, uint32_t den1,
+                                                                  uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
+                                                                  uint32_t* __restrict__ o2,
+                                                                  uint32_t* __restrict__ o3) {
+    uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= (1u << log_eval)) return;
+    uint32_t a = cols[0][row], b = cols[1][row];
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    const uint32_t nc = n_cols - 2;
+    for (uint32_t i = 0; i < nc; i++) {
+        uint32_t cv = cols[i + 2][row];
+        uint32_t cons = msub(cv, madd(msqr(a), msqr(b)));
+        uint4 pw = powers[nc - 1 - i];
+        s0 += mfold((uint64_t)pw.x * cons);
+        s1 += mfold((uint64_t)pw.y * cons);
+        s2 += mfold((uint64_t)pw.z * cons);
+        s3 += mfold((uint64_t)pw.w * cons);
+        a = b;
+        b = cv;
+    }
+    uint32_t d = (row >> log_n) ? den1 : den0;
+    o0[row] = mmul(mred64(s0), d);
+    o1[row] = mmul(mred64(s1), d);
+    o2[row] = mmul(mred64(s2), d);
+    o3[row] = mmul(mred64(s3), d);
+}
+
+void launch_widefib_composition(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_eval,
+                                uint32_t log_n, const uint32_t* d_powers, uint32_t den0, uint32_t den1,
+                                uint32_t* const* out) {
+    SR_CHECK(n_cols >= 3, "composition needs at least 3 columns");
+    uint32_t n = 1u << log_eval;
+    hipLaunchKernelGGL(widefib_composition_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_cols, n_cols,
+                       log_eval, log_n, (const uint4*)d_powers, den0, den1, out[0], out[1], out[2], out[3]);
+    SR_HIP(hipGetLastError());
+}
+
+// Wide-Fibonacci trace in bit-reversed circle-domain order
+// (examples/wide_fibonacci/trace.zig:40-162).  mode 0: (1, logicalRow(s));
+// mode 1: seeded uniform M31 pair per storage index.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+__global__ __launch_bounds__(256) void widefib_trace_kernel(uint32_t log_n, uint32_t seq_len, uint32_t mode,
+                                                            uint64_t seed, uint32_t* __restrict__ out,
+                                                            uint64_t stride) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = 1u << log_n;
+    if (s >= n) return;
+    uint32_t a, b;
+    if (mode == 0) {
+        uint32_t ci = bitrev32(s, log_n);
+        uint32_t row = ci < n / 2 ? ci * 2 : (n - 1 - ci) * 2 + 1;
+        a = 1;
+        b = row;
+    } else {
+        a = (uint32_t)(splitmix64(seed ^ ((2ull * s) * 0xD1B54A32D192ED03ull)) % P);
+        b = (uint32_t)(splitmix64(seed ^ ((2ull * s + 1) * 0xD1B54A32D192ED03ull)) % P);
+    }
+    out[s] = a;
+    out[stride + s] = b;
+    for (uint32_t col = 2; col < seq_len; col++) {
+        uint32_t nx = madd(msqr(a), msqr(b));
+        out[(uint64_t)col * stride + s] = nx;
+        a = b;
+        b = nx;
+    }
+}
+void launch_widefib_trace(Ctx& c, uint32_t log_n, uint32_t seq_len, uint32_t mode, uint64_t seed, uint32_t* d_out,
+                          uint64_t col_stride) {
+    uint32_t n = 1u << log_n;
+    hipLaunchKernelGGL(widefib_trace_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, log_n, seq_len, mode,
+                       seed, d_out, col_stride);
+    SR_HIP(hipGetLastError());
+}
+
+// Point evaluation (prover/poly/circle/point_evaluation.zig:14-38,469-488):
+// p(P) = sum_i c_i * w_i, w_i = prod_{bit j of i} f_j, f = [P.y, P.x, 2x^2-1, ...].
+struct Factors {
+    qm31 f[30];
+};
+__global__ __launch_bounds__(256) void eval_weights_kernel(Factors fs, uint32_t log_size, uint4* __restrict__ w) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (1u << log_size)) return;
+    qm31 acc = qone();
+    for (uint32_t j = 0; j < log_size; j++)
+        if ((i >> j) & 1) acc = qmul(acc, fs.f[j]);
+    w[i] = make_uint4(acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b);
+}
+void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w) {
+    SR_CHECK(log_size <= 30, "eval weights: log size too large");
+    Factors fs;
+    for (uint32_t j = 0; j < log_size; j++) fs.f[j] = factors[j];
+    uint32_t n = 1u << log_size;
+    hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, fs, log_size, (uint4*)d_w);
+    SR_HIP(hipGetLastError());
+}
+
+// Grid (chunks, columns): each block sums 256 * EV_PER_THREAD products of one
+// column into four 64-bit partial sums, then one 64-bit atomic add per
+// coordinate (integer sums are exact and order independent).
+constexpr uint32_t EV_PER_THREAD = 16;
+__global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const* __restrict__ cols,
+                                                           uint32_t log_size, const uint4* __restrict__ w,
+                                                           unsigned long long* __restrict__ acc) {
+    __shared__ unsigned long long red[4][256];
+    const uint32_t col = blockIdx.y;
+    const uint32_t* __restrict__ cv = cols[col];
+    const uint32_t n = 1u << log_size;
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    uint32_t base = blockIdx.x * (256 * EV_PER_THREAD);
+#pragma unroll 4
+    for (uint32_t k = 0; k < EV_PER_THREAD; k++) {
+        uint32_t i = base + k * 256 + threadIdx.x;
+        if (i < n) {
+            uint32_t v = cv[i];
+            uint4 ww = w[i];
+            s0 += mfold((uint64_t)v * ww.x);
+            s1 += mfold((uint64_t)v * ww.y);
+            s2 += mfold((uint64_t)v * ww.z);
+            s3 += mfold((uint64_t)v * ww.w);
+        }
+    }
+    red[0][threadIdx.x] = s0;
+    red[1][threadIdx.x] = s1;
+    red[2][threadIdx.x] = s2;
+    red[3][threadIdx.x] = s3;
+    __syncthreads();
+    for (uint32_t st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) atomicAdd(&acc[col * 4 + threadIdx.x], red[threadIdx.x][0]);
+}
+__global__ void zero_u64_kernel(unsigned long long* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+__global__ void reduce_u64_kernel(const unsigned long long* __restrict__ acc, uint32_t n, uint32_t* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = mred64(acc[i]);
+}
+void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
+                         const uint32_t* d_w, uint32_t* d_out) {
+    if (n_cols == 0) return;
+    DBuf acc(&c, (size_t)n_cols * 4 * 8);
+    uint32_t nw = n_cols * 4;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((nw + 255) / 256), dim3(256), 0, c.stream,
+                       acc.as<unsigned long long>(), nw);
+    uint32_t n = 1u << log_size;
+    uint32_t chunks = (n + 256 * EV_PER_THREAD - 1) / (256 * EV_PER_THREAD);
+    hipLaunchKernelGGL(eval_columns_kernel, dim3(chunks, n_cols), dim3(256), 0, c.stream, d_cols, log_size,
+                       (const uint4*)d_w, acc.as<unsigned long long>());
+    hipLaunchKernelGGL(reduce_u64_kernel, dim3((nw + 255) / 256), dim3(256), 0, c.stream,
+                       acc.as<unsigned long long>(), nw, d_out);
+    SR_HIP(hipGetLastError());
+}
+
+}  // namespace sr

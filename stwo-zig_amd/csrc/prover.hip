@@ -1,0 +1,750 @@
+// Wide-Fibonacci proof on one MI355X: host mirror of the reference's generic
+// prover (transcript order, FRI orchestration, decommitment and proof wire)
+// driving the rocm kernels.  Every heavy stage runs on the device: trace
+// generation, interpolation, LDE, Merkle, composition, sampled values,
+// quotients, FRI folds and PoW.  The host only keeps the Fiat-Shamir channel
+// (a few hundred bytes per draw) and assembles the proof from one gather.
+//
+// Order of operations (src/examples/common/prover_transaction.zig:222-339,
+// src/prover/prove.zig:209-389, src/prover/pcs/scheme.zig:571-800,
+// src/prover/fri.zig:163-621).
+#include <algorithm>
+#include <chrono>
+#include <functional>
+#include <memory>
+#include <string>
+
+#include "prover.hpp"
+
+namespace sr {
+
+// ---------------------------------------------------------------- channel
+// Blake2sChannel (src/core/channel/blake2s.zig:32-287).
+void Channel::mix_bytes(const void* data, size_t n) {
+    B2Stream s;
+    s.update(digest, 32);
+    s.update(data, n);
+    s.finalize(digest);
+    n_draws = 0;
+}
+void Channel::mix_felts(const qm31* f, size_t n) {
+    std::vector<uint32_t> w(4 * n);
+    for (size_t i = 0; i < n; i++) qstore(f[i], &w[4 * i]);
+    mix_bytes(w.data(), 16 * n);
+}
+void Channel::mix_u32s(const uint32_t* d, size_t n) { mix_bytes(d, 4 * n); }
+void Channel::mix_u64(uint64_t v) {
+    uint32_t w[2] = {(uint32_t)v, (uint32_t)(v >> 32)};
+    mix_u32s(w, 2);
+}
+// MerkleChannel.mixRoot (src/core/vcs_lifted/blake2_merkle.zig:237-249).
+void Channel::mix_root(const uint8_t root[32]) { mix_bytes(root, 32); }
+void Channel::draw_u32s(uint32_t out[8]) {
+    uint8_t in[37];
+    memcpy(in, digest, 32);
+    memcpy(in + 32, &n_draws, 4);
+    in[36] = 0;
+    n_draws++;
+    uint8_t h[32];
+    b2_hash(in, 37, h);
+    memcpy(out, h, 32);
+}
+void Channel::draw_base_felts(uint32_t out[8]) {
+    for (;;) {
+        uint32_t w[8];
+        draw_u32s(w);
+        bool ok = true;
+        for (int i = 0; i < 8; i++)
+            if (w[i] >= 2 * P) ok = false;
+        if (!ok) continue;
+        for (int i = 0; i < 8; i++) out[i] = mred64(w[i]);
+        return;
+    }
+}
+qm31 Channel::draw_secure_felt() {
+    uint32_t f[8];
+    draw_base_felts(f);
+    return qload(f);
+}
+// Proof-of-work prefix H(LE32(0x12345678) || 0^12 || digest || LE32(bits)) (blake2s.zig:142-154).
+void Channel::pow_prefix(uint32_t bits, uint8_t out[32]) const {
+    uint8_t in[52];
+    uint32_t pp = 0x12345678u;
+    memcpy(in, &pp, 4);
+    memset(in + 4, 0, 12);
+    memcpy(in + 16, digest, 32);
+    memcpy(in + 48, &bits, 4);
+    b2_hash(in, 52, out);
+}
+
+// ---------------------------------------------------------------- helpers
+static double host_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct QPoint {
+    qm31 x, y;
+};
+static QPoint qp_double(QPoint p) {
+    qm31 xx = qsqr(p.x), xy = qmul(p.x, p.y);
+    return {qsub(qadd(xx, xx), qone()), qadd(xy, xy)};
+}
+static QPoint qp_add_m31(QPoint a, pt b) {
+    qm31 bx = qfromm(b.x), by = qfromm(b.y);
+    return {qsub(qmul(a.x, bx), qmul(a.y, by)), qadd(qmul(a.x, by), qmul(a.y, bx))};
+}
+
+// Witness positions of a lifted Merkle decommitment (prover/vcs_lifted/decommit.zig:171-236):
+// (layer, index) of each sibling hash not derivable from the queried ones.
+static void merkle_witness_positions(uint32_t max_log, const std::vector<uint64_t>& positions,
+                                     std::vector<std::pair<uint32_t, uint64_t>>& out) {
+    std::vector<uint64_t> prev;
+    for (size_t i = 0; i < positions.size(); i++)
+        if (i == 0 || positions[i - 1] != positions[i]) prev.push_back(positions[i]);
+    for (int32_t layer = (int32_t)max_log - 1; layer >= 0; layer--) {
+        std::vector<uint64_t> cur;
+        size_t q = 0;
+        while (q < prev.size()) {
+            uint64_t first = prev[q];
+            bool sib = q + 1 < prev.size() && (first ^ 1) == prev[q + 1];
+            if (!sib) out.push_back({(uint32_t)layer + 1, first ^ 1});
+            cur.push_back(first >> 1);
+            q += sib ? 2 : 1;
+        }
+        prev.swap(cur);
+    }
+}
+
+// Gather plan: absolute device word addresses, downloaded with one kernel.
+struct Gather {
+    std::vector<uint64_t> addr;
+    size_t add_words(const uint32_t* p, uint64_t n) {
+        size_t at = addr.size();
+        for (uint64_t i = 0; i < n; i++) addr.push_back((uint64_t)(uintptr_t)(p + i));
+        return at;
+    }
+    size_t add_hash(const DeviceTree& t, uint32_t layer, uint64_t idx) {
+        return add_words((const uint32_t*)(t.layer(layer) + idx * 32), 8);
+    }
+};
+
+struct JsonOut {
+    std::string s;
+    void raw(const char* t) { s += t; }
+    void u64(uint64_t v) { s += std::to_string(v); }
+    void hash(const uint8_t* h) {
+        s += '[';
+        for (int i = 0; i < 32; i++) {
+            if (i) s += ',';
+            s += std::to_string(h[i]);
+        }
+        s += ']';
+    }
+    void qm(const uint32_t* w) {
+        s += '[';
+        for (int i = 0; i < 4; i++) {
+            if (i) s += ',';
+            s += std::to_string(w[i]);
+        }
+        s += ']';
+    }
+};
+
+// ---------------------------------------------------------------- prove
+// Owns resident trees for the duration of one proof (freed on every exit path).
+struct TreeOwner {
+    Ctx& c;
+    std::vector<DeviceTree*> trees;
+    explicit TreeOwner(Ctx& cc) : c(cc) {}
+    ~TreeOwner() {
+        for (auto t : trees) tree_free(c, t);
+    }
+    DeviceTree* add(DeviceTree* t) {
+        trees.push_back(t);
+        return t;
+    }
+    DeviceTree* make(uint32_t max_log) {
+        DeviceTree* t = new DeviceTree();
+        t->max_log = max_log;
+        trees.push_back(t);
+        t->layers = (uint8_t*)c.alloc(((2ull << max_log) - 1) * 32);
+        return t;
+    }
+};
+// Committed tree: LDE columns, retained coefficients and the resident Merkle tree.
+struct Committed {
+    std::vector<uint32_t*> lde;
+    std::vector<uint32_t> lde_logs;
+    std::vector<const uint32_t*> coeffs;
+    std::vector<uint32_t> coeff_logs;
+    DBuf lde_buf;
+    DeviceTree* tree = nullptr;
+};
+
+// LDE of coefficient columns (pointer list) by `blowup` and Merkle commit
+// (commit_polys.zig:10-67; preparation.zig:113-210).
+static void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out,
+                                const std::vector<const uint32_t*>& coeffs, uint32_t coeff_log, uint32_t blowup) {
+    uint32_t n = (uint32_t)coeffs.size();
+    uint32_t m = coeff_log + blowup;
+    uint64_t N = 1ull << m;
+    out.coeffs = coeffs;
+    out.coeff_logs.assign(n, coeff_log);
+    out.lde_logs.assign(n, m);
+    out.lde.resize(n);
+    if (n) {
+        out.lde_buf = DBuf(&c, (size_t)n * N * 4);
+        for (uint32_t i = 0; i < n; i++) out.lde[i] = out.lde_buf.as<uint32_t>() + i * N;
+        DBuf d_src = upload(c, coeffs.data(), n);
+        DBuf d_dst = upload(c, out.lde.data(), n);
+        launch_circle_fft(c, false, d_src.as<const uint32_t* const>(), d_dst.as<uint32_t* const>(), n, m, coeff_log);
+    }
+    std::vector<const uint32_t*> cols(out.lde.begin(), out.lde.end());
+    out.tree = owner.add(merkle_commit(c, cols, out.lde_logs));
+}
+
+void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trace_in, std::string& proof,
+                          ProveStats* stats) {
+    const uint32_t n = prm.log_n_rows, W = prm.sequence_len, B = prm.log_blowup_factor;
+    SR_CHECK(n >= 1 && n <= 26, "log_n_rows out of range");
+    SR_CHECK(W >= 3, "sequence_len must be >= 3");
+    SR_CHECK(B >= 1 && B <= 4, "log_blowup_factor out of range");
+    SR_CHECK(prm.n_queries >= 1 && prm.n_queries <= 1024, "n_queries out of range");
+    SR_CHECK(prm.log_last_layer_degree_bound <= 10, "log_last_layer_degree_bound out of range");
+    SR_CHECK(prm.pow_bits <= 40, "pow_bits out of range");
+    const uint32_t L = n + B;  // lifting log size: every committed column is n + B
+    SR_CHECK(L > prm.log_last_layer_degree_bound + B, "trace too small for the FRI configuration");
+    const uint64_t N = 1ull << n;
+    c.ensure_tower(L > n + 1 ? L : n + 1);
+
+    std::vector<hipEvent_t> evs;
+    std::vector<double> hms;
+    auto mark = [&]() {
+        hipEvent_t e;
+        SR_HIP(hipEventCreate(&e));
+        SR_HIP(hipEventRecord(e, c.stream));
+        evs.push_back(e);
+        hms.push_back(host_ms());
+    };
+    mark();
+
+    TreeOwner owner(c);
+    Channel ch;
+    // PcsConfig.mixInto (core/pcs/mod.zig:32-52): default fold_step, no lifting override.
+    qm31 cfg = {{prm.pow_bits, B}, {prm.n_queries, prm.log_last_layer_degree_bound}};
+    ch.mix_felts(&cfg, 1);
+
+    // Tree 0: preprocessed, empty (examples/wide_fibonacci/trace.zig:184-186).
+    Committed t0;
+    commit_coefficients(c, owner, t0, {}, 1, B);
+    {
+        uint8_t prefix[64] = {'l', 'e', 'a', 'f'};
+        b2_hash(prefix, 64, t0.tree->root);
+    }
+    ch.mix_root(t0.tree->root);
+
+    // Tree 1: main trace -> coefficients -> LDE -> Merkle.
+    DBuf trace_own;
+    const uint32_t* d_trace = d_trace_in;
+    if (!d_trace) {
+        trace_own = DBuf(&c, (size_t)W * N * 4);
+        launch_widefib_trace(c, n, W, prm.trace_mode, prm.trace_seed, trace_own.as<uint32_t>(), N);
+        d_trace = trace_own.as<uint32_t>();
+    }
+    DBuf coeff1(&c, (size_t)W * N * 4);
+    {
+        std::vector<const uint32_t*> src(W);
+        std::vector<uint32_t*> dst(W);
+        for (uint32_t i = 0; i < W; i++) {
+            src[i] = d_trace + i * N;
+            dst[i] = coeff1.as<uint32_t>() + i * N;
+        }
+        DBuf d_src = upload(c, src.data(), W), d_dst = upload(c, dst.data(), W);
+        launch_circle_fft(c, true, d_src.as<const uint32_t* const>(), d_dst.as<uint32_t* const>(), W, n, n);
+    }
+    Committed t1;
+    {
+        std::vector<const uint32_t*> cf(W);
+        for (uint32_t i = 0; i < W; i++) cf[i] = coeff1.as<uint32_t>() + i * N;
+        commit_coefficients(c, owner, t1, cf, n, B);
+    }
+    trace_own.reset();
+    tree_read_root(c, *t1.tree);
+    ch.mix_root(t1.tree->root);
+    mark();  // main commit
+
+    // mixStatement (wide_fibonacci.zig:496-498).
+    uint32_t st[2] = {n, W};
+    ch.mix_u32s(st, 2);
+
+    // Composition (prove.zig:237-318; wide_fibonacci/component.zig:146-228).
+    qm31 alpha = ch.draw_secure_felt();
+    const uint32_t NC = W - 2, CL = n + 1;
+    const uint64_t CN = 1ull << CL;
+    std::vector<uint32_t> pw(4 * NC);
+    {
+        qm31 cur = qone();
+        for (uint32_t k = 0; k < NC; k++) {
+            qstore(cur, &pw[4 * k]);
+            cur = qmul(cur, alpha);
+        }
+    }
+    DBuf d_pw = upload(c, pw.data(), pw.size());
+    // cosetVanishing(CanonicCoset(n).coset(), domain.at(0/1)) (constraints.zig:16-27).
+    uint32_t den_inv[2];
+    for (int r = 0; r < 2; r++) {
+        pt dp = idx_to_point(c.gen, domain_index_at(CL, (uint64_t)r));
+        pt init = idx_to_point(c.gen, subgroup_gen(n + 1));     // odds(n).initial
+        pt half_step = idx_to_point(c.gen, subgroup_gen(n + 1));  // step_size(n) / 2
+        pt s = padd(padd(dp, pconj(init)), half_step);
+        uint32_t x = s.x;
+        for (uint32_t i = 1; i < n; i++) x = mdouble_x(x);
+        den_inv[r] = minv(x);
+    }
+    DBuf ev_own;
+    std::vector<const uint32_t*> ev(W);
+    if (B == 1) {
+        for (uint32_t i = 0; i < W; i++) ev[i] = t1.lde[i];
+    } else {
+        // Trace polynomials evaluated on CanonicCoset(n + 1) from their coefficients.
+        ev_own = DBuf(&c, (size_t)W * CN * 4);
+        std::vector<uint32_t*> dst(W);
+        for (uint32_t i = 0; i < W; i++) {
+            dst[i] = ev_own.as<uint32_t>() + i * CN;
+            ev[i] = dst[i];
+        }
+        DBuf d_src = upload(c, t1.coeffs.data(), W), d_dst = upload(c, dst.data(), W);
+        launch_circle_fft(c, false, d_src.as<const uint32_t* const>(), d_dst.as<uint32_t* const>(), W, CL, n);
+    }
+    DBuf comp(&c, 4 * CN * 4);
+    uint32_t* comp_c[4];
+    for (int k = 0; k < 4; k++) comp_c[k] = comp.as<uint32_t>() + k * CN;
+    {
+        DBuf d_ev = upload(c, ev.data(), W);
+        launch_widefib_composition(c, d_ev.as<const uint32_t* const>(), W, CL, n, d_pw.as<uint32_t>(), den_inv[0],
+                                   den_inv[1], comp_c);
+        // interpolate (secure_poly.zig:164-215) in place
+        DBuf d_cc = upload(c, comp_c, 4);
+        launch_circle_fft(c, true, d_cc.as<const uint32_t* const>(), d_cc.as<uint32_t* const>(), 4, CL, CL);
+    }
+    ev_own.reset();
+    // splitAtMid -> [left0..3, right0..3] (poly.zig:302-318; prove.zig:423-445)
+    Committed t2;
+    {
+        std::vector<const uint32_t*> cp(8);
+        for (int k = 0; k < 4; k++) {
+            cp[k] = comp_c[k];
+            cp[4 + k] = comp_c[k] + N;
+        }
+        commit_coefficients(c, owner, t2, cp, n, B);
+    }
+    tree_read_root(c, *t2.tree);
+    ch.mix_root(t2.tree->root);
+    mark();  // composition
+
+    // OODS point (circle.zig:159-167).
+    qm31 tt = ch.draw_secure_felt();
+    qm31 t2q = qsqr(tt);
+    qm31 inv1 = qinv(qadd(t2q, qone()));
+    QPoint oods = {qmul(qsub(qone(), t2q), inv1), qmul(qadd(tt, tt), inv1)};
+
+    // Sampled values (sampled_values.zig): all columns share log n coefficients and
+    // the lifting log, so one weight vector serves every column.
+    const Committed* trees[3] = {&t0, &t1, &t2};
+    std::vector<const uint32_t*> all_coeffs;
+    std::vector<uint32_t*> all_lde;
+    std::vector<uint32_t> all_lde_logs;
+    for (int tr = 1; tr < 3; tr++)
+        for (size_t i = 0; i < trees[tr]->coeffs.size(); i++) {
+            all_coeffs.push_back(trees[tr]->coeffs[i]);
+            all_lde.push_back(trees[tr]->lde[i]);
+            all_lde_logs.push_back(trees[tr]->lde_logs[i]);
+        }
+    const uint32_t NS = (uint32_t)all_coeffs.size();
+    std::vector<QPoint> spt(NS, oods);  // fold count L - lde_log = 0 for every column here
+    std::vector<uint32_t> sv(4 * NS);
+    {
+        qm31 f[32];
+        f[0] = spt[0].y;
+        f[1] = spt[0].x;
+        for (uint32_t k = 2; k < n; k++) {
+            qm31 s2 = qsqr(f[k - 1]);
+            f[k] = qsub(qadd(s2, s2), qone());
+        }
+        DBuf w(&c, N * 16);
+        launch_eval_weights(c, f, n, w.as<uint32_t>());
+        DBuf d_cols = upload(c, all_coeffs.data(), NS);
+        DBuf d_out(&c, (size_t)NS * 16);
+        launch_eval_columns(c, d_cols.as<const uint32_t* const>(), NS, n, w.as<uint32_t>(), d_out.as<uint32_t>());
+        uint32_t* h = (uint32_t*)c.pinned(NS * 16);
+        SR_HIP(hipMemcpyAsync(h, d_out.p, NS * 16, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        memcpy(sv.data(), h, NS * 16);
+    }
+    // sampled_value_transcript.mixIntoChannel: tree, column, sample order.
+    ch.mix_bytes(sv.data(), sv.size() * 4);
+    mark();  // sampled values
+
+    // Quotients (samples.zig:245-312; row_evaluation.zig:73-106,402-521; constraints.zig:87-97).
+    qm31 qalpha = ch.draw_secure_felt();
+    std::vector<QuotientBatchDev> batches;
+    std::vector<uint32_t> entry_cols, entry_coeffs;
+    {
+        // One sample per column at the (unfolded) OODS point: one batch, powers in column order.
+        std::vector<std::vector<uint32_t>> per_batch_cols;
+        std::vector<std::vector<qm31>> per_batch_c;
+        std::vector<QPoint> bpts;
+        std::vector<qm31> sa, sb;
+        qm31 rp = qone();
+        for (uint32_t i = 0; i < NS; i++) {
+            QPoint p = spt[i];
+            size_t bi = 0;
+            for (; bi < bpts.size(); bi++)
+                if (qeq(bpts[bi].x, p.x) && qeq(bpts[bi].y, p.y)) break;
+            if (bi == bpts.size()) {
+                bpts.push_back(p);
+                per_batch_cols.emplace_back();
+                per_batch_c.emplace_back();
+                sa.push_back(qzero());
+                sb.push_back(qzero());
+            }
+            qm31 v = qload(&sv[4 * i]);
+            qm31 a = qsub(qconj(v), v);
+            qm31 cc = qsub(qconj(p.y), p.y);
+            qm31 b = qsub(qmul(v, cc), qmul(a, p.y));
+            per_batch_cols[bi].push_back(i);
+            per_batch_c[bi].push_back(qmul(rp, cc));
+            sa[bi] = qadd(sa[bi], qmul(rp, a));
+            sb[bi] = qadd(sb[bi], qmul(rp, b));
+            rp = qmul(rp, qalpha);
+        }
+        for (size_t bi = 0; bi < bpts.size(); bi++) {
+            QuotientBatchDev bd;
+            cm31 prx = bpts[bi].x.c0, pry = bpts[bi].y.c0, pix = bpts[bi].x.c1, piy = bpts[bi].y.c1;
+            bd.det = csub(cmul(prx, piy), cmul(pry, pix));
+            bd.pix = pix;
+            bd.piy = piy;
+            bd.sum_a = sa[bi];
+            bd.sum_b = sb[bi];
+            bd.first = (uint32_t)entry_cols.size();
+            bd.count = (uint32_t)per_batch_cols[bi].size();
+            for (size_t k = 0; k < per_batch_cols[bi].size(); k++) {
+                entry_cols.push_back(per_batch_cols[bi][k]);
+                uint32_t w4[4];
+                qstore(per_batch_c[bi][k], w4);
+                entry_coeffs.insert(entry_coeffs.end(), w4, w4 + 4);
+            }
+            batches.push_back(bd);
+        }
+    }
+    const uint64_t LN = 1ull << L;
+    DBuf quot(&c, 4 * LN * 4);
+    uint32_t* qc[4];
+    for (int k = 0; k < 4; k++) qc[k] = quot.as<uint32_t>() + k * LN;
+    DeviceTree* ftree0 = owner.make(L);
+    {
+        DBuf d_cols = upload(c, all_lde.data(), NS);
+        DBuf d_logs = upload(c, all_lde_logs.data(), NS);
+        DBuf d_ec = upload(c, entry_cols.data(), entry_cols.size());
+        DBuf d_ecf = upload(c, entry_coeffs.data(), entry_coeffs.size());
+        DBuf d_b = upload(c, batches.data(), batches.size());
+        launch_quotients(c, d_cols.as<const uint32_t* const>(), d_logs.as<uint32_t>(), d_ec.as<uint32_t>(),
+                         d_ecf.as<uint32_t>(), d_b.as<QuotientBatchDev>(), (uint32_t)batches.size(), L, qc,
+                         ftree0->layer(L));
+        launch_merkle_parents(c, *ftree0);
+    }
+    tree_read_root(c, *ftree0);
+    ch.mix_root(ftree0->root);
+    mark();  // quotients + first layer commit
+
+    // FRI (prover/fri.zig:377-621).
+    const uint32_t last_log = prm.log_last_layer_degree_bound + B;
+    std::vector<DBuf> lines;        // inner layer evaluations (QM31 AoS)
+    std::vector<DeviceTree*> ftrees;  // inner layer trees
+    qm31 fa = ch.draw_secure_felt();
+    uint32_t llog = L - 1;
+    lines.emplace_back(&c, (1ull << llog) * 16);
+    DeviceTree* next_tree = llog > last_log ? owner.make(llog) : nullptr;
+    {
+        const uint32_t* cc[4] = {qc[0], qc[1], qc[2], qc[3]};
+        launch_fold_circle_into_line(c, cc, L, fa, lines.back().as<uint32_t>(), false,
+                                     next_tree ? next_tree->layer(llog) : nullptr);
+    }
+    while (llog > last_log) {
+        DeviceTree* t = next_tree;
+        launch_merkle_parents(c, *t);
+        tree_read_root(c, *t);
+        ftrees.push_back(t);
+        ch.mix_root(t->root);
+        qm31 la = ch.draw_secure_felt();
+        uint32_t nlog = llog - 1;
+        next_tree = nlog > last_log ? owner.make(nlog) : nullptr;
+        DBuf nxt(&c, (1ull << nlog) * 16);
+        launch_fold_line(c, lines.back().as<uint32_t>(), llog, la, nxt.as<uint32_t>(),
+                         next_tree ? next_tree->layer(nlog) : nullptr);
+        lines.push_back(std::move(nxt));
+        llog = nlog;
+    }
+    // Last layer (fri.zig:594-621; line_evaluation.zig:97-151; line.zig:134-142).
+    const uint64_t lsz = 1ull << llog;
+    std::vector<qm31> lv(lsz);
+    {
+        uint32_t* h = (uint32_t*)c.pinned(lsz * 16);
+        SR_HIP(hipMemcpyAsync(h, lines.back().p, lsz * 16, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        for (uint64_t i = 0; i < lsz; i++) lv[i] = qload(h + 4 * i);
+    }
+    auto bitrev_vec = [](std::vector<qm31>& v) {
+        uint32_t lg = 0;
+        while ((1ull << lg) < v.size()) lg++;
+        for (uint64_t i = 0; i < v.size(); i++) {
+            uint64_t j = bitrev32((uint32_t)i, lg);
+            if (j > i) std::swap(v[i], v[j]);
+        }
+    };
+    bitrev_vec(lv);
+    {
+        // line IFFT over LineDomain(halfOdds(llog)) and its doublings
+        uint32_t init = half_odds_initial(llog), step = subgroup_gen(llog), clog = llog;
+        while (clog > 0) {
+            uint64_t csz = 1ull << clog, half = csz / 2;
+            for (uint64_t s0 = 0; s0 < lsz; s0 += csz) {
+                pt cur = idx_to_point(c.gen, init), stp = idx_to_point(c.gen, step);
+                for (uint64_t i = 0; i < half; i++) {
+                    uint32_t ix = minv(cur.x);
+                    qm31 v0 = lv[s0 + i], v1 = lv[s0 + half + i];
+                    lv[s0 + i] = qadd(v0, v1);
+                    lv[s0 + half + i] = qmulm(qsub(v0, v1), ix);
+                    cur = padd(cur, stp);
+                }
+            }
+            init = (init * 2) & ORDER_MASK;
+            step = (step * 2) & ORDER_MASK;
+            clog--;
+        }
+        uint32_t li = minv((uint32_t)(lsz % P));
+        for (auto& v : lv) v = qmulm(v, li);
+    }
+    bitrev_vec(lv);  // ordered coefficients
+    const uint64_t bound = 1ull << prm.log_last_layer_degree_bound;
+    for (uint64_t i = bound; i < lsz; i++)
+        SR_CHECK(qiszero(lv[i]), "FRI last layer exceeds the degree bound (constraints not satisfied)");
+    std::vector<qm31> lastp(lv.begin(), lv.begin() + bound);
+    bitrev_vec(lastp);  // LinePoly.fromOrderedCoefficients
+    ch.mix_felts(lastp.data(), lastp.size());
+    mark();  // FRI commit
+
+    // Proof of work (scheme.zig:733-735; lowest nonce, channel/blake2s.zig:163-167).
+    uint64_t nonce = 0;
+    if (prm.pow_bits > 0) {
+        uint8_t prefix[32];
+        ch.pow_prefix(prm.pow_bits, prefix);
+        uint64_t chunk = prm.pow_bits <= 16 ? (1ull << 20) : (1ull << 24);
+        for (uint64_t base = 0;; base += chunk) {
+            uint64_t r = launch_pow_grind(c, prefix, prm.pow_bits, base, chunk);
+            if (r != UINT64_MAX) {
+                nonce = r;
+                break;
+            }
+            SR_CHECK(base < (1ull << 50), "pow: no nonce found");
+        }
+    }
+    ch.mix_u64(nonce);
+    mark();  // pow
+
+    // Queries (core/queries.zig:8-60).
+    std::vector<uint64_t> qs;
+    {
+        while (qs.size() < prm.n_queries) {
+            uint32_t w[8];
+            ch.draw_u32s(w);
+            for (int i = 0; i < 8 && qs.size() < prm.n_queries; i++) qs.push_back(w[i] & ((1u << L) - 1));
+        }
+        std::sort(qs.begin(), qs.end());
+        qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
+    }
+    const size_t nq = qs.size();
+
+    // Decommitment plan: everything read from HBM in one gather.
+    Gather g;
+    // trees 1, 2: queried values (scheme_decommit.zig; decommit.zig:77-89) and witnesses.
+    size_t qv_at[3] = {0, 0, 0};
+    std::vector<std::pair<uint32_t, uint64_t>> tw_pos[3];
+    std::vector<size_t> tw_at[3];
+    for (int tr = 1; tr < 3; tr++) {
+        const Committed& T = *trees[tr];
+        qv_at[tr] = g.addr.size();
+        for (size_t col = 0; col < T.lde.size(); col++)
+            for (size_t q = 0; q < nq; q++) {
+                uint32_t sh = T.tree->max_log - T.lde_logs[col] + 1;
+                uint64_t idx = ((qs[q] >> sh) << 1) + (qs[q] & 1);
+                g.add_words(T.lde[col] + idx, 1);
+            }
+        merkle_witness_positions(T.tree->max_log, qs, tw_pos[tr]);
+        for (auto& lp : tw_pos[tr]) tw_at[tr].push_back(g.add_hash(*T.tree, lp.first, lp.second));
+    }
+    // FRI layers (fri.zig:626-740): subsets {2k, 2k+1}, witness = non-queried partner.
+    struct LayerPlan {
+        std::vector<size_t> wit_at;
+        std::vector<size_t> hw_at;
+    };
+    std::vector<LayerPlan> lplans;
+    auto plan_layer = [&](const DeviceTree& t, const std::vector<uint64_t>& q,
+                          std::function<size_t(uint64_t)> add_val) {
+        LayerPlan lp;
+        std::vector<uint64_t> dpos;
+        size_t i = 0;
+        while (i < q.size()) {
+            uint64_t key = q[i] >> 1;
+            size_t j = i + 1;
+            while (j < q.size() && (q[j] >> 1) == key) j++;
+            size_t at = i;
+            for (uint64_t p = key << 1; p < (key << 1) + 2; p++) {
+                dpos.push_back(p);
+                if (at < j && q[at] == p) at++;
+                else lp.wit_at.push_back(add_val(p));
+            }
+            i = j;
+        }
+        std::vector<std::pair<uint32_t, uint64_t>> hw;
+        merkle_witness_positions(t.max_log, dpos, hw);
+        for (auto& h : hw) lp.hw_at.push_back(g.add_hash(t, h.first, h.second));
+        lplans.push_back(lp);
+    };
+    plan_layer(*ftree0, qs, [&](uint64_t p) {
+        size_t at = g.addr.size();
+        for (int k = 0; k < 4; k++) g.add_words(qc[k] + p, 1);
+        return at;
+    });
+    std::vector<uint64_t> lq;
+    for (uint64_t q : qs)
+        if (lq.empty() || lq.back() != (q >> 1)) lq.push_back(q >> 1);
+    for (size_t k = 0; k < ftrees.size(); k++) {
+        const uint32_t* base = lines[k].as<uint32_t>();
+        plan_layer(*ftrees[k], lq, [&](uint64_t p) { return g.add_words(base + 4 * p, 4); });
+        std::vector<uint64_t> nx;
+        for (uint64_t q : lq)
+            if (nx.empty() || nx.back() != (q >> 1)) nx.push_back(q >> 1);
+        lq.swap(nx);
+    }
+    std::vector<uint32_t> gw(g.addr.size());
+    if (!g.addr.empty()) {
+        DBuf d_addr = upload(c, g.addr.data(), g.addr.size());
+        DBuf d_out(&c, g.addr.size() * 4);
+        launch_gather(c, d_addr.as<uint64_t>(), g.addr.size(), d_out.as<uint32_t>());
+        uint32_t* h = (uint32_t*)c.pinned(g.addr.size() * 4);
+        SR_HIP(hipMemcpyAsync(h, d_out.p, g.addr.size() * 4, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        memcpy(gw.data(), h, g.addr.size() * 4);
+    }
+    mark();  // decommit gather
+
+    // Proof wire (interop/proof_wire.zig:19-76).
+    JsonOut j;
+    j.s.reserve(1 << 17);
+    j.raw("{\"config\":{\"pow_bits\":");
+    j.u64(prm.pow_bits);
+    j.raw(",\"fri_config\":{\"log_blowup_factor\":");
+    j.u64(B);
+    j.raw(",\"log_last_layer_degree_bound\":");
+    j.u64(prm.log_last_layer_degree_bound);
+    j.raw(",\"n_queries\":");
+    j.u64(prm.n_queries);
+    j.raw(",\"fold_step\":1},\"lifting_log_size\":null},\"commitments\":[");
+    for (int tr = 0; tr < 3; tr++) {
+        if (tr) j.raw(",");
+        j.hash(trees[tr]->tree->root);
+    }
+    j.raw("],\"sampled_values\":[[]");
+    {
+        size_t at = 0;
+        for (int tr = 1; tr < 3; tr++) {
+            j.raw(",[");
+            for (size_t col = 0; col < trees[tr]->lde.size(); col++) {
+                if (col) j.raw(",");
+                j.raw("[");
+                j.qm(&sv[4 * at++]);
+                j.raw("]");
+            }
+            j.raw("]");
+        }
+    }
+    j.raw("],\"decommitments\":[{\"hash_witness\":[]}");
+    for (int tr = 1; tr < 3; tr++) {
+        j.raw(",{\"hash_witness\":[");
+        for (size_t k = 0; k < tw_at[tr].size(); k++) {
+            if (k) j.raw(",");
+            j.hash((const uint8_t*)&gw[tw_at[tr][k]]);
+        }
+        j.raw("]}");
+    }
+    j.raw("],\"queried_values\":[[]");
+    for (int tr = 1; tr < 3; tr++) {
+        j.raw(",[");
+        size_t at = qv_at[tr];
+        for (size_t col = 0; col < trees[tr]->lde.size(); col++) {
+            if (col) j.raw(",");
+            j.raw("[");
+            for (size_t q = 0; q < nq; q++) {
+                if (q) j.raw(",");
+                j.u64(gw[at++]);
+            }
+            j.raw("]");
+        }
+        j.raw("]");
+    }
+    j.raw("],\"proof_of_work\":");
+    j.u64(nonce);
+    j.raw(",\"fri_proof\":{\"first_layer\":");
+    auto emit_layer = [&](const LayerPlan& lp, const DeviceTree& t) {
+        j.raw("{\"fri_witness\":[");
+        for (size_t k = 0; k < lp.wit_at.size(); k++) {
+            if (k) j.raw(",");
+            j.qm(&gw[lp.wit_at[k]]);
+        }
+        j.raw("],\"decommitment\":{\"hash_witness\":[");
+        for (size_t k = 0; k < lp.hw_at.size(); k++) {
+            if (k) j.raw(",");
+            j.hash((const uint8_t*)&gw[lp.hw_at[k]]);
+        }
+        j.raw("]},\"commitment\":");
+        j.hash(t.root);
+        j.raw("}");
+    };
+    emit_layer(lplans[0], *ftree0);
+    j.raw(",\"inner_layers\":[");
+    for (size_t k = 0; k < ftrees.size(); k++) {
+        if (k) j.raw(",");
+        emit_layer(lplans[1 + k], *ftrees[k]);
+    }
+    j.raw("],\"last_layer_poly\":[");
+    for (size_t k = 0; k < lastp.size(); k++) {
+        if (k) j.raw(",");
+        uint32_t w4[4];
+        qstore(lastp[k], w4);
+        j.qm(w4);
+    }
+    j.raw("]}}");
+    proof.swap(j.s);
+    mark();  // encode
+
+    SR_HIP(hipEventSynchronize(evs.back()));
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        for (size_t i = 1; i < evs.size() && i < 1 + STAGES; i++) {
+            float ms = 0;
+            SR_HIP(hipEventElapsedTime(&ms, evs[i - 1], evs[i]));
+            stats->stage_gpu_ms[i - 1] = ms;
+            stats->stage_host_ms[i - 1] = hms[i] - hms[i - 1];
+        }
+        float tot = 0;
+        SR_HIP(hipEventElapsedTime(&tot, evs.front(), evs.back()));
+        stats->total_gpu_ms = tot;
+        stats->total_host_ms = hms.back() - hms.front();
+        stats->peak_hbm_bytes = c.peak_live;
+    }
+    for (auto e : evs) (void)hipEventDestroy(e);
+}
+
+}  // namespace sr

@@ -1,0 +1,182 @@
+// Internal runtime of the rocm backend: device context, caching device arena,
+// resident twiddle tower, resident Merkle trees and the launch wrappers of
+// every kernel.  Not part of the C-ABI (include/stwo_rocm.h is).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "blake2s.hpp"
+#include "fields.hpp"
+
+namespace sr {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SR_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t _e = (call);                                                                        \
+        if (_e != hipSuccess)                                                                          \
+            throw ::sr::Error(3, std::string(#call " failed: ") + hipGetErrorString(_e) + " at " +     \
+                                     __FILE__ + ":" + std::to_string(__LINE__));                       \
+    } while (0)
+
+#define SR_CHECK(cond, msg)                                \
+    do {                                                   \
+        if (!(cond)) throw ::sr::Error(1, std::string(msg)); \
+    } while (0)
+
+// Lifted Merkle tree resident in HBM: layer k (2^k hashes) at hash offset
+// 2^k - 1 (root first, as MerkleProverLifted.layers, prover.zig:23).
+struct DeviceTree {
+    uint32_t max_log = 0;
+    uint8_t* layers = nullptr;  // device, (2^(max_log+1)-1) * 32 bytes
+    uint8_t root[32];
+    uint8_t* layer(uint32_t k) const { return layers + (((uint64_t)1 << k) - 1) * 32; }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string last_error;
+    uint64_t hbm_budget = 0;  // 0 = unlimited
+    uint64_t bytes_live = 0, bytes_cached = 0, peak_live = 0;
+    std::multimap<size_t, void*> free_blocks;
+    std::unordered_map<void*, size_t> live_blocks;
+
+    // Twiddle tower of CanonicCoset(tower_log) (twiddle_tower.zig:29-115).
+    uint32_t tower_log = 0;
+    uint32_t* d_tw = nullptr;
+    uint32_t* d_itw = nullptr;
+    // Domain tables per circle log: bit-reversed domain points and inverse y of the half coset.
+    std::map<uint32_t, uint32_t*> domain_xy;     // 2 * 2^log words: x[] then y[]
+    std::map<uint32_t, uint32_t*> circle_inv_y;  // 2^(log-1) words
+    B2State leaf_seed, node_seed;
+    GenDoubles gen;
+
+    // pinned host staging: a readback buffer and a ring for small uploads.
+    uint8_t* h_pinned = nullptr;
+    size_t h_pinned_size = 0;
+    uint8_t* h_ring = nullptr;
+    size_t ring_cap = 0, ring_off = 0;
+
+    // events for GPU timing
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    void* alloc(size_t bytes);
+    void free(void* p);
+    void trim();
+    void sync();
+    uint8_t* pinned(size_t bytes);
+    // Copies `bytes` into the pinned upload ring and returns the pinned copy; the
+    // slot stays valid until the next sync (the ring syncs the stream when it wraps).
+    void* stage(const void* src, size_t bytes);
+    void ensure_tower(uint32_t log);
+    const uint32_t* twiddle_layer(bool inverse, uint32_t m, uint32_t s) const;
+    const uint32_t* domain_points(uint32_t log);
+    const uint32_t* inv_circle_y(uint32_t log);
+};
+
+// RAII device buffer from the context arena.
+struct DBuf {
+    Ctx* ctx = nullptr;
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(Ctx* c, size_t b) : ctx(c), p(b ? c->alloc(b) : nullptr), bytes(b) {}
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : ctx(o.ctx), p(o.p), bytes(o.bytes) { o.p = nullptr; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        reset();
+        ctx = o.ctx;
+        p = o.p;
+        bytes = o.bytes;
+        o.p = nullptr;
+        return *this;
+    }
+    ~DBuf() { reset(); }
+    void reset() {
+        if (p && ctx) ctx->free(p);
+        p = nullptr;
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+// ---- launch wrappers (kernels_*.hip) ----
+// Circle FFT over columns given by device pointer arrays.  Forward evaluates
+// coefficients (length 2^src_log, zero padded) on CanonicCoset(m); inverse
+// interpolates evaluations of CanonicCoset(m) (src_log == m) and scales by 2^-m.
+void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint32_t* const* d_dst, uint32_t n_cols,
+                       uint32_t m, uint32_t src_log);
+// Lifted Merkle: columns sorted by log size ascending (d_cols/d_logs device arrays).
+void launch_merkle_leaves(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_logs, uint32_t n_cols,
+                          uint32_t max_log, uint8_t* d_leaves);
+void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size, uint8_t* d_leaves);
+void launch_merkle_parents(Ctx& c, DeviceTree& t);
+void launch_merkle_parent_layer(Ctx& c, const uint8_t* d_children, uint32_t log_size, uint8_t* d_parents);
+// FRI folds (folding.zig): line domain halfOdds(log_src) / circle domain CanonicCoset(log_src).
+void launch_fold_line(Ctx& c, const uint32_t* d_src, uint32_t log_src, qm31 alpha, uint32_t* d_dst,
+                      uint8_t* d_leaf_out);
+// `coords` is a host array of 4 device pointers.
+void launch_fold_circle_into_line(Ctx& c, const uint32_t* const* coords, uint32_t log_src, qm31 alpha,
+                                  uint32_t* d_dst, bool accumulate, uint8_t* d_leaf_out);
+// Quotients (row_evaluation.zig).
+struct QuotientBatchDev {
+    cm31 det, pix, piy;  // sample point components
+    qm31 sum_a, sum_b;   // linear term
+    uint32_t first, count;  // range into the (column, coeff) entry arrays
+};
+void launch_quotients(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_col_logs, const uint32_t* d_entry_cols,
+                      const uint32_t* d_entry_coeffs, const QuotientBatchDev* d_batches, uint32_t n_batches,
+                      uint32_t log_size, uint32_t* const* out_coords /* host array of 4 */, uint8_t* d_leaf_out);
+// Point evaluation: sum_i coeff_i * w_i with weights of `point` (2^log).
+void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w);
+void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
+                         const uint32_t* d_w, uint32_t* d_out /* n_cols * 4 words */);
+// Wide Fibonacci.
+void launch_widefib_trace(Ctx& c, uint32_t log_n, uint32_t seq_len, uint32_t mode, uint64_t seed, uint32_t* d_out,
+                          uint64_t col_stride);
+void launch_widefib_composition(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_rows_eval,
+                                uint32_t log_n_rows, const uint32_t* d_powers, uint32_t den_inv0,
+                                uint32_t den_inv1, uint32_t* const* out_coords /* host array of 4 */);
+// PoW: lowest nonce in [base, base+count) or UINT64_MAX.
+uint64_t launch_pow_grind(Ctx& c, const uint8_t prefix[32], uint32_t bits, uint64_t base, uint64_t count);
+// Gather words: out[i] = word at absolute device address d_addr[i].
+void launch_gather(Ctx& c, const uint64_t* d_addr, uint64_t n, uint32_t* d_out);
+void launch_tower(Ctx& c, uint32_t log, uint32_t* d_tw, uint32_t* d_itw);
+void launch_domain_points(Ctx& c, uint32_t log, uint32_t* d_xy);
+void launch_circle_inv_y(Ctx& c, uint32_t log, uint32_t* d_out);
+void launch_batch_inverse(Ctx& c, const uint32_t* d_in, uint64_t n, uint32_t* d_out);
+void launch_bit_reverse(Ctx& c, uint32_t* d_vals, uint32_t log_size, uint32_t words_per_elem);
+void launch_accumulate(Ctx& c, uint32_t* d_acc, const uint32_t* d_src, uint64_t n_words);
+void launch_qm31_to_coords(Ctx& c, const uint32_t* d_qm31, uint64_t n, uint32_t* const* coords /* host array */);
+
+// Merkle commit of arbitrary columns (device pointers with log sizes); returns a resident tree.
+DeviceTree* merkle_commit(Ctx& c, const std::vector<const uint32_t*>& cols, const std::vector<uint32_t>& logs);
+DeviceTree* merkle_commit_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size);
+void tree_read_root(Ctx& c, DeviceTree& t);
+void tree_free(Ctx& c, DeviceTree* t);
+
+// Upload a small host array into an arena buffer.
+template <class T>
+DBuf upload(Ctx& c, const T* host, size_t n) {
+    DBuf b(&c, n * sizeof(T));
+    if (n) SR_HIP(hipMemcpyAsync(b.p, c.stage(host, n * sizeof(T)), n * sizeof(T), hipMemcpyHostToDevice, c.stream));
+    return b;
+}
+
+}  // namespace sr

@@ -1,0 +1,394 @@
+"""Python binding of the rocm proving backend (ctypes over include/stwo_rocm.h).
+
+The method names follow the reference backend hooks a Zig ``RocmBackend`` would
+forward to the same C-ABI (src/backend/*.zig contracts, the MetalCommitBackend
+hook set in src/backends/metal/commit_backend.zig): interpolate/evaluate circle
+buffers, commit_merkle, fold_line, fold_circle_into_line, compute quotients,
+grind, and the whole wide-Fibonacci proof.
+
+There is no fallback: constructing a ``Context`` without the built library or
+without a HIP device raises ``RocmError``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libstwo_rocm.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "stwo_rocm.h")
+P = (1 << 31) - 1
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class RocmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("stwo_rocm error %d: %s" % (code, msg))
+        self.code = code
+
+
+class WideFibParams(ctypes.Structure):
+    _fields_ = [
+        ("log_n_rows", ctypes.c_uint32),
+        ("sequence_len", ctypes.c_uint32),
+        ("pow_bits", ctypes.c_uint32),
+        ("log_blowup_factor", ctypes.c_uint32),
+        ("log_last_layer_degree_bound", ctypes.c_uint32),
+        ("n_queries", ctypes.c_uint32),
+        ("trace_mode", ctypes.c_uint32),
+        ("trace_seed", ctypes.c_uint64),
+    ]
+
+
+class ProveStats(ctypes.Structure):
+    _fields_ = [
+        ("stage_gpu_ms", ctypes.c_double * 8),
+        ("stage_host_ms", ctypes.c_double * 8),
+        ("total_gpu_ms", ctypes.c_double),
+        ("total_host_ms", ctypes.c_double),
+        ("peak_hbm_bytes", ctypes.c_uint64),
+    ]
+
+    STAGES = ["main_commit", "composition", "sampled_values", "quotients", "fri_commit", "pow", "decommit", "encode"]
+
+    def as_dict(self):
+        return {
+            "stage_gpu_ms": dict(zip(self.STAGES, list(self.stage_gpu_ms))),
+            "stage_host_ms": dict(zip(self.STAGES, list(self.stage_host_ms))),
+            "total_gpu_ms": self.total_gpu_ms,
+            "total_host_ms": self.total_host_ms,
+            "peak_hbm_bytes": self.peak_hbm_bytes,
+        }
+
+
+# (name, restype, argtypes) for every entry point of include/stwo_rocm.h
+_SIGNATURES = [
+    ("stwo_rocm_ctx_create", ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp), ctypes.c_char_p,
+                                             ctypes.c_size_t]),
+    ("stwo_rocm_ctx_destroy", ctypes.c_int, [_vp]),
+    ("stwo_rocm_last_error", ctypes.c_char_p, [_vp]),
+    ("stwo_rocm_ctx_identity", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t]),
+    ("stwo_rocm_synchronize", ctypes.c_int, [_vp]),
+    ("stwo_rocm_memory_info", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                              ctypes.POINTER(ctypes.c_uint64)]),
+    ("stwo_rocm_release_cached", ctypes.c_int, [_vp]),
+    ("stwo_rocm_buffer_alloc", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    ("stwo_rocm_buffer_free", ctypes.c_int, [_vp, _vp]),
+    ("stwo_rocm_buffer_upload", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64]),
+    ("stwo_rocm_buffer_download", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64]),
+    ("stwo_rocm_twiddles_prepare", ctypes.c_int, [_vp, ctypes.c_uint32]),
+    ("stwo_rocm_twiddles_read", ctypes.c_int, [_vp, ctypes.c_uint32, _u32p, _u32p]),
+    ("stwo_rocm_batch_inverse", ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp]),
+    ("stwo_rocm_bit_reverse", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32]),
+    ("stwo_rocm_accumulate", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64]),
+    ("stwo_rocm_circle_interpolate", ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32,
+                                                     ctypes.POINTER(ctypes.c_double)]),
+    ("stwo_rocm_circle_evaluate", ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32,
+                                                  ctypes.c_uint32, ctypes.POINTER(_vp),
+                                                  ctypes.POINTER(ctypes.c_double)]),
+    ("stwo_rocm_eval_at_point", ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32, _u32p,
+                                                _u32p]),
+    ("stwo_rocm_merkle_commit", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_uint32, ctypes.POINTER(_vp),
+                                                ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]),
+    ("stwo_rocm_merkle_parent_layer", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp]),
+    ("stwo_rocm_tree_root", ctypes.c_int, [_vp, _vp, ctypes.c_char_p]),
+    ("stwo_rocm_tree_max_log", ctypes.c_int, [_vp, _u32p]),
+    ("stwo_rocm_tree_read_hashes", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, ctypes.c_uint32,
+                                                   ctypes.c_char_p]),
+    ("stwo_rocm_tree_decommit", ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32,
+                                                ctypes.c_char_p, ctypes.c_uint32, _u32p]),
+    ("stwo_rocm_tree_free", ctypes.c_int, [_vp, _vp]),
+    ("stwo_rocm_fri_fold_line", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _vp]),
+    ("stwo_rocm_fri_fold_circle_into_line", ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, _u32p, _vp]),
+    ("stwo_rocm_compute_quotients", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(_vp), _u32p, ctypes.c_uint32,
+                                                    _u32p, _u32p, _u32p, ctypes.c_uint32, _u32p,
+                                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    ("stwo_rocm_widefib_trace", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                                ctypes.c_uint64, _vp]),
+    ("stwo_rocm_widefib_composition", ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32,
+                                                      _u32p, ctypes.POINTER(_vp)]),
+    ("stwo_rocm_pow_grind", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
+    ("stwo_rocm_prove_wide_fibonacci", ctypes.c_int, [_vp, ctypes.POINTER(WideFibParams), _vp, ctypes.c_char_p,
+                                                       ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
+                                                       ctypes.POINTER(ProveStats)]),
+]
+
+SYMBOLS = [s[0] for s in _SIGNATURES]
+
+
+def load_library(path=LIB_PATH):
+    """Load the built HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RocmError(-1, "HIP library not built (%s): run __graft_entry__.build()" % path)
+    lib = ctypes.CDLL(path)
+    for name, res, args in _SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(_u32p)
+
+
+def _ptr_array(ptrs):
+    arr = (_vp * max(len(ptrs), 1))(*[int(p) for p in ptrs])
+    return arr
+
+
+class DeviceBuffer:
+    """A buffer of the context's HBM arena (freed with ``free`` or context close)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _vp()
+        ctx._check(ctx.lib.stwo_rocm_buffer_alloc(ctx.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value or 0
+
+    def __int__(self):
+        return self.ptr
+
+    def offset(self, words):
+        return self.ptr + 4 * int(words)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.ctx._check(self.ctx.lib.stwo_rocm_buffer_upload(self.ctx.h, self.ptr, arr.ctypes.data, arr.nbytes))
+        return self
+
+    def download(self, dtype=np.uint32, count=None, offset_bytes=0):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset_bytes) // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        self.ctx._check(self.ctx.lib.stwo_rocm_buffer_download(self.ctx.h, out.ctypes.data, self.ptr + offset_bytes,
+                                                               out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx._check(self.ctx.lib.stwo_rocm_buffer_free(self.ctx.h, self.ptr))
+            self.ptr = 0
+
+
+class Tree:
+    """A resident lifted Merkle tree (MerkleTree(H): root / maxLogSize / readHashes / decommit)."""
+
+    def __init__(self, ctx, handle, root):
+        self.ctx, self.h, self.root = ctx, handle, root
+
+    def max_log_size(self):
+        out = ctypes.c_uint32()
+        self.ctx._check(self.ctx.lib.stwo_rocm_tree_max_log(self.h, ctypes.byref(out)))
+        return out.value
+
+    def read_hashes(self, layer_log, indices):
+        idx, pidx = _u32(indices)
+        buf = ctypes.create_string_buffer(32 * max(len(idx), 1))
+        self.ctx._check(self.ctx.lib.stwo_rocm_tree_read_hashes(self.ctx.h, self.h, layer_log, pidx, len(idx), buf))
+        return [buf.raw[32 * i: 32 * i + 32] for i in range(len(idx))]
+
+    def decommit_witness(self, positions):
+        pos = np.ascontiguousarray(positions, dtype=np.uint64)
+        cap = 64 * (len(pos) + 1) * (self.max_log_size() + 1)
+        buf = ctypes.create_string_buffer(32 * cap)
+        n = ctypes.c_uint32()
+        self.ctx._check(self.ctx.lib.stwo_rocm_tree_decommit(self.ctx.h, self.h,
+                                                             pos.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                             len(pos), buf, cap, ctypes.byref(n)))
+        return [buf.raw[32 * i: 32 * i + 32] for i in range(n.value)]
+
+    def free(self):
+        if self.h:
+            self.ctx._check(self.ctx.lib.stwo_rocm_tree_free(self.ctx.h, self.h))
+            self.h = None
+
+
+class Context:
+    """One HIP device context (stream, HBM arena, twiddle tower)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        self.lib = load_library()
+        h = _vp()
+        err = ctypes.create_string_buffer(512)
+        rc = self.lib.stwo_rocm_ctx_create(device, hbm_budget_bytes, ctypes.byref(h), err, 512)
+        if rc != 0:
+            raise RocmError(rc, err.value.decode(errors="replace"))
+        self.h = h.value
+
+    # ---- lifecycle
+    def _check(self, rc):
+        if rc != 0:
+            raise RocmError(rc, self.lib.stwo_rocm_last_error(self.h).decode(errors="replace"))
+
+    def close(self):
+        if self.h:
+            self.lib.stwo_rocm_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def identity(self):
+        buf = ctypes.create_string_buffer(512)
+        self._check(self.lib.stwo_rocm_ctx_identity(self.h, buf, 512))
+        return buf.value.decode()
+
+    def synchronize(self):
+        self._check(self.lib.stwo_rocm_synchronize(self.h))
+
+    def memory_info(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.lib.stwo_rocm_memory_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"live": a.value, "cached": b.value, "peak": c.value}
+
+    def release_cached(self):
+        self._check(self.lib.stwo_rocm_release_cached(self.h))
+
+    # ---- buffers
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        return DeviceBuffer(self, max(arr.nbytes, 4)).upload(arr)
+
+    # ---- twiddles (M31TwiddleTower)
+    def prepare_twiddles(self, max_circle_log):
+        self._check(self.lib.stwo_rocm_twiddles_prepare(self.h, max_circle_log))
+
+    def read_twiddles(self, circle_log):
+        n = 1 << (circle_log - 1)
+        tw, ptw = _u32(np.zeros(n))
+        itw, pitw = _u32(np.zeros(n))
+        self._check(self.lib.stwo_rocm_twiddles_read(self.h, circle_log, ptw, pitw))
+        return tw, itw
+
+    # ---- FieldOps / AccumulationOps
+    def batch_inverse(self, d_in, n, d_out):
+        self._check(self.lib.stwo_rocm_batch_inverse(self.h, int(d_in), n, int(d_out)))
+
+    def bit_reverse(self, d_vals, log_size, words_per_elem=1):
+        self._check(self.lib.stwo_rocm_bit_reverse(self.h, int(d_vals), log_size, words_per_elem))
+
+    def accumulate(self, d_acc, d_src, n_words):
+        self._check(self.lib.stwo_rocm_accumulate(self.h, int(d_acc), int(d_src), n_words))
+
+    # ---- PolyOps: interpolateCircleBuffers / evaluateCircleBuffers / evalAtPoint
+    def interpolate_circle_buffers(self, cols, log_size):
+        arr = _ptr_array(cols)
+        ms = ctypes.c_double()
+        self._check(self.lib.stwo_rocm_circle_interpolate(self.h, arr, len(cols), log_size, ctypes.byref(ms)))
+        return ms.value
+
+    def evaluate_circle_buffers(self, coeffs, log_size, log_blowup, out):
+        a, b = _ptr_array(coeffs), _ptr_array(out)
+        ms = ctypes.c_double()
+        self._check(self.lib.stwo_rocm_circle_evaluate(self.h, a, len(coeffs), log_size, log_blowup, b,
+                                                       ctypes.byref(ms)))
+        return ms.value
+
+    def eval_at_point(self, coeffs, log_size, point8):
+        a = _ptr_array(coeffs)
+        p, pp = _u32(point8)
+        o, po = _u32(np.zeros(4 * max(len(coeffs), 1)))
+        self._check(self.lib.stwo_rocm_eval_at_point(self.h, a, len(coeffs), log_size, pp, po))
+        return o[: 4 * len(coeffs)].reshape(-1, 4)
+
+    # ---- MerkleOps
+    def commit_merkle(self, cols, log_sizes):
+        a = _ptr_array(cols)
+        logs, plogs = _u32(log_sizes if len(log_sizes) else [0])
+        h = _vp()
+        root = ctypes.create_string_buffer(32)
+        ms = ctypes.c_double()
+        self._check(self.lib.stwo_rocm_merkle_commit(self.h, a, plogs, len(cols), ctypes.byref(h), root,
+                                                     ctypes.byref(ms)))
+        return Tree(self, h.value, root.raw)
+
+    def merkle_parent_layer(self, d_children, log_size, d_parents):
+        self._check(self.lib.stwo_rocm_merkle_parent_layer(self.h, int(d_children), log_size, int(d_parents)))
+
+    # ---- FriOps
+    def fold_line(self, d_src, log_src, alpha, d_dst):
+        a, pa = _u32(alpha)
+        self._check(self.lib.stwo_rocm_fri_fold_line(self.h, int(d_src), log_src, pa, int(d_dst)))
+
+    def fold_circle_into_line(self, coords, log_src, alpha, d_dst):
+        a, pa = _u32(alpha)
+        self._check(self.lib.stwo_rocm_fri_fold_circle_into_line(self.h, _ptr_array(coords), log_src, pa,
+                                                                 int(d_dst)))
+
+    # ---- quotients (computeLazyQuotients / commitLazyMerkle)
+    def compute_quotients(self, lifting_log_size, columns, column_log_sizes, sample_columns, sample_points,
+                          sample_values, random_coeff, out_coords, commit=False):
+        logs, plogs = _u32(column_log_sizes if len(column_log_sizes) else [0])
+        sc, psc = _u32(sample_columns if len(sample_columns) else [0])
+        sp, psp = _u32(np.asarray(sample_points, dtype=np.uint32).reshape(-1) if len(sample_points) else [0])
+        sv, psv = _u32(np.asarray(sample_values, dtype=np.uint32).reshape(-1) if len(sample_values) else [0])
+        rc, prc = _u32(random_coeff)
+        h = _vp()
+        self._check(self.lib.stwo_rocm_compute_quotients(self.h, lifting_log_size, _ptr_array(columns), plogs,
+                                                         len(columns), psc, psp, psv, len(sample_columns), prc,
+                                                         _ptr_array(out_coords),
+                                                         ctypes.byref(h) if commit else None))
+        if commit:
+            root = ctypes.create_string_buffer(32)
+            self._check(self.lib.stwo_rocm_tree_root(self.h, h.value, root))
+            return Tree(self, h.value, root.raw)
+        return None
+
+    # ---- wide Fibonacci
+    def fill_quadratic_recurrence_trace(self, log_n_rows, seq_len, d_out, mode=0, seed=0):
+        self._check(self.lib.stwo_rocm_widefib_trace(self.h, log_n_rows, seq_len, mode, seed, int(d_out)))
+
+    def compute_composition_evaluation(self, eval_cols, log_n_rows, random_coeff, out_coords):
+        rc, prc = _u32(random_coeff)
+        self._check(self.lib.stwo_rocm_widefib_composition(self.h, _ptr_array(eval_cols), len(eval_cols), log_n_rows,
+                                                           prc, _ptr_array(out_coords)))
+
+    # ---- channel
+    def grind(self, digest, pow_bits):
+        out = ctypes.c_uint64()
+        self._check(self.lib.stwo_rocm_pow_grind(self.h, bytes(digest), pow_bits, ctypes.byref(out)))
+        return out.value
+
+    # ---- whole proof
+    def prove_wide_fibonacci(self, log_n_rows, sequence_len, pow_bits=10, log_blowup_factor=1,
+                             log_last_layer_degree_bound=0, n_queries=3, trace_mode=0, trace_seed=0, d_trace=None):
+        prm = WideFibParams(log_n_rows, sequence_len, pow_bits, log_blowup_factor, log_last_layer_degree_bound,
+                            n_queries, trace_mode, trace_seed)
+        stats = ProveStats()
+        cap = 1 << 20
+        n = ctypes.c_uint64()
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = self.lib.stwo_rocm_prove_wide_fibonacci(self.h, ctypes.byref(prm),
+                                                         int(d_trace) if d_trace is not None else None, buf, cap,
+                                                         ctypes.byref(n), ctypes.byref(stats))
+            if rc == 1 and n.value > cap:
+                cap = n.value
+                continue
+            self._check(rc)
+            return buf.raw[: n.value], stats.as_dict()
+
+
+# Protocol presets (src/prover/native/config.zig:139-159).
+PROTOCOLS = {
+    "functional": dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3),
+    "secure": dict(pow_bits=26, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=70),
+}

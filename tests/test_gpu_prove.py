@@ -1,0 +1,97 @@
+"""Whole wide-Fibonacci proofs on the GPU through the C-ABI.
+
+- Byte equality with every committed golden proof of the reference (functional
+  protocol, reference trace seeds), including 2^20 x 100 (sha256 e6609d05...).
+- Byte equality with the oracle on configurations no golden covers: seeded
+  uniform traces, larger query counts, other blowups / last-layer bounds, and a
+  higher PoW difficulty.
+- Determinism across repeated proofs and between a resident (caller-provided)
+  trace and one generated inside the call.
+- Error paths leave the context usable.
+"""
+import hashlib
+import sys
+
+import pytest
+
+import oracle
+import stwo_rocm
+from conftest import golden_kwargs, load_goldens
+
+pytestmark = pytest.mark.gpu
+GOLDENS = load_goldens()
+
+
+@pytest.mark.parametrize("g", GOLDENS, ids=[g["name"] for g in GOLDENS])
+def test_gpu_proof_equals_golden(ctx, g):
+    print("golden", g["name"], flush=True)
+    proof, stats = ctx.prove_wide_fibonacci(**golden_kwargs(g))
+    assert hashlib.sha256(proof).hexdigest() == g["sha256"]
+    assert proof == g["proof"]
+    print("  gpu ms %.2f host ms %.2f" % (stats["total_gpu_ms"], stats["total_host_ms"]), flush=True)
+
+
+CASES = [
+    dict(log_n_rows=8, sequence_len=16, pow_bits=5, n_queries=70, trace_mode=1, trace_seed=1),
+    dict(log_n_rows=12, sequence_len=16, pow_bits=12, n_queries=20, trace_mode=1, trace_seed=7),
+    dict(log_n_rows=10, sequence_len=33, pow_bits=3, log_blowup_factor=2, n_queries=5, trace_mode=1, trace_seed=3),
+    dict(log_n_rows=11, sequence_len=20, pow_bits=4, log_last_layer_degree_bound=2, n_queries=9, trace_mode=0),
+    dict(log_n_rows=9, sequence_len=12, pow_bits=2, log_blowup_factor=3, log_last_layer_degree_bound=1,
+         n_queries=4, trace_mode=1, trace_seed=11),
+    dict(log_n_rows=16, sequence_len=64, pow_bits=10, n_queries=3, trace_mode=1, trace_seed=2026),
+    dict(log_n_rows=2, sequence_len=3, pow_bits=1, n_queries=2, trace_mode=1, trace_seed=5),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[str(i) for i in range(len(CASES))])
+def test_gpu_proof_equals_oracle(ctx, case):
+    print("case", case, flush=True)
+    exp = oracle.prove_wide_fibonacci(**case)
+    got, _ = ctx.prove_wide_fibonacci(**case)
+    assert got == exp
+
+
+def test_resident_trace_and_determinism(ctx):
+    n, W = 14, 32
+    d = ctx.alloc(4 * W << n)
+    ctx.fill_quadratic_recurrence_trace(n, W, d.ptr, mode=1, seed=77)
+    a, _ = ctx.prove_wide_fibonacci(n, W, trace_mode=1, trace_seed=77, d_trace=d)
+    b, _ = ctx.prove_wide_fibonacci(n, W, trace_mode=1, trace_seed=77)
+    c, _ = ctx.prove_wide_fibonacci(n, W, trace_mode=1, trace_seed=77, d_trace=d)
+    assert a == b == c
+    assert a == oracle.prove_wide_fibonacci(n, W, trace_mode=1, trace_seed=77)
+    d.free()
+
+
+def test_invalid_arguments_return_errors_and_context_survives(ctx):
+    with pytest.raises(stwo_rocm.RocmError) as e:
+        ctx.prove_wide_fibonacci(5, 2)  # sequence_len < 3
+    assert e.value.code == 1
+    with pytest.raises(stwo_rocm.RocmError):
+        ctx.prove_wide_fibonacci(5, 16, log_blowup_factor=0)
+    src, dst = ctx.alloc(16 * 8), ctx.alloc(16 * 4)
+    with pytest.raises(stwo_rocm.RocmError):
+        ctx.fold_line(src.ptr, 0, [0, 0, 0, 0], dst.ptr)  # log size 0 rejected
+    with pytest.raises(stwo_rocm.RocmError):
+        ctx.fold_line(src.ptr, 3, [0x7fffffff, 0, 0, 0], dst.ptr)  # non-canonical alpha rejected
+    src.free()
+    dst.free()
+    g = GOLDENS[0]
+    proof, _ = ctx.prove_wide_fibonacci(**golden_kwargs(g))
+    assert proof == g["proof"]
+
+
+def test_hbm_budget_is_enforced():
+    small = stwo_rocm.Context(0, hbm_budget_bytes=64 << 20)
+    try:
+        g = GOLDENS[0]
+        with pytest.raises(stwo_rocm.RocmError) as e:
+            small.prove_wide_fibonacci(18, 100)
+        assert e.value.code == 4
+        # a failed proof releases its buffers: only resident tables stay live
+        assert small.memory_info()["live"] < (8 << 20)
+        proof, _ = small.prove_wide_fibonacci(**golden_kwargs(g))
+        assert proof == g["proof"]
+        assert small.memory_info()["live"] < (8 << 20)
+    finally:
+        small.close()

@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Wide-Fibonacci 2^20 x 100 proof on one MI355X through the rocm C-ABI.
+
+One step = one whole proof (channel init -> proof bytes; the reference's
+prove_timer, src/prover/native/runner.zig:620-663) of the wide-Fibonacci AIR
+with log_n_rows=20, sequence_len=100 under the functional protocol (pow 10,
+3 queries, blowup 1), the configuration of the reference's published
+2^20 x 100 number.  The trace is generated on the device before the timed
+region (the reference times trace generation separately as input_seconds).
+Every timed proof is checked byte for byte against the reference's golden
+proof (sha256 e6609d05...).
+
+Prints one JSON line.  `value` is the median prove time in ms (lower is
+better).  `roofline` describes the dominant kernel class, timed with HIP events
+on the context stream during the timed proofs.  `cpu_baseline` times the C
+oracle (a multithreaded restatement of the reference CPU backend) on this
+host on one proof of the same workload.
+"""
+import argparse
+import hashlib
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "stwo-zig_amd"))
+
+METRIC = "wide-Fib 2^20x100 prove ms; batched proofs/s + p99; LDE/Merkle HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+PUBLISHED_CPU_MS = 900.664792  # conformance/evidence/metrics-v2/w1-log20x100/cpu-report.json (M5 Max, 18 threads)
+GOLDEN_SHA = {(20, 100, "functional"): "e6609d0564a47192212bec7973e2660c2eea88bef90c573c3df09569cc3c7e86"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log-n-rows", type=int, default=20)
+    ap.add_argument("--sequence-len", type=int, default=100)
+    ap.add_argument("--protocol", choices=["functional", "secure"], default="functional")
+    ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import stwo_rocm
+
+    n, W = args.log_n_rows, args.sequence_len
+    proto = dict(stwo_rocm.PROTOCOLS[args.protocol])
+    ctx = stwo_rocm.Context(0)
+    ident = ctx.identity()
+    ctx.prepare_twiddles(n + proto["log_blowup_factor"] + 1)
+    d_trace = ctx.alloc(4 * W << n)
+    ctx.fill_quadratic_recurrence_trace(n, W, d_trace.ptr, mode=0)
+    kw = dict(log_n_rows=n, sequence_len=W, trace_mode=0, d_trace=d_trace, **proto)
+
+    for _ in range(args.warmup):
+        ctx.prove_wide_fibonacci(**kw)
+
+    golden = GOLDEN_SHA.get((n, W, args.protocol))
+    ctx.profile_enable(True)
+    ctx.synchronize()
+    times, shas, stats_all = [], set(), []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        proof, st = ctx.prove_wide_fibonacci(**kw)
+        times.append((time.perf_counter() - t0) * 1e3)
+        shas.add(hashlib.sha256(proof).hexdigest())
+        stats_all.append(st)
+    ctx.synchronize()
+    wall_ms = (time.perf_counter() - t_start) * 1e3
+    prof = ctx.profile_read()
+    ctx.profile_enable(False)
+    if len(shas) != 1:
+        raise SystemExit("non-deterministic proofs across steps")
+    sha = shas.pop()
+    if golden is not None and sha != golden:
+        raise SystemExit("proof differs from the reference golden proof: %s" % sha)
+
+    median_ms = statistics.median(times)
+    # Dominant kernel class by device time over the timed proofs.
+    dom, dv = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    avg_ms = dv["ms"] / dv["launches"]
+    bytes_per_launch = dv["bytes"] / dv["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {"launches_per_proof": v["launches"] / args.steps, "ms_per_proof": v["ms"] / args.steps,
+                   "GBs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+    stage_gpu = {k: statistics.median(s["stage_gpu_ms"][k] for s in stats_all) for k in stats_all[0]["stage_gpu_ms"]}
+
+    cpu = None
+    if args.cpu_baseline == "on":
+        cpu = cpu_baseline(n, W, proto)
+
+    line = {
+        "metric": METRIC,
+        "value": round(median_ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms / args.steps, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": round(median_ms / PUBLISHED_CPU_MS, 6) if (n, W, args.protocol) == (20, 100, "functional")
+        else None,
+        "dtype": "u32 (exact M31/CM31/QM31)",
+        "data": "synthetic: reference wide-Fibonacci trace seeds (1, i), generated on device before timing",
+        "config": {"workload": "wide_fibonacci log_n_rows=%d sequence_len=%d protocol=%s (pow_bits=%d, n_queries=%d,"
+                               " log_blowup=%d), one proof" % (n, W, args.protocol, proto["pow_bits"],
+                                                               proto["n_queries"], proto["log_blowup_factor"]),
+                   "log_n_rows": n, "sequence_len": W, "protocol": args.protocol, "device": ident},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+        "cpu_baseline": cpu,
+        "proof_sha256": sha,
+        "bit_exact_vs_reference_golden": golden is not None and sha == golden,
+        "prove_ms": {"median": round(median_ms, 3), "min": round(min(times), 3), "max": round(max(times), 3)},
+        "device_ms_per_proof": round(statistics.median(s["total_gpu_ms"] for s in stats_all), 3),
+        "stage_device_ms": {k: round(v, 3) for k, v in stage_gpu.items()},
+        "kernels": kernels,
+        "published_cpu_ms_m5max_18t": PUBLISHED_CPU_MS,
+        "speedup_vs_published_cpu": round(PUBLISHED_CPU_MS / median_ms, 1),
+    }
+    if cpu:
+        line["speedup_vs_cpu_baseline"] = round(cpu["value"] / median_ms, 1)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def cpu_baseline(n, W, proto):
+    """The oracle (C + OpenMP restatement of the reference CPU backend) on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        import oracle
+    except Exception as e:  # oracle not built
+        return {"error": str(e)}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    runs = []
+    proof = None
+    for _ in range(2):
+        t0 = time.perf_counter()
+        proof = oracle.prove_wide_fibonacci(n, W, **proto)
+        runs.append(oracle.last_timings()["prove_total"])
+    return {"value": round(min(runs), 1), "unit": "ms", "cores": threads, "kind": "port",
+            "sample": "2 full proofs of the same workload (wide_fibonacci %dx%d); best prove time (trace generation"
+                      " excluded, as the reference's prove_seconds); proof sha256 %s"
+                      % (n, W, hashlib.sha256(proof).hexdigest()[:16])}
+
+
+if __name__ == "__main__":
+    main()

@@ -184,6 +184,16 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
                                    const uint32_t* d_trace, uint8_t* proof_out, uint64_t cap, uint64_t* proof_len,
                                    stwo_rocm_prove_stats* stats);
 
+/* ------------------------------------------------------------------ measurement */
+/* No counterpart in the reference: a measurement hook for callers that need per-kernel
+ * device time (the reference's stage profiler, src/prover/stage_profile.zig, times stages
+ * on the host).  While enabled, every kernel launch of the context is bracketed by HIP
+ * events on the context stream.  profile_read synchronises, returns a JSON object
+ * {class: {launches, ms, max_ms, bytes}} (bytes = algorithmic HBM bytes of the launches)
+ * and clears the records. */
+int stwo_rocm_profile_enable(stwo_rocm_ctx* ctx, int enable);
+int stwo_rocm_profile_read(stwo_rocm_ctx* ctx, char* json_out, size_t cap, size_t* len);
+
 #ifdef __cplusplus
 }
 #endif

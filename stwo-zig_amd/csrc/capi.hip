@@ -476,6 +476,8 @@ int stwo_rocm_compute_quotients(stwo_rocm_ctx* ctx, uint32_t L, const uint32_t* 
             DBuf dc = upload(c, columns, n_columns), dl = upload(c, column_log_sizes, n_columns);
             DBuf de = upload(c, ecols.data(), ecols.size()), df = upload(c, ecf.data(), ecf.size());
             DBuf db = upload(c, batches.data(), batches.size());
+            c.kbytes_hint = 0;
+            for (uint32_t i = 0; i < n_columns; i++) c.kbytes_hint += 4.0 * (double)(1ull << column_log_sizes[i]);
             launch_quotients(c, dc.as<const uint32_t* const>(), dl.as<uint32_t>(), de.as<uint32_t>(),
                              df.as<uint32_t>(), db.as<QuotientBatchDev>(), (uint32_t)batches.size(), L, out_coords,
                              t ? t->layer(L) : nullptr);
@@ -584,6 +586,53 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
         SR_CHECK(proof.size() <= cap, "proof buffer too small");
         SR_CHECK(proof_out, "null proof buffer");
         memcpy(proof_out, proof.data(), proof.size());
+    });
+}
+
+int stwo_rocm_profile_enable(stwo_rocm_ctx* ctx, int enable) {
+    return guarded(ctx, [&](Ctx& c) {
+        c.sync();
+        c.profiling = enable != 0;
+        c.krecs.clear();
+        c.ev_used = 0;
+    });
+}
+
+int stwo_rocm_profile_read(stwo_rocm_ctx* ctx, char* json_out, size_t cap, size_t* len) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(len, "null argument");
+        c.sync();
+        struct Agg {
+            uint64_t launches = 0;
+            double ms = 0, bytes = 0, max_ms = 0;
+        };
+        std::map<std::string, Agg> agg;
+        for (auto& r : c.krecs) {
+            float ms = 0;
+            SR_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            Agg& a = agg[r.cls];
+            a.launches++;
+            a.ms += ms;
+            a.bytes += r.bytes;
+            if (ms > a.max_ms) a.max_ms = ms;
+        }
+        std::string s = "{";
+        bool first = true;
+        for (auto& kv : agg) {
+            if (!first) s += ",";
+            first = false;
+            char buf[256];
+            snprintf(buf, sizeof buf, "\"%s\":{\"launches\":%llu,\"ms\":%.6f,\"max_ms\":%.6f,\"bytes\":%.0f}",
+                     kv.first.c_str(), (unsigned long long)kv.second.launches, kv.second.ms, kv.second.max_ms,
+                     kv.second.bytes);
+            s += buf;
+        }
+        s += "}";
+        *len = s.size();
+        SR_CHECK(json_out && s.size() < cap, "profile buffer too small");
+        memcpy(json_out, s.c_str(), s.size() + 1);
+        c.krecs.clear();
+        c.ev_used = 0;
     });
 }
 

@@ -18,6 +18,8 @@
 // tiles of 2^R strided rows x C consecutive words so that each row is a
 // coalesced C-word segment.  Each element is read from and written to HBM
 // once per pass.
+#include <algorithm>
+
 #include "runtime.hpp"
 
 namespace sr {
@@ -109,6 +111,163 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_pass_kernel(const uint32_t* c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Register-radix pass (used for every transform of log size >= 6).
+//
+// A pass covers local stages [0, R) of a tile of 2^R rows x C lanes (8192
+// elements for every pass but small transforms).  Each of the NT = tile/16
+// threads owns 16 elements of one lane whose rows differ in 4 consecutive row
+// bits [b, b+4); it runs up to 4 stages on them in registers (radix-16), then
+// the tile is exchanged through LDS so that the next 4 row bits become
+// thread-local.  A pass of R stages therefore costs ceil(R/4) LDS exchanges
+// instead of R.  Twiddles come from the resident tower (L2/MALL) with one load
+// per distinct value.  First group loads from HBM, last group stores to HBM.
+constexpr int RX_LOG = 4;  // 16 elements per thread
+constexpr int RX = 1 << RX_LOG;
+
+struct RxPass {
+    uint32_t m, src_log;
+    uint32_t s_lo, R, log_c;  // stages [s_lo, s_lo + R), tile 2^R x 2^log_c
+    uint32_t tower_log;
+    uint32_t scale;
+};
+
+__device__ __forceinline__ uint32_t rx_lds_index(uint32_t j, uint32_t lane, uint32_t log_c) {
+    // [row][lane]; contiguous tiles (C == 1) are skewed by one word per 16 rows.
+    return log_c == 0 ? (j + (j >> 4)) : ((j << log_c) | lane);
+}
+
+// Row of register slot k for a thread owning row bits [b, b+4).
+__device__ __forceinline__ uint32_t rx_row(uint32_t r, uint32_t k, uint32_t b) {
+    return (r & ((1u << b) - 1)) | (k << b) | ((r >> b) << (b + RX_LOG));
+}
+
+// Butterflies of local stage `ls` (= b + q) on the 16 register elements.
+template <bool INV, int Q>
+__device__ __forceinline__ void rx_stage(uint32_t (&v)[RX], const uint32_t* __restrict__ tower, const RxPass& ps,
+                                         uint32_t hi, uint32_t r, uint32_t b, uint32_t base) {
+    const uint32_t ls = b + Q;
+    const uint32_t s = ps.s_lo + ls;
+    const uint32_t s_hi = ps.s_lo + ps.R;
+#pragma unroll
+    for (int hk = 0; hk < (RX >> (Q + 1)); hk++) {
+        // rows with slot bit Q clear and slot bits above Q = hk share one twiddle for s > 0
+        uint32_t k_first = (uint32_t)hk << (Q + 1);
+        uint32_t j0 = rx_row(r, k_first, b);
+        uint32_t tw = 0;
+        if (s > 0) {
+            uint32_t h = (hi << (s_hi - s - 1)) | (j0 >> (ls + 1));
+            tw = tower[((1u << (ps.tower_log - 1)) - (1u << (ps.m - s))) + h];
+        }
+#pragma unroll
+        for (int lk = 0; lk < (1 << Q); lk++) {
+            const int k0 = (hk << (Q + 1)) | lk, k1 = k0 | (1 << Q);
+            uint32_t t = tw;
+            if (s == 0) {
+                // circle stage: pairs (2h', 2h'+1), twiddles [y, -y, -x, x] from T_0 pairs
+                uint32_t hp = (base | rx_row(r, (uint32_t)k0, b)) >> 1;
+                uint32_t kk = hp >> 2, rr = hp & 3;
+                const uint32_t* T0 = tower + ((1u << (ps.tower_log - 1)) - (1u << (ps.m - 1)));
+                uint32_t w = (rr < 2) ? T0[2 * kk + 1] : T0[2 * kk];
+                t = (rr == 1 || rr == 2) ? mneg(w) : w;
+            }
+            uint32_t a = v[k0], c = v[k1];
+            if (!INV) {
+                uint32_t tmp = mmul(c, t);
+                v[k0] = madd(a, tmp);
+                v[k1] = msub(a, tmp);
+            } else {
+                v[k0] = madd(a, c);
+                v[k1] = mmul(msub(a, c), t);
+            }
+        }
+    }
+}
+
+// Runs the active stages of a group (local bits [lo_act, hi_act) within [b, b+4)).
+template <bool INV>
+__device__ __forceinline__ void rx_group(uint32_t (&v)[RX], const uint32_t* __restrict__ tower, const RxPass& ps,
+                                         uint32_t hi, uint32_t r, uint32_t b, uint32_t lo_act, uint32_t hi_act,
+                                         uint32_t base) {
+    if (!INV) {
+        if (b + 3 >= lo_act && b + 3 < hi_act) rx_stage<false, 3>(v, tower, ps, hi, r, b, base);
+        if (b + 2 >= lo_act && b + 2 < hi_act) rx_stage<false, 2>(v, tower, ps, hi, r, b, base);
+        if (b + 1 >= lo_act && b + 1 < hi_act) rx_stage<false, 1>(v, tower, ps, hi, r, b, base);
+        if (b + 0 >= lo_act && b + 0 < hi_act) rx_stage<false, 0>(v, tower, ps, hi, r, b, base);
+    } else {
+        if (b + 0 >= lo_act && b + 0 < hi_act) rx_stage<true, 0>(v, tower, ps, hi, r, b, base);
+        if (b + 1 >= lo_act && b + 1 < hi_act) rx_stage<true, 1>(v, tower, ps, hi, r, b, base);
+        if (b + 2 >= lo_act && b + 2 < hi_act) rx_stage<true, 2>(v, tower, ps, hi, r, b, base);
+        if (b + 3 >= lo_act && b + 3 < hi_act) rx_stage<true, 3>(v, tower, ps, hi, r, b, base);
+    }
+}
+
+template <bool INV>
+__global__ __launch_bounds__(512) void fft_rx_kernel(const uint32_t* const* __restrict__ src_cols,
+                                                     uint32_t* const* __restrict__ dst_cols,
+                                                     const uint32_t* __restrict__ tower, RxPass ps) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t C = 1u << ps.log_c;
+    const uint32_t tile_log = ps.R + ps.log_c;
+    const uint32_t lo_blocks_log = ps.s_lo - ps.log_c;
+    const uint32_t blocks_per_col = 1u << (ps.m - tile_log);
+    const uint32_t col = blockIdx.x >> (ps.m - tile_log);
+    const uint32_t bb = blockIdx.x & (blocks_per_col - 1);
+    const uint32_t lo_blk = bb & ((1u << lo_blocks_log) - 1);
+    const uint32_t hi = bb >> lo_blocks_log;
+    const uint32_t s_hi = ps.s_lo + ps.R;
+    const uint32_t base = (hi << s_hi) | (lo_blk << ps.log_c);
+    const uint32_t t = threadIdx.x;
+    const uint32_t lane = t & (C - 1);
+    const uint32_t r = t >> ps.log_c;  // R - 4 row bits not owned by registers
+    const uint32_t* __restrict__ src = src_cols[col];
+    uint32_t* __restrict__ dst = dst_cols[col];
+    const uint32_t src_n = 1u << ps.src_log;
+
+    // Groups of 4 row bits: forward top-down, inverse bottom-up.  The outermost
+    // group is aligned to the tile edge and may be partial (fewer active stages).
+    const uint32_t n_groups = (ps.R + RX_LOG - 1) / RX_LOG;
+    uint32_t v[RX];
+    for (uint32_t gi = 0; gi < n_groups; gi++) {
+        uint32_t b, lo_act, hi_act;
+        if (!INV) {
+            uint32_t top = ps.R - gi * RX_LOG;  // active bits [top-4, top) clipped at 0
+            lo_act = top >= (uint32_t)RX_LOG ? top - RX_LOG : 0;
+            hi_act = top;
+            b = top >= (uint32_t)RX_LOG ? top - RX_LOG : 0;
+        } else {
+            lo_act = gi * RX_LOG;
+            hi_act = lo_act + RX_LOG < ps.R ? lo_act + RX_LOG : ps.R;
+            b = lo_act + RX_LOG <= ps.R ? lo_act : ps.R - RX_LOG;
+        }
+        if (gi == 0) {
+#pragma unroll
+            for (int k = 0; k < RX; k++) {
+                uint32_t idx = base | (rx_row(r, k, b) << ps.s_lo) | lane;
+                v[k] = idx < src_n ? src[idx] : 0u;
+            }
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < RX; k++) v[k] = lds[rx_lds_index(rx_row(r, k, b), lane, ps.log_c)];
+        }
+        rx_group<INV>(v, tower, ps, hi, r, b, lo_act, hi_act, base);
+        if (gi + 1 < n_groups) {
+            if (gi > 0) __syncthreads();
+#pragma unroll
+            for (int k = 0; k < RX; k++) lds[rx_lds_index(rx_row(r, k, b), lane, ps.log_c)] = v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < RX; k++) {
+                uint32_t idx = base | (rx_row(r, k, b) << ps.s_lo) | lane;
+                uint32_t x = v[k];
+                if (ps.scale != 1) x = mmul(x, ps.scale);
+                dst[idx] = x;
+            }
+        }
+    }
+}
+
 // m <= 2: closed forms (transforms.zig:155-179,256-292), one thread per column.
 __global__ void fft_small_kernel(const uint32_t* const* __restrict__ src_cols, uint32_t* const* __restrict__ dst_cols,
                                  uint32_t n_cols, uint32_t m, uint32_t src_log, bool inverse, uint32_t x, uint32_t y,
@@ -162,12 +321,32 @@ static void run_pass(Ctx& c, bool inverse, const uint32_t* const* src, uint32_t*
     SR_CHECK(blocks < (1ull << 31), "fft grid too large");
     SR_CHECK(tile >= 2, "fft tile too small");
     const uint32_t* tower = inverse ? c.d_itw : c.d_tw;
+    KTimer kt(c, inverse ? "fft_ifft_pass" : "fft_lde_pass",
+              4.0 * n_cols * ((double)(1ull << (ps.src_log < ps.m ? ps.src_log : ps.m)) + (double)(1ull << ps.m)));
     if (inverse)
         hipLaunchKernelGGL(fft_pass_kernel<true>, dim3((uint32_t)blocks), dim3(FFT_THREADS), lds, c.stream, src, dst,
                            tower, ps);
     else
         hipLaunchKernelGGL(fft_pass_kernel<false>, dim3((uint32_t)blocks), dim3(FFT_THREADS), lds, c.stream, src,
                            dst, tower, ps);
+    SR_HIP(hipGetLastError());
+}
+
+static void run_rx_pass(Ctx& c, bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols,
+                        const RxPass& ps) {
+    uint32_t tile_log = ps.R + ps.log_c;
+    uint32_t nt = 1u << (tile_log - RX_LOG);
+    size_t lds = ps.log_c == 0 ? ((size_t)(1u << tile_log) + ((1u << tile_log) >> 4)) * 4 : ((size_t)4 << tile_log);
+    uint64_t blocks = (uint64_t)n_cols << (ps.m - tile_log);
+    SR_CHECK(blocks < (1ull << 31), "fft grid too large");
+    SR_CHECK(ps.R >= (uint32_t)RX_LOG && nt <= 512 && ps.log_c <= ps.s_lo, "fft pass shape");
+    const uint32_t* tower = inverse ? c.d_itw : c.d_tw;
+    KTimer kt(c, inverse ? "fft_ifft_pass" : "fft_lde_pass",
+              4.0 * n_cols * ((double)(1ull << (ps.src_log < ps.m ? ps.src_log : ps.m)) + (double)(1ull << ps.m)));
+    if (inverse)
+        hipLaunchKernelGGL(fft_rx_kernel<true>, dim3((uint32_t)blocks), dim3(nt), lds, c.stream, src, dst, tower, ps);
+    else
+        hipLaunchKernelGGL(fft_rx_kernel<false>, dim3((uint32_t)blocks), dim3(nt), lds, c.stream, src, dst, tower, ps);
     SR_HIP(hipGetLastError());
 }
 
@@ -188,45 +367,61 @@ void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint3
         return;
     }
     c.ensure_tower(m);
-    // Pass plan: [0, low) contiguous, then strided passes of <= 10 stages.
-    std::vector<std::pair<uint32_t, uint32_t>> passes;
-    uint32_t low = m < 11 ? m : 11;
-    passes.push_back({0, low});
-    for (uint32_t s = low; s < m; s += 10) passes.push_back({s, s + 10 < m ? s + 10 : m});
     uint32_t n_inv = minv((uint32_t)((1ull << m) % P));
-    auto mk = [&](std::pair<uint32_t, uint32_t> rg, bool last) {
+    if (m < 6) {
+        // Tiny transforms: one LDS pass, one stage per barrier.
         FftPass ps;
         ps.m = m;
-        ps.s_lo = rg.first;
-        ps.s_hi = rg.second;
+        ps.src_log = src_log;
+        ps.s_lo = 0;
+        ps.s_hi = m;
+        ps.log_c = 0;
         ps.tower_log = c.tower_log;
-        uint32_t R = rg.second - rg.first;
-        uint32_t log_c = 0;
-        if (rg.first > 0) {
-            // C consecutive words per strided row: >= 16 and a tile of ~4K..16K words.
-            log_c = 4;
-            while ((R + log_c) < 12 && log_c < rg.first) log_c++;
-            if (log_c > rg.first) log_c = rg.first;
-            while (((1u << R) << log_c) > FFT_MAX_TILE && log_c > 0) log_c--;
+        ps.scale = inverse ? n_inv : 1u;
+        run_pass(c, inverse, d_src, d_dst, n_cols, ps);
+        return;
+    }
+    // Pass plan: a contiguous pass of R0 <= 13 stages, then strided passes of 4..8
+    // stages over 8192-element tiles (>= 32 consecutive words per row).
+    const uint32_t TILE_LOG = 13;
+    std::vector<std::pair<uint32_t, uint32_t>> passes;  // (s_lo, R)
+    if (m <= TILE_LOG) {
+        passes.push_back({0, m});
+    } else {
+        uint32_t n_high = (m - TILE_LOG + 7) / 8;
+        uint32_t r0 = std::min<uint32_t>(TILE_LOG, m - 4 * n_high);
+        passes.push_back({0, r0});
+        uint32_t rem = m - r0, s = r0;
+        for (uint32_t i = 0; i < n_high; i++) {
+            uint32_t left = n_high - i;
+            uint32_t rr = (rem + left - 1) / left;
+            passes.push_back({s, rr});
+            s += rr;
+            rem -= rr;
         }
-        ps.log_c = log_c;
+    }
+    auto mk = [&](std::pair<uint32_t, uint32_t> p, bool last) {
+        RxPass ps;
+        ps.m = m;
+        ps.s_lo = p.first;
+        ps.R = p.second;
+        ps.log_c = p.first == 0 ? 0 : std::min<uint32_t>(TILE_LOG - p.second, p.first);
+        ps.tower_log = c.tower_log;
         ps.scale = (inverse && last) ? n_inv : 1u;
         return ps;
     };
     size_t np = passes.size();
     if (!inverse) {
-        // top-down: last pass in the plan first; it reads the (short) source.
         for (size_t i = 0; i < np; i++) {
-            size_t pi = np - 1 - i;
-            FftPass ps = mk(passes[pi], false);
-            ps.src_log = (i == 0) ? src_log : m;
-            run_pass(c, false, i == 0 ? d_src : (const uint32_t* const*)d_dst, d_dst, n_cols, ps);
+            RxPass ps = mk(passes[np - 1 - i], false);
+            ps.src_log = i == 0 ? src_log : m;
+            run_rx_pass(c, false, i == 0 ? d_src : (const uint32_t* const*)d_dst, d_dst, n_cols, ps);
         }
     } else {
         for (size_t i = 0; i < np; i++) {
-            FftPass ps = mk(passes[i], i + 1 == np);
+            RxPass ps = mk(passes[i], i + 1 == np);
             ps.src_log = m;
-            run_pass(c, true, i == 0 ? d_src : (const uint32_t* const*)d_dst, d_dst, n_cols, ps);
+            run_rx_pass(c, true, i == 0 ? d_src : (const uint32_t* const*)d_dst, d_dst, n_cols, ps);
         }
     }
 }

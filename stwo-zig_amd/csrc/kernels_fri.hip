@@ -63,6 +63,7 @@ void launch_fold_line(Ctx& c, const uint32_t* d_src, uint32_t log_src, qm31 alph
     // inverse twiddle layer of halfOdds(log_src): length 2^(log_src-1) at 2^(M-1) - 2^log_src.
     const uint32_t* inv_x = c.d_itw + ((1u << (c.tower_log - 1)) - (1u << log_src));
     uint32_t n = 1u << (log_src - 1);
+    KTimer kt(c, "fri_fold_line", (32.0 + 4.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n);
     hipLaunchKernelGGL(fold_line_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, (const uint4*)d_src, log_src,
                        inv_x, alpha, (uint4*)d_dst, d_leaf_out, c.leaf_seed);
     SR_HIP(hipGetLastError());
@@ -73,6 +74,7 @@ void launch_fold_circle_into_line(Ctx& c, const uint32_t* const* coords, uint32_
     SR_CHECK(log_src >= 1, "fold_circle: source too short");
     const uint32_t* inv_y = c.inv_circle_y(log_src);
     uint32_t n = 1u << (log_src - 1);
+    KTimer kt(c, "fri_fold_circle", (32.0 + 4.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n);
     hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, coords[0], coords[1],
                        coords[2], coords[3], log_src, inv_y, alpha, qsqr(alpha), accumulate, (uint4*)d_dst,
                        d_leaf_out, c.leaf_seed);

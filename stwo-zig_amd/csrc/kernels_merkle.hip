@@ -142,6 +142,7 @@ void launch_merkle_parent_layer(Ctx& c, const uint8_t* d_children, uint32_t log_
 void launch_merkle_leaves(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_logs, uint32_t n_cols,
                           uint32_t max_log, uint8_t* d_leaves) {
     uint64_t n = 1ull << max_log;
+    KTimer kt(c, "merkle_leaves", (4.0 * n_cols + 32.0) * (double)n);
     hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)), dim3(MK_THREADS), 0,
                        c.stream, d_cols, d_logs, n_cols, max_log, c.leaf_seed, d_leaves);
     SR_HIP(hipGetLastError());
@@ -149,6 +150,7 @@ void launch_merkle_leaves(Ctx& c, const uint32_t* const* d_cols, const uint32_t*
 
 void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size, uint8_t* d_leaves) {
     uint64_t n = 1ull << log_size;
+    KTimer kt(c, "merkle_leaves_qm31", 48.0 * (double)n);
     hipLaunchKernelGGL(merkle_leaves_qm31_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)),
                        dim3(MK_THREADS), 0, c.stream, (const uint4*)d_qm31, log_size, c.leaf_seed, d_leaves);
     SR_HIP(hipGetLastError());
@@ -159,6 +161,7 @@ void launch_merkle_parents(Ctx& c, DeviceTree& t) {
     while (k > 0) {
         uint32_t levels = k < 9 ? k : 9;
         uint32_t blocks = 1u << (k - levels);
+        KTimer kt(c, "merkle_parents", 32.0 * (double)((1ull << k) + (1ull << k) - (1ull << (k - levels))));
         hipLaunchKernelGGL(merkle_parents_kernel, dim3(blocks), dim3(MK_THREADS), 0, c.stream, t.layers, k, levels,
                            c.node_seed);
         SR_HIP(hipGetLastError());

@@ -38,6 +38,7 @@ uint64_t launch_pow_grind(Ctx& c, const uint8_t prefix[32], uint32_t bits, uint6
     memcpy(w, prefix, 32);
     uint64_t blocks = (count + 255) / 256;
     SR_CHECK(blocks < (1ull << 31), "pow: launch range too large");
+    KTimer kt(c, "pow_grind", 0.0);
     hipLaunchKernelGGL(pow_kernel, dim3((uint32_t)blocks), dim3(256), 0, c.stream, st,
                        make_uint4(w[0], w[1], w[2], w[3]), make_uint4(w[4], w[5], w[6], w[7]), bits, base, count,
                        best.as<unsigned long long>());

@@ -72,6 +72,7 @@ void launch_quotients(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_c
                       uint32_t log_size, uint32_t* const* out, uint8_t* d_leaf_out) {
     const uint32_t* xy = c.domain_points(log_size);
     uint32_t n = 1u << log_size;
+    KTimer kt(c, "quotients", c.kbytes_hint + (8.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n);
     hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_cols, d_col_logs,
                        d_entry_cols, (const uint4*)d_entry_coeffs, d_batches, n_batches, log_size, xy, out[0], out[1],
                        out[2], out[3], d_leaf_out, c.leaf_seed);
@@ -116,6 +117,7 @@ void launch_widefib_composition(Ctx& c, const uint32_t* const* d_cols, uint32_t 
                                 uint32_t* const* out) {
     SR_CHECK(n_cols >= 3, "composition needs at least 3 columns");
     uint32_t n = 1u << log_eval;
+    KTimer kt(c, "composition", (4.0 * n_cols + 16.0) * n);
     hipLaunchKernelGGL(widefib_composition_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_cols, n_cols,
                        log_eval, log_n, (const uint4*)d_powers, den0, den1, out[0], out[1], out[2], out[3]);
     SR_HIP(hipGetLastError());
@@ -158,6 +160,7 @@ __global__ __launch_bounds__(256) void widefib_trace_kernel(uint32_t log_n, uint
 void launch_widefib_trace(Ctx& c, uint32_t log_n, uint32_t seq_len, uint32_t mode, uint64_t seed, uint32_t* d_out,
                           uint64_t col_stride) {
     uint32_t n = 1u << log_n;
+    KTimer kt(c, "widefib_trace", 4.0 * seq_len * n);
     hipLaunchKernelGGL(widefib_trace_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, log_n, seq_len, mode,
                        seed, d_out, col_stride);
     SR_HIP(hipGetLastError());
@@ -181,6 +184,7 @@ void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint
     Factors fs;
     for (uint32_t j = 0; j < log_size; j++) fs.f[j] = factors[j];
     uint32_t n = 1u << log_size;
+    KTimer kt(c, "eval_weights", 16.0 * n);
     hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, fs, log_size, (uint4*)d_w);
     SR_HIP(hipGetLastError());
 }
@@ -239,6 +243,7 @@ void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols,
                        acc.as<unsigned long long>(), nw);
     uint32_t n = 1u << log_size;
     uint32_t chunks = (n + 256 * EV_PER_THREAD - 1) / (256 * EV_PER_THREAD);
+    KTimer kt(c, "eval_columns", (4.0 * n_cols + 16.0) * n);
     hipLaunchKernelGGL(eval_columns_kernel, dim3(chunks, n_cols), dim3(256), 0, c.stream, d_cols, log_size,
                        (const uint4*)d_w, acc.as<unsigned long long>());
     hipLaunchKernelGGL(reduce_u64_kernel, dim3((nw + 255) / 256), dim3(256), 0, c.stream,

@@ -449,6 +449,8 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         DBuf d_ec = upload(c, entry_cols.data(), entry_cols.size());
         DBuf d_ecf = upload(c, entry_coeffs.data(), entry_coeffs.size());
         DBuf d_b = upload(c, batches.data(), batches.size());
+        c.kbytes_hint = 0;
+        for (uint32_t i = 0; i < NS; i++) c.kbytes_hint += 4.0 * (double)(1ull << all_lde_logs[i]);
         launch_quotients(c, d_cols.as<const uint32_t* const>(), d_logs.as<uint32_t>(), d_ec.as<uint32_t>(),
                          d_ecf.as<uint32_t>(), d_b.as<QuotientBatchDev>(), (uint32_t)batches.size(), L, qc,
                          ftree0->layer(L));

@@ -69,6 +69,15 @@ void Ctx::trim() {
     bytes_cached = 0;
 }
 
+hipEvent_t Ctx::pool_event() {
+    if (ev_used == ev_pool.size()) {
+        hipEvent_t e;
+        SR_HIP(hipEventCreate(&e));
+        ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+}
+
 void Ctx::sync() {
     SR_HIP(hipStreamSynchronize(stream));
     ring_off = 0;

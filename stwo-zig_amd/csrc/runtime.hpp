@@ -73,6 +73,20 @@ struct Ctx {
     // events for GPU timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
+    // Opt-in per-kernel profile: events around each launch on `stream`,
+    // aggregated by kernel class with the launch's algorithmic HBM bytes.
+    struct KRec {
+        const char* cls;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    bool profiling = false;
+    double kbytes_hint = 0;  // input bytes of the next quotient launch (set by its caller)
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<KRec> krecs;
+    hipEvent_t pool_event();
+
     void* alloc(size_t bytes);
     void free(void* p);
     void trim();
@@ -85,6 +99,27 @@ struct Ctx {
     const uint32_t* twiddle_layer(bool inverse, uint32_t m, uint32_t s) const;
     const uint32_t* domain_points(uint32_t log);
     const uint32_t* inv_circle_y(uint32_t log);
+};
+
+// Records one kernel launch in the context profile (no-op unless profiling).
+struct KTimer {
+    Ctx& c;
+    const char* cls;
+    double bytes;
+    hipEvent_t a = nullptr;
+    KTimer(Ctx& cc, const char* k, double b) : c(cc), cls(k), bytes(b) {
+        if (c.profiling) {
+            a = c.pool_event();
+            (void)hipEventRecord(a, c.stream);
+        }
+    }
+    ~KTimer() {
+        if (c.profiling && a) {
+            hipEvent_t e = c.pool_event();
+            (void)hipEventRecord(e, c.stream);
+            c.krecs.push_back({cls, a, e, bytes});
+        }
+    }
 };
 
 // RAII device buffer from the context arena.

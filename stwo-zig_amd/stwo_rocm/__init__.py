@@ -116,6 +116,12 @@ _SIGNATURES = [
                                                        ctypes.POINTER(ProveStats)]),
 ]
 
+_SIGNATURES += [
+    ("stwo_rocm_profile_enable", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("stwo_rocm_profile_read", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t,
+                                               ctypes.POINTER(ctypes.c_size_t)]),
+]
+
 SYMBOLS = [s[0] for s in _SIGNATURES]
 
 
@@ -258,6 +264,19 @@ class Context:
 
     def release_cached(self):
         self._check(self.lib.stwo_rocm_release_cached(self.h))
+
+    # ---- per-kernel device-time profile (HIP events on the context stream)
+    def profile_enable(self, enable=True):
+        self._check(self.lib.stwo_rocm_profile_enable(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        import json
+
+        cap = 1 << 16
+        buf = ctypes.create_string_buffer(cap)
+        n = ctypes.c_size_t()
+        self._check(self.lib.stwo_rocm_profile_read(self.h, buf, cap, ctypes.byref(n)))
+        return json.loads(buf.value.decode())
 
     # ---- buffers
     def alloc(self, nbytes):

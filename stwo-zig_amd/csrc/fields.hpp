@@ -15,18 +15,24 @@ namespace sr {
 
 constexpr uint32_t P = 0x7fffffffu;
 
+// Canonical add/sub in three ops: the unsigned minimum picks the reduced
+// candidate because the other one wraps above 2^31.
+SR_HD uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
 SR_HD uint32_t madd(uint32_t a, uint32_t b) {
     uint32_t s = a + b;
-    return s >= P ? s - P : s;
+    return umin32(s, s - P);
 }
-SR_HD uint32_t msub(uint32_t a, uint32_t b) { return a >= b ? a - b : a + P - b; }
+SR_HD uint32_t msub(uint32_t a, uint32_t b) {
+    uint32_t d = a - b;
+    return umin32(d, d + P);
+}
 SR_HD uint32_t mneg(uint32_t a) { return a == 0 ? 0 : P - a; }
 // Product of two canonical words is < p^2 < 2^62: one Mersenne fold lands in
 // [0, 2p) and a conditional subtraction canonicalises (m31.zig:164-171).
 SR_HD uint32_t mmul(uint32_t a, uint32_t b) {
     uint64_t x = (uint64_t)a * b;
     uint32_t r = (uint32_t)(x & P) + (uint32_t)(x >> 31);
-    return r >= P ? r - P : r;
+    return umin32(r, r - P);
 }
 // Fold a 64-bit accumulator of folded products to a canonical word (m31.zig:150-158).
 SR_HD uint32_t mred64(uint64_t x) {

@@ -112,65 +112,79 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_pass_kernel(const uint32_t* c
 }
 
 // ---------------------------------------------------------------------------
-// Register-radix pass (used for every transform of log size >= 6).
+// Register-radix pass (every transform of log size >= 6).
 //
-// A pass covers local stages [0, R) of a tile of 2^R rows x C lanes (8192
-// elements for every pass but small transforms).  Each of the NT = tile/16
-// threads owns 16 elements of one lane whose rows differ in 4 consecutive row
-// bits [b, b+4); it runs up to 4 stages on them in registers (radix-16), then
-// the tile is exchanged through LDS so that the next 4 row bits become
-// thread-local.  A pass of R stages therefore costs ceil(R/4) LDS exchanges
-// instead of R.  Twiddles come from the resident tower (L2/MALL) with one load
-// per distinct value.  First group loads from HBM, last group stores to HBM.
+// A pass covers local stages [0, R) of a tile of 2^R rows x 2^LOGC lanes
+// (8192 elements except for small transforms).  Each of the tile/16 threads
+// owns 16 elements of one lane whose rows differ in 4 consecutive row bits
+// [b, b+4); it runs up to 4 butterfly stages on them in registers (radix 16),
+// then the tile is exchanged through LDS so that the next 4 row bits become
+// thread-local: ceil(R/4) LDS exchanges per pass instead of R barriers.  R and
+// LOGC are template parameters, so the row mapping, strides and twiddle
+// offsets of every group are compile-time shifts.  Twiddles of one stage are
+// consecutive tower words (one base + constant offsets).  The first group
+// loads from HBM, the last one stores to HBM.
 constexpr int RX_LOG = 4;  // 16 elements per thread
 constexpr int RX = 1 << RX_LOG;
 
 struct RxPass {
     uint32_t m, src_log;
-    uint32_t s_lo, R, log_c;  // stages [s_lo, s_lo + R), tile 2^R x 2^log_c
+    uint32_t s_lo;       // first stage of the pass
     uint32_t tower_log;
-    uint32_t scale;
+    uint32_t scale;      // multiply on store when != 1 (last inverse pass)
+    uint32_t dup_top;    // forward: top stage pairs with zero padding (v1 = 0)
 };
 
-__device__ __forceinline__ uint32_t rx_lds_index(uint32_t j, uint32_t lane, uint32_t log_c) {
-    // [row][lane]; contiguous tiles (C == 1) are skewed by one word per 16 rows.
-    return log_c == 0 ? (j + (j >> 4)) : ((j << log_c) | lane);
+template <int LOGC>
+__device__ __forceinline__ uint32_t rx_lds(uint32_t j, uint32_t lane) {
+    // [row][lane]; contiguous tiles are skewed by one word per 16 rows.
+    return LOGC == 0 ? (j + (j >> 4)) : ((j << LOGC) | lane);
 }
 
-// Row of register slot k for a thread owning row bits [b, b+4).
-__device__ __forceinline__ uint32_t rx_row(uint32_t r, uint32_t k, uint32_t b) {
-    return (r & ((1u << b) - 1)) | (k << b) | ((r >> b) << (b + RX_LOG));
-}
-
-// Butterflies of local stage `ls` (= b + q) on the 16 register elements.
-template <bool INV, int Q>
+template <bool INV, int R, int LOGC, int B, int Q>
 __device__ __forceinline__ void rx_stage(uint32_t (&v)[RX], const uint32_t* __restrict__ tower, const RxPass& ps,
-                                         uint32_t hi, uint32_t r, uint32_t b, uint32_t base) {
-    const uint32_t ls = b + Q;
-    const uint32_t s = ps.s_lo + ls;
-    const uint32_t s_hi = ps.s_lo + ps.R;
+                                         uint32_t hi, uint32_t r, uint32_t base, bool dup) {
+    constexpr int LS = B + Q;
+    constexpr bool MAY_BE_CIRCLE = (LS == 0 && LOGC == 0);
+    const uint32_t s = ps.s_lo + LS;
+    if (!MAY_BE_CIRCLE || s > 0) {
+        // h = (hi << (R-LS-1)) | (r_high << (3-Q)) | hk  with r_high = r >> B
+        const uint32_t tb = ((1u << (ps.tower_log - 1)) - (1u << (ps.m - s))) + ((hi << (R - LS - 1)) |
+                                                                                 ((r >> B) << (3 - Q)));
 #pragma unroll
-    for (int hk = 0; hk < (RX >> (Q + 1)); hk++) {
-        // rows with slot bit Q clear and slot bits above Q = hk share one twiddle for s > 0
-        uint32_t k_first = (uint32_t)hk << (Q + 1);
-        uint32_t j0 = rx_row(r, k_first, b);
-        uint32_t tw = 0;
-        if (s > 0) {
-            uint32_t h = (hi << (s_hi - s - 1)) | (j0 >> (ls + 1));
-            tw = tower[((1u << (ps.tower_log - 1)) - (1u << (ps.m - s))) + h];
-        }
+        for (int hk = 0; hk < (RX >> (Q + 1)); hk++) {
+            const uint32_t tw = tower[tb + hk];
 #pragma unroll
-        for (int lk = 0; lk < (1 << Q); lk++) {
-            const int k0 = (hk << (Q + 1)) | lk, k1 = k0 | (1 << Q);
-            uint32_t t = tw;
-            if (s == 0) {
-                // circle stage: pairs (2h', 2h'+1), twiddles [y, -y, -x, x] from T_0 pairs
-                uint32_t hp = (base | rx_row(r, (uint32_t)k0, b)) >> 1;
-                uint32_t kk = hp >> 2, rr = hp & 3;
-                const uint32_t* T0 = tower + ((1u << (ps.tower_log - 1)) - (1u << (ps.m - 1)));
-                uint32_t w = (rr < 2) ? T0[2 * kk + 1] : T0[2 * kk];
-                t = (rr == 1 || rr == 2) ? mneg(w) : w;
+            for (int lk = 0; lk < (1 << Q); lk++) {
+                const int k0 = (hk << (Q + 1)) | lk, k1 = k0 | (1 << Q);
+                uint32_t a = v[k0], c = v[k1];
+                if (!INV) {
+                    if (dup) {
+                        v[k1] = a;  // zero-padded partner: (a + 0*t, a - 0*t)
+                    } else {
+                        uint32_t tmp = mmul(c, tw);
+                        v[k0] = madd(a, tmp);
+                        v[k1] = msub(a, tmp);
+                    }
+                } else {
+                    v[k0] = madd(a, c);
+                    v[k1] = mmul(msub(a, c), tw);
+                }
             }
+        }
+    } else if constexpr (MAY_BE_CIRCLE) {
+        // Circle stage (B == 0, Q == 0, s_lo == 0): pairs (2h', 2h'+1), h' = hp0 + hk,
+        // twiddles [y, -y, -x, x] from consecutive pairs (x, y) of T_0.
+        const uint32_t* T0 = tower + ((1u << (ps.tower_log - 1)) - (1u << (ps.m - 1)));
+        const uint32_t kk0 = ((base >> 1) | (r << 3)) >> 2;
+        const uint32_t x0 = T0[2 * kk0], y0 = T0[2 * kk0 + 1], x1 = T0[2 * kk0 + 2], y1 = T0[2 * kk0 + 3];
+#pragma unroll
+        for (int hk = 0; hk < RX / 2; hk++) {
+            const int k0 = 2 * hk, k1 = k0 + 1;
+            const uint32_t x = hk < 4 ? x0 : x1, y = hk < 4 ? y0 : y1;
+            const int rr = hk & 3;
+            const uint32_t w = rr < 2 ? y : x;
+            const uint32_t t = (rr == 1 || rr == 2) ? mneg(w) : w;
             uint32_t a = v[k0], c = v[k1];
             if (!INV) {
                 uint32_t tmp = mmul(c, t);
@@ -184,88 +198,91 @@ __device__ __forceinline__ void rx_stage(uint32_t (&v)[RX], const uint32_t* __re
     }
 }
 
-// Runs the active stages of a group (local bits [lo_act, hi_act) within [b, b+4)).
-template <bool INV>
+// One group: row bits [B, B+4) owned, active local stages [LO, HI).
+template <bool INV, int R, int LOGC, int B, int LO, int HI>
 __device__ __forceinline__ void rx_group(uint32_t (&v)[RX], const uint32_t* __restrict__ tower, const RxPass& ps,
-                                         uint32_t hi, uint32_t r, uint32_t b, uint32_t lo_act, uint32_t hi_act,
-                                         uint32_t base) {
+                                         uint32_t hi, uint32_t r, uint32_t base, bool first) {
     if (!INV) {
-        if (b + 3 >= lo_act && b + 3 < hi_act) rx_stage<false, 3>(v, tower, ps, hi, r, b, base);
-        if (b + 2 >= lo_act && b + 2 < hi_act) rx_stage<false, 2>(v, tower, ps, hi, r, b, base);
-        if (b + 1 >= lo_act && b + 1 < hi_act) rx_stage<false, 1>(v, tower, ps, hi, r, b, base);
-        if (b + 0 >= lo_act && b + 0 < hi_act) rx_stage<false, 0>(v, tower, ps, hi, r, b, base);
+        if (B + 3 >= LO && B + 3 < HI) rx_stage<INV, R, LOGC, B, 3>(v, tower, ps, hi, r, base, first && ps.dup_top);
+        if (B + 2 >= LO && B + 2 < HI) rx_stage<INV, R, LOGC, B, 2>(v, tower, ps, hi, r, base, false);
+        if (B + 1 >= LO && B + 1 < HI) rx_stage<INV, R, LOGC, B, 1>(v, tower, ps, hi, r, base, false);
+        if (B + 0 >= LO && B + 0 < HI) rx_stage<INV, R, LOGC, B, 0>(v, tower, ps, hi, r, base, false);
     } else {
-        if (b + 0 >= lo_act && b + 0 < hi_act) rx_stage<true, 0>(v, tower, ps, hi, r, b, base);
-        if (b + 1 >= lo_act && b + 1 < hi_act) rx_stage<true, 1>(v, tower, ps, hi, r, b, base);
-        if (b + 2 >= lo_act && b + 2 < hi_act) rx_stage<true, 2>(v, tower, ps, hi, r, b, base);
-        if (b + 3 >= lo_act && b + 3 < hi_act) rx_stage<true, 3>(v, tower, ps, hi, r, b, base);
+        if (B + 0 >= LO && B + 0 < HI) rx_stage<INV, R, LOGC, B, 0>(v, tower, ps, hi, r, base, false);
+        if (B + 1 >= LO && B + 1 < HI) rx_stage<INV, R, LOGC, B, 1>(v, tower, ps, hi, r, base, false);
+        if (B + 2 >= LO && B + 2 < HI) rx_stage<INV, R, LOGC, B, 2>(v, tower, ps, hi, r, base, false);
+        if (B + 3 >= LO && B + 3 < HI) rx_stage<INV, R, LOGC, B, 3>(v, tower, ps, hi, r, base, false);
     }
 }
 
-template <bool INV>
-__global__ __launch_bounds__(512) void fft_rx_kernel(const uint32_t* const* __restrict__ src_cols,
-                                                     uint32_t* const* __restrict__ dst_cols,
-                                                     const uint32_t* __restrict__ tower, RxPass ps) {
-    extern __shared__ uint32_t lds[];
-    const uint32_t C = 1u << ps.log_c;
-    const uint32_t tile_log = ps.R + ps.log_c;
-    const uint32_t lo_blocks_log = ps.s_lo - ps.log_c;
-    const uint32_t blocks_per_col = 1u << (ps.m - tile_log);
-    const uint32_t col = blockIdx.x >> (ps.m - tile_log);
-    const uint32_t bb = blockIdx.x & (blocks_per_col - 1);
-    const uint32_t lo_blk = bb & ((1u << lo_blocks_log) - 1);
-    const uint32_t hi = bb >> lo_blocks_log;
-    const uint32_t s_hi = ps.s_lo + ps.R;
-    const uint32_t base = (hi << s_hi) | (lo_blk << ps.log_c);
-    const uint32_t t = threadIdx.x;
-    const uint32_t lane = t & (C - 1);
-    const uint32_t r = t >> ps.log_c;  // R - 4 row bits not owned by registers
-    const uint32_t* __restrict__ src = src_cols[col];
-    uint32_t* __restrict__ dst = dst_cols[col];
-    const uint32_t src_n = 1u << ps.src_log;
+// Group gi (compile time): forward top-down, inverse bottom-up; the outermost
+// group is aligned to the tile edge (B clamped) and may have fewer active stages.
+template <bool INV, int R, int GI>
+struct RxGroupShape {
+    static constexpr int TOP = R - GI * RX_LOG;
+    static constexpr int LO = INV ? GI * RX_LOG : (TOP >= RX_LOG ? TOP - RX_LOG : 0);
+    static constexpr int HI = INV ? (GI * RX_LOG + RX_LOG < R ? GI * RX_LOG + RX_LOG : R) : TOP;
+    static constexpr int B = INV ? (GI * RX_LOG + RX_LOG <= R ? GI * RX_LOG : R - RX_LOG) : LO;
+};
 
-    // Groups of 4 row bits: forward top-down, inverse bottom-up.  The outermost
-    // group is aligned to the tile edge and may be partial (fewer active stages).
-    const uint32_t n_groups = (ps.R + RX_LOG - 1) / RX_LOG;
-    uint32_t v[RX];
-    for (uint32_t gi = 0; gi < n_groups; gi++) {
-        uint32_t b, lo_act, hi_act;
-        if (!INV) {
-            uint32_t top = ps.R - gi * RX_LOG;  // active bits [top-4, top) clipped at 0
-            lo_act = top >= (uint32_t)RX_LOG ? top - RX_LOG : 0;
-            hi_act = top;
-            b = top >= (uint32_t)RX_LOG ? top - RX_LOG : 0;
-        } else {
-            lo_act = gi * RX_LOG;
-            hi_act = lo_act + RX_LOG < ps.R ? lo_act + RX_LOG : ps.R;
-            b = lo_act + RX_LOG <= ps.R ? lo_act : ps.R - RX_LOG;
-        }
-        if (gi == 0) {
+template <bool INV, int R, int LOGC, int GI>
+__device__ __forceinline__ void rx_run(uint32_t (&v)[RX], uint32_t* lds, const uint32_t* __restrict__ src,
+                                       uint32_t* __restrict__ dst, const uint32_t* __restrict__ tower,
+                                       const RxPass& ps, uint32_t hi, uint32_t r, uint32_t lane, uint32_t base) {
+    constexpr int NG = (R + RX_LOG - 1) / RX_LOG;
+    if constexpr (GI < NG) {
+        using S = RxGroupShape<INV, R, GI>;
+        constexpr int B = S::B;
+        const uint32_t row0 = (r & ((1u << B) - 1)) | ((r >> B) << (B + RX_LOG));
+        if (GI == 0) {
+            const uint32_t g0 = base | (row0 << ps.s_lo) | lane;
+            const uint32_t src_n = 1u << ps.src_log;
 #pragma unroll
             for (int k = 0; k < RX; k++) {
-                uint32_t idx = base | (rx_row(r, k, b) << ps.s_lo) | lane;
+                const uint32_t idx = g0 + ((uint32_t)k << (B + ps.s_lo));
                 v[k] = idx < src_n ? src[idx] : 0u;
             }
         } else {
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < RX; k++) v[k] = lds[rx_lds_index(rx_row(r, k, b), lane, ps.log_c)];
+            for (int k = 0; k < RX; k++) v[k] = lds[rx_lds<LOGC>(row0 + ((uint32_t)k << B), lane)];
         }
-        rx_group<INV>(v, tower, ps, hi, r, b, lo_act, hi_act, base);
-        if (gi + 1 < n_groups) {
-            if (gi > 0) __syncthreads();
+        rx_group<INV, R, LOGC, B, S::LO, S::HI>(v, tower, ps, hi, r, base, GI == 0);
+        if (GI + 1 < NG) {
+            if (GI > 0) __syncthreads();
 #pragma unroll
-            for (int k = 0; k < RX; k++) lds[rx_lds_index(rx_row(r, k, b), lane, ps.log_c)] = v[k];
+            for (int k = 0; k < RX; k++) lds[rx_lds<LOGC>(row0 + ((uint32_t)k << B), lane)] = v[k];
+            rx_run<INV, R, LOGC, GI + 1>(v, lds, src, dst, tower, ps, hi, r, lane, base);
         } else {
+            const uint32_t g0 = base | (row0 << ps.s_lo) | lane;
 #pragma unroll
             for (int k = 0; k < RX; k++) {
-                uint32_t idx = base | (rx_row(r, k, b) << ps.s_lo) | lane;
                 uint32_t x = v[k];
                 if (ps.scale != 1) x = mmul(x, ps.scale);
-                dst[idx] = x;
+                dst[g0 + ((uint32_t)k << (B + ps.s_lo))] = x;
             }
         }
     }
+}
+
+template <bool INV, int R, int LOGC>
+__global__ __launch_bounds__(512) void fft_rx_kernel(const uint32_t* const* __restrict__ src_cols,
+                                                     uint32_t* const* __restrict__ dst_cols,
+                                                     const uint32_t* __restrict__ tower, RxPass ps) {
+    extern __shared__ uint32_t lds[];
+    constexpr int TILE_LOG = R + LOGC;
+    const uint32_t per_col_log = ps.m - TILE_LOG;
+    const uint32_t col = blockIdx.x >> per_col_log;
+    const uint32_t bb = blockIdx.x & ((1u << per_col_log) - 1);
+    const uint32_t lo_blocks_log = ps.s_lo - LOGC;
+    const uint32_t lo_blk = bb & ((1u << lo_blocks_log) - 1);
+    const uint32_t hi = bb >> lo_blocks_log;
+    const uint32_t base = (hi << (ps.s_lo + R)) | (lo_blk << LOGC);
+    const uint32_t t = threadIdx.x;
+    const uint32_t lane = t & ((1u << LOGC) - 1);
+    const uint32_t r = t >> LOGC;
+    uint32_t v[RX];
+    rx_run<INV, R, LOGC, 0>(v, lds, src_cols[col], dst_cols[col], tower, ps, hi, r, lane, base);
 }
 
 // m <= 2: closed forms (transforms.zig:155-179,256-292), one thread per column.
@@ -332,21 +349,35 @@ static void run_pass(Ctx& c, bool inverse, const uint32_t* const* src, uint32_t*
     SR_HIP(hipGetLastError());
 }
 
+template <bool INV>
+static void rx_dispatch(uint32_t R, uint32_t logc, dim3 grid, dim3 block, size_t lds, hipStream_t st,
+                        const uint32_t* const* src, uint32_t* const* dst, const uint32_t* tower, const RxPass& ps) {
+#define SR_RX(RR, CC)                                                                                       \
+    if (R == RR && logc == CC) {                                                                            \
+        hipLaunchKernelGGL((fft_rx_kernel<INV, RR, CC>), grid, block, lds, st, src, dst, tower, ps);        \
+        return;                                                                                             \
+    }
+    SR_RX(6, 0) SR_RX(7, 0) SR_RX(8, 0) SR_RX(9, 0) SR_RX(10, 0) SR_RX(11, 0) SR_RX(12, 0) SR_RX(13, 0)
+    SR_RX(4, 9) SR_RX(5, 8) SR_RX(6, 7) SR_RX(7, 6) SR_RX(8, 5)
+#undef SR_RX
+    throw Error(1, "fft: no kernel for pass shape R=" + std::to_string(R) + " logc=" + std::to_string(logc));
+}
+
 static void run_rx_pass(Ctx& c, bool inverse, const uint32_t* const* src, uint32_t* const* dst, uint32_t n_cols,
-                        const RxPass& ps) {
-    uint32_t tile_log = ps.R + ps.log_c;
+                        uint32_t R, uint32_t logc, const RxPass& ps) {
+    uint32_t tile_log = R + logc;
     uint32_t nt = 1u << (tile_log - RX_LOG);
-    size_t lds = ps.log_c == 0 ? ((size_t)(1u << tile_log) + ((1u << tile_log) >> 4)) * 4 : ((size_t)4 << tile_log);
+    size_t lds = logc == 0 ? ((size_t)(1u << tile_log) + ((1u << tile_log) >> 4)) * 4 : ((size_t)4 << tile_log);
     uint64_t blocks = (uint64_t)n_cols << (ps.m - tile_log);
     SR_CHECK(blocks < (1ull << 31), "fft grid too large");
-    SR_CHECK(ps.R >= (uint32_t)RX_LOG && nt <= 512 && ps.log_c <= ps.s_lo, "fft pass shape");
+    SR_CHECK(R >= (uint32_t)RX_LOG && nt <= 512 && logc <= ps.s_lo && tile_log <= ps.m, "fft pass shape");
     const uint32_t* tower = inverse ? c.d_itw : c.d_tw;
     KTimer kt(c, inverse ? "fft_ifft_pass" : "fft_lde_pass",
               4.0 * n_cols * ((double)(1ull << (ps.src_log < ps.m ? ps.src_log : ps.m)) + (double)(1ull << ps.m)));
     if (inverse)
-        hipLaunchKernelGGL(fft_rx_kernel<true>, dim3((uint32_t)blocks), dim3(nt), lds, c.stream, src, dst, tower, ps);
+        rx_dispatch<true>(R, logc, dim3((uint32_t)blocks), dim3(nt), lds, c.stream, src, dst, tower, ps);
     else
-        hipLaunchKernelGGL(fft_rx_kernel<false>, dim3((uint32_t)blocks), dim3(nt), lds, c.stream, src, dst, tower, ps);
+        rx_dispatch<false>(R, logc, dim3((uint32_t)blocks), dim3(nt), lds, c.stream, src, dst, tower, ps);
     SR_HIP(hipGetLastError());
 }
 
@@ -400,28 +431,35 @@ void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint3
             rem -= rr;
         }
     }
+    auto logc_of = [&](std::pair<uint32_t, uint32_t> p) {
+        return p.first == 0 ? 0u : std::min<uint32_t>(TILE_LOG - p.second, p.first);
+    };
     auto mk = [&](std::pair<uint32_t, uint32_t> p, bool last) {
         RxPass ps;
         ps.m = m;
         ps.s_lo = p.first;
-        ps.R = p.second;
-        ps.log_c = p.first == 0 ? 0 : std::min<uint32_t>(TILE_LOG - p.second, p.first);
         ps.tower_log = c.tower_log;
         ps.scale = (inverse && last) ? n_inv : 1u;
+        ps.dup_top = 0;
         return ps;
     };
     size_t np = passes.size();
     if (!inverse) {
         for (size_t i = 0; i < np; i++) {
-            RxPass ps = mk(passes[np - 1 - i], false);
+            auto pp = passes[np - 1 - i];
+            RxPass ps = mk(pp, false);
             ps.src_log = i == 0 ? src_log : m;
-            run_rx_pass(c, false, i == 0 ? d_src : (const uint32_t* const*)d_dst, d_dst, n_cols, ps);
+            // The top stage pairs row i with the zero-padded row i + 2^(m-1).
+            ps.dup_top = (i == 0 && src_log < m) ? 1u : 0u;
+            run_rx_pass(c, false, i == 0 ? d_src : (const uint32_t* const*)d_dst, d_dst, n_cols, pp.second,
+                        logc_of(pp), ps);
         }
     } else {
         for (size_t i = 0; i < np; i++) {
             RxPass ps = mk(passes[i], i + 1 == np);
             ps.src_log = m;
-            run_rx_pass(c, true, i == 0 ? d_src : (const uint32_t* const*)d_dst, d_dst, n_cols, ps);
+            run_rx_pass(c, true, i == 0 ? d_src : (const uint32_t* const*)d_dst, d_dst, n_cols, passes[i].second,
+                        logc_of(passes[i]), ps);
         }
     }
 }

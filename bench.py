@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--sequence-len", type=int, default=100)
     ap.add_argument("--protocol", choices=["functional", "secure"], default="functional")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain streaming kernel of known size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -49,6 +51,9 @@ def main():
 
     import stwo_rocm
 
+    if args.copy_probe:
+        copy_probe(stwo_rocm)
+        return
     n, W = args.log_n_rows, args.sequence_len
     proto = dict(stwo_rocm.PROTOCOLS[args.protocol])
     ctx = stwo_rocm.Context(0)
@@ -83,6 +88,7 @@ def main():
         raise SystemExit("proof differs from the reference golden proof: %s" % sha)
 
     median_ms = statistics.median(times)
+    traffic = profiled_traffic()
     # Dominant kernel class by device time over the timed proofs.
     dom, dv = max(prof.items(), key=lambda kv: kv[1]["ms"])
     avg_ms = dv["ms"] / dv["launches"]
@@ -116,7 +122,7 @@ def main():
                                                                proto["n_queries"], proto["log_blowup_factor"]),
                    "log_n_rows": n, "sequence_len": W, "protocol": args.protocol, "device": ident},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
                      "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu,
         "proof_sha256": sha,
@@ -131,6 +137,33 @@ def main():
     if cpu:
         line["speedup_vs_cpu_baseline"] = round(cpu["value"] / median_ms, 1)
     print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def profiled_traffic():
+    """HBM bytes per launch per kernel class, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summaries (profiles/*.md lines `traffic_bytes_per_launch[<class>]: <bytes>`)."""
+    import glob
+    import re
+
+    out = {}
+    for path in glob.glob(os.path.join(ROOT, "profiles", "*.md")):
+        for line in open(path):
+            m = re.match(r"traffic_bytes_per_launch\[(.+)\]: (\d+)", line.strip())
+            if m:
+                out[m.group(1)] = int(m.group(2))
+    return out
+
+
+def copy_probe(stwo_rocm):
+    """acc[i] += src[i] over 2^26 words: reads 512 MiB, writes 256 MiB per launch (5 launches)."""
+    ctx = stwo_rocm.Context(0)
+    n = 1 << 26
+    a, b = ctx.alloc(4 * n), ctx.alloc(4 * n)
+    for _ in range(5):
+        ctx.accumulate(a.ptr, b.ptr, n)
+    ctx.synchronize()
+    print(json.dumps({"copy_probe": {"launches": 5, "read_bytes": 8 * n, "write_bytes": 4 * n}}), flush=True)
     ctx.close()
 
 

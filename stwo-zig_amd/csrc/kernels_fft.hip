@@ -372,7 +372,9 @@ static void run_rx_pass(Ctx& c, bool inverse, const uint32_t* const* src, uint32
     SR_CHECK(blocks < (1ull << 31), "fft grid too large");
     SR_CHECK(R >= (uint32_t)RX_LOG && nt <= 512 && logc <= ps.s_lo && tile_log <= ps.m, "fft pass shape");
     const uint32_t* tower = inverse ? c.d_itw : c.d_tw;
-    KTimer kt(c, inverse ? "fft_ifft_pass" : "fft_lde_pass",
+    KTimer kt(c,
+              std::string(inverse ? "fft_ifft_pass" : "fft_lde_pass") + " m=" + std::to_string(ps.m) +
+                  " R=" + std::to_string(R) + " C=" + std::to_string(1u << logc) + " cols=" + std::to_string(n_cols),
               4.0 * n_cols * ((double)(1ull << (ps.src_log < ps.m ? ps.src_log : ps.m)) + (double)(1ull << ps.m)));
     if (inverse)
         rx_dispatch<true>(R, logc, dim3((uint32_t)blocks), dim3(nt), lds, c.stream, src, dst, tower, ps);

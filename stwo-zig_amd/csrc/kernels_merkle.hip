@@ -142,7 +142,8 @@ void launch_merkle_parent_layer(Ctx& c, const uint8_t* d_children, uint32_t log_
 void launch_merkle_leaves(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_logs, uint32_t n_cols,
                           uint32_t max_log, uint8_t* d_leaves) {
     uint64_t n = 1ull << max_log;
-    KTimer kt(c, "merkle_leaves", (4.0 * n_cols + 32.0) * (double)n);
+    KTimer kt(c, "merkle_leaves L=" + std::to_string(max_log) + " cols=" + std::to_string(n_cols),
+              (4.0 * n_cols + 32.0) * (double)n);
     hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)), dim3(MK_THREADS), 0,
                        c.stream, d_cols, d_logs, n_cols, max_log, c.leaf_seed, d_leaves);
     SR_HIP(hipGetLastError());

@@ -76,7 +76,7 @@ struct Ctx {
     // Opt-in per-kernel profile: events around each launch on `stream`,
     // aggregated by kernel class with the launch's algorithmic HBM bytes.
     struct KRec {
-        const char* cls;
+        std::string cls;
         hipEvent_t a, b;
         double bytes;
     };
@@ -104,10 +104,10 @@ struct Ctx {
 // Records one kernel launch in the context profile (no-op unless profiling).
 struct KTimer {
     Ctx& c;
-    const char* cls;
+    std::string cls;
     double bytes;
     hipEvent_t a = nullptr;
-    KTimer(Ctx& cc, const char* k, double b) : c(cc), cls(k), bytes(b) {
+    KTimer(Ctx& cc, std::string k, double b) : c(cc), cls(std::move(k)), bytes(b) {
         if (c.profiling) {
             a = c.pool_event();
             (void)hipEventRecord(a, c.stream);

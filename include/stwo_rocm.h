@@ -190,8 +190,11 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
  * on the host).  While enabled, every kernel launch of the context is bracketed by HIP
  * events on the context stream.  profile_read synchronises, returns a JSON object
  * {class: {launches, ms, max_ms, bytes}} (bytes = algorithmic HBM bytes of the launches)
- * and clears the records. */
+ * and clears the records.  profile_select restricts the records to one kernel class
+ * (NULL or "" = every class), so a timed run can carry events around its dominant
+ * kernel only. */
 int stwo_rocm_profile_enable(stwo_rocm_ctx* ctx, int enable);
+int stwo_rocm_profile_select(stwo_rocm_ctx* ctx, const char* kernel_class);
 int stwo_rocm_profile_read(stwo_rocm_ctx* ctx, char* json_out, size_t cap, size_t* len);
 
 #ifdef __cplusplus

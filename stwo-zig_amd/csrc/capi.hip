@@ -598,6 +598,10 @@ int stwo_rocm_profile_enable(stwo_rocm_ctx* ctx, int enable) {
     });
 }
 
+int stwo_rocm_profile_select(stwo_rocm_ctx* ctx, const char* kernel_class) {
+    return guarded(ctx, [&](Ctx& c) { c.profile_only = kernel_class ? kernel_class : ""; });
+}
+
 int stwo_rocm_profile_read(stwo_rocm_ctx* ctx, char* json_out, size_t cap, size_t* len) {
     return guarded(ctx, [&](Ctx& c) {
         SR_CHECK(len, "null argument");

@@ -12,7 +12,7 @@
 // launch builds up to 9 levels: each block hashes 256 parents from HBM and
 // then halves them in LDS, writing every level, so a 2^21-leaf tree takes three
 // launches instead of 21.
-#include "runtime.hpp"
+#include "chan.hpp"
 
 namespace sr {
 
@@ -69,49 +69,113 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_leaves_qm31_kernel(const ui
     store_hash(leaves, p, h);
 }
 
-// Builds `levels` parent levels below layer k.  Block b covers parents
-// [b*P0, (b+1)*P0) of level k-1 with P0 = 2^(levels-1).
-__global__ __launch_bounds__(MK_THREADS) void merkle_parents_kernel(uint8_t* __restrict__ tree, uint32_t k,
-                                                                    uint32_t levels, B2State seed) {
-    __shared__ uint32_t sh[MK_THREADS * 8];
-    const uint32_t P0 = 1u << (levels - 1);
+// Builds `levels` (2..10) parent levels below layer k.  Block b covers the
+// 2^levels * ... children [b*4T, (b+1)*4T) of layer k for T = 2^(levels-2)
+// threads: every thread hashes two parents of level k-1 from four children
+// (two independent compressions) and their parent of level k-2 in registers,
+// so the first two levels keep every lane busy; the remaining levels halve in
+// LDS.  levels == 1 uses one thread per parent.
+constexpr int MP_THREADS = 256;
+__device__ __forceinline__ void node_hash(const B2State& seed, const uint32_t* l, const uint32_t* r, uint32_t h[8]) {
+    uint32_t m[16];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        m[i] = l[i];
+        m[8 + i] = r[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) h[i] = seed.h[i];
+    b2_compress(h, m, 128, true);
+}
+
+// When the launch reaches the root (k == levels, one block) and `ch` is set,
+// the thread holding the root also runs the channel step on it.
+__global__ __launch_bounds__(MP_THREADS) void merkle_parents_kernel(uint8_t* __restrict__ tree, uint32_t k,
+                                                                    uint32_t levels, B2State seed,
+                                                                    DevChannel* __restrict__ ch,
+                                                                    uint32_t* __restrict__ root_out) {
+    __shared__ uint32_t sh[MP_THREADS * 8];
     const uint32_t t = threadIdx.x;
     const uint8_t* child = tree + (((uint64_t)1 << k) - 1) * 32;
-    uint32_t count = P0;
-    uint32_t level = k - 1;
-    uint64_t first = (uint64_t)blockIdx.x * P0;
-    // level k-1 from HBM children
-    if (t < count) {
-        const uint4* c4 = (const uint4*)(child + (first + t) * 64);
+    const bool chan_here = ch != nullptr && k == levels;
+    if (levels == 1) {
+        uint64_t i = (uint64_t)blockIdx.x * MP_THREADS + t;
+        if (i >= (1ull << (k - 1))) return;
+        const uint4* c4 = (const uint4*)(child + i * 64);
         uint4 a = c4[0], b = c4[1], c = c4[2], d = c4[3];
-        uint32_t m[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
+        uint32_t l[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}, r[8] = {c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
         uint32_t h[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) h[i] = seed.h[i];
-        b2_compress(h, m, 128, true);
-        store_hash(tree + (((uint64_t)1 << level) - 1) * 32, first + t, h);
-#pragma unroll
-        for (int i = 0; i < 8; i++) sh[t * 8 + i] = h[i];
+        node_hash(seed, l, r, h);
+        store_hash(tree + (((uint64_t)1 << (k - 1)) - 1) * 32, i, h);
+        if (chan_here && i == 0) chan_mix_root_draw_thread(ch, h, root_out);
+        return;
     }
-    for (uint32_t lv = 1; lv < levels; lv++) {
+    const uint32_t T = 1u << (levels - 2);  // threads with work (<= 256)
+    uint64_t g = (uint64_t)blockIdx.x * T + t;  // index at level k-2
+    uint32_t h2[8];
+    if (t < T) {
+        const uint4* c4 = (const uint4*)(child + g * 128);
+        uint32_t c[32];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            uint4 v = c4[q];
+            c[4 * q] = v.x;
+            c[4 * q + 1] = v.y;
+            c[4 * q + 2] = v.z;
+            c[4 * q + 3] = v.w;
+        }
+        uint32_t p0[8], p1[8];
+        node_hash(seed, c, c + 8, p0);
+        node_hash(seed, c + 16, c + 24, p1);
+        uint8_t* l1 = tree + (((uint64_t)1 << (k - 1)) - 1) * 32;
+        store_hash(l1, 2 * g, p0);
+        store_hash(l1, 2 * g + 1, p1);
+        node_hash(seed, p0, p1, h2);
+        store_hash(tree + (((uint64_t)1 << (k - 2)) - 1) * 32, g, h2);
+#pragma unroll
+        for (int i = 0; i < 8; i++) sh[t * 8 + i] = h2[i];
+        if (chan_here && levels == 2 && t == 0) chan_mix_root_draw_thread(ch, h2, root_out);
+    }
+    uint32_t count = T, level = k - 2;
+    uint64_t first = (uint64_t)blockIdx.x * T;
+    const uint32_t q = t & 3, j = t >> 2;
+    const QuadLane L(q);
+    for (uint32_t lv = 2; lv < levels; lv++) {
         __syncthreads();
         count >>= 1;
         level--;
         first >>= 1;
-        uint32_t h[8];
-        if (t < count) {
-            uint32_t m[16];
+        uint8_t* out = tree + (((uint64_t)1 << level) - 1) * 32;
+        if (count > MP_THREADS / 4) {
+            uint32_t h[8];
+            if (t < count) node_hash(seed, &sh[t * 16], &sh[t * 16 + 8], h);
+            __syncthreads();
+            if (t < count) {
 #pragma unroll
-            for (int i = 0; i < 16; i++) m[i] = sh[t * 16 + i];
+                for (int i = 0; i < 8; i++) sh[t * 8 + i] = h[i];
+                store_hash(out, first + t, h);
+            }
+        } else {
+            // short levels: one compression per quad
+            uint32_t hlo = 0, hhi = 0;
+            const bool act = j < count;
+            if (act) {
+                uint32_t m[16];
 #pragma unroll
-            for (int i = 0; i < 8; i++) h[i] = seed.h[i];
-            b2_compress(h, m, 128, true);
-            store_hash(tree + (((uint64_t)1 << level) - 1) * 32, first + t, h);
-        }
-        __syncthreads();
-        if (t < count) {
-#pragma unroll
-            for (int i = 0; i < 8; i++) sh[t * 8 + i] = h[i];
+                for (int i = 0; i < 16; i++) m[i] = sh[j * 16 + i];
+                hlo = L.sel(seed.h[0], seed.h[1], seed.h[2], seed.h[3]);
+                hhi = L.sel(seed.h[4], seed.h[5], seed.h[6], seed.h[7]);
+                b2_compress_quad(L, hlo, hhi, m, 128, true);
+            }
+            __syncthreads();
+            if (act) {
+                sh[j * 8 + q] = hlo;
+                sh[j * 8 + 4 + q] = hhi;
+                uint32_t* o = (uint32_t*)(out + (first + j) * 32);
+                o[q] = hlo;
+                o[4 + q] = hhi;
+            }
+            if (chan_here && lv + 1 == levels && j == 0) chan_mix_root_draw_quad(ch, hlo, hhi, root_out);
         }
     }
 }
@@ -157,14 +221,16 @@ void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size
     SR_HIP(hipGetLastError());
 }
 
-void launch_merkle_parents(Ctx& c, DeviceTree& t) {
+void launch_merkle_parents(Ctx& c, DeviceTree& t, DevChannel* d_ch, uint8_t* d_root_out) {
     uint32_t k = t.max_log;
+    if (d_ch && k == 0) launch_chan_mix_root_draw(c, d_ch, t.layers, d_root_out);
     while (k > 0) {
-        uint32_t levels = k < 9 ? k : 9;
-        uint32_t blocks = 1u << (k - levels);
+        uint32_t levels = k < 10 ? k : 10;
+        uint32_t blocks = levels == 1 ? (uint32_t)(((1ull << (k - 1)) + MP_THREADS - 1) / MP_THREADS)
+                                      : (1u << (k - levels));
         KTimer kt(c, "merkle_parents", 32.0 * (double)((1ull << k) + (1ull << k) - (1ull << (k - levels))));
-        hipLaunchKernelGGL(merkle_parents_kernel, dim3(blocks), dim3(MK_THREADS), 0, c.stream, t.layers, k, levels,
-                           c.node_seed);
+        hipLaunchKernelGGL(merkle_parents_kernel, dim3(blocks), dim3(MP_THREADS), 0, c.stream, t.layers, k, levels,
+                           c.node_seed, d_ch, (uint32_t*)d_root_out);
         SR_HIP(hipGetLastError());
         k -= levels;
     }

@@ -443,6 +443,26 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     uint32_t* qc[4];
     for (int k = 0; k < 4; k++) qc[k] = quot.as<uint32_t>() + k * LN;
     DeviceTree* ftree0 = owner.make(L);
+
+    // FRI (prover/fri.zig:377-621).  The commit cascade runs without host round
+    // trips: the launch that produces each layer's root mixes it into a device
+    // copy of the channel and draws the next alpha, which the fold reads from
+    // HBM.  The host channel resumes from the device state after the single
+    // readback below.
+    const uint32_t last_log = prm.log_last_layer_degree_bound + B;
+    uint32_t n_inner = L - 1 - last_log;  // inner layers log L-1 .. last_log+1
+    constexpr size_t CH_BYTES = 128;  // DevChannel slot ahead of the roots
+    static_assert(sizeof(DevChannel) <= CH_BYTES, "DevChannel slot");
+    const size_t fbytes = CH_BYTES + 32 * (1 + (size_t)n_inner);
+    DBuf fst(&c, fbytes);  // DevChannel | roots (first layer, inner layers)
+    DevChannel* d_ch = fst.as<DevChannel>();
+    uint8_t* d_roots = fst.as<uint8_t>() + CH_BYTES;
+    {
+        DevChannel hc{};
+        memcpy(hc.digest, ch.digest, 32);
+        hc.n_draws = ch.n_draws;
+        SR_HIP(hipMemcpyAsync(d_ch, c.stage(&hc, sizeof(hc)), sizeof(hc), hipMemcpyHostToDevice, c.stream));
+    }
     {
         DBuf d_cols = upload(c, all_lde.data(), NS);
         DBuf d_logs = upload(c, all_lde_logs.data(), NS);
@@ -454,48 +474,69 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         launch_quotients(c, d_cols.as<const uint32_t* const>(), d_logs.as<uint32_t>(), d_ec.as<uint32_t>(),
                          d_ecf.as<uint32_t>(), d_b.as<QuotientBatchDev>(), (uint32_t)batches.size(), L, qc,
                          ftree0->layer(L));
-        launch_merkle_parents(c, *ftree0);
+        launch_merkle_parents(c, *ftree0, d_ch, d_roots);
     }
-    tree_read_root(c, *ftree0);
-    ch.mix_root(ftree0->root);
     mark();  // quotients + first layer commit
 
-    // FRI (prover/fri.zig:377-621).
-    const uint32_t last_log = prm.log_last_layer_degree_bound + B;
     std::vector<DBuf> lines;        // inner layer evaluations (QM31 AoS)
     std::vector<DeviceTree*> ftrees;  // inner layer trees
-    qm31 fa = ch.draw_secure_felt();
     uint32_t llog = L - 1;
     lines.emplace_back(&c, (1ull << llog) * 16);
     DeviceTree* next_tree = llog > last_log ? owner.make(llog) : nullptr;
     {
         const uint32_t* cc[4] = {qc[0], qc[1], qc[2], qc[3]};
-        launch_fold_circle_into_line(c, cc, L, fa, lines.back().as<uint32_t>(), false,
-                                     next_tree ? next_tree->layer(llog) : nullptr);
+        launch_fold_circle_into_line(c, cc, L, qzero(), lines.back().as<uint32_t>(), false,
+                                     next_tree ? next_tree->layer(llog) : nullptr, d_ch);
     }
     while (llog > last_log) {
+        if (llog <= FRI_TAIL_MAX_LOG) {
+            // Remaining layers in one workgroup.
+            FriTailArgs ta{};
+            ta.top_log = llog;
+            ta.last_log = last_log;
+            ta.ch = d_ch;
+            for (uint32_t l = llog;; l--) {
+                if (l < llog) lines.emplace_back(&c, (1ull << l) * 16);
+                ta.lines[l] = lines.back().as<uint4>();
+                if (l == last_log) break;
+                DeviceTree* t = l == llog ? next_tree : owner.make(l);
+                ta.trees[l] = t->layers;
+                ta.roots[l] = (uint32_t*)(d_roots + 32 * (1 + ftrees.size()));
+                ftrees.push_back(t);
+            }
+            launch_fri_tail(c, ta);
+            llog = last_log;
+            break;
+        }
         DeviceTree* t = next_tree;
-        launch_merkle_parents(c, *t);
-        tree_read_root(c, *t);
+        launch_merkle_parents(c, *t, d_ch, d_roots + 32 * (1 + ftrees.size()));
         ftrees.push_back(t);
-        ch.mix_root(t->root);
-        qm31 la = ch.draw_secure_felt();
         uint32_t nlog = llog - 1;
         next_tree = nlog > last_log ? owner.make(nlog) : nullptr;
         DBuf nxt(&c, (1ull << nlog) * 16);
-        launch_fold_line(c, lines.back().as<uint32_t>(), llog, la, nxt.as<uint32_t>(),
-                         next_tree ? next_tree->layer(nlog) : nullptr);
+        launch_fold_line(c, lines.back().as<uint32_t>(), llog, qzero(), nxt.as<uint32_t>(),
+                         next_tree ? next_tree->layer(nlog) : nullptr, d_ch);
         lines.push_back(std::move(nxt));
         llog = nlog;
     }
+    SR_CHECK(ftrees.size() == n_inner, "FRI layer count");
     // Last layer (fri.zig:594-621; line_evaluation.zig:97-151; line.zig:134-142).
     const uint64_t lsz = 1ull << llog;
     std::vector<qm31> lv(lsz);
     {
-        uint32_t* h = (uint32_t*)c.pinned(lsz * 16);
-        SR_HIP(hipMemcpyAsync(h, lines.back().p, lsz * 16, hipMemcpyDeviceToHost, c.stream));
+        uint8_t* h = c.pinned(fbytes + lsz * 16);
+        SR_HIP(hipMemcpyAsync(h, fst.p, fbytes, hipMemcpyDeviceToHost, c.stream));
+        SR_HIP(hipMemcpyAsync(h + fbytes, lines.back().p, lsz * 16, hipMemcpyDeviceToHost, c.stream));
         c.sync();
-        for (uint64_t i = 0; i < lsz; i++) lv[i] = qload(h + 4 * i);
+        DevChannel hc;
+        memcpy(&hc, h, sizeof(hc));
+        SR_CHECK(hc.status == 0, "FRI: channel draw rejected 64 times in a row");
+        memcpy(ch.digest, hc.digest, 32);
+        ch.n_draws = hc.n_draws;
+        memcpy(ftree0->root, h + CH_BYTES, 32);
+        for (size_t k = 0; k < ftrees.size(); k++) memcpy(ftrees[k]->root, h + CH_BYTES + 32 * (1 + k), 32);
+        const uint32_t* w = (const uint32_t*)(h + fbytes);
+        for (uint64_t i = 0; i < lsz; i++) lv[i] = qload(w + 4 * i);
     }
     auto bitrev_vec = [](std::vector<qm31>& v) {
         uint32_t lg = 0;

@@ -44,6 +44,15 @@ struct DeviceTree {
     uint8_t* layer(uint32_t k) const { return layers + (((uint64_t)1 << k) - 1) * 32; }
 };
 
+// Fiat-Shamir channel state kept in HBM during the FRI commit loop.
+struct DevChannel {
+    uint32_t digest[8];
+    uint32_t n_draws;
+    uint32_t status;     // non-zero: draw rejection cap reached
+    uint32_t alpha[4];   // last drawn secure felt
+    uint32_t alpha2[4];  // its square
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -81,6 +90,7 @@ struct Ctx {
         double bytes;
     };
     bool profiling = false;
+    std::string profile_only;  // non-empty: record only launches of this class
     double kbytes_hint = 0;  // input bytes of the next quotient launch (set by its caller)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -108,13 +118,13 @@ struct KTimer {
     double bytes;
     hipEvent_t a = nullptr;
     KTimer(Ctx& cc, std::string k, double b) : c(cc), cls(std::move(k)), bytes(b) {
-        if (c.profiling) {
+        if (c.profiling && (c.profile_only.empty() || c.profile_only == cls)) {
             a = c.pool_event();
             (void)hipEventRecord(a, c.stream);
         }
     }
     ~KTimer() {
-        if (c.profiling && a) {
+        if (a) {
             hipEvent_t e = c.pool_event();
             (void)hipEventRecord(e, c.stream);
             c.krecs.push_back({cls, a, e, bytes});
@@ -161,14 +171,34 @@ void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint3
 void launch_merkle_leaves(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_logs, uint32_t n_cols,
                           uint32_t max_log, uint8_t* d_leaves);
 void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size, uint8_t* d_leaves);
-void launch_merkle_parents(Ctx& c, DeviceTree& t);
+// All parent layers above t's leaves.  With d_ch, the launch reaching the root
+// also mixes it into the device channel and draws the next alpha (root copied to d_root_out).
+void launch_merkle_parents(Ctx& c, DeviceTree& t, DevChannel* d_ch = nullptr, uint8_t* d_root_out = nullptr);
 void launch_merkle_parent_layer(Ctx& c, const uint8_t* d_children, uint32_t log_size, uint8_t* d_parents);
 // FRI folds (folding.zig): line domain halfOdds(log_src) / circle domain CanonicCoset(log_src).
+// d_alpha (optional): read alpha (and alpha^2) from a DevChannel in HBM instead of `alpha`.
 void launch_fold_line(Ctx& c, const uint32_t* d_src, uint32_t log_src, qm31 alpha, uint32_t* d_dst,
-                      uint8_t* d_leaf_out);
+                      uint8_t* d_leaf_out, const DevChannel* d_alpha = nullptr);
 // `coords` is a host array of 4 device pointers.
 void launch_fold_circle_into_line(Ctx& c, const uint32_t* const* coords, uint32_t log_src, qm31 alpha,
-                                  uint32_t* d_dst, bool accumulate, uint8_t* d_leaf_out);
+                                  uint32_t* d_dst, bool accumulate, uint8_t* d_leaf_out,
+                                  const DevChannel* d_alpha = nullptr);
+// digest <- H(digest || root); alpha <- drawSecureFelt; root copied to d_root_out (optional).
+void launch_chan_mix_root_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, uint8_t* d_root_out);
+// FRI inner layers log top_log .. last_log+1 in one workgroup.  In: line top_log
+// and its tree's leaf layer.  Out: every parent layer, root (roots[l]), line l-1
+// and its leaves for each layer l, the device channel advanced per layer.
+constexpr uint32_t FRI_TAIL_MAX_LOG = 10;
+struct FriTailArgs {
+    uint32_t top_log, last_log;
+    uint8_t* trees[FRI_TAIL_MAX_LOG + 1];  // DeviceTree::layers per log (last_log+1 .. top_log)
+    uint4* lines[FRI_TAIL_MAX_LOG + 1];    // line evaluations per log (last_log .. top_log)
+    uint32_t* roots[FRI_TAIL_MAX_LOG + 1]; // root copies per log
+    DevChannel* ch;
+    const uint32_t* itw_half;  // inverse tower + 2^(M-1) (set by the launcher)
+    B2State leaf_seed, node_seed;  // set by the launcher
+};
+void launch_fri_tail(Ctx& c, FriTailArgs& a);
 // Quotients (row_evaluation.zig).
 struct QuotientBatchDev {
     cm31 det, pix, piy;  // sample point components

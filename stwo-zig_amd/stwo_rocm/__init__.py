@@ -118,6 +118,7 @@ _SIGNATURES = [
 
 _SIGNATURES += [
     ("stwo_rocm_profile_enable", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("stwo_rocm_profile_select", ctypes.c_int, [_vp, ctypes.c_char_p]),
     ("stwo_rocm_profile_read", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t,
                                                ctypes.POINTER(ctypes.c_size_t)]),
 ]
@@ -268,6 +269,10 @@ class Context:
     # ---- per-kernel device-time profile (HIP events on the context stream)
     def profile_enable(self, enable=True):
         self._check(self.lib.stwo_rocm_profile_enable(self.h, 1 if enable else 0))
+
+    def profile_select(self, kernel_class=None):
+        """Record only launches of `kernel_class` (None: every class)."""
+        self._check(self.lib.stwo_rocm_profile_select(self.h, kernel_class.encode() if kernel_class else None))
 
     def profile_read(self):
         import json

@@ -84,7 +84,8 @@ def test_merkle_mixed_sizes_matches_oracle(ctx):
         b.free()
 
 
-@pytest.mark.parametrize("n_cols,log", [(0, 0), (1, 1), (16, 8), (17, 8), (100, 14), (8, 21)])
+@pytest.mark.parametrize("n_cols,log", [(0, 0), (1, 1), (1, 2), (3, 3), (5, 6), (16, 8), (17, 8), (2, 10), (2, 11),
+                                        (100, 14), (8, 21)])
 def test_merkle_uniform_matches_oracle(ctx, n_cols, log):
     rng = np.random.default_rng(n_cols * 31 + log)
     cols = rand_cols(rng, n_cols, log)

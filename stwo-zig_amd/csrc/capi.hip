@@ -473,14 +473,8 @@ int stwo_rocm_compute_quotients(stwo_rocm_ctx* ctx, uint32_t L, const uint32_t* 
                 t->max_log = L;
                 t->layers = (uint8_t*)c.alloc(((2ull << L) - 1) * 32);
             }
-            DBuf dc = upload(c, columns, n_columns), dl = upload(c, column_log_sizes, n_columns);
-            DBuf de = upload(c, ecols.data(), ecols.size()), df = upload(c, ecf.data(), ecf.size());
-            DBuf db = upload(c, batches.data(), batches.size());
-            c.kbytes_hint = 0;
-            for (uint32_t i = 0; i < n_columns; i++) c.kbytes_hint += 4.0 * (double)(1ull << column_log_sizes[i]);
-            launch_quotients(c, dc.as<const uint32_t* const>(), dl.as<uint32_t>(), de.as<uint32_t>(),
-                             df.as<uint32_t>(), db.as<QuotientBatchDev>(), (uint32_t)batches.size(), L, out_coords,
-                             t ? t->layer(L) : nullptr);
+            launch_quotients(c, columns, column_log_sizes, ecols.data(), ecf.data(), (uint32_t)ecols.size(),
+                             batches.data(), (uint32_t)batches.size(), L, out_coords, t ? t->layer(L) : nullptr);
             if (t) {
                 launch_merkle_parents(c, *t);
                 tree_read_root(c, *t);

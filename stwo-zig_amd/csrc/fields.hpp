@@ -11,6 +11,14 @@
 
 #define SR_HD __host__ __device__ __forceinline__
 
+// Global-address-space views of device pointers read from pointer tables, so
+// that loads and stores compile to global_* rather than flat_* instructions
+// (flat accesses also count against the LDS wait counter).
+#define SR_GLOBAL __attribute__((address_space(1)))
+typedef SR_GLOBAL uint32_t gu32;
+typedef SR_GLOBAL const uint32_t gcu32;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // 16-byte global accesses
+
 namespace sr {
 
 constexpr uint32_t P = 0x7fffffffu;

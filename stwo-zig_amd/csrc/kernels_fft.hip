@@ -151,9 +151,10 @@ __device__ __forceinline__ void rx_stage(uint32_t (&v)[RX], const uint32_t* __re
         // h = (hi << (R-LS-1)) | (r_high << (3-Q)) | hk  with r_high = r >> B
         const uint32_t tb = ((1u << (ps.tower_log - 1)) - (1u << (ps.m - s))) + ((hi << (R - LS - 1)) |
                                                                                  ((r >> B) << (3 - Q)));
+        const uint32_t* __restrict__ twp = tower + tb;  // constant offsets below
 #pragma unroll
         for (int hk = 0; hk < (RX >> (Q + 1)); hk++) {
-            const uint32_t tw = tower[tb + hk];
+            const uint32_t tw = twp[hk];
 #pragma unroll
             for (int lk = 0; lk < (1 << Q); lk++) {
                 const int k0 = (hk << (Q + 1)) | lk, k1 = k0 | (1 << Q);
@@ -226,8 +227,8 @@ struct RxGroupShape {
 };
 
 template <bool INV, int R, int LOGC, int GI>
-__device__ __forceinline__ void rx_run(uint32_t (&v)[RX], uint32_t* lds, const uint32_t* __restrict__ src,
-                                       uint32_t* __restrict__ dst, const uint32_t* __restrict__ tower,
+__device__ __forceinline__ void rx_run(uint32_t (&v)[RX], uint32_t* lds, gcu32* __restrict__ src,
+                                       gu32* __restrict__ dst, const uint32_t* __restrict__ tower,
                                        const RxPass& ps, uint32_t hi, uint32_t r, uint32_t lane, uint32_t base) {
     constexpr int NG = (R + RX_LOG - 1) / RX_LOG;
     if constexpr (GI < NG) {
@@ -237,10 +238,23 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], uint32_t* lds, const u
         if (GI == 0) {
             const uint32_t g0 = base | (row0 << ps.s_lo) | lane;
             const uint32_t src_n = 1u << ps.src_log;
+            if (B == 0 && LOGC == 0 && g0 + RX <= src_n) {
+                // the thread's 16 words are consecutive (g0 is a multiple of 16): 16-byte loads
+                const SR_GLOBAL u32x4* s4 = (const SR_GLOBAL u32x4*)(src + g0);
 #pragma unroll
-            for (int k = 0; k < RX; k++) {
-                const uint32_t idx = g0 + ((uint32_t)k << (B + ps.s_lo));
-                v[k] = idx < src_n ? src[idx] : 0u;
+                for (int k = 0; k < RX / 4; k++) {
+                    const u32x4 w = s4[k];
+                    v[4 * k] = w.x;
+                    v[4 * k + 1] = w.y;
+                    v[4 * k + 2] = w.z;
+                    v[4 * k + 3] = w.w;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < RX; k++) {
+                    const uint32_t idx = g0 + ((uint32_t)k << (B + ps.s_lo));
+                    v[k] = idx < src_n ? src[idx] : 0u;
+                }
             }
         } else {
             __syncthreads();
@@ -255,11 +269,17 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], uint32_t* lds, const u
             rx_run<INV, R, LOGC, GI + 1>(v, lds, src, dst, tower, ps, hi, r, lane, base);
         } else {
             const uint32_t g0 = base | (row0 << ps.s_lo) | lane;
+            if (ps.scale != 1) {
 #pragma unroll
-            for (int k = 0; k < RX; k++) {
-                uint32_t x = v[k];
-                if (ps.scale != 1) x = mmul(x, ps.scale);
-                dst[g0 + ((uint32_t)k << (B + ps.s_lo))] = x;
+                for (int k = 0; k < RX; k++) v[k] = mmul(v[k], ps.scale);
+            }
+            if constexpr (B == 0 && LOGC == 0) {
+                SR_GLOBAL u32x4* d4 = (SR_GLOBAL u32x4*)(dst + g0);
+#pragma unroll
+                for (int k = 0; k < RX / 4; k++) d4[k] = u32x4{v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]};
+            } else {
+#pragma unroll
+                for (int k = 0; k < RX; k++) dst[g0 + ((uint32_t)k << (B + ps.s_lo))] = v[k];
             }
         }
     }
@@ -282,7 +302,7 @@ __global__ __launch_bounds__(512) void fft_rx_kernel(const uint32_t* const* __re
     const uint32_t lane = t & ((1u << LOGC) - 1);
     const uint32_t r = t >> LOGC;
     uint32_t v[RX];
-    rx_run<INV, R, LOGC, 0>(v, lds, src_cols[col], dst_cols[col], tower, ps, hi, r, lane, base);
+    rx_run<INV, R, LOGC, 0>(v, lds, (gcu32*)src_cols[col], (gu32*)dst_cols[col], tower, ps, hi, r, lane, base);
 }
 
 // m <= 2: closed forms (transforms.zig:155-179,256-292), one thread per column.
@@ -371,6 +391,7 @@ static void run_rx_pass(Ctx& c, bool inverse, const uint32_t* const* src, uint32
     uint64_t blocks = (uint64_t)n_cols << (ps.m - tile_log);
     SR_CHECK(blocks < (1ull << 31), "fft grid too large");
     SR_CHECK(R >= (uint32_t)RX_LOG && nt <= 512 && logc <= ps.s_lo && tile_log <= ps.m, "fft pass shape");
+    SR_CHECK(logc != 0 || ps.s_lo == 0, "fft: contiguous pass must start at stage 0");
     const uint32_t* tower = inverse ? c.d_itw : c.d_tw;
     KTimer kt(c,
               std::string(inverse ? "fft_ifft_pass" : "fft_lde_pass") + " m=" + std::to_string(ps.m) +

@@ -9,9 +9,12 @@
 //
 // Parents (layers.zig:99-140; blake2_merkle.zig:112-154): node = Blake2s(
 // "node"+0^60 || left || right), one compression from the node seed.  One
-// launch builds up to 9 levels: each block hashes 256 parents from HBM and
-// then halves them in LDS, writing every level, so a 2^21-leaf tree takes three
-// launches instead of 21.
+// launch builds up to 10 levels: each thread hashes four children from HBM
+// into two parents and their parent in registers, then the block halves the
+// level in LDS, writing every level; levels of at most 64 nodes use one quad of
+// lanes per compression.  A 2^21-leaf tree takes three launches instead of 21.
+#include <vector>
+
 #include "chan.hpp"
 
 namespace sr {
@@ -24,32 +27,47 @@ __device__ __forceinline__ void store_hash(uint8_t* __restrict__ out, uint64_t i
     o[1] = make_uint4(h[4], h[5], h[6], h[7]);
 }
 
-__global__ __launch_bounds__(MK_THREADS) void merkle_leaves_kernel(const uint32_t* const* __restrict__ cols,
-                                                                   const uint32_t* __restrict__ logs, uint32_t n_cols,
-                                                                   uint32_t max_log, B2State seed,
+// Column descriptor for leaf hashing; the table is padded to a multiple of 16
+// entries with mask 0 (padding reads column 0 and contributes a zero word).
+struct LeafColDev {
+    const uint32_t* col;
+    uint32_t shift;  // lift shift max_log - log + 1 (1 = full size)
+    uint32_t mask;   // 0xffffffff, or 0 for padding
+};
+static_assert(sizeof(LeafColDev) == 16, "leaf column layout");
+
+__device__ __forceinline__ void leaf_block_load(const LeafColDev* __restrict__ cd, uint32_t p, uint32_t m[16]) {
+#pragma unroll
+    for (int w = 0; w < 16; w++) {
+        const LeafColDev d = cd[w];
+        m[w] = ((gcu32*)d.col)[((p >> d.shift) << 1) + (p & 1)] & d.mask;
+    }
+}
+
+// One leaf per lane; the 16 words of the next message block are loaded while
+// the current block is compressed.
+__global__ __launch_bounds__(MK_THREADS) void merkle_leaves_kernel(const LeafColDev* __restrict__ cols,
+                                                                   uint32_t n_cols, uint32_t max_log, B2State seed,
                                                                    uint8_t* __restrict__ leaves) {
-    uint64_t p = (uint64_t)blockIdx.x * MK_THREADS + threadIdx.x;
-    if (p >= (1ull << max_log)) return;
+    const uint64_t p64 = (uint64_t)blockIdx.x * MK_THREADS + threadIdx.x;
+    if (p64 >= (1ull << max_log)) return;
+    const uint32_t p = (uint32_t)p64;
     uint32_t h[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) h[i] = seed.h[i];
     const uint32_t nblocks = (n_cols + 15) >> 4;
+    uint32_t cur[16];
+    leaf_block_load(cols, p, cur);
     for (uint32_t blk = 0; blk < nblocks; blk++) {
-        uint32_t m[16];
+        uint32_t nxt[16];
+        const bool more = blk + 1 < nblocks;
+        if (more) leaf_block_load(cols + 16 * (blk + 1), p, nxt);
+        const uint32_t t = 64 + (more ? 64 * (blk + 1) : 4 * n_cols);
+        b2_compress(h, cur, t, !more);
+        if (more) {
 #pragma unroll
-        for (int w = 0; w < 16; w++) {
-            uint32_t c = blk * 16 + w;
-            uint32_t v = 0;
-            if (c < n_cols) {
-                uint32_t sh = max_log - logs[c] + 1;
-                uint64_t idx = ((p >> sh) << 1) + (p & 1);
-                v = cols[c][idx];
-            }
-            m[w] = v;
+            for (int w = 0; w < 16; w++) cur[w] = nxt[w];
         }
-        const bool last = blk + 1 == nblocks;
-        const uint32_t t = 64 + (last ? 4 * n_cols : 64 * (blk + 1));
-        b2_compress(h, m, t, last);
     }
     store_hash(leaves, p, h);
 }
@@ -90,6 +108,9 @@ __device__ __forceinline__ void node_hash(const B2State& seed, const uint32_t* l
 
 // When the launch reaches the root (k == levels, one block) and `ch` is set,
 // the thread holding the root also runs the channel step on it.
+// CHAN is a separate instantiation so that the channel code's registers do not
+// lower the occupancy of the wide launches.
+template <bool CHAN>
 __global__ __launch_bounds__(MP_THREADS) void merkle_parents_kernel(uint8_t* __restrict__ tree, uint32_t k,
                                                                     uint32_t levels, B2State seed,
                                                                     DevChannel* __restrict__ ch,
@@ -97,7 +118,7 @@ __global__ __launch_bounds__(MP_THREADS) void merkle_parents_kernel(uint8_t* __r
     __shared__ uint32_t sh[MP_THREADS * 8];
     const uint32_t t = threadIdx.x;
     const uint8_t* child = tree + (((uint64_t)1 << k) - 1) * 32;
-    const bool chan_here = ch != nullptr && k == levels;
+    constexpr bool chan_here = CHAN;
     if (levels == 1) {
         uint64_t i = (uint64_t)blockIdx.x * MP_THREADS + t;
         if (i >= (1ull << (k - 1))) return;
@@ -203,13 +224,24 @@ void launch_merkle_parent_layer(Ctx& c, const uint8_t* d_children, uint32_t log_
     SR_HIP(hipGetLastError());
 }
 
-void launch_merkle_leaves(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_logs, uint32_t n_cols,
+void launch_merkle_leaves(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
                           uint32_t max_log, uint8_t* d_leaves) {
+    SR_CHECK(n_cols > 0 && max_log <= 31, "merkle leaves: shape");
+    std::vector<LeafColDev> tab(((n_cols + 15) / 16) * 16);
+    for (size_t i = 0; i < tab.size(); i++) {
+        const bool real = i < n_cols;
+        const uint32_t lg = real ? h_logs[i] : h_logs[0];
+        SR_CHECK(lg >= 1 && lg <= max_log, "merkle leaves: column log size");
+        tab[i].col = real ? h_cols[i] : h_cols[0];
+        tab[i].shift = max_log - lg + 1;
+        tab[i].mask = real ? 0xffffffffu : 0u;
+    }
+    DBuf d_tab = upload(c, tab.data(), tab.size());
     uint64_t n = 1ull << max_log;
     KTimer kt(c, "merkle_leaves L=" + std::to_string(max_log) + " cols=" + std::to_string(n_cols),
               (4.0 * n_cols + 32.0) * (double)n);
     hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)), dim3(MK_THREADS), 0,
-                       c.stream, d_cols, d_logs, n_cols, max_log, c.leaf_seed, d_leaves);
+                       c.stream, d_tab.as<LeafColDev>(), n_cols, max_log, c.leaf_seed, d_leaves);
     SR_HIP(hipGetLastError());
 }
 
@@ -228,9 +260,16 @@ void launch_merkle_parents(Ctx& c, DeviceTree& t, DevChannel* d_ch, uint8_t* d_r
         uint32_t levels = k < 10 ? k : 10;
         uint32_t blocks = levels == 1 ? (uint32_t)(((1ull << (k - 1)) + MP_THREADS - 1) / MP_THREADS)
                                       : (1u << (k - levels));
-        KTimer kt(c, "merkle_parents", 32.0 * (double)((1ull << k) + (1ull << k) - (1ull << (k - levels))));
-        hipLaunchKernelGGL(merkle_parents_kernel, dim3(blocks), dim3(MP_THREADS), 0, c.stream, t.layers, k, levels,
-                           c.node_seed, d_ch, (uint32_t*)d_root_out);
+        KTimer kt(c,
+                  "merkle_parents L=" + std::to_string(k) + " levels=" + std::to_string(levels) +
+                      (d_ch && levels == k ? " +channel" : ""),
+                  32.0 * (double)((1ull << k) + (1ull << k) - (1ull << (k - levels))));
+        if (d_ch && levels == k)
+            hipLaunchKernelGGL(merkle_parents_kernel<true>, dim3(blocks), dim3(MP_THREADS), 0, c.stream, t.layers, k,
+                               levels, c.node_seed, d_ch, (uint32_t*)d_root_out);
+        else
+            hipLaunchKernelGGL(merkle_parents_kernel<false>, dim3(blocks), dim3(MP_THREADS), 0, c.stream, t.layers, k,
+                               levels, c.node_seed, nullptr, nullptr);
         SR_HIP(hipGetLastError());
         k -= levels;
     }
@@ -270,10 +309,7 @@ DeviceTree* merkle_commit(Ctx& c, const std::vector<const uint32_t*>& cols, cons
         sc[i] = cols[order[i]];
         sl[i] = logs[order[i]];
     }
-    DBuf d_cols = upload(c, sc.data(), sc.size());
-    DBuf d_logs = upload(c, sl.data(), sl.size());
-    launch_merkle_leaves(c, d_cols.as<const uint32_t* const>(), d_logs.as<uint32_t>(), (uint32_t)sc.size(), max_log,
-                         t->layer(max_log));
+    launch_merkle_leaves(c, sc.data(), sl.data(), (uint32_t)sc.size(), max_log, t->layer(max_log));
     launch_merkle_parents(c, *t);
     return t;
 }

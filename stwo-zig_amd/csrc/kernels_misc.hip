@@ -52,7 +52,7 @@ uint64_t launch_pow_grind(Ctx& c, const uint8_t prefix[32], uint32_t bits, uint6
 // Decommitment gather: out[i] = word at absolute device address addr[i].
 __global__ void gather_kernel(const uint64_t* __restrict__ addr, uint64_t n, uint32_t* __restrict__ out) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = *(const uint32_t*)(uintptr_t)addr[i];
+    if (i < n) out[i] = *(gcu32*)(uintptr_t)addr[i];
 }
 void launch_gather(Ctx& c, const uint64_t* d_addr, uint64_t n, uint32_t* d_out) {
     if (!n) return;

@@ -6,49 +6,84 @@
 // coalesced loads (consecutive lanes = consecutive rows of one column) and
 // accumulates M31 products in 64-bit registers (one Mersenne fold per product,
 // one canonicalisation per sum), which is exact and order independent.
+#include <vector>
+
 #include "runtime.hpp"
 
 namespace sr {
-
-__device__ __forceinline__ uint32_t lifted_row(uint64_t row, uint32_t log_size, uint32_t col_log) {
-    uint32_t sh = log_size - col_log + 1;
-    return (uint32_t)(((row >> sh) << 1) + (row & 1));
-}
 
 // Quotients (core/pcs/quotients/row_evaluation.zig:402-521): for row p with
 // domain point (x, y) = domain.at(bitrev(p)):
 //   q(p) = sum_b ( sum_{e in b} c_e * v_e(p) - (sum_a_b * y + sum_b_b) ) / den_b(p),
 //   den_b = det_b - piy_b * x + pix_b * y  (CM31).
-__global__ __launch_bounds__(256) void quotients_kernel(const uint32_t* const* __restrict__ cols,
-                                                        const uint32_t* __restrict__ col_logs,
-                                                        const uint32_t* __restrict__ entry_cols,
-                                                        const uint4* __restrict__ entry_coeffs,
+// Entries are flat 32-byte descriptors (column, lift shift, coefficient) read
+// with scalar loads; QB column values are loaded at a time so that QB loads
+// are in flight per wave, and four raw products are summed before a fold.
+struct QEntryDev {
+    const uint32_t* col;
+    uint32_t shift;  // lift: row -> ((row >> shift) << 1) + (row & 1); 1 = same size
+    uint32_t pad;
+    uint32_t coeff[4];
+};
+static_assert(sizeof(QEntryDev) == 32, "quotient entry layout");
+
+__device__ __forceinline__ uint32_t lift_row(uint32_t row, uint32_t shift) {
+    return ((row >> shift) << 1) + (row & 1);
+}
+
+__global__ __launch_bounds__(256) void quotients_kernel(const QEntryDev* __restrict__ entries,
                                                         const QuotientBatchDev* __restrict__ batches,
                                                         uint32_t n_batches, uint32_t log_size,
                                                         const uint32_t* __restrict__ domain_xy,
                                                         uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
                                                         uint32_t* __restrict__ o2, uint32_t* __restrict__ o3,
                                                         uint8_t* __restrict__ leaves, B2State seed) {
-    uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = 1u << log_size;
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = 1u << log_size;
     if (row >= n) return;
-    uint32_t x = domain_xy[row], y = domain_xy[n + row];
+    const uint32_t x = domain_xy[row], y = domain_xy[n + row];
+    constexpr uint32_t QB = 8;
     qm31 acc = qzero();
     for (uint32_t bi = 0; bi < n_batches; bi++) {
-        QuotientBatchDev bt = batches[bi];
+        const QuotientBatchDev bt = batches[bi];
         uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-        for (uint32_t e = bt.first; e < bt.first + bt.count; e++) {
-            uint32_t col = entry_cols[e];
-            uint32_t v = cols[col][lifted_row(row, log_size, col_logs[col])];
-            uint4 cf = entry_coeffs[e];
-            s0 += mfold((uint64_t)cf.x * v);
-            s1 += mfold((uint64_t)cf.y * v);
-            s2 += mfold((uint64_t)cf.z * v);
-            s3 += mfold((uint64_t)cf.w * v);
+        uint32_t e = bt.first;
+        const uint32_t end = bt.first + bt.count;
+        for (; e + QB <= end; e += QB) {
+            uint32_t v[QB];
+#pragma unroll
+            for (uint32_t k = 0; k < QB; k++) {
+                const QEntryDev& en = entries[e + k];
+                v[k] = ((gcu32*)en.col)[lift_row(row, en.shift)];
+            }
+#pragma unroll
+            for (uint32_t h = 0; h < QB; h += 4) {
+                uint64_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+                for (uint32_t k = h; k < h + 4; k++) {
+                    const QEntryDev& en = entries[e + k];
+                    t0 += (uint64_t)en.coeff[0] * v[k];
+                    t1 += (uint64_t)en.coeff[1] * v[k];
+                    t2 += (uint64_t)en.coeff[2] * v[k];
+                    t3 += (uint64_t)en.coeff[3] * v[k];
+                }
+                s0 += mfold(t0);
+                s1 += mfold(t1);
+                s2 += mfold(t2);
+                s3 += mfold(t3);
+            }
+        }
+        for (; e < end; e++) {
+            const QEntryDev& en = entries[e];
+            const uint32_t v = ((gcu32*)en.col)[lift_row(row, en.shift)];
+            s0 += mfold((uint64_t)en.coeff[0] * v);
+            s1 += mfold((uint64_t)en.coeff[1] * v);
+            s2 += mfold((uint64_t)en.coeff[2] * v);
+            s3 += mfold((uint64_t)en.coeff[3] * v);
         }
         qm31 num = {{mred64(s0), mred64(s1)}, {mred64(s2), mred64(s3)}};
         num = qsub(num, qadd(qmulm(bt.sum_a, y), bt.sum_b));
-        cm31 den = cadd(csub(bt.det, cmulm(bt.piy, x)), cmulm(bt.pix, y));
+        const cm31 den = cadd(csub(bt.det, cmulm(bt.piy, x)), cmulm(bt.pix, y));
         acc = qadd(acc, qmulc(num, cinv(den)));
     }
     o0[row] = acc.c0.a;
@@ -67,15 +102,30 @@ __global__ __launch_bounds__(256) void quotients_kernel(const uint32_t* const* _
     }
 }
 
-void launch_quotients(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_col_logs, const uint32_t* d_entry_cols,
-                      const uint32_t* d_entry_coeffs, const QuotientBatchDev* d_batches, uint32_t n_batches,
-                      uint32_t log_size, uint32_t* const* out, uint8_t* d_leaf_out) {
+void launch_quotients(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_col_logs, const uint32_t* h_entry_cols,
+                      const uint32_t* h_entry_coeffs, uint32_t n_entries, const QuotientBatchDev* h_batches,
+                      uint32_t n_batches, uint32_t log_size, uint32_t* const* out, uint8_t* d_leaf_out) {
+    std::vector<QEntryDev> ents(n_entries);
+    double in_bytes = 0;
+    for (uint32_t e = 0; e < n_entries; e++) {
+        const uint32_t col = h_entry_cols[e];
+        SR_CHECK(h_col_logs[col] >= 1 && h_col_logs[col] <= log_size, "quotients: column larger than the domain");
+        ents[e].col = h_cols[col];
+        ents[e].shift = log_size - h_col_logs[col] + 1;
+        ents[e].pad = 0;
+        for (int k = 0; k < 4; k++) ents[e].coeff[k] = h_entry_coeffs[4 * e + k];
+        in_bytes += 4.0 * (double)(1ull << h_col_logs[col]);
+    }
+    for (uint32_t b = 0; b < n_batches; b++)
+        SR_CHECK((uint64_t)h_batches[b].first + h_batches[b].count <= n_entries, "quotients: batch range");
+    DBuf d_ent = upload(c, ents.data(), ents.size());
+    DBuf d_bat = upload(c, h_batches, n_batches);
     const uint32_t* xy = c.domain_points(log_size);
     uint32_t n = 1u << log_size;
-    KTimer kt(c, "quotients", c.kbytes_hint + (8.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n);
-    hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_cols, d_col_logs,
-                       d_entry_cols, (const uint4*)d_entry_coeffs, d_batches, n_batches, log_size, xy, out[0], out[1],
-                       out[2], out[3], d_leaf_out, c.leaf_seed);
+    KTimer kt(c, "quotients", in_bytes + (8.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n);
+    hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_ent.as<QEntryDev>(),
+                       d_bat.as<QuotientBatchDev>(), n_batches, log_size, xy, out[0], out[1], out[2], out[3],
+                       d_leaf_out, c.leaf_seed);
     SR_HIP(hipGetLastError());
 }
 
@@ -91,19 +141,50 @@ __global__ __launch_bounds__(256) void widefib_composition_kernel(const uint32_t
                                                                   uint32_t* __restrict__ o3) {
     uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= (1u << log_eval)) return;
-    uint32_t a = cols[0][row], b = cols[1][row];
+    uint32_t a = ((gcu32*)cols[0])[row], b = ((gcu32*)cols[1])[row];
+    // sa, sb: squares of the two previous columns (each column is squared once).
+    uint32_t sa = msqr(a), sb = msqr(b);
     uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     const uint32_t nc = n_cols - 2;
-    for (uint32_t i = 0; i < nc; i++) {
-        uint32_t cv = cols[i + 2][row];
-        uint32_t cons = msub(cv, madd(msqr(a), msqr(b)));
-        uint4 pw = powers[nc - 1 - i];
+    // Column values are loaded CB at a time so that CB loads are in flight per
+    // wave; four raw products (< 2^62 each) are summed in 64 bits before one
+    // Mersenne fold.
+    constexpr uint32_t CB = 8;
+    uint32_t i = 0;
+    for (; i + CB <= nc; i += CB) {
+        uint32_t v[CB];
+#pragma unroll
+        for (uint32_t k = 0; k < CB; k++) v[k] = ((gcu32*)cols[i + 2 + k])[row];
+#pragma unroll
+        for (uint32_t h = 0; h < CB; h += 4) {
+            uint64_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+            for (uint32_t k = h; k < h + 4; k++) {
+                const uint32_t cons = msub(v[k], madd(sa, sb));
+                const uint4 pw = powers[nc - 1 - (i + k)];
+                t0 += (uint64_t)pw.x * cons;
+                t1 += (uint64_t)pw.y * cons;
+                t2 += (uint64_t)pw.z * cons;
+                t3 += (uint64_t)pw.w * cons;
+                sa = sb;
+                sb = msqr(v[k]);
+            }
+            s0 += mfold(t0);
+            s1 += mfold(t1);
+            s2 += mfold(t2);
+            s3 += mfold(t3);
+        }
+    }
+    for (; i < nc; i++) {
+        const uint32_t cv = ((gcu32*)cols[i + 2])[row];
+        const uint32_t cons = msub(cv, madd(sa, sb));
+        const uint4 pw = powers[nc - 1 - i];
         s0 += mfold((uint64_t)pw.x * cons);
         s1 += mfold((uint64_t)pw.y * cons);
         s2 += mfold((uint64_t)pw.z * cons);
         s3 += mfold((uint64_t)pw.w * cons);
-        a = b;
-        b = cv;
+        sa = sb;
+        sb = msqr(cv);
     }
     uint32_t d = (row >> log_n) ? den1 : den0;
     o0[row] = mmul(mred64(s0), d);
@@ -198,7 +279,7 @@ __global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const
                                                            unsigned long long* __restrict__ acc) {
     __shared__ unsigned long long red[4][256];
     const uint32_t col = blockIdx.y;
-    const uint32_t* __restrict__ cv = cols[col];
+    gcu32* __restrict__ cv = (gcu32*)cols[col];
     const uint32_t n = 1u << log_size;
     uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     uint32_t base = blockIdx.x * (256 * EV_PER_THREAD);

@@ -464,15 +464,8 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         SR_HIP(hipMemcpyAsync(d_ch, c.stage(&hc, sizeof(hc)), sizeof(hc), hipMemcpyHostToDevice, c.stream));
     }
     {
-        DBuf d_cols = upload(c, all_lde.data(), NS);
-        DBuf d_logs = upload(c, all_lde_logs.data(), NS);
-        DBuf d_ec = upload(c, entry_cols.data(), entry_cols.size());
-        DBuf d_ecf = upload(c, entry_coeffs.data(), entry_coeffs.size());
-        DBuf d_b = upload(c, batches.data(), batches.size());
-        c.kbytes_hint = 0;
-        for (uint32_t i = 0; i < NS; i++) c.kbytes_hint += 4.0 * (double)(1ull << all_lde_logs[i]);
-        launch_quotients(c, d_cols.as<const uint32_t* const>(), d_logs.as<uint32_t>(), d_ec.as<uint32_t>(),
-                         d_ecf.as<uint32_t>(), d_b.as<QuotientBatchDev>(), (uint32_t)batches.size(), L, qc,
+        launch_quotients(c, all_lde.data(), all_lde_logs.data(), entry_cols.data(), entry_coeffs.data(),
+                         (uint32_t)entry_cols.size(), batches.data(), (uint32_t)batches.size(), L, qc,
                          ftree0->layer(L));
         launch_merkle_parents(c, *ftree0, d_ch, d_roots);
     }

@@ -91,7 +91,6 @@ struct Ctx {
     };
     bool profiling = false;
     std::string profile_only;  // non-empty: record only launches of this class
-    double kbytes_hint = 0;  // input bytes of the next quotient launch (set by its caller)
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<KRec> krecs;
@@ -168,7 +167,8 @@ struct DBuf {
 void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint32_t* const* d_dst, uint32_t n_cols,
                        uint32_t m, uint32_t src_log);
 // Lifted Merkle: columns sorted by log size ascending (d_cols/d_logs device arrays).
-void launch_merkle_leaves(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_logs, uint32_t n_cols,
+// Host arrays in: column device pointers and log sizes, sorted by log size.
+void launch_merkle_leaves(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
                           uint32_t max_log, uint8_t* d_leaves);
 void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size, uint8_t* d_leaves);
 // All parent layers above t's leaves.  With d_ch, the launch reaching the root
@@ -205,9 +205,12 @@ struct QuotientBatchDev {
     qm31 sum_a, sum_b;   // linear term
     uint32_t first, count;  // range into the (column, coeff) entry arrays
 };
-void launch_quotients(Ctx& c, const uint32_t* const* d_cols, const uint32_t* d_col_logs, const uint32_t* d_entry_cols,
-                      const uint32_t* d_entry_coeffs, const QuotientBatchDev* d_batches, uint32_t n_batches,
-                      uint32_t log_size, uint32_t* const* out_coords /* host array of 4 */, uint8_t* d_leaf_out);
+// Host arrays in: column device pointers and log sizes, per-entry column index
+// and 4 coefficient words, batches (ranges into the entries).
+void launch_quotients(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_col_logs, const uint32_t* h_entry_cols,
+                      const uint32_t* h_entry_coeffs, uint32_t n_entries, const QuotientBatchDev* h_batches,
+                      uint32_t n_batches, uint32_t log_size, uint32_t* const* out_coords /* host array of 4 */,
+                      uint8_t* d_leaf_out);
 // Point evaluation: sum_i coeff_i * w_i with weights of `point` (2^log).
 void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w);
 void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,

@@ -184,6 +184,71 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
                                    const uint32_t* d_trace, uint8_t* proof_out, uint64_t cap, uint64_t* proof_len,
                                    stwo_rocm_prove_stats* stats);
 
+/* ------------------------------------------------------------------ LogUp / GKR */
+/* Layer kinds of src/prover/lookups/gkr_circuit.zig:79-92. */
+#define STWO_ROCM_GKR_GRAND_PRODUCT 0
+#define STWO_ROCM_GKR_LOGUP_GENERIC 1       /* numerators QM31 (AoS, 4 words) */
+#define STWO_ROCM_GKR_LOGUP_MULTIPLICITIES 2 /* numerators M31 (1 word) */
+#define STWO_ROCM_GKR_LOGUP_SINGLES 3       /* numerators all one (d_num ignored) */
+
+/* XOR-lookup AIR of the LogUp workload (no reference counterpart: the AIR is defined by this
+ * backend's LogUp benchmark): 2^log_size rows of a, b uniform 8-bit (splitmix64 of
+ * seed ^ i*0xD1B54A32D192ED03), c = a ^ b, and mult[x*256+y] = occurrences of (x, y). */
+int stwo_rocm_xor_trace(stwo_rocm_ctx* ctx, uint32_t log_size, uint64_t seed, uint32_t* d_a, uint32_t* d_b,
+                        uint32_t* d_c, uint32_t* d_mult);
+/* den[i] = z - (a + b*alpha + c*alpha^2) as QM31 AoS (the relation combination of
+ * src/frontends/riscv/air/logup.zig:131-175 for a 3-tuple); with d_a = d_b = d_c = NULL the rows are
+ * the table ((i >> 8) & 255, i & 255, their xor), repeating every 2^16 rows. */
+int stwo_rocm_logup_denominators(stwo_rocm_ctx* ctx, const uint32_t* d_a, const uint32_t* d_b, const uint32_t* d_c,
+                                 uint64_t n, const uint32_t z[4], const uint32_t alpha[4], uint32_t* d_den);
+/* Cumulative LogUp column in trace order, replaces cumulativeColumn
+ * (src/frontends/riscv/air/logup.zig:74-91): d_sums[i] = sum_{j<=i} num_j / den_j (QM31 AoS),
+ * claimed = the total.  num_kind: 0 QM31 AoS, 1 M31, 2 all ones (d_num ignored).  A zero
+ * denominator returns STWO_ROCM_EINVAL. */
+int stwo_rocm_logup_cumulative(stwo_rocm_ctx* ctx, int num_kind, const uint32_t* d_num, const uint32_t* d_den,
+                               uint64_t n, uint32_t* d_sums, uint32_t claimed[4]);
+/* GkrOps.genEqEvals (src/backend/gkr_ops.zig:1-23; gkr_circuit.zig:651-674): d_out[i] (2^n QM31) =
+ * scale * prod_j (bit_{n-1-j}(i) ? y_j : 1 - y_j); y is a host array of n QM31. */
+int stwo_rocm_gkr_gen_eq_evals(stwo_rocm_ctx* ctx, const uint32_t* y, uint32_t n, const uint32_t scale[4],
+                               uint32_t* d_out);
+/* GkrOps.nextLayer (gkr_circuit.zig:139-160,449-503): halves a layer of n_vars variables
+ * (pairs 2i, 2i+1); LogUp layers come out generic (QM31 numerators). */
+int stwo_rocm_gkr_next_layer(stwo_rocm_ctx* ctx, int kind, const uint32_t* d_num, const uint32_t* d_den,
+                             uint32_t n_vars, uint32_t* d_out_num, uint32_t* d_out_den);
+/* GkrOps.sumAsPolyInFirstVariable (gkr_circuit.zig:261-316,390-429) of the sum-check oracle over a layer of
+ * n_vars variables with eq table d_eq, batching lambda, claim, eq point y (host, y_len QM31) and
+ * eq_fixed_var_correction: up to 4 QM31 coefficients (ascending, trailing zeros trimmed). */
+int stwo_rocm_gkr_sum_as_poly_in_first_variable(stwo_rocm_ctx* ctx, int kind, const uint32_t* d_eq,
+                                                const uint32_t* d_num, const uint32_t* d_den, uint32_t n_vars,
+                                                const uint32_t lambda[4], const uint32_t claim[4], const uint32_t* y,
+                                                uint32_t y_len, const uint32_t correction[4], uint32_t coeffs_out[16],
+                                                uint32_t* n_coeffs);
+/* Mle.fixFirstVariable (src/prover/lookups/mle.zig:80-95): d_out[i] = r*(in[i+h]-in[i]) + in[i]
+ * (QM31 AoS out; input M31 when base_field != 0). */
+int stwo_rocm_mle_fix_first_variable(stwo_rocm_ctx* ctx, int base_field, const uint32_t* d_in, uint32_t n_vars,
+                                     const uint32_t r[4], uint32_t* d_out);
+/* Batch GKR proof, replaces proveBatch (src/prover/lookups/gkr_prover.zig:34-298 with
+ * sumcheck.zig:76-197) on a Blake2s channel starting at channel_digest.  Writes the proof word
+ * stream: per GKR layer [n_rounds, each (len, len QM31)], [n_masks, each (instance, n_cols,
+ * 2*n_cols QM31)]; per instance [n, output claims]; [n, ood point]; per instance [n, claims to
+ * verify], n_vars; then the final channel digest (8 words).  When cap is too small returns
+ * STWO_ROCM_EINVAL with *n_words set. */
+int stwo_rocm_gkr_prove_batch(stwo_rocm_ctx* ctx, uint32_t n_instances, const int* kinds,
+                              const uint32_t* const* d_nums, const uint32_t* const* d_dens, const uint32_t* n_vars,
+                              const uint8_t channel_digest[32], uint32_t* words_out, uint64_t cap,
+                              uint64_t* n_words);
+
+/* LogUp workload end to end (no single reference counterpart: the sequence of commit
+ * (src/prover/pcs/commit_polys.zig:10-67), LogUp cumulative columns (logup.zig:74-91), batch GKR
+ * (gkr_prover.zig:34-298), sampled values (sampled_values.zig:67-80) and quotient accumulation
+ * (row_evaluation.zig:402-521) applied to the XOR-lookup AIR of stwo_rocm_xor_trace on 2^log_n rows,
+ * log_n in 16..26).  Writes: main, interaction and quotient Merkle roots (3 x 8 words), claimed sums
+ * of lookups and table (2 x 4), 12 sampled values (12 x 4), the GKR word count and proof words, and
+ * the final channel digest (8).  total_ms (optional): host wall time of the call.  Returns
+ * STWO_ROCM_EINVAL if the lookups do not balance or cap is too small (*n_words set). */
+int stwo_rocm_logup_xor_pipeline(stwo_rocm_ctx* ctx, uint32_t log_n, uint64_t seed, uint32_t* words_out,
+                                 uint64_t cap, uint64_t* n_words, double* total_ms);
+
 /* ------------------------------------------------------------------ measurement */
 /* No counterpart in the reference: a measurement hook for callers that need per-kernel
  * device time (the reference's stage profiler, src/prover/stage_profile.zig, times stages

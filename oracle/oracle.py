@@ -66,6 +66,18 @@ def lib():
         L.ora_prove_wide_fibonacci.argtypes = [ctypes.POINTER(WideFibParams), ctypes.c_char_p, ctypes.c_size_t,
                                                ctypes.POINTER(ctypes.c_size_t)]
         L.ora_last_timings.argtypes = [ctypes.POINTER(ctypes.c_double)]
+        L.ora_xor_trace.argtypes = [ctypes.c_uint32, ctypes.c_uint64, u32p, u32p, u32p, u32p]
+        L.ora_logup_denominators.argtypes = [u32p, u32p, u32p, ctypes.c_uint64, u32p, u32p, u32p]
+        L.ora_logup_cumulative.restype = ctypes.c_int
+        L.ora_logup_cumulative.argtypes = [ctypes.c_int, u32p, u32p, ctypes.c_uint64, u32p, u32p]
+        L.ora_gkr_gen_eq_evals.argtypes = [u32p, ctypes.c_uint32, u32p, u32p]
+        L.ora_gkr_next_layer.argtypes = [ctypes.c_int, u32p, u32p, ctypes.c_uint32, u32p, u32p]
+        L.ora_mle_fix_first.argtypes = [ctypes.c_int, u32p, ctypes.c_uint32, u32p, u32p]
+        L.ora_gkr_prove.restype = ctypes.c_size_t
+        L.ora_gkr_prove.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(u32p),
+                                    ctypes.POINTER(u32p), u32p, ctypes.c_char_p, u32p, ctypes.c_size_t]
+        L.ora_logup_xor_pipeline.restype = ctypes.c_size_t
+        L.ora_logup_xor_pipeline.argtypes = [ctypes.c_uint32, ctypes.c_uint64, u32p, ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -222,3 +234,128 @@ def last_timings():
     lib().ora_last_timings(arr)
     keys = ["trace_gen", "main_commit", "composition", "sampled_values", "quotients", "fri_commit", "pow", "prove_total"]
     return dict(zip(keys, list(arr)))
+
+
+# ---------------------------------------------------------------- LogUp / GKR
+GKR_GRAND_PRODUCT, GKR_LOGUP_GENERIC, GKR_LOGUP_MULTIPLICITIES, GKR_LOGUP_SINGLES = 0, 1, 2, 3
+_NULL = ctypes.POINTER(ctypes.c_uint32)()
+
+
+def xor_trace(log_size, seed):
+    """(a, b, c, multiplicities[65536]) of the XOR-lookup AIR."""
+    n = 1 << log_size
+    a, b, c = (np.zeros(n, np.uint32) for _ in range(3))
+    m = np.zeros(1 << 16, np.uint32)
+    _, pa = _u32(a)
+    _, pb = _u32(b)
+    _, pc = _u32(c)
+    _, pm = _u32(m)
+    lib().ora_xor_trace(log_size, seed, pa, pb, pc, pm)
+    return a, b, c, m
+
+
+def logup_denominators(a, b, c, z, alpha, n=None):
+    """QM31 AoS denominators z - (a + b*alpha + c*alpha^2); table rows (x, y, x^y) of index x*256+y when a is None."""
+    zz, pz = _u32(z)
+    al, pal = _u32(alpha)
+    if a is None:
+        n = 1 << 16 if n is None else n
+        out = np.zeros(4 * n, np.uint32)
+        _, po = _u32(out)
+        lib().ora_logup_denominators(_NULL, _NULL, _NULL, n, pz, pal, po)
+        return out
+    a, pa = _u32(a)
+    b, pb = _u32(b)
+    c, pc = _u32(c)
+    out = np.zeros(4 * len(a), np.uint32)
+    _, po = _u32(out)
+    lib().ora_logup_denominators(pa, pb, pc, len(a), pz, pal, po)
+    return out
+
+
+def logup_cumulative(num_kind, num, den):
+    """(sums QM31 AoS, claimed[4]); num_kind 0 = QM31, 1 = M31, 2 = ones (num ignored)."""
+    d, pd = _u32(den)
+    n = len(d) // 4
+    sums = np.zeros(4 * n, np.uint32)
+    claimed = np.zeros(4, np.uint32)
+    _, ps = _u32(sums)
+    _, pcl = _u32(claimed)
+    if num is None:
+        pn = _NULL
+    else:
+        num, pn = _u32(num)
+    rc = lib().ora_logup_cumulative(num_kind, pn, pd, n, ps, pcl)
+    if rc != 0:
+        raise ZeroDivisionError("zero LogUp denominator")
+    return sums, claimed
+
+
+def gkr_gen_eq_evals(y, scale):
+    """y: QM31 AoS (n points); returns 2^n QM31 AoS."""
+    y, py = _u32(y)
+    n = len(y) // 4
+    sc, psc = _u32(scale)
+    out = np.zeros(4 << n, np.uint32)
+    _, po = _u32(out)
+    lib().ora_gkr_gen_eq_evals(py, n, psc, po)
+    return out
+
+
+def gkr_next_layer(kind, num, den, n_vars):
+    d, pd = _u32(den)
+    pn = _NULL
+    if num is not None:
+        num, pn = _u32(num)
+    half = 1 << (n_vars - 1)
+    on = np.zeros(4 * half, np.uint32)
+    od = np.zeros(4 * half, np.uint32)
+    _, pon = _u32(on)
+    _, pod = _u32(od)
+    lib().ora_gkr_next_layer(kind, pn, pd, n_vars, pon, pod)
+    return (None if kind == GKR_GRAND_PRODUCT else on), od
+
+
+def mle_fix_first(base, values, n_vars, assignment):
+    v, pv = _u32(values)
+    a, pa = _u32(assignment)
+    out = np.zeros(4 << (n_vars - 1), np.uint32)
+    _, po = _u32(out)
+    lib().ora_mle_fix_first(1 if base else 0, pv, n_vars, pa, po)
+    return out
+
+
+def gkr_prove(instances, digest=bytes(32)):
+    """instances: list of (kind, num or None, den, n_vars).  Returns the proof word stream."""
+    n = len(instances)
+    kinds = (ctypes.c_int * n)(*[k for k, _, _, _ in instances])
+    keep = []
+    nums = (ctypes.POINTER(ctypes.c_uint32) * n)()
+    dens = (ctypes.POINTER(ctypes.c_uint32) * n)()
+    for i, (k, num, den, nv) in enumerate(instances):
+        d, pd = _u32(den)
+        keep.append(d)
+        dens[i] = pd
+        if num is not None:
+            nn, pn = _u32(num)
+            keep.append(nn)
+            nums[i] = pn
+        else:
+            nums[i] = _NULL
+    nv, pnv = _u32([nv for _, _, _, nv in instances])
+    need = lib().ora_gkr_prove(n, kinds, nums, dens, pnv, digest, _NULL, 0)
+    out = np.zeros(need, np.uint32)
+    _, po = _u32(out)
+    lib().ora_gkr_prove(n, kinds, nums, dens, pnv, digest, po, need)
+    return out
+
+
+def logup_xor_pipeline(log_n, seed):
+    """Word stream of the LogUp workload (see ora_logup_xor_pipeline)."""
+    need = lib().ora_logup_xor_pipeline(log_n, seed, _NULL, 0)
+    if need == 0:
+        raise ValueError("LogUp workload did not balance")
+    out = np.zeros(need, np.uint32)
+    _, po = _u32(out)
+    lib().ora_logup_xor_pipeline(log_n, seed, po, need)
+    return out

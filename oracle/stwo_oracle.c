@@ -1347,3 +1347,636 @@ int ora_prove_wide_fibonacci(const ora_widefib_params *prm, uint8_t *out, size_t
     for (int tr = 0; tr < 3; tr++) ctree_free(&trees[tr]);
     return rc;
 }
+
+/* ======================================================================
+ * LogUp / GKR (src/core/fraction.zig:9-73; src/frontends/riscv/air/logup.zig:74-105;
+ * src/prover/lookups/{gkr_circuit,gkr_prover,sumcheck,mle,utils}.zig) and the
+ * XOR-lookup AIR of the LogUp workload (defined by this build: a, b uniform
+ * 8-bit from splitmix64, c = a ^ b; table of all 2^16 (x, y, x ^ y) with
+ * multiplicities; denominators z - (a + b*alpha + c*alpha^2)).
+ * ==================================================================== */
+static inline uint64_t ora_splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+void ora_xor_trace(uint32_t log_size, uint64_t seed, uint32_t *a, uint32_t *b, uint32_t *c, uint32_t *mult) {
+    uint64_t n = 1ull << log_size;
+    memset(mult, 0, sizeof(uint32_t) << 16);
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t w = ora_splitmix64(seed ^ (i * 0xD1B54A32D192ED03ull));
+        a[i] = (uint32_t)(w & 0xff);
+        b[i] = (uint32_t)((w >> 8) & 0xff);
+        c[i] = a[i] ^ b[i];
+        mult[(a[i] << 8) | b[i]]++;
+    }
+}
+
+static inline qm31 logup_combine(uint32_t a, uint32_t b, uint32_t c, qm31 alpha, qm31 alpha2) {
+    return q_add(q_add(q_from_m(a), q_mul_m(alpha, b)), q_mul_m(alpha2, c));
+}
+
+/* den[i] = z - (a[i] + b[i]*alpha + c[i]*alpha^2); table rows when a == NULL. */
+void ora_logup_denominators(const uint32_t *a, const uint32_t *b, const uint32_t *c, uint64_t n, const uint32_t z[4],
+                            const uint32_t alpha[4], uint32_t *den) {
+    qm31 zz = q_load(z), al = q_load(alpha), al2 = q_square(al);
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t x, y, r;
+        if (a) {
+            x = a[i];
+            y = b[i];
+            r = c[i];
+        } else {
+            x = (uint32_t)((i >> 8) & 0xff);
+            y = (uint32_t)(i & 0xff);
+            r = x ^ y;
+        }
+        q_store(q_sub(zz, logup_combine(x, y, r, al, al2)), den + 4 * i);
+    }
+}
+
+/* Cumulative column in trace order (logup.zig:74-91): S_i = sum_{j<=i} num_j / den_j.
+ * num: QM31 AoS (num_kind 0), M31 (num_kind 1) or all ones (num_kind 2).
+ * Returns -1 on a zero denominator. */
+int ora_logup_cumulative(int num_kind, const uint32_t *num, const uint32_t *den, uint64_t n, uint32_t *sums,
+                         uint32_t claimed[4]) {
+    qm31 acc = Q_ZERO;
+    for (uint64_t i = 0; i < n; i++) {
+        qm31 d = q_load(den + 4 * i);
+        if (q_is_zero(d)) return -1;
+        qm31 nu = num_kind == 0 ? q_load(num + 4 * i) : num_kind == 1 ? q_from_m(num[i]) : Q_ONE;
+        acc = q_add(acc, q_mul(nu, q_inv(d)));
+        if (sums) q_store(acc, sums + 4 * i);
+    }
+    q_store(acc, claimed);
+    return 0;
+}
+
+/* EqEvals over the hypercube: out[i] = scale * prod_j (bit_{n-1-j}(i) ? y_j : 1 - y_j)
+ * (gkr_circuit.zig:651-674: y[0] selects the upper half). */
+void ora_gkr_gen_eq_evals(const uint32_t *y, uint32_t n, const uint32_t scale[4], uint32_t *out) {
+    uint64_t len = 1ull << n;
+    q_store(q_load(scale), out);
+    uint64_t cur = 1;
+    for (int32_t j = (int32_t)n - 1; j >= 0; j--) {
+        qm31 yj = q_load(y + 4 * j), y0 = q_sub(Q_ONE, yj);
+        for (uint64_t i = 0; i < cur; i++) {
+            qm31 v = q_load(out + 4 * i);
+            q_store(q_mul(v, yj), out + 4 * (i + cur));
+            q_store(q_mul(v, y0), out + 4 * i);
+        }
+        cur *= 2;
+    }
+    (void)len;
+}
+
+/* Layer kinds (gkr_circuit.zig:79-92). */
+enum { GKR_GRAND_PRODUCT = 0, GKR_LOGUP_GENERIC = 1, GKR_LOGUP_MULTIPLICITIES = 2, GKR_LOGUP_SINGLES = 3 };
+
+static inline qm31 layer_num(int kind, const uint32_t *num, uint64_t i) {
+    if (kind == GKR_LOGUP_GENERIC) return q_load(num + 4 * i);
+    if (kind == GKR_LOGUP_MULTIPLICITIES) return q_from_m(num[i]);
+    return Q_ONE;
+}
+
+/* nextLayer (gkr_circuit.zig:449-503): pairs (2i, 2i+1); LogUp layers become generic. */
+void ora_gkr_next_layer(int kind, const uint32_t *num, const uint32_t *den, uint32_t n_vars, uint32_t *out_num,
+                        uint32_t *out_den) {
+    uint64_t half = 1ull << (n_vars - 1);
+    for (uint64_t i = 0; i < half; i++) {
+        if (kind == GKR_GRAND_PRODUCT) {
+            q_store(q_mul(q_load(den + 8 * i), q_load(den + 8 * i + 4)), out_den + 4 * i);
+            continue;
+        }
+        qm31 n0 = layer_num(kind, num, 2 * i), n1 = layer_num(kind, num, 2 * i + 1);
+        qm31 d0 = q_load(den + 8 * i), d1 = q_load(den + 8 * i + 4);
+        q_store(q_add(q_mul(d1, n0), q_mul(d0, n1)), out_num + 4 * i);
+        q_store(q_mul(d0, d1), out_den + 4 * i);
+    }
+}
+
+/* Mle.fixFirstVariable (mle.zig:80-95; utils.zig:234-244): out[i] = a*(e[i+h]-e[i]) + e[i]. */
+void ora_mle_fix_first(int base, const uint32_t *in, uint32_t n_vars, const uint32_t assignment[4], uint32_t *out) {
+    uint64_t half = 1ull << (n_vars - 1);
+    qm31 as = q_load(assignment);
+    for (uint64_t i = 0; i < half; i++) {
+        qm31 e0 = base ? q_from_m(in[i]) : q_load(in + 4 * i);
+        qm31 e1 = base ? q_from_m(in[i + half]) : q_load(in + 4 * (i + half));
+        q_store(q_add(q_mul(as, q_sub(e1, e0)), e0), out + 4 * i);
+    }
+}
+
+/* --- univariate polynomials (utils.zig:14-214): coefficients ascending, trailing zeros trimmed --- */
+typedef struct { qm31 c[8]; uint32_t len; } upoly;
+static void up_trim(upoly *p) {
+    while (p->len > 0 && q_is_zero(p->c[p->len - 1])) p->len--;
+}
+static qm31 up_eval(const upoly *p, qm31 x) {
+    qm31 acc = Q_ZERO;
+    for (uint32_t i = p->len; i > 0; i--) acc = q_add(q_mul(acc, x), p->c[i - 1]);
+    return acc;
+}
+static qm31 q_div(qm31 a, qm31 b) { return q_mul(a, q_inv(b)); }
+/* interpolateLagrange (utils.zig:147-190) for 4 points; the interpolant is unique. */
+static upoly up_lagrange4(const qm31 xs[4], const qm31 ys[4]) {
+    upoly acc = {.len = 0};
+    for (int i = 0; i < 4; i++) {
+        qm31 prod = ys[i];
+        for (int j = 0; j < 4; j++)
+            if (j != i) prod = q_div(prod, q_sub(xs[i], xs[j]));
+        upoly term = {.len = 1};
+        term.c[0] = prod;
+        up_trim(&term);
+        for (int j = 0; j < 4; j++) {
+            if (j == i || term.len == 0) continue;
+            upoly nt = {.len = term.len + 1};
+            for (uint32_t k = 0; k < nt.len; k++) nt.c[k] = Q_ZERO;
+            for (uint32_t k = 0; k < term.len; k++) {
+                nt.c[k] = q_add(nt.c[k], q_mul(term.c[k], q_neg(xs[j])));
+                nt.c[k + 1] = q_add(nt.c[k + 1], term.c[k]);
+            }
+            up_trim(&nt);
+            term = nt;
+        }
+        uint32_t n = acc.len > term.len ? acc.len : term.len;
+        upoly s = {.len = n};
+        for (uint32_t k = 0; k < n; k++)
+            s.c[k] = q_add(k < acc.len ? acc.c[k] : Q_ZERO, k < term.len ? term.c[k] : Q_ZERO);
+        up_trim(&s);
+        acc = s;
+    }
+    return acc;
+}
+static qm31 eq1(qm31 x, qm31 y) { /* utils.zig:220-231 */
+    return q_add(q_mul(x, y), q_mul(q_sub(Q_ONE, x), q_sub(Q_ONE, y)));
+}
+/* correctSumAsPolyInFirstVariable (gkr_circuit.zig:390-429). */
+static upoly gkr_correct_sum(qm31 f0, qm31 f2, qm31 claim, const qm31 *y, uint32_t ylen, uint32_t k) {
+    qm31 eq_prefix = Q_ONE;
+    for (uint32_t i = 0; i < ylen - k + 1; i++) eq_prefix = q_mul(eq_prefix, q_sub(Q_ONE, y[i]));
+    qm31 a_const = q_div(Q_ONE, eq_prefix);
+    qm31 yi = y[ylen - k];
+    qm31 b_const = q_div(q_sub(Q_ONE, yi), q_sub(Q_ONE, q_add(yi, yi)));
+    qm31 two = q_from_m(2);
+    qm31 r0 = q_mul(q_mul(f0, q_sub(Q_ONE, yi)), a_const);
+    qm31 r1 = q_sub(claim, r0);
+    qm31 r2 = q_mul(q_mul(f2, eq1(two, yi)), a_const);
+    qm31 xs[4] = {Q_ZERO, Q_ONE, two, b_const}, ys[4] = {r0, r1, r2, Q_ZERO};
+    return up_lagrange4(xs, ys);
+}
+
+/* One GKR instance's sum-check oracle (gkr_circuit.zig:231-388). */
+typedef struct {
+    int kind;          /* current layer kind (multiplicities become generic once fixed) */
+    uint32_t n_vars;   /* variables of the layer (oracle has n_vars - 1) */
+    uint32_t *num;     /* M31 (multiplicities) or QM31 AoS */
+    uint32_t *den;     /* QM31 AoS (values for grand product) */
+    qm31 corr;         /* eq_fixed_var_correction */
+} gkr_oracle;
+
+/* sum over i < n_terms of eq[i] * term(i) at r = 0 and r = 2 (gkr_circuit.zig:505-643). */
+static void gkr_eval_sums(const gkr_oracle *o, const qm31 *eq, qm31 lambda, qm31 out[2]) {
+    uint64_t nt = 1ull << (o->n_vars - 2);
+    qm31 e0 = Q_ZERO, e2 = Q_ZERO;
+    for (uint64_t i = 0; i < nt; i++) {
+        qm31 t0, t2;
+        if (o->kind == GKR_GRAND_PRODUCT) {
+            qm31 a0 = q_load(o->den + 8 * i), a1 = q_load(o->den + 8 * i + 4);
+            qm31 b0 = q_load(o->den + 8 * (nt + i)), b1 = q_load(o->den + 8 * (nt + i) + 4);
+            qm31 c0 = q_sub(q_add(b0, b0), a0), c1 = q_sub(q_add(b1, b1), a1);
+            t0 = q_mul(a0, a1);
+            t2 = q_mul(c0, c1);
+        } else {
+            qm31 d00 = q_load(o->den + 8 * i), d01 = q_load(o->den + 8 * i + 4);
+            qm31 d10 = q_load(o->den + 8 * (nt + i)), d11 = q_load(o->den + 8 * (nt + i) + 4);
+            qm31 d20 = q_sub(q_add(d10, d10), d00), d21 = q_sub(q_add(d11, d11), d01);
+            qm31 n00 = layer_num(o->kind, o->num, 2 * i), n01 = layer_num(o->kind, o->num, 2 * i + 1);
+            qm31 n10 = layer_num(o->kind, o->num, 2 * (nt + i)), n11 = layer_num(o->kind, o->num, 2 * (nt + i) + 1);
+            qm31 n20 = q_sub(q_add(n10, n10), n00), n21 = q_sub(q_add(n11, n11), n01);
+            qm31 fn0 = q_add(q_mul(d01, n00), q_mul(d00, n01)), fd0 = q_mul(d00, d01);
+            qm31 fn2 = q_add(q_mul(d21, n20), q_mul(d20, n21)), fd2 = q_mul(d20, d21);
+            t0 = q_add(fn0, q_mul(lambda, fd0));
+            t2 = q_add(fn2, q_mul(lambda, fd2));
+        }
+        e0 = q_add(e0, q_mul(eq[i], t0));
+        e2 = q_add(e2, q_mul(eq[i], t2));
+    }
+    out[0] = e0;
+    out[1] = e2;
+}
+
+static void gkr_fix(gkr_oracle *o, qm31 r) {
+    uint32_t nv = o->n_vars;
+    uint64_t half = 1ull << (nv - 1);
+    uint32_t *nd = malloc(16 * half);
+    ora_mle_fix_first(0, o->den, nv, (uint32_t[4]){r.c0.a, r.c0.b, r.c1.a, r.c1.b}, nd);
+    free(o->den);
+    o->den = nd;
+    if (o->kind == GKR_LOGUP_GENERIC || o->kind == GKR_LOGUP_MULTIPLICITIES) {
+        uint32_t *nn = malloc(16 * half);
+        ora_mle_fix_first(o->kind == GKR_LOGUP_MULTIPLICITIES, o->num, nv, (uint32_t[4]){r.c0.a, r.c0.b, r.c1.a, r.c1.b},
+                          nn);
+        free(o->num);
+        o->num = nn;
+        o->kind = GKR_LOGUP_GENERIC;
+    }
+    o->n_vars = nv - 1;
+}
+
+/* Word stream writer. */
+typedef struct { uint32_t *w; size_t n, cap; } wbuf;
+static void wb_u(wbuf *b, uint32_t v) {
+    if (b->n < b->cap) b->w[b->n] = v;
+    b->n++;
+}
+static void wb_q(wbuf *b, qm31 q) {
+    wb_u(b, q.c0.a), wb_u(b, q.c0.b), wb_u(b, q.c1.a), wb_u(b, q.c1.b);
+}
+
+/* GKR batch prover (gkr_prover.zig:34-298, sumcheck.zig:76-197) over instances given as
+ * input layers.  Returns the word count of the proof stream (written when it fits `cap`):
+ *   per GKR layer: n_round_polys, each [len, coeffs]; n_masks, each [instance, n_cols, cols];
+ *   then per instance: output claims [n, values]; ood point [n, values];
+ *   per instance: claims to verify [n, values], n_vars; final channel digest (8 words). */
+static size_t gkr_prove_on(channel *ch, uint32_t n_inst, const int *kinds, const uint32_t *const *nums,
+                           const uint32_t *const *dens, const uint32_t *n_vars_in, uint32_t *out, size_t cap) {
+    wbuf wb = {out, 0, cap};
+    uint32_t n_layers = 0;
+    for (uint32_t k = 0; k < n_inst; k++) n_layers = n_vars_in[k] > n_layers ? n_vars_in[k] : n_layers;
+    /* layers[k][l] = layer with l variables of instance k (l = n_vars .. 0) */
+    uint32_t ***lnum = calloc(n_inst, sizeof(uint32_t **)), ***lden = calloc(n_inst, sizeof(uint32_t **));
+    int **lkind = calloc(n_inst, sizeof(int *));
+    for (uint32_t k = 0; k < n_inst; k++) {
+        uint32_t nv = n_vars_in[k];
+        lnum[k] = calloc(nv + 1, sizeof(uint32_t *));
+        lden[k] = calloc(nv + 1, sizeof(uint32_t *));
+        lkind[k] = calloc(nv + 1, sizeof(int));
+        uint64_t len = 1ull << nv;
+        int kind = kinds[k];
+        lkind[k][nv] = kind;
+        lden[k][nv] = malloc(16 * len);
+        memcpy(lden[k][nv], dens[k], 16 * len);
+        if (kind == GKR_LOGUP_GENERIC) {
+            lnum[k][nv] = malloc(16 * len);
+            memcpy(lnum[k][nv], nums[k], 16 * len);
+        } else if (kind == GKR_LOGUP_MULTIPLICITIES) {
+            lnum[k][nv] = malloc(4 * len);
+            memcpy(lnum[k][nv], nums[k], 4 * len);
+        }
+        for (int32_t l = (int32_t)nv - 1; l >= 0; l--) {
+            uint64_t h = 1ull << l;
+            lden[k][l] = malloc(16 * h);
+            lnum[k][l] = kind == GKR_GRAND_PRODUCT ? NULL : malloc(16 * h);
+            ora_gkr_next_layer(lkind[k][l + 1], lnum[k][l + 1], lden[k][l + 1], (uint32_t)l + 1, lnum[k][l],
+                               lden[k][l]);
+            lkind[k][l] = kind == GKR_GRAND_PRODUCT ? GKR_GRAND_PRODUCT : GKR_LOGUP_GENERIC;
+        }
+    }
+    qm31 **claims = calloc(n_inst, sizeof(qm31 *)), **outputs = calloc(n_inst, sizeof(qm31 *));
+    uint32_t *n_claims = calloc(n_inst, sizeof(uint32_t));
+    qm31 *ood = NULL;
+    uint32_t ood_len = 0;
+    for (uint32_t layer = 0; layer < n_layers; layer++) {
+        uint32_t remaining = n_layers - layer;
+        for (uint32_t k = 0; k < n_inst; k++) {
+            if (n_vars_in[k] != remaining) continue;
+            /* output layer values (gkr_circuit.zig:162-190) */
+            int gp = lkind[k][0] == GKR_GRAND_PRODUCT;
+            n_claims[k] = gp ? 1 : 2;
+            claims[k] = malloc(2 * sizeof(qm31));
+            outputs[k] = malloc(2 * sizeof(qm31));
+            if (gp) {
+                claims[k][0] = q_load(lden[k][0]);
+            } else {
+                claims[k][0] = q_load(lnum[k][0]);
+                claims[k][1] = q_load(lden[k][0]);
+            }
+            memcpy(outputs[k], claims[k], n_claims[k] * sizeof(qm31));
+        }
+        for (uint32_t k = 0; k < n_inst; k++)
+            if (claims[k]) ch_mix_felts(ch, claims[k], n_claims[k]);
+        /* EqEvals.generate(ood) = genEqEvals(ood[1..], 1 - ood[0]) */
+        uint64_t eq_len = ood_len ? (1ull << (ood_len - 1)) : 1;
+        qm31 *eq = malloc(sizeof(qm31) * eq_len);
+        if (ood_len == 0) {
+            eq[0] = Q_ONE;
+        } else {
+            uint32_t *yw = malloc(16 * ood_len);
+            for (uint32_t i = 0; i < ood_len; i++) q_store(ood[i], yw + 4 * i);
+            qm31 sc = q_sub(Q_ONE, ood[0]);
+            uint32_t scw[4];
+            q_store(sc, scw);
+            uint32_t *ew = malloc(16 * eq_len);
+            ora_gkr_gen_eq_evals(yw + 4, ood_len - 1, scw, ew);
+            for (uint64_t i = 0; i < eq_len; i++) eq[i] = q_load(ew + 4 * i);
+            free(ew);
+            free(yw);
+        }
+        qm31 sc_alpha = ch_draw_secure_felt(ch);
+        qm31 lambda = ch_draw_secure_felt(ch);
+        /* sum-check instances: those with claims, layer with `layer + 1` variables */
+        uint32_t n_sc = 0;
+        uint32_t *sc_inst = malloc(sizeof(uint32_t) * n_inst);
+        gkr_oracle *orc = calloc(n_inst, sizeof(gkr_oracle));
+        qm31 *sc_claims = malloc(sizeof(qm31) * n_inst);
+        for (uint32_t k = 0; k < n_inst; k++) {
+            if (!claims[k]) continue;
+            uint32_t lv = n_vars_in[k] - remaining + 1; /* layer of this instance with lv variables */
+            gkr_oracle o;
+            o.kind = lkind[k][lv];
+            o.n_vars = lv;
+            uint64_t len = 1ull << lv;
+            o.den = malloc(16 * len);
+            memcpy(o.den, lden[k][lv], 16 * len);
+            o.num = NULL;
+            if (o.kind == GKR_LOGUP_GENERIC) {
+                o.num = malloc(16 * len);
+                memcpy(o.num, lnum[k][lv], 16 * len);
+            } else if (o.kind == GKR_LOGUP_MULTIPLICITIES) {
+                o.num = malloc(4 * len);
+                memcpy(o.num, lnum[k][lv], 4 * len);
+            }
+            o.corr = Q_ONE;
+            /* randomLinearCombination(claims, lambda) = Horner */
+            qm31 cl = Q_ZERO;
+            for (uint32_t i = n_claims[k]; i > 0; i--) cl = q_add(q_mul(cl, lambda), claims[k][i - 1]);
+            orc[n_sc] = o;
+            sc_claims[n_sc] = cl;
+            sc_inst[n_sc++] = k;
+        }
+        /* sumcheck.proveBatch */
+        uint32_t sc_nv = 0;
+        for (uint32_t i = 0; i < n_sc; i++) sc_nv = orc[i].n_vars - 1 > sc_nv ? orc[i].n_vars - 1 : sc_nv;
+        for (uint32_t i = 0; i < n_sc; i++) {
+            uint32_t unused = sc_nv - (orc[i].n_vars - 1);
+            sc_claims[i] = q_mul_m(sc_claims[i], m_pow(2, unused));
+        }
+        wb_u(&wb, sc_nv);
+        qm31 *assign = malloc(sizeof(qm31) * (sc_nv + 1));
+        upoly *rp = malloc(sizeof(upoly) * (n_sc ? n_sc : 1));
+        for (uint32_t round = 0; round < sc_nv; round++) {
+            uint32_t rem = sc_nv - round;
+            for (uint32_t i = 0; i < n_sc; i++) {
+                uint32_t onv = orc[i].n_vars - 1;
+                if (rem == onv) {
+                    qm31 ev[2];
+                    gkr_eval_sums(&orc[i], eq, lambda, ev);
+                    ev[0] = q_mul(ev[0], orc[i].corr);
+                    ev[1] = q_mul(ev[1], orc[i].corr);
+                    rp[i] = gkr_correct_sum(ev[0], ev[1], sc_claims[i], ood, ood_len, onv);
+                } else {
+                    rp[i].len = 1;
+                    rp[i].c[0] = q_mul_m(sc_claims[i], m_inv(2));
+                    up_trim(&rp[i]);
+                }
+            }
+            /* combined round poly: Horner over polys with sc_alpha */
+            upoly acc = {.len = 0};
+            for (uint32_t i = n_sc; i > 0; i--) {
+                upoly sc = acc;
+                for (uint32_t k = 0; k < sc.len; k++) sc.c[k] = q_mul(sc.c[k], sc_alpha);
+                if (q_is_zero(sc_alpha)) sc.len = 0;
+                up_trim(&sc);
+                uint32_t n = sc.len > rp[i - 1].len ? sc.len : rp[i - 1].len;
+                upoly s = {.len = n};
+                for (uint32_t k = 0; k < n; k++)
+                    s.c[k] = q_add(k < sc.len ? sc.c[k] : Q_ZERO, k < rp[i - 1].len ? rp[i - 1].c[k] : Q_ZERO);
+                up_trim(&s);
+                acc = s;
+            }
+            ch_mix_felts(ch, acc.c, acc.len);
+            qm31 r = ch_draw_secure_felt(ch);
+            wb_u(&wb, acc.len);
+            for (uint32_t k = 0; k < acc.len; k++) wb_q(&wb, acc.c[k]);
+            for (uint32_t i = 0; i < n_sc; i++) sc_claims[i] = up_eval(&rp[i], r);
+            for (uint32_t i = 0; i < n_sc; i++) {
+                if (rem != orc[i].n_vars - 1) continue;
+                uint32_t onv = orc[i].n_vars - 1;
+                qm31 z0 = ood[ood_len - onv];
+                orc[i].corr = q_mul(orc[i].corr, eq1(r, z0));
+                gkr_fix(&orc[i], r);
+            }
+            assign[round] = r;
+        }
+        /* masks (tryIntoMask) and their transcript */
+        wb_u(&wb, n_sc);
+        qm31 (*masks)[2][2] = malloc(sizeof(qm31[2][2]) * (n_sc ? n_sc : 1));
+        uint32_t *mcols = malloc(sizeof(uint32_t) * (n_sc ? n_sc : 1));
+        for (uint32_t i = 0; i < n_sc; i++) {
+            gkr_oracle *o = &orc[i];
+            if (o->kind == GKR_GRAND_PRODUCT) {
+                mcols[i] = 1;
+                masks[i][0][0] = q_load(o->den);
+                masks[i][0][1] = q_load(o->den + 4);
+            } else {
+                mcols[i] = 2;
+                masks[i][0][0] = layer_num(o->kind, o->num, 0);
+                masks[i][0][1] = layer_num(o->kind, o->num, 1);
+                masks[i][1][0] = q_load(o->den);
+                masks[i][1][1] = q_load(o->den + 4);
+            }
+            qm31 flat[4];
+            for (uint32_t c2 = 0; c2 < mcols[i]; c2++) {
+                flat[2 * c2] = masks[i][c2][0];
+                flat[2 * c2 + 1] = masks[i][c2][1];
+            }
+            ch_mix_felts(ch, flat, 2 * mcols[i]);
+            wb_u(&wb, sc_inst[i]);
+            wb_u(&wb, mcols[i]);
+            for (uint32_t c2 = 0; c2 < 2 * mcols[i]; c2++) wb_q(&wb, flat[c2]);
+        }
+        qm31 challenge = ch_draw_secure_felt(ch);
+        qm31 *nood = malloc(sizeof(qm31) * (sc_nv + 1));
+        memcpy(nood, assign, sizeof(qm31) * sc_nv);
+        nood[sc_nv] = challenge;
+        free(ood);
+        ood = nood;
+        ood_len = sc_nv + 1;
+        for (uint32_t i = 0; i < n_sc; i++) {
+            uint32_t k = sc_inst[i];
+            n_claims[k] = mcols[i];
+            for (uint32_t c2 = 0; c2 < mcols[i]; c2++)
+                claims[k][c2] = q_add(q_mul(challenge, q_sub(masks[i][c2][1], masks[i][c2][0])), masks[i][c2][0]);
+        }
+        for (uint32_t i = 0; i < n_sc; i++) {
+            free(orc[i].num);
+            free(orc[i].den);
+        }
+        free(masks), free(mcols), free(assign), free(rp), free(orc), free(sc_claims), free(sc_inst), free(eq);
+    }
+    for (uint32_t k = 0; k < n_inst; k++) {
+        uint32_t n = outputs[k] ? (lkind[k][0] == GKR_GRAND_PRODUCT ? 1 : 2) : 0;
+        wb_u(&wb, n);
+        for (uint32_t i = 0; i < n; i++) wb_q(&wb, outputs[k][i]);
+    }
+    wb_u(&wb, ood_len);
+    for (uint32_t i = 0; i < ood_len; i++) wb_q(&wb, ood[i]);
+    for (uint32_t k = 0; k < n_inst; k++) {
+        wb_u(&wb, claims[k] ? n_claims[k] : 0);
+        for (uint32_t i = 0; claims[k] && i < n_claims[k]; i++) wb_q(&wb, claims[k][i]);
+        wb_u(&wb, n_vars_in[k]);
+    }
+    for (int i = 0; i < 8; i++) {
+        uint32_t d;
+        memcpy(&d, ch->digest + 4 * i, 4);
+        wb_u(&wb, d);
+    }
+    for (uint32_t k = 0; k < n_inst; k++) {
+        for (uint32_t l = 0; l <= n_vars_in[k]; l++) {
+            free(lnum[k][l]);
+            free(lden[k][l]);
+        }
+        free(lnum[k]), free(lden[k]), free(lkind[k]), free(claims[k]), free(outputs[k]);
+    }
+    free(lnum), free(lden), free(lkind), free(claims), free(outputs), free(n_claims), free(ood);
+    return wb.n;
+}
+
+size_t ora_gkr_prove(uint32_t n_inst, const int *kinds, const uint32_t *const *nums, const uint32_t *const *dens,
+                     const uint32_t *n_vars_in, const uint8_t digest_in[32], uint32_t *out, size_t cap) {
+    channel ch;
+    memcpy(ch.digest, digest_in, 32);
+    ch.n_draws = 0;
+    return gkr_prove_on(&ch, n_inst, kinds, nums, dens, n_vars_in, out, cap);
+}
+
+/* LogUp workload (XOR-lookup AIR) from trace to quotient commitment, the same sequence as the
+ * device pipeline: trace + multiplicities (zero-padded to n rows) -> commit [a, b, c, m] ->
+ * z, alpha -> denominators -> cumulative columns (claimed sums mixed) -> commit the 8 coordinate
+ * columns -> batch GKR proof -> OODS point, sampled values of the 12 polynomials -> quotients at
+ * the OODS point -> quotient commitment.  Output words: 3 roots, 2 claimed sums, 12 sampled
+ * values, GKR word count and words, final channel digest.  Returns the word count. */
+size_t ora_logup_xor_pipeline(uint32_t log_n, uint64_t seed, uint32_t *out, size_t cap) {
+    const uint32_t n = log_n, B = 1, L = n + 1;
+    const uint64_t N = 1ull << n;
+    channel ch;
+    memset(ch.digest, 0, 32);
+    ch.n_draws = 0;
+    {
+        uint32_t st[3] = {n, (uint32_t)seed, (uint32_t)(seed >> 32)};
+        ch_mix_u32s(&ch, st, 3);
+    }
+    uint32_t *a = malloc(4 * N), *b = malloc(4 * N), *c = malloc(4 * N), *mult = malloc(4u << 16);
+    ora_xor_trace(n, seed, a, b, c, mult);
+    uint32_t *mpad = calloc(N, 4), *negm = malloc(4 * N);
+    for (uint64_t i = 0; i < N; i++) {
+        mpad[i] = i < (1u << 16) ? mult[i] : 0;
+        negm[i] = m_neg(mpad[i]);
+    }
+    /* main commitment */
+    ctree tm, ti;
+    {
+        uint32_t **cf = malloc(sizeof(uint32_t *) * 4);
+        const uint32_t *src[4] = {a, b, c, mpad};
+        uint32_t logs[4] = {n, n, n, n};
+        get_twiddles(n);
+        for (int k = 0; k < 4; k++) {
+            cf[k] = malloc(4 * N);
+            memcpy(cf[k], src[k], 4 * N);
+            circle_interpolate(cf[k], n);
+        }
+        commit_coeffs(&tm, 4, cf, logs, B, &ch);
+    }
+    qm31 z = ch_draw_secure_felt(&ch), alpha = ch_draw_secure_felt(&ch);
+    uint32_t zw[4], aw[4];
+    q_store(z, zw);
+    q_store(alpha, aw);
+    uint32_t *dl = malloc(16 * N), *dt = malloc(16 * N), *sl = malloc(16 * N), *st = malloc(16 * N);
+    ora_logup_denominators(a, b, c, N, zw, aw, dl);
+    ora_logup_denominators(NULL, NULL, NULL, N, zw, aw, dt);
+    uint32_t clw[4], ctw[4];
+    int bad = ora_logup_cumulative(2, NULL, dl, N, sl, clw) | ora_logup_cumulative(1, negm, dt, N, st, ctw);
+    qm31 cs[2] = {q_load(clw), q_load(ctw)};
+    if (bad || !q_is_zero(q_add(cs[0], cs[1]))) return 0;
+    ch_mix_felts(&ch, cs, 2);
+    {
+        uint32_t **cf = malloc(sizeof(uint32_t *) * 8);
+        uint32_t logs[8];
+        for (int k = 0; k < 8; k++) {
+            const uint32_t *s2 = k < 4 ? sl : st;
+            cf[k] = malloc(4 * N);
+            for (uint64_t i = 0; i < N; i++) cf[k][i] = s2[4 * i + (k & 3)];
+            circle_interpolate(cf[k], n);
+            logs[k] = n;
+        }
+        commit_coeffs(&ti, 8, cf, logs, B, &ch);
+    }
+    /* GKR */
+    int kinds[2] = {GKR_LOGUP_SINGLES, GKR_LOGUP_MULTIPLICITIES};
+    const uint32_t *nums[2] = {NULL, negm}, *dens[2] = {dl, dt};
+    uint32_t nvs[2] = {n, n};
+    channel ch_probe = ch;
+    size_t gn = gkr_prove_on(&ch_probe, 2, kinds, nums, dens, nvs, NULL, 0);
+    uint32_t *gw = malloc(4 * gn);
+    gkr_prove_on(&ch, 2, kinds, nums, dens, nvs, gw, gn);
+    /* OODS point and sampled values */
+    qm31 tt = ch_draw_secure_felt(&ch);
+    qm31 t2_ = q_square(tt);
+    qm31 inv1 = q_inv(q_add(t2_, Q_ONE));
+    qpt oods = {q_mul(q_sub(Q_ONE, t2_), inv1), q_mul(q_add(tt, tt), inv1)};
+    qm31 sv[12];
+    {
+        qm31 *w = eval_weights(oods, n);
+        for (int k = 0; k < 12; k++) sv[k] = dot_weights(k < 4 ? tm.coeffs[k] : ti.coeffs[k - 4], w, N);
+        free(w);
+    }
+    ch_mix_felts(&ch, sv, 12);
+    /* quotients at the OODS point */
+    qm31 qa = ch_draw_secure_felt(&ch);
+    qm31 lc[12], sum_a = Q_ZERO, sum_b = Q_ZERO, rp = Q_ONE;
+    qm31 cc = q_sub(q_conj(oods.y), oods.y);
+    for (int k = 0; k < 12; k++) {
+        qm31 av = q_sub(q_conj(sv[k]), sv[k]);
+        qm31 bv = q_sub(q_mul(sv[k], cc), q_mul(av, oods.y));
+        lc[k] = q_mul(rp, cc);
+        sum_a = q_add(sum_a, q_mul(rp, av));
+        sum_b = q_add(sum_b, q_mul(rp, bv));
+        rp = q_mul(rp, qa);
+    }
+    const uint64_t LN = 1ull << L;
+    cm31 det = c_sub(c_mul(oods.x.c0, oods.y.c1), c_mul(oods.y.c0, oods.x.c1));
+    qm31 *quot = malloc(sizeof(qm31) * LN);
+#pragma omp parallel for schedule(static)
+    for (int64_t row = 0; row < (int64_t)LN; row++) {
+        pt dp = domain_at(L, bitrev((uint64_t)row, L));
+        qm31 num = Q_ZERO;
+        for (int k = 0; k < 12; k++) num = q_add(num, q_mul_m(lc[k], (k < 4 ? tm.lde[k] : ti.lde[k - 4])[row]));
+        num = q_sub(num, q_add(q_mul_m(sum_a, dp.y), sum_b));
+        cm31 den = c_add(c_sub(det, c_mul_m(oods.y.c1, dp.x)), c_mul_m(oods.x.c1, dp.y));
+        quot[row] = q_mul_c(num, c_inv(den));
+    }
+    ora_tree *qt = commit_qm31_column(quot, L, NULL);
+    uint8_t r3[32];
+    ora_tree_root(qt, r3);
+    ch_mix_root(&ch, r3);
+    /* output */
+    wbuf wb = {out, 0, cap};
+    uint8_t r1[32], r2[32];
+    ora_tree_root(tm.tree, r1);
+    ora_tree_root(ti.tree, r2);
+    const uint8_t *roots[3] = {r1, r2, r3};
+    for (int r = 0; r < 3; r++)
+        for (int i = 0; i < 8; i++) {
+            uint32_t v;
+            memcpy(&v, roots[r] + 4 * i, 4);
+            wb_u(&wb, v);
+        }
+    wb_q(&wb, cs[0]);
+    wb_q(&wb, cs[1]);
+    for (int k = 0; k < 12; k++) wb_q(&wb, sv[k]);
+    wb_u(&wb, (uint32_t)gn);
+    for (size_t i = 0; i < gn; i++) wb_u(&wb, gw[i]);
+    for (int i = 0; i < 8; i++) {
+        uint32_t v;
+        memcpy(&v, ch.digest + 4 * i, 4);
+        wb_u(&wb, v);
+    }
+    ora_tree_free(qt);
+    free(quot), free(gw), free(dl), free(dt), free(sl), free(st), free(a), free(b), free(c), free(mult), free(mpad),
+        free(negm);
+    ctree_free(&tm);
+    ctree_free(&ti);
+    return wb.n;
+}

@@ -94,6 +94,20 @@ int ora_prove_wide_fibonacci(const ora_widefib_params *p, uint8_t *out, size_t c
 /* Stage timings of the last ora_prove_wide_fibonacci call (milliseconds). */
 void ora_last_timings(double out[8]);
 
+/* LogUp / GKR (see stwo_oracle.c). */
+void ora_xor_trace(uint32_t log_size, uint64_t seed, uint32_t *a, uint32_t *b, uint32_t *c, uint32_t *mult);
+void ora_logup_denominators(const uint32_t *a, const uint32_t *b, const uint32_t *c, uint64_t n, const uint32_t z[4],
+                            const uint32_t alpha[4], uint32_t *den);
+int ora_logup_cumulative(int num_kind, const uint32_t *num, const uint32_t *den, uint64_t n, uint32_t *sums,
+                         uint32_t claimed[4]);
+void ora_gkr_gen_eq_evals(const uint32_t *y, uint32_t n, const uint32_t scale[4], uint32_t *out);
+void ora_gkr_next_layer(int kind, const uint32_t *num, const uint32_t *den, uint32_t n_vars, uint32_t *out_num,
+                        uint32_t *out_den);
+void ora_mle_fix_first(int base, const uint32_t *in, uint32_t n_vars, const uint32_t assignment[4], uint32_t *out);
+size_t ora_gkr_prove(uint32_t n_inst, const int *kinds, const uint32_t *const *nums, const uint32_t *const *dens,
+                     const uint32_t *n_vars, const uint8_t digest_in[32], uint32_t *out, size_t cap);
+size_t ora_logup_xor_pipeline(uint32_t log_n, uint64_t seed, uint32_t *out, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

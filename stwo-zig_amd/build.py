@@ -15,7 +15,9 @@ OBJ = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libstwo_rocm.so")
 SOURCES = ["runtime.hip", "kernels_fft.hip", "kernels_merkle.hip", "kernels_fri.hip", "kernels_prove.hip",
-           "kernels_misc.hip", "kernels_chan.hip", "prover.hip", "capi.hip"]
+           "kernels_misc.hip", "kernels_chan.hip", "kernels_logup.hip", "gkr.hip", "logup_pipeline.hip",
+           "prover.hip",
+           "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("STWO_ROCM_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

@@ -583,6 +583,110 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
     });
 }
 
+// ---------------------------------------------------------------- LogUp / GKR
+int stwo_rocm_xor_trace(stwo_rocm_ctx* ctx, uint32_t log_size, uint64_t seed, uint32_t* d_a, uint32_t* d_b,
+                        uint32_t* d_c, uint32_t* d_mult) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(d_a && d_b && d_c && d_mult, "null argument");
+        launch_xor_trace(c, log_size, seed, d_a, d_b, d_c, d_mult);
+        c.sync();
+    });
+}
+
+int stwo_rocm_logup_denominators(stwo_rocm_ctx* ctx, const uint32_t* d_a, const uint32_t* d_b, const uint32_t* d_c,
+                                 uint64_t n, const uint32_t z[4], const uint32_t alpha[4], uint32_t* d_den) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(z && alpha && d_den && n > 0, "null argument");
+        SR_CHECK((d_a && d_b && d_c) || (!d_a && !d_b && !d_c), "lookup columns: all or none");
+        launch_logup_denominators(c, d_a, d_b, d_c, n, qload(z), qload(alpha), d_den);
+        c.sync();
+    });
+}
+
+int stwo_rocm_logup_cumulative(stwo_rocm_ctx* ctx, int num_kind, const uint32_t* d_num, const uint32_t* d_den,
+                               uint64_t n, uint32_t* d_sums, uint32_t claimed[4]) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(d_den && d_sums && claimed, "null argument");
+        qstore(logup_cumulative(c, num_kind, d_num, d_den, n, d_sums), claimed);
+    });
+}
+
+int stwo_rocm_gkr_gen_eq_evals(stwo_rocm_ctx* ctx, const uint32_t* y, uint32_t n, const uint32_t scale[4],
+                               uint32_t* d_out) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(scale && d_out && (y || n == 0), "null argument");
+        DBuf dy = upload(c, y ? y : scale, y ? 4 * (size_t)n : 4);
+        launch_gkr_eq_evals(c, dy.as<uint32_t>(), n, qload(scale), d_out);
+        c.sync();
+    });
+}
+
+int stwo_rocm_gkr_next_layer(stwo_rocm_ctx* ctx, int kind, const uint32_t* d_num, const uint32_t* d_den,
+                             uint32_t n_vars, uint32_t* d_out_num, uint32_t* d_out_den) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(d_den && d_out_den, "null argument");
+        launch_gkr_next_layer(c, kind, d_num, d_den, n_vars, d_out_num, d_out_den);
+        c.sync();
+    });
+}
+
+int stwo_rocm_gkr_sum_as_poly_in_first_variable(stwo_rocm_ctx* ctx, int kind, const uint32_t* d_eq,
+                                                const uint32_t* d_num, const uint32_t* d_den, uint32_t n_vars,
+                                                const uint32_t lambda[4], const uint32_t claim[4], const uint32_t* y,
+                                                uint32_t y_len, const uint32_t correction[4], uint32_t coeffs_out[16],
+                                                uint32_t* n_coeffs) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(d_eq && d_den && lambda && claim && y && correction && coeffs_out && n_coeffs, "null argument");
+        SR_CHECK(kind >= 0 && kind <= 3, "gkr: layer kind");
+        SR_CHECK(kind == GKR_GRAND_PRODUCT || kind == GKR_LOGUP_SINGLES || d_num, "gkr: numerators");
+        std::vector<qm31> yy(y_len);
+        for (uint32_t i = 0; i < y_len; i++) yy[i] = qload(y + 4 * i);
+        std::vector<qm31> cf = gkr_sum_as_poly(c, kind, d_eq, d_num, d_den, n_vars, qload(lambda), qload(claim), yy,
+                                               qload(correction));
+        SR_CHECK(cf.size() <= 4, "gkr: round polynomial degree");
+        *n_coeffs = (uint32_t)cf.size();
+        for (size_t i = 0; i < cf.size(); i++) qstore(cf[i], coeffs_out + 4 * i);
+    });
+}
+
+int stwo_rocm_mle_fix_first_variable(stwo_rocm_ctx* ctx, int base_field, const uint32_t* d_in, uint32_t n_vars,
+                                     const uint32_t r[4], uint32_t* d_out) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(d_in && r && d_out, "null argument");
+        launch_mle_fix_first(c, base_field != 0, d_in, n_vars, qload(r), d_out);
+        c.sync();
+    });
+}
+
+int stwo_rocm_gkr_prove_batch(stwo_rocm_ctx* ctx, uint32_t n_instances, const int* kinds,
+                              const uint32_t* const* d_nums, const uint32_t* const* d_dens, const uint32_t* n_vars,
+                              const uint8_t channel_digest[32], uint32_t* words_out, uint64_t cap,
+                              uint64_t* n_words) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(kinds && d_nums && d_dens && n_vars && channel_digest && n_words && n_instances > 0,
+                 "null argument");
+        std::vector<GkrInstance> inst(n_instances);
+        for (uint32_t k = 0; k < n_instances; k++) inst[k] = {kinds[k], d_nums[k], d_dens[k], n_vars[k]};
+        Channel ch;
+        memcpy(ch.digest, channel_digest, 32);
+        std::vector<uint32_t> w = gkr_prove_batch(c, ch, inst);
+        *n_words = w.size();
+        SR_CHECK(words_out && w.size() <= cap, "gkr: output buffer too small");
+        memcpy(words_out, w.data(), 4 * w.size());
+    });
+}
+
+int stwo_rocm_logup_xor_pipeline(stwo_rocm_ctx* ctx, uint32_t log_n, uint64_t seed, uint32_t* words_out,
+                                 uint64_t cap, uint64_t* n_words, double* total_ms) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(n_words, "null argument");
+        std::vector<uint32_t> w = logup_xor_pipeline(c, log_n, seed, total_ms);
+        *n_words = w.size();
+        SR_CHECK(words_out && w.size() <= cap, "logup: output buffer too small");
+        memcpy(words_out, w.data(), 4 * w.size());
+    });
+}
+
 int stwo_rocm_profile_enable(stwo_rocm_ctx* ctx, int enable) {
     return guarded(ctx, [&](Ctx& c) {
         c.sync();

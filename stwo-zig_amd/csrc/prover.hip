@@ -153,39 +153,10 @@ struct JsonOut {
 
 // ---------------------------------------------------------------- prove
 // Owns resident trees for the duration of one proof (freed on every exit path).
-struct TreeOwner {
-    Ctx& c;
-    std::vector<DeviceTree*> trees;
-    explicit TreeOwner(Ctx& cc) : c(cc) {}
-    ~TreeOwner() {
-        for (auto t : trees) tree_free(c, t);
-    }
-    DeviceTree* add(DeviceTree* t) {
-        trees.push_back(t);
-        return t;
-    }
-    DeviceTree* make(uint32_t max_log) {
-        DeviceTree* t = new DeviceTree();
-        t->max_log = max_log;
-        trees.push_back(t);
-        t->layers = (uint8_t*)c.alloc(((2ull << max_log) - 1) * 32);
-        return t;
-    }
-};
-// Committed tree: LDE columns, retained coefficients and the resident Merkle tree.
-struct Committed {
-    std::vector<uint32_t*> lde;
-    std::vector<uint32_t> lde_logs;
-    std::vector<const uint32_t*> coeffs;
-    std::vector<uint32_t> coeff_logs;
-    DBuf lde_buf;
-    DeviceTree* tree = nullptr;
-};
-
 // LDE of coefficient columns (pointer list) by `blowup` and Merkle commit
 // (commit_polys.zig:10-67; preparation.zig:113-210).
-static void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out,
-                                const std::vector<const uint32_t*>& coeffs, uint32_t coeff_log, uint32_t blowup) {
+void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
+                         uint32_t coeff_log, uint32_t blowup) {
     uint32_t n = (uint32_t)coeffs.size();
     uint32_t m = coeff_log + blowup;
     uint64_t N = 1ull << m;

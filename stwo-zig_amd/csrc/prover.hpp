@@ -3,6 +3,7 @@
 #pragma once
 #include <functional>
 #include <string>
+#include <vector>
 
 #include "runtime.hpp"
 
@@ -22,6 +23,41 @@ struct Channel {
     qm31 draw_secure_felt();
     void pow_prefix(uint32_t bits, uint8_t out[32]) const;
 };
+
+// Owns the device Merkle trees of one proof (freed with it).
+struct TreeOwner {
+    Ctx& c;
+    std::vector<DeviceTree*> trees;
+    explicit TreeOwner(Ctx& cc) : c(cc) {}
+    ~TreeOwner() {
+        for (auto t : trees) tree_free(c, t);
+    }
+    DeviceTree* add(DeviceTree* t) {
+        trees.push_back(t);
+        return t;
+    }
+    DeviceTree* make(uint32_t max_log) {
+        DeviceTree* t = new DeviceTree();
+        t->max_log = max_log;
+        trees.push_back(t);
+        t->layers = (uint8_t*)c.alloc(((2ull << max_log) - 1) * 32);
+        return t;
+    }
+};
+// Committed tree: LDE columns, retained coefficients and the resident Merkle tree.
+struct Committed {
+    std::vector<uint32_t*> lde;
+    std::vector<uint32_t> lde_logs;
+    std::vector<const uint32_t*> coeffs;
+    std::vector<uint32_t> coeff_logs;
+    DBuf lde_buf;
+    DeviceTree* tree = nullptr;
+};
+
+// LDE of coefficient columns (pointer list) by `blowup` and Merkle commit
+// (commit_polys.zig:10-67; preparation.zig:113-210).
+void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
+                         uint32_t coeff_log, uint32_t blowup);
 
 struct ProveParams {
     uint32_t log_n_rows;
@@ -48,5 +84,17 @@ struct ProveStats {
 // with ProveParams.trace_mode / trace_seed inside the proof).
 void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trace, std::string& proof,
                           ProveStats* stats);
+
+// GKR batch proof of the given input layers on `ch` (device-resident layers);
+// returns the proof word stream described in gkr.hip.
+std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<GkrInstance>& inst);
+// One sum-check round polynomial of a layer (eq table on the device, y = eq point).
+std::vector<qm31> gkr_sum_as_poly(Ctx& c, int kind, const uint32_t* d_eq, const uint32_t* d_num,
+                                  const uint32_t* d_den, uint32_t n_vars, qm31 lambda, qm31 claim,
+                                  const std::vector<qm31>& y, qm31 correction);
+
+// LogUp workload (XOR-lookup AIR) from trace to quotient commitment; returns the word
+// stream described in logup_pipeline.hip.  total_ms (optional): host wall time.
+std::vector<uint32_t> logup_xor_pipeline(Ctx& c, uint32_t log_n, uint64_t seed, double* total_ms);
 
 }  // namespace sr

@@ -199,6 +199,33 @@ struct FriTailArgs {
     B2State leaf_seed, node_seed;  // set by the launcher
 };
 void launch_fri_tail(Ctx& c, FriTailArgs& a);
+
+// LogUp / GKR (kernels_logup.hip).  Layer kinds of gkr_circuit.zig:79-92.
+enum GkrKind { GKR_GRAND_PRODUCT = 0, GKR_LOGUP_GENERIC = 1, GKR_LOGUP_MULTIPLICITIES = 2, GKR_LOGUP_SINGLES = 3 };
+void launch_fill_u32(Ctx& c, uint32_t* d, uint64_t n, uint32_t v);
+void launch_zero_u64(Ctx& c, unsigned long long* d, uint32_t n);
+void launch_xor_trace(Ctx& c, uint32_t log_size, uint64_t seed, uint32_t* a, uint32_t* b, uint32_t* cc,
+                      uint32_t* mult /* 2^16 words */);
+// den = z - (a + b alpha + c alpha^2) (QM31 AoS); table rows (i >> 8, i & 255, xor) when a == nullptr.
+void launch_logup_denominators(Ctx& c, const uint32_t* a, const uint32_t* b, const uint32_t* cc, uint64_t n, qm31 z,
+                               qm31 alpha, uint32_t* den);
+// Cumulative column S_i = sum_{j<=i} num_j / den_j (num_kind 0 QM31, 1 M31, 2 ones); returns the claimed sum
+// (synchronises); throws on a zero denominator.
+qm31 logup_cumulative(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n, uint32_t* sums);
+void launch_gkr_eq_evals(Ctx& c, const uint32_t* d_y, uint32_t n, qm31 scale, uint32_t* d_out);
+void launch_gkr_next_layer(Ctx& c, int kind, const uint32_t* d_num, const uint32_t* d_den, uint32_t n_vars,
+                           uint32_t* d_out_num, uint32_t* d_out_den);
+// Adds the 64-bit coordinate sums of (f(0), f(2)) into d_acc8[0..8).
+void launch_gkr_sums(Ctx& c, int kind, const uint32_t* d_eq, const uint32_t* d_num, const uint32_t* d_den,
+                     uint32_t n_vars, qm31 lambda, unsigned long long* d_acc8);
+void launch_mle_fix_first(Ctx& c, bool base, const uint32_t* d_in, uint32_t n_vars, qm31 r, uint32_t* d_out);
+void launch_logup_table_cols(Ctx& c, const uint32_t* mult16, uint64_t n, uint32_t* mpad, uint32_t* negm);
+struct GkrInstance {
+    int kind;
+    const uint32_t* d_num;  // QM31 AoS (generic), M31 (multiplicities) or null
+    const uint32_t* d_den;  // QM31 AoS (values for grand product)
+    uint32_t n_vars;
+};
 // Quotients (row_evaluation.zig).
 struct QuotientBatchDev {
     cm31 det, pix, piy;  // sample point components

@@ -117,6 +117,20 @@ _SIGNATURES = [
 ]
 
 _SIGNATURES += [
+    ("stwo_rocm_xor_trace", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, _vp, _vp]),
+    ("stwo_rocm_logup_denominators", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_uint64, _u32p, _u32p, _vp]),
+    ("stwo_rocm_logup_cumulative", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64, _vp, _u32p]),
+    ("stwo_rocm_gkr_gen_eq_evals", ctypes.c_int, [_vp, _u32p, ctypes.c_uint32, _u32p, _vp]),
+    ("stwo_rocm_gkr_next_layer", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    ("stwo_rocm_gkr_sum_as_poly_in_first_variable", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_uint32,
+                                                                    _u32p, _u32p, _u32p, ctypes.c_uint32, _u32p, _u32p,
+                                                                    _u32p]),
+    ("stwo_rocm_mle_fix_first_variable", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint32, _u32p, _vp]),
+    ("stwo_rocm_gkr_prove_batch", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int),
+                                                  ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, ctypes.c_char_p,
+                                                  _u32p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    ("stwo_rocm_logup_xor_pipeline", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint64, _u32p, ctypes.c_uint64,
+                                                     ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]),
     ("stwo_rocm_profile_enable", ctypes.c_int, [_vp, ctypes.c_int]),
     ("stwo_rocm_profile_select", ctypes.c_int, [_vp, ctypes.c_char_p]),
     ("stwo_rocm_profile_read", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t,
@@ -409,6 +423,88 @@ class Context:
                 continue
             self._check(rc)
             return buf.raw[: n.value], stats.as_dict()
+
+
+    # ---- LogUp / GKR (kernels_logup.hip, gkr.hip)
+    def xor_trace(self, log_size, seed, d_a, d_b, d_c, d_mult):
+        self._check(self.lib.stwo_rocm_xor_trace(self.h, log_size, seed, int(d_a), int(d_b), int(d_c), int(d_mult)))
+
+    def logup_denominators(self, d_a, d_b, d_c, n, z, alpha, d_den):
+        zz, pz = _u32(z)
+        al, pal = _u32(alpha)
+        self._check(self.lib.stwo_rocm_logup_denominators(
+            self.h, int(d_a) if d_a is not None else None, int(d_b) if d_b is not None else None,
+            int(d_c) if d_c is not None else None, n, pz, pal, int(d_den)))
+
+    def logup_cumulative(self, num_kind, d_num, d_den, n, d_sums):
+        cl, pcl = _u32(np.zeros(4))
+        self._check(self.lib.stwo_rocm_logup_cumulative(self.h, num_kind, int(d_num) if d_num is not None else None,
+                                                        int(d_den), n, int(d_sums), pcl))
+        return cl
+
+    def gkr_gen_eq_evals(self, y, scale, d_out):
+        y = np.ascontiguousarray(y, dtype=np.uint32)
+        yy, py = _u32(y if y.size else np.zeros(4))
+        sc, psc = _u32(scale)
+        self._check(self.lib.stwo_rocm_gkr_gen_eq_evals(self.h, py, y.size // 4, psc, int(d_out)))
+
+    def gkr_next_layer(self, kind, d_num, d_den, n_vars, d_out_num, d_out_den):
+        self._check(self.lib.stwo_rocm_gkr_next_layer(self.h, kind, int(d_num) if d_num is not None else None,
+                                                      int(d_den), n_vars,
+                                                      int(d_out_num) if d_out_num is not None else None,
+                                                      int(d_out_den)))
+
+    def gkr_sum_as_poly_in_first_variable(self, kind, d_eq, d_num, d_den, n_vars, lam, claim, y, correction):
+        la, pla = _u32(lam)
+        cl, pcl = _u32(claim)
+        yy, py = _u32(y)
+        co, pco = _u32(correction)
+        out, pout = _u32(np.zeros(16))
+        n, pn = _u32(np.zeros(1))
+        self._check(self.lib.stwo_rocm_gkr_sum_as_poly_in_first_variable(
+            self.h, kind, int(d_eq), int(d_num) if d_num is not None else None, int(d_den), n_vars, pla, pcl, py,
+            yy.size // 4, pco, pout, pn))
+        return out[: 4 * int(n[0])]
+
+    def mle_fix_first_variable(self, base_field, d_in, n_vars, r, d_out):
+        rr, pr = _u32(r)
+        self._check(self.lib.stwo_rocm_mle_fix_first_variable(self.h, 1 if base_field else 0, int(d_in), n_vars, pr,
+                                                              int(d_out)))
+
+    def gkr_prove_batch(self, instances, digest=bytes(32)):
+        """instances: list of (kind, d_num or None, d_den, n_vars).  Returns the proof word stream."""
+        k = len(instances)
+        kinds = (ctypes.c_int * k)(*[i[0] for i in instances])
+        nums = (_vp * k)(*[int(i[1]) if i[1] is not None else None for i in instances])
+        dens = (_vp * k)(*[int(i[2]) for i in instances])
+        nv, pnv = _u32([i[3] for i in instances])
+        cap = 1 << 16
+        n = ctypes.c_uint64()
+        while True:
+            out, pout = _u32(np.zeros(cap))
+            rc = self.lib.stwo_rocm_gkr_prove_batch(self.h, k, kinds, nums, dens, pnv, bytes(digest), pout, cap,
+                                                    ctypes.byref(n))
+            if rc == 1 and n.value > cap:
+                cap = n.value
+                continue
+            self._check(rc)
+            return out[: n.value].copy()
+
+
+    def logup_xor_pipeline(self, log_n, seed):
+        """LogUp workload word stream and the call's host wall time (ms)."""
+        cap = 1 << 16
+        n = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        while True:
+            out, pout = _u32(np.zeros(cap))
+            rc = self.lib.stwo_rocm_logup_xor_pipeline(self.h, log_n, seed, pout, cap, ctypes.byref(n),
+                                                       ctypes.byref(ms))
+            if rc == 1 and n.value > cap:
+                cap = n.value
+                continue
+            self._check(rc)
+            return out[: n.value].copy(), ms.value
 
 
 # Protocol presets (src/prover/native/config.zig:139-159).

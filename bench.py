@@ -27,6 +27,8 @@ import statistics
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "stwo-zig_amd"))
 
@@ -47,6 +49,10 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain streaming kernel of known size (HBM counter calibration)")
+    ap.add_argument("--workload", choices=["widefib", "logup", "batched", "serving"], default="widefib",
+                    help="widefib: the 2^20x100 proof (default); logup: XOR-lookup LogUp workload at 2^18; "
+                         "batched: 64 uneven columns LDE + Merkle + 3 FRI folds as one hipGraph; "
+                         "serving: 256 Poisson requests (wide-Fib and LogUp mix), 1 slot vs continuous batching")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -56,6 +62,15 @@ def main():
 
     if args.copy_probe:
         copy_probe(stwo_rocm)
+        return
+    if args.workload == "logup":
+        bench_logup(stwo_rocm, args)
+        return
+    if args.workload == "batched":
+        bench_batched(stwo_rocm, args)
+        return
+    if args.workload == "serving":
+        bench_serving(stwo_rocm, args)
         return
     n, W = args.log_n_rows, args.sequence_len
     proto = dict(stwo_rocm.PROTOCOLS[args.protocol])
@@ -152,6 +167,205 @@ def main():
     }
     if cpu:
         line["speedup_vs_cpu_baseline"] = round(cpu["value"] / median_ms, 1)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def _dominant(prof, steps):
+    """(class, per-launch ms, per-launch algorithmic bytes, kernels table) of a profile."""
+    dom, dv = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    kernels = {k: {"launches_per_step": v["launches"] / steps, "ms_per_step": v["ms"] / steps,
+                   "GBs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+    return dom, dv["ms"] / dv["launches"], dv["bytes"] / dv["launches"], kernels
+
+
+def _roofline(dom, avg_ms, bytes_per_launch, traffic):
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
+            "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+
+
+def bench_logup(stwo_rocm, args):
+    """XOR-lookup LogUp workload, log 18: trace -> commit -> cumulative columns -> commit -> GKR ->
+    sampled values -> quotients -> commit, one call; checked word for word against the oracle."""
+    log_n, seed = 18, 0x5EED
+    ctx = stwo_rocm.Context(0)
+    ident = ctx.identity()
+    for _ in range(max(args.warmup, 1)):
+        words, _ = ctx.logup_xor_pipeline(log_n, seed)
+    prof_steps = 2
+    ctx.profile_select(None)
+    ctx.profile_enable(True)
+    for _ in range(prof_steps):
+        ctx.logup_xor_pipeline(log_n, seed)
+    prof = ctx.profile_read()
+    ctx.profile_enable(False)
+    dom, _, _, kernels = _dominant(prof, prof_steps)
+    ctx.profile_select(dom)
+    ctx.profile_enable(True)
+    times = []
+    shas = set()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        words, ms = ctx.logup_xor_pipeline(log_n, seed)
+        times.append(ms)
+        shas.add(hashlib.sha256(words.tobytes()).hexdigest())
+    wall_ms = (time.perf_counter() - t_start) * 1e3
+    dprof = ctx.profile_read()
+    ctx.profile_enable(False)
+    ctx.profile_select(None)
+    if len(shas) != 1:
+        raise SystemExit("non-deterministic LogUp workload")
+    dv = dprof[dom]
+    cpu, ref_sha = None, None
+    if args.cpu_baseline == "on":
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        t0 = time.perf_counter()
+        ref = oracle.logup_xor_pipeline(log_n, seed)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        ref_sha = hashlib.sha256(ref.tobytes()).hexdigest()
+        if ref_sha not in shas:
+            raise SystemExit("LogUp workload differs from the oracle")
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+               "sample": "1 full run of the same workload (XOR LogUp 2^18) by the C oracle"}
+    med = statistics.median(times)
+    line = {"metric": "LogUp XOR-lookup 2^18 workload ms (trace, commits, cumulative columns, GKR, quotients)",
+            "value": round(med, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(wall_ms / args.steps, 3), "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u32 (exact M31/QM31)",
+            "data": "synthetic: seeded 8-bit a, b (splitmix64), c = a xor b",
+            "config": {"workload": "xor_logup log_n=18 (lookups 2^18, table 2^16 padded), blowup 2", "device": ident},
+            "roofline": _roofline(dom, dv["ms"] / dv["launches"], dv["bytes"] / dv["launches"], profiled_traffic()),
+            "cpu_baseline": cpu, "result_sha256": shas.pop(), "bit_exact_vs_oracle": ref_sha is not None,
+            "kernels": kernels}
+    if cpu:
+        line["speedup_vs_cpu_baseline"] = round(cpu["value"] / med, 1)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def bench_batched(stwo_rocm, args):
+    """64 seeded columns, log sizes 14..22, LDE at blowup 4, one lifted Merkle tree, 3 FRI fold_line
+    rounds on a log-22 line; autotuned launch shapes; one hipGraph replay per step."""
+    spec = (64, 14, 22, 2, 22, 3, 1)
+    ctx = stwo_rocm.Context(0)
+    ident = ctx.identity()
+    bp = ctx.batched(*spec)
+    tile, levels, table = bp.autotune(reps=2)
+    bp.run(max(args.warmup, 1), graph=True)
+    eager_words, eager_ms = bp.run(3, graph=False)
+    # per-kernel profile of eager runs (a graph carries no events)
+    prof_steps = 2
+    ctx.profile_select(None)
+    ctx.profile_enable(True)
+    bp.run(prof_steps, graph=False)
+    prof = ctx.profile_read()
+    ctx.profile_enable(False)
+    dom, avg_ms, bpl, kernels = _dominant(prof, prof_steps)
+    times = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        words, ms = bp.run(1, graph=True)
+        times.append(ms)
+    wall_ms = (time.perf_counter() - t_start) * 1e3
+    if not np.array_equal(words, eager_words):
+        raise SystemExit("graph and eager results differ")
+    cpu, ok = None, None
+    if args.cpu_baseline == "on":
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        t0 = time.perf_counter()
+        ref = oracle.batched_pipeline(*spec)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        ok = bool(np.array_equal(ref, words))
+        if not ok:
+            raise SystemExit("batched pipeline differs from the oracle")
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
+               "sample": "1 full run of the same pipeline (64 columns, logs 14..22) by the C oracle"}
+    bp.close()
+    med = statistics.median(times)
+    line = {"metric": "batched primitive pipeline ms (64 cols log 14..22, LDE x4, lifted Merkle, 3 FRI folds)",
+            "value": round(med, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(wall_ms / args.steps, 3), "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u32 (exact M31/QM31)",
+            "data": "synthetic: seeded uniform M31 columns and QM31 line (splitmix64)",
+            "config": {"workload": "batched: 64 columns, log sizes 14..22, log_blowup 2, fri_log 22, 3 folds",
+                       "device": ident, "tuning": {"fft_tile_log": tile, "merkle_levels": levels},
+                       "autotune_table": table},
+            "eager_ms": round(eager_ms, 3), "graph_speedup_vs_eager": round(eager_ms / med, 3),
+            "roofline": _roofline(dom, avg_ms, bpl, profiled_traffic()),
+            "cpu_baseline": cpu, "bit_exact_vs_oracle": ok, "kernels": kernels}
+    if cpu:
+        line["speedup_vs_cpu_baseline"] = round(cpu["value"] / med, 1)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def serving_requests(n=256, rate_per_s=1500.0, seed=2024):
+    """Seeded request stream: Poisson arrivals at `rate_per_s`; 4 of 5 requests are wide-Fibonacci
+    proofs (log_n_rows 14..20, sequence_len 16..100, seeded uniform trace), 1 of 5 the XOR-lookup
+    LogUp workload (log 16..18)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for i in range(n):
+        t += rng.exponential(1000.0 / rate_per_s)
+        if rng.random() < 0.8:
+            jobs.append((0, int(rng.integers(14, 21)), int(rng.integers(16, 101)), int(rng.integers(1, 1 << 40)), t))
+        else:
+            jobs.append((1, int(rng.integers(16, 19)), 0, int(rng.integers(1, 1 << 40)), t))
+    return jobs
+
+
+def bench_serving(stwo_rocm, args):
+    """256 requests with Poisson arrivals: the one-slot service (the unbatched GPU path: one proof
+    at a time) against continuous batching over several slots; proofs/s (value), p50/p99 latency."""
+    jobs = serving_requests()
+    ctx = stwo_rocm.Context(0)
+    ident = ctx.identity()
+    stwo_rocm.serve(ctx, jobs[:8], n_slots=2)  # warm-up: page in kernels and allocators
+    runs = {}
+    digests = None
+    for slots in (1, 2, 4, 8):
+        res, st = stwo_rocm.serve(ctx, jobs, n_slots=slots)
+        d = [r["digest"] for r in res]
+        if digests is None:
+            digests = d
+        elif d != digests:
+            raise SystemExit("serving results differ between slot counts")
+        runs[slots] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
+    best = max((s for s in runs if s > 1), key=lambda s: runs[s]["jobs_per_s"])
+    cpu = None
+    if args.cpu_baseline == "on":
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        k = 12
+        t0 = time.perf_counter()
+        for kind, n, w, seed, _ in jobs[:k]:
+            if kind == 0:
+                oracle.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=seed)
+            else:
+                oracle.logup_xor_pipeline(n, seed)
+        el = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        cpu = {"value": round(k / el, 2), "unit": "proofs/s", "cores": threads, "kind": "port",
+               "sample": "the first %d requests of the same stream run back to back by the C oracle" % k}
+    line = {"metric": "serving proofs/s (256 Poisson requests, wide-Fib 14..20 x 16..100 + LogUp 16..18) + p50/p99",
+            "value": runs[best]["jobs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": len(jobs),
+            "warmup": 8, "ms_per_step": round(runs[best]["makespan_ms"] / len(jobs), 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 (exact M31/QM31)",
+            "data": "synthetic: seeded request stream and seeded traces",
+            "config": {"workload": "serving: 256 requests, Poisson 1500/s, 80% wide-Fib, 20% LogUp",
+                       "device": ident, "slots": best},
+            "p50_ms": runs[best]["p50_ms"], "p99_ms": runs[best]["p99_ms"],
+            "unbatched_one_slot": runs[1], "by_slots": runs,
+            "speedup_vs_one_slot": round(runs[best]["jobs_per_s"] / runs[1]["jobs_per_s"], 3),
+            "cpu_baseline": cpu, "results_equal_across_slot_counts": True}
     print(json.dumps(line), flush=True)
     ctx.close()
 

@@ -249,6 +249,67 @@ int stwo_rocm_gkr_prove_batch(stwo_rocm_ctx* ctx, uint32_t n_instances, const in
 int stwo_rocm_logup_xor_pipeline(stwo_rocm_ctx* ctx, uint32_t log_n, uint64_t seed, uint32_t* words_out,
                                  uint64_t cap, uint64_t* n_words, double* total_ms);
 
+/* ------------------------------------------------------------------ batched pipeline */
+/* No single reference counterpart: the batching pattern around the reference's per-column
+ * hooks interpolateCircleBuffers / evaluateCircleBuffers (src/prover/pcs/columns/
+ * circle_transforms.zig:248-250,443-451), commitMerkle (src/backend/merkle_ops.zig:20-56) and
+ * foldLine (src/prover/fri.zig:434-586).  n_columns seeded uniform-M31 columns with log sizes drawn
+ * from [log_min, log_max] are interpolated, LDE'd at blowup 2^log_blowup and committed in one
+ * lifted Merkle tree; then n_folds FRI fold_line rounds run on a seeded QM31 line evaluation of log
+ * fri_log, each layer committed and its root mixed into a device Fiat-Shamir channel that draws the
+ * fold alpha.  All buffers and descriptor tables are planned at create time and the launch
+ * sequence is captured as one hipGraph.  Result words: main root, n_folds + 1 line roots (8 words
+ * each), final channel digest (8), draw counter. */
+typedef struct stwo_rocm_batched stwo_rocm_batched;
+typedef struct {
+    uint32_t n_columns, log_min, log_max, log_blowup, fri_log, n_folds;
+    uint64_t seed;
+} stwo_rocm_batched_spec;
+int stwo_rocm_batched_create(stwo_rocm_ctx* ctx, const stwo_rocm_batched_spec* spec, stwo_rocm_batched** out);
+/* Launch-shape tuning (results are bit-identical for every setting): FFT LDS tile 2^11..2^13 words,
+ * Merkle parent levels per launch 2..10.  Applies to the context and recaptures the graph. */
+int stwo_rocm_batched_set_tuning(stwo_rocm_ctx* ctx, stwo_rocm_batched* b, uint32_t fft_tile_log,
+                                 uint32_t merkle_levels);
+/* Times every (fft_tile_log in 11..13) x (merkle_levels in {6, 8, 10}) graph over reps replays, checks
+ * that all candidates produce the same result words, keeps the fastest; json_out (optional) gets
+ * [{fft_tile_log, merkle_levels, ms}, ...]. */
+int stwo_rocm_batched_autotune(stwo_rocm_ctx* ctx, stwo_rocm_batched* b, uint32_t reps, uint32_t* fft_tile_log,
+                               uint32_t* merkle_levels, char* json_out, size_t cap);
+/* n_runs replays of the graph (use_graph != 0) or of the same launches issued eagerly; device ms per
+ * run; the result words of the last run. */
+int stwo_rocm_batched_run(stwo_rocm_ctx* ctx, stwo_rocm_batched* b, uint32_t n_runs, int use_graph,
+                          double* ms_per_run, uint32_t* words_out, uint64_t cap, uint64_t* n_words);
+int stwo_rocm_batched_destroy(stwo_rocm_ctx* ctx, stwo_rocm_batched* b);
+
+/* ------------------------------------------------------------------ serving */
+/* Continuous-batching proof service, extending the reference's one-job-at-a-time resource
+ * admission (src/prover/native/resource_admission.zig) and its prover session loop
+ * (src/tools/metal_prover_session/app.zig): n_slots proof slots (1..16), each with its own device
+ * context and HIP stream; requests become eligible at arrival_ms (relative to the call, sorted) and
+ * the FIFO head is admitted when a slot is idle and its estimated HBM footprint fits
+ * hbm_budget_bytes (0 = unlimited).  Jobs on different slots overlap on the device.  Job kinds: 0
+ * wide-Fibonacci proof (functional protocol, seeded uniform trace: trace_mode 1 of
+ * stwo_rocm_prove_wide_fibonacci), 1 XOR-lookup LogUp workload (stwo_rocm_logup_xor_pipeline,
+ * log_n 16..24).  Each result carries the Blake2s-256 digest of its output bytes. */
+typedef struct {
+    uint32_t kind, log_n, seq_len, reserved;
+    uint64_t seed;
+    double arrival_ms;
+} stwo_rocm_serve_job;
+typedef struct {
+    uint8_t digest[32];
+    double arrival_ms, start_ms, end_ms;
+    uint32_t slot, status;
+    uint64_t output_bytes;
+} stwo_rocm_serve_result;
+typedef struct {
+    double makespan_ms, p50_ms, p99_ms, mean_ms, jobs_per_s;
+    uint64_t peak_admitted_bytes;
+    uint32_t max_in_flight, n_slots;
+} stwo_rocm_serve_stats;
+int stwo_rocm_serve(stwo_rocm_ctx* ctx, const stwo_rocm_serve_job* jobs, uint32_t n_jobs, uint32_t n_slots,
+                    uint64_t hbm_budget_bytes, stwo_rocm_serve_result* results, stwo_rocm_serve_stats* stats);
+
 /* ------------------------------------------------------------------ measurement */
 /* No counterpart in the reference: a measurement hook for callers that need per-kernel
  * device time (the reference's stage profiler, src/prover/stage_profile.zig, times stages

@@ -76,6 +76,8 @@ def lib():
         L.ora_gkr_prove.restype = ctypes.c_size_t
         L.ora_gkr_prove.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(u32p),
                                     ctypes.POINTER(u32p), u32p, ctypes.c_char_p, u32p, ctypes.c_size_t]
+        L.ora_batched_pipeline.restype = ctypes.c_size_t
+        L.ora_batched_pipeline.argtypes = [ctypes.c_uint32] * 6 + [ctypes.c_uint64, u32p, ctypes.c_size_t]
         L.ora_logup_xor_pipeline.restype = ctypes.c_size_t
         L.ora_logup_xor_pipeline.argtypes = [ctypes.c_uint32, ctypes.c_uint64, u32p, ctypes.c_size_t]
         _lib = L
@@ -358,4 +360,14 @@ def logup_xor_pipeline(log_n, seed):
     out = np.zeros(need, np.uint32)
     _, po = _u32(out)
     lib().ora_logup_xor_pipeline(log_n, seed, po, need)
+    return out
+
+
+def batched_pipeline(n_columns, log_min, log_max, log_blowup, fri_log, n_folds, seed):
+    """Word stream of the batched primitive pipeline (see ora_batched_pipeline)."""
+    args = (n_columns, log_min, log_max, log_blowup, fri_log, n_folds, seed)
+    need = lib().ora_batched_pipeline(*args, _NULL, 0)
+    out = np.zeros(need, np.uint32)
+    _, po = _u32(out)
+    lib().ora_batched_pipeline(*args, po, need)
     return out

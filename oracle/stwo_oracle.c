@@ -1980,3 +1980,80 @@ size_t ora_logup_xor_pipeline(uint32_t log_n, uint64_t seed, uint32_t *out, size
     ctree_free(&ti);
     return wb.n;
 }
+
+/* Batched primitive pipeline, the same sequence as the device graph: n_columns seeded
+ * uniform-M31 evaluations of log sizes drawn from [log_min, log_max] -> interpolate -> LDE at
+ * blowup -> one lifted Merkle commitment; channel <- main root (draw); then for each of n_folds
+ * rounds: commit the QM31 line (4 coordinate columns), mix its root and draw alpha, fold_line;
+ * the last line is committed and mixed (draw) too.  Output: main root, n_folds + 1 line roots,
+ * final channel digest, draw counter.  Returns the word count. */
+size_t ora_batched_pipeline(uint32_t n_columns, uint32_t log_min, uint32_t log_max, uint32_t log_blowup,
+                            uint32_t fri_log, uint32_t n_folds, uint64_t seed, uint32_t *out, size_t cap) {
+    uint32_t *logs = malloc(4 * n_columns), *lde_logs = malloc(4 * n_columns);
+    uint32_t **lde = malloc(sizeof(uint32_t *) * n_columns);
+    for (uint32_t i = 0; i < n_columns; i++) {
+        logs[i] = log_min + (uint32_t)(ora_splitmix64(seed ^ 0xA5A5A5A5ull ^ (i * 0x9E3779B97F4A7C15ull)) %
+                                       (log_max - log_min + 1));
+        lde_logs[i] = logs[i] + log_blowup;
+        get_twiddles(lde_logs[i] > 2 ? lde_logs[i] : 3);
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = 0; i < (int64_t)n_columns; i++) {
+        uint64_t n = 1ull << logs[i];
+        uint32_t *v = malloc(4 * n);
+        for (uint64_t r = 0; r < n; r++) v[r] = (uint32_t)(ora_splitmix64(seed + ((uint64_t)i << 40) + r) % P);
+        circle_interpolate(v, logs[i]);
+        lde[i] = malloc(4ull << lde_logs[i]);
+        ora_circle_lde(v, logs[i], log_blowup, lde[i]);
+        free(v);
+    }
+    ora_tree *t = ora_merkle_commit((const uint32_t *const *)lde, lde_logs, n_columns);
+    wbuf wb = {out, 0, cap};
+    uint8_t root[32];
+    ora_tree_root(t, root);
+    channel ch;
+    memset(ch.digest, 0, 32);
+    ch.n_draws = 0;
+    ch_mix_root(&ch, root);
+    (void)ch_draw_secure_felt(&ch);
+    for (int i = 0; i < 8; i++) {
+        uint32_t v;
+        memcpy(&v, root + 4 * i, 4);
+        wb_u(&wb, v);
+    }
+    uint64_t n0 = 1ull << fri_log;
+    qm31 *line = malloc(sizeof(qm31) * n0);
+    for (uint64_t i = 0; i < 4 * n0; i++) ((uint32_t *)line)[i] = (uint32_t)(ora_splitmix64((seed ^ 0x51F15EEDull) + i) % P);
+    for (uint32_t j = 0; j <= n_folds; j++) {
+        uint32_t lg = fri_log - j;
+        ora_tree *ft = commit_qm31_column(line, lg, NULL);
+        uint8_t r[32];
+        ora_tree_root(ft, r);
+        ora_tree_free(ft);
+        ch_mix_root(&ch, r);
+        qm31 a = ch_draw_secure_felt(&ch);
+        for (int i = 0; i < 8; i++) {
+            uint32_t v;
+            memcpy(&v, r + 4 * i, 4);
+            wb_u(&wb, v);
+        }
+        if (j == n_folds) break;
+        qm31 *next = malloc(sizeof(qm31) << (lg - 1));
+        uint32_t aw[4];
+        q_store(a, aw);
+        ora_fold_line((const uint32_t *)line, lg, coset_half_odds(lg).initial_index, aw, (uint32_t *)next);
+        free(line);
+        line = next;
+    }
+    for (int i = 0; i < 8; i++) {
+        uint32_t v;
+        memcpy(&v, ch.digest + 4 * i, 4);
+        wb_u(&wb, v);
+    }
+    wb_u(&wb, ch.n_draws);
+    free(line);
+    ora_tree_free(t);
+    for (uint32_t i = 0; i < n_columns; i++) free(lde[i]);
+    free(lde), free(logs), free(lde_logs);
+    return wb.n;
+}

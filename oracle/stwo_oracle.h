@@ -107,6 +107,8 @@ void ora_mle_fix_first(int base, const uint32_t *in, uint32_t n_vars, const uint
 size_t ora_gkr_prove(uint32_t n_inst, const int *kinds, const uint32_t *const *nums, const uint32_t *const *dens,
                      const uint32_t *n_vars, const uint8_t digest_in[32], uint32_t *out, size_t cap);
 size_t ora_logup_xor_pipeline(uint32_t log_n, uint64_t seed, uint32_t *out, size_t cap);
+size_t ora_batched_pipeline(uint32_t n_columns, uint32_t log_min, uint32_t log_max, uint32_t log_blowup,
+                            uint32_t fri_log, uint32_t n_folds, uint64_t seed, uint32_t *out, size_t cap);
 
 #ifdef __cplusplus
 }

@@ -46,16 +46,7 @@ int stwo_rocm_ctx_create(int device, uint64_t hbm_budget_bytes, stwo_rocm_ctx** 
     *out = nullptr;
     stwo_rocm_ctx* ctx = new stwo_rocm_ctx();
     try {
-        int n = 0;
-        SR_HIP(hipGetDeviceCount(&n));
-        SR_CHECK(device >= 0 && device < n, "no such HIP device");
-        ctx->c.device = device;
-        SR_HIP(hipSetDevice(device));
-        SR_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
-        ctx->c.hbm_budget = hbm_budget_bytes;
-        ctx->c.leaf_seed = b2_prefix_seed("leaf");
-        ctx->c.node_seed = b2_prefix_seed("node");
-        ctx->c.gen = make_gen_doubles();
+        ctx_init(ctx->c, device, hbm_budget_bytes);
         *out = ctx;
         return STWO_ROCM_OK;
     } catch (const Error& e) {
@@ -63,7 +54,7 @@ int stwo_rocm_ctx_create(int device, uint64_t hbm_budget_bytes, stwo_rocm_ctx** 
             strncpy(err, e.what(), err_len - 1);
             err[err_len - 1] = 0;
         }
-        if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+        ctx_release(ctx->c);
         delete ctx;
         return e.code;
     }
@@ -73,16 +64,7 @@ int stwo_rocm_ctx_destroy(stwo_rocm_ctx* ctx) {
     if (!ctx) return STWO_ROCM_EINVAL;
     {
         std::lock_guard<std::mutex> lk(ctx->c.mu);
-        Ctx& c = ctx->c;
-        (void)hipSetDevice(c.device);
-        (void)hipStreamSynchronize(c.stream);
-        for (auto& kv : c.live_blocks) (void)hipFree(kv.first);
-        for (auto& kv : c.free_blocks) (void)hipFree(kv.second);
-        c.live_blocks.clear();
-        c.free_blocks.clear();
-        if (c.h_pinned) (void)hipHostFree(c.h_pinned);
-        if (c.h_ring) (void)hipHostFree(c.h_ring);
-        (void)hipStreamDestroy(c.stream);
+        ctx_release(ctx->c);
     }
     delete ctx;
     return STWO_ROCM_OK;
@@ -684,6 +666,133 @@ int stwo_rocm_logup_xor_pipeline(stwo_rocm_ctx* ctx, uint32_t log_n, uint64_t se
         *n_words = w.size();
         SR_CHECK(words_out && w.size() <= cap, "logup: output buffer too small");
         memcpy(words_out, w.data(), 4 * w.size());
+    });
+}
+
+// ---------------------------------------------------------------- batched pipeline
+struct stwo_rocm_batched {
+    BatchedPipeline* p;
+};
+
+int stwo_rocm_batched_create(stwo_rocm_ctx* ctx, const stwo_rocm_batched_spec* spec, stwo_rocm_batched** out) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(spec && out, "null argument");
+        BatchedSpec s{spec->n_columns, spec->log_min, spec->log_max, spec->log_blowup, spec->fri_log, spec->n_folds,
+                      spec->seed};
+        auto* b = new stwo_rocm_batched{nullptr};
+        try {
+            b->p = new BatchedPipeline(c, s);
+            b->p->capture();
+        } catch (...) {
+            delete b->p;
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int stwo_rocm_batched_set_tuning(stwo_rocm_ctx* ctx, stwo_rocm_batched* b, uint32_t fft_tile_log,
+                                 uint32_t merkle_levels) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(b && b->p, "null argument");
+        SR_CHECK(fft_tile_log >= 11 && fft_tile_log <= 13, "fft_tile_log must be 11..13");
+        SR_CHECK(merkle_levels >= 2 && merkle_levels <= 10, "merkle_levels must be 2..10");
+        c.fft_tile_log = fft_tile_log;
+        c.merkle_levels = merkle_levels;
+        b->p->capture();
+    });
+}
+
+int stwo_rocm_batched_autotune(stwo_rocm_ctx* ctx, stwo_rocm_batched* b, uint32_t reps, uint32_t* fft_tile_log,
+                               uint32_t* merkle_levels, char* json_out, size_t cap) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(b && b->p && fft_tile_log && merkle_levels, "null argument");
+        b->p->autotune(reps ? reps : 3);
+        *fft_tile_log = c.fft_tile_log;
+        *merkle_levels = c.merkle_levels;
+        if (json_out && cap) {
+            std::string js = "[";
+            for (size_t i = 0; i < b->p->tuning.size(); i++) {
+                char buf[128];
+                snprintf(buf, sizeof buf, "%s{\"fft_tile_log\":%u,\"merkle_levels\":%u,\"ms\":%.4f}", i ? "," : "",
+                         b->p->tuning[i].fft_tile_log, b->p->tuning[i].merkle_levels, b->p->tuning[i].ms);
+                js += buf;
+            }
+            js += "]";
+            SR_CHECK(js.size() < cap, "autotune: json buffer too small");
+            memcpy(json_out, js.c_str(), js.size() + 1);
+        }
+    });
+}
+
+int stwo_rocm_batched_run(stwo_rocm_ctx* ctx, stwo_rocm_batched* b, uint32_t n_runs, int use_graph,
+                          double* ms_per_run, uint32_t* words_out, uint64_t cap, uint64_t* n_words) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(b && b->p && n_words, "null argument");
+        double ms = 0;
+        if (use_graph) {
+            ms = b->p->replay(n_runs);
+        } else {
+            hipEvent_t e0, e1;
+            SR_HIP(hipEventCreate(&e0));
+            SR_HIP(hipEventCreate(&e1));
+            SR_HIP(hipEventRecord(e0, c.stream));
+            for (uint32_t i = 0; i < n_runs; i++) b->p->record();
+            SR_HIP(hipEventRecord(e1, c.stream));
+            SR_HIP(hipEventSynchronize(e1));
+            float f = 0;
+            SR_HIP(hipEventElapsedTime(&f, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            ms = n_runs ? f / n_runs : 0.0;
+        }
+        if (ms_per_run) *ms_per_run = ms;
+        std::vector<uint32_t> w = b->p->result();
+        *n_words = w.size();
+        SR_CHECK(words_out && w.size() <= cap, "batched: output buffer too small");
+        memcpy(words_out, w.data(), 4 * w.size());
+    });
+}
+
+int stwo_rocm_batched_destroy(stwo_rocm_ctx* ctx, stwo_rocm_batched* b) {
+    return guarded(ctx, [&](Ctx& c) {
+        (void)c;
+        if (!b) return;
+        c.sync();
+        delete b->p;
+        delete b;
+    });
+}
+
+// ---------------------------------------------------------------- serving
+int stwo_rocm_serve(stwo_rocm_ctx* ctx, const stwo_rocm_serve_job* jobs, uint32_t n_jobs, uint32_t n_slots,
+                    uint64_t hbm_budget_bytes, stwo_rocm_serve_result* results, stwo_rocm_serve_stats* stats) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(jobs && results && stats && n_jobs > 0, "null argument");
+        std::vector<ServeJob> js(n_jobs);
+        for (uint32_t i = 0; i < n_jobs; i++)
+            js[i] = ServeJob{jobs[i].kind, jobs[i].log_n, jobs[i].seq_len, 0, jobs[i].seed, jobs[i].arrival_ms};
+        std::vector<ServeOutcome> out;
+        ServeStats st{};
+        serve(c.device, js, n_slots, hbm_budget_bytes, out, st);
+        for (uint32_t i = 0; i < n_jobs; i++) {
+            memcpy(results[i].digest, out[i].digest, 32);
+            results[i].arrival_ms = out[i].arrival_ms;
+            results[i].start_ms = out[i].start_ms;
+            results[i].end_ms = out[i].end_ms;
+            results[i].slot = out[i].slot;
+            results[i].status = out[i].status;
+            results[i].output_bytes = out[i].output_bytes;
+        }
+        stats->makespan_ms = st.makespan_ms;
+        stats->p50_ms = st.p50_ms;
+        stats->p99_ms = st.p99_ms;
+        stats->mean_ms = st.mean_ms;
+        stats->jobs_per_s = st.jobs_per_s;
+        stats->peak_admitted_bytes = st.peak_admitted_bytes;
+        stats->max_in_flight = st.max_in_flight;
+        stats->n_slots = st.n_slots;
     });
 }
 

@@ -378,7 +378,9 @@ static void rx_dispatch(uint32_t R, uint32_t logc, dim3 grid, dim3 block, size_t
         return;                                                                                             \
     }
     SR_RX(6, 0) SR_RX(7, 0) SR_RX(8, 0) SR_RX(9, 0) SR_RX(10, 0) SR_RX(11, 0) SR_RX(12, 0) SR_RX(13, 0)
-    SR_RX(4, 9) SR_RX(5, 8) SR_RX(6, 7) SR_RX(7, 6) SR_RX(8, 5)
+    SR_RX(4, 9) SR_RX(5, 8) SR_RX(6, 7) SR_RX(7, 6) SR_RX(8, 5)  // 8192-word tiles
+    SR_RX(4, 8) SR_RX(5, 7) SR_RX(6, 6) SR_RX(7, 5) SR_RX(8, 4)  // 4096-word tiles
+    SR_RX(4, 7) SR_RX(5, 6) SR_RX(6, 5) SR_RX(7, 4) SR_RX(8, 3)  // 2048-word tiles
 #undef SR_RX
     throw Error(1, "fft: no kernel for pass shape R=" + std::to_string(R) + " logc=" + std::to_string(logc));
 }
@@ -437,7 +439,7 @@ void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint3
     }
     // Pass plan: a contiguous pass of R0 <= 13 stages, then strided passes of 4..8
     // stages over 8192-element tiles (>= 32 consecutive words per row).
-    const uint32_t TILE_LOG = 13;
+    const uint32_t TILE_LOG = c.fft_tile_log;  // 11..13 (tuning parameter, results independent of it)
     std::vector<std::pair<uint32_t, uint32_t>> passes;  // (s_lo, R)
     if (m <= TILE_LOG) {
         passes.push_back({0, m});

@@ -13,6 +13,7 @@
 // into two parents and their parent in registers, then the block halves the
 // level in LDS, writing every level; levels of at most 64 nodes use one quad of
 // lanes per compression.  A 2^21-leaf tree takes three launches instead of 21.
+#include <algorithm>
 #include <vector>
 
 #include "chan.hpp"
@@ -26,15 +27,6 @@ __device__ __forceinline__ void store_hash(uint8_t* __restrict__ out, uint64_t i
     o[0] = make_uint4(h[0], h[1], h[2], h[3]);
     o[1] = make_uint4(h[4], h[5], h[6], h[7]);
 }
-
-// Column descriptor for leaf hashing; the table is padded to a multiple of 16
-// entries with mask 0 (padding reads column 0 and contributes a zero word).
-struct LeafColDev {
-    const uint32_t* col;
-    uint32_t shift;  // lift shift max_log - log + 1 (1 = full size)
-    uint32_t mask;   // 0xffffffff, or 0 for padding
-};
-static_assert(sizeof(LeafColDev) == 16, "leaf column layout");
 
 __device__ __forceinline__ void leaf_block_load(const LeafColDev* __restrict__ cd, uint32_t p, uint32_t m[16]) {
 #pragma unroll
@@ -167,7 +159,7 @@ __global__ __launch_bounds__(MP_THREADS) void merkle_parents_kernel(uint8_t* __r
         level--;
         first >>= 1;
         uint8_t* out = tree + (((uint64_t)1 << level) - 1) * 32;
-        if (count > MP_THREADS / 4) {
+        if (count > blockDim.x / 4) {
             uint32_t h[8];
             if (t < count) node_hash(seed, &sh[t * 16], &sh[t * 16 + 8], h);
             __syncthreads();
@@ -224,8 +216,8 @@ void launch_merkle_parent_layer(Ctx& c, const uint8_t* d_children, uint32_t log_
     SR_HIP(hipGetLastError());
 }
 
-void launch_merkle_leaves(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
-                          uint32_t max_log, uint8_t* d_leaves) {
+std::vector<LeafColDev> merkle_leaf_table(const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
+                                          uint32_t max_log) {
     SR_CHECK(n_cols > 0 && max_log <= 31, "merkle leaves: shape");
     std::vector<LeafColDev> tab(((n_cols + 15) / 16) * 16);
     for (size_t i = 0; i < tab.size(); i++) {
@@ -236,13 +228,23 @@ void launch_merkle_leaves(Ctx& c, const uint32_t* const* h_cols, const uint32_t*
         tab[i].shift = max_log - lg + 1;
         tab[i].mask = real ? 0xffffffffu : 0u;
     }
-    DBuf d_tab = upload(c, tab.data(), tab.size());
+    return tab;
+}
+
+void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, uint32_t n_cols, uint32_t max_log, uint8_t* d_leaves) {
     uint64_t n = 1ull << max_log;
     KTimer kt(c, "merkle_leaves L=" + std::to_string(max_log) + " cols=" + std::to_string(n_cols),
               (4.0 * n_cols + 32.0) * (double)n);
     hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)), dim3(MK_THREADS), 0,
-                       c.stream, d_tab.as<LeafColDev>(), n_cols, max_log, c.leaf_seed, d_leaves);
+                       c.stream, d_tab, n_cols, max_log, c.leaf_seed, d_leaves);
     SR_HIP(hipGetLastError());
+}
+
+void launch_merkle_leaves(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
+                          uint32_t max_log, uint8_t* d_leaves) {
+    std::vector<LeafColDev> tab = merkle_leaf_table(h_cols, h_logs, n_cols, max_log);
+    DBuf d_tab = upload(c, tab.data(), tab.size());
+    launch_merkle_leaves_dev(c, d_tab.as<LeafColDev>(), n_cols, max_log, d_leaves);
 }
 
 void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size, uint8_t* d_leaves) {
@@ -257,18 +259,21 @@ void launch_merkle_parents(Ctx& c, DeviceTree& t, DevChannel* d_ch, uint8_t* d_r
     uint32_t k = t.max_log;
     if (d_ch && k == 0) launch_chan_mix_root_draw(c, d_ch, t.layers, d_root_out);
     while (k > 0) {
-        uint32_t levels = k < 10 ? k : 10;
+        const uint32_t lmax = c.merkle_levels;  // 2..10 (tuning parameter)
+        uint32_t levels = k < lmax ? k : lmax;
         uint32_t blocks = levels == 1 ? (uint32_t)(((1ull << (k - 1)) + MP_THREADS - 1) / MP_THREADS)
                                       : (1u << (k - levels));
+        // levels >= 2: 2^(levels-2) threads carry work; at least one wave per block
+        const uint32_t threads = levels == 1 ? MP_THREADS : std::max(64u, 1u << (levels - 2));
         KTimer kt(c,
                   "merkle_parents L=" + std::to_string(k) + " levels=" + std::to_string(levels) +
                       (d_ch && levels == k ? " +channel" : ""),
                   32.0 * (double)((1ull << k) + (1ull << k) - (1ull << (k - levels))));
         if (d_ch && levels == k)
-            hipLaunchKernelGGL(merkle_parents_kernel<true>, dim3(blocks), dim3(MP_THREADS), 0, c.stream, t.layers, k,
+            hipLaunchKernelGGL(merkle_parents_kernel<true>, dim3(blocks), dim3(threads), 0, c.stream, t.layers, k,
                                levels, c.node_seed, d_ch, (uint32_t*)d_root_out);
         else
-            hipLaunchKernelGGL(merkle_parents_kernel<false>, dim3(blocks), dim3(MP_THREADS), 0, c.stream, t.layers, k,
+            hipLaunchKernelGGL(merkle_parents_kernel<false>, dim3(blocks), dim3(threads), 0, c.stream, t.layers, k,
                                levels, c.node_seed, nullptr, nullptr);
         SR_HIP(hipGetLastError());
         k -= levels;

@@ -131,13 +131,22 @@ struct Gather {
 
 struct JsonOut {
     std::string s;
+    JsonOut() { s.reserve(1 << 17); }
     void raw(const char* t) { s += t; }
-    void u64(uint64_t v) { s += std::to_string(v); }
+    void u64(uint64_t v) {
+        char b[24];
+        int n = 0;
+        do {
+            b[n++] = (char)('0' + v % 10);
+            v /= 10;
+        } while (v);
+        while (n) s.push_back(b[--n]);
+    }
     void hash(const uint8_t* h) {
         s += '[';
         for (int i = 0; i < 32; i++) {
             if (i) s += ',';
-            s += std::to_string(h[i]);
+            u64(h[i]);
         }
         s += ']';
     }
@@ -145,14 +154,13 @@ struct JsonOut {
         s += '[';
         for (int i = 0; i < 4; i++) {
             if (i) s += ',';
-            s += std::to_string(w[i]);
+            u64(w[i]);
         }
         s += ']';
     }
 };
 
 // ---------------------------------------------------------------- prove
-// Owns resident trees for the duration of one proof (freed on every exit path).
 // LDE of coefficient columns (pointer list) by `blowup` and Merkle commit
 // (commit_polys.zig:10-67; preparation.zig:113-210).
 void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
@@ -422,8 +430,6 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     // readback below.
     const uint32_t last_log = prm.log_last_layer_degree_bound + B;
     uint32_t n_inner = L - 1 - last_log;  // inner layers log L-1 .. last_log+1
-    constexpr size_t CH_BYTES = 128;  // DevChannel slot ahead of the roots
-    static_assert(sizeof(DevChannel) <= CH_BYTES, "DevChannel slot");
     const size_t fbytes = CH_BYTES + 32 * (1 + (size_t)n_inner);
     DBuf fst(&c, fbytes);  // DevChannel | roots (first layer, inner layers)
     DevChannel* d_ch = fst.as<DevChannel>();

@@ -97,4 +97,70 @@ std::vector<qm31> gkr_sum_as_poly(Ctx& c, int kind, const uint32_t* d_eq, const 
 // stream described in logup_pipeline.hip.  total_ms (optional): host wall time.
 std::vector<uint32_t> logup_xor_pipeline(Ctx& c, uint32_t log_n, uint64_t seed, double* total_ms);
 
+// Batched primitive pipeline (batched_pipeline.hip): LDE + lifted Merkle of many
+// columns and FRI fold_line rounds, planned once and replayed as one hipGraph.
+struct BatchedSpec {
+    uint32_t n_columns, log_min, log_max, log_blowup, fri_log, n_folds;
+    uint64_t seed;
+};
+uint32_t batched_column_log(const BatchedSpec& s, uint32_t i);
+constexpr size_t CH_BYTES = 128;  // DevChannel slot ahead of the root copies
+static_assert(sizeof(DevChannel) <= CH_BYTES, "DevChannel slot");
+class BatchedPipeline {
+   public:
+    struct Tuning {
+        uint32_t fft_tile_log, merkle_levels;
+        double ms;
+    };
+    BatchedPipeline(Ctx& c, const BatchedSpec& s);
+    ~BatchedPipeline();
+    void record();                  // the launch sequence (eager, or under capture)
+    void capture();                 // capture record() into the graph with the current tuning
+    double replay(uint32_t n);      // n graph launches; device ms per launch
+    std::vector<uint32_t> result();  // roots, final channel digest and draw counter
+    void autotune(uint32_t reps);    // pick fft_tile_log x merkle_levels, recapture
+    std::vector<Tuning> tuning;
+    uint64_t input_words() const;
+
+   private:
+    struct SizeClass {
+        uint32_t log = 0, n = 0;
+        DBuf d_eval, d_coeff, d_lde;
+    };
+    Ctx& c;
+    BatchedSpec sp;
+    TreeOwner owner;
+    std::vector<uint32_t> logs;
+    uint32_t max_lde = 0;
+    DBuf evals, coeffs, lde, leaf_tab, chan;
+    std::vector<SizeClass> classes;
+    DeviceTree* tree = nullptr;
+    std::vector<DBuf> lines;
+    std::vector<DeviceTree*> ftrees;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+// Continuous-batching proof service (serving.hip).
+struct ServeJob {
+    uint32_t kind;  // 0 wide-Fibonacci proof (seeded uniform trace), 1 XOR-lookup LogUp workload
+    uint32_t log_n, seq_len, reserved;
+    uint64_t seed;
+    double arrival_ms;  // relative to the start of serve(); non-decreasing
+};
+struct ServeOutcome {
+    uint8_t digest[32];  // Blake2s-256 of the output bytes
+    double arrival_ms, start_ms, end_ms;
+    uint32_t slot, status;
+    uint64_t output_bytes;
+};
+struct ServeStats {
+    double makespan_ms, p50_ms, p99_ms, mean_ms, jobs_per_s;
+    uint64_t peak_admitted_bytes;
+    uint32_t max_in_flight, n_slots;
+};
+uint64_t serve_job_bytes(const ServeJob& j);  // admission estimate of a job's HBM footprint
+void serve(int device, const std::vector<ServeJob>& jobs, uint32_t n_slots, uint64_t hbm_budget,
+           std::vector<ServeOutcome>& out, ServeStats& st);
+
 }  // namespace sr

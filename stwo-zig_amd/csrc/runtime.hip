@@ -61,6 +61,38 @@ void Ctx::free(void* p) {
     bytes_cached += sz;
 }
 
+void ctx_init(Ctx& c, int device, uint64_t hbm_budget) {
+    int n = 0;
+    SR_HIP(hipGetDeviceCount(&n));
+    SR_CHECK(device >= 0 && device < n, "no such HIP device");
+    c.device = device;
+    SR_HIP(hipSetDevice(device));
+    SR_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));  // default priority
+    c.hbm_budget = hbm_budget;
+    c.leaf_seed = b2_prefix_seed("leaf");
+    c.node_seed = b2_prefix_seed("node");
+    c.gen = make_gen_doubles();
+}
+
+void ctx_release(Ctx& c) {
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    for (auto& kv : c.live_blocks) (void)hipFree(kv.first);
+    for (auto& kv : c.free_blocks) (void)hipFree(kv.second);
+    c.live_blocks.clear();
+    c.free_blocks.clear();
+    if (c.h_pinned) (void)hipHostFree(c.h_pinned);
+    if (c.h_ring) (void)hipHostFree(c.h_ring);
+    c.h_pinned = nullptr;
+    c.h_ring = nullptr;
+    for (auto e : c.ev_pool) (void)hipEventDestroy(e);
+    c.ev_pool.clear();
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.stream = nullptr;
+    c.d_tw = c.d_itw = nullptr;
+    c.tower_log = 0;
+}
+
 void Ctx::trim() {
     if (free_blocks.empty()) return;
     (void)hipStreamSynchronize(stream);

@@ -89,6 +89,9 @@ struct Ctx {
         hipEvent_t a, b;
         double bytes;
     };
+    // Tuning parameters (results are bit-identical for every setting):
+    uint32_t fft_tile_log = 13;   // FFT LDS tile, 2^11..2^13 words
+    uint32_t merkle_levels = 10;  // Merkle parent levels per launch, 2..10
     bool profiling = false;
     std::string profile_only;  // non-empty: record only launches of this class
     std::vector<hipEvent_t> ev_pool;
@@ -109,6 +112,11 @@ struct Ctx {
     const uint32_t* domain_points(uint32_t log);
     const uint32_t* inv_circle_y(uint32_t log);
 };
+
+// Context set-up on `device` (stream at default priority, hash seeds, circle generator) and
+// release of every resource it holds (device blocks, pinned staging, events, stream).
+void ctx_init(Ctx& c, int device, uint64_t hbm_budget);
+void ctx_release(Ctx& c);
 
 // Records one kernel launch in the context profile (no-op unless profiling).
 struct KTimer {
@@ -167,6 +175,18 @@ struct DBuf {
 void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint32_t* const* d_dst, uint32_t n_cols,
                        uint32_t m, uint32_t src_log);
 // Lifted Merkle: columns sorted by log size ascending (d_cols/d_logs device arrays).
+// Column descriptor of the leaf kernel; tables are padded to a multiple of 16 entries with
+// mask 0 (padding reads column 0 and contributes a zero word).
+struct LeafColDev {
+    const uint32_t* col;
+    uint32_t shift;  // lift shift max_log - log + 1 (1 = full size)
+    uint32_t mask;   // 0xffffffff, or 0 for padding
+};
+static_assert(sizeof(LeafColDev) == 16, "leaf column layout");
+std::vector<LeafColDev> merkle_leaf_table(const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
+                                          uint32_t max_log);
+// Leaves from a device-resident descriptor table (no host traffic: capturable in a graph).
+void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, uint32_t n_cols, uint32_t max_log, uint8_t* d_leaves);
 // Host arrays in: column device pointers and log sizes, sorted by log size.
 void launch_merkle_leaves(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
                           uint32_t max_log, uint8_t* d_leaves);

@@ -1,0 +1,189 @@
+// Continuous-batching proof service on one GPU.
+//
+// The reference admits one job at a time against a resource profile
+// (src/prover/native/resource_admission.zig) and proves it; here a request loop
+// keeps n_slots proof slots busy: every slot owns a device context (its own HIP
+// stream, caching arena, twiddle tower and resident Merkle layers), requests
+// become eligible at their arrival time, and the head of the FIFO is admitted as
+// soon as a slot is idle and its estimated HBM footprint fits the shared budget.
+// Jobs of different slots overlap on the device (independent streams).  Every
+// result is reduced to a Blake2s-256 digest of its output bytes, so a job's result
+// can be compared alone and in the mix.
+//
+// Job kinds: 0 = wide-Fibonacci proof (functional protocol, seeded uniform trace,
+// examples/wide_fibonacci.zig:181-343), 1 = XOR-lookup LogUp workload
+// (logup_pipeline.hip).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "prover.hpp"
+
+namespace sr {
+
+uint64_t serve_job_bytes(const ServeJob& j) {
+    const uint64_t N = 1ull << j.log_n;
+    if (j.kind == 0) return 16ull * N * j.seq_len + 1024ull * N + (64ull << 20);
+    return 1024ull * N + (64ull << 20);
+}
+
+namespace {
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Slot {
+    Ctx c;
+    ~Slot() { ctx_release(c); }
+};
+
+}  // namespace
+
+void serve(int device, const std::vector<ServeJob>& jobs, uint32_t n_slots, uint64_t hbm_budget,
+           std::vector<ServeOutcome>& out, ServeStats& st) {
+    SR_CHECK(n_slots >= 1 && n_slots <= 16, "serve: n_slots must be 1..16");
+    SR_CHECK(!jobs.empty(), "serve: no jobs");
+    uint32_t max_log = 0;
+    for (size_t i = 0; i < jobs.size(); i++) {
+        const ServeJob& j = jobs[i];
+        SR_CHECK(j.kind <= 1, "serve: unknown job kind");
+        SR_CHECK(j.kind != 0 || (j.log_n >= 5 && j.log_n <= 24 && j.seq_len >= 3 && j.seq_len <= 256),
+                 "serve: wide-Fibonacci shape");
+        SR_CHECK(j.kind != 1 || (j.log_n >= 16 && j.log_n <= 24), "serve: LogUp shape");
+        SR_CHECK(i == 0 || jobs[i - 1].arrival_ms <= j.arrival_ms, "serve: arrivals must be sorted");
+        SR_CHECK(hbm_budget == 0 || serve_job_bytes(j) <= hbm_budget, "serve: a job exceeds the HBM budget");
+        max_log = std::max(max_log, j.log_n + 2);
+    }
+    // Slots: contexts with the twiddle tower ready (warm), so no job pays set-up.
+    std::vector<std::unique_ptr<Slot>> slots;
+    for (uint32_t s = 0; s < n_slots; s++) {
+        slots.emplace_back(new Slot());
+        ctx_init(slots.back()->c, device, 0);
+        slots.back()->c.ensure_tower(max_log);
+        slots.back()->c.sync();
+    }
+    out.assign(jobs.size(), ServeOutcome{});
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<size_t> ready;  // arrived, not yet admitted (FIFO)
+    size_t arrived = 0;
+    uint64_t used = 0, peak = 0;
+    uint32_t in_flight = 0, max_in_flight = 0;
+    bool done_arriving = false;
+    std::string first_error;
+    const double t0 = now_ms();
+
+    auto worker = [&](uint32_t sid) {
+        Ctx& c = slots[sid]->c;
+        if (hipSetDevice(device) != hipSuccess) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (first_error.empty()) first_error = "hipSetDevice failed in a serving slot";
+            cv.notify_all();
+            return;
+        }
+        for (;;) {
+            size_t ji;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] {
+                    if (!first_error.empty()) return true;
+                    if (ready.empty()) return done_arriving;
+                    const uint64_t need = serve_job_bytes(jobs[ready.front()]);
+                    return hbm_budget == 0 || used + need <= hbm_budget;
+                });
+                if (!first_error.empty() || ready.empty()) return;
+                ji = ready.front();
+                ready.pop_front();
+                used += serve_job_bytes(jobs[ji]);
+                peak = std::max(peak, used);
+                in_flight++;
+                max_in_flight = std::max(max_in_flight, in_flight);
+            }
+            ServeOutcome& o = out[ji];
+            o.slot = sid;
+            o.start_ms = now_ms() - t0;
+            try {
+                const ServeJob& j = jobs[ji];
+                if (j.kind == 0) {
+                    ProveParams prm{};
+                    prm.log_n_rows = j.log_n;
+                    prm.sequence_len = j.seq_len;
+                    prm.pow_bits = 10;
+                    prm.log_blowup_factor = 1;
+                    prm.log_last_layer_degree_bound = 0;
+                    prm.n_queries = 3;
+                    prm.trace_mode = 1;
+                    prm.trace_seed = j.seed;
+                    std::string proof;
+                    prove_wide_fibonacci(c, prm, nullptr, proof, nullptr);
+                    b2_hash(proof.data(), proof.size(), o.digest);
+                    o.output_bytes = proof.size();
+                } else {
+                    std::vector<uint32_t> w = logup_xor_pipeline(c, j.log_n, j.seed, nullptr);
+                    b2_hash(w.data(), 4 * w.size(), o.digest);
+                    o.output_bytes = 4 * w.size();
+                }
+                o.status = 0;
+            } catch (const std::exception& e) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_error.empty()) first_error = e.what();
+                o.status = 1;
+            }
+            o.end_ms = now_ms() - t0;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                used -= serve_job_bytes(jobs[ji]);
+                in_flight--;
+            }
+            cv.notify_all();
+        }
+    };
+    std::vector<std::thread> th;
+    for (uint32_t s = 0; s < n_slots; s++) th.emplace_back(worker, s);
+    // Arrivals: each request becomes eligible at its arrival time.
+    for (size_t i = 0; i < jobs.size(); i++) {
+        const double wait = jobs[i].arrival_ms - (now_ms() - t0);
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 1000.0)));
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            out[i].arrival_ms = jobs[i].arrival_ms;
+            ready.push_back(i);
+            arrived++;
+            if (i + 1 == jobs.size()) done_arriving = true;
+        }
+        cv.notify_all();
+    }
+    for (auto& t : th) t.join();
+    slots.clear();
+    SR_CHECK(first_error.empty(), "serve: job failed: " + first_error);
+    std::vector<double> lat(jobs.size());
+    double last_end = 0, sum = 0;
+    for (size_t i = 0; i < jobs.size(); i++) {
+        lat[i] = out[i].end_ms - out[i].arrival_ms;
+        sum += lat[i];
+        last_end = std::max(last_end, out[i].end_ms);
+    }
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double q) {
+        size_t k = (size_t)std::ceil(q * (double)lat.size()) - 1;
+        return lat[std::min(k, lat.size() - 1)];
+    };
+    st.makespan_ms = last_end - jobs.front().arrival_ms;
+    st.p50_ms = pct(0.50);
+    st.p99_ms = pct(0.99);
+    st.mean_ms = sum / (double)jobs.size();
+    st.jobs_per_s = (double)jobs.size() / (st.makespan_ms * 1e-3);
+    st.peak_admitted_bytes = peak;
+    st.max_in_flight = max_in_flight;
+    st.n_slots = n_slots;
+}
+
+}  // namespace sr

@@ -131,6 +131,16 @@ _SIGNATURES += [
                                                   _u32p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     ("stwo_rocm_logup_xor_pipeline", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint64, _u32p, ctypes.c_uint64,
                                                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double)]),
+    ("stwo_rocm_batched_create", ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.POINTER(_vp)]),
+    ("stwo_rocm_batched_set_tuning", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32]),
+    ("stwo_rocm_batched_autotune", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _u32p, ctypes.c_char_p,
+                                                   ctypes.c_size_t]),
+    ("stwo_rocm_batched_run", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_double), _u32p, ctypes.c_uint64,
+                                              ctypes.POINTER(ctypes.c_uint64)]),
+    ("stwo_rocm_batched_destroy", ctypes.c_int, [_vp, _vp]),
+    ("stwo_rocm_serve", ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                        ctypes.c_void_p, ctypes.c_void_p]),
     ("stwo_rocm_profile_enable", ctypes.c_int, [_vp, ctypes.c_int]),
     ("stwo_rocm_profile_select", ctypes.c_int, [_vp, ctypes.c_char_p]),
     ("stwo_rocm_profile_read", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t,
@@ -505,6 +515,82 @@ class Context:
                 continue
             self._check(rc)
             return out[: n.value].copy(), ms.value
+
+
+    def batched(self, n_columns=64, log_min=14, log_max=22, log_blowup=2, fri_log=22, n_folds=3, seed=0):
+        return BatchedPipeline(self, n_columns, log_min, log_max, log_blowup, fri_log, n_folds, seed)
+
+
+class BatchedSpec(ctypes.Structure):
+    _fields_ = [("n_columns", ctypes.c_uint32), ("log_min", ctypes.c_uint32), ("log_max", ctypes.c_uint32),
+                ("log_blowup", ctypes.c_uint32), ("fri_log", ctypes.c_uint32), ("n_folds", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64)]
+
+
+class BatchedPipeline:
+    """Planned batched LDE + Merkle + FRI pipeline replayed as one hipGraph."""
+
+    def __init__(self, ctx, *spec):
+        self.ctx = ctx
+        self.spec = BatchedSpec(*spec)
+        h = _vp()
+        ctx._check(ctx.lib.stwo_rocm_batched_create(ctx.h, ctypes.addressof(self.spec), ctypes.byref(h)))
+        self.h = h
+
+    def set_tuning(self, fft_tile_log, merkle_levels):
+        self.ctx._check(self.ctx.lib.stwo_rocm_batched_set_tuning(self.ctx.h, self.h, fft_tile_log, merkle_levels))
+
+    def autotune(self, reps=3):
+        import json
+        t, pt = _u32(np.zeros(1))
+        l, pl = _u32(np.zeros(1))
+        buf = ctypes.create_string_buffer(1 << 14)
+        self.ctx._check(self.ctx.lib.stwo_rocm_batched_autotune(self.ctx.h, self.h, reps, pt, pl, buf, 1 << 14))
+        return int(t[0]), int(l[0]), json.loads(buf.value.decode())
+
+    def run(self, n_runs=1, graph=True):
+        out, pout = _u32(np.zeros(4096))
+        n = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        self.ctx._check(self.ctx.lib.stwo_rocm_batched_run(self.ctx.h, self.h, n_runs, 1 if graph else 0,
+                                                           ctypes.byref(ms), pout, 4096, ctypes.byref(n)))
+        return out[: n.value].copy(), ms.value
+
+    def close(self):
+        if self.h:
+            self.ctx._check(self.ctx.lib.stwo_rocm_batched_destroy(self.ctx.h, self.h))
+            self.h = None
+
+
+class ServeJob(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("log_n", ctypes.c_uint32), ("seq_len", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("arrival_ms", ctypes.c_double)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("digest", ctypes.c_uint8 * 32), ("arrival_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("end_ms", ctypes.c_double), ("slot", ctypes.c_uint32), ("status", ctypes.c_uint32),
+                ("output_bytes", ctypes.c_uint64)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("makespan_ms", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("jobs_per_s", ctypes.c_double),
+                ("peak_admitted_bytes", ctypes.c_uint64), ("max_in_flight", ctypes.c_uint32),
+                ("n_slots", ctypes.c_uint32)]
+
+
+def serve(ctx, jobs, n_slots, hbm_budget_bytes=0):
+    """jobs: list of (kind, log_n, seq_len, seed, arrival_ms).  Returns (results as dicts, stats dict)."""
+    arr = (ServeJob * len(jobs))(*[ServeJob(k, n, w, 0, s, a) for k, n, w, s, a in jobs])
+    res = (ServeResult * len(jobs))()
+    st = ServeStats()
+    ctx._check(ctx.lib.stwo_rocm_serve(ctx.h, ctypes.addressof(arr), len(jobs), n_slots, hbm_budget_bytes,
+                                       ctypes.addressof(res), ctypes.addressof(st)))
+    out = [{"digest": bytes(r.digest).hex(), "arrival_ms": r.arrival_ms, "start_ms": r.start_ms,
+            "end_ms": r.end_ms, "slot": r.slot, "output_bytes": r.output_bytes} for r in res]
+    stats = {k: getattr(st, k) for k, _ in ServeStats._fields_}
+    return out, stats
 
 
 # Protocol presets (src/prover/native/config.zig:139-159).

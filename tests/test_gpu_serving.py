@@ -1,0 +1,59 @@
+"""Continuous-batching service on one GPU: every job's result is the same alone and in the mix,
+wide-Fibonacci proofs in the mix equal the oracle's proofs byte for byte (Blake2s digest of the
+proof bytes), LogUp jobs equal the oracle's LogUp workload words, and the HBM budget bounds the
+number of jobs admitted together."""
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import stwo_rocm
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+JOBS = [  # (kind, log_n, seq_len, seed, arrival_ms)
+    (0, 10, 16, 101, 0.0), (1, 16, 0, 7, 0.5), (0, 8, 8, 102, 1.0), (0, 12, 32, 103, 1.5),
+    (0, 9, 64, 104, 2.0), (1, 16, 0, 8, 3.0), (0, 11, 24, 105, 3.5), (0, 10, 100, 106, 6.0),
+]
+
+
+def _ref_digest(job):
+    kind, n, w, seed, _ = job
+    if kind == 0:
+        proof = oracle.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=seed)
+        return hashlib.blake2s(proof).hexdigest()
+    words = oracle.logup_xor_pipeline(n, seed)
+    return hashlib.blake2s(words.astype(np.uint32).tobytes()).hexdigest()
+
+
+def test_serving_mix_equals_alone_and_oracle(ctx):
+    _say("serving mix", len(JOBS))
+    mix, st = stwo_rocm.serve(ctx, JOBS, n_slots=3)
+    assert st["n_slots"] == 3 and 1 <= st["max_in_flight"] <= 3
+    for job, r in zip(JOBS, mix):
+        alone, _ = stwo_rocm.serve(ctx, [job[:4] + (0.0,)], n_slots=1)
+        assert alone[0]["digest"] == r["digest"], job
+        assert r["digest"] == _ref_digest(job), job
+        assert r["end_ms"] >= r["start_ms"] >= r["arrival_ms"] - 1e-6
+
+
+def test_serving_budget_limits_concurrency(ctx):
+    jobs = [(0, 12, 32, 200 + i, 0.0) for i in range(6)]
+    per_job = 16 * (1 << 12) * 32 + 1024 * (1 << 12) + (64 << 20)  # the admission estimate
+    res, st = stwo_rocm.serve(ctx, jobs, n_slots=4, hbm_budget_bytes=2 * per_job)
+    assert st["max_in_flight"] <= 2 and st["peak_admitted_bytes"] <= 2 * per_job
+    ref, _ = stwo_rocm.serve(ctx, jobs, n_slots=1)
+    assert [r["digest"] for r in res] == [r["digest"] for r in ref]
+
+
+def test_serving_rejects_job_over_budget(ctx):
+    with pytest.raises(stwo_rocm.RocmError):
+        stwo_rocm.serve(ctx, [(0, 12, 32, 1, 0.0)], n_slots=1, hbm_budget_bytes=1 << 20)

@@ -118,8 +118,9 @@ BatchedPipeline::BatchedPipeline(Ctx& cc, const BatchedSpec& s) : c(cc), sp(s), 
         sc[i] = ld[order[i]];
         sl[i] = logs[order[i]] + B;
     }
-    std::vector<LeafColDev> tab = merkle_leaf_table(sc.data(), sl.data(), sp.n_columns, max_lde);
-    leaf_tab = upload(c, tab.data(), tab.size());
+    LeafPlan plan = merkle_leaf_plan(sc.data(), sl.data(), sp.n_columns);
+    leaf_tab = upload(c, plan.tab.data(), plan.tab.size());
+    leaf_segs = plan.segs;
     tree = owner.make(max_lde);
     // FRI lines and trees
     lines.reserve(sp.n_folds + 1);
@@ -150,7 +151,7 @@ void BatchedPipeline::record() {
     for (auto& sc : classes)
         launch_circle_fft(c, false, sc.d_coeff.as<const uint32_t* const>(), sc.d_lde.as<uint32_t* const>(), sc.n,
                           sc.log + B, sc.log);
-    launch_merkle_leaves_dev(c, leaf_tab.as<LeafColDev>(), sp.n_columns, max_lde, tree->layer(max_lde));
+    launch_merkle_leaves_dev(c, leaf_tab.as<LeafColDev>(), leaf_segs, sp.n_columns, *tree);
     launch_merkle_parents(c, *tree);
     DevChannel* d_ch = chan.as<DevChannel>();
     uint8_t* d_roots = chan.as<uint8_t>() + CH_BYTES;

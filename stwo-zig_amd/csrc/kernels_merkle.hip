@@ -6,6 +6,13 @@
 // One leaf per lane; the 64-byte prefix is pre-absorbed into a seed state, then
 // 16 column words per compression are read with coalesced loads (consecutive
 // lanes read consecutive rows of the same column).
+// Shared prefixes: with the columns sorted by size, the hash state after message
+// block k depends only on the lifted row at level M_k, the largest log among the
+// columns of blocks 0..k (lifts compose: lifting L -> M then M -> l equals
+// lifting L -> l).  Blocks of equal M_k form a segment; each segment is one
+// launch over its 2^M_k states that starts from the previous segment's states
+// (lifted) and writes its own, so a block is compressed 2^M_k times instead of
+// 2^L times.  The result is the same leaf hash.
 //
 // Parents (layers.zig:99-140; blake2_merkle.zig:112-154): node = Blake2s(
 // "node"+0^60 || left || right), one compression from the node seed.  One
@@ -36,32 +43,44 @@ __device__ __forceinline__ void leaf_block_load(const LeafColDev* __restrict__ c
     }
 }
 
-// One leaf per lane; the 16 words of the next message block are loaded while
-// the current block is compressed.
+// One state per lane over blocks [blk0, blk0 + nblk) of a segment at `level`: the
+// lane starts from the seed (first segment) or from the previous segment's state
+// at the lifted row, loads the 16 words of the next block while the current one is
+// compressed, and stores the state (the leaf hash after the last block).
 __global__ __launch_bounds__(MK_THREADS) void merkle_leaves_kernel(const LeafColDev* __restrict__ cols,
-                                                                   uint32_t n_cols, uint32_t max_log, B2State seed,
-                                                                   uint8_t* __restrict__ leaves) {
+                                                                   uint32_t blk0, uint32_t nblk, uint32_t n_cols,
+                                                                   uint32_t level, const uint32_t* __restrict__ in,
+                                                                   uint32_t in_shift, B2State seed,
+                                                                   uint8_t* __restrict__ out) {
     const uint64_t p64 = (uint64_t)blockIdx.x * MK_THREADS + threadIdx.x;
-    if (p64 >= (1ull << max_log)) return;
+    if (p64 >= (1ull << level)) return;
     const uint32_t p = (uint32_t)p64;
     uint32_t h[8];
+    if (in) {
+        const SR_GLOBAL u32x4* s4 = (const SR_GLOBAL u32x4*)((gcu32*)in + 8ull * (((p >> in_shift) << 1) + (p & 1)));
+        const u32x4 a = s4[0], b = s4[1];
+        h[0] = a.x, h[1] = a.y, h[2] = a.z, h[3] = a.w, h[4] = b.x, h[5] = b.y, h[6] = b.z, h[7] = b.w;
+    } else {
 #pragma unroll
-    for (int i = 0; i < 8; i++) h[i] = seed.h[i];
-    const uint32_t nblocks = (n_cols + 15) >> 4;
+        for (int i = 0; i < 8; i++) h[i] = seed.h[i];
+    }
+    const uint32_t nblocks_total = (n_cols + 15) >> 4;
+    const uint32_t blk_end = blk0 + nblk;
     uint32_t cur[16];
-    leaf_block_load(cols, p, cur);
-    for (uint32_t blk = 0; blk < nblocks; blk++) {
+    leaf_block_load(cols + 16 * blk0, p, cur);
+    for (uint32_t blk = blk0; blk < blk_end; blk++) {
         uint32_t nxt[16];
-        const bool more = blk + 1 < nblocks;
+        const bool more = blk + 1 < blk_end;
         if (more) leaf_block_load(cols + 16 * (blk + 1), p, nxt);
-        const uint32_t t = 64 + (more ? 64 * (blk + 1) : 4 * n_cols);
-        b2_compress(h, cur, t, !more);
+        const bool last = blk + 1 == nblocks_total;
+        const uint32_t t = 64 + (last ? 4 * n_cols : 64 * (blk + 1));
+        b2_compress(h, cur, t, last);
         if (more) {
 #pragma unroll
             for (int w = 0; w < 16; w++) cur[w] = nxt[w];
         }
     }
-    store_hash(leaves, p, h);
+    store_hash(out, p, h);
 }
 
 // Leaves of a QM31 column in AoS form: 4 coordinate words per leaf (FRI layers).
@@ -216,35 +235,73 @@ void launch_merkle_parent_layer(Ctx& c, const uint8_t* d_children, uint32_t log_
     SR_HIP(hipGetLastError());
 }
 
-std::vector<LeafColDev> merkle_leaf_table(const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
-                                          uint32_t max_log) {
-    SR_CHECK(n_cols > 0 && max_log <= 31, "merkle leaves: shape");
-    std::vector<LeafColDev> tab(((n_cols + 15) / 16) * 16);
-    for (size_t i = 0; i < tab.size(); i++) {
-        const bool real = i < n_cols;
-        const uint32_t lg = real ? h_logs[i] : h_logs[0];
-        SR_CHECK(lg >= 1 && lg <= max_log, "merkle leaves: column log size");
-        tab[i].col = real ? h_cols[i] : h_cols[0];
-        tab[i].shift = max_log - lg + 1;
-        tab[i].mask = real ? 0xffffffffu : 0u;
+LeafPlan merkle_leaf_plan(const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols) {
+    SR_CHECK(n_cols > 0, "merkle leaves: no columns");
+    const uint32_t nblocks = (n_cols + 15) / 16;
+    LeafPlan plan;
+    plan.tab.resize(16ull * nblocks);
+    // block k's level: the running maximum of the column logs up to the end of block k
+    uint32_t level = 0;
+    for (uint32_t k = 0; k < nblocks; k++) {
+        for (uint32_t i = 16 * k; i < std::min(n_cols, 16 * k + 16); i++) {
+            SR_CHECK(h_logs[i] >= 1 && h_logs[i] <= 30, "merkle leaves: column log size");
+            SR_CHECK(i == 0 || h_logs[i - 1] <= h_logs[i], "merkle leaves: columns must be sorted by log size");
+            level = std::max(level, h_logs[i]);
+        }
+        if (plan.segs.empty() || plan.segs.back().level != level) plan.segs.push_back(LeafSeg{k, 0, level, 0});
+        LeafSeg& sg = plan.segs.back();
+        sg.nblk++;
+        for (uint32_t w = 0; w < 16; w++) {
+            const uint32_t i = 16 * k + w;
+            const bool real = i < n_cols;
+            const uint32_t lg = real ? h_logs[i] : h_logs[0];
+            LeafColDev& d = plan.tab[i];
+            d.col = real ? h_cols[i] : h_cols[0];
+            d.shift = level - lg + 1;
+            d.mask = real ? 0xffffffffu : 0u;
+            if (real) sg.col_words += 1ull << lg;
+        }
     }
-    return tab;
+    return plan;
 }
 
-void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, uint32_t n_cols, uint32_t max_log, uint8_t* d_leaves) {
-    uint64_t n = 1ull << max_log;
-    KTimer kt(c, "merkle_leaves L=" + std::to_string(max_log) + " cols=" + std::to_string(n_cols),
-              (4.0 * n_cols + 32.0) * (double)n);
-    hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)), dim3(MK_THREADS), 0,
-                       c.stream, d_tab, n_cols, max_log, c.leaf_seed, d_leaves);
-    SR_HIP(hipGetLastError());
+void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, const std::vector<LeafSeg>& segs, uint32_t n_cols,
+                              DeviceTree& t) {
+    const uint32_t L = t.max_log;
+    const size_t S = segs.size();
+    SR_CHECK(S >= 1 && segs.back().level == L && segs.back().blk0 + segs.back().nblk == (n_cols + 15) / 16,
+             "merkle leaves: segment plan does not match the tree");
+    const uint32_t* prev = nullptr;
+    uint32_t prev_level = 0;
+    for (size_t s = 0; s < S; s++) {
+        const LeafSeg& sg = segs[s];
+        SR_CHECK(s == 0 || sg.level > prev_level, "merkle leaves: segment levels must increase");
+        const bool final = s + 1 == S;
+        // intermediate states: layers L-1 and L-2 alternately, counted back from the final
+        // segment (level M_s <= L - (S-1-s) fits layer L-1 or L-2 by parity); the parent
+        // launches overwrite both afterwards
+        uint8_t* out = final ? t.layer(L) : t.layer(((S - 1 - s) & 1) ? L - 1 : L - 2);
+        const uint64_t n = 1ull << sg.level;
+        double bytes = 4.0 * (double)sg.col_words + 32.0 * (double)n + (prev ? 32.0 * (double)(1ull << prev_level) : 0.0);
+        KTimer kt(c,
+                  final ? "merkle_leaves L=" + std::to_string(L) + " cols=" + std::to_string(n_cols) +
+                              (S > 1 ? " blocks=" + std::to_string(sg.nblk) : "")
+                        : "merkle_leaves_prefix L=" + std::to_string(sg.level) + " blocks=" + std::to_string(sg.nblk),
+                  bytes);
+        hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)), dim3(MK_THREADS),
+                           0, c.stream, d_tab, sg.blk0, sg.nblk, n_cols, sg.level, prev,
+                           prev ? sg.level - prev_level + 1 : 0u, c.leaf_seed, out);
+        SR_HIP(hipGetLastError());
+        prev = (const uint32_t*)out;
+        prev_level = sg.level;
+    }
 }
 
 void launch_merkle_leaves(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
-                          uint32_t max_log, uint8_t* d_leaves) {
-    std::vector<LeafColDev> tab = merkle_leaf_table(h_cols, h_logs, n_cols, max_log);
-    DBuf d_tab = upload(c, tab.data(), tab.size());
-    launch_merkle_leaves_dev(c, d_tab.as<LeafColDev>(), n_cols, max_log, d_leaves);
+                          DeviceTree& t) {
+    LeafPlan plan = merkle_leaf_plan(h_cols, h_logs, n_cols);
+    DBuf d_tab = upload(c, plan.tab.data(), plan.tab.size());
+    launch_merkle_leaves_dev(c, d_tab.as<LeafColDev>(), plan.segs, n_cols, t);
 }
 
 void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size, uint8_t* d_leaves) {
@@ -314,7 +371,7 @@ DeviceTree* merkle_commit(Ctx& c, const std::vector<const uint32_t*>& cols, cons
         sc[i] = cols[order[i]];
         sl[i] = logs[order[i]];
     }
-    launch_merkle_leaves(c, sc.data(), sl.data(), (uint32_t)sc.size(), max_log, t->layer(max_log));
+    launch_merkle_leaves(c, sc.data(), sl.data(), (uint32_t)sc.size(), *t);
     launch_merkle_parents(c, *t);
     return t;
 }

@@ -132,6 +132,7 @@ class BatchedPipeline {
     TreeOwner owner;
     std::vector<uint32_t> logs;
     uint32_t max_lde = 0;
+    std::vector<LeafSeg> leaf_segs;  // leaf-hash segment plan of the lifted tree
     DBuf evals, coeffs, lde, leaf_tab, chan;
     std::vector<SizeClass> classes;
     DeviceTree* tree = nullptr;

@@ -179,17 +179,30 @@ void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint3
 // mask 0 (padding reads column 0 and contributes a zero word).
 struct LeafColDev {
     const uint32_t* col;
-    uint32_t shift;  // lift shift max_log - log + 1 (1 = full size)
+    uint32_t shift;  // lift shift level - log + 1 for the level of the column's segment (1 = full size)
     uint32_t mask;   // 0xffffffff, or 0 for padding
 };
 static_assert(sizeof(LeafColDev) == 16, "leaf column layout");
-std::vector<LeafColDev> merkle_leaf_table(const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
-                                          uint32_t max_log);
-// Leaves from a device-resident descriptor table (no host traffic: capturable in a graph).
-void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, uint32_t n_cols, uint32_t max_log, uint8_t* d_leaves);
-// Host arrays in: column device pointers and log sizes, sorted by log size.
+// Message blocks [blk0, blk0 + nblk) of the leaf hash, absorbed over 2^level lifted rows.
+struct LeafSeg {
+    uint32_t blk0, nblk, level;
+    uint64_t col_words;  // sum of the segment's column sizes (words): algorithmic byte count only
+};
+// Descriptor table (16 entries per message block) and the segment plan for columns sorted
+// by log size; every column's shift is relative to its segment's level.
+struct LeafPlan {
+    std::vector<LeafColDev> tab;
+    std::vector<LeafSeg> segs;
+};
+LeafPlan merkle_leaf_plan(const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols);
+// Leaves of t from a device-resident copy of plan.tab (no host traffic: capturable in a graph).
+// Intermediate segment states live in t's two parent layers below the leaves until the
+// parents are built.
+void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, const std::vector<LeafSeg>& segs, uint32_t n_cols,
+                              DeviceTree& t);
+// Host arrays in: column device pointers and log sizes, sorted by log size; max log = t.max_log.
 void launch_merkle_leaves(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
-                          uint32_t max_log, uint8_t* d_leaves);
+                          DeviceTree& t);
 void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size, uint8_t* d_leaves);
 // All parent layers above t's leaves.  With d_ch, the launch reaching the root
 // also mixes it into the device channel and draws the next alpha (root copied to d_root_out).

@@ -84,6 +84,42 @@ def test_merkle_mixed_sizes_matches_oracle(ctx):
         b.free()
 
 
+STAGED = {
+    # (count, log) groups; every 16-column message block whose largest log grows starts a new
+    # shared-prefix segment of the leaf hash
+    "four_segments": [(16, 3), (16, 5), (16, 8), (5, 11)],
+    "two_levels_tiny": [(17, 1), (1, 2)],
+    "same_level_blocks": [(32, 6), (3, 7)],
+    "full_last_block": [(16, 4), (16, 9), (16, 10)],
+    "five_segments_wide": [(3, 8), (13, 9), (16, 10), (16, 12), (16, 14), (2, 16)],
+}
+
+
+@pytest.mark.parametrize("name", list(STAGED))
+def test_merkle_staged_sizes_shared_prefixes(ctx, name):
+    rng = np.random.default_rng(len(name))
+    logs = [k for n, k in STAGED[name] for _ in range(n)]
+    perm = rng.permutation(len(logs))  # commit in shuffled order: the library sorts stably
+    logs = [logs[i] for i in perm]
+    cols = [rng.integers(0, mr.P, 1 << k, dtype=np.uint32) for k in logs]
+    bufs = [ctx.to_device(c) for c in cols]
+    _say("merkle staged", name, len(cols))
+    tree = ctx.commit_merkle([b.ptr for b in bufs], logs)
+    root, layers = oracle.merkle_commit(cols)
+    L = max(logs)
+    assert tree.root == root
+    idx = sorted({0, 1, 2, (1 << L) - 1, (1 << L) // 3})
+    assert b"".join(tree.read_hashes(L, idx)) == b"".join(layers[L][32 * i: 32 * i + 32] for i in idx)
+    # the two layers below the leaves held intermediate states before the parents were built
+    for lay in (L - 1, L - 2):
+        if lay >= 0:
+            n = 1 << lay
+            assert b"".join(tree.read_hashes(lay, list(range(min(n, 8))))) == layers[lay][: 32 * min(n, 8)]
+    tree.free()
+    for b in bufs:
+        b.free()
+
+
 @pytest.mark.parametrize("n_cols,log", [(0, 0), (1, 1), (1, 2), (3, 3), (5, 6), (16, 8), (17, 8), (2, 10), (2, 11),
                                         (100, 14), (8, 21)])
 def test_merkle_uniform_matches_oracle(ctx, n_cols, log):

@@ -1,10 +1,27 @@
-// GKR batch prover over device-resident layers (src/prover/lookups/
+// GKR batch prover with the whole transcript on the device (src/prover/lookups/
 // gkr_prover.zig:34-298 with sumcheck.zig:76-197 and the oracle of
-// gkr_circuit.zig:231-429).  The layer stack, the eq table, the per-round
-// sums and the variable fixing run on the GPU (kernels_logup.hip); the host
-// keeps the transcript, the per-instance claims and the degree-3 round
-// polynomials (four QM31 coefficients each).  One device round trip per
-// sum-check round serves every instance of the batch.
+// gkr_circuit.zig:231-429).
+//
+// The layer stacks (nextLayer), the eq tables, the sum-check rounds, the
+// channel (mixFelts / drawSecureFelt on one quad of lanes, chan.hpp) and the
+// masks all run on the GPU; the host only enqueues launches and reads the
+// transcript back once.  Per GKR layer L (L sum-check rounds):
+//   * begin: newly active instances take their output claims, the claims are
+//     mixed, alpha and lambda drawn, the batched claims formed; one lane per
+//     round rho precomputes what the round polynomial needs from the eq point
+//     y (the eq-prefix inverse prod_{i<=rho} 1/(1-y_i), the extra Lagrange node
+//     b = (1-y_rho)/(1-2y_rho) and the basis coefficients for the nodes
+//     0, 1, 2, b), so no round inverts anything;
+//   * rounds: fixing the previous round's variable is fused into this round's
+//     sums (in place: index j < h is written by the thread that reads j and
+//     j + h); rounds on layers above 2^GD_SMALL_LOG values are one launch over
+//     many workgroups whose last workgroup (atomic counter, no waiting) turns
+//     the sums into the round polynomial, mixes it and draws the challenge;
+//     smaller rounds, whole small layers and the layer end (masks, challenge,
+//     next claims) run in one workgroup.
+// The round polynomial r(x) through (0, r0), (1, claim - r0), (2, r2), (b, 0)
+// (correctSumAsPolyInFirstVariable, gkr_circuit.zig:390-429) is unique, so the
+// closed-form coefficients equal the reference's Lagrange interpolation.
 //
 // Proof word stream (what stwo_rocm_gkr_prove_batch returns):
 //   per GKR layer: n_rounds, each [len, len QM31 coefficients];
@@ -13,6 +30,7 @@
 //   per instance: [n, claims to verify], n_variables; final channel digest (8 words).
 #include <vector>
 
+#include "chan.hpp"
 #include "prover.hpp"
 
 namespace sr {
@@ -88,280 +106,9 @@ Poly correct_sum(qm31 f0, qm31 f2, qm31 claim, const std::vector<qm31>& y, uint3
     return lagrange(xs, ys);
 }
 
-// One instance's sum-check oracle: its current layer lives in device memory.
-struct Oracle {
-    size_t inst;
-    int kind;
-    uint32_t n_vars;  // variables of the layer (the oracle has n_vars - 1)
-    const uint32_t* num;
-    const uint32_t* den;
-    std::vector<DBuf> owned;  // buffers produced by fixing variables
-    qm31 corr = qone();
-};
-
 qm31 from_acc(const uint64_t* a) { return {{mred64(a[0]), mred64(a[1])}, {mred64(a[2]), mred64(a[3])}}; }
 
 }  // namespace
-
-std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<GkrInstance>& inst) {
-    SR_CHECK(!inst.empty(), "gkr: empty batch");
-    const size_t n_inst = inst.size();
-    uint32_t n_layers = 0;
-    for (auto& in : inst) {
-        SR_CHECK(in.kind >= 0 && in.kind <= 3, "gkr: layer kind");
-        SR_CHECK(in.n_vars >= 1 && in.n_vars <= 30, "gkr: an instance needs 1..30 variables");
-        SR_CHECK(in.d_den && (in.kind == GKR_GRAND_PRODUCT || in.kind == GKR_LOGUP_SINGLES || in.d_num),
-                 "gkr: missing layer columns");
-        n_layers = std::max(n_layers, in.n_vars);
-    }
-    // Layer stacks: layer l of instance k has l variables (buildLayers, gkr_circuit.zig:431-447).
-    struct Stack {
-        std::vector<DBuf> num, den;  // index l < n_vars
-        std::vector<int> kind;
-    };
-    std::vector<Stack> st(n_inst);
-    for (size_t k = 0; k < n_inst; k++) {
-        const uint32_t nv = inst[k].n_vars;
-        st[k].kind.assign(nv + 1, inst[k].kind == GKR_GRAND_PRODUCT ? GKR_GRAND_PRODUCT : GKR_LOGUP_GENERIC);
-        st[k].kind[nv] = inst[k].kind;
-        st[k].num.reserve(nv);
-        st[k].den.reserve(nv);
-        for (uint32_t l = 0; l < nv; l++) {
-            st[k].num.emplace_back(&c, inst[k].kind == GKR_GRAND_PRODUCT ? 16 : (16ull << l));
-            st[k].den.emplace_back(&c, 16ull << l);
-        }
-    }
-    auto layer_num = [&](size_t k, uint32_t l) -> const uint32_t* {
-        return l == inst[k].n_vars ? inst[k].d_num : st[k].num[l].as<uint32_t>();
-    };
-    auto layer_den = [&](size_t k, uint32_t l) -> const uint32_t* {
-        return l == inst[k].n_vars ? inst[k].d_den : st[k].den[l].as<uint32_t>();
-    };
-    for (size_t k = 0; k < n_inst; k++)
-        for (int32_t l = (int32_t)inst[k].n_vars - 1; l >= 0; l--)
-            launch_gkr_next_layer(c, st[k].kind[l + 1], layer_num(k, l + 1), layer_den(k, l + 1), (uint32_t)l + 1,
-                                  st[k].kind[l] == GKR_GRAND_PRODUCT ? nullptr : st[k].num[l].as<uint32_t>(),
-                                  st[k].den[l].as<uint32_t>());
-    // Output layers (gkr_circuit.zig:162-190): one download for every instance.
-    std::vector<std::vector<qm31>> outputs(n_inst);
-    {
-        uint32_t* h = (uint32_t*)c.pinned(32 * n_inst);
-        for (size_t k = 0; k < n_inst; k++) {
-            const uint32_t* n0 = layer_num(k, 0);
-            if (st[k].kind[0] != GKR_GRAND_PRODUCT && n0)
-                SR_HIP(hipMemcpyAsync(h + 8 * k, n0, 16, hipMemcpyDeviceToHost, c.stream));
-            SR_HIP(hipMemcpyAsync(h + 8 * k + 4, layer_den(k, 0), 16, hipMemcpyDeviceToHost, c.stream));
-        }
-        c.sync();
-        for (size_t k = 0; k < n_inst; k++) {
-            if (st[k].kind[0] == GKR_GRAND_PRODUCT) {
-                outputs[k] = {qload(h + 8 * k + 4)};
-            } else {
-                qm31 nu = st[k].kind[0] == GKR_LOGUP_SINGLES ? qone() : qload(h + 8 * k);
-                outputs[k] = {nu, qload(h + 8 * k + 4)};
-            }
-        }
-    }
-
-    std::vector<uint32_t> w;
-    auto put_q = [&](qm31 v) {
-        uint32_t x[4];
-        qstore(v, x);
-        w.insert(w.end(), x, x + 4);
-    };
-    std::vector<std::vector<qm31>> claims(n_inst);
-    std::vector<bool> active(n_inst, false);
-    std::vector<qm31> ood;
-    DBuf d_acc(&c, 8 * 8 * n_inst);
-    DBuf d_y(&c, 16 * (n_layers + 1));
-    for (uint32_t layer = 0; layer < n_layers; layer++) {
-        const uint32_t remaining = n_layers - layer;
-        for (size_t k = 0; k < n_inst; k++)
-            if (inst[k].n_vars == remaining) {
-                claims[k] = outputs[k];
-                active[k] = true;
-            }
-        for (size_t k = 0; k < n_inst; k++)
-            if (active[k]) ch.mix_felts(claims[k].data(), claims[k].size());
-        // EqEvals.generate(ood) = genEqEvals(ood[1..], 1 - ood[0]) on the device.
-        const uint32_t ylen = (uint32_t)ood.size();
-        DBuf d_eq(&c, ylen ? (16ull << (ylen - 1)) : 16);
-        if (ylen == 0) {
-            const uint32_t one[4] = {1, 0, 0, 0};
-            SR_HIP(hipMemcpyAsync(d_eq.p, c.stage(one, 16), 16, hipMemcpyHostToDevice, c.stream));
-        } else {
-            std::vector<uint32_t> yw(4 * ylen);
-            for (uint32_t i = 0; i < ylen; i++) qstore(ood[i], &yw[4 * i]);
-            SR_HIP(hipMemcpyAsync(d_y.p, c.stage(yw.data(), 16 * ylen), 16 * ylen, hipMemcpyHostToDevice, c.stream));
-            launch_gkr_eq_evals(c, d_y.as<uint32_t>() + 4, ylen - 1, qsub(qone(), ood[0]), d_eq.as<uint32_t>());
-        }
-        const qm31 sc_alpha = ch.draw_secure_felt();
-        const qm31 lambda = ch.draw_secure_felt();
-        // Sum-check over the active instances (sumcheck.zig:76-197).
-        std::vector<Oracle> orc;
-        std::vector<qm31> sc_claims;
-        for (size_t k = 0; k < n_inst; k++) {
-            if (!active[k]) continue;
-            const uint32_t lv = inst[k].n_vars - remaining + 1;
-            Oracle o;
-            o.inst = k;
-            o.kind = st[k].kind[lv];
-            o.n_vars = lv;
-            o.num = layer_num(k, lv);
-            o.den = layer_den(k, lv);
-            qm31 cl = qzero();  // randomLinearCombination(claims, lambda)
-            for (size_t i = claims[k].size(); i > 0; i--) cl = qadd(qmul(cl, lambda), claims[k][i - 1]);
-            sc_claims.push_back(cl);
-            orc.push_back(std::move(o));
-        }
-        uint32_t sc_nv = 0;
-        for (auto& o : orc) sc_nv = std::max(sc_nv, o.n_vars - 1);
-        for (size_t i = 0; i < orc.size(); i++) {
-            uint32_t unused = sc_nv - (orc[i].n_vars - 1);
-            uint32_t p2 = 1;
-            for (uint32_t u = 0; u < unused; u++) p2 = madd(p2, p2);
-            sc_claims[i] = qmulm(sc_claims[i], p2);
-        }
-        w.push_back(sc_nv);
-        std::vector<qm31> assignment;
-        const uint32_t inv2 = minv(2);
-        for (uint32_t round = 0; round < sc_nv; round++) {
-            const uint32_t rem = sc_nv - round;
-            // device sums of every instance whose variables are all still free
-            launch_zero_u64(c, d_acc.as<unsigned long long>(), (uint32_t)(8 * orc.size()));
-            bool any = false;
-            for (size_t i = 0; i < orc.size(); i++)
-                if (orc[i].n_vars - 1 == rem) {
-                    launch_gkr_sums(c, orc[i].kind, d_eq.as<uint32_t>(), orc[i].num, orc[i].den, orc[i].n_vars, lambda,
-                                    d_acc.as<unsigned long long>() + 8 * i);
-                    any = true;
-                }
-            std::vector<uint64_t> acc(8 * orc.size(), 0);
-            if (any) {
-                uint64_t* h = (uint64_t*)c.pinned(64 * orc.size());
-                SR_HIP(hipMemcpyAsync(h, d_acc.p, 64 * orc.size(), hipMemcpyDeviceToHost, c.stream));
-                c.sync();
-                std::copy(h, h + 8 * orc.size(), acc.begin());
-            }
-            std::vector<Poly> rp(orc.size());
-            for (size_t i = 0; i < orc.size(); i++) {
-                const uint32_t onv = orc[i].n_vars - 1;
-                if (rem == onv) {
-                    const qm31 f0 = qmul(from_acc(&acc[8 * i]), orc[i].corr);
-                    const qm31 f2 = qmul(from_acc(&acc[8 * i + 4]), orc[i].corr);
-                    rp[i] = correct_sum(f0, f2, sc_claims[i], ood, onv);
-                } else {
-                    rp[i].c = {qmulm(sc_claims[i], inv2)};
-                    rp[i].trim();
-                }
-            }
-            Poly comb;  // Horner combination with sc_alpha (sumcheck.zig:215-238)
-            for (size_t i = orc.size(); i > 0; i--) {
-                Poly s;
-                if (!qiszero(sc_alpha))
-                    for (auto& v : comb.c) s.c.push_back(qmul(v, sc_alpha));
-                s.trim();
-                Poly t;
-                t.c.assign(std::max(s.c.size(), rp[i - 1].c.size()), qzero());
-                for (size_t k = 0; k < t.c.size(); k++)
-                    t.c[k] = qadd(k < s.c.size() ? s.c[k] : qzero(), k < rp[i - 1].c.size() ? rp[i - 1].c[k] : qzero());
-                t.trim();
-                comb = t;
-            }
-            ch.mix_felts(comb.c.data(), comb.c.size());
-            const qm31 r = ch.draw_secure_felt();
-            w.push_back((uint32_t)comb.c.size());
-            for (auto& v : comb.c) put_q(v);
-            for (size_t i = 0; i < orc.size(); i++) sc_claims[i] = rp[i].eval(r);
-            for (auto& o : orc) {
-                const uint32_t onv = o.n_vars - 1;
-                if (rem != onv) continue;
-                o.corr = qmul(o.corr, eq_one(r, ood[ood.size() - onv]));
-                const uint64_t half = 1ull << (o.n_vars - 1);
-                DBuf nd(&c, 16 * half);
-                launch_mle_fix_first(c, false, o.den, o.n_vars, r, nd.as<uint32_t>());
-                o.den = nd.as<uint32_t>();
-                o.owned.push_back(std::move(nd));
-                if (o.kind == GKR_LOGUP_GENERIC || o.kind == GKR_LOGUP_MULTIPLICITIES) {
-                    DBuf nn(&c, 16 * half);
-                    launch_mle_fix_first(c, o.kind == GKR_LOGUP_MULTIPLICITIES, o.num, o.n_vars, r,
-                                         nn.as<uint32_t>());
-                    o.num = nn.as<uint32_t>();
-                    o.owned.push_back(std::move(nn));
-                    o.kind = GKR_LOGUP_GENERIC;
-                }
-                o.n_vars--;
-            }
-            assignment.push_back(r);
-        }
-        // Masks of the constant oracles (gkr_circuit.zig:341-388): two values per column.
-        std::vector<std::vector<std::pair<qm31, qm31>>> masks(orc.size());
-        {
-            uint32_t* h = (uint32_t*)c.pinned(64 * orc.size());
-            for (size_t i = 0; i < orc.size(); i++) {
-                const Oracle& o = orc[i];
-                if (o.kind == GKR_LOGUP_GENERIC)
-                    SR_HIP(hipMemcpyAsync(h + 16 * i, o.num, 32, hipMemcpyDeviceToHost, c.stream));
-                else if (o.kind == GKR_LOGUP_MULTIPLICITIES)
-                    SR_HIP(hipMemcpyAsync(h + 16 * i, o.num, 8, hipMemcpyDeviceToHost, c.stream));
-                SR_HIP(hipMemcpyAsync(h + 16 * i + 8, o.den, 32, hipMemcpyDeviceToHost, c.stream));
-            }
-            c.sync();
-            for (size_t i = 0; i < orc.size(); i++) {
-                const Oracle& o = orc[i];
-                const uint32_t* b = h + 16 * i;
-                const qm31 d0 = qload(b + 8), d1 = qload(b + 12);
-                if (o.kind == GKR_GRAND_PRODUCT) {
-                    masks[i] = {{d0, d1}};
-                } else if (o.kind == GKR_LOGUP_GENERIC) {
-                    masks[i] = {{qload(b), qload(b + 4)}, {d0, d1}};
-                } else if (o.kind == GKR_LOGUP_MULTIPLICITIES) {
-                    masks[i] = {{qfromm(b[0]), qfromm(b[1])}, {d0, d1}};
-                } else {
-                    masks[i] = {{qone(), qone()}, {d0, d1}};
-                }
-            }
-        }
-        w.push_back((uint32_t)orc.size());
-        for (size_t i = 0; i < orc.size(); i++) {
-            std::vector<qm31> flat;
-            for (auto& col : masks[i]) {
-                flat.push_back(col.first);
-                flat.push_back(col.second);
-            }
-            ch.mix_felts(flat.data(), flat.size());
-            w.push_back((uint32_t)orc[i].inst);
-            w.push_back((uint32_t)masks[i].size());
-            for (auto& v : flat) put_q(v);
-        }
-        const qm31 challenge = ch.draw_secure_felt();
-        ood = assignment;
-        ood.push_back(challenge);
-        for (size_t i = 0; i < orc.size(); i++) {
-            std::vector<qm31>& cl = claims[orc[i].inst];
-            cl.clear();
-            for (auto& col : masks[i]) cl.push_back(qadd(qmul(challenge, qsub(col.second, col.first)), col.first));
-        }
-    }
-    for (size_t k = 0; k < n_inst; k++) {
-        w.push_back((uint32_t)outputs[k].size());
-        for (auto& v : outputs[k]) put_q(v);
-    }
-    w.push_back((uint32_t)ood.size());
-    for (auto& v : ood) put_q(v);
-    for (size_t k = 0; k < n_inst; k++) {
-        w.push_back(active[k] ? (uint32_t)claims[k].size() : 0u);
-        if (active[k])
-            for (auto& v : claims[k]) put_q(v);
-        w.push_back(inst[k].n_vars);
-    }
-    for (int i = 0; i < 8; i++) {
-        uint32_t d;
-        memcpy(&d, ch.digest + 4 * i, 4);
-        w.push_back(d);
-    }
-    return w;
-}
 
 // GkrMultivariatePolyOracle.sumAsPolyInFirstVariable (gkr_circuit.zig:261-316) for one
 // layer on the device: returns the trimmed coefficients of the round polynomial.
@@ -378,6 +125,757 @@ std::vector<qm31> gkr_sum_as_poly(Ctx& c, int kind, const uint32_t* d_eq, const 
     c.sync();
     const qm31 f0 = qmul(from_acc(h), correction), f2 = qmul(from_acc(h + 4), correction);
     return correct_sum(f0, f2, claim, y, n_vars - 1).c;
+}
+
+
+// ====================================================================== device-resident batch prover
+namespace {
+
+constexpr uint32_t GD_MAX_INST = 8;
+constexpr uint32_t GD_MAX_VARS = 30;
+constexpr uint32_t GD_SMALL_LOG = 8;  // rounds on layers of <= 2^8 values run in one workgroup
+constexpr uint32_t GD_FT = 512;       // one-workgroup kernel: 7 summing waves + 1 for the pending update
+constexpr uint32_t GD_RT = 256;       // threads per workgroup of the round and begin kernels
+constexpr uint32_t GD_REC_ROUND = 17;  // [len, 4 QM31] per sum-check round
+constexpr uint32_t GD_REC_MASK = 16;   // [num0, num1, den0, den1] per instance and layer
+
+// Layer stack of one instance: layer l has l variables (2^l values); num is QM31 AoS,
+// M31 for a multiplicities input layer, absent for grand products and singles.
+struct GdInst {
+    const uint32_t* num[GD_MAX_VARS + 1];
+    const uint32_t* den[GD_MAX_VARS + 1];
+    uint32_t kind[GD_MAX_VARS + 1];
+    uint32_t n_vars;
+    uint32_t* scr_num;  // fixed layers: 2^(n_vars-1) QM31 each
+    uint32_t* scr_den;
+};
+
+struct GdState {
+    uint32_t dg[8];
+    uint32_t n_draws, status, counter, pad0;
+    uint32_t outputs[GD_MAX_INST][2][4];  // transcript order: [den] or [num, den]
+    uint32_t claims[GD_MAX_INST][2][4];
+    uint32_t sc_claim[GD_MAX_INST][4];
+    uint32_t corr[GD_MAX_INST][4];
+    uint32_t alpha_pow[GD_MAX_INST][4];
+    uint32_t lambda[4];
+    uint32_t r[4];
+    uint32_t ood[GD_MAX_VARS + 1][4];  // eq point of the current layer
+    uint32_t asg[GD_MAX_VARS][4];      // challenges of the current layer's rounds
+    uint32_t pre[GD_MAX_VARS][11][4];  // per round: F0, F2, basis rows 1..3 (3 each)
+    uint32_t mask[GD_MAX_INST][4][4];
+    // the last round's claim update, applied while the next round sums (or dropped at the
+    // layer end, where the claims are rebuilt): its coefficients, round and instance flags
+    uint32_t craw[GD_MAX_INST][4][4];
+    uint32_t pend, pend_rho, pad1, pad2;
+    uint32_t pend_flags[GD_MAX_INST];  // bit 0: active, bit 1: summed in that round
+    unsigned long long acc[GD_MAX_INST][8];
+};
+static_assert(sizeof(GdState) % 8 == 0, "state copy in 8-byte words");
+
+__device__ __forceinline__ qm31 gd_ld(const uint32_t* p, uint64_t i) {
+    const uint4 v = ((const uint4*)p)[i];
+    return {{v.x, v.y}, {v.z, v.w}};
+}
+__device__ __forceinline__ void gd_st(uint32_t* p, uint64_t i, qm31 v) {
+    ((uint4*)p)[i] = make_uint4(v.c0.a, v.c0.b, v.c1.a, v.c1.b);
+}
+__device__ __forceinline__ qm31 gd_num(uint32_t kind, const uint32_t* num, uint64_t j) {
+    if (kind == GKR_LOGUP_GENERIC) return gd_ld(num, j);
+    if (kind == GKR_LOGUP_MULTIPLICITIES) return qfromm(num[j]);
+    return qone();
+}
+__device__ __forceinline__ uint32_t gd_fixed_kind(uint32_t kind) {
+    return kind == GKR_LOGUP_MULTIPLICITIES ? (uint32_t)GKR_LOGUP_GENERIC : kind;
+}
+__device__ __forceinline__ bool gd_has_num(uint32_t kind) {
+    return kind == GKR_LOGUP_GENERIC || kind == GKR_LOGUP_MULTIPLICITIES;
+}
+__device__ __forceinline__ qm31 gd_shfl(qm31 v, int src) {
+    return {{(uint32_t)__shfl((int)v.c0.a, src), (uint32_t)__shfl((int)v.c0.b, src)},
+            {(uint32_t)__shfl((int)v.c1.a, src), (uint32_t)__shfl((int)v.c1.b, src)}};
+}
+__device__ __forceinline__ qm31 gd_shfl_xor(qm31 v, int m) {
+    return {{(uint32_t)__shfl_xor((int)v.c0.a, m), (uint32_t)__shfl_xor((int)v.c0.b, m)},
+            {(uint32_t)__shfl_xor((int)v.c1.a, m), (uint32_t)__shfl_xor((int)v.c1.b, m)}};
+}
+template <int CTRL>
+__device__ __forceinline__ qm31 gd_qperm(qm31 v) {
+    return {{quad_perm<CTRL>(v.c0.a), quad_perm<CTRL>(v.c0.b)}, {quad_perm<CTRL>(v.c1.a), quad_perm<CTRL>(v.c1.b)}};
+}
+__device__ __forceinline__ uint64_t gd_wave_sum(uint64_t v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m), hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+__device__ __forceinline__ void gd_qstore(uint32_t* w, qm31 v) { qstore(v, w); }
+
+// Instance view at GKR layer L: active once n_vars + L >= n_layers; its oracle layer has
+// onv + 1 variables, onv = L + n_vars - n_layers.
+struct GdView {
+    bool active;
+    uint32_t onv;
+};
+__device__ __forceinline__ GdView gd_view(const GdInst& in, uint32_t n_layers, uint32_t L) {
+    GdView v;
+    v.active = in.n_vars + L >= n_layers;
+    v.onv = v.active ? in.n_vars + L - n_layers : 0;
+    return v;
+}
+
+// Sum-check term k of one instance at round rho (nt = 2^(rem-1) terms): the four values
+// of the current layer at rows 2k, 2k+1, 2(nt+k), 2(nt+k)+1, fixed on the fly from the
+// previous layer (2x as long) when `fix`, then the terms at x = 0 and x = 2
+// (gkr_circuit.zig:505-643), each multiplied by eq[k].
+// The term is spread over a quad (all four lanes active, same k): lane j fixes and
+// stores value j, the four values are exchanged with DPP, and every lane forms one
+// product pair f * (A*B + C*D) chosen by its lane index (branch-free); lanes 0 and 2 end
+// with t0 and t2.  Returns e * t in lanes 0 (x = 0) and 2 (x = 2), zero elsewhere.
+__device__ __forceinline__ qm31 gd_term_quad(bool fix, uint32_t kind, const uint32_t* num, const uint32_t* den,
+                                             uint32_t* dnum, uint32_t* dden, qm31 r, uint64_t nt, uint64_t k,
+                                             qm31 lambda, const uint32_t* eq, uint32_t j) {
+    const uint64_t jj = j < 2 ? 2 * k + j : 2 * (nt + k) + (j - 2);
+    const uint64_t h = 4 * nt;
+    const bool has_num = gd_has_num(kind);
+    const qm31 e = gd_ld(eq, k);
+    const qm31 da = gd_ld(den, jj);
+    const qm31 db = fix ? gd_ld(den, jj + h) : qzero();
+    const qm31 na = has_num ? gd_num(kind, num, jj) : qone();
+    const qm31 nb = has_num && fix ? gd_num(kind, num, jj + h) : qone();
+    const qm31 d = fix ? qadd(qmul(r, qsub(db, da)), da) : da;
+    const qm31 n = has_num && fix ? qadd(qmul(r, qsub(nb, na)), na) : na;
+    if (fix) {
+        gd_st(dden, jj, d);
+        if (has_num) gd_st(dnum, jj, n);
+    }
+    const qm31 d0 = quad_bcast_q<0>(d), d1 = quad_bcast_q<1>(d), d2 = quad_bcast_q<2>(d), d3 = quad_bcast_q<3>(d);
+    const qm31 d20 = qsub(qadd(d2, d2), d0), d21 = qsub(qadd(d3, d3), d1);
+    const bool l0 = j == 0, l1 = j == 1, l2 = j == 2;
+    qm31 x;
+    if (kind == GKR_GRAND_PRODUCT) {
+        // lane 0: d0 d1 = t0; lane 2: d20 d21 = t2
+        const qm31 A = l0 ? d0 : (l2 ? d20 : qzero());
+        const qm31 B = l0 ? d1 : d21;
+        x = qmul(A, B);
+    } else {
+        const qm31 n0 = quad_bcast_q<0>(n), n1 = quad_bcast_q<1>(n), n2 = quad_bcast_q<2>(n), n3 = quad_bcast_q<3>(n);
+        const qm31 n20 = qsub(qadd(n2, n2), n0), n21 = qsub(qadd(n3, n3), n1);
+        // t0 = (d1 n0 + d0 n1) + lambda d0 d1,  t2 = (d21 n20 + d20 n21) + lambda d20 d21
+        const qm31 A = l0 ? d1 : (l1 ? d0 : (l2 ? d21 : d20));
+        const qm31 B = l0 ? n0 : (l1 ? d1 : (l2 ? n20 : d21));
+        const qm31 C = l0 ? d0 : (l2 ? d20 : qzero());
+        const qm31 D = l0 ? n1 : n21;
+        x = qadd(qmul(A, B), qmul(C, D));
+        x = qmul((j & 1) ? lambda : qone(), x);
+        x = qadd(x, gd_qperm<QP_ROT1>(x));
+    }
+    return (l0 || l2) ? qmul(e, x) : qzero();
+}
+
+// Where instance `in` reads round rho of layer L from: the stack layer (no fix yet this
+// layer), or the fix scratch; `fix` = the previous round's variable is fixed on the fly.
+struct GdSrc {
+    bool fix;
+    uint32_t kind;  // kind of the values this round sums over
+    uint32_t src_kind;
+    const uint32_t* num;
+    const uint32_t* den;
+};
+__device__ __forceinline__ GdSrc gd_src(const GdInst& in, uint32_t onv, uint32_t L, uint32_t rho) {
+    const uint32_t lv = onv + 1;
+    const uint32_t fixes = rho - (L - onv);  // fixes done before this round's sums
+    GdSrc s;
+    s.fix = fixes >= 1;
+    const bool from_stack = fixes <= 1;
+    s.src_kind = from_stack ? in.kind[lv] : gd_fixed_kind(in.kind[lv]);
+    s.kind = s.fix ? gd_fixed_kind(s.src_kind) : s.src_kind;
+    s.num = from_stack ? in.num[lv] : in.scr_num;
+    s.den = from_stack ? in.den[lv] : in.scr_den;
+    return s;
+}
+
+// The pending claim update of the previous round: sc_claim_i = rp_i(r) and, for the
+// instances summed in that round, corr_i *= eq(r, y_rho).  One full wave; lane t handles
+// instance t>>2, coefficient t&3.
+__device__ __forceinline__ void gd_apply_pending(GdState* S, uint32_t n_inst) {
+    if (!S->pend) return;
+    const uint32_t t = threadIdx.x & 63;
+    const uint32_t i = t >> 2, q = t & 3;
+    const qm31 r = qload(S->r);
+    const qm31 craw = i < n_inst ? qload(S->craw[i][q]) : qzero();
+    qm31 pw = qone();
+    if (q >= 1) pw = r;
+    if (q >= 2) pw = qmul(pw, r);
+    if (q >= 3) pw = qmul(pw, r);
+    qm31 v = qmul(craw, pw);
+    v = qadd(v, gd_qperm<QP_ROT2>(v));
+    v = qadd(v, gd_qperm<QP_ROT1>(v));
+    if (i < n_inst && q == 0 && (S->pend_flags[i] & 1)) {
+        gd_qstore(S->sc_claim[i], v);
+        if (S->pend_flags[i] & 2) {
+            const qm31 y = qload(S->ood[S->pend_rho]);
+            const qm31 e = qadd(qmul(r, y), qmul(qsub(qone(), r), qsub(qone(), y)));
+            gd_qstore(S->corr[i], qmul(qload(S->corr[i]), e));
+        }
+    }
+}
+
+// Turns the round sums (sums[8 i + c]: r(0) then r(2) coordinates before the eq
+// correction, 64-bit) into the batched round polynomial, mixes it and draws the round
+// challenge; the claim update is left pending.  Wave 0 only, with the previous round's
+// update already applied; lane t handles instance t>>2, coefficient t&3.
+__device__ __forceinline__ void gd_finish(GdState* S, const GdInst* I, uint32_t n_inst, uint32_t n_layers, uint32_t L, uint32_t rho,
+                          const unsigned long long* sums, uint32_t* rec) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t i = t >> 2, q = t & 3;
+    const uint32_t rem = L - rho;
+    qm31 craw = qzero();
+    bool act = false, part = false;
+    if (i < n_inst) {
+        const GdView v = gd_view(I[i], n_layers, L);
+        act = v.active;
+        part = act && v.onv >= rem;
+        if (act) {
+            const qm31 sc = qload(S->sc_claim[i]);
+            if (part) {
+                const qm31 corr = qload(S->corr[i]);
+                const unsigned long long* a = sums + 8 * i;
+                const qm31 s0 = {{mred64(a[0]), mred64(a[1])}, {mred64(a[2]), mred64(a[3])}};
+                const qm31 s2 = {{mred64(a[4]), mred64(a[5])}, {mred64(a[6]), mred64(a[7])}};
+                const uint32_t(*pr)[4] = S->pre[rho];
+                const qm31 y0 = qmul(qmul(s0, corr), qload(pr[0]));
+                const qm31 y2 = qmul(qmul(s2, corr), qload(pr[1]));
+                const qm31 y1 = qsub(sc, y0);
+                if (q == 0) {
+                    craw = y0;
+                } else {
+                    const uint32_t b = 2 + 3 * (q - 1);
+                    craw = qadd(qadd(qmul(qload(pr[b]), y0), qmul(qload(pr[b + 1]), y1)), qmul(qload(pr[b + 2]), y2));
+                }
+            } else if (q == 0) {
+                craw = qmulm(sc, (P + 1) / 2);
+            }
+        }
+        gd_qstore(S->craw[i][q], craw);
+        if (q == 0) S->pend_flags[i] = (act ? 1u : 0u) | (part ? 2u : 0u);
+    }
+    qm31 cv = i < n_inst ? qmul(craw, qload(S->alpha_pow[i])) : qzero();
+#pragma unroll
+    for (int m = 4; m < 32; m <<= 1) cv = qadd(cv, gd_shfl_xor(cv, m));
+    const uint64_t nzmask = __ballot(t < 4 && !qiszero(cv)) & 0xFull;
+    const uint32_t len = nzmask ? 64 - __clzll(nzmask) : 0;
+    if (t < 4) {
+        gd_qstore(rec + 1 + 4 * t, cv);
+        if (t == 0) rec[0] = len;
+        const QuadLane Lq(t);
+        uint32_t dg[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) dg[k] = S->dg[k];
+        uint32_t pay[16];
+        qstore(quad_bcast_q<0>(cv), pay);
+        qstore(quad_bcast_q<1>(cv), pay + 4);
+        qstore(quad_bcast_q<2>(cv), pay + 8);
+        qstore(quad_bcast_q<3>(cv), pay + 12);
+        qchan_mix(Lq, dg, pay, 4 * len);
+        uint32_t nd = 0, st = 0;
+        const qm31 r = qchan_draw(Lq, dg, nd, st);
+        if (t == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) S->dg[k] = dg[k];
+            S->n_draws = nd;
+            if (st) S->status = 1;
+            gd_qstore(S->r, r);
+            gd_qstore(S->asg[rho], r);
+            S->pend = 1;
+            S->pend_rho = rho;
+        }
+    }
+}
+
+// Layer begin (all threads of the block call it; it synchronises the block).
+__device__ __forceinline__ void gd_begin(GdState* S, const GdInst* I, uint32_t n_inst, uint32_t n_layers, uint32_t L) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t remaining = n_layers - L;
+    if (t < 4) {
+        const QuadLane Lq(t);
+        uint32_t dg[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) dg[k] = S->dg[k];
+        uint32_t nd = S->n_draws, st = 0;
+        for (uint32_t i = 0; i < n_inst; i++) {
+            if (!gd_view(I[i], n_layers, L).active) continue;
+            const uint32_t nc = I[i].kind[0] == GKR_GRAND_PRODUCT ? 1 : 2;
+            const bool fresh = I[i].n_vars == remaining;
+            uint32_t pay[16];
+#pragma unroll
+            for (int k = 0; k < 8; k++) pay[k] = fresh ? (&S->outputs[i][0][0])[k] : (&S->claims[i][0][0])[k];
+#pragma unroll
+            for (int k = 8; k < 16; k++) pay[k] = 0;
+            if (fresh && t == 0)
+                for (int k = 0; k < 8; k++) (&S->claims[i][0][0])[k] = pay[k];
+            qchan_mix(Lq, dg, pay, 4 * nc);
+            nd = 0;
+        }
+        const qm31 alpha = qchan_draw(Lq, dg, nd, st);
+        const qm31 lambda = qchan_draw(Lq, dg, nd, st);
+        if (t == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) S->dg[k] = dg[k];
+            S->n_draws = nd;
+            if (st) S->status = 1;
+            S->pend = 0;
+            gd_qstore(S->lambda, lambda);
+            // the batch combines the active instances only, in order (sumcheck.zig:215-238)
+            qm31 ap = qone();
+            for (uint32_t i = 0; i < n_inst; i++) {
+                const bool a = gd_view(I[i], n_layers, L).active;
+                gd_qstore(S->alpha_pow[i], a ? ap : qzero());
+                if (a) ap = qmul(ap, alpha);
+            }
+        }
+    } else if (t >= 64 && t < 64 + L) {
+        // round rho's constants from y = ood[0..L)
+        const uint32_t rho = t - 64;
+        qm31 pp = qone();
+        for (uint32_t k = 0; k <= rho; k++) pp = qmul(pp, qsub(qone(), qload(S->ood[k])));
+        const qm31 y = qload(S->ood[rho]);
+        const qm31 dd = qsub(qone(), qadd(y, y));
+        const qm31 pd = qmul(pp, dd);
+        const qm31 ipd = qinv(pd);
+        const qm31 ipp = qmul(ipd, dd), idd = qmul(ipd, pp);
+        const qm31 one = qone(), two = qfromm(2), three = qfromm(3);
+        const qm31 b = qmul(qsub(one, y), idd);
+        const qm31 bm1 = qsub(b, one), tmb = qsub(two, b);
+        const qm31 qq = qmul(qmul(b, bm1), tmb);
+        const qm31 iq = qinv(qq);
+        const qm31 ib = qmul(qmul(bm1, tmb), iq), ibm1 = qmul(qmul(b, tmb), iq), itmb = qmul(qmul(b, bm1), iq);
+        const uint32_t inv2 = (P + 1) / 2;
+        const qm31 s0 = qneg(qmulm(ib, inv2)), s1 = ibm1, s2 = qmulm(itmb, inv2);
+        uint32_t(*pr)[4] = S->pre[rho];
+        gd_qstore(pr[0], qmul(qsub(one, y), ipp));                          // F0 = (1-y) / prefix
+        gd_qstore(pr[1], qmul(qsub(qmul(three, y), one), ipp));            // F2 = eq(2, y) / prefix
+        gd_qstore(pr[2], qmul(s0, qadd(two, qmul(three, b))));            // x^1
+        gd_qstore(pr[3], qmul(s1, qadd(b, b)));
+        gd_qstore(pr[4], qmul(s2, b));
+        gd_qstore(pr[5], qneg(qmul(s0, qadd(three, b))));                 // x^2
+        gd_qstore(pr[6], qneg(qmul(s1, qadd(two, b))));
+        gd_qstore(pr[7], qneg(qmul(s2, qadd(one, b))));
+        gd_qstore(pr[8], s0);                                               // x^3
+        gd_qstore(pr[9], s1);
+        gd_qstore(pr[10], s2);
+        if (qiszero(pd) || qiszero(qq)) S->status = 2;
+    }
+    __syncthreads();
+    if (t < n_inst) {
+        const GdView v = gd_view(I[t], n_layers, L);
+        if (v.active) {
+            const uint32_t nc = I[t].kind[0] == GKR_GRAND_PRODUCT ? 1 : 2;
+            const qm31 lambda = qload(S->lambda);
+            qm31 cl = qzero();
+            for (uint32_t c = nc; c > 0; c--) cl = qadd(qmul(cl, lambda), qload(S->claims[t][c - 1]));
+            gd_qstore(S->sc_claim[t], qmulm(cl, 1u << (L - v.onv)));
+            gd_qstore(S->corr[t], qone());
+        }
+    }
+    __syncthreads();
+}
+
+// eq table of layer L: eq[i] = (1 - y_0) prod_{j < L-1} (bit_{L-2-j}(i) ? y_{j+1} : 1 - y_{j+1})
+// (EqEvals.generate(ood) = genEqEvals(ood[1..], 1 - ood[0]), gkr_circuit.zig:651-674).
+__device__ __forceinline__ void gd_eq_entry(const GdState* S, uint32_t L, uint64_t i, uint32_t* eq) {
+    qm31 acc = qsub(qone(), qload(S->ood[0]));
+    for (uint32_t j = 0; j + 1 < L; j++) {
+        const qm31 y = qload(S->ood[j + 1]);
+        acc = qmul(acc, ((i >> (L - 2 - j)) & 1) ? y : qsub(qone(), y));
+    }
+    gd_st(eq, i, acc);
+}
+
+// Layer end: masks of every active instance, mixed in order; challenge; next claims and
+// eq point (gkr_prover.zig:232-276, gkr_circuit.zig:341-388).
+__device__ __forceinline__ void gd_end(GdState* S, const GdInst* I, uint32_t n_inst, uint32_t n_layers, uint32_t L, uint32_t* rec_mask) {
+    const uint32_t t = threadIdx.x;
+    if (t < n_inst) {
+        const GdView v = gd_view(I[t], n_layers, L);
+        if (v.active) {
+            const GdInst& in = I[t];
+            const uint32_t lv = v.onv + 1;
+            qm31 n0, n1, d0, d1;
+            if (v.onv == 0) {
+                const uint32_t k = in.kind[lv];
+                d0 = gd_ld(in.den[lv], 0);
+                d1 = gd_ld(in.den[lv], 1);
+                n0 = k == GKR_GRAND_PRODUCT ? qzero() : gd_num(k, in.num[lv], 0);
+                n1 = k == GKR_GRAND_PRODUCT ? qzero() : gd_num(k, in.num[lv], 1);
+            } else {
+                const qm31 r = qload(S->r);
+                const bool stack = v.onv == 1;
+                const uint32_t k = stack ? in.kind[lv] : gd_fixed_kind(in.kind[lv]);
+                const uint32_t* dn = stack ? in.den[lv] : in.scr_den;
+                const uint32_t* nn = stack ? in.num[lv] : in.scr_num;
+                qm31 a = gd_ld(dn, 0), b = gd_ld(dn, 2);
+                d0 = qadd(qmul(r, qsub(b, a)), a);
+                a = gd_ld(dn, 1), b = gd_ld(dn, 3);
+                d1 = qadd(qmul(r, qsub(b, a)), a);
+                if (gd_has_num(k)) {
+                    a = gd_num(k, nn, 0), b = gd_num(k, nn, 2);
+                    n0 = qadd(qmul(r, qsub(b, a)), a);
+                    a = gd_num(k, nn, 1), b = gd_num(k, nn, 3);
+                    n1 = qadd(qmul(r, qsub(b, a)), a);
+                } else {
+                    n0 = k == GKR_GRAND_PRODUCT ? qzero() : qone();
+                    n1 = n0;
+                }
+            }
+            gd_qstore(S->mask[t][0], n0);
+            gd_qstore(S->mask[t][1], n1);
+            gd_qstore(S->mask[t][2], d0);
+            gd_qstore(S->mask[t][3], d1);
+            uint32_t* rm = rec_mask + (uint64_t)t * GD_REC_MASK;
+            gd_qstore(rm, n0);
+            gd_qstore(rm + 4, n1);
+            gd_qstore(rm + 8, d0);
+            gd_qstore(rm + 12, d1);
+        }
+    }
+    __syncthreads();
+    if (t < 4) {
+        const QuadLane Lq(t);
+        uint32_t dg[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) dg[k] = S->dg[k];
+        uint32_t nd = S->n_draws, st = 0;
+        for (uint32_t i = 0; i < n_inst; i++) {
+            if (!gd_view(I[i], n_layers, L).active) continue;
+            const bool gp = I[i].kind[0] == GKR_GRAND_PRODUCT;
+            uint32_t pay[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) pay[k] = gp ? (k < 8 ? (&S->mask[i][2][0])[k] : 0u) : (&S->mask[i][0][0])[k];
+            qchan_mix(Lq, dg, pay, gp ? 8 : 16);
+            nd = 0;
+        }
+        const qm31 ch = qchan_draw(Lq, dg, nd, st);
+        if (t == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) S->dg[k] = dg[k];
+            S->n_draws = nd;
+            if (st) S->status = 1;
+            S->pend = 0;  // the last round's claim update is not needed: the claims come from the masks
+            for (uint32_t i = 0; i < n_inst; i++) {
+                if (!gd_view(I[i], n_layers, L).active) continue;
+                const bool gp = I[i].kind[0] == GKR_GRAND_PRODUCT;
+                for (uint32_t c = 0; c < (gp ? 1u : 2u); c++) {
+                    const uint32_t col = gp ? 1 : c;  // mask column: den (GP) or num, den
+                    const qm31 m0 = qload(S->mask[i][2 * col]), m1 = qload(S->mask[i][2 * col + 1]);
+                    gd_qstore(S->claims[i][c], qadd(qmul(ch, qsub(m1, m0)), m0));
+                }
+            }
+            for (uint32_t j = 0; j < L; j++)
+                for (int k = 0; k < 4; k++) S->ood[j][k] = S->asg[j][k];
+            gd_qstore(S->ood[L], ch);
+        }
+    }
+    __syncthreads();
+}
+
+// One workgroup: whole layers [L0, L1), the first one starting at round rho0 (its begin
+// already done when rho0 > 0); every layer ends here.  The state lives in LDS meanwhile.
+__global__ __launch_bounds__(GD_FT) void gkr_fused_kernel(GdState* __restrict__ gS, const GdInst* __restrict__ I,
+                                                          uint32_t n_inst, uint32_t n_layers, uint32_t L0, uint32_t L1,
+                                                          uint32_t rho0, uint32_t* __restrict__ eq,
+                                                          uint32_t* __restrict__ rec) {
+    __shared__ GdState S;
+    // round sums, double-buffered by round parity: the pending-update wave clears the other
+    // buffer while a round accumulates into its own (LDS atomics, one add per term and word)
+    __shared__ unsigned long long red[2][GD_MAX_INST][8];
+    const uint32_t t = threadIdx.x;
+    const uint32_t wv = t >> 6, ln = t & 63;
+    {
+        const uint64_t* src = (const uint64_t*)gS;
+        uint64_t* dst = (uint64_t*)&S;
+        for (uint32_t k = t; k < sizeof(GdState) / 8; k += GD_FT) dst[k] = src[k];
+        for (uint32_t k = t; k < 2 * GD_MAX_INST * 8; k += GD_FT) (&red[0][0][0])[k] = 0;
+    }
+    __syncthreads();
+    if (L0 == 0 && rho0 == 0 && t < n_inst) {
+        // output layer (gkr_circuit.zig:162-190)
+        const GdInst& in = I[t];
+        const qm31 d = gd_ld(in.den[0], 0);
+        if (in.kind[0] == GKR_GRAND_PRODUCT) {
+            gd_qstore(S.outputs[t][0], d);
+        } else {
+            gd_qstore(S.outputs[t][0], gd_num(in.kind[0], in.num[0], 0));
+            gd_qstore(S.outputs[t][1], d);
+        }
+    }
+    __syncthreads();
+    const uint32_t total_rounds = n_layers * (n_layers - 1) / 2;
+    uint32_t rc = 0;  // rounds run by this launch (sum buffer parity)
+    for (uint32_t L = L0; L < L1; L++) {
+        const uint32_t rb = L == L0 ? rho0 : 0;
+        if (rb == 0) {
+            gd_begin(&S, I, n_inst, n_layers, L);
+            if (L >= 1)
+                for (uint32_t k = t; k < (1u << (L - 1)); k += GD_FT) gd_eq_entry(&S, L, k, eq);
+            __syncthreads();
+        }
+        for (uint32_t rho = rb; rho < L; rho++, rc++) {
+            const uint32_t rem = L - rho;
+            const uint64_t nt = 1ull << (rem - 1);
+            const uint32_t b = rc & 1;
+            if (wv == 7) {
+                gd_apply_pending(&S, n_inst);
+                (&red[b ^ 1][0][0])[ln] = 0;
+            } else {
+                // (instance, term) pairs over the 112 quads of waves 0..6; pair x = p * nt + k for
+                // the p-th instance summed in this round
+                const qm31 lambda = qload(S.lambda), r = qload(S.r);
+                uint32_t n_part = 0;
+                for (uint32_t i = 0; i < n_inst; i++) {
+                    const GdView v = gd_view(I[i], n_layers, L);
+                    n_part += v.active && v.onv >= rem;
+                }
+                const uint32_t j = t & 3;
+                for (uint64_t x = t >> 2; x < (uint64_t)n_part * nt; x += 7 * 16) {
+                    const uint32_t p = (uint32_t)(x >> (rem - 1));
+                    const uint64_t k = x & (nt - 1);
+                    uint32_t i = 0;
+                    for (uint32_t ii = 0, cnt = 0; ii < n_inst; ii++) {
+                        const GdView v = gd_view(I[ii], n_layers, L);
+                        if (!(v.active && v.onv >= rem)) continue;
+                        if (cnt++ == p) {
+                            i = ii;
+                            break;
+                        }
+                    }
+                    const GdView v = gd_view(I[i], n_layers, L);
+                    const GdSrc s = gd_src(I[i], v.onv, L, rho);
+                    const qm31 y = gd_term_quad(s.fix, s.src_kind, s.num, s.den, I[i].scr_num, I[i].scr_den, r, nt, k,
+                                                lambda, eq, j);
+                    if (j == 0 || j == 2) {
+                        unsigned long long* dst = &red[b][i][j == 0 ? 0 : 4];
+                        atomicAdd(dst + 0, (unsigned long long)y.c0.a);
+                        atomicAdd(dst + 1, (unsigned long long)y.c0.b);
+                        atomicAdd(dst + 2, (unsigned long long)y.c1.a);
+                        atomicAdd(dst + 3, (unsigned long long)y.c1.b);
+                    }
+                }
+            }
+            __syncthreads();
+            if (t < 64)
+                gd_finish(&S, I, n_inst, n_layers, L, rho, &red[b][0][0],
+                          rec + (uint64_t)(L * (L - 1) / 2 + rho) * GD_REC_ROUND);
+            __syncthreads();
+        }
+        gd_end(&S, I, n_inst, n_layers, L,
+               rec + (uint64_t)total_rounds * GD_REC_ROUND + (uint64_t)L * n_inst * GD_REC_MASK);
+    }
+    {
+        const uint64_t* src = (const uint64_t*)&S;
+        uint64_t* dst = (uint64_t*)gS;
+        for (uint32_t k = t; k < sizeof(GdState) / 8; k += GD_FT) dst[k] = src[k];
+    }
+}
+
+// Layer begin of a large layer plus its eq table over many workgroups (block 0 also runs the begin).
+__global__ __launch_bounds__(GD_RT) void gkr_begin_eq_kernel(GdState* __restrict__ S, const GdInst* __restrict__ I,
+                                                             uint32_t n_inst, uint32_t n_layers, uint32_t L,
+                                                             uint32_t* __restrict__ eq) {
+    const uint64_t k = (uint64_t)blockIdx.x * GD_RT + threadIdx.x;
+    if (L >= 1 && k < (1ull << (L - 1))) gd_eq_entry(S, L, k, eq);
+    if (blockIdx.x == 0) gd_begin(S, I, n_inst, n_layers, L);
+}
+
+// One large sum-check round: grid (x: terms, y: instance).  Every workgroup adds its sums
+// atomically and counts itself done; the last one finishes the round.
+__global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ S, const GdInst* __restrict__ I,
+                                                          uint32_t n_inst, uint32_t n_layers, uint32_t L, uint32_t rho,
+                                                          const uint32_t* __restrict__ eq, uint32_t* __restrict__ rec) {
+    __shared__ unsigned long long red[GD_RT / 64][8];
+    __shared__ unsigned long long sums[GD_MAX_INST][8];
+    __shared__ uint32_t last;
+    const uint32_t t = threadIdx.x;
+    const uint32_t i = blockIdx.y;
+    const uint32_t rem = L - rho;
+    const uint64_t nt = 1ull << (rem - 1);
+    const GdView v = gd_view(I[i], n_layers, L);
+    const bool part = v.active && v.onv >= rem;
+    if (part) {
+        const GdSrc s = gd_src(I[i], v.onv, L, rho);
+        const qm31 lambda = qload(S->lambda), r = qload(S->r);
+        uint64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const uint32_t j = t & 3;
+        for (uint64_t k = (uint64_t)blockIdx.x * (GD_RT / 4) + (t >> 2); k < nt; k += (uint64_t)gridDim.x * (GD_RT / 4)) {
+            const qm31 y = gd_term_quad(s.fix, s.src_kind, s.num, s.den, I[i].scr_num, I[i].scr_den, r, nt, k, lambda,
+                                        eq, j);
+            if (j == 0) {
+                a[0] += y.c0.a, a[1] += y.c0.b, a[2] += y.c1.a, a[3] += y.c1.b;
+            } else if (j == 2) {
+                a[4] += y.c0.a, a[5] += y.c0.b, a[6] += y.c1.a, a[7] += y.c1.b;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const uint64_t w = gd_wave_sum(a[c]);
+            if ((t & 63) == 0) red[t >> 6][c] = w;
+        }
+        __syncthreads();
+        if (t < 8) {
+            uint64_t w = 0;
+            for (uint32_t k = 0; k < GD_RT / 64; k++) w += red[k][t];
+            atomicAdd(&S->acc[i][t], (unsigned long long)w);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    if (t == 0) last = atomicAdd(&S->counter, 1u) == gridDim.x * gridDim.y - 1 ? 1u : 0u;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (t < 8 * n_inst) sums[t >> 3][t & 7] = atomicExch(&S->acc[t >> 3][t & 7], 0ull);
+    if (t == 0) atomicExch(&S->counter, 0u);
+    if (t < 64) gd_apply_pending(S, n_inst);
+    __syncthreads();
+    if (t < 64) gd_finish(S, I, n_inst, n_layers, L, rho, &sums[0][0], rec + (uint64_t)(L * (L - 1) / 2 + rho) * GD_REC_ROUND);
+}
+
+}  // namespace
+
+std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<GkrInstance>& inst) {
+    SR_CHECK(!inst.empty(), "gkr: empty batch");
+    SR_CHECK(inst.size() <= GD_MAX_INST, "gkr: at most 8 instances per batch");
+    const uint32_t n_inst = (uint32_t)inst.size();
+    uint32_t n_layers = 0;
+    for (auto& in : inst) {
+        SR_CHECK(in.kind >= 0 && in.kind <= 3, "gkr: layer kind");
+        SR_CHECK(in.n_vars >= 1 && in.n_vars <= GD_MAX_VARS, "gkr: an instance needs 1..30 variables");
+        SR_CHECK(in.d_den && (in.kind == GKR_GRAND_PRODUCT || in.kind == GKR_LOGUP_SINGLES || in.d_num),
+                 "gkr: missing layer columns");
+        n_layers = std::max(n_layers, in.n_vars);
+    }
+    // Layer stacks (buildLayers, gkr_circuit.zig:431-447) and fix scratch.
+    std::vector<GdInst> h_inst(n_inst);
+    std::vector<DBuf> bufs;
+    for (uint32_t k = 0; k < n_inst; k++) {
+        GdInst& g = h_inst[k];
+        memset(&g, 0, sizeof(g));
+        const uint32_t nv = inst[k].n_vars;
+        const bool gp = inst[k].kind == GKR_GRAND_PRODUCT;
+        g.n_vars = nv;
+        for (uint32_t l = 0; l < nv; l++) g.kind[l] = gp ? GKR_GRAND_PRODUCT : GKR_LOGUP_GENERIC;
+        g.kind[nv] = (uint32_t)inst[k].kind;
+        g.num[nv] = inst[k].d_num;
+        g.den[nv] = inst[k].d_den;
+        for (uint32_t l = 0; l < nv; l++) {
+            if (!gp) {
+                bufs.emplace_back(&c, 16ull << l);
+                g.num[l] = bufs.back().as<uint32_t>();
+            }
+            bufs.emplace_back(&c, 16ull << l);
+            g.den[l] = bufs.back().as<uint32_t>();
+        }
+        for (int32_t l = (int32_t)nv - 1; l >= 0; l--)
+            launch_gkr_next_layer(c, (int)g.kind[l + 1], g.num[l + 1], g.den[l + 1], (uint32_t)l + 1,
+                                  (uint32_t*)g.num[l], (uint32_t*)g.den[l]);
+        bufs.emplace_back(&c, 16ull << (nv - 1));
+        g.scr_den = bufs.back().as<uint32_t>();
+        // every LogUp stack layer below the input is generic (QM31 numerators), singles included
+        if (!gp) {
+            bufs.emplace_back(&c, 16ull << (nv - 1));
+            g.scr_num = bufs.back().as<uint32_t>();
+        }
+    }
+    DBuf d_inst = upload(c, h_inst.data(), h_inst.size());
+    GdState* hs = (GdState*)c.pinned(sizeof(GdState));
+    memset(hs, 0, sizeof(GdState));
+    memcpy(hs->dg, ch.digest, 32);
+    hs->n_draws = ch.n_draws;
+    DBuf d_state = upload(c, hs, 1);
+    const uint32_t total_rounds = n_layers * (n_layers - 1) / 2;
+    const uint64_t rec_words = (uint64_t)total_rounds * GD_REC_ROUND + (uint64_t)n_layers * n_inst * GD_REC_MASK;
+    DBuf d_rec(&c, 4 * rec_words);
+    DBuf d_eq(&c, n_layers >= 2 ? (16ull << (n_layers - 2)) : 16);
+    GdState* S = d_state.as<GdState>();
+    const GdInst* I = d_inst.as<GdInst>();
+    uint32_t* eq = d_eq.as<uint32_t>();
+    uint32_t* rec = d_rec.as<uint32_t>();
+    {
+        KTimer kt(c, "gkr_small_layers", 0.0);
+        hipLaunchKernelGGL(gkr_fused_kernel, dim3(1), dim3(GD_FT), 0, c.stream, S, I, n_inst, n_layers, 0u,
+                           std::min(n_layers, GD_SMALL_LOG), 0u, eq, rec);
+        SR_HIP(hipGetLastError());
+    }
+    for (uint32_t L = GD_SMALL_LOG; L < n_layers; L++) {
+        {
+            KTimer kt(c, "gkr_begin_eq", 16.0 * (double)(1ull << (L - 1)));
+            const uint64_t blocks = ((1ull << (L - 1)) + GD_RT - 1) / GD_RT;
+            hipLaunchKernelGGL(gkr_begin_eq_kernel, dim3((uint32_t)blocks), dim3(GD_RT), 0, c.stream, S, I, n_inst,
+                               n_layers, L, eq);
+            SR_HIP(hipGetLastError());
+        }
+        uint32_t rho = 0;
+        for (; L - rho >= GD_SMALL_LOG; rho++) {
+            const uint64_t nt = 1ull << (L - rho - 1);
+            const uint64_t bx = std::min<uint64_t>((nt + GD_RT / 4 - 1) / (GD_RT / 4), 1024);
+            KTimer kt(c, "gkr_round", 96.0 * (double)nt * n_inst);
+            hipLaunchKernelGGL(gkr_round_kernel, dim3((uint32_t)bx, n_inst), dim3(GD_RT), 0, c.stream, S, I, n_inst,
+                               n_layers, L, rho, (const uint32_t*)eq, rec);
+            SR_HIP(hipGetLastError());
+        }
+        KTimer kt(c, "gkr_small_rounds", 0.0);
+        hipLaunchKernelGGL(gkr_fused_kernel, dim3(1), dim3(GD_FT), 0, c.stream, S, I, n_inst, n_layers, L, L + 1, rho,
+                           eq, rec);
+        SR_HIP(hipGetLastError());
+    }
+    std::vector<uint32_t> hrec(rec_words);
+    SR_HIP(hipMemcpyAsync(hs, S, sizeof(GdState), hipMemcpyDeviceToHost, c.stream));
+    SR_HIP(hipMemcpyAsync(hrec.data(), rec, 4 * rec_words, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    SR_CHECK(hs->status != 1, "gkr: channel draw rejected 64 times");
+    SR_CHECK(hs->status == 0, "gkr: division by zero in a round polynomial");
+    // Assemble the proof word stream.
+    std::vector<uint32_t> w;
+    auto ncols = [&](uint32_t k) { return inst[k].kind == GKR_GRAND_PRODUCT ? 1u : 2u; };
+    for (uint32_t L = 0; L < n_layers; L++) {
+        w.push_back(L);
+        for (uint32_t rho = 0; rho < L; rho++) {
+            const uint32_t* rr = &hrec[(uint64_t)(L * (L - 1) / 2 + rho) * GD_REC_ROUND];
+            SR_CHECK(rr[0] <= 4, "gkr: round record");
+            w.push_back(rr[0]);
+            w.insert(w.end(), rr + 1, rr + 1 + 4 * rr[0]);
+        }
+        uint32_t n_act = 0;
+        for (uint32_t k = 0; k < n_inst; k++) n_act += inst[k].n_vars + L >= n_layers;
+        w.push_back(n_act);
+        for (uint32_t k = 0; k < n_inst; k++) {
+            if (inst[k].n_vars + L < n_layers) continue;
+            const uint32_t* rm = &hrec[(uint64_t)total_rounds * GD_REC_ROUND + ((uint64_t)L * n_inst + k) * GD_REC_MASK];
+            w.push_back(k);
+            w.push_back(ncols(k));
+            if (ncols(k) == 1)
+                w.insert(w.end(), rm + 8, rm + 16);
+            else
+                w.insert(w.end(), rm, rm + 16);
+        }
+    }
+    for (uint32_t k = 0; k < n_inst; k++) {
+        w.push_back(ncols(k));
+        w.insert(w.end(), &hs->outputs[k][0][0], &hs->outputs[k][0][0] + 4 * ncols(k));
+    }
+    w.push_back(n_layers);
+    w.insert(w.end(), &hs->ood[0][0], &hs->ood[0][0] + 4 * n_layers);
+    for (uint32_t k = 0; k < n_inst; k++) {
+        w.push_back(ncols(k));
+        w.insert(w.end(), &hs->claims[k][0][0], &hs->claims[k][0][0] + 4 * ncols(k));
+        w.push_back(inst[k].n_vars);
+    }
+    w.insert(w.end(), hs->dg, hs->dg + 8);
+    memcpy(ch.digest, hs->dg, 32);
+    ch.n_draws = hs->n_draws;
+    return w;
 }
 
 }  // namespace sr

@@ -74,7 +74,6 @@ void serve(int device, const std::vector<ServeJob>& jobs, uint32_t n_slots, uint
     std::mutex mu;
     std::condition_variable cv;
     std::deque<size_t> ready;  // arrived, not yet admitted (FIFO)
-    size_t arrived = 0;
     uint64_t used = 0, peak = 0;
     uint32_t in_flight = 0, max_in_flight = 0;
     bool done_arriving = false;
@@ -156,7 +155,6 @@ void serve(int device, const std::vector<ServeJob>& jobs, uint32_t n_slots, uint
             std::lock_guard<std::mutex> lk(mu);
             out[i].arrival_ms = jobs[i].arrival_ms;
             ready.push_back(i);
-            arrived++;
             if (i + 1 == jobs.size()) done_arriving = true;
         }
         cv.notify_all();

@@ -181,9 +181,27 @@ def test_sum_as_poly_matches_definition(ctx):
 
 
 def _proof_case(ctx, shape):
-    rng = np.random.default_rng({"gp": 1, "generic": 2, "mixed": 3, "xor12": 4, "xor18": 5}[shape])
+    rng = np.random.default_rng({"gp": 1, "generic": 2, "mixed": 3, "xor12": 4, "xor18": 5, "gp_big": 6,
+                                 "mixed_big": 7, "eight": 8, "one_var": 9, "edge8": 10, "edge9": 11}[shape])
     if shape == "gp":
         return [(0, None, rand_q(rng, 32), 5)]
+    if shape == "gp_big":  # rounds above 2^8 values run over many workgroups
+        return [(0, None, rand_q(rng, 1 << 11), 11)]
+    if shape == "mixed_big":
+        return [(1, rand_q(rng, 1 << 10), rand_q(rng, 1 << 10), 10), (0, None, rand_q(rng, 1 << 9), 9),
+                (3, None, rand_q(rng, 1 << 8), 8), (2, rand_m(rng, 1 << 7, 1000), rand_q(rng, 1 << 7), 7)]
+    if shape == "eight":  # the largest batch
+        kinds = [0, 1, 2, 3, 1, 0, 3, 2]
+        out = []
+        for k, nv in zip(kinds, [9, 1, 4, 9, 2, 6, 3, 8]):
+            num = rand_q(rng, 1 << nv) if k == 1 else (rand_m(rng, 1 << nv, 7) if k == 2 else None)
+            out.append((k, num, rand_q(rng, 1 << nv), nv))
+        return out
+    if shape == "one_var":
+        return [(1, rand_q(rng, 2), rand_q(rng, 2), 1)]
+    if shape in ("edge8", "edge9"):  # around the one-workgroup threshold
+        nv = 8 if shape == "edge8" else 9
+        return [(1, rand_q(rng, 1 << nv), rand_q(rng, 1 << nv), nv), (3, None, rand_q(rng, 1 << nv), nv)]
     if shape == "generic":
         return [(1, rand_q(rng, 256), rand_q(rng, 256), 8)]
     if shape == "mixed":
@@ -198,7 +216,33 @@ def _proof_case(ctx, shape):
     return [(3, None, dl, log), (2, negm, dt, 16)]
 
 
-@pytest.mark.parametrize("shape", ["gp", "generic", "mixed", "xor12", "xor18"])
+def _locate(words, n_layers, idx):
+    """Names the part of a GKR proof word stream that holds word idx."""
+    p = 0
+    for layer in range(n_layers):
+        nr = int(words[p])
+        if idx == p:
+            return "layer %d round count" % layer
+        p += 1
+        for rnd in range(nr):
+            ln = int(words[p])
+            if p <= idx < p + 1 + 4 * ln:
+                return "layer %d round %d word %d" % (layer, rnd, idx - p)
+            p += 1 + 4 * ln
+        nm = int(words[p])
+        if idx == p:
+            return "layer %d mask count" % layer
+        p += 1
+        for _ in range(nm):
+            nc = int(words[p + 1])
+            if p <= idx < p + 2 + 8 * nc:
+                return "layer %d mask of instance %d word %d" % (layer, int(words[p]), idx - p)
+            p += 2 + 8 * nc
+    return "tail (outputs, ood, claims, digest) word %d" % (idx - p)
+
+
+@pytest.mark.parametrize("shape", ["gp", "generic", "mixed", "one_var", "edge8", "edge9", "gp_big", "mixed_big",
+                                   "eight", "xor12", "xor18"])
 def test_gkr_prove_batch_matches_oracle(ctx, shape):
     _say("gkr prove", shape)
     insts = _proof_case(ctx, shape)
@@ -213,7 +257,9 @@ def test_gkr_prove_batch_matches_oracle(ctx, shape):
     words = ctx.gkr_prove_batch(dinst, digest)
     ref = oracle.gkr_prove(insts, digest)
     assert words.size == ref.size
-    assert np.array_equal(words, ref)
+    diff = np.nonzero(words != ref)[0]
+    where = _locate(ref, max(nv for _, _, _, nv in insts), int(diff[0])) if diff.size else ""
+    assert diff.size == 0, "first difference at word %d: %s" % (int(diff[0]), where)
 
 
 def test_xor_logup_balances_in_cumulative_columns_and_gkr(ctx):

@@ -131,11 +131,19 @@ def _layer_values(kind, num, den, n):
     return [nums, [q_of(den, i) for i in range(n)]]
 
 
-@pytest.mark.parametrize("shape", ["gp", "generic", "xor_small"])
+@pytest.mark.parametrize("shape", ["gp", "generic", "xor_small", "eight", "one_var"])
 def test_gkr_proof_verifies_and_final_claims_are_input_mle_evals(shape):
-    rng = np.random.default_rng({"gp": 5, "generic": 6, "xor_small": 7}[shape])
+    rng = np.random.default_rng({"gp": 5, "generic": 6, "xor_small": 7, "eight": 12, "one_var": 13}[shape])
     if shape == "gp":
         insts = [(oracle.GKR_GRAND_PRODUCT, None, rand_q(rng, 32), 5)]
+    elif shape == "eight":  # every kind, eight instances of different sizes
+        insts = []
+        for k, nv in zip([0, 1, 2, 3, 1, 0, 3, 2], [5, 1, 4, 5, 2, 3, 3, 4]):
+            num = rand_q(rng, 1 << nv) if k == 1 else (
+                rng.integers(0, 7, size=1 << nv, dtype=np.uint64).astype(np.uint32) if k == 2 else None)
+            insts.append((k, num, rand_q(rng, 1 << nv), nv))
+    elif shape == "one_var":
+        insts = [(oracle.GKR_LOGUP_GENERIC, rand_q(rng, 2), rand_q(rng, 2), 1)]
     elif shape == "generic":
         insts = [(oracle.GKR_LOGUP_GENERIC, rand_q(rng, 16), rand_q(rng, 16), 4)]
     else:

@@ -133,7 +133,7 @@ namespace {
 
 constexpr uint32_t GD_MAX_INST = 8;
 constexpr uint32_t GD_MAX_VARS = 30;
-constexpr uint32_t GD_SMALL_LOG = 8;  // rounds on layers of <= 2^8 values run in one workgroup
+constexpr uint32_t GD_SMALL_LOG = 9;  // rounds on layers of <= 2^9 values run in one workgroup (measured best 8..10)
 constexpr uint32_t GD_FT = 512;       // one-workgroup kernel: 7 summing waves + 1 for the pending update
 constexpr uint32_t GD_RT = 256;       // threads per workgroup of the round and begin kernels
 constexpr uint32_t GD_REC_ROUND = 17;  // [len, 4 QM31] per sum-check round
@@ -152,7 +152,7 @@ struct GdInst {
 
 struct GdState {
     uint32_t dg[8];
-    uint32_t n_draws, status, counter, pad0;
+    uint32_t n_draws, status, pad0, pad00;
     uint32_t outputs[GD_MAX_INST][2][4];  // transcript order: [den] or [num, den]
     uint32_t claims[GD_MAX_INST][2][4];
     uint32_t sc_claim[GD_MAX_INST][4];
@@ -169,9 +169,11 @@ struct GdState {
     uint32_t craw[GD_MAX_INST][4][4];
     uint32_t pend, pend_rho, pad1, pad2;
     uint32_t pend_flags[GD_MAX_INST];  // bit 0: active, bit 1: summed in that round
-    unsigned long long acc[GD_MAX_INST][8];
 };
 static_assert(sizeof(GdState) % 8 == 0, "state copy in 8-byte words");
+constexpr uint32_t GD_MAX_BX = 1024;   // workgroups per instance in a round launch
+constexpr uint32_t GD_CNT_GROUPS = 16;  // completion sub-counters, one 128-byte line each
+constexpr uint32_t GD_CNT_STRIDE = 32;  // words between counters
 
 __device__ __forceinline__ qm31 gd_ld(const uint32_t* p, uint64_t i) {
     const uint4 v = ((const uint4*)p)[i];
@@ -689,24 +691,38 @@ __global__ __launch_bounds__(GD_RT) void gkr_begin_eq_kernel(GdState* __restrict
     if (blockIdx.x == 0) gd_begin(S, I, n_inst, n_layers, L);
 }
 
-// One large sum-check round: grid (x: terms, y: instance).  Every workgroup adds its sums
-// atomically and counts itself done; the last one finishes the round.
+// One large sum-check round: grid (x: terms, y: instance).  Every workgroup stores its
+// sums in its own slot of `part` and counts itself done on one of 16 sub-counters (separate
+// cache lines; the workgroup completing a sub-counter counts on the top counter), so no
+// address takes more than ~nb/16 atomics; the last workgroup adds the slots up and finishes
+// the round.  Workgroup (0, 0) applies the previous round's pending claim update first; the
+// fence before its count publishes it to the last workgroup.
 __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ S, const GdInst* __restrict__ I,
                                                           uint32_t n_inst, uint32_t n_layers, uint32_t L, uint32_t rho,
-                                                          const uint32_t* __restrict__ eq, uint32_t* __restrict__ rec) {
+                                                          const uint32_t* __restrict__ eq, uint32_t* __restrict__ rec,
+                                                          unsigned long long* __restrict__ part, uint32_t* __restrict__ cnt) {
     __shared__ unsigned long long red[GD_RT / 64][8];
     __shared__ unsigned long long sums[GD_MAX_INST][8];
     __shared__ uint32_t last;
     const uint32_t t = threadIdx.x;
-    const uint32_t i = blockIdx.y;
     const uint32_t rem = L - rho;
     const uint64_t nt = 1ull << (rem - 1);
+    // blockIdx.y = rank among the instances summed in this round
+    uint32_t i = 0;
+    for (uint32_t ii = 0, cnt_p = 0; ii < n_inst; ii++) {
+        const GdView vi = gd_view(I[ii], n_layers, L);
+        if (!(vi.active && vi.onv >= rem)) continue;
+        if (cnt_p++ == blockIdx.y) {
+            i = ii;
+            break;
+        }
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && t < 64) gd_apply_pending(S, n_inst);
     const GdView v = gd_view(I[i], n_layers, L);
-    const bool part = v.active && v.onv >= rem;
-    if (part) {
+    uint64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    {
         const GdSrc s = gd_src(I[i], v.onv, L, rho);
         const qm31 lambda = qload(S->lambda), r = qload(S->r);
-        uint64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const uint32_t j = t & 3;
         for (uint64_t k = (uint64_t)blockIdx.x * (GD_RT / 4) + (t >> 2); k < nt; k += (uint64_t)gridDim.x * (GD_RT / 4)) {
             const qm31 y = gd_term_quad(s.fix, s.src_kind, s.num, s.den, I[i].scr_num, I[i].scr_den, r, nt, k, lambda,
@@ -717,28 +733,57 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
                 a[4] += y.c0.a, a[5] += y.c0.b, a[6] += y.c1.a, a[7] += y.c1.b;
             }
         }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const uint64_t w = gd_wave_sum(a[c]);
+        if ((t & 63) == 0) red[t >> 6][c] = w;
+    }
+    __syncthreads();
+    if (t < 8) {
+        uint64_t w = 0;
+        for (uint32_t k = 0; k < GD_RT / 64; k++) w += red[k][t];
+        part[((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * 8 + t] = w;
+    }
+    // the barrier completes the slot stores; one agent-scope release (thread 0) publishes them
+    __syncthreads();
+    if (t == 0) {
+        __threadfence();
+        const uint32_t nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+        const uint32_t g = b % GD_CNT_GROUPS, ng = nb < GD_CNT_GROUPS ? nb : GD_CNT_GROUPS;
+        const uint32_t gsize = (nb - g + GD_CNT_GROUPS - 1) / GD_CNT_GROUPS;
+        uint32_t lst = 0;
+        if (atomicAdd(&cnt[GD_CNT_STRIDE * g], 1u) == gsize - 1)
+            lst = atomicAdd(&cnt[GD_CNT_STRIDE * GD_CNT_GROUPS], 1u) == ng - 1 ? 1u : 0u;
+        last = lst;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (t <= GD_CNT_GROUPS) atomicExch(&cnt[GD_CNT_STRIDE * t], 0u);
+    for (uint32_t ii = 0, py = 0; ii < n_inst; ii++) {
+        const GdView vi = gd_view(I[ii], n_layers, L);
+        if (!(vi.active && vi.onv >= rem)) continue;
+        uint64_t b8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const uint32_t row = py++;
+        for (uint32_t x = t; x < gridDim.x; x += GD_RT) {
+            const unsigned long long* pp = part + ((uint64_t)row * gridDim.x + x) * 8;
+#pragma unroll
+            for (int c = 0; c < 8; c++) b8[c] += pp[c];
+        }
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-            const uint64_t w = gd_wave_sum(a[c]);
+            const uint64_t w = gd_wave_sum(b8[c]);
             if ((t & 63) == 0) red[t >> 6][c] = w;
         }
         __syncthreads();
         if (t < 8) {
             uint64_t w = 0;
             for (uint32_t k = 0; k < GD_RT / 64; k++) w += red[k][t];
-            atomicAdd(&S->acc[i][t], (unsigned long long)w);
+            sums[ii][t] = w;
         }
+        __syncthreads();
     }
-    __threadfence();
-    __syncthreads();
-    if (t == 0) last = atomicAdd(&S->counter, 1u) == gridDim.x * gridDim.y - 1 ? 1u : 0u;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (t < 8 * n_inst) sums[t >> 3][t & 7] = atomicExch(&S->acc[t >> 3][t & 7], 0ull);
-    if (t == 0) atomicExch(&S->counter, 0u);
-    if (t < 64) gd_apply_pending(S, n_inst);
-    __syncthreads();
     if (t < 64) gd_finish(S, I, n_inst, n_layers, L, rho, &sums[0][0], rec + (uint64_t)(L * (L - 1) / 2 + rho) * GD_REC_ROUND);
 }
 
@@ -798,6 +843,9 @@ std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<Gkr
     const uint64_t rec_words = (uint64_t)total_rounds * GD_REC_ROUND + (uint64_t)n_layers * n_inst * GD_REC_MASK;
     DBuf d_rec(&c, 4 * rec_words);
     DBuf d_eq(&c, n_layers >= 2 ? (16ull << (n_layers - 2)) : 16);
+    DBuf d_part(&c, 64ull * n_inst * GD_MAX_BX);
+    DBuf d_cnt(&c, 4ull * GD_CNT_STRIDE * (GD_CNT_GROUPS + 1));
+    launch_fill_u32(c, d_cnt.as<uint32_t>(), GD_CNT_STRIDE * (GD_CNT_GROUPS + 1), 0);
     GdState* S = d_state.as<GdState>();
     const GdInst* I = d_inst.as<GdInst>();
     uint32_t* eq = d_eq.as<uint32_t>();
@@ -819,10 +867,14 @@ std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<Gkr
         uint32_t rho = 0;
         for (; L - rho >= GD_SMALL_LOG; rho++) {
             const uint64_t nt = 1ull << (L - rho - 1);
-            const uint64_t bx = std::min<uint64_t>((nt + GD_RT / 4 - 1) / (GD_RT / 4), 1024);
-            KTimer kt(c, "gkr_round", 96.0 * (double)nt * n_inst);
-            hipLaunchKernelGGL(gkr_round_kernel, dim3((uint32_t)bx, n_inst), dim3(GD_RT), 0, c.stream, S, I, n_inst,
-                               n_layers, L, rho, (const uint32_t*)eq, rec);
+            const uint64_t bx = std::min<uint64_t>((nt + GD_RT / 4 - 1) / (GD_RT / 4), GD_MAX_BX);
+            uint32_t n_part = 0;  // instances summed in this round: active with onv >= L - rho
+            for (uint32_t k = 0; k < n_inst; k++)
+                n_part += inst[k].n_vars + L >= n_layers && inst[k].n_vars + L - n_layers >= L - rho;
+            KTimer kt(c, "gkr_round", 96.0 * (double)nt * n_part);
+            hipLaunchKernelGGL(gkr_round_kernel, dim3((uint32_t)bx, n_part), dim3(GD_RT), 0, c.stream, S, I, n_inst,
+                               n_layers, L, rho, (const uint32_t*)eq, rec, d_part.as<unsigned long long>(),
+                               d_cnt.as<uint32_t>());
             SR_HIP(hipGetLastError());
         }
         KTimer kt(c, "gkr_small_rounds", 0.0);

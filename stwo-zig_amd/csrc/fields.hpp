@@ -42,6 +42,16 @@ SR_HD uint32_t mmul(uint32_t a, uint32_t b) {
     uint32_t r = (uint32_t)(x & P) + (uint32_t)(x >> 31);
     return umin32(r, r - P);
 }
+// b * t mod p for canonical b, t given t2 = 2t (a plain u32): b*t2 = 2bt splits as
+// hi = floor(bt / 2^31) and lo / 2 = bt mod 2^31, so bt = hi + (lo >> 1) (mod p) with
+// hi + (lo >> 1) < 2p; one conditional subtraction canonicalises.
+SR_HD uint32_t mmul_t2(uint32_t b, uint32_t t2) {
+    uint64_t x = (uint64_t)b * t2;
+    uint32_t r = (uint32_t)(x >> 32) + ((uint32_t)x >> 1);
+    return umin32(r, r - P);
+}
+// 2 * (p - w) for a doubled word w2 = 2w: the doubled negation (0 stays 0).
+SR_HD uint32_t mneg_t2(uint32_t w2) { return w2 == 0 ? 0u : 0xFFFFFFFEu - w2; }
 // Fold a 64-bit accumulator of folded products to a canonical word (m31.zig:150-158).
 SR_HD uint32_t mred64(uint64_t x) {
     uint64_t t = (x & P) + (x >> 31);

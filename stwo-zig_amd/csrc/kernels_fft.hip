@@ -141,9 +141,11 @@ __device__ __forceinline__ uint32_t rx_lds(uint32_t j, uint32_t lane) {
     return LOGC == 0 ? (j + (j >> 4)) : ((j << LOGC) | lane);
 }
 
+// Twiddles of one stage (doubled tower words): RX >> (Q+1) consecutive words of T_{s-1}
+// for a line stage; for the circle stage the two (x, y) pairs of T_0 the thread needs.
 template <bool INV, int R, int LOGC, int B, int Q>
-__device__ __forceinline__ void rx_stage(uint32_t (&v)[RX], const uint32_t* __restrict__ tower, const RxPass& ps,
-                                         uint32_t hi, uint32_t r, uint32_t base, bool dup) {
+__device__ __forceinline__ void rx_tw_load(uint32_t (&w)[8], const uint32_t* __restrict__ tower, const RxPass& ps,
+                                           uint32_t hi, uint32_t r, uint32_t base) {
     constexpr int LS = B + Q;
     constexpr bool MAY_BE_CIRCLE = (LS == 0 && LOGC == 0);
     const uint32_t s = ps.s_lo + LS;
@@ -153,8 +155,24 @@ __device__ __forceinline__ void rx_stage(uint32_t (&v)[RX], const uint32_t* __re
                                                                                  ((r >> B) << (3 - Q)));
         const uint32_t* __restrict__ twp = tower + tb;  // constant offsets below
 #pragma unroll
+        for (int hk = 0; hk < (RX >> (Q + 1)); hk++) w[hk] = twp[hk];
+    } else if constexpr (MAY_BE_CIRCLE) {
+        const uint32_t* T0 = tower + ((1u << (ps.tower_log - 1)) - (1u << (ps.m - 1)));
+        const uint32_t kk0 = ((base >> 1) | (r << 3)) >> 2;
+#pragma unroll
+        for (int k = 0; k < 4; k++) w[k] = T0[2 * kk0 + k];
+    }
+}
+
+template <bool INV, int R, int LOGC, int B, int Q>
+__device__ __forceinline__ void rx_stage(uint32_t (&v)[RX], const uint32_t (&w)[8], const RxPass& ps, bool dup) {
+    constexpr int LS = B + Q;
+    constexpr bool MAY_BE_CIRCLE = (LS == 0 && LOGC == 0);
+    const uint32_t s = ps.s_lo + LS;
+    if (!MAY_BE_CIRCLE || s > 0) {
+#pragma unroll
         for (int hk = 0; hk < (RX >> (Q + 1)); hk++) {
-            const uint32_t tw = twp[hk];
+            const uint32_t tw = w[hk];
 #pragma unroll
             for (int lk = 0; lk < (1 << Q); lk++) {
                 const int k0 = (hk << (Q + 1)) | lk, k1 = k0 | (1 << Q);
@@ -163,56 +181,63 @@ __device__ __forceinline__ void rx_stage(uint32_t (&v)[RX], const uint32_t* __re
                     if (dup) {
                         v[k1] = a;  // zero-padded partner: (a + 0*t, a - 0*t)
                     } else {
-                        uint32_t tmp = mmul(c, tw);
+                        uint32_t tmp = mmul_t2(c, tw);
                         v[k0] = madd(a, tmp);
                         v[k1] = msub(a, tmp);
                     }
                 } else {
                     v[k0] = madd(a, c);
-                    v[k1] = mmul(msub(a, c), tw);
+                    v[k1] = mmul_t2(msub(a, c), tw);
                 }
             }
         }
     } else if constexpr (MAY_BE_CIRCLE) {
         // Circle stage (B == 0, Q == 0, s_lo == 0): pairs (2h', 2h'+1), h' = hp0 + hk,
         // twiddles [y, -y, -x, x] from consecutive pairs (x, y) of T_0.
-        const uint32_t* T0 = tower + ((1u << (ps.tower_log - 1)) - (1u << (ps.m - 1)));
-        const uint32_t kk0 = ((base >> 1) | (r << 3)) >> 2;
-        const uint32_t x0 = T0[2 * kk0], y0 = T0[2 * kk0 + 1], x1 = T0[2 * kk0 + 2], y1 = T0[2 * kk0 + 3];
+        const uint32_t x0 = w[0], y0 = w[1], x1 = w[2], y1 = w[3];
 #pragma unroll
         for (int hk = 0; hk < RX / 2; hk++) {
             const int k0 = 2 * hk, k1 = k0 + 1;
             const uint32_t x = hk < 4 ? x0 : x1, y = hk < 4 ? y0 : y1;
             const int rr = hk & 3;
-            const uint32_t w = rr < 2 ? y : x;
-            const uint32_t t = (rr == 1 || rr == 2) ? mneg(w) : w;
+            const uint32_t wv = rr < 2 ? y : x;
+            const uint32_t t = (rr == 1 || rr == 2) ? mneg_t2(wv) : wv;  // doubled words
             uint32_t a = v[k0], c = v[k1];
             if (!INV) {
-                uint32_t tmp = mmul(c, t);
+                uint32_t tmp = mmul_t2(c, t);
                 v[k0] = madd(a, tmp);
                 v[k1] = msub(a, tmp);
             } else {
                 v[k0] = madd(a, c);
-                v[k1] = mmul(msub(a, c), t);
+                v[k1] = mmul_t2(msub(a, c), t);
             }
         }
     }
 }
 
-// One group: row bits [B, B+4) owned, active local stages [LO, HI).
+// One group: row bits [B, B+4) owned, active local stages [LO, HI); twiddles w[Q] of
+// stage B + Q are loaded ahead (rx_group_tw).
 template <bool INV, int R, int LOGC, int B, int LO, int HI>
-__device__ __forceinline__ void rx_group(uint32_t (&v)[RX], const uint32_t* __restrict__ tower, const RxPass& ps,
-                                         uint32_t hi, uint32_t r, uint32_t base, bool first) {
+__device__ __forceinline__ void rx_group_tw(uint32_t (&w)[4][8], const uint32_t* __restrict__ tower, const RxPass& ps,
+                                            uint32_t hi, uint32_t r, uint32_t base) {
+    if (B + 3 >= LO && B + 3 < HI) rx_tw_load<INV, R, LOGC, B, 3>(w[3], tower, ps, hi, r, base);
+    if (B + 2 >= LO && B + 2 < HI) rx_tw_load<INV, R, LOGC, B, 2>(w[2], tower, ps, hi, r, base);
+    if (B + 1 >= LO && B + 1 < HI) rx_tw_load<INV, R, LOGC, B, 1>(w[1], tower, ps, hi, r, base);
+    if (B + 0 >= LO && B + 0 < HI) rx_tw_load<INV, R, LOGC, B, 0>(w[0], tower, ps, hi, r, base);
+}
+
+template <bool INV, int R, int LOGC, int B, int LO, int HI>
+__device__ __forceinline__ void rx_group(uint32_t (&v)[RX], const uint32_t (&w)[4][8], const RxPass& ps, bool first) {
     if (!INV) {
-        if (B + 3 >= LO && B + 3 < HI) rx_stage<INV, R, LOGC, B, 3>(v, tower, ps, hi, r, base, first && ps.dup_top);
-        if (B + 2 >= LO && B + 2 < HI) rx_stage<INV, R, LOGC, B, 2>(v, tower, ps, hi, r, base, false);
-        if (B + 1 >= LO && B + 1 < HI) rx_stage<INV, R, LOGC, B, 1>(v, tower, ps, hi, r, base, false);
-        if (B + 0 >= LO && B + 0 < HI) rx_stage<INV, R, LOGC, B, 0>(v, tower, ps, hi, r, base, false);
+        if (B + 3 >= LO && B + 3 < HI) rx_stage<INV, R, LOGC, B, 3>(v, w[3], ps, first && ps.dup_top);
+        if (B + 2 >= LO && B + 2 < HI) rx_stage<INV, R, LOGC, B, 2>(v, w[2], ps, false);
+        if (B + 1 >= LO && B + 1 < HI) rx_stage<INV, R, LOGC, B, 1>(v, w[1], ps, false);
+        if (B + 0 >= LO && B + 0 < HI) rx_stage<INV, R, LOGC, B, 0>(v, w[0], ps, false);
     } else {
-        if (B + 0 >= LO && B + 0 < HI) rx_stage<INV, R, LOGC, B, 0>(v, tower, ps, hi, r, base, false);
-        if (B + 1 >= LO && B + 1 < HI) rx_stage<INV, R, LOGC, B, 1>(v, tower, ps, hi, r, base, false);
-        if (B + 2 >= LO && B + 2 < HI) rx_stage<INV, R, LOGC, B, 2>(v, tower, ps, hi, r, base, false);
-        if (B + 3 >= LO && B + 3 < HI) rx_stage<INV, R, LOGC, B, 3>(v, tower, ps, hi, r, base, false);
+        if (B + 0 >= LO && B + 0 < HI) rx_stage<INV, R, LOGC, B, 0>(v, w[0], ps, false);
+        if (B + 1 >= LO && B + 1 < HI) rx_stage<INV, R, LOGC, B, 1>(v, w[1], ps, false);
+        if (B + 2 >= LO && B + 2 < HI) rx_stage<INV, R, LOGC, B, 2>(v, w[2], ps, false);
+        if (B + 3 >= LO && B + 3 < HI) rx_stage<INV, R, LOGC, B, 3>(v, w[3], ps, false);
     }
 }
 
@@ -226,10 +251,13 @@ struct RxGroupShape {
     static constexpr int B = INV ? (GI * RX_LOG + RX_LOG <= R ? GI * RX_LOG : R - RX_LOG) : LO;
 };
 
+// w holds this group's twiddles (loaded before the previous exchange); the next group's
+// twiddles are loaded before this group's exchange so their latency overlaps it.
 template <bool INV, int R, int LOGC, int GI>
-__device__ __forceinline__ void rx_run(uint32_t (&v)[RX], uint32_t* lds, gcu32* __restrict__ src,
-                                       gu32* __restrict__ dst, const uint32_t* __restrict__ tower,
-                                       const RxPass& ps, uint32_t hi, uint32_t r, uint32_t lane, uint32_t base) {
+__device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4][8], uint32_t* lds,
+                                       gcu32* __restrict__ src, gu32* __restrict__ dst,
+                                       const uint32_t* __restrict__ tower, const RxPass& ps, uint32_t hi, uint32_t r,
+                                       uint32_t lane, uint32_t base) {
     constexpr int NG = (R + RX_LOG - 1) / RX_LOG;
     if constexpr (GI < NG) {
         using S = RxGroupShape<INV, R, GI>;
@@ -261,12 +289,15 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], uint32_t* lds, gcu32* 
 #pragma unroll
             for (int k = 0; k < RX; k++) v[k] = lds[rx_lds<LOGC>(row0 + ((uint32_t)k << B), lane)];
         }
-        rx_group<INV, R, LOGC, B, S::LO, S::HI>(v, tower, ps, hi, r, base, GI == 0);
-        if (GI + 1 < NG) {
+        rx_group<INV, R, LOGC, B, S::LO, S::HI>(v, w, ps, GI == 0);
+        if constexpr (GI + 1 < NG) {
+            using SN = RxGroupShape<INV, R, GI + 1>;
+            uint32_t wn[4][8];
+            rx_group_tw<INV, R, LOGC, SN::B, SN::LO, SN::HI>(wn, tower, ps, hi, r, base);
             if (GI > 0) __syncthreads();
 #pragma unroll
             for (int k = 0; k < RX; k++) lds[rx_lds<LOGC>(row0 + ((uint32_t)k << B), lane)] = v[k];
-            rx_run<INV, R, LOGC, GI + 1>(v, lds, src, dst, tower, ps, hi, r, lane, base);
+            rx_run<INV, R, LOGC, GI + 1>(v, wn, lds, src, dst, tower, ps, hi, r, lane, base);
         } else {
             const uint32_t g0 = base | (row0 << ps.s_lo) | lane;
             if (ps.scale != 1) {
@@ -302,7 +333,10 @@ __global__ __launch_bounds__(512) void fft_rx_kernel(const uint32_t* const* __re
     const uint32_t lane = t & ((1u << LOGC) - 1);
     const uint32_t r = t >> LOGC;
     uint32_t v[RX];
-    rx_run<INV, R, LOGC, 0>(v, lds, (gcu32*)src_cols[col], (gu32*)dst_cols[col], tower, ps, hi, r, lane, base);
+    uint32_t w[4][8];
+    using S0 = RxGroupShape<INV, R, 0>;
+    rx_group_tw<INV, R, LOGC, S0::B, S0::LO, S0::HI>(w, tower, ps, hi, r, base);
+    rx_run<INV, R, LOGC, 0>(v, w, lds, (gcu32*)src_cols[col], (gu32*)dst_cols[col], tower, ps, hi, r, lane, base);
 }
 
 // m <= 2: closed forms (transforms.zig:155-179,256-292), one thread per column.
@@ -394,7 +428,7 @@ static void run_rx_pass(Ctx& c, bool inverse, const uint32_t* const* src, uint32
     SR_CHECK(blocks < (1ull << 31), "fft grid too large");
     SR_CHECK(R >= (uint32_t)RX_LOG && nt <= 512 && logc <= ps.s_lo && tile_log <= ps.m, "fft pass shape");
     SR_CHECK(logc != 0 || ps.s_lo == 0, "fft: contiguous pass must start at stage 0");
-    const uint32_t* tower = inverse ? c.d_itw : c.d_tw;
+    const uint32_t* tower = inverse ? c.d_itw2 : c.d_tw2;  // doubled: products via mmul_t2
     KTimer kt(c,
               std::string(inverse ? "fft_ifft_pass" : "fft_lde_pass") + " m=" + std::to_string(ps.m) +
                   " R=" + std::to_string(R) + " C=" + std::to_string(1u << logc) + " cols=" + std::to_string(n_cols),
@@ -493,13 +527,16 @@ void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint3
 // Layer of length 2^(k-1) holds the x-coordinates of the first half of
 // Coset.halfOdds(k) in bit-reversed order (twiddles.zig:67-91); layers for
 // k = M-1 .. 1 are concatenated and end with a trailing 1.
-__global__ void tower_kernel(uint32_t M, uint32_t* __restrict__ tw, uint32_t* __restrict__ itw, GenDoubles g) {
+__global__ void tower_kernel(uint32_t M, uint32_t* __restrict__ tw, uint32_t* __restrict__ itw,
+                             uint32_t* __restrict__ tw2, uint32_t* __restrict__ itw2, GenDoubles g) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t total = 1ull << (M - 1);
     if (i >= total) return;
     if (i == total - 1) {
         tw[i] = 1;
         itw[i] = 1;
+        tw2[i] = 2;
+        itw2[i] = 2;
         return;
     }
     // find layer: offsets 2^(M-1) - 2^k .. 2^(M-1) - 2^(k-1) hold the layer of length 2^(k-1).
@@ -511,14 +548,17 @@ __global__ void tower_kernel(uint32_t M, uint32_t* __restrict__ tw, uint32_t* __
     uint32_t nat = bitrev32(j, lg);
     uint32_t init = half_odds_initial(k), step = subgroup_gen(k);
     pt p = idx_to_point(g, (uint32_t)((init + (uint64_t)step * nat) & ORDER_MASK));
+    const uint32_t ix = minv(p.x);
     tw[i] = p.x;
-    itw[i] = minv(p.x);
+    itw[i] = ix;
+    tw2[i] = p.x << 1;
+    itw2[i] = ix << 1;
 }
 
-void launch_tower(Ctx& c, uint32_t log, uint32_t* d_tw, uint32_t* d_itw) {
+void launch_tower(Ctx& c, uint32_t log, uint32_t* d_tw, uint32_t* d_itw, uint32_t* d_tw2, uint32_t* d_itw2) {
     uint64_t n = 1ull << (log - 1);
     hipLaunchKernelGGL(tower_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, c.stream, log, d_tw, d_itw,
-                       c.gen);
+                       d_tw2, d_itw2, c.gen);
     SR_HIP(hipGetLastError());
 }
 

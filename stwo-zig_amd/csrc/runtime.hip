@@ -89,7 +89,7 @@ void ctx_release(Ctx& c) {
     c.ev_pool.clear();
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
-    c.d_tw = c.d_itw = nullptr;
+    c.d_tw = c.d_itw = c.d_tw2 = c.d_itw2 = nullptr;
     c.tower_log = 0;
 }
 
@@ -153,13 +153,19 @@ void Ctx::ensure_tower(uint32_t log) {
     uint64_t n = 1ull << (log - 1);
     uint32_t* tw = (uint32_t*)alloc(n * 4);
     uint32_t* itw = (uint32_t*)alloc(n * 4);
-    launch_tower(*this, log, tw, itw);
+    uint32_t* tw2 = (uint32_t*)alloc(n * 4);
+    uint32_t* itw2 = (uint32_t*)alloc(n * 4);
+    launch_tower(*this, log, tw, itw, tw2, itw2);
     if (d_tw) {
         free(d_tw);
         free(d_itw);
+        free(d_tw2);
+        free(d_itw2);
     }
     d_tw = tw;
     d_itw = itw;
+    d_tw2 = tw2;
+    d_itw2 = itw2;
     tower_log = log;
 }
 

@@ -67,6 +67,10 @@ struct Ctx {
     uint32_t tower_log = 0;
     uint32_t* d_tw = nullptr;
     uint32_t* d_itw = nullptr;
+    // The same towers doubled (2t as a plain u32 word, < 2^32): the register FFT multiplies by
+    // 2t so that b*t mod p reduces with one shift (mmul_t2).
+    uint32_t* d_tw2 = nullptr;
+    uint32_t* d_itw2 = nullptr;
     // Domain tables per circle log: bit-reversed domain points and inverse y of the half coset.
     std::map<uint32_t, uint32_t*> domain_xy;     // 2 * 2^log words: x[] then y[]
     std::map<uint32_t, uint32_t*> circle_inv_y;  // 2^(log-1) words
@@ -285,7 +289,7 @@ void launch_widefib_composition(Ctx& c, const uint32_t* const* d_cols, uint32_t 
 uint64_t launch_pow_grind(Ctx& c, const uint8_t prefix[32], uint32_t bits, uint64_t base, uint64_t count);
 // Gather words: out[i] = word at absolute device address d_addr[i].
 void launch_gather(Ctx& c, const uint64_t* d_addr, uint64_t n, uint32_t* d_out);
-void launch_tower(Ctx& c, uint32_t log, uint32_t* d_tw, uint32_t* d_itw);
+void launch_tower(Ctx& c, uint32_t log, uint32_t* d_tw, uint32_t* d_itw, uint32_t* d_tw2, uint32_t* d_itw2);
 void launch_domain_points(Ctx& c, uint32_t log, uint32_t* d_xy);
 void launch_circle_inv_y(Ctx& c, uint32_t log, uint32_t* d_out);
 void launch_batch_inverse(Ctx& c, const uint32_t* d_in, uint64_t n, uint32_t* d_out);

@@ -11,7 +11,10 @@
  *   - sizes are uint64_t / uint32_t log sizes; no C++ or torch types cross the ABI;
  *   - "device pointers" are HBM addresses obtained from stwo_rocm_buffer_alloc;
  *     arrays *of* device pointers (`cols`) are host arrays, as in ffi.zig
- *     (`values: [*][*]u32`);
+ *     (`values: [*][*]u32`); every device pointer must be 16-byte aligned (columns and
+ *     QM31 arrays are read with 16-byte loads; buffer_alloc returns 256-byte aligned
+ *     memory, and a column inside a buffer starts at a multiple of 4 words); the column
+ *     entries check this and fail with STWO_ROCM_EINVAL;
  *   - M31 values are canonical u32 words in [0, 2^31-1); a QM31 is 4 words
  *     (c0.a, c0.b, c1.a, c1.b) (src/core/fields/qm31.zig:64-70); a circle point over
  *     QM31 is 8 words (x then y); hashes are 32 bytes;

@@ -34,6 +34,12 @@ static int guarded(stwo_rocm_ctx* ctx, F&& f) {
     }
 }
 
+// Column and QM31 arrays are read with 16-byte loads.
+static void check_aligned(const void* const* ptrs, uint32_t n, const char* what) {
+    for (uint32_t i = 0; i < n; i++)
+        SR_CHECK(((uintptr_t)ptrs[i] & 15) == 0, std::string(what) + ": device pointers must be 16-byte aligned");
+}
+
 static qm31 qin(const uint32_t* w) {
     for (int i = 0; i < 4; i++) SR_CHECK(w[i] < P, "non-canonical M31 word");
     return qload(w);
@@ -191,6 +197,7 @@ int stwo_rocm_circle_interpolate(stwo_rocm_ctx* ctx, uint32_t* const* cols, uint
     return guarded(ctx, [&](Ctx& c) {
         SR_CHECK(cols || !n_cols, "null columns");
         SR_CHECK(log_size >= 1 && log_size <= 30, "log size out of range");
+        check_aligned((const void* const*)cols, n_cols, "circle_interpolate");
         if (log_size >= 3) c.ensure_tower(log_size);
         DBuf d = upload(c, cols, n_cols);
         double ms = timed(c, [&] {
@@ -205,6 +212,8 @@ int stwo_rocm_circle_evaluate(stwo_rocm_ctx* ctx, const uint32_t* const* coeffs,
                               uint32_t log_blowup, uint32_t* const* out, double* gpu_ms) {
     return guarded(ctx, [&](Ctx& c) {
         SR_CHECK((coeffs && out) || !n_cols, "null columns");
+        check_aligned((const void* const*)coeffs, n_cols, "circle_evaluate");
+        check_aligned((const void* const*)out, n_cols, "circle_evaluate");
         uint32_t m = log_size + log_blowup;
         SR_CHECK(m >= 1 && m <= 30, "log size out of range");
         if (m >= 3) c.ensure_tower(m);
@@ -223,6 +232,7 @@ int stwo_rocm_eval_at_point(stwo_rocm_ctx* ctx, const uint32_t* const* coeffs, u
         SR_CHECK(point && out_host && (coeffs || !n_cols), "null argument");
         SR_CHECK(log_size <= 30, "log size out of range");
         if (!n_cols) return;
+        check_aligned((const void* const*)coeffs, n_cols, "eval_at_point");
         qm31 px = qin(point), py = qin(point + 4);
         qm31 f[32];
         if (log_size >= 1) f[0] = py;

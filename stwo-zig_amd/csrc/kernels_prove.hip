@@ -252,83 +252,148 @@ void launch_widefib_trace(Ctx& c, uint32_t log_n, uint32_t seq_len, uint32_t mod
 struct Factors {
     qm31 f[30];
 };
-__global__ __launch_bounds__(256) void eval_weights_kernel(Factors fs, uint32_t log_size, uint4* __restrict__ w) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (1u << log_size)) return;
+// w_i = T_hi[i >> h] * T_lo[i & (2^h - 1)] with T_lo[l] = prod_{bit j of l} f_j (j < h) and
+// T_hi[u] = prod_{bit j of u} f_{h+j}: one product per weight (field products are exact, so
+// the split gives the same values).  The tables come first, one thread per entry.
+__global__ __launch_bounds__(256) void eval_weight_tables_kernel(Factors fs, uint32_t log_size, uint32_t h,
+                                                                 uint4* __restrict__ tab) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_lo = 1u << h, n_hi = 1u << (log_size - h);
+    if (k >= n_lo + n_hi) return;
+    const bool lo = k < n_lo;
+    const uint32_t idx = lo ? k : k - n_lo, j0 = lo ? 0 : h, nb = lo ? h : log_size - h;
     qm31 acc = qone();
-    for (uint32_t j = 0; j < log_size; j++)
-        if ((i >> j) & 1) acc = qmul(acc, fs.f[j]);
-    w[i] = make_uint4(acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b);
+    for (uint32_t j = 0; j < nb; j++)
+        if ((idx >> j) & 1) acc = qmul(acc, fs.f[j0 + j]);
+    tab[k] = make_uint4(acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b);
+}
+__global__ __launch_bounds__(256) void eval_weights_kernel(const uint4* __restrict__ tab, uint32_t log_size, uint32_t h,
+                                                           uint4* __restrict__ w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (1u << log_size)) return;
+    const uint4 a = tab[i & ((1u << h) - 1)], b = tab[(1u << h) + (i >> h)];
+    const qm31 v = qmul(qm31{{a.x, a.y}, {a.z, a.w}}, qm31{{b.x, b.y}, {b.z, b.w}});
+    w[i] = make_uint4(v.c0.a, v.c0.b, v.c1.a, v.c1.b);
 }
 void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w) {
     SR_CHECK(log_size <= 30, "eval weights: log size too large");
     Factors fs;
     for (uint32_t j = 0; j < log_size; j++) fs.f[j] = factors[j];
-    uint32_t n = 1u << log_size;
+    const uint32_t n = 1u << log_size, h = log_size / 2;
+    const uint32_t nt = (1u << h) + (1u << (log_size - h));
+    DBuf tab(&c, 16ull * nt);
     KTimer kt(c, "eval_weights", 16.0 * n);
-    hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, fs, log_size, (uint4*)d_w);
+    hipLaunchKernelGGL(eval_weight_tables_kernel, dim3((nt + 255) / 256), dim3(256), 0, c.stream, fs, log_size, h,
+                       tab.as<uint4>());
+    hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, (const uint4*)tab.as<uint4>(),
+                       log_size, h, (uint4*)d_w);
     SR_HIP(hipGetLastError());
 }
 
-// Grid (chunks, columns): each block sums 256 * EV_PER_THREAD products of one
-// column into four 64-bit partial sums, then one 64-bit atomic add per
-// coordinate (integer sums are exact and order independent).
+// Grid (chunks, column groups of EV_CB): each block loads a weight once for EV_CB columns,
+// accumulates raw 62-bit products four elements at a time (< 2^64) before one Mersenne
+// fold, and stores its 4 * EV_CB partial sums in its own slot; a second kernel adds the
+// slots (integer sums are exact and order independent).
 constexpr uint32_t EV_PER_THREAD = 16;
-__global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const* __restrict__ cols,
+constexpr uint32_t EV_CB = 8;
+__device__ __forceinline__ uint64_t ev_wave_sum(uint64_t v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m), hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+__global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
                                                            uint32_t log_size, const uint4* __restrict__ w,
-                                                           unsigned long long* __restrict__ acc) {
-    __shared__ unsigned long long red[4][256];
-    const uint32_t col = blockIdx.y;
-    gcu32* __restrict__ cv = (gcu32*)cols[col];
+                                                           unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long red[4][EV_CB * 4];
+    const uint32_t t = threadIdx.x;
+    const uint32_t c0 = blockIdx.y * EV_CB;
+    const uint32_t nc = n_cols - c0 < EV_CB ? n_cols - c0 : EV_CB;
     const uint32_t n = 1u << log_size;
-    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    uint32_t base = blockIdx.x * (256 * EV_PER_THREAD);
-#pragma unroll 4
-    for (uint32_t k = 0; k < EV_PER_THREAD; k++) {
-        uint32_t i = base + k * 256 + threadIdx.x;
-        if (i < n) {
-            uint32_t v = cv[i];
-            uint4 ww = w[i];
-            s0 += mfold((uint64_t)v * ww.x);
-            s1 += mfold((uint64_t)v * ww.y);
-            s2 += mfold((uint64_t)v * ww.z);
-            s3 += mfold((uint64_t)v * ww.w);
+    uint64_t s[EV_CB][4];
+#pragma unroll
+    for (uint32_t c = 0; c < EV_CB; c++) s[c][0] = s[c][1] = s[c][2] = s[c][3] = 0;
+    if (n >= 4) {
+        // lane t takes 4 consecutive elements per group: one 16-byte coefficient load per column
+        // and 64 contiguous bytes of weights (the block covers 256 * EV_PER_THREAD elements)
+#pragma unroll
+        for (uint32_t g = 0; g < EV_PER_THREAD / 4; g++) {
+            const uint32_t i0 = blockIdx.x * (256 * EV_PER_THREAD) + g * 1024 + 4 * t;
+            if (i0 >= n) break;
+            uint4 ww[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) ww[e] = w[i0 + e];
+#pragma unroll
+            for (uint32_t c = 0; c < EV_CB; c++) {
+                if (c < nc) {
+                    const u32x4 v4 = *(const SR_GLOBAL u32x4*)((gcu32*)cols[c0 + c] + i0);
+                    const uint64_t v[4] = {v4.x, v4.y, v4.z, v4.w};
+                    uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        r0 += v[e] * ww[e].x;
+                        r1 += v[e] * ww[e].y;
+                        r2 += v[e] * ww[e].z;
+                        r3 += v[e] * ww[e].w;
+                    }
+                    s[c][0] += mfold(r0);
+                    s[c][1] += mfold(r1);
+                    s[c][2] += mfold(r2);
+                    s[c][3] += mfold(r3);
+                }
+            }
+        }
+    } else if (t < n) {
+        const uint4 ww = w[t];
+#pragma unroll
+        for (uint32_t c = 0; c < EV_CB; c++) {
+            if (c < nc) {
+                const uint64_t v = cols[c0 + c][t];
+                s[c][0] += mfold(v * ww.x);
+                s[c][1] += mfold(v * ww.y);
+                s[c][2] += mfold(v * ww.z);
+                s[c][3] += mfold(v * ww.w);
+            }
         }
     }
-    red[0][threadIdx.x] = s0;
-    red[1][threadIdx.x] = s1;
-    red[2][threadIdx.x] = s2;
-    red[3][threadIdx.x] = s3;
+#pragma unroll
+    for (uint32_t c = 0; c < EV_CB; c++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint64_t x = ev_wave_sum(s[c][k]);
+            if ((t & 63) == 0) red[t >> 6][4 * c + k] = x;
+        }
     __syncthreads();
-    for (uint32_t st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st)
-            for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
-        __syncthreads();
+    if (t < 4 * nc) {
+        const uint64_t x = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+        part[(uint64_t)(4 * c0 + t) * gridDim.x + blockIdx.x] = x;
     }
-    if (threadIdx.x < 4) atomicAdd(&acc[col * 4 + threadIdx.x], red[threadIdx.x][0]);
 }
-__global__ void zero_u64_kernel(unsigned long long* p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-__global__ void reduce_u64_kernel(const unsigned long long* __restrict__ acc, uint32_t n, uint32_t* __restrict__ out) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = mred64(acc[i]);
+// out[k] = sum over chunks of part[k][chunk] mod p: one wave per output word.
+__global__ __launch_bounds__(256) void eval_reduce_kernel(const unsigned long long* __restrict__ part, uint32_t chunks,
+                                                          uint32_t n_out, uint32_t* __restrict__ out) {
+    const uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    if (k >= n_out) return;
+    uint64_t x = 0;
+    for (uint32_t j = l; j < chunks; j += 64) x += part[(uint64_t)k * chunks + j];
+    x = ev_wave_sum(x);
+    if (l == 0) out[k] = mred64(x);
 }
 void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
                          const uint32_t* d_w, uint32_t* d_out) {
     if (n_cols == 0) return;
-    DBuf acc(&c, (size_t)n_cols * 4 * 8);
-    uint32_t nw = n_cols * 4;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3((nw + 255) / 256), dim3(256), 0, c.stream,
-                       acc.as<unsigned long long>(), nw);
-    uint32_t n = 1u << log_size;
-    uint32_t chunks = (n + 256 * EV_PER_THREAD - 1) / (256 * EV_PER_THREAD);
+    const uint32_t n = 1u << log_size;
+    const uint32_t chunks = (n + 256 * EV_PER_THREAD - 1) / (256 * EV_PER_THREAD);
+    const uint32_t groups = (n_cols + EV_CB - 1) / EV_CB;
+    const uint32_t nw = n_cols * 4;
+    DBuf part(&c, 8ull * nw * chunks);
     KTimer kt(c, "eval_columns", (4.0 * n_cols + 16.0) * n);
-    hipLaunchKernelGGL(eval_columns_kernel, dim3(chunks, n_cols), dim3(256), 0, c.stream, d_cols, log_size,
-                       (const uint4*)d_w, acc.as<unsigned long long>());
-    hipLaunchKernelGGL(reduce_u64_kernel, dim3((nw + 255) / 256), dim3(256), 0, c.stream,
-                       acc.as<unsigned long long>(), nw, d_out);
+    hipLaunchKernelGGL(eval_columns_kernel, dim3(chunks, groups), dim3(256), 0, c.stream, d_cols, n_cols, log_size,
+                       (const uint4*)d_w, part.as<unsigned long long>());
+    hipLaunchKernelGGL(eval_reduce_kernel, dim3((nw + 3) / 4), dim3(256), 0, c.stream,
+                       (const unsigned long long*)part.as<unsigned long long>(), chunks, nw, d_out);
     SR_HIP(hipGetLastError());
 }
 

@@ -346,3 +346,19 @@ def test_composition_matches_oracle_proof_path(ctx):
             a, b = b, cv
         d = den[row >> n]
         assert tuple(int(v) for v in got[row]) == tuple(w * d % mr.P for w in acc)
+
+
+def test_unaligned_column_pointers_are_rejected(ctx):
+    """Column entries read 16-byte words: a pointer 4 bytes into a buffer is refused on the host,
+    before any kernel is launched."""
+    import stwo_rocm
+    rng = np.random.default_rng(9)
+    col = rng.integers(0, mr.P, 1 << 6, dtype=np.uint32)
+    buf = ctx.to_device(np.concatenate([col, col]))
+    with pytest.raises(stwo_rocm.RocmError) as e:
+        ctx.eval_at_point([buf.ptr + 4], 6, [1, 2, 3, 4, 5, 6, 7, 8])
+    assert e.value.code == 1 and "16-byte aligned" in str(e.value)
+    # an aligned column inside the same buffer (its second copy, 64 words in) works
+    got = ctx.eval_at_point([buf.ptr + 4 * 64], 6, [1, 2, 3, 4, 5, 6, 7, 8])
+    assert list(got[0]) == oracle.eval_at_point(col, 6, [1, 2, 3, 4, 5, 6, 7, 8])
+    buf.free()

@@ -15,7 +15,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libstwo_rocm.so")
 SOURCES = ["runtime.hip", "kernels_fft.hip", "kernels_merkle.hip", "kernels_fri.hip", "kernels_prove.hip",
-           "kernels_misc.hip", "kernels_chan.hip", "kernels_logup.hip", "gkr.hip", "logup_pipeline.hip",
+           "kernels_misc.hip", "kernels_chan.hip", "kernels_logup.hip", "gkr.hip", "transcript.hip", "logup_pipeline.hip",
            "batched_pipeline.hip", "serving.hip",
            "prover.hip",
            "capi.hip"]

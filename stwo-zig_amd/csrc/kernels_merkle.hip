@@ -386,7 +386,7 @@ DeviceTree* merkle_commit_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size
 void tree_read_root(Ctx& c, DeviceTree& t) {
     uint8_t* h = c.pinned(32);
     SR_HIP(hipMemcpyAsync(h, t.layers, 32, hipMemcpyDeviceToHost, c.stream));
-    SR_HIP(hipStreamSynchronize(c.stream));
+    c.sync();
     memcpy(t.root, h, 32);
 }
 

@@ -19,13 +19,6 @@ namespace sr {
 // Entries are flat 32-byte descriptors (column, lift shift, coefficient) read
 // with scalar loads; QB column values are loaded at a time so that QB loads
 // are in flight per wave, and four raw products are summed before a fold.
-struct QEntryDev {
-    const uint32_t* col;
-    uint32_t shift;  // lift: row -> ((row >> shift) << 1) + (row & 1); 1 = same size
-    uint32_t pad;
-    uint32_t coeff[4];
-};
-static_assert(sizeof(QEntryDev) == 32, "quotient entry layout");
 
 __device__ __forceinline__ uint32_t lift_row(uint32_t row, uint32_t shift) {
     return ((row >> shift) << 1) + (row & 1);
@@ -120,12 +113,19 @@ void launch_quotients(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_c
         SR_CHECK((uint64_t)h_batches[b].first + h_batches[b].count <= n_entries, "quotients: batch range");
     DBuf d_ent = upload(c, ents.data(), ents.size());
     DBuf d_bat = upload(c, h_batches, n_batches);
+    launch_quotients_dev(c, d_ent.as<QEntryDev>(), n_entries, d_bat.as<QuotientBatchDev>(), n_batches, in_bytes,
+                         log_size, out, d_leaf_out);
+}
+
+void launch_quotients_dev(Ctx& c, const QEntryDev* d_ent, uint32_t n_entries, const QuotientBatchDev* d_bat,
+                          uint32_t n_batches, double in_bytes, uint32_t log_size, uint32_t* const* out,
+                          uint8_t* d_leaf_out) {
+    SR_CHECK(n_entries >= 1 && n_batches >= 1, "quotients: no samples");
     const uint32_t* xy = c.domain_points(log_size);
     uint32_t n = 1u << log_size;
     KTimer kt(c, "quotients", in_bytes + (8.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n);
-    hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_ent.as<QEntryDev>(),
-                       d_bat.as<QuotientBatchDev>(), n_batches, log_size, xy, out[0], out[1], out[2], out[3],
-                       d_leaf_out, c.leaf_seed);
+    hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_ent, d_bat, n_batches,
+                       log_size, xy, out[0], out[1], out[2], out[3], d_leaf_out, c.leaf_seed);
     SR_HIP(hipGetLastError());
 }
 
@@ -267,6 +267,21 @@ __global__ __launch_bounds__(256) void eval_weight_tables_kernel(Factors fs, uin
         if ((idx >> j) & 1) acc = qmul(acc, fs.f[j0 + j]);
     tab[k] = make_uint4(acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b);
 }
+__global__ __launch_bounds__(256) void eval_weight_tables_dev_kernel(const uint4* __restrict__ fs, uint32_t log_size,
+                                                                     uint32_t h, uint4* __restrict__ tab) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_lo = 1u << h, n_hi = 1u << (log_size - h);
+    if (k >= n_lo + n_hi) return;
+    const bool lo = k < n_lo;
+    const uint32_t idx = lo ? k : k - n_lo, j0 = lo ? 0 : h, nb = lo ? h : log_size - h;
+    qm31 acc = qone();
+    for (uint32_t j = 0; j < nb; j++)
+        if ((idx >> j) & 1) {
+            const uint4 f = fs[j0 + j];
+            acc = qmul(acc, qm31{{f.x, f.y}, {f.z, f.w}});
+        }
+    tab[k] = make_uint4(acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b);
+}
 __global__ __launch_bounds__(256) void eval_weights_kernel(const uint4* __restrict__ tab, uint32_t log_size, uint32_t h,
                                                            uint4* __restrict__ w) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -285,6 +300,19 @@ void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint
     KTimer kt(c, "eval_weights", 16.0 * n);
     hipLaunchKernelGGL(eval_weight_tables_kernel, dim3((nt + 255) / 256), dim3(256), 0, c.stream, fs, log_size, h,
                        tab.as<uint4>());
+    hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, (const uint4*)tab.as<uint4>(),
+                       log_size, h, (uint4*)d_w);
+    SR_HIP(hipGetLastError());
+}
+
+void launch_eval_weights_dev(Ctx& c, const uint32_t* d_factors, uint32_t log_size, uint32_t* d_w) {
+    SR_CHECK(log_size <= 30, "eval weights: log size too large");
+    const uint32_t n = 1u << log_size, h = log_size / 2;
+    const uint32_t nt = (1u << h) + (1u << (log_size - h));
+    DBuf tab(&c, 16ull * nt);
+    KTimer kt(c, "eval_weights", 16.0 * n);
+    hipLaunchKernelGGL(eval_weight_tables_dev_kernel, dim3((nt + 255) / 256), dim3(256), 0, c.stream,
+                       (const uint4*)d_factors, log_size, h, tab.as<uint4>());
     hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, (const uint4*)tab.as<uint4>(),
                        log_size, h, (uint4*)d_w);
     SR_HIP(hipGetLastError());

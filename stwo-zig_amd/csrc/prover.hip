@@ -129,34 +129,80 @@ struct Gather {
     }
 };
 
+// Compact JSON writer into a growing char buffer: byte values come from a table of their
+// decimal forms, wider integers are formatted two digits at a time.
 struct JsonOut {
     std::string s;
-    JsonOut() { s.reserve(1 << 17); }
-    void raw(const char* t) { s += t; }
+    size_t n = 0;
+    JsonOut() { s.resize(1 << 17); }
+    char* room(size_t k) {
+        if (n + k > s.size()) s.resize(std::max(2 * s.size(), n + k));
+        return &s[n];
+    }
+    void raw(const char* t) {
+        const size_t k = strlen(t);
+        memcpy(room(k), t, k);
+        n += k;
+    }
     void u64(uint64_t v) {
+        static const char d2[] = "00010203040506070809101112131415161718192021222324252627282930313233343536373839"
+                                 "40414243444546474849505152535455565758596061626364656667686970717273747576777879"
+                                 "8081828384858687888990919293949596979899";
         char b[24];
-        int n = 0;
-        do {
-            b[n++] = (char)('0' + v % 10);
-            v /= 10;
-        } while (v);
-        while (n) s.push_back(b[--n]);
+        int k = 24;
+        while (v >= 100) {
+            const unsigned r = (unsigned)(v % 100);
+            v /= 100;
+            b[--k] = d2[2 * r + 1];
+            b[--k] = d2[2 * r];
+        }
+        if (v >= 10) {
+            b[--k] = d2[2 * v + 1];
+            b[--k] = d2[2 * v];
+        } else {
+            b[--k] = (char)('0' + v);
+        }
+        memcpy(room(24 - k), b + k, 24 - k);
+        n += 24 - k;
     }
     void hash(const uint8_t* h) {
-        s += '[';
+        // "[b0,b1,...,b31]": at most 32 * 4 + 1 bytes
+        struct Tab {
+            char txt[256][4];
+            uint8_t len[256];
+            Tab() {
+                for (int v = 0; v < 256; v++) {
+                    int k = 0;
+                    if (v >= 100) txt[v][k++] = (char)('0' + v / 100);
+                    if (v >= 10) txt[v][k++] = (char)('0' + (v / 10) % 10);
+                    txt[v][k++] = (char)('0' + v % 10);
+                    len[v] = (uint8_t)k;
+                }
+            }
+        };
+        static const Tab tab;
+        char* o = room(32 * 4 + 1);
+        char* p = o;
+        *p++ = '[';
         for (int i = 0; i < 32; i++) {
-            if (i) s += ',';
-            u64(h[i]);
+            memcpy(p, tab.txt[h[i]], 4);
+            p += tab.len[h[i]];
+            *p++ = ',';
         }
-        s += ']';
+        p[-1] = ']';
+        n += (size_t)(p - o);
     }
     void qm(const uint32_t* w) {
-        s += '[';
+        raw("[");
         for (int i = 0; i < 4; i++) {
-            if (i) s += ',';
+            if (i) raw(",");
             u64(w[i]);
         }
-        s += ']';
+        raw("]");
+    }
+    std::string finish() {
+        s.resize(n);
+        return std::move(s);
     }
 };
 
@@ -223,6 +269,27 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     }
     ch.mix_root(t0.tree->root);
 
+    // From here to the FRI readback the transcript runs on a device copy of the channel:
+    // the launches that need a drawn value read it from HBM, and the host reads the
+    // commitments, sampled values and channel back once (transcript.hip).
+    const uint32_t last_log = prm.log_last_layer_degree_bound + B;
+    const uint32_t n_inner = L - 1 - last_log;  // inner FRI layers log L-1 .. last_log+1
+    const size_t fbytes = CH_BYTES + 32 * (1 + (size_t)n_inner);
+    DBuf fst(&c, fbytes);  // DevChannel | roots (first FRI layer, inner layers)
+    DevChannel* d_ch = fst.as<DevChannel>();
+    uint8_t* d_roots = fst.as<uint8_t>() + CH_BYTES;
+    {
+        DevChannel hc{};
+        memcpy(hc.digest, ch.digest, 32);
+        hc.n_draws = ch.n_draws;
+        SR_HIP(hipMemcpyAsync(d_ch, c.stage(&hc, sizeof(hc)), sizeof(hc), hipMemcpyHostToDevice, c.stream));
+    }
+    const uint32_t NS = W + 8;  // sampled columns: trace (W) and composition (8)
+    DBuf trs(&c, 16 * (2 + 32 + (size_t)NS));  // OODS point | evaluation factors | sampled values
+    uint32_t* d_oods = trs.as<uint32_t>();
+    uint32_t* d_factors = d_oods + 4 * 2;
+    uint32_t* d_sv = d_factors + 4 * 32;
+
     // Tree 1: main trace -> coefficients -> LDE -> Merkle.
     DBuf trace_own;
     const uint32_t* d_trace = d_trace_in;
@@ -249,27 +316,17 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         commit_coefficients(c, owner, t1, cf, n, B);
     }
     trace_own.reset();
-    tree_read_root(c, *t1.tree);
-    ch.mix_root(t1.tree->root);
     mark();  // main commit
 
-    // mixStatement (wide_fibonacci.zig:496-498).
-    uint32_t st[2] = {n, W};
-    ch.mix_u32s(st, 2);
-
-    // Composition (prove.zig:237-318; wide_fibonacci/component.zig:146-228).
-    qm31 alpha = ch.draw_secure_felt();
+    // mixRoot(main), mixStatement (wide_fibonacci.zig:496-498), composition alpha and its
+    // powers (prove.zig:237-318; wide_fibonacci/component.zig:146-228), on the device.
     const uint32_t NC = W - 2, CL = n + 1;
     const uint64_t CN = 1ull << CL;
-    std::vector<uint32_t> pw(4 * NC);
+    DBuf d_pw(&c, 16ull * NC);
     {
-        qm31 cur = qone();
-        for (uint32_t k = 0; k < NC; k++) {
-            qstore(cur, &pw[4 * k]);
-            cur = qmul(cur, alpha);
-        }
+        const uint32_t st[2] = {n, W};
+        launch_tr_commit_draw(c, d_ch, t1.tree->layers, st, 2, d_pw.as<uint32_t>(), NC);
     }
-    DBuf d_pw = upload(c, pw.data(), pw.size());
     // cosetVanishing(CanonicCoset(n).coset(), domain.at(0/1)) (constraints.zig:16-27).
     uint32_t den_inv[2];
     for (int r = 0; r < 2; r++) {
@@ -318,15 +375,11 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         }
         commit_coefficients(c, owner, t2, cp, n, B);
     }
-    tree_read_root(c, *t2.tree);
-    ch.mix_root(t2.tree->root);
     mark();  // composition
 
-    // OODS point (circle.zig:159-167).
-    qm31 tt = ch.draw_secure_felt();
-    qm31 t2q = qsqr(tt);
-    qm31 inv1 = qinv(qadd(t2q, qone()));
-    QPoint oods = {qmul(qsub(qone(), t2q), inv1), qmul(qadd(tt, tt), inv1)};
+    // mixRoot(composition), OODS point (circle.zig:159-167) and the point-evaluation
+    // factors, on the device.
+    launch_tr_oods(c, d_ch, t2.tree->layers, n, d_oods, d_factors);
 
     // Sampled values (sampled_values.zig): all columns share log n coefficients and
     // the lifting log, so one weight vector serves every column.
@@ -340,110 +393,43 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
             all_lde.push_back(trees[tr]->lde[i]);
             all_lde_logs.push_back(trees[tr]->lde_logs[i]);
         }
-    const uint32_t NS = (uint32_t)all_coeffs.size();
-    std::vector<QPoint> spt(NS, oods);  // fold count L - lde_log = 0 for every column here
-    std::vector<uint32_t> sv(4 * NS);
+    SR_CHECK(all_coeffs.size() == NS, "sampled column count");
+    std::vector<uint32_t> sv(4 * NS);  // read back with the FRI state
     {
-        qm31 f[32];
-        f[0] = spt[0].y;
-        f[1] = spt[0].x;
-        for (uint32_t k = 2; k < n; k++) {
-            qm31 s2 = qsqr(f[k - 1]);
-            f[k] = qsub(qadd(s2, s2), qone());
-        }
         DBuf w(&c, N * 16);
-        launch_eval_weights(c, f, n, w.as<uint32_t>());
+        launch_eval_weights_dev(c, d_factors, n, w.as<uint32_t>());
         DBuf d_cols = upload(c, all_coeffs.data(), NS);
-        DBuf d_out(&c, (size_t)NS * 16);
-        launch_eval_columns(c, d_cols.as<const uint32_t* const>(), NS, n, w.as<uint32_t>(), d_out.as<uint32_t>());
-        uint32_t* h = (uint32_t*)c.pinned(NS * 16);
-        SR_HIP(hipMemcpyAsync(h, d_out.p, NS * 16, hipMemcpyDeviceToHost, c.stream));
-        c.sync();
-        memcpy(sv.data(), h, NS * 16);
+        launch_eval_columns(c, d_cols.as<const uint32_t* const>(), NS, n, w.as<uint32_t>(), d_sv);
     }
-    // sampled_value_transcript.mixIntoChannel: tree, column, sample order.
-    ch.mix_bytes(sv.data(), sv.size() * 4);
     mark();  // sampled values
 
-    // Quotients (samples.zig:245-312; row_evaluation.zig:73-106,402-521; constraints.zig:87-97).
-    qm31 qalpha = ch.draw_secure_felt();
-    std::vector<QuotientBatchDev> batches;
-    std::vector<uint32_t> entry_cols, entry_coeffs;
-    {
-        // One sample per column at the (unfolded) OODS point: one batch, powers in column order.
-        std::vector<std::vector<uint32_t>> per_batch_cols;
-        std::vector<std::vector<qm31>> per_batch_c;
-        std::vector<QPoint> bpts;
-        std::vector<qm31> sa, sb;
-        qm31 rp = qone();
-        for (uint32_t i = 0; i < NS; i++) {
-            QPoint p = spt[i];
-            size_t bi = 0;
-            for (; bi < bpts.size(); bi++)
-                if (qeq(bpts[bi].x, p.x) && qeq(bpts[bi].y, p.y)) break;
-            if (bi == bpts.size()) {
-                bpts.push_back(p);
-                per_batch_cols.emplace_back();
-                per_batch_c.emplace_back();
-                sa.push_back(qzero());
-                sb.push_back(qzero());
-            }
-            qm31 v = qload(&sv[4 * i]);
-            qm31 a = qsub(qconj(v), v);
-            qm31 cc = qsub(qconj(p.y), p.y);
-            qm31 b = qsub(qmul(v, cc), qmul(a, p.y));
-            per_batch_cols[bi].push_back(i);
-            per_batch_c[bi].push_back(qmul(rp, cc));
-            sa[bi] = qadd(sa[bi], qmul(rp, a));
-            sb[bi] = qadd(sb[bi], qmul(rp, b));
-            rp = qmul(rp, qalpha);
-        }
-        for (size_t bi = 0; bi < bpts.size(); bi++) {
-            QuotientBatchDev bd;
-            cm31 prx = bpts[bi].x.c0, pry = bpts[bi].y.c0, pix = bpts[bi].x.c1, piy = bpts[bi].y.c1;
-            bd.det = csub(cmul(prx, piy), cmul(pry, pix));
-            bd.pix = pix;
-            bd.piy = piy;
-            bd.sum_a = sa[bi];
-            bd.sum_b = sb[bi];
-            bd.first = (uint32_t)entry_cols.size();
-            bd.count = (uint32_t)per_batch_cols[bi].size();
-            for (size_t k = 0; k < per_batch_cols[bi].size(); k++) {
-                entry_cols.push_back(per_batch_cols[bi][k]);
-                uint32_t w4[4];
-                qstore(per_batch_c[bi][k], w4);
-                entry_coeffs.insert(entry_coeffs.end(), w4, w4 + 4);
-            }
-            batches.push_back(bd);
-        }
-    }
+    // Quotients (samples.zig:245-312; row_evaluation.zig:73-106,402-521; constraints.zig:87-97):
+    // mixFelts(sampled values), the quotient coefficient and the one batch (every sample is at
+    // the OODS point: the fold count L - lde_log is 0 for every column here) on the device.
     const uint64_t LN = 1ull << L;
     DBuf quot(&c, 4 * LN * 4);
     uint32_t* qc[4];
     for (int k = 0; k < 4; k++) qc[k] = quot.as<uint32_t>() + k * LN;
     DeviceTree* ftree0 = owner.make(L);
-
-    // FRI (prover/fri.zig:377-621).  The commit cascade runs without host round
-    // trips: the launch that produces each layer's root mixes it into a device
-    // copy of the channel and draws the next alpha, which the fold reads from
-    // HBM.  The host channel resumes from the device state after the single
-    // readback below.
-    const uint32_t last_log = prm.log_last_layer_degree_bound + B;
-    uint32_t n_inner = L - 1 - last_log;  // inner layers log L-1 .. last_log+1
-    const size_t fbytes = CH_BYTES + 32 * (1 + (size_t)n_inner);
-    DBuf fst(&c, fbytes);  // DevChannel | roots (first layer, inner layers)
-    DevChannel* d_ch = fst.as<DevChannel>();
-    uint8_t* d_roots = fst.as<uint8_t>() + CH_BYTES;
     {
-        DevChannel hc{};
-        memcpy(hc.digest, ch.digest, 32);
-        hc.n_draws = ch.n_draws;
-        SR_HIP(hipMemcpyAsync(d_ch, c.stage(&hc, sizeof(hc)), sizeof(hc), hipMemcpyHostToDevice, c.stream));
-    }
-    {
-        launch_quotients(c, all_lde.data(), all_lde_logs.data(), entry_cols.data(), entry_coeffs.data(),
-                         (uint32_t)entry_cols.size(), batches.data(), (uint32_t)batches.size(), L, qc,
-                         ftree0->layer(L));
+        std::vector<QEntryDev> ents(NS);
+        double in_bytes = 0;
+        for (uint32_t i = 0; i < NS; i++) {
+            SR_CHECK(all_lde_logs[i] >= 1 && all_lde_logs[i] <= L, "quotients: column larger than the domain");
+            ents[i].col = all_lde[i];
+            ents[i].shift = L - all_lde_logs[i] + 1;
+            ents[i].pad = 0;
+            for (int k = 0; k < 4; k++) ents[i].coeff[k] = 0;
+            in_bytes += 4.0 * (double)(1ull << all_lde_logs[i]);
+        }
+        DBuf d_ents = upload(c, ents.data(), ents.size());
+        DBuf d_bat(&c, sizeof(QuotientBatchDev));
+        launch_tr_quotient(c, d_ch, d_sv, NS, d_oods, d_ents.as<QEntryDev>(), d_bat.as<QuotientBatchDev>());
+        // FRI (prover/fri.zig:377-621).  The commit cascade continues on the device channel: the
+        // launch that produces each layer's root mixes it and draws the next alpha, which the
+        // fold reads from HBM.
+        launch_quotients_dev(c, d_ents.as<QEntryDev>(), NS, d_bat.as<QuotientBatchDev>(), 1, in_bytes, L, qc,
+                             ftree0->layer(L));
         launch_merkle_parents(c, *ftree0, d_ch, d_roots);
     }
     mark();  // quotients + first layer commit
@@ -494,10 +480,18 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     const uint64_t lsz = 1ull << llog;
     std::vector<qm31> lv(lsz);
     {
-        uint8_t* h = c.pinned(fbytes + lsz * 16);
+        // one readback: channel and FRI roots, last layer, the two commitments, sampled values
+        const size_t at_roots = fbytes + lsz * 16, at_sv = at_roots + 64;
+        uint8_t* h = c.pinned(at_sv + 16 * (size_t)NS);
         SR_HIP(hipMemcpyAsync(h, fst.p, fbytes, hipMemcpyDeviceToHost, c.stream));
         SR_HIP(hipMemcpyAsync(h + fbytes, lines.back().p, lsz * 16, hipMemcpyDeviceToHost, c.stream));
+        SR_HIP(hipMemcpyAsync(h + at_roots, t1.tree->layers, 32, hipMemcpyDeviceToHost, c.stream));
+        SR_HIP(hipMemcpyAsync(h + at_roots + 32, t2.tree->layers, 32, hipMemcpyDeviceToHost, c.stream));
+        SR_HIP(hipMemcpyAsync(h + at_sv, d_sv, 16 * (size_t)NS, hipMemcpyDeviceToHost, c.stream));
         c.sync();
+        memcpy(t1.tree->root, h + at_roots, 32);
+        memcpy(t2.tree->root, h + at_roots + 32, 32);
+        memcpy(sv.data(), h + at_sv, 16 * (size_t)NS);
         DevChannel hc;
         memcpy(&hc, h, sizeof(hc));
         SR_CHECK(hc.status == 0, "FRI: channel draw rejected 64 times in a row");
@@ -553,7 +547,10 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     if (prm.pow_bits > 0) {
         uint8_t prefix[32];
         ch.pow_prefix(prm.pow_bits, prefix);
-        uint64_t chunk = prm.pow_bits <= 16 ? (1ull << 20) : (1ull << 24);
+        // chunks are searched in order and each returns its lowest nonce, so the size only sets
+        // how much is ground past the first solution: ~16x the expected work, 2^12..2^26
+        const uint32_t cl = std::min(26u, std::max(12u, prm.pow_bits + 4));
+        const uint64_t chunk = 1ull << cl;
         for (uint64_t base = 0;; base += chunk) {
             uint64_t r = launch_pow_grind(c, prefix, prm.pow_bits, base, chunk);
             if (r != UINT64_MAX) {
@@ -655,7 +652,6 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
 
     // Proof wire (interop/proof_wire.zig:19-76).
     JsonOut j;
-    j.s.reserve(1 << 17);
     j.raw("{\"config\":{\"pow_bits\":");
     j.u64(prm.pow_bits);
     j.raw(",\"fri_config\":{\"log_blowup_factor\":");
@@ -739,7 +735,7 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         j.qm(w4);
     }
     j.raw("]}}");
-    proof.swap(j.s);
+    proof = j.finish();
     mark();  // encode
 
     SR_HIP(hipEventSynchronize(evs.back()));

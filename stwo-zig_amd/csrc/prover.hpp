@@ -142,6 +142,18 @@ class BatchedPipeline {
     hipGraphExec_t exec = nullptr;
 };
 
+// Transcript steps on the device channel between commitments (transcript.hip).
+// mixRoot(root) [+ mixU32s(words[0..n_words))], alpha = drawSecureFelt, powers[k] = alpha^k.
+void launch_tr_commit_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, const uint32_t* words, uint32_t n_words,
+                           uint32_t* d_powers, uint32_t n_pow);
+// mixRoot(root), t = drawSecureFelt, OODS point (x, y) and the point-evaluation factors.
+void launch_tr_oods(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, uint32_t n_factors, uint32_t* d_oods,
+                    uint32_t* d_factors);
+// mixFelts(sampled values), q = drawSecureFelt, coefficients of the entries (column order)
+// and the single batch at the OODS point (entries' col/shift set by the caller).
+void launch_tr_quotient(Ctx& c, DevChannel* d_ch, const uint32_t* d_sv, uint32_t ns, const uint32_t* d_oods,
+                        QEntryDev* d_ents, QuotientBatchDev* d_bat);
+
 // Continuous-batching proof service (serving.hip).
 struct ServeJob {
     uint32_t kind;  // 0 wide-Fibonacci proof (seeded uniform trace), 1 XOR-lookup LogUp workload

@@ -68,6 +68,7 @@ void ctx_init(Ctx& c, int device, uint64_t hbm_budget) {
     c.device = device;
     SR_HIP(hipSetDevice(device));
     SR_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));  // default priority
+    SR_HIP(hipEventCreateWithFlags(&c.ev_sync, hipEventDisableTiming));
     c.hbm_budget = hbm_budget;
     c.leaf_seed = b2_prefix_seed("leaf");
     c.node_seed = b2_prefix_seed("node");
@@ -77,6 +78,8 @@ void ctx_init(Ctx& c, int device, uint64_t hbm_budget) {
 void ctx_release(Ctx& c) {
     (void)hipSetDevice(c.device);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
+    if (c.ev_sync) (void)hipEventDestroy(c.ev_sync);
+    c.ev_sync = nullptr;
     for (auto& kv : c.live_blocks) (void)hipFree(kv.first);
     for (auto& kv : c.free_blocks) (void)hipFree(kv.second);
     c.live_blocks.clear();
@@ -110,8 +113,14 @@ hipEvent_t Ctx::pool_event() {
     return ev_pool[ev_used++];
 }
 
+// Waits for the stream by polling an event: a blocking wait may put the thread to sleep
+// and add its wake-up latency to every transcript round trip.
 void Ctx::sync() {
-    SR_HIP(hipStreamSynchronize(stream));
+    SR_HIP(hipEventRecord(ev_sync, stream));
+    hipError_t e;
+    while ((e = hipEventQuery(ev_sync)) == hipErrorNotReady) {
+    }
+    SR_HIP(e);
     ring_off = 0;
 }
 

@@ -85,6 +85,7 @@ struct Ctx {
 
     // events for GPU timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_sync = nullptr;  // sync(): recorded, then polled (no blocking wait)
 
     // Opt-in per-kernel profile: events around each launch on `stream`,
     // aggregated by kernel class with the launch's algorithmic HBM bytes.
@@ -264,6 +265,14 @@ struct GkrInstance {
     uint32_t n_vars;
 };
 // Quotients (row_evaluation.zig).
+// One (column, coefficient) term of the quotient row sums.
+struct QEntryDev {
+    const uint32_t* col;
+    uint32_t shift;  // lift: row -> ((row >> shift) << 1) + (row & 1); 1 = same size
+    uint32_t pad;
+    uint32_t coeff[4];
+};
+static_assert(sizeof(QEntryDev) == 32, "quotient entry layout");
 struct QuotientBatchDev {
     cm31 det, pix, piy;  // sample point components
     qm31 sum_a, sum_b;   // linear term
@@ -275,8 +284,15 @@ void launch_quotients(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_c
                       const uint32_t* h_entry_coeffs, uint32_t n_entries, const QuotientBatchDev* h_batches,
                       uint32_t n_batches, uint32_t log_size, uint32_t* const* out_coords /* host array of 4 */,
                       uint8_t* d_leaf_out);
+// The same with the entries and batches already in device memory (in_bytes: algorithmic
+// bytes read, for the profile only).
+void launch_quotients_dev(Ctx& c, const QEntryDev* d_ent, uint32_t n_entries, const QuotientBatchDev* d_bat,
+                          uint32_t n_batches, double in_bytes, uint32_t log_size, uint32_t* const* out_coords,
+                          uint8_t* d_leaf_out);
 // Point evaluation: sum_i coeff_i * w_i with weights of `point` (2^log).
 void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w);
+// The same with the log_size factors (QM31) in device memory.
+void launch_eval_weights_dev(Ctx& c, const uint32_t* d_factors, uint32_t log_size, uint32_t* d_w);
 void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
                          const uint32_t* d_w, uint32_t* d_out /* n_cols * 4 words */);
 // Wide Fibonacci.

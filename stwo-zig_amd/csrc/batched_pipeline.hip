@@ -158,13 +158,10 @@ void BatchedPipeline::record() {
     launch_fill_u32(c, chan.as<uint32_t>(), CH_BYTES / 4, 0);
     launch_chan_mix_root_draw(c, d_ch, tree->layers, d_roots);
     launch_merkle_leaves_qm31(c, lines[0].as<uint32_t>(), sp.fri_log, ftrees[0]->layer(sp.fri_log));
-    for (uint32_t j = 0; j <= sp.n_folds; j++) {
-        const uint32_t lg = sp.fri_log - j;
-        launch_merkle_parents(c, *ftrees[j], d_ch, d_roots + 32 * (1 + j));
-        if (j == sp.n_folds) break;
-        launch_fold_line(c, lines[j].as<uint32_t>(), lg, qzero(), lines[j + 1].as<uint32_t>(),
-                         ftrees[j + 1]->layer(lg - 1), d_ch);
-    }
+    launch_merkle_parents(c, *ftrees[0], d_ch, d_roots + 32);
+    for (uint32_t j = 1; j <= sp.n_folds; j++)
+        launch_fri_commit_line(c, lines[j - 1].as<uint32_t>(), sp.fri_log - j + 1, lines[j].as<uint32_t>(), *ftrees[j],
+                               d_ch, d_roots + 32 * (1 + j));
 }
 
 void BatchedPipeline::capture() {

@@ -19,7 +19,9 @@
 // launch builds up to 10 levels: each thread hashes four children from HBM
 // into two parents and their parent in registers, then the block halves the
 // level in LDS, writing every level; levels of at most 64 nodes use one quad of
-// lanes per compression.  A 2^21-leaf tree takes three launches instead of 21.
+// lanes per compression.  The last 16 or fewer levels are one tree-layer launch
+// (kernels_fri.hip) whose workgroups reduce up to 2^10 hashes each and whose last
+// workgroup reduces their roots.  A 2^21-leaf tree takes two launches instead of 21.
 #include <algorithm>
 #include <vector>
 
@@ -117,19 +119,12 @@ __device__ __forceinline__ void node_hash(const B2State& seed, const uint32_t* l
     b2_compress(h, m, 128, true);
 }
 
-// When the launch reaches the root (k == levels, one block) and `ch` is set,
-// the thread holding the root also runs the channel step on it.
-// CHAN is a separate instantiation so that the channel code's registers do not
-// lower the occupancy of the wide launches.
-template <bool CHAN>
+// Levels k-1 .. k-levels of a tree: block b reduces its 2^levels hashes of level k.
 __global__ __launch_bounds__(MP_THREADS) void merkle_parents_kernel(uint8_t* __restrict__ tree, uint32_t k,
-                                                                    uint32_t levels, B2State seed,
-                                                                    DevChannel* __restrict__ ch,
-                                                                    uint32_t* __restrict__ root_out) {
+                                                                    uint32_t levels, B2State seed) {
     __shared__ uint32_t sh[MP_THREADS * 8];
     const uint32_t t = threadIdx.x;
     const uint8_t* child = tree + (((uint64_t)1 << k) - 1) * 32;
-    constexpr bool chan_here = CHAN;
     if (levels == 1) {
         uint64_t i = (uint64_t)blockIdx.x * MP_THREADS + t;
         if (i >= (1ull << (k - 1))) return;
@@ -139,7 +134,6 @@ __global__ __launch_bounds__(MP_THREADS) void merkle_parents_kernel(uint8_t* __r
         uint32_t h[8];
         node_hash(seed, l, r, h);
         store_hash(tree + (((uint64_t)1 << (k - 1)) - 1) * 32, i, h);
-        if (chan_here && i == 0) chan_mix_root_draw_thread(ch, h, root_out);
         return;
     }
     const uint32_t T = 1u << (levels - 2);  // threads with work (<= 256)
@@ -166,7 +160,6 @@ __global__ __launch_bounds__(MP_THREADS) void merkle_parents_kernel(uint8_t* __r
         store_hash(tree + (((uint64_t)1 << (k - 2)) - 1) * 32, g, h2);
 #pragma unroll
         for (int i = 0; i < 8; i++) sh[t * 8 + i] = h2[i];
-        if (chan_here && levels == 2 && t == 0) chan_mix_root_draw_thread(ch, h2, root_out);
     }
     uint32_t count = T, level = k - 2;
     uint64_t first = (uint64_t)blockIdx.x * T;
@@ -207,7 +200,6 @@ __global__ __launch_bounds__(MP_THREADS) void merkle_parents_kernel(uint8_t* __r
                 o[q] = hlo;
                 o[4 + q] = hhi;
             }
-            if (chan_here && lv + 1 == levels && j == 0) chan_mix_root_draw_quad(ch, hlo, hhi, root_out);
         }
     }
 }
@@ -312,29 +304,31 @@ void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size
     SR_HIP(hipGetLastError());
 }
 
+// Wide parent levels (throughput-bound) with merkle_parents_kernel while more than
+// TREE_LAYER_MAX_LOG levels remain, then the top levels and the channel step in one
+// tree-layer launch (latency-bound: one quad of lanes per compression on the narrow levels).
 void launch_merkle_parents(Ctx& c, DeviceTree& t, DevChannel* d_ch, uint8_t* d_root_out) {
     uint32_t k = t.max_log;
-    if (d_ch && k == 0) launch_chan_mix_root_draw(c, d_ch, t.layers, d_root_out);
-    while (k > 0) {
-        const uint32_t lmax = c.merkle_levels;  // 2..10 (tuning parameter)
-        uint32_t levels = k < lmax ? k : lmax;
-        uint32_t blocks = levels == 1 ? (uint32_t)(((1ull << (k - 1)) + MP_THREADS - 1) / MP_THREADS)
-                                      : (1u << (k - levels));
+    if (k == 0) {
+        if (d_ch) launch_chan_mix_root_draw(c, d_ch, t.layers, d_root_out);
+        else if (d_root_out) SR_HIP(hipMemcpyAsync(d_root_out, t.layers, 32, hipMemcpyDeviceToDevice, c.stream));
+        return;
+    }
+    while (k > TREE_LAYER_MAX_LOG) {
+        // at least 8 levels stay for the top launch (up to 2^8 hashes reduce in one workgroup)
+        const uint32_t levels = std::min(c.merkle_levels, k - 8);
+        const uint32_t blocks = levels == 1 ? (uint32_t)(((1ull << (k - 1)) + MP_THREADS - 1) / MP_THREADS)
+                                            : (1u << (k - levels));
         // levels >= 2: 2^(levels-2) threads carry work; at least one wave per block
         const uint32_t threads = levels == 1 ? MP_THREADS : std::max(64u, 1u << (levels - 2));
-        KTimer kt(c,
-                  "merkle_parents L=" + std::to_string(k) + " levels=" + std::to_string(levels) +
-                      (d_ch && levels == k ? " +channel" : ""),
+        KTimer kt(c, "merkle_parents L=" + std::to_string(k) + " levels=" + std::to_string(levels),
                   32.0 * (double)((1ull << k) + (1ull << k) - (1ull << (k - levels))));
-        if (d_ch && levels == k)
-            hipLaunchKernelGGL(merkle_parents_kernel<true>, dim3(blocks), dim3(threads), 0, c.stream, t.layers, k,
-                               levels, c.node_seed, d_ch, (uint32_t*)d_root_out);
-        else
-            hipLaunchKernelGGL(merkle_parents_kernel<false>, dim3(blocks), dim3(threads), 0, c.stream, t.layers, k,
-                               levels, c.node_seed, nullptr, nullptr);
+        hipLaunchKernelGGL(merkle_parents_kernel, dim3(blocks), dim3(threads), 0, c.stream, t.layers, k, levels,
+                           c.node_seed);
         SR_HIP(hipGetLastError());
         k -= levels;
     }
+    launch_tree_top(c, t, k, d_ch, d_root_out);
 }
 
 static DeviceTree* new_tree(Ctx& c, uint32_t max_log) {

@@ -434,47 +434,50 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     }
     mark();  // quotients + first layer commit
 
-    std::vector<DBuf> lines;        // inner layer evaluations (QM31 AoS)
+    std::vector<DBuf> lines;          // inner layer evaluations (QM31 AoS), log L-1 first
     std::vector<DeviceTree*> ftrees;  // inner layer trees
-    uint32_t llog = L - 1;
-    lines.emplace_back(&c, (1ull << llog) * 16);
-    DeviceTree* next_tree = llog > last_log ? owner.make(llog) : nullptr;
-    {
-        const uint32_t* cc[4] = {qc[0], qc[1], qc[2], qc[3]};
-        launch_fold_circle_into_line(c, cc, L, qzero(), lines.back().as<uint32_t>(), false,
-                                     next_tree ? next_tree->layer(llog) : nullptr, d_ch);
-    }
-    while (llog > last_log) {
-        if (llog <= FRI_TAIL_MAX_LOG) {
-            // Remaining layers in one workgroup.
+    const uint32_t* cc[4] = {qc[0], qc[1], qc[2], qc[3]};
+    if (L - 1 == last_log) {
+        lines.emplace_back(&c, (1ull << last_log) * 16);
+        launch_fold_circle_into_line(c, cc, L, qzero(), lines.back().as<uint32_t>(), false, nullptr, d_ch);
+    } else {
+        // layers L-1 .. tail_top: one launch each (fold, commit, channel step); below, the
+        // tail kernel, or a plain fold when the last layer is larger than the tail
+        const uint32_t tail_top = std::max(last_log + 1, std::min(L - 1, FRI_TAIL_MAX_LOG + 1));
+        for (uint32_t l = L - 1; l >= tail_top; l--) {
+            lines.emplace_back(&c, (1ull << l) * 16);
+            DeviceTree* t = owner.make(l);
+            uint8_t* root_out = d_roots + 32 * (1 + ftrees.size());
+            if (l == L - 1)
+                launch_fri_commit_circle(c, cc, L, lines.back().as<uint32_t>(), *t, d_ch, root_out);
+            else
+                launch_fri_commit_line(c, lines[lines.size() - 2].as<uint32_t>(), l + 1, lines.back().as<uint32_t>(),
+                                       *t, d_ch, root_out);
+            ftrees.push_back(t);
+        }
+        if (tail_top <= FRI_TAIL_MAX_LOG + 1) {
             FriTailArgs ta{};
-            ta.top_log = llog;
+            ta.top_log = tail_top;
             ta.last_log = last_log;
             ta.ch = d_ch;
-            for (uint32_t l = llog;; l--) {
-                if (l < llog) lines.emplace_back(&c, (1ull << l) * 16);
+            ta.lines[tail_top] = lines.back().as<uint4>();
+            for (uint32_t l = tail_top - 1;; l--) {
+                lines.emplace_back(&c, (1ull << l) * 16);
                 ta.lines[l] = lines.back().as<uint4>();
                 if (l == last_log) break;
-                DeviceTree* t = l == llog ? next_tree : owner.make(l);
+                DeviceTree* t = owner.make(l);
                 ta.trees[l] = t->layers;
                 ta.roots[l] = (uint32_t*)(d_roots + 32 * (1 + ftrees.size()));
                 ftrees.push_back(t);
             }
             launch_fri_tail(c, ta);
-            llog = last_log;
-            break;
+        } else {
+            lines.emplace_back(&c, (1ull << last_log) * 16);
+            launch_fold_line(c, lines[lines.size() - 2].as<uint32_t>(), last_log + 1, qzero(),
+                             lines.back().as<uint32_t>(), nullptr, d_ch);
         }
-        DeviceTree* t = next_tree;
-        launch_merkle_parents(c, *t, d_ch, d_roots + 32 * (1 + ftrees.size()));
-        ftrees.push_back(t);
-        uint32_t nlog = llog - 1;
-        next_tree = nlog > last_log ? owner.make(nlog) : nullptr;
-        DBuf nxt(&c, (1ull << nlog) * 16);
-        launch_fold_line(c, lines.back().as<uint32_t>(), llog, qzero(), nxt.as<uint32_t>(),
-                         next_tree ? next_tree->layer(nlog) : nullptr, d_ch);
-        lines.push_back(std::move(nxt));
-        llog = nlog;
     }
+    const uint32_t llog = last_log;
     SR_CHECK(ftrees.size() == n_inner, "FRI layer count");
     // Last layer (fri.zig:594-621; line_evaluation.zig:97-151; line.zig:134-142).
     const uint64_t lsz = 1ull << llog;

@@ -73,6 +73,10 @@ void ctx_init(Ctx& c, int device, uint64_t hbm_budget) {
     c.leaf_seed = b2_prefix_seed("leaf");
     c.node_seed = b2_prefix_seed("node");
     c.gen = make_gen_doubles();
+    const size_t cnt_bytes = 4ull * TREE_CNT_STRIDE * (TREE_CNT_GROUPS + 1);
+    c.d_tree_cnt = (uint32_t*)c.alloc(cnt_bytes);
+    SR_HIP(hipMemsetAsync(c.d_tree_cnt, 0, cnt_bytes, c.stream));
+    SR_HIP(hipStreamSynchronize(c.stream));
 }
 
 void ctx_release(Ctx& c) {
@@ -93,6 +97,7 @@ void ctx_release(Ctx& c) {
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.d_tw = c.d_itw = c.d_tw2 = c.d_itw2 = nullptr;
+    c.d_tree_cnt = nullptr;
     c.tower_log = 0;
 }
 

@@ -71,6 +71,8 @@ struct Ctx {
     // 2t so that b*t mod p reduces with one shift (mmul_t2).
     uint32_t* d_tw2 = nullptr;
     uint32_t* d_itw2 = nullptr;
+    // Completion counters of the tree-layer launches (zero between launches).
+    uint32_t* d_tree_cnt = nullptr;
     // Domain tables per circle log: bit-reversed domain points and inverse y of the half coset.
     std::map<uint32_t, uint32_t*> domain_xy;     // 2 * 2^log words: x[] then y[]
     std::map<uint32_t, uint32_t*> circle_inv_y;  // 2^(log-1) words
@@ -223,15 +225,44 @@ void launch_fold_circle_into_line(Ctx& c, const uint32_t* const* coords, uint32_
                                   const DevChannel* d_alpha = nullptr);
 // digest <- H(digest || root); alpha <- drawSecureFelt; root copied to d_root_out (optional).
 void launch_chan_mix_root_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, uint8_t* d_root_out);
-// FRI inner layers log top_log .. last_log+1 in one workgroup.  In: line top_log
-// and its tree's leaf layer.  Out: every parent layer, root (roots[l]), line l-1
-// and its leaves for each layer l, the device channel advanced per layer.
+// One tree reduction per launch from level k0 (tree_layer_kernel, kernels_fri.hip), for
+// k0 <= TREE_LAYER_MAX_LOG.  Larger trees (and FRI layers) reduce their wide levels with
+// merkle_parents launches first.  The launches share the context's completion counters,
+// which every launch leaves at zero.
+constexpr uint32_t TREE_LAYER_MAX_LOG = 16;
+constexpr uint32_t TREE_CNT_GROUPS = 16;  // completion sub-counters, then the top counter
+constexpr uint32_t TREE_CNT_STRIDE = 32;  // words between counters (a cache line each)
+enum FriSrc { FRI_SRC_HASHES = 0, FRI_SRC_LINE = 1, FRI_SRC_CIRCLE = 2 };
+struct FriLayerArgs {
+    const uint4* src_line;      // FRI_SRC_LINE: the previous line (2^(k0+1) QM31 values)
+    const uint32_t* coord[4];   // FRI_SRC_CIRCLE: the 4 coordinate columns (2^(k0+1) words each)
+    const uint32_t* inv;        // inverse x (line) or y (circle) of the fold, 2^k0 words
+    uint4* dst_line;            // the folded line (2^k0 values)
+    uint8_t* tree;              // DeviceTree::layers
+    uint32_t k0, s;             // level of the first hashes; log2 hashes per workgroup
+    DevChannel* ch;             // channel step on the root (required by the folds)
+    uint32_t* root_out;         // root copy (optional)
+    uint32_t* cnt;              // completion counters (set by the launcher)
+    B2State leaf_seed, node_seed;  // set by the launcher
+};
+// Levels k0-1 .. 0 of `t` from its hashes at level k0; with d_ch, the channel step on the root.
+void launch_tree_top(Ctx& c, DeviceTree& t, uint32_t k0, DevChannel* d_ch, uint8_t* d_root_out);
+// One FRI inner layer: fold the line of log_src (or the 4 circle coordinates of log_src) with
+// the alpha in d_ch into d_dst, commit it into `t` (max_log = log_src - 1) and run the channel
+// step on its root (root copied to d_root_out).
+void launch_fri_commit_line(Ctx& c, const uint32_t* d_src, uint32_t log_src, uint32_t* d_dst, DeviceTree& t,
+                            DevChannel* d_ch, uint8_t* d_root_out);
+void launch_fri_commit_circle(Ctx& c, const uint32_t* const* coords, uint32_t log_src, uint32_t* d_dst,
+                              DeviceTree& t, DevChannel* d_ch, uint8_t* d_root_out);
+// FRI inner layers below the committed line top_log (<= FRI_TAIL_MAX_LOG + 1, its alpha in
+// ch) in one workgroup.  Out: lines top_log-1 .. last_log; for each log l in
+// (last_log, top_log) the tree (every layer), its root copy (roots[l]) and the channel step.
 constexpr uint32_t FRI_TAIL_MAX_LOG = 10;
 struct FriTailArgs {
     uint32_t top_log, last_log;
-    uint8_t* trees[FRI_TAIL_MAX_LOG + 1];  // DeviceTree::layers per log (last_log+1 .. top_log)
-    uint4* lines[FRI_TAIL_MAX_LOG + 1];    // line evaluations per log (last_log .. top_log)
-    uint32_t* roots[FRI_TAIL_MAX_LOG + 1]; // root copies per log
+    uint8_t* trees[FRI_TAIL_MAX_LOG + 2];  // DeviceTree::layers per log (last_log+1 .. top_log-1)
+    uint4* lines[FRI_TAIL_MAX_LOG + 2];    // line evaluations per log (last_log .. top_log)
+    uint32_t* roots[FRI_TAIL_MAX_LOG + 2]; // root copies per log
     DevChannel* ch;
     const uint32_t* itw_half;  // inverse tower + 2^(M-1) (set by the launcher)
     B2State leaf_seed, node_seed;  // set by the launcher

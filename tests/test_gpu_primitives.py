@@ -120,6 +120,26 @@ def test_merkle_staged_sizes_shared_prefixes(ctx, name):
         b.free()
 
 
+@pytest.mark.parametrize("log", [9, 13, 18])
+def test_merkle_every_level_matches_oracle(ctx, log):
+    """Every level of the tree (the wide parent launches and the one-launch top reduction with
+    its last-workgroup hand-off) at its first, last and two inner positions."""
+    rng = np.random.default_rng(log)
+    cols = rand_cols(rng, 3, log)
+    bufs = [ctx.to_device(c) for c in cols]
+    _say("merkle levels", log)
+    tree = ctx.commit_merkle([b.ptr for b in bufs], [log] * 3)
+    root, layers = oracle.merkle_commit(cols)
+    assert tree.root == root
+    for lay in range(log + 1):
+        n = 1 << lay
+        idx = sorted({0, n - 1, n // 3, (2 * n) // 3})
+        assert b"".join(tree.read_hashes(lay, idx)) == b"".join(layers[lay][32 * i: 32 * i + 32] for i in idx), lay
+    tree.free()
+    for b in bufs:
+        b.free()
+
+
 @pytest.mark.parametrize("n_cols,log", [(0, 0), (1, 1), (1, 2), (3, 3), (5, 6), (16, 8), (17, 8), (2, 10), (2, 11),
                                         (100, 14), (8, 21)])
 def test_merkle_uniform_matches_oracle(ctx, n_cols, log):

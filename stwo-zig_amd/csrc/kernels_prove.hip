@@ -318,12 +318,16 @@ void launch_eval_weights_dev(Ctx& c, const uint32_t* d_factors, uint32_t log_siz
     SR_HIP(hipGetLastError());
 }
 
-// Grid (chunks, column groups of EV_CB): each block loads a weight once for EV_CB columns,
-// accumulates raw 62-bit products four elements at a time (< 2^64) before one Mersenne
-// fold, and stores its 4 * EV_CB partial sums in its own slot; a second kernel adds the
-// slots (integer sums are exact and order independent).
+// One block per (chunk, column group of EV_CB): each block loads a weight once for EV_CB
+// columns, accumulates raw 62-bit products four elements at a time (< 2^64) before one
+// Mersenne fold, and stores its 4 * EV_CB partial sums in its own slot; a second kernel adds
+// the slots (integer sums are exact and order independent).  Workgroups are dealt to the
+// 8 XCDs round-robin, so block b runs on XCD b % 8: the blocks b, b + 8, ... of one XCD
+// take all column groups of one chunk in a row, and the chunk's weights are read from HBM
+// once and then served by that XCD's L2.
 constexpr uint32_t EV_PER_THREAD = 16;
 constexpr uint32_t EV_CB = 8;
+constexpr uint32_t EV_XCDS = 8;
 __device__ __forceinline__ uint64_t ev_wave_sum(uint64_t v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
@@ -334,10 +338,14 @@ __device__ __forceinline__ uint64_t ev_wave_sum(uint64_t v) {
 }
 __global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
                                                            uint32_t log_size, const uint4* __restrict__ w,
+                                                           uint32_t chunks, uint32_t groups,
                                                            unsigned long long* __restrict__ part) {
     __shared__ unsigned long long red[4][EV_CB * 4];
     const uint32_t t = threadIdx.x;
-    const uint32_t c0 = blockIdx.y * EV_CB;
+    const uint32_t k = blockIdx.x / EV_XCDS;
+    const uint32_t chunk = (k / groups) * EV_XCDS + blockIdx.x % EV_XCDS;
+    if (chunk >= chunks) return;
+    const uint32_t c0 = (k % groups) * EV_CB;
     const uint32_t nc = n_cols - c0 < EV_CB ? n_cols - c0 : EV_CB;
     const uint32_t n = 1u << log_size;
     uint64_t s[EV_CB][4];
@@ -348,15 +356,21 @@ __global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const
         // and 64 contiguous bytes of weights (the block covers 256 * EV_PER_THREAD elements)
 #pragma unroll
         for (uint32_t g = 0; g < EV_PER_THREAD / 4; g++) {
-            const uint32_t i0 = blockIdx.x * (256 * EV_PER_THREAD) + g * 1024 + 4 * t;
+            const uint32_t i0 = chunk * (256 * EV_PER_THREAD) + g * 1024 + 4 * t;
             if (i0 >= n) break;
             uint4 ww[4];
 #pragma unroll
             for (int e = 0; e < 4; e++) ww[e] = w[i0 + e];
+            // every column load is issued before the first is used (a short last group
+            // re-reads its last column instead of branching around the loads)
+            u32x4 cv[EV_CB];
+#pragma unroll
+            for (uint32_t c = 0; c < EV_CB; c++)
+                cv[c] = *(const SR_GLOBAL u32x4*)((gcu32*)cols[c0 + (c < nc ? c : nc - 1)] + i0);
 #pragma unroll
             for (uint32_t c = 0; c < EV_CB; c++) {
                 if (c < nc) {
-                    const u32x4 v4 = *(const SR_GLOBAL u32x4*)((gcu32*)cols[c0 + c] + i0);
+                    const u32x4 v4 = cv[c];
                     const uint64_t v[4] = {v4.x, v4.y, v4.z, v4.w};
                     uint64_t r0 = 0, r1 = 0, r2 = 0, r3 = 0;
 #pragma unroll
@@ -396,7 +410,7 @@ __global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const
     __syncthreads();
     if (t < 4 * nc) {
         const uint64_t x = red[0][t] + red[1][t] + red[2][t] + red[3][t];
-        part[(uint64_t)(4 * c0 + t) * gridDim.x + blockIdx.x] = x;
+        part[(uint64_t)(4 * c0 + t) * chunks + chunk] = x;
     }
 }
 // out[k] = sum over chunks of part[k][chunk] mod p: one wave per output word.
@@ -418,8 +432,9 @@ void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols,
     const uint32_t nw = n_cols * 4;
     DBuf part(&c, 8ull * nw * chunks);
     KTimer kt(c, "eval_columns", (4.0 * n_cols + 16.0) * n);
-    hipLaunchKernelGGL(eval_columns_kernel, dim3(chunks, groups), dim3(256), 0, c.stream, d_cols, n_cols, log_size,
-                       (const uint4*)d_w, part.as<unsigned long long>());
+    const uint32_t blocks = (chunks + EV_XCDS - 1) / EV_XCDS * EV_XCDS * groups;
+    hipLaunchKernelGGL(eval_columns_kernel, dim3(blocks), dim3(256), 0, c.stream, d_cols, n_cols, log_size,
+                       (const uint4*)d_w, chunks, groups, part.as<unsigned long long>());
     hipLaunchKernelGGL(eval_reduce_kernel, dim3((nw + 3) / 4), dim3(256), 0, c.stream,
                        (const unsigned long long*)part.as<unsigned long long>(), chunks, nw, d_out);
     SR_HIP(hipGetLastError());

@@ -219,10 +219,12 @@ def test_fold_circle_into_line_matches_oracle(ctx, log_src):
     assert np.array_equal(dd.download(), exp)
 
 
-@pytest.mark.parametrize("log", [0, 1, 5, 14, 20])
-def test_eval_at_point_matches_oracle(ctx, log):
+@pytest.mark.parametrize("log,n_cols", [(0, 3), (1, 3), (5, 3), (14, 3), (20, 3), (12, 20), (16, 9)])
+def test_eval_at_point_matches_oracle(ctx, log, n_cols):
+    """Several column groups per chunk (the blocks of one chunk share an XCD) and chunk counts
+    that are not a multiple of the XCD count."""
     rng = np.random.default_rng(500 + log)
-    cols = rand_cols(rng, 3, log)
+    cols = rand_cols(rng, n_cols, log)
     pt = [int(v) for v in rng.integers(0, mr.P, 8)]
     bufs = [ctx.to_device(c) for c in cols]
     got = ctx.eval_at_point([b.ptr for b in bufs], log, pt)

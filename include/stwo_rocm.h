@@ -182,7 +182,8 @@ typedef struct {
  * d_trace: optional resident trace (sequence_len columns of 2^log_n_rows, column-major);
  * NULL generates it on the device inside the call.  Writes the canonical proof bytes
  * (compact JSON wire, src/interop/proof_wire.zig:15-76) to proof_out; when cap is too
- * small returns STWO_ROCM_EINVAL with *proof_len set to the required size. */
+ * small returns STWO_ROCM_EINVAL with *proof_len set to the required size.  stats is optional:
+ * NULL skips the per-stage timing events (each costs a few microseconds of GPU time). */
 int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params,
                                    const uint32_t* d_trace, uint8_t* proof_out, uint64_t cap, uint64_t* proof_len,
                                    stwo_rocm_prove_stats* stats);

@@ -558,7 +558,7 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
         SR_CHECK(p.trace_mode <= 1, "trace_mode must be 0 or 1");
         std::string proof;
         ProveStats st;
-        prove_wide_fibonacci(c, p, d_trace, proof, &st);
+        prove_wide_fibonacci(c, p, d_trace, proof, stats ? &st : nullptr);
         *proof_len = proof.size();
         if (stats) {
             for (int i = 0; i < STAGES; i++) {

@@ -229,6 +229,64 @@ void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out, const std::ve
     out.tree = owner.add(merkle_commit(c, cols, out.lde_logs));
 }
 
+// Host-side image of the small device tables a proof needs (column pointer lists, leaf plans,
+// quotient entries, the initial channel): built before the first launch and uploaded with
+// one copy, each table at a 16-byte aligned offset.
+struct HostPack {
+    std::vector<uint8_t> buf;
+    template <typename T>
+    size_t add(const T* p, size_t n) {
+        const size_t at = (buf.size() + 15) & ~(size_t)15;
+        buf.resize(at + n * sizeof(T));
+        if (n) memcpy(buf.data() + at, p, n * sizeof(T));
+        return at;
+    }
+    size_t reserve(size_t bytes) {
+        const size_t at = (buf.size() + 15) & ~(size_t)15;
+        buf.resize(at + bytes, 0);
+        return at;
+    }
+};
+
+// commit_coefficients in two halves around the one upload: the plan allocates the LDE block
+// and the tree and records the pointer tables and the leaf plan; the launch runs the LDE and
+// the Merkle commitment from the uploaded copies.
+struct CommitPlan {
+    uint32_t n = 0, m = 0, coeff_log = 0;
+    size_t at_src = 0, at_dst = 0, at_tab = 0;
+    std::vector<LeafSeg> segs;
+};
+
+static void plan_commit(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
+                        uint32_t coeff_log, uint32_t blowup, HostPack& pack, CommitPlan& cp) {
+    cp.n = (uint32_t)coeffs.size();
+    cp.m = coeff_log + blowup;
+    cp.coeff_log = coeff_log;
+    SR_CHECK(cp.n >= 1, "commit: no columns");
+    const uint64_t N = 1ull << cp.m;
+    out.coeffs = coeffs;
+    out.coeff_logs.assign(cp.n, coeff_log);
+    out.lde_logs.assign(cp.n, cp.m);
+    out.lde.resize(cp.n);
+    out.lde_buf = DBuf(&c, (size_t)cp.n * N * 4);
+    for (uint32_t i = 0; i < cp.n; i++) out.lde[i] = out.lde_buf.as<uint32_t>() + i * N;
+    cp.at_src = pack.add(coeffs.data(), cp.n);
+    cp.at_dst = pack.add(out.lde.data(), cp.n);
+    // equal log sizes: the stable sort by log size (columns.zig:12-46) keeps the column order
+    std::vector<const uint32_t*> cols(out.lde.begin(), out.lde.end());
+    LeafPlan plan = merkle_leaf_plan(cols.data(), out.lde_logs.data(), cp.n);
+    cp.at_tab = pack.add(plan.tab.data(), plan.tab.size());
+    cp.segs = plan.segs;
+    out.tree = owner.make(cp.m);
+}
+
+static void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack, uint8_t* d_root_out) {
+    launch_circle_fft(c, false, (const uint32_t* const*)(d_pack + cp.at_src), (uint32_t* const*)(d_pack + cp.at_dst),
+                      cp.n, cp.m, cp.coeff_log);
+    launch_merkle_leaves_dev(c, (const LeafColDev*)(d_pack + cp.at_tab), cp.segs, cp.n, *out.tree);
+    launch_merkle_parents(c, *out.tree, nullptr, d_root_out);
+}
+
 void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trace_in, std::string& proof,
                           ProveStats* stats) {
     const uint32_t n = prm.log_n_rows, W = prm.sequence_len, B = prm.log_blowup_factor;
@@ -245,7 +303,10 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
 
     std::vector<hipEvent_t> evs;
     std::vector<double> hms;
+    // stage events only when stage timings are asked for: each event on the stream costs a
+    // few microseconds of GPU time
     auto mark = [&]() {
+        if (!stats) return;
         hipEvent_t e;
         SR_HIP(hipEventCreate(&e));
         SR_HIP(hipEventRecord(e, c.stream));
@@ -260,9 +321,10 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     qm31 cfg = {{prm.pow_bits, B}, {prm.n_queries, prm.log_last_layer_degree_bound}};
     ch.mix_felts(&cfg, 1);
 
-    // Tree 0: preprocessed, empty (examples/wide_fibonacci/trace.zig:184-186).
+    // Tree 0: preprocessed, empty (examples/wide_fibonacci/trace.zig:184-186): a single leaf
+    // Blake2s(prefix) (leaves.zig:30-34), of which only the root is used.
     Committed t0;
-    commit_coefficients(c, owner, t0, {}, 1, B);
+    t0.tree = owner.make(0);
     {
         uint8_t prefix[64] = {'l', 'e', 'a', 'f'};
         b2_hash(prefix, 64, t0.tree->root);
@@ -272,56 +334,131 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     // From here to the FRI readback the transcript runs on a device copy of the channel:
     // the launches that need a drawn value read it from HBM, and the host reads the
     // commitments, sampled values and channel back once (transcript.hip).
+    // Block `blk`: the readback region [DevChannel | roots of the first FRI layer and the inner
+    // layers | last FRI line | roots of trees 1 and 2 | sampled values], the OODS point and
+    // evaluation factors, then the tables of every launch up to the quotients; the host image
+    // of all of it goes up in one copy before the first launch.
     const uint32_t last_log = prm.log_last_layer_degree_bound + B;
     const uint32_t n_inner = L - 1 - last_log;  // inner FRI layers log L-1 .. last_log+1
+    const uint32_t NS = W + 8;                  // sampled columns: trace (W) and composition (8)
+    const uint64_t lsz = 1ull << last_log;
     const size_t fbytes = CH_BYTES + 32 * (1 + (size_t)n_inner);
-    DBuf fst(&c, fbytes);  // DevChannel | roots (first FRI layer, inner layers)
-    DevChannel* d_ch = fst.as<DevChannel>();
-    uint8_t* d_roots = fst.as<uint8_t>() + CH_BYTES;
+    const size_t at_line = fbytes, at_troots = at_line + 16 * lsz, at_sv = at_troots + 64;
+    const size_t rb_bytes = at_sv + 16 * (size_t)NS;
+    HostPack pack;
     {
         DevChannel hc{};
         memcpy(hc.digest, ch.digest, 32);
         hc.n_draws = ch.n_draws;
-        SR_HIP(hipMemcpyAsync(d_ch, c.stage(&hc, sizeof(hc)), sizeof(hc), hipMemcpyHostToDevice, c.stream));
+        pack.add(&hc, 1);  // offset 0
+        pack.buf.resize(rb_bytes, 0);
     }
-    const uint32_t NS = W + 8;  // sampled columns: trace (W) and composition (8)
-    DBuf trs(&c, 16 * (2 + 32 + (size_t)NS));  // OODS point | evaluation factors | sampled values
-    uint32_t* d_oods = trs.as<uint32_t>();
-    uint32_t* d_factors = d_oods + 4 * 2;
-    uint32_t* d_sv = d_factors + 4 * 32;
+    const size_t at_trs = pack.reserve(16 * (2 + 32));  // OODS point | evaluation factors
 
-    // Tree 1: main trace -> coefficients -> LDE -> Merkle.
+    // Buffers of the main trace and composition phases and their tables.
     DBuf trace_own;
     const uint32_t* d_trace = d_trace_in;
-    if (!d_trace) {
-        trace_own = DBuf(&c, (size_t)W * N * 4);
-        launch_widefib_trace(c, n, W, prm.trace_mode, prm.trace_seed, trace_own.as<uint32_t>(), N);
-        d_trace = trace_own.as<uint32_t>();
-    }
+    if (!d_trace) trace_own = DBuf(&c, (size_t)W * N * 4);
+    if (!d_trace) d_trace = trace_own.as<uint32_t>();
     DBuf coeff1(&c, (size_t)W * N * 4);
+    std::vector<const uint32_t*> cf(W);
+    size_t at_ifft_src, at_ifft_dst;
     {
         std::vector<const uint32_t*> src(W);
-        std::vector<uint32_t*> dst(W);
         for (uint32_t i = 0; i < W; i++) {
             src[i] = d_trace + i * N;
-            dst[i] = coeff1.as<uint32_t>() + i * N;
+            cf[i] = coeff1.as<uint32_t>() + i * N;
         }
-        DBuf d_src = upload(c, src.data(), W), d_dst = upload(c, dst.data(), W);
-        launch_circle_fft(c, true, d_src.as<const uint32_t* const>(), d_dst.as<uint32_t* const>(), W, n, n);
+        at_ifft_src = pack.add(src.data(), W);
+        at_ifft_dst = pack.add(cf.data(), W);
     }
     Committed t1;
-    {
-        std::vector<const uint32_t*> cf(W);
-        for (uint32_t i = 0; i < W; i++) cf[i] = coeff1.as<uint32_t>() + i * N;
-        commit_coefficients(c, owner, t1, cf, n, B);
+    CommitPlan cp1;
+    plan_commit(c, owner, t1, cf, n, B, pack, cp1);
+
+    const uint32_t NC = W - 2, CL = n + 1;
+    const uint64_t CN = 1ull << CL;
+    DBuf ev_own;
+    std::vector<const uint32_t*> ev(W);
+    size_t at_ev_src = 0, at_ev_dst = 0;
+    if (B == 1) {
+        for (uint32_t i = 0; i < W; i++) ev[i] = t1.lde[i];
+    } else {
+        // trace polynomials evaluated on CanonicCoset(n + 1) from their coefficients
+        ev_own = DBuf(&c, (size_t)W * CN * 4);
+        std::vector<uint32_t*> dst(W);
+        for (uint32_t i = 0; i < W; i++) {
+            dst[i] = ev_own.as<uint32_t>() + i * CN;
+            ev[i] = dst[i];
+        }
+        at_ev_src = pack.add(cf.data(), W);
+        at_ev_dst = pack.add(dst.data(), W);
     }
+    const size_t at_ev = pack.add(ev.data(), W);
+    DBuf comp(&c, 4 * CN * 4);
+    uint32_t* comp_c[4];
+    for (int k = 0; k < 4; k++) comp_c[k] = comp.as<uint32_t>() + k * CN;
+    const size_t at_cc = pack.add(comp_c, 4);
+    // splitAtMid -> [left0..3, right0..3] (poly.zig:302-318; prove.zig:423-445)
+    Committed t2;
+    CommitPlan cp2;
+    {
+        std::vector<const uint32_t*> cp(8);
+        for (int k = 0; k < 4; k++) {
+            cp[k] = comp_c[k];
+            cp[4 + k] = comp_c[k] + N;
+        }
+        plan_commit(c, owner, t2, cp, n, B, pack, cp2);
+    }
+    // Sampled values and quotients (sampled_values.zig; samples.zig:245-312): every column
+    // shares log n coefficients and the lifting log.
+    const Committed* trees[3] = {&t0, &t1, &t2};
+    std::vector<const uint32_t*> all_coeffs;
+    std::vector<uint32_t*> all_lde;
+    std::vector<uint32_t> all_lde_logs;
+    for (int tr = 1; tr < 3; tr++)
+        for (size_t i = 0; i < trees[tr]->coeffs.size(); i++) {
+            all_coeffs.push_back(trees[tr]->coeffs[i]);
+            all_lde.push_back(trees[tr]->lde[i]);
+            all_lde_logs.push_back(trees[tr]->lde_logs[i]);
+        }
+    SR_CHECK(all_coeffs.size() == NS, "sampled column count");
+    const size_t at_cols = pack.add(all_coeffs.data(), NS);
+    double q_in_bytes = 0;
+    size_t at_ents;
+    {
+        // the coefficients are written on the device by the quotient transcript step
+        std::vector<QEntryDev> ents(NS);
+        for (uint32_t i = 0; i < NS; i++) {
+            SR_CHECK(all_lde_logs[i] >= 1 && all_lde_logs[i] <= L, "quotients: column larger than the domain");
+            ents[i].col = all_lde[i];
+            ents[i].shift = L - all_lde_logs[i] + 1;
+            ents[i].pad = 0;
+            for (int k = 0; k < 4; k++) ents[i].coeff[k] = 0;
+            q_in_bytes += 4.0 * (double)(1ull << all_lde_logs[i]);
+        }
+        at_ents = pack.add(ents.data(), NS);
+    }
+    DBuf blk = upload(c, pack.buf.data(), pack.buf.size());
+    uint8_t* d_pack = blk.as<uint8_t>();
+    DevChannel* d_ch = (DevChannel*)d_pack;
+    uint8_t* d_roots = d_pack + CH_BYTES;
+    uint32_t* d_last_line = (uint32_t*)(d_pack + at_line);
+    uint8_t* d_troots = d_pack + at_troots;
+    uint32_t* d_sv = (uint32_t*)(d_pack + at_sv);
+    uint32_t* d_oods = (uint32_t*)(d_pack + at_trs);
+    uint32_t* d_factors = d_oods + 4 * 2;
+
+    // Tree 1: main trace -> coefficients -> LDE -> Merkle.
+    if (!d_trace_in) launch_widefib_trace(c, n, W, prm.trace_mode, prm.trace_seed, trace_own.as<uint32_t>(), N);
+    launch_circle_fft(c, true, (const uint32_t* const*)(d_pack + at_ifft_src), (uint32_t* const*)(d_pack + at_ifft_dst),
+                      W, n, n);
+    launch_commit(c, t1, cp1, d_pack, d_troots);
     trace_own.reset();
     mark();  // main commit
 
     // mixRoot(main), mixStatement (wide_fibonacci.zig:496-498), composition alpha and its
     // powers (prove.zig:237-318; wide_fibonacci/component.zig:146-228), on the device.
-    const uint32_t NC = W - 2, CL = n + 1;
-    const uint64_t CN = 1ull << CL;
     DBuf d_pw(&c, 16ull * NC);
     {
         const uint32_t st[2] = {n, W};
@@ -338,68 +475,27 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         for (uint32_t i = 1; i < n; i++) x = mdouble_x(x);
         den_inv[r] = minv(x);
     }
-    DBuf ev_own;
-    std::vector<const uint32_t*> ev(W);
-    if (B == 1) {
-        for (uint32_t i = 0; i < W; i++) ev[i] = t1.lde[i];
-    } else {
-        // Trace polynomials evaluated on CanonicCoset(n + 1) from their coefficients.
-        ev_own = DBuf(&c, (size_t)W * CN * 4);
-        std::vector<uint32_t*> dst(W);
-        for (uint32_t i = 0; i < W; i++) {
-            dst[i] = ev_own.as<uint32_t>() + i * CN;
-            ev[i] = dst[i];
-        }
-        DBuf d_src = upload(c, t1.coeffs.data(), W), d_dst = upload(c, dst.data(), W);
-        launch_circle_fft(c, false, d_src.as<const uint32_t* const>(), d_dst.as<uint32_t* const>(), W, CL, n);
-    }
-    DBuf comp(&c, 4 * CN * 4);
-    uint32_t* comp_c[4];
-    for (int k = 0; k < 4; k++) comp_c[k] = comp.as<uint32_t>() + k * CN;
-    {
-        DBuf d_ev = upload(c, ev.data(), W);
-        launch_widefib_composition(c, d_ev.as<const uint32_t* const>(), W, CL, n, d_pw.as<uint32_t>(), den_inv[0],
-                                   den_inv[1], comp_c);
-        // interpolate (secure_poly.zig:164-215) in place
-        DBuf d_cc = upload(c, comp_c, 4);
-        launch_circle_fft(c, true, d_cc.as<const uint32_t* const>(), d_cc.as<uint32_t* const>(), 4, CL, CL);
-    }
+    if (B != 1)
+        launch_circle_fft(c, false, (const uint32_t* const*)(d_pack + at_ev_src), (uint32_t* const*)(d_pack + at_ev_dst),
+                          W, CL, n);
+    launch_widefib_composition(c, (const uint32_t* const*)(d_pack + at_ev), W, CL, n, d_pw.as<uint32_t>(), den_inv[0],
+                               den_inv[1], comp_c);
+    // interpolate (secure_poly.zig:164-215) in place
+    launch_circle_fft(c, true, (const uint32_t* const*)(d_pack + at_cc), (uint32_t* const*)(d_pack + at_cc), 4, CL, CL);
     ev_own.reset();
-    // splitAtMid -> [left0..3, right0..3] (poly.zig:302-318; prove.zig:423-445)
-    Committed t2;
-    {
-        std::vector<const uint32_t*> cp(8);
-        for (int k = 0; k < 4; k++) {
-            cp[k] = comp_c[k];
-            cp[4 + k] = comp_c[k] + N;
-        }
-        commit_coefficients(c, owner, t2, cp, n, B);
-    }
+    launch_commit(c, t2, cp2, d_pack, d_troots + 32);
     mark();  // composition
 
     // mixRoot(composition), OODS point (circle.zig:159-167) and the point-evaluation
     // factors, on the device.
     launch_tr_oods(c, d_ch, t2.tree->layers, n, d_oods, d_factors);
 
-    // Sampled values (sampled_values.zig): all columns share log n coefficients and
-    // the lifting log, so one weight vector serves every column.
-    const Committed* trees[3] = {&t0, &t1, &t2};
-    std::vector<const uint32_t*> all_coeffs;
-    std::vector<uint32_t*> all_lde;
-    std::vector<uint32_t> all_lde_logs;
-    for (int tr = 1; tr < 3; tr++)
-        for (size_t i = 0; i < trees[tr]->coeffs.size(); i++) {
-            all_coeffs.push_back(trees[tr]->coeffs[i]);
-            all_lde.push_back(trees[tr]->lde[i]);
-            all_lde_logs.push_back(trees[tr]->lde_logs[i]);
-        }
-    SR_CHECK(all_coeffs.size() == NS, "sampled column count");
+    // Sampled values: one weight vector serves every column.
     std::vector<uint32_t> sv(4 * NS);  // read back with the FRI state
     {
         DBuf w(&c, N * 16);
         launch_eval_weights_dev(c, d_factors, n, w.as<uint32_t>());
-        DBuf d_cols = upload(c, all_coeffs.data(), NS);
-        launch_eval_columns(c, d_cols.as<const uint32_t* const>(), NS, n, w.as<uint32_t>(), d_sv);
+        launch_eval_columns(c, (const uint32_t* const*)(d_pack + at_cols), NS, n, w.as<uint32_t>(), d_sv);
     }
     mark();  // sampled values
 
@@ -412,24 +508,13 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     for (int k = 0; k < 4; k++) qc[k] = quot.as<uint32_t>() + k * LN;
     DeviceTree* ftree0 = owner.make(L);
     {
-        std::vector<QEntryDev> ents(NS);
-        double in_bytes = 0;
-        for (uint32_t i = 0; i < NS; i++) {
-            SR_CHECK(all_lde_logs[i] >= 1 && all_lde_logs[i] <= L, "quotients: column larger than the domain");
-            ents[i].col = all_lde[i];
-            ents[i].shift = L - all_lde_logs[i] + 1;
-            ents[i].pad = 0;
-            for (int k = 0; k < 4; k++) ents[i].coeff[k] = 0;
-            in_bytes += 4.0 * (double)(1ull << all_lde_logs[i]);
-        }
-        DBuf d_ents = upload(c, ents.data(), ents.size());
+        QEntryDev* d_ents = (QEntryDev*)(d_pack + at_ents);
         DBuf d_bat(&c, sizeof(QuotientBatchDev));
-        launch_tr_quotient(c, d_ch, d_sv, NS, d_oods, d_ents.as<QEntryDev>(), d_bat.as<QuotientBatchDev>());
+        launch_tr_quotient(c, d_ch, d_sv, NS, d_oods, d_ents, d_bat.as<QuotientBatchDev>());
         // FRI (prover/fri.zig:377-621).  The commit cascade continues on the device channel: the
         // launch that produces each layer's root mixes it and draws the next alpha, which the
         // fold reads from HBM.
-        launch_quotients_dev(c, d_ents.as<QEntryDev>(), NS, d_bat.as<QuotientBatchDev>(), 1, in_bytes, L, qc,
-                             ftree0->layer(L));
+        launch_quotients_dev(c, d_ents, NS, d_bat.as<QuotientBatchDev>(), 1, q_in_bytes, L, qc, ftree0->layer(L));
         launch_merkle_parents(c, *ftree0, d_ch, d_roots);
     }
     mark();  // quotients + first layer commit
@@ -437,9 +522,9 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     std::vector<DBuf> lines;          // inner layer evaluations (QM31 AoS), log L-1 first
     std::vector<DeviceTree*> ftrees;  // inner layer trees
     const uint32_t* cc[4] = {qc[0], qc[1], qc[2], qc[3]};
+    // the last line (log last_log) is written into the readback region
     if (L - 1 == last_log) {
-        lines.emplace_back(&c, (1ull << last_log) * 16);
-        launch_fold_circle_into_line(c, cc, L, qzero(), lines.back().as<uint32_t>(), false, nullptr, d_ch);
+        launch_fold_circle_into_line(c, cc, L, qzero(), d_last_line, false, nullptr, d_ch);
     } else {
         // layers L-1 .. tail_top: one launch each (fold, commit, channel step); below, the
         // tail kernel, or a plain fold when the last layer is larger than the tail
@@ -462,9 +547,12 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
             ta.ch = d_ch;
             ta.lines[tail_top] = lines.back().as<uint4>();
             for (uint32_t l = tail_top - 1;; l--) {
+                if (l == last_log) {
+                    ta.lines[l] = (uint4*)d_last_line;
+                    break;
+                }
                 lines.emplace_back(&c, (1ull << l) * 16);
                 ta.lines[l] = lines.back().as<uint4>();
-                if (l == last_log) break;
                 DeviceTree* t = owner.make(l);
                 ta.trees[l] = t->layers;
                 ta.roots[l] = (uint32_t*)(d_roots + 32 * (1 + ftrees.size()));
@@ -472,28 +560,21 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
             }
             launch_fri_tail(c, ta);
         } else {
-            lines.emplace_back(&c, (1ull << last_log) * 16);
-            launch_fold_line(c, lines[lines.size() - 2].as<uint32_t>(), last_log + 1, qzero(),
-                             lines.back().as<uint32_t>(), nullptr, d_ch);
+            launch_fold_line(c, lines.back().as<uint32_t>(), last_log + 1, qzero(), d_last_line, nullptr, d_ch);
         }
     }
     const uint32_t llog = last_log;
     SR_CHECK(ftrees.size() == n_inner, "FRI layer count");
+    SR_CHECK(lines.size() == n_inner, "FRI line count");
     // Last layer (fri.zig:594-621; line_evaluation.zig:97-151; line.zig:134-142).
-    const uint64_t lsz = 1ull << llog;
     std::vector<qm31> lv(lsz);
     {
         // one readback: channel and FRI roots, last layer, the two commitments, sampled values
-        const size_t at_roots = fbytes + lsz * 16, at_sv = at_roots + 64;
-        uint8_t* h = c.pinned(at_sv + 16 * (size_t)NS);
-        SR_HIP(hipMemcpyAsync(h, fst.p, fbytes, hipMemcpyDeviceToHost, c.stream));
-        SR_HIP(hipMemcpyAsync(h + fbytes, lines.back().p, lsz * 16, hipMemcpyDeviceToHost, c.stream));
-        SR_HIP(hipMemcpyAsync(h + at_roots, t1.tree->layers, 32, hipMemcpyDeviceToHost, c.stream));
-        SR_HIP(hipMemcpyAsync(h + at_roots + 32, t2.tree->layers, 32, hipMemcpyDeviceToHost, c.stream));
-        SR_HIP(hipMemcpyAsync(h + at_sv, d_sv, 16 * (size_t)NS, hipMemcpyDeviceToHost, c.stream));
+        uint8_t* h = c.pinned(rb_bytes);
+        SR_HIP(hipMemcpyAsync(h, d_pack, rb_bytes, hipMemcpyDeviceToHost, c.stream));
         c.sync();
-        memcpy(t1.tree->root, h + at_roots, 32);
-        memcpy(t2.tree->root, h + at_roots + 32, 32);
+        memcpy(t1.tree->root, h + at_troots, 32);
+        memcpy(t2.tree->root, h + at_troots + 32, 32);
         memcpy(sv.data(), h + at_sv, 16 * (size_t)NS);
         DevChannel hc;
         memcpy(&hc, h, sizeof(hc));
@@ -502,7 +583,7 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         ch.n_draws = hc.n_draws;
         memcpy(ftree0->root, h + CH_BYTES, 32);
         for (size_t k = 0; k < ftrees.size(); k++) memcpy(ftrees[k]->root, h + CH_BYTES + 32 * (1 + k), 32);
-        const uint32_t* w = (const uint32_t*)(h + fbytes);
+        const uint32_t* w = (const uint32_t*)(h + at_line);
         for (uint64_t i = 0; i < lsz; i++) lv[i] = qload(w + 4 * i);
     }
     auto bitrev_vec = [](std::vector<qm31>& v) {
@@ -643,11 +724,11 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     }
     std::vector<uint32_t> gw(g.addr.size());
     if (!g.addr.empty()) {
-        DBuf d_addr = upload(c, g.addr.data(), g.addr.size());
-        DBuf d_out(&c, g.addr.size() * 4);
-        launch_gather(c, d_addr.as<uint64_t>(), g.addr.size(), d_out.as<uint32_t>());
+        // the gather reads its address list from pinned host memory and writes the words
+        // there too: no copy operations on the stream
         uint32_t* h = (uint32_t*)c.pinned(g.addr.size() * 4);
-        SR_HIP(hipMemcpyAsync(h, d_out.p, g.addr.size() * 4, hipMemcpyDeviceToHost, c.stream));
+        const uint64_t* h_addr = (const uint64_t*)c.stage(g.addr.data(), g.addr.size() * 8);
+        launch_gather(c, h_addr, g.addr.size(), h);
         c.sync();
         memcpy(gw.data(), h, g.addr.size() * 4);
     }
@@ -741,8 +822,8 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     proof = j.finish();
     mark();  // encode
 
-    SR_HIP(hipEventSynchronize(evs.back()));
     if (stats) {
+        SR_HIP(hipEventSynchronize(evs.back()));
         memset(stats, 0, sizeof(*stats));
         for (size_t i = 1; i < evs.size() && i < 1 + STAGES; i++) {
             float ms = 0;

@@ -417,22 +417,25 @@ class Context:
 
     # ---- whole proof
     def prove_wide_fibonacci(self, log_n_rows, sequence_len, pow_bits=10, log_blowup_factor=1,
-                             log_last_layer_degree_bound=0, n_queries=3, trace_mode=0, trace_seed=0, d_trace=None):
+                             log_last_layer_degree_bound=0, n_queries=3, trace_mode=0, trace_seed=0, d_trace=None,
+                             stats=True):
+        """Returns (proof bytes, stage timings dict or None).  stats=False records no stage
+        events on the stream (each costs a few microseconds of GPU time)."""
         prm = WideFibParams(log_n_rows, sequence_len, pow_bits, log_blowup_factor, log_last_layer_degree_bound,
                             n_queries, trace_mode, trace_seed)
-        stats = ProveStats()
+        st = ProveStats() if stats else None
         cap = 1 << 20
         n = ctypes.c_uint64()
         while True:
             buf = ctypes.create_string_buffer(cap)
             rc = self.lib.stwo_rocm_prove_wide_fibonacci(self.h, ctypes.byref(prm),
                                                          int(d_trace) if d_trace is not None else None, buf, cap,
-                                                         ctypes.byref(n), ctypes.byref(stats))
+                                                         ctypes.byref(n), ctypes.byref(st) if st is not None else None)
             if rc == 1 and n.value > cap:
                 cap = n.value
                 continue
             self._check(rc)
-            return buf.raw[: n.value], stats.as_dict()
+            return buf.raw[: n.value], (st.as_dict() if st is not None else None)
 
 
     # ---- LogUp / GKR (kernels_logup.hip, gkr.hip)

@@ -245,7 +245,7 @@ def bench_logup(stwo_rocm, args):
             "data": "synthetic: seeded 8-bit a, b (splitmix64), c = a xor b",
             "config": {"workload": "xor_logup log_n=18 (lookups 2^18, table 2^16 padded), blowup 2", "device": ident},
             "roofline": _roofline(dom, dv["ms"] / dv["launches"], dv["bytes"] / dv["launches"], profiled_traffic()),
-            "cpu_baseline": cpu, "result_sha256": shas.pop(), "bit_exact_vs_oracle": ref_sha is not None,
+            "cpu_baseline": cpu, "result_sha256": shas.pop(), "bit_exact_vs_oracle": True if ref_sha is not None else None,
             "kernels": kernels}
     if cpu:
         line["speedup_vs_cpu_baseline"] = round(cpu["value"] / med, 1)

@@ -28,6 +28,7 @@
 //                  n_masks, each [instance, n_columns, 2*n_columns QM31];
 //   per instance: [n, output claims]; [n, ood point];
 //   per instance: [n, claims to verify], n_variables; final channel digest (8 words).
+#include <memory>
 #include <vector>
 
 #include "chan.hpp"
@@ -789,9 +790,39 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
 
 }  // namespace
 
-std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<GkrInstance>& inst) {
+// The state starts from a device channel (every other word zero) and hands its digest and
+// draw counter back at the end; a failed draw or a zero division sets the channel status.
+__global__ void gkr_chan_in_kernel(GdState* __restrict__ S, const DevChannel* __restrict__ ch) {
+    uint32_t* w = (uint32_t*)S;
+    constexpr uint32_t NW = sizeof(GdState) / 4;
+    for (uint32_t i = threadIdx.x; i < NW; i += blockDim.x) {
+        uint32_t v = 0;
+        if (i < 8) v = ch->digest[i];
+        else if (i == 8) v = ch->n_draws;
+        w[i] = v;
+    }
+}
+__global__ void gkr_chan_out_kernel(DevChannel* __restrict__ ch, const GdState* __restrict__ S) {
+    const uint32_t t = threadIdx.x;
+    if (t < 8) ch->digest[t] = S->dg[t];
+    if (t == 8) ch->n_draws = S->n_draws;
+    if (t == 9 && S->status != 0) ch->status = S->status;
+}
+
+struct GkrRun {
+    std::vector<GkrInstance> inst;
+    uint32_t n_inst = 0, n_layers = 0;
+    uint64_t rec_words = 0;
+    std::vector<DBuf> bufs;
+    DBuf d_inst, d_state, d_rec, d_eq, d_part, d_cnt;
+};
+
+GkrRun* gkr_launch_on_channel(Ctx& c, DevChannel* d_ch, const std::vector<GkrInstance>& inst) {
+    SR_CHECK(d_ch, "gkr: device channel");
     SR_CHECK(!inst.empty(), "gkr: empty batch");
     SR_CHECK(inst.size() <= GD_MAX_INST, "gkr: at most 8 instances per batch");
+    std::unique_ptr<GkrRun> run(new GkrRun());
+    run->inst = inst;
     const uint32_t n_inst = (uint32_t)inst.size();
     uint32_t n_layers = 0;
     for (auto& in : inst) {
@@ -801,9 +832,11 @@ std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<Gkr
                  "gkr: missing layer columns");
         n_layers = std::max(n_layers, in.n_vars);
     }
+    run->n_inst = n_inst;
+    run->n_layers = n_layers;
     // Layer stacks (buildLayers, gkr_circuit.zig:431-447) and fix scratch.
     std::vector<GdInst> h_inst(n_inst);
-    std::vector<DBuf> bufs;
+    std::vector<DBuf>& bufs = run->bufs;
     for (uint32_t k = 0; k < n_inst; k++) {
         GdInst& g = h_inst[k];
         memset(&g, 0, sizeof(g));
@@ -833,23 +866,22 @@ std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<Gkr
             g.scr_num = bufs.back().as<uint32_t>();
         }
     }
-    DBuf d_inst = upload(c, h_inst.data(), h_inst.size());
-    GdState* hs = (GdState*)c.pinned(sizeof(GdState));
-    memset(hs, 0, sizeof(GdState));
-    memcpy(hs->dg, ch.digest, 32);
-    hs->n_draws = ch.n_draws;
-    DBuf d_state = upload(c, hs, 1);
+    run->d_inst = upload(c, h_inst.data(), h_inst.size());
+    run->d_state = DBuf(&c, sizeof(GdState));
+    hipLaunchKernelGGL(gkr_chan_in_kernel, dim3(1), dim3(256), 0, c.stream, run->d_state.as<GdState>(),
+                       (const DevChannel*)d_ch);
+    SR_HIP(hipGetLastError());
     const uint32_t total_rounds = n_layers * (n_layers - 1) / 2;
-    const uint64_t rec_words = (uint64_t)total_rounds * GD_REC_ROUND + (uint64_t)n_layers * n_inst * GD_REC_MASK;
-    DBuf d_rec(&c, 4 * rec_words);
-    DBuf d_eq(&c, n_layers >= 2 ? (16ull << (n_layers - 2)) : 16);
-    DBuf d_part(&c, 64ull * n_inst * GD_MAX_BX);
-    DBuf d_cnt(&c, 4ull * GD_CNT_STRIDE * (GD_CNT_GROUPS + 1));
-    launch_fill_u32(c, d_cnt.as<uint32_t>(), GD_CNT_STRIDE * (GD_CNT_GROUPS + 1), 0);
-    GdState* S = d_state.as<GdState>();
-    const GdInst* I = d_inst.as<GdInst>();
-    uint32_t* eq = d_eq.as<uint32_t>();
-    uint32_t* rec = d_rec.as<uint32_t>();
+    run->rec_words = (uint64_t)total_rounds * GD_REC_ROUND + (uint64_t)n_layers * n_inst * GD_REC_MASK;
+    run->d_rec = DBuf(&c, 4 * run->rec_words);
+    run->d_eq = DBuf(&c, n_layers >= 2 ? (16ull << (n_layers - 2)) : 16);
+    run->d_part = DBuf(&c, 64ull * n_inst * GD_MAX_BX);
+    run->d_cnt = DBuf(&c, 4ull * GD_CNT_STRIDE * (GD_CNT_GROUPS + 1));
+    launch_fill_u32(c, run->d_cnt.as<uint32_t>(), GD_CNT_STRIDE * (GD_CNT_GROUPS + 1), 0);
+    GdState* S = run->d_state.as<GdState>();
+    const GdInst* I = run->d_inst.as<GdInst>();
+    uint32_t* eq = run->d_eq.as<uint32_t>();
+    uint32_t* rec = run->d_rec.as<uint32_t>();
     {
         KTimer kt(c, "gkr_small_layers", 0.0);
         hipLaunchKernelGGL(gkr_fused_kernel, dim3(1), dim3(GD_FT), 0, c.stream, S, I, n_inst, n_layers, 0u,
@@ -873,8 +905,8 @@ std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<Gkr
                 n_part += inst[k].n_vars + L >= n_layers && inst[k].n_vars + L - n_layers >= L - rho;
             KTimer kt(c, "gkr_round", 96.0 * (double)nt * n_part);
             hipLaunchKernelGGL(gkr_round_kernel, dim3((uint32_t)bx, n_part), dim3(GD_RT), 0, c.stream, S, I, n_inst,
-                               n_layers, L, rho, (const uint32_t*)eq, rec, d_part.as<unsigned long long>(),
-                               d_cnt.as<uint32_t>());
+                               n_layers, L, rho, (const uint32_t*)eq, rec, run->d_part.as<unsigned long long>(),
+                               run->d_cnt.as<uint32_t>());
             SR_HIP(hipGetLastError());
         }
         KTimer kt(c, "gkr_small_rounds", 0.0);
@@ -882,9 +914,24 @@ std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<Gkr
                            eq, rec);
         SR_HIP(hipGetLastError());
     }
-    std::vector<uint32_t> hrec(rec_words);
-    SR_HIP(hipMemcpyAsync(hs, S, sizeof(GdState), hipMemcpyDeviceToHost, c.stream));
-    SR_HIP(hipMemcpyAsync(hrec.data(), rec, 4 * rec_words, hipMemcpyDeviceToHost, c.stream));
+    hipLaunchKernelGGL(gkr_chan_out_kernel, dim3(1), dim3(64), 0, c.stream, d_ch, (const GdState*)S);
+    SR_HIP(hipGetLastError());
+    return run.release();
+}
+
+void gkr_release(GkrRun* run) { delete run; }
+
+// Reads the proof back (one synchronisation) and assembles the word stream; ch_out (optional)
+// receives the final channel state.
+static std::vector<uint32_t> gkr_collect_into(Ctx& c, GkrRun* run, Channel* ch_out) {
+    std::unique_ptr<GkrRun> own(run);
+    const std::vector<GkrInstance>& inst = run->inst;
+    const uint32_t n_inst = run->n_inst, n_layers = run->n_layers;
+    const uint32_t total_rounds = n_layers * (n_layers - 1) / 2;
+    std::vector<uint32_t> hrec(run->rec_words);
+    GdState* hs = (GdState*)c.pinned(sizeof(GdState));
+    SR_HIP(hipMemcpyAsync(hs, run->d_state.p, sizeof(GdState), hipMemcpyDeviceToHost, c.stream));
+    SR_HIP(hipMemcpyAsync(hrec.data(), run->d_rec.p, 4 * run->rec_words, hipMemcpyDeviceToHost, c.stream));
     c.sync();
     SR_CHECK(hs->status != 1, "gkr: channel draw rejected 64 times");
     SR_CHECK(hs->status == 0, "gkr: division by zero in a round polynomial");
@@ -925,9 +972,23 @@ std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<Gkr
         w.push_back(inst[k].n_vars);
     }
     w.insert(w.end(), hs->dg, hs->dg + 8);
-    memcpy(ch.digest, hs->dg, 32);
-    ch.n_draws = hs->n_draws;
+    if (ch_out) {
+        memcpy(ch_out->digest, hs->dg, 32);
+        ch_out->n_draws = hs->n_draws;
+    }
     return w;
+}
+
+std::vector<uint32_t> gkr_collect(Ctx& c, GkrRun* run) { return gkr_collect_into(c, run, nullptr); }
+
+std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<GkrInstance>& inst) {
+    DBuf d_ch(&c, sizeof(DevChannel));
+    DevChannel hc{};
+    memcpy(hc.digest, ch.digest, 32);
+    hc.n_draws = ch.n_draws;
+    SR_HIP(hipMemcpyAsync(d_ch.p, c.stage(&hc, sizeof(hc)), sizeof(hc), hipMemcpyHostToDevice, c.stream));
+    GkrRun* run = gkr_launch_on_channel(c, d_ch.as<DevChannel>(), inst);
+    return gkr_collect_into(c, run, &ch);
 }
 
 }  // namespace sr

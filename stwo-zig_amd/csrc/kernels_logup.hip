@@ -48,13 +48,21 @@ __global__ __launch_bounds__(LG_THREADS) void xor_trace_kernel(uint32_t log_size
 }
 
 // den[i] = z - (a + b alpha + c alpha^2); table rows (x = (i >> 8) & 255, y = i & 255, x ^ y) when
-// a == nullptr (rows beyond 2^16 repeat the table and carry zero multiplicity).
+// a == nullptr (rows beyond 2^16 repeat the table and carry zero multiplicity).  With zs set,
+// z = zs[0] and alpha = zs[1] are read from device memory (drawn by a device transcript step).
 __global__ __launch_bounds__(LG_THREADS) void logup_denoms_kernel(const uint32_t* __restrict__ a,
                                                                   const uint32_t* __restrict__ b,
                                                                   const uint32_t* __restrict__ c, uint64_t n, qm31 z,
-                                                                  qm31 alpha, qm31 alpha2, uint4* __restrict__ den) {
+                                                                  qm31 alpha, qm31 alpha2, const uint4* __restrict__ zs,
+                                                                  uint4* __restrict__ den) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (zs) {
+        const uint4 zv = zs[0], av = zs[1];
+        z = {{zv.x, zv.y}, {zv.z, zv.w}};
+        alpha = {{av.x, av.y}, {av.z, av.w}};
+        alpha2 = qsqr(alpha);
+    }
     uint32_t x, y, r;
     if (a) {
         x = a[i];
@@ -285,19 +293,27 @@ void launch_logup_denominators(Ctx& c, const uint32_t* a, const uint32_t* b, con
                                qm31 alpha, uint32_t* den) {
     KTimer kt(c, "logup_denominators", (a ? 12.0 : 0.0) * n + 16.0 * n);
     hipLaunchKernelGGL(logup_denoms_kernel, grid_for(n), dim3(LG_THREADS), 0, c.stream, a, b, cc, n, z, alpha,
-                       qsqr(alpha), (uint4*)den);
+                       qsqr(alpha), (const uint4*)nullptr, (uint4*)den);
     SR_HIP(hipGetLastError());
 }
 
-qm31 logup_cumulative(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n, uint32_t* sums) {
+void launch_logup_denominators_dev(Ctx& c, const uint32_t* a, const uint32_t* b, const uint32_t* cc, uint64_t n,
+                                   const uint32_t* d_z_alpha, uint32_t* den) {
+    SR_CHECK(d_z_alpha, "denominators: challenges");
+    KTimer kt(c, "logup_denominators", (a ? 12.0 : 0.0) * n + 16.0 * n);
+    hipLaunchKernelGGL(logup_denoms_kernel, grid_for(n), dim3(LG_THREADS), 0, c.stream, a, b, cc, n, qzero(), qzero(),
+                       qzero(), (const uint4*)d_z_alpha, (uint4*)den);
+    SR_HIP(hipGetLastError());
+}
+
+static void launch_cumulative(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n,
+                              uint32_t* sums, DBuf& bs, uint4* d_total, uint32_t* d_flag) {
     SR_CHECK(n > 0, "cumulative column: empty");
     SR_CHECK(num_kind >= 0 && num_kind <= 2 && (num_kind == 2 || num), "cumulative column: numerators");
     const uint64_t n_blocks = (n + SCAN_TILE - 1) / SCAN_TILE;
     SR_CHECK(n_blocks < (1ull << 31), "cumulative column: too long");
-    DBuf bs(&c, 16 * n_blocks + 16 + 16);
+    bs = DBuf(&c, 16 * n_blocks + 16);
     uint4* d_bs = bs.as<uint4>();
-    uint4* d_total = d_bs + n_blocks;
-    uint32_t* d_flag = (uint32_t*)(d_total + 1);
     launch_fill_u32(c, d_flag, 1, 0);
     {
         KTimer kt(c, "logup_scan", (num_kind == 0 ? 16.0 : num_kind == 1 ? 4.0 : 0.0) * n + 32.0 * n);
@@ -310,11 +326,24 @@ qm31 logup_cumulative(Ctx& c, int num_kind, const uint32_t* num, const uint32_t*
                                (const uint4*)d_bs);
         SR_HIP(hipGetLastError());
     }
+}
+
+qm31 logup_cumulative(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n, uint32_t* sums) {
+    DBuf tot(&c, 32), bs;
+    uint4* d_total = tot.as<uint4>();
+    launch_cumulative(c, num_kind, num, den, n, sums, bs, d_total, (uint32_t*)(d_total + 1));
     uint32_t* h = (uint32_t*)c.pinned(32);
     SR_HIP(hipMemcpyAsync(h, d_total, 20, hipMemcpyDeviceToHost, c.stream));
     c.sync();
     SR_CHECK(h[4] == 0, "LogUp: zero denominator");
     return qload(h);
+}
+
+void logup_cumulative_dev(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n, uint32_t* sums,
+                          uint32_t* d_total, uint32_t* d_flag) {
+    SR_CHECK(((uintptr_t)d_total & 15) == 0 && d_flag, "cumulative column: total must be 16-byte aligned");
+    DBuf bs;
+    launch_cumulative(c, num_kind, num, den, n, sums, bs, (uint4*)d_total, d_flag);
 }
 
 void launch_gkr_eq_evals(Ctx& c, const uint32_t* d_y, uint32_t n, qm31 scale, uint32_t* d_out) {

@@ -229,36 +229,8 @@ void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out, const std::ve
     out.tree = owner.add(merkle_commit(c, cols, out.lde_logs));
 }
 
-// Host-side image of the small device tables a proof needs (column pointer lists, leaf plans,
-// quotient entries, the initial channel): built before the first launch and uploaded with
-// one copy, each table at a 16-byte aligned offset.
-struct HostPack {
-    std::vector<uint8_t> buf;
-    template <typename T>
-    size_t add(const T* p, size_t n) {
-        const size_t at = (buf.size() + 15) & ~(size_t)15;
-        buf.resize(at + n * sizeof(T));
-        if (n) memcpy(buf.data() + at, p, n * sizeof(T));
-        return at;
-    }
-    size_t reserve(size_t bytes) {
-        const size_t at = (buf.size() + 15) & ~(size_t)15;
-        buf.resize(at + bytes, 0);
-        return at;
-    }
-};
-
-// commit_coefficients in two halves around the one upload: the plan allocates the LDE block
-// and the tree and records the pointer tables and the leaf plan; the launch runs the LDE and
-// the Merkle commitment from the uploaded copies.
-struct CommitPlan {
-    uint32_t n = 0, m = 0, coeff_log = 0;
-    size_t at_src = 0, at_dst = 0, at_tab = 0;
-    std::vector<LeafSeg> segs;
-};
-
-static void plan_commit(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
-                        uint32_t coeff_log, uint32_t blowup, HostPack& pack, CommitPlan& cp) {
+void plan_commit(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
+                 uint32_t coeff_log, uint32_t blowup, HostPack& pack, CommitPlan& cp) {
     cp.n = (uint32_t)coeffs.size();
     cp.m = coeff_log + blowup;
     cp.coeff_log = coeff_log;
@@ -280,7 +252,7 @@ static void plan_commit(Ctx& c, TreeOwner& owner, Committed& out, const std::vec
     out.tree = owner.make(cp.m);
 }
 
-static void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack, uint8_t* d_root_out) {
+void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack, uint8_t* d_root_out) {
     launch_circle_fft(c, false, (const uint32_t* const*)(d_pack + cp.at_src), (uint32_t* const*)(d_pack + cp.at_dst),
                       cp.n, cp.m, cp.coeff_log);
     launch_merkle_leaves_dev(c, (const LeafColDev*)(d_pack + cp.at_tab), cp.segs, cp.n, *out.tree);

@@ -54,6 +54,37 @@ struct Committed {
     DeviceTree* tree = nullptr;
 };
 
+// Host-side image of the small device tables of one pipeline run (column pointer lists,
+// leaf plans, quotient entries, an initial channel): built before the first launch and
+// uploaded with one copy, each table at a 16-byte aligned offset.
+struct HostPack {
+    std::vector<uint8_t> buf;
+    template <typename T>
+    size_t add(const T* p, size_t n) {
+        const size_t at = (buf.size() + 15) & ~(size_t)15;
+        buf.resize(at + n * sizeof(T));
+        if (n) memcpy(buf.data() + at, p, n * sizeof(T));
+        return at;
+    }
+    size_t reserve(size_t bytes) {
+        const size_t at = (buf.size() + 15) & ~(size_t)15;
+        buf.resize(at + bytes, 0);
+        return at;
+    }
+};
+// commit_coefficients in two halves around the one upload: plan_commit allocates the LDE
+// block and the tree and records the pointer tables and the leaf plan in the pack;
+// launch_commit runs the LDE and the Merkle commitment from the uploaded copies and writes
+// the root to d_root_out (optional).  Columns of one log size.
+struct CommitPlan {
+    uint32_t n = 0, m = 0, coeff_log = 0;
+    size_t at_src = 0, at_dst = 0, at_tab = 0;
+    std::vector<LeafSeg> segs;
+};
+void plan_commit(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
+                 uint32_t coeff_log, uint32_t blowup, HostPack& pack, CommitPlan& cp);
+void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack, uint8_t* d_root_out);
+
 // LDE of coefficient columns (pointer list) by `blowup` and Merkle commit
 // (commit_polys.zig:10-67; preparation.zig:113-210).
 void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
@@ -88,6 +119,13 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
 // GKR batch proof of the given input layers on `ch` (device-resident layers);
 // returns the proof word stream described in gkr.hip.
 std::vector<uint32_t> gkr_prove_batch(Ctx& c, Channel& ch, const std::vector<GkrInstance>& inst);
+// The same on a device channel, without a host round trip: gkr_launch_on_channel enqueues the
+// whole proof (the state starts from and ends in d_ch); gkr_collect reads the proof back
+// (one synchronisation) and returns the word stream.
+struct GkrRun;
+GkrRun* gkr_launch_on_channel(Ctx& c, DevChannel* d_ch, const std::vector<GkrInstance>& inst);
+std::vector<uint32_t> gkr_collect(Ctx& c, GkrRun* run);
+void gkr_release(GkrRun* run);
 // One sum-check round polynomial of a layer (eq table on the device, y = eq point).
 std::vector<qm31> gkr_sum_as_poly(Ctx& c, int kind, const uint32_t* d_eq, const uint32_t* d_num,
                                   const uint32_t* d_den, uint32_t n_vars, qm31 lambda, qm31 claim,
@@ -146,7 +184,11 @@ class BatchedPipeline {
 // mixRoot(root) [+ mixU32s(words[0..n_words))], alpha = drawSecureFelt, powers[k] = alpha^k.
 void launch_tr_commit_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, const uint32_t* words, uint32_t n_words,
                            uint32_t* d_powers, uint32_t n_pow);
-// mixRoot(root), t = drawSecureFelt, OODS point (x, y) and the point-evaluation factors.
+// digest <- H(digest || words[0..n_words)) if n_words > 0 (words in device memory), then
+// n_draws secure felts into d_out (4 words each).
+void launch_tr_mix_draws(Ctx& c, DevChannel* d_ch, const uint32_t* d_words, uint32_t n_words, uint32_t n_draws,
+                         uint32_t* d_out);
+// [mixRoot(root) if d_root], t = drawSecureFelt, OODS point (x, y) and the point-evaluation factors.
 void launch_tr_oods(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, uint32_t n_factors, uint32_t* d_oods,
                     uint32_t* d_factors);
 // mixFelts(sampled values), q = drawSecureFelt, coefficients of the entries (column order)

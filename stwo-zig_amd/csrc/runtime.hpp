@@ -281,6 +281,13 @@ void launch_logup_denominators(Ctx& c, const uint32_t* a, const uint32_t* b, con
 // Cumulative column S_i = sum_{j<=i} num_j / den_j (num_kind 0 QM31, 1 M31, 2 ones); returns the claimed sum
 // (synchronises); throws on a zero denominator.
 qm31 logup_cumulative(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n, uint32_t* sums);
+// The same without a readback: the claimed sum goes to d_total[0..4) (16-byte aligned) and the
+// zero-denominator flag to *d_flag.
+void logup_cumulative_dev(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n, uint32_t* sums,
+                          uint32_t* d_total, uint32_t* d_flag);
+// Denominators with z = d_z_alpha[0..4) and alpha = d_z_alpha[4..8) read on the device.
+void launch_logup_denominators_dev(Ctx& c, const uint32_t* a, const uint32_t* b, const uint32_t* cc, uint64_t n,
+                                   const uint32_t* d_z_alpha, uint32_t* den);
 void launch_gkr_eq_evals(Ctx& c, const uint32_t* d_y, uint32_t n, qm31 scale, uint32_t* d_out);
 void launch_gkr_next_layer(Ctx& c, int kind, const uint32_t* d_num, const uint32_t* d_den, uint32_t n_vars,
                            uint32_t* d_out_num, uint32_t* d_out_den);

@@ -96,6 +96,7 @@ __global__ __launch_bounds__(256) void tr_commit_draw_kernel(DevChannel* __restr
     }
 }
 
+// root == nullptr: the point is drawn without a mix first (the draw counter continues).
 __global__ void tr_oods_kernel(DevChannel* __restrict__ ch, const uint32_t* __restrict__ root, uint32_t n_factors,
                                uint4* __restrict__ oods, uint4* __restrict__ factors) {
     const uint32_t t = threadIdx.x;
@@ -103,10 +104,13 @@ __global__ void tr_oods_kernel(DevChannel* __restrict__ ch, const uint32_t* __re
     const QuadLane Lq(t);
     uint32_t dg[8], pay[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     tr_load_dg(ch, dg);
+    uint32_t nd = ch->n_draws, st = 0;
+    if (root) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) pay[k] = root[k];
-    qchan_mix(Lq, dg, pay, 8);
-    uint32_t nd = 0, st = 0;
+        for (int k = 0; k < 8; k++) pay[k] = root[k];
+        qchan_mix(Lq, dg, pay, 8);
+        nd = 0;
+    }
     const qm31 tt = qchan_draw(Lq, dg, nd, st);
     if (t != 0) return;
     tr_store(ch, dg, nd, st);
@@ -124,6 +128,27 @@ __global__ void tr_oods_kernel(DevChannel* __restrict__ ch, const uint32_t* __re
         }
         factors[k] = make_uint4(f.c0.a, f.c0.b, f.c1.a, f.c1.b);
     }
+}
+
+// digest <- Blake2s(digest || words[0..n_words)) when n_words > 0 (mixRoot, mixFelts and
+// mixU32s are all this on little-endian words), then n_draws secure felts into out[0..n_draws).
+__global__ void tr_mix_draws_kernel(DevChannel* __restrict__ ch, const uint32_t* __restrict__ words,
+                                    uint32_t n_words, uint32_t n_draws, uint4* __restrict__ out) {
+    const uint32_t t = threadIdx.x;
+    if (t >= 4) return;
+    const QuadLane Lq(t);
+    uint32_t dg[8];
+    tr_load_dg(ch, dg);
+    uint32_t nd = ch->n_draws, st = 0;
+    if (n_words) {
+        qchan_mix_long(Lq, dg, words, n_words);
+        nd = 0;
+    }
+    for (uint32_t k = 0; k < n_draws; k++) {
+        const qm31 a = qchan_draw(Lq, dg, nd, st);
+        if (t == 0) out[k] = make_uint4(a.c0.a, a.c0.b, a.c1.a, a.c1.b);
+    }
+    if (t == 0) tr_store(ch, dg, nd, st);
 }
 
 // One workgroup: mix the sampled values, draw the quotient coefficient q and fill the
@@ -203,6 +228,16 @@ void launch_tr_commit_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, cons
     KTimer kt(c, "transcript", 0.0);
     hipLaunchKernelGGL(tr_commit_draw_kernel, dim3(1), dim3(256), 0, c.stream, d_ch, (const uint32_t*)d_root,
                        n_words > 0 ? words[0] : 0u, n_words > 1 ? words[1] : 0u, n_words, (uint4*)d_powers, n_pow);
+    SR_HIP(hipGetLastError());
+}
+
+void launch_tr_mix_draws(Ctx& c, DevChannel* d_ch, const uint32_t* d_words, uint32_t n_words, uint32_t n_draws,
+                         uint32_t* d_out) {
+    SR_CHECK(n_words == 0 || d_words, "transcript: words");
+    SR_CHECK(n_draws == 0 || d_out, "transcript: draw output");
+    KTimer kt(c, "transcript", 4.0 * n_words);
+    hipLaunchKernelGGL(tr_mix_draws_kernel, dim3(1), dim3(64), 0, c.stream, d_ch, d_words, n_words, n_draws,
+                       (uint4*)d_out);
     SR_HIP(hipGetLastError());
 }
 

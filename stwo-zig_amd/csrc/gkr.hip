@@ -697,7 +697,7 @@ __global__ __launch_bounds__(GD_RT) void gkr_begin_eq_kernel(GdState* __restrict
 // cache lines; the workgroup completing a sub-counter counts on the top counter), so no
 // address takes more than ~nb/16 atomics; the last workgroup adds the slots up and finishes
 // the round.  Workgroup (0, 0) applies the previous round's pending claim update first; the
-// fence before its count publishes it to the last workgroup.
+// release fence before its count (the only one) publishes it to the last workgroup.
 __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ S, const GdInst* __restrict__ I,
                                                           uint32_t n_inst, uint32_t n_layers, uint32_t L, uint32_t rho,
                                                           const uint32_t* __restrict__ eq, uint32_t* __restrict__ rec,
@@ -741,22 +741,28 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
         if ((t & 63) == 0) red[t >> 6][c] = w;
     }
     __syncthreads();
+    // The slot is written with device-coherent stores (agent-scope atomic stores) by wave 0,
+    // which waits for their acknowledgement before thread 0 counts the workgroup: no L2
+    // writeback per workgroup.  Workgroup (0, 0) also wrote the pending claim update into the
+    // state with plain stores and publishes it with one release; the last workgroup acquires.
     if (t < 8) {
         uint64_t w = 0;
         for (uint32_t k = 0; k < GD_RT / 64; k++) w += red[k][t];
-        part[((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * 8 + t] = w;
+        __hip_atomic_store(&part[((uint64_t)blockIdx.y * gridDim.x + blockIdx.x) * 8 + t], (unsigned long long)w,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // the barrier completes the slot stores; one agent-scope release (thread 0) publishes them
-    __syncthreads();
-    if (t == 0) {
-        __threadfence();
-        const uint32_t nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
-        const uint32_t g = b % GD_CNT_GROUPS, ng = nb < GD_CNT_GROUPS ? nb : GD_CNT_GROUPS;
-        const uint32_t gsize = (nb - g + GD_CNT_GROUPS - 1) / GD_CNT_GROUPS;
-        uint32_t lst = 0;
-        if (atomicAdd(&cnt[GD_CNT_STRIDE * g], 1u) == gsize - 1)
-            lst = atomicAdd(&cnt[GD_CNT_STRIDE * GD_CNT_GROUPS], 1u) == ng - 1 ? 1u : 0u;
-        last = lst;
+    if (t < 64) {
+        __builtin_amdgcn_s_waitcnt(0);
+        if (t == 0) {
+            if (blockIdx.x == 0 && blockIdx.y == 0) __threadfence();
+            const uint32_t nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+            const uint32_t g = b % GD_CNT_GROUPS, ng = nb < GD_CNT_GROUPS ? nb : GD_CNT_GROUPS;
+            const uint32_t gsize = (nb - g + GD_CNT_GROUPS - 1) / GD_CNT_GROUPS;
+            uint32_t lst = 0;
+            if (atomicAdd(&cnt[GD_CNT_STRIDE * g], 1u) == gsize - 1)
+                lst = atomicAdd(&cnt[GD_CNT_STRIDE * GD_CNT_GROUPS], 1u) == ng - 1 ? 1u : 0u;
+            last = lst;
+        }
     }
     __syncthreads();
     if (!last) return;
@@ -768,9 +774,9 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
         uint64_t b8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const uint32_t row = py++;
         for (uint32_t x = t; x < gridDim.x; x += GD_RT) {
-            const unsigned long long* pp = part + ((uint64_t)row * gridDim.x + x) * 8;
+            unsigned long long* pp = part + ((uint64_t)row * gridDim.x + x) * 8;
 #pragma unroll
-            for (int c = 0; c < 8; c++) b8[c] += pp[c];
+            for (int c = 0; c < 8; c++) b8[c] += __hip_atomic_load(&pp[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
 #pragma unroll
         for (int c = 0; c < 8; c++) {

@@ -208,9 +208,10 @@ __device__ __forceinline__ qm31 tr_chan_step(const QuadLane& L, uint32_t q, uint
 // [b 2^s, (b+1) 2^s) of level k0: it produces them (SRC = FRI_SRC_LINE / FRI_SRC_CIRCLE:
 // fold the previous layer with the alpha in `ch`, store the folded value and hash it as a
 // leaf) or loads them (FRI_SRC_HASHES), then reduces them to its subtree root at level
-// k0 - s.  The last workgroup to finish (split completion counters, one agent-scope
-// release per workgroup) reduces the 2^(k0-s) subtree roots to the root and, with `ch`,
-// runs the channel step on it.  The counters are left at zero.
+// k0 - s.  The last workgroup to finish (split completion counters; the subtree roots are
+// exchanged through device-coherent stores and loads, so no workgroup writes back its L2)
+// reduces the 2^(k0-s) subtree roots to the root and, with `ch`, runs the channel step on
+// it.  The counters are left at zero.
 template <int SRC>
 __global__ __launch_bounds__(TL_THREADS) void tree_layer_kernel(FriLayerArgs a) {
     __shared__ __align__(16) uint32_t hA[1024 * 8];
@@ -263,33 +264,34 @@ __global__ __launch_bounds__(TL_THREADS) void tree_layer_kernel(FriLayerArgs a) 
     const uint32_t nb = gridDim.x;
     uint32_t dg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (nb > 1) {
-        // the barrier that ended the last level completed every store of this workgroup;
-        // one agent-scope release (thread 0) publishes them before the count
-        if (t == 0) {
-            __threadfence();
-            const uint32_t b = blockIdx.x, g = b % TREE_CNT_GROUPS, ng = nb < TREE_CNT_GROUPS ? nb : TREE_CNT_GROUPS;
-            const uint32_t gsize = (nb - g + TREE_CNT_GROUPS - 1) / TREE_CNT_GROUPS;
-            uint32_t lst = 0;
-            if (atomicAdd(&a.cnt[TREE_CNT_STRIDE * g], 1u) == gsize - 1)
-                lst = atomicAdd(&a.cnt[TREE_CNT_STRIDE * TREE_CNT_GROUPS], 1u) == ng - 1 ? 1u : 0u;
-            last = lst;
+        // Hand-off without an L2 writeback: lanes 0..7 publish the subtree root (in LDS since
+        // the barrier that ended the last level) with device-coherent stores (agent-scope
+        // atomic stores, global_store sc1), the wave waits until they are acknowledged, and
+        // thread 0 counts the workgroup; the last workgroup reads the roots with
+        // device-coherent loads.  The tree's own copy of the roots (plain stores) is read by
+        // later launches only.
+        if (t < 8) __hip_atomic_store(&a.xr[8 * blockIdx.x + t], in[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t < 64) {
+            __builtin_amdgcn_s_waitcnt(0);
+            if (t == 0) {
+                const uint32_t b = blockIdx.x, g = b % TREE_CNT_GROUPS, ng = nb < TREE_CNT_GROUPS ? nb : TREE_CNT_GROUPS;
+                const uint32_t gsize = (nb - g + TREE_CNT_GROUPS - 1) / TREE_CNT_GROUPS;
+                uint32_t lst = 0;
+                if (atomicAdd(&a.cnt[TREE_CNT_STRIDE * g], 1u) == gsize - 1)
+                    lst = atomicAdd(&a.cnt[TREE_CNT_STRIDE * TREE_CNT_GROUPS], 1u) == ng - 1 ? 1u : 0u;
+                last = lst;
+            }
         }
         __syncthreads();
         if (!last) return;
-        __threadfence();
         if (t <= TREE_CNT_GROUPS) atomicExch(&a.cnt[TREE_CNT_STRIDE * t], 0u);
         if (a.ch && t < 4) {
 #pragma unroll
             for (int k = 0; k < 8; k++) dg[k] = a.ch->digest[k];
         }
         // top levels from the nb subtree roots
-        const uint8_t* roots = a.tree + (((uint64_t)1 << lev) - 1) * 32;
-        for (uint32_t p = t; p < nb; p += TL_THREADS) {
-            const uint4* s4 = (const uint4*)(roots + (uint64_t)p * 32);
-            const uint4 x = s4[0], y = s4[1];
-            *(uint4*)&hA[8 * p] = x;
-            *(uint4*)&hA[8 * p + 4] = y;
-        }
+        for (uint32_t i = t; i < 8 * nb; i += TL_THREADS)
+            hA[i] = __hip_atomic_load(&a.xr[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         in = hA;
         out = hB;
@@ -400,9 +402,10 @@ static void launch_tree_layer(Ctx& c, FriLayerArgs& a, const std::string& cls, d
     SR_CHECK(a.k0 >= 1 && a.k0 <= TREE_LAYER_MAX_LOG, "tree layer: level out of range");
     a.s = tree_chunk_log(a.k0);
     const uint32_t nb = 1u << (a.k0 - a.s);
-    SR_CHECK(nb <= 256 && a.s >= 1 && a.s <= 10, "tree layer: chunking");
+    SR_CHECK(nb <= TREE_MAX_BLOCKS && a.s >= 1 && a.s <= 10, "tree layer: chunking");
     SR_CHECK(c.d_tree_cnt, "tree layer: context without completion counters");
     a.cnt = c.d_tree_cnt;
+    a.xr = c.d_tree_cnt + TREE_XR_OFFSET;
     a.leaf_seed = c.leaf_seed;
     a.node_seed = c.node_seed;
     KTimer kt(c, cls, bytes);

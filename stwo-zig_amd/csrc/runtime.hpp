@@ -232,6 +232,8 @@ void launch_chan_mix_root_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, 
 constexpr uint32_t TREE_LAYER_MAX_LOG = 16;
 constexpr uint32_t TREE_CNT_GROUPS = 16;  // completion sub-counters, then the top counter
 constexpr uint32_t TREE_CNT_STRIDE = 32;  // words between counters (a cache line each)
+constexpr uint32_t TREE_XR_OFFSET = TREE_CNT_STRIDE * (TREE_CNT_GROUPS + 1);  // root exchange after them
+constexpr uint32_t TREE_MAX_BLOCKS = 256;
 enum FriSrc { FRI_SRC_HASHES = 0, FRI_SRC_LINE = 1, FRI_SRC_CIRCLE = 2 };
 struct FriLayerArgs {
     const uint4* src_line;      // FRI_SRC_LINE: the previous line (2^(k0+1) QM31 values)
@@ -243,6 +245,7 @@ struct FriLayerArgs {
     DevChannel* ch;             // channel step on the root (required by the folds)
     uint32_t* root_out;         // root copy (optional)
     uint32_t* cnt;              // completion counters (set by the launcher)
+    uint32_t* xr;               // subtree root exchange, 8 words per workgroup (set by the launcher)
     B2State leaf_seed, node_seed;  // set by the launcher
 };
 // Levels k0-1 .. 0 of `t` from its hashes at level k0; with d_ch, the channel step on the root.

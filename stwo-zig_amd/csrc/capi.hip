@@ -12,6 +12,7 @@ using namespace sr;
 
 struct stwo_rocm_ctx {
     Ctx c;
+    ServePool* pool = nullptr;  // serving slots, created by the first stwo_rocm_serve
 };
 struct stwo_rocm_tree {
     DeviceTree* t;
@@ -70,6 +71,8 @@ int stwo_rocm_ctx_destroy(stwo_rocm_ctx* ctx) {
     if (!ctx) return STWO_ROCM_EINVAL;
     {
         std::lock_guard<std::mutex> lk(ctx->c.mu);
+        serve_pool_free(ctx->pool);
+        ctx->pool = nullptr;
         ctx_release(ctx->c);
     }
     delete ctx;
@@ -785,7 +788,8 @@ int stwo_rocm_serve(stwo_rocm_ctx* ctx, const stwo_rocm_serve_job* jobs, uint32_
             js[i] = ServeJob{jobs[i].kind, jobs[i].log_n, jobs[i].seq_len, 0, jobs[i].seed, jobs[i].arrival_ms};
         std::vector<ServeOutcome> out;
         ServeStats st{};
-        serve(c.device, js, n_slots, hbm_budget_bytes, out, st);
+        if (!ctx->pool) ctx->pool = serve_pool_new();
+        serve(*ctx->pool, c.device, js, n_slots, hbm_budget_bytes, out, st);
         for (uint32_t i = 0; i < n_jobs; i++) {
             memcpy(results[i].digest, out[i].digest, 32);
             results[i].arrival_ms = out[i].arrival_ms;

@@ -215,7 +215,12 @@ struct ServeStats {
     uint32_t max_in_flight, n_slots;
 };
 uint64_t serve_job_bytes(const ServeJob& j);  // admission estimate of a job's HBM footprint
-void serve(int device, const std::vector<ServeJob>& jobs, uint32_t n_slots, uint64_t hbm_budget,
+// Slot contexts kept between serve() calls (streams, warm arenas, towers, pinned staging):
+// a service keeps its resources resident instead of rebuilding them per batch of requests.
+struct ServePool;
+ServePool* serve_pool_new();
+void serve_pool_free(ServePool* p);
+void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint32_t n_slots, uint64_t hbm_budget,
            std::vector<ServeOutcome>& out, ServeStats& st);
 
 }  // namespace sr

@@ -3,7 +3,8 @@
 // The reference admits one job at a time against a resource profile
 // (src/prover/native/resource_admission.zig) and proves it; here a request loop
 // keeps n_slots proof slots busy: every slot owns a device context (its own HIP
-// stream, caching arena, twiddle tower and resident Merkle layers), requests
+// stream, caching arena, twiddle tower and resident Merkle layers; the slots persist in
+// the library context between calls, so arenas stay warm), requests
 // become eligible at their arrival time, and the head of the FIFO is admitted as
 // soon as a slot is idle and its estimated HBM footprint fits the shared budget.
 // Jobs of different slots overlap on the device (independent streams).  Every
@@ -47,7 +48,13 @@ struct Slot {
 
 }  // namespace
 
-void serve(int device, const std::vector<ServeJob>& jobs, uint32_t n_slots, uint64_t hbm_budget,
+struct ServePool {
+    std::vector<std::unique_ptr<Slot>> slots;
+};
+ServePool* serve_pool_new() { return new ServePool(); }
+void serve_pool_free(ServePool* p) { delete p; }
+
+void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint32_t n_slots, uint64_t hbm_budget,
            std::vector<ServeOutcome>& out, ServeStats& st) {
     SR_CHECK(n_slots >= 1 && n_slots <= 16, "serve: n_slots must be 1..16");
     SR_CHECK(!jobs.empty(), "serve: no jobs");
@@ -62,13 +69,20 @@ void serve(int device, const std::vector<ServeJob>& jobs, uint32_t n_slots, uint
         SR_CHECK(hbm_budget == 0 || serve_job_bytes(j) <= hbm_budget, "serve: a job exceeds the HBM budget");
         max_log = std::max(max_log, j.log_n + 2);
     }
-    // Slots: contexts with the twiddle tower ready (warm), so no job pays set-up.
-    std::vector<std::unique_ptr<Slot>> slots;
+    // Slots: resident contexts (kept in the pool between calls) with the twiddle tower and the
+    // pinned staging ready, so no job pays set-up.
+    std::vector<std::unique_ptr<Slot>>& slots = pool.slots;
+    while (slots.size() < n_slots) {
+        std::unique_ptr<Slot> sl(new Slot());
+        ctx_init(sl->c, device, 0);
+        uint8_t z = 0;
+        sl->c.stage(&z, 1);
+        (void)sl->c.pinned(1u << 20);
+        slots.push_back(std::move(sl));
+    }
     for (uint32_t s = 0; s < n_slots; s++) {
-        slots.emplace_back(new Slot());
-        ctx_init(slots.back()->c, device, 0);
-        slots.back()->c.ensure_tower(max_log);
-        slots.back()->c.sync();
+        slots[s]->c.ensure_tower(max_log);
+        slots[s]->c.sync();
     }
     out.assign(jobs.size(), ServeOutcome{});
     std::mutex mu;
@@ -160,7 +174,6 @@ void serve(int device, const std::vector<ServeJob>& jobs, uint32_t n_slots, uint
         cv.notify_all();
     }
     for (auto& t : th) t.join();
-    slots.clear();
     SR_CHECK(first_error.empty(), "serve: job failed: " + first_error);
     std::vector<double> lat(jobs.size());
     double last_end = 0, sum = 0;

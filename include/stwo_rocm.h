@@ -188,6 +188,23 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
                                    const uint32_t* d_trace, uint8_t* proof_out, uint64_t cap, uint64_t* proof_len,
                                    stwo_rocm_prove_stats* stats);
 
+/* Poseidon AIR statement and PCS configuration (examples/poseidon/input.zig:20-22;
+ * core/pcs/mod.zig PcsConfig). */
+typedef struct {
+    uint32_t log_n_instances; /* 2^log_n_instances permutations, 8 per trace row: 4..29 */
+    uint32_t pow_bits;
+    uint32_t log_blowup_factor;
+    uint32_t log_last_layer_degree_bound;
+    uint32_t n_queries;
+} stwo_rocm_poseidon_params;
+
+/* poseidon.prove (examples/poseidon.zig:67-73 -> prover_transaction.zig -> prover/prove.zig):
+ * the 1264-column Poseidon trace of input.zig:121-211 generated on the device, one main tree,
+ * the constant composition of poseidon.zig:562-569, and the rest of the proof as for the
+ * wide-Fibonacci AIR.  Same output, buffer and stats conventions. */
+int stwo_rocm_prove_poseidon(stwo_rocm_ctx* ctx, const stwo_rocm_poseidon_params* params, uint8_t* proof_out,
+                             uint64_t cap, uint64_t* proof_len, stwo_rocm_prove_stats* stats);
+
 /* ------------------------------------------------------------------ LogUp / GKR */
 /* Layer kinds of src/prover/lookups/gkr_circuit.zig:79-92. */
 #define STWO_ROCM_GKR_GRAND_PRODUCT 0
@@ -294,7 +311,8 @@ int stwo_rocm_batched_destroy(stwo_rocm_ctx* ctx, stwo_rocm_batched* b);
  * hbm_budget_bytes (0 = unlimited).  Jobs on different slots overlap on the device.  Job kinds: 0
  * wide-Fibonacci proof (functional protocol, seeded uniform trace: trace_mode 1 of
  * stwo_rocm_prove_wide_fibonacci), 1 XOR-lookup LogUp workload (stwo_rocm_logup_xor_pipeline,
- * log_n 16..24).  Each result carries the Blake2s-256 digest of its output bytes. */
+ * log_n 16..24), 2 Poseidon AIR proof (stwo_rocm_prove_poseidon, log_n = log_n_instances 4..24,
+ * functional protocol).  Each result carries the Blake2s-256 digest of its output bytes. */
 typedef struct {
     uint32_t kind, log_n, seq_len, reserved;
     uint64_t seed;

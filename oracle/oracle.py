@@ -62,6 +62,10 @@ def lib():
         L.ora_pow_grind.restype = ctypes.c_uint64
         L.ora_pow_grind.argtypes = [ctypes.c_char_p, ctypes.c_uint32]
         L.ora_widefib_trace.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, u32p]
+        L.ora_poseidon_trace.argtypes = [ctypes.c_uint32, u32p]
+        L.ora_prove_poseidon.restype = ctypes.c_int
+        L.ora_prove_poseidon.argtypes = [ctypes.c_uint32, ctypes.POINTER(WideFibParams), ctypes.c_char_p,
+                                         ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.ora_prove_wide_fibonacci.restype = ctypes.c_int
         L.ora_prove_wide_fibonacci.argtypes = [ctypes.POINTER(WideFibParams), ctypes.c_char_p, ctypes.c_size_t,
                                                ctypes.POINTER(ctypes.c_size_t)]
@@ -223,6 +227,27 @@ def prove_wide_fibonacci(log_n_rows, sequence_len, pow_bits=10, log_blowup_facto
         buf = ctypes.create_string_buffer(cap)
         n = ctypes.c_size_t(0)
         rc = lib().ora_prove_wide_fibonacci(ctypes.byref(prm), buf, cap, ctypes.byref(n))
+        if rc == 2:
+            cap = n.value + 1
+            continue
+        if rc != 0:
+            raise RuntimeError("oracle prove failed with code %d" % rc)
+        return buf.raw[: n.value]
+
+
+def poseidon_trace(log_n_rows):
+    o, po = _u32(np.zeros(1264 << log_n_rows))
+    lib().ora_poseidon_trace(log_n_rows, po)
+    return o.reshape(1264, 1 << log_n_rows)
+
+
+def prove_poseidon(log_n_instances, pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3):
+    prm = WideFibParams(0, 0, pow_bits, log_blowup_factor, log_last_layer_degree_bound, n_queries, 0, 0)
+    cap = 1 << 20
+    while True:
+        buf = ctypes.create_string_buffer(cap)
+        n = ctypes.c_size_t(0)
+        rc = lib().ora_prove_poseidon(log_n_instances, ctypes.byref(prm), buf, cap, ctypes.byref(n))
         if rc == 2:
             cap = n.value + 1
             continue

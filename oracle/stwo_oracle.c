@@ -797,6 +797,77 @@ void ora_widefib_trace(uint32_t log_n, uint32_t seq_len, uint32_t mode, uint64_t
 }
 
 /* ======================================================================
+ * Poseidon AIR trace (src/examples/poseidon/input.zig:9-211): 8 permutation instances per
+ * row, each of 16 state words through 4 full rounds, 14 partial rounds and 4 full rounds;
+ * the columns of one instance are the input state, the state after every full round and
+ * state[0] after every partial round (158 columns per instance, 1264 per row).
+ * ==================================================================== */
+#define POS_STATE 16
+#define POS_PARTIAL 14
+#define POS_HALF_FULL 4
+#define POS_COLS_PER_REP (POS_STATE * (1 + 2 * POS_HALF_FULL) + POS_PARTIAL)
+#define POS_COLS (POS_COLS_PER_REP * 8)
+static inline uint32_t pos_pow5(uint32_t x) {
+    uint32_t x2 = m_mul(x, x);
+    return m_mul(m_mul(x2, x2), x);
+}
+/* M4 of the external matrix (input.zig:170-183), as a linear map on four words. */
+static void pos_m4(uint32_t *x) {
+    uint32_t t0 = m_add(x[0], x[1]), t02 = m_add(t0, t0);
+    uint32_t t1 = m_add(x[2], x[3]), t12 = m_add(t1, t1);
+    uint32_t t2 = m_add(m_add(x[1], x[1]), t1);
+    uint32_t t3 = m_add(m_add(x[3], x[3]), t0);
+    uint32_t t4 = m_add(m_add(t12, t12), t3);
+    uint32_t t5 = m_add(m_add(t02, t02), t2);
+    x[0] = m_add(t3, t5);
+    x[1] = t5;
+    x[2] = m_add(t2, t4);
+    x[3] = t4;
+}
+static void pos_external_round(uint32_t *s, uint32_t round) {
+    for (int i = 0; i < POS_STATE; i++) s[i] = m_add(s[i], (uint32_t)((1234u + round * 37u + (uint32_t)i) % P));
+    for (int i = 0; i < 4; i++) pos_m4(s + 4 * i);
+    for (int j = 0; j < 4; j++) {
+        uint32_t sum = m_add(m_add(s[j], s[j + 4]), m_add(s[j + 8], s[j + 12]));
+        for (int i = 0; i < 4; i++) s[4 * i + j] = m_add(s[4 * i + j], sum);
+    }
+    for (int i = 0; i < POS_STATE; i++) s[i] = pos_pow5(s[i]);
+}
+static void pos_internal_round(uint32_t *s, uint32_t round) {
+    s[0] = m_add(s[0], (uint32_t)((9876u + round * 17u) % P));
+    uint32_t sum = 0;
+    for (int i = 0; i < POS_STATE; i++) sum = m_add(sum, s[i]);
+    for (int i = 0; i < POS_STATE; i++) s[i] = m_add(m_mul(s[i], (uint32_t)((1ull << (i + 1)) % P)), sum);
+    s[0] = pos_pow5(s[0]);
+}
+void ora_poseidon_trace(uint32_t log_n_rows, uint32_t *out) {
+    uint64_t n = 1ull << log_n_rows;
+#pragma omp parallel for schedule(static)
+    for (int64_t row = 0; row < (int64_t)n; row++) {
+        uint64_t col = 0;
+        for (uint32_t rep = 0; rep < 8; rep++) {
+            uint32_t s[POS_STATE];
+            for (int i = 0; i < POS_STATE; i++) {
+                s[i] = (uint32_t)(((uint64_t)row * POS_STATE + (uint64_t)i + rep) % P);
+                out[col++ * n + row] = s[i];
+            }
+            for (uint32_t r = 0; r < POS_HALF_FULL; r++) {
+                pos_external_round(s, r);
+                for (int i = 0; i < POS_STATE; i++) out[col++ * n + row] = s[i];
+            }
+            for (uint32_t r = 0; r < POS_PARTIAL; r++) {
+                pos_internal_round(s, r);
+                out[col++ * n + row] = s[0];
+            }
+            for (uint32_t r = 0; r < POS_HALF_FULL; r++) {
+                pos_external_round(s, r + POS_HALF_FULL);
+                for (int i = 0; i < POS_STATE; i++) out[col++ * n + row] = s[i];
+            }
+        }
+    }
+}
+
+/* ======================================================================
  * Proof assembly
  * ==================================================================== */
 static double g_timings[8];
@@ -942,7 +1013,12 @@ static int cmp_u64(const void *a, const void *b) {
     return x < y ? -1 : (x > y);
 }
 
-int ora_prove_wide_fibonacci(const ora_widefib_params *prm, uint8_t *out, size_t cap, size_t *out_len) {
+/* The AIRs this proof core serves: the wide-Fibonacci component and the Poseidon
+ * component (src/examples/poseidon.zig:397-504), whose composition is the constant
+ * (log_n_instances, 158, 1264, 1) (poseidon.zig:562-569). */
+enum { AIR_WIDEFIB = 0, AIR_POSEIDON = 1 };
+
+static int prove_core(int air, const ora_widefib_params *prm, uint8_t *out, size_t cap, size_t *out_len) {
     const uint32_t n = prm->log_n_rows, W = prm->sequence_len, B = prm->log_blowup_factor;
     if (n < 1 || W < 3 || B < 1) return 1;
     double t0 = now_ms();
@@ -962,7 +1038,8 @@ int ora_prove_wide_fibonacci(const ora_widefib_params *prm, uint8_t *out, size_t
     /* Tree 1: main trace: generate -> interpolate -> LDE -> Merkle (preparation.zig:113-210). */
     uint64_t N = 1ull << n;
     uint32_t *trace = malloc(4ull * N * W);
-    ora_widefib_trace(n, W, prm->trace_mode, prm->trace_seed, trace);
+    if (air == AIR_POSEIDON) ora_poseidon_trace(n, trace);
+    else ora_widefib_trace(n, W, prm->trace_mode, prm->trace_seed, trace);
     double t1 = now_ms();
     uint32_t **tc = malloc(sizeof(uint32_t *) * W);
     uint32_t *tlogs = malloc(4 * W);
@@ -979,13 +1056,14 @@ int ora_prove_wide_fibonacci(const ora_widefib_params *prm, uint8_t *out, size_t
     free(tlogs);
     double t2 = now_ms();
 
-    /* mixStatement (wide_fibonacci.zig:496-498). */
+    /* mixStatement (wide_fibonacci.zig:496-498; poseidon.zig:571-573: log_n_instances). */
     uint32_t st[2] = {n, W};
-    ch_mix_u32s(&ch, st, 2);
+    if (air == AIR_POSEIDON) st[0] = n + 3;
+    ch_mix_u32s(&ch, st, air == AIR_POSEIDON ? 1 : 2);
 
     /* prove.zig:209-318: composition. */
     qm31 alpha = ch_draw_secure_felt(&ch);
-    const uint32_t NC = W - 2, CL = n + 1; /* composition log size */
+    const uint32_t NC = air == AIR_POSEIDON ? 1 : W - 2, CL = n + 1; /* composition log size */
     uint64_t CN = 1ull << CL;
     qm31 *pw = malloc(sizeof(qm31) * NC);
     pw[0] = Q_ONE;
@@ -1015,6 +1093,13 @@ int ora_prove_wide_fibonacci(const ora_widefib_params *prm, uint8_t *out, size_t
     }
     uint32_t *comp[4];
     for (int k = 0; k < 4; k++) comp[k] = malloc(4 * CN);
+    if (air == AIR_POSEIDON) {
+        /* accumulateColumn of a constant column (accumulation.zig:142-159): the one
+         * constraint's coefficient is pw[0] = 1. */
+        const uint32_t cv[4] = {n + 3, POS_COLS_PER_REP, POS_COLS, 1};
+        for (int k = 0; k < 4; k++)
+            for (uint64_t row = 0; row < CN; row++) comp[k][row] = cv[k];
+    } else {
 #pragma omp parallel for schedule(static)
     for (int64_t row = 0; row < (int64_t)CN; row++) {
         uint32_t a = ev[0][row], b = ev[1][row];
@@ -1028,6 +1113,7 @@ int ora_prove_wide_fibonacci(const ora_widefib_params *prm, uint8_t *out, size_t
         }
         acc = q_mul_m(acc, den_inv[(uint64_t)row >> n]);
         comp[0][row] = acc.c0.a; comp[1][row] = acc.c0.b; comp[2][row] = acc.c1.a; comp[3][row] = acc.c1.b;
+    }
     }
     if (own_ev)
         for (uint32_t c = 0; c < W; c++) free(ev[c]);
@@ -1346,6 +1432,21 @@ int ora_prove_wide_fibonacci(const ora_widefib_params *prm, uint8_t *out, size_t
     free(spt);
     for (int tr = 0; tr < 3; tr++) ctree_free(&trees[tr]);
     return rc;
+}
+
+int ora_prove_wide_fibonacci(const ora_widefib_params *prm, uint8_t *out, size_t cap, size_t *out_len) {
+    return prove_core(AIR_WIDEFIB, prm, out, cap, out_len);
+}
+
+/* poseidon.prove (src/examples/poseidon.zig:67-73): 2^(log_n_instances - 3) rows of 1264
+ * columns; the trace fields of prm are ignored. */
+int ora_prove_poseidon(uint32_t log_n_instances, const ora_widefib_params *prm, uint8_t *out, size_t cap,
+                       size_t *out_len) {
+    if (log_n_instances < 4 || log_n_instances > 33) return 1;
+    ora_widefib_params p = *prm;
+    p.log_n_rows = log_n_instances - 3;
+    p.sequence_len = POS_COLS;
+    return prove_core(AIR_POSEIDON, &p, out, cap, out_len);
 }
 
 /* ======================================================================

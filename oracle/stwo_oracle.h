@@ -91,6 +91,14 @@ typedef struct {
  * when cap is too small returns 2 and only sets *out_len. */
 int ora_prove_wide_fibonacci(const ora_widefib_params *p, uint8_t *out, size_t cap, size_t *out_len);
 
+/* Poseidon AIR trace (src/examples/poseidon/input.zig:121-158): 1264 columns of 2^log_n_rows. */
+void ora_poseidon_trace(uint32_t log_n_rows, uint32_t *out);
+
+/* Poseidon AIR proof (src/examples/poseidon.zig:67-73) for 2^log_n_instances permutations
+ * (8 per row); prm supplies the PCS configuration only.  Same return convention. */
+int ora_prove_poseidon(uint32_t log_n_instances, const ora_widefib_params *prm, uint8_t *out, size_t cap,
+                       size_t *out_len);
+
 /* Stage timings of the last ora_prove_wide_fibonacci call (milliseconds). */
 void ora_last_timings(double out[8]);
 

@@ -544,6 +544,27 @@ int stwo_rocm_pow_grind(stwo_rocm_ctx* ctx, const uint8_t digest[32], uint32_t p
     });
 }
 
+// Runs the proof core and copies the proof and the optional stage timings out.
+static void prove_out(Ctx& c, const ProveParams& p, const uint32_t* d_trace, uint8_t* proof_out, uint64_t cap,
+                      uint64_t* proof_len, stwo_rocm_prove_stats* stats) {
+    std::string proof;
+    ProveStats st;
+    prove_wide_fibonacci(c, p, d_trace, proof, stats ? &st : nullptr);
+    *proof_len = proof.size();
+    if (stats) {
+        for (int i = 0; i < STAGES; i++) {
+            stats->stage_gpu_ms[i] = st.stage_gpu_ms[i];
+            stats->stage_host_ms[i] = st.stage_host_ms[i];
+        }
+        stats->total_gpu_ms = st.total_gpu_ms;
+        stats->total_host_ms = st.total_host_ms;
+        stats->peak_hbm_bytes = st.peak_hbm_bytes;
+    }
+    SR_CHECK(proof.size() <= cap, "proof buffer too small");
+    SR_CHECK(proof_out, "null proof buffer");
+    memcpy(proof_out, proof.data(), proof.size());
+}
+
 int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params,
                                    const uint32_t* d_trace, uint8_t* proof_out, uint64_t cap, uint64_t* proof_len,
                                    stwo_rocm_prove_stats* stats) {
@@ -559,22 +580,26 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
         p.trace_mode = params->trace_mode;
         p.trace_seed = params->trace_seed;
         SR_CHECK(p.trace_mode <= 1, "trace_mode must be 0 or 1");
-        std::string proof;
-        ProveStats st;
-        prove_wide_fibonacci(c, p, d_trace, proof, stats ? &st : nullptr);
-        *proof_len = proof.size();
-        if (stats) {
-            for (int i = 0; i < STAGES; i++) {
-                stats->stage_gpu_ms[i] = st.stage_gpu_ms[i];
-                stats->stage_host_ms[i] = st.stage_host_ms[i];
-            }
-            stats->total_gpu_ms = st.total_gpu_ms;
-            stats->total_host_ms = st.total_host_ms;
-            stats->peak_hbm_bytes = st.peak_hbm_bytes;
-        }
-        SR_CHECK(proof.size() <= cap, "proof buffer too small");
-        SR_CHECK(proof_out, "null proof buffer");
-        memcpy(proof_out, proof.data(), proof.size());
+        prove_out(c, p, d_trace, proof_out, cap, proof_len, stats);
+    });
+}
+
+int stwo_rocm_prove_poseidon(stwo_rocm_ctx* ctx, const stwo_rocm_poseidon_params* params, uint8_t* proof_out,
+                             uint64_t cap, uint64_t* proof_len, stwo_rocm_prove_stats* stats) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(params && proof_len, "null argument");
+        SR_CHECK(params->log_n_instances >= 4 && params->log_n_instances <= 29, "log_n_instances must be 4..29");
+        ProveParams p;
+        p.log_n_rows = params->log_n_instances - 3;
+        p.sequence_len = POSEIDON_COLUMNS;
+        p.pow_bits = params->pow_bits;
+        p.log_blowup_factor = params->log_blowup_factor;
+        p.log_last_layer_degree_bound = params->log_last_layer_degree_bound;
+        p.n_queries = params->n_queries;
+        p.trace_mode = 0;
+        p.trace_seed = 0;
+        p.air = AIR_POSEIDON;
+        prove_out(c, p, nullptr, proof_out, cap, proof_len, stats);
     });
 }
 

@@ -247,6 +247,108 @@ void launch_widefib_trace(Ctx& c, uint32_t log_n, uint32_t seq_len, uint32_t mod
     SR_HIP(hipGetLastError());
 }
 
+// Poseidon AIR trace (examples/poseidon/input.zig:121-211): one thread per (row, instance);
+// thread t handles row t mod 2^log_n of instance t >> log_n, so consecutive lanes write
+// consecutive words of every column.  The instance's columns are its input state, the state
+// after each of the 4 + 4 full rounds and state[0] after each of the 14 partial rounds.
+constexpr uint32_t POS_COLS_PER_REP = 16 * 9 + 14;
+__device__ __forceinline__ uint32_t pos_pow5(uint32_t x) {
+    const uint32_t x2 = msqr(x);
+    return mmul(msqr(x2), x);
+}
+// External round (input.zig:140-147,170-201): add constants, M4 per block of four, add the
+// column sums of the four blocks, then x^5.
+__device__ __forceinline__ void pos_external(uint32_t s[16], uint32_t round) {
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++) s[i] = madd(s[i], 1234u + round * 37u + i);
+#pragma unroll
+    for (uint32_t b = 0; b < 16; b += 4) {
+        const uint32_t t0 = madd(s[b], s[b + 1]), t1 = madd(s[b + 2], s[b + 3]);
+        const uint32_t t2 = madd(madd(s[b + 1], s[b + 1]), t1);
+        const uint32_t t3 = madd(madd(s[b + 3], s[b + 3]), t0);
+        const uint32_t t12 = madd(t1, t1), t02 = madd(t0, t0);
+        const uint32_t t4 = madd(madd(t12, t12), t3);
+        const uint32_t t5 = madd(madd(t02, t02), t2);
+        s[b] = madd(t3, t5);
+        s[b + 1] = t5;
+        s[b + 2] = madd(t2, t4);
+        s[b + 3] = t4;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+        const uint32_t sum = madd(madd(s[j], s[j + 4]), madd(s[j + 8], s[j + 12]));
+#pragma unroll
+        for (uint32_t b = 0; b < 16; b += 4) s[b + j] = madd(s[b + j], sum);
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++) s[i] = pos_pow5(s[i]);
+}
+__global__ __launch_bounds__(256) void poseidon_trace_kernel(uint32_t log_n, uint32_t* __restrict__ out,
+                                                             uint64_t stride) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t row = t & ((1u << log_n) - 1), rep = t >> log_n;
+    if (rep >= 8) return;
+    uint32_t* o = out + (uint64_t)rep * POS_COLS_PER_REP * stride + row;
+    uint32_t s[16];
+#pragma unroll
+    for (uint32_t i = 0; i < 16; i++) {
+        s[i] = (uint32_t)(((uint64_t)row * 16 + i + rep) % P);
+        o[(uint64_t)i * stride] = s[i];
+    }
+    uint32_t col = 16;
+    for (uint32_t r = 0; r < 4; r++) {
+        pos_external(s, r);
+#pragma unroll
+        for (uint32_t i = 0; i < 16; i++) o[(uint64_t)(col + i) * stride] = s[i];
+        col += 16;
+    }
+    for (uint32_t r = 0; r < 14; r++) {
+        // internal round (input.zig:135-139,203-210): state[0] += c, state[i] = state[i]*2^(i+1) + sum
+        s[0] = madd(s[0], 9876u + r * 17u);
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 16; i++) sum = madd(sum, s[i]);
+#pragma unroll
+        for (uint32_t i = 0; i < 16; i++) s[i] = madd(mmul(s[i], 2u << i), sum);
+        s[0] = pos_pow5(s[0]);
+        o[(uint64_t)col * stride] = s[0];
+        col++;
+    }
+    for (uint32_t r = 4; r < 8; r++) {
+        pos_external(s, r);
+#pragma unroll
+        for (uint32_t i = 0; i < 16; i++) o[(uint64_t)(col + i) * stride] = s[i];
+        col += 16;
+    }
+}
+void launch_poseidon_trace(Ctx& c, uint32_t log_n, uint32_t* d_out, uint64_t col_stride) {
+    SR_CHECK(log_n >= 1 && log_n <= 27, "poseidon trace: log_n_rows out of range");
+    const uint64_t threads = 8ull << log_n;
+    KTimer kt(c, "poseidon_trace", 4.0 * 8 * POS_COLS_PER_REP * (double)(1ull << log_n));
+    hipLaunchKernelGGL(poseidon_trace_kernel, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, c.stream, log_n,
+                       d_out, col_stride);
+    SR_HIP(hipGetLastError());
+}
+
+// A constant composition evaluation (accumulation.zig:142-159 with one constraint, whose
+// coefficient is 1): every row of the four coordinate columns holds v.
+__global__ __launch_bounds__(256) void fill_qm31_kernel(uint32_t n, uint4 v, uint32_t* __restrict__ o0,
+                                                        uint32_t* __restrict__ o1, uint32_t* __restrict__ o2,
+                                                        uint32_t* __restrict__ o3) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    o0[i] = v.x;
+    o1[i] = v.y;
+    o2[i] = v.z;
+    o3[i] = v.w;
+}
+void launch_constant_composition(Ctx& c, uint32_t log_eval, const uint32_t v[4], uint32_t* const* out) {
+    const uint32_t n = 1u << log_eval;
+    hipLaunchKernelGGL(fill_qm31_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, n,
+                       make_uint4(v[0], v[1], v[2], v[3]), out[0], out[1], out[2], out[3]);
+    SR_HIP(hipGetLastError());
+}
+
 // Point evaluation (prover/poly/circle/point_evaluation.zig:14-38,469-488):
 // p(P) = sum_i c_i * w_i, w_i = prod_{bit j of i} f_j, f = [P.y, P.x, 2x^2-1, ...].
 struct Factors {

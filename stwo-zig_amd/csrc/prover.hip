@@ -262,8 +262,11 @@ void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack
 void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trace_in, std::string& proof,
                           ProveStats* stats) {
     const uint32_t n = prm.log_n_rows, W = prm.sequence_len, B = prm.log_blowup_factor;
+    const bool poseidon = prm.air == AIR_POSEIDON;
+    SR_CHECK(prm.air == AIR_WIDE_FIBONACCI || poseidon, "unknown AIR");
     SR_CHECK(n >= 1 && n <= 26, "log_n_rows out of range");
     SR_CHECK(W >= 3, "sequence_len must be >= 3");
+    SR_CHECK(!poseidon || W == POSEIDON_COLUMNS, "the Poseidon AIR has 1264 columns");
     SR_CHECK(B >= 1 && B <= 4, "log_blowup_factor out of range");
     SR_CHECK(prm.n_queries >= 1 && prm.n_queries <= 1024, "n_queries out of range");
     SR_CHECK(prm.log_last_layer_degree_bound <= 10, "log_last_layer_degree_bound out of range");
@@ -348,12 +351,16 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     CommitPlan cp1;
     plan_commit(c, owner, t1, cf, n, B, pack, cp1);
 
-    const uint32_t NC = W - 2, CL = n + 1;
+    // constraints: W - 2 recurrence steps; the Poseidon component has one (poseidon.zig:414-416)
+    const uint32_t NC = poseidon ? 1 : W - 2, CL = n + 1;
     const uint64_t CN = 1ull << CL;
     DBuf ev_own;
     std::vector<const uint32_t*> ev(W);
     size_t at_ev_src = 0, at_ev_dst = 0;
-    if (B == 1) {
+    if (poseidon) {
+        // the composition evaluation reads no trace values
+        for (uint32_t i = 0; i < W; i++) ev[i] = t1.lde[i];
+    } else if (B == 1) {
         for (uint32_t i = 0; i < W; i++) ev[i] = t1.lde[i];
     } else {
         // trace polynomials evaluated on CanonicCoset(n + 1) from their coefficients
@@ -422,19 +429,23 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     uint32_t* d_factors = d_oods + 4 * 2;
 
     // Tree 1: main trace -> coefficients -> LDE -> Merkle.
-    if (!d_trace_in) launch_widefib_trace(c, n, W, prm.trace_mode, prm.trace_seed, trace_own.as<uint32_t>(), N);
+    if (!d_trace_in) {
+        if (poseidon) launch_poseidon_trace(c, n, trace_own.as<uint32_t>(), N);
+        else launch_widefib_trace(c, n, W, prm.trace_mode, prm.trace_seed, trace_own.as<uint32_t>(), N);
+    }
     launch_circle_fft(c, true, (const uint32_t* const*)(d_pack + at_ifft_src), (uint32_t* const*)(d_pack + at_ifft_dst),
                       W, n, n);
     launch_commit(c, t1, cp1, d_pack, d_troots);
     trace_own.reset();
     mark();  // main commit
 
-    // mixRoot(main), mixStatement (wide_fibonacci.zig:496-498), composition alpha and its
-    // powers (prove.zig:237-318; wide_fibonacci/component.zig:146-228), on the device.
+    // mixRoot(main), mixStatement (wide_fibonacci.zig:496-498: log_n_rows, sequence_len;
+    // poseidon.zig:571-573: log_n_instances), composition alpha and its powers
+    // (prove.zig:237-318; wide_fibonacci/component.zig:146-228), on the device.
     DBuf d_pw(&c, 16ull * NC);
     {
-        const uint32_t st[2] = {n, W};
-        launch_tr_commit_draw(c, d_ch, t1.tree->layers, st, 2, d_pw.as<uint32_t>(), NC);
+        const uint32_t st[2] = {poseidon ? n + 3 : n, W};
+        launch_tr_commit_draw(c, d_ch, t1.tree->layers, st, poseidon ? 1 : 2, d_pw.as<uint32_t>(), NC);
     }
     // cosetVanishing(CanonicCoset(n).coset(), domain.at(0/1)) (constraints.zig:16-27).
     uint32_t den_inv[2];
@@ -447,11 +458,18 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         for (uint32_t i = 1; i < n; i++) x = mdouble_x(x);
         den_inv[r] = minv(x);
     }
-    if (B != 1)
-        launch_circle_fft(c, false, (const uint32_t* const*)(d_pack + at_ev_src), (uint32_t* const*)(d_pack + at_ev_dst),
-                          W, CL, n);
-    launch_widefib_composition(c, (const uint32_t* const*)(d_pack + at_ev), W, CL, n, d_pw.as<uint32_t>(), den_inv[0],
-                               den_inv[1], comp_c);
+    if (poseidon) {
+        // compositionEval (poseidon.zig:562-569): (log_n_instances, 158, 1264, 1) times the
+        // constraint's coefficient 1 (accumulation.zig:142-159)
+        const uint32_t cv[4] = {n + 3, POSEIDON_COLUMNS_PER_REP, POSEIDON_COLUMNS, 1};
+        launch_constant_composition(c, CL, cv, comp_c);
+    } else {
+        if (B != 1)
+            launch_circle_fft(c, false, (const uint32_t* const*)(d_pack + at_ev_src),
+                              (uint32_t* const*)(d_pack + at_ev_dst), W, CL, n);
+        launch_widefib_composition(c, (const uint32_t* const*)(d_pack + at_ev), W, CL, n, d_pw.as<uint32_t>(),
+                                   den_inv[0], den_inv[1], comp_c);
+    }
     // interpolate (secure_poly.zig:164-215) in place
     launch_circle_fft(c, true, (const uint32_t* const*)(d_pack + at_cc), (uint32_t* const*)(d_pack + at_cc), 4, CL, CL);
     ev_own.reset();

@@ -90,6 +90,11 @@ void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack
 void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
                          uint32_t coeff_log, uint32_t blowup);
 
+// Components served by the one-trace-tree proof core: the wide-Fibonacci AIR and the Poseidon
+// AIR (examples/poseidon.zig:397-504: 1264 columns, a constant composition evaluation).
+enum : uint32_t { AIR_WIDE_FIBONACCI = 0, AIR_POSEIDON = 1 };
+constexpr uint32_t POSEIDON_COLUMNS = 1264, POSEIDON_COLUMNS_PER_REP = 158;
+
 struct ProveParams {
     uint32_t log_n_rows;
     uint32_t sequence_len;
@@ -99,6 +104,7 @@ struct ProveParams {
     uint32_t n_queries;
     uint32_t trace_mode;
     uint64_t trace_seed;
+    uint32_t air = AIR_WIDE_FIBONACCI;  // the component proven (sequence_len is its column count)
 };
 
 constexpr int STAGES = 8;

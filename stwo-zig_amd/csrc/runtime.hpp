@@ -342,6 +342,10 @@ void launch_widefib_trace(Ctx& c, uint32_t log_n, uint32_t seq_len, uint32_t mod
 void launch_widefib_composition(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_rows_eval,
                                 uint32_t log_n_rows, const uint32_t* d_powers, uint32_t den_inv0,
                                 uint32_t den_inv1, uint32_t* const* out_coords /* host array of 4 */);
+// Poseidon AIR trace: 1264 columns of 2^log_n rows at d_out + column * col_stride.
+void launch_poseidon_trace(Ctx& c, uint32_t log_n, uint32_t* d_out, uint64_t col_stride);
+// Composition evaluation that is one QM31 constant on 2^log_eval rows.
+void launch_constant_composition(Ctx& c, uint32_t log_eval, const uint32_t v[4], uint32_t* const* out_coords);
 // PoW: lowest nonce in [base, base+count) or UINT64_MAX.
 uint64_t launch_pow_grind(Ctx& c, const uint8_t prefix[32], uint32_t bits, uint64_t base, uint64_t count);
 // Gather words: out[i] = word at absolute device address d_addr[i].

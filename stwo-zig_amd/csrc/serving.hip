@@ -29,6 +29,10 @@
 namespace sr {
 
 uint64_t serve_job_bytes(const ServeJob& j) {
+    if (j.kind == 2) {  // Poseidon: 1264 columns of 2^(log_n_instances - 3) rows
+        const uint64_t R = 1ull << (j.log_n - 3);
+        return 16ull * R * POSEIDON_COLUMNS + 1024ull * R + (64ull << 20);
+    }
     const uint64_t N = 1ull << j.log_n;
     if (j.kind == 0) return 16ull * N * j.seq_len + 1024ull * N + (64ull << 20);
     return 1024ull * N + (64ull << 20);
@@ -61,13 +65,14 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
     uint32_t max_log = 0;
     for (size_t i = 0; i < jobs.size(); i++) {
         const ServeJob& j = jobs[i];
-        SR_CHECK(j.kind <= 1, "serve: unknown job kind");
+        SR_CHECK(j.kind <= 2, "serve: unknown job kind");
+        SR_CHECK(j.kind != 2 || (j.log_n >= 4 && j.log_n <= 24), "serve: Poseidon shape");
         SR_CHECK(j.kind != 0 || (j.log_n >= 5 && j.log_n <= 24 && j.seq_len >= 3 && j.seq_len <= 256),
                  "serve: wide-Fibonacci shape");
         SR_CHECK(j.kind != 1 || (j.log_n >= 16 && j.log_n <= 24), "serve: LogUp shape");
         SR_CHECK(i == 0 || jobs[i - 1].arrival_ms <= j.arrival_ms, "serve: arrivals must be sorted");
         SR_CHECK(hbm_budget == 0 || serve_job_bytes(j) <= hbm_budget, "serve: a job exceeds the HBM budget");
-        max_log = std::max(max_log, j.log_n + 2);
+        max_log = std::max(max_log, (j.kind == 2 ? j.log_n - 3 : j.log_n) + 2);
     }
     // Slots: resident contexts (kept in the pool between calls) with the twiddle tower and the
     // pinned staging ready, so no job pays set-up.
@@ -135,6 +140,19 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
                     prm.n_queries = 3;
                     prm.trace_mode = 1;
                     prm.trace_seed = j.seed;
+                    std::string proof;
+                    prove_wide_fibonacci(c, prm, nullptr, proof, nullptr);
+                    b2_hash(proof.data(), proof.size(), o.digest);
+                    o.output_bytes = proof.size();
+                } else if (j.kind == 2) {
+                    ProveParams prm{};
+                    prm.log_n_rows = j.log_n - 3;
+                    prm.sequence_len = POSEIDON_COLUMNS;
+                    prm.pow_bits = 10;
+                    prm.log_blowup_factor = 1;
+                    prm.log_last_layer_degree_bound = 0;
+                    prm.n_queries = 3;
+                    prm.air = AIR_POSEIDON;
                     std::string proof;
                     prove_wide_fibonacci(c, prm, nullptr, proof, nullptr);
                     b2_hash(proof.data(), proof.size(), o.digest);

@@ -156,7 +156,7 @@ __global__ void tr_mix_draws_kernel(DevChannel* __restrict__ ch, const uint32_t*
 // entry i: q^i * (conj(py) - py); sum_a = sum q^i (conj(v_i) - v_i);
 // sum_b = sum q^i (v_i (conj(py) - py) - (conj(v_i) - v_i) py).
 constexpr uint32_t TRQ_THREADS = 256;
-constexpr uint32_t TRQ_MAX_NS = 1024;  // sampled values staged in LDS (16 KiB)
+constexpr uint32_t TRQ_MAX_NS = 2048;  // sampled values staged in LDS (32 KiB)
 __global__ __launch_bounds__(TRQ_THREADS) void tr_quotient_kernel(DevChannel* __restrict__ ch,
                                                                   const uint32_t* __restrict__ sv, uint32_t ns,
                                                                   const uint4* __restrict__ oods,

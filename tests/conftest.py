@@ -39,6 +39,27 @@ def load_goldens():
     return out
 
 
+def load_poseidon_goldens():
+    """Poseidon AIR proofs of the reference CPU backend (benchmark rows log_n_instances 10, 13)."""
+    import json
+
+    with open(os.path.join(GOLDEN_DIR, "poseidon_goldens.json")) as f:
+        idx = json.load(f)
+    out = []
+    for name, e in sorted(idx.items(), key=lambda kv: kv[1]["statement"]["log_n_instances"]):
+        with open(os.path.join(GOLDEN_DIR, e["file"]), "rb") as f:
+            out.append(dict(e, name=name, proof=f.read()))
+    return out
+
+
+def poseidon_kwargs(e):
+    c = e["pcs_config"]
+    return dict(log_n_instances=e["statement"]["log_n_instances"], pow_bits=c["pow_bits"],
+                log_blowup_factor=c["fri_config"]["log_blowup_factor"],
+                log_last_layer_degree_bound=c["fri_config"]["log_last_layer_degree_bound"],
+                n_queries=c["fri_config"]["n_queries"])
+
+
 def golden_kwargs(e):
     c = e["pcs_config"]
     st = e["statement"]

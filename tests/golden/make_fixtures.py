@@ -1,10 +1,11 @@
-"""Extract the reference's committed wide-Fibonacci golden proofs into small fixtures.
+"""Extract the reference's committed wide-Fibonacci and Poseidon golden proofs into small fixtures.
 
 Reads only data files the reference holds (proof artifacts written by its CPU
 backend and cross-verified against pinned Rust Stwo):
   vectors/reports/interop_history/objects/sha256/ac/ac89...json      (5x16, pow 0)
   vectors/reports/benchmark_history/runs/*/bundle/tree/row-*wide*/cpu.proof-artifact.json
   conformance/evidence/metrics-v2/w1-log20x100/proof-artifact.json   (20x100)
+  vectors/reports/benchmark_history/runs/*/bundle/tree/row-*poseidon*/cpu.proof-artifact.json
 Each fixture is the canonical proof byte string (compact JSON wire) plus its
 statement and PCS config.  Run from the repository root:
     python tests/golden/make_fixtures.py /root/reference
@@ -66,6 +67,34 @@ def main(ref):
             "source": os.path.relpath(p, ref),
         }
     with open(os.path.join(HERE, "wide_fibonacci_goldens.json"), "w") as f:
+        json.dump(index, f, indent=1, sort_keys=True)
+    for k, v in sorted(index.items()):
+        print(k, v["bytes"], v["sha256"][:16], v.get("conflicts", ""))
+    poseidon(ref)
+
+
+def poseidon(ref):
+    index = {}
+    for p in sorted(glob.glob(os.path.join(ref, "vectors/reports/benchmark_history/runs/*/bundle/tree/*poseidon*/cpu.proof-artifact.json"))):
+        art = json.load(open(p))
+        if art.get("example") != "poseidon" or "proof_bytes_hex" not in art or art.get("prove_mode") not in (None, "prove"):
+            continue
+        proof = bytes.fromhex(art["proof_bytes_hex"])
+        if json.loads(proof).get("config") != art["pcs_config"]:
+            continue
+        st = art["poseidon_statement"]
+        name = "poseidon_log%d_pow%d" % (st["log_n_instances"], art["pcs_config"]["pow_bits"])
+        sha = hashlib.sha256(proof).hexdigest()
+        if name in index:
+            if index[name]["sha256"] != sha:
+                index[name].setdefault("conflicts", []).append(os.path.relpath(p, ref))
+            continue
+        fname = name + ".proof.json"
+        with open(os.path.join(HERE, fname), "wb") as f:
+            f.write(proof)
+        index[name] = {"file": fname, "statement": st, "pcs_config": art["pcs_config"], "sha256": sha,
+                       "bytes": len(proof), "source": os.path.relpath(p, ref)}
+    with open(os.path.join(HERE, "poseidon_goldens.json"), "w") as f:
         json.dump(index, f, indent=1, sort_keys=True)
     for k, v in sorted(index.items()):
         print(k, v["bytes"], v["sha256"][:16], v.get("conflicts", ""))

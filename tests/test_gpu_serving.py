@@ -1,6 +1,7 @@
 """Continuous-batching service on one GPU: every job's result is the same alone and in the mix,
 wide-Fibonacci proofs in the mix equal the oracle's proofs byte for byte (Blake2s digest of the
-proof bytes), LogUp jobs equal the oracle's LogUp workload words, and the HBM budget bounds the
+proof bytes), Poseidon AIR proofs equal the oracle's, LogUp jobs equal the oracle's LogUp
+workload words, and the HBM budget bounds the
 number of jobs admitted together."""
 import hashlib
 import sys
@@ -20,8 +21,8 @@ def _say(*a):
 
 
 JOBS = [  # (kind, log_n, seq_len, seed, arrival_ms)
-    (0, 10, 16, 101, 0.0), (1, 16, 0, 7, 0.5), (0, 8, 8, 102, 1.0), (0, 12, 32, 103, 1.5),
-    (0, 9, 64, 104, 2.0), (1, 16, 0, 8, 3.0), (0, 11, 24, 105, 3.5), (0, 10, 100, 106, 6.0),
+    (0, 10, 16, 101, 0.0), (1, 16, 0, 7, 0.5), (0, 8, 8, 102, 1.0), (2, 12, 0, 0, 1.2), (0, 12, 32, 103, 1.5),
+    (0, 9, 64, 104, 2.0), (1, 16, 0, 8, 3.0), (2, 14, 0, 0, 3.2), (0, 11, 24, 105, 3.5), (0, 10, 100, 106, 6.0),
 ]
 
 
@@ -30,6 +31,8 @@ def _ref_digest(job):
     if kind == 0:
         proof = oracle.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=seed)
         return hashlib.blake2s(proof).hexdigest()
+    if kind == 2:
+        return hashlib.blake2s(oracle.prove_poseidon(n)).hexdigest()
     words = oracle.logup_xor_pipeline(n, seed)
     return hashlib.blake2s(words.astype(np.uint32).tobytes()).hexdigest()
 

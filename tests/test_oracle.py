@@ -2,7 +2,8 @@
 
 1. End to end: the oracle reproduces, byte for byte, every committed golden
    wide-Fibonacci proof of the reference (5x16 pow0 interop proof, 10x8, 14x32,
-   16x64, 18x100, 20x100 functional).  Each proof pins the trace LDE + Merkle
+   16x64, 18x100, 20x100 functional) and both Poseidon AIR proofs (log_n_instances
+   10 and 13, 1264 columns).  Each proof pins the trace LDE + Merkle
    root, the composition root, sampled values, quotients, all FRI layer roots
    and witnesses, the last layer, the PoW nonce and the decommitments.
 2. Primitives, against closed forms / independent restatements:
@@ -18,9 +19,10 @@ import pytest
 
 import mathref as mr
 import oracle
-from conftest import golden_kwargs, load_goldens
+from conftest import golden_kwargs, load_goldens, load_poseidon_goldens, poseidon_kwargs
 
 GOLDENS = load_goldens()
+POSEIDON_GOLDENS = load_poseidon_goldens()
 
 
 @pytest.mark.parametrize("g", GOLDENS, ids=[g["name"] for g in GOLDENS])
@@ -28,6 +30,37 @@ def test_oracle_reproduces_golden_proof(g):
     proof = oracle.prove_wide_fibonacci(**golden_kwargs(g))
     assert hashlib.sha256(proof).hexdigest() == g["sha256"]
     assert proof == g["proof"]
+
+
+@pytest.mark.parametrize("g", POSEIDON_GOLDENS, ids=[g["name"] for g in POSEIDON_GOLDENS])
+def test_oracle_reproduces_poseidon_golden_proof(g):
+    # src/examples/poseidon.zig: 1264-column trace, constant composition, statement log_n_instances
+    proof = oracle.prove_poseidon(**poseidon_kwargs(g))
+    assert hashlib.sha256(proof).hexdigest() == g["sha256"]
+    assert proof == g["proof"]
+
+
+def test_poseidon_trace_inputs_and_first_round():
+    # input.zig:121-133: instance r of row k starts from the state (16k + i + r)_i; the first
+    # external round of the all-zero-offset state is checked against big-int arithmetic.
+    t = oracle.poseidon_trace(2)
+    assert t.shape == (1264, 4)
+    for rep in range(8):
+        for k in range(4):
+            assert [int(t[158 * rep + i, k]) for i in range(16)] == [16 * k + i + rep for i in range(16)]
+    P = (1 << 31) - 1
+    s = [(i + 1234 + i) % P for i in range(16)]  # row 0, instance 0: state i, round constant 1234 + i
+
+    def m4(x):
+        t0, t1 = x[0] + x[1], x[2] + x[3]
+        t2, t3 = 2 * x[1] + t1, 2 * x[3] + t0
+        t4, t5 = 4 * t1 + t3, 4 * t0 + t2
+        return [t3 + t5, t5, t2 + t4, t4]
+
+    s = sum((m4(s[4 * i:4 * i + 4]) for i in range(4)), [])
+    sums = [s[j] + s[j + 4] + s[j + 8] + s[j + 12] for j in range(4)]
+    s = [pow((s[i] + sums[i % 4]) % P, 5, P) for i in range(16)]
+    assert [int(v) for v in t[16:32, 0]] == s
 
 
 def test_golden_20x100_is_the_receipt_proof():

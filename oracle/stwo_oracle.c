@@ -868,6 +868,26 @@ void ora_poseidon_trace(uint32_t log_n_rows, uint32_t *out) {
 }
 
 /* ======================================================================
+ * Periodic XOR example AIR (src/examples/xor/input.zig:31-107; core/air/utils.zig:23-65):
+ * preprocessed IsFirst and IsStepWithOffset indicators and one main column, each in
+ * bit-reversed circle-domain order.
+ * ==================================================================== */
+static uint64_t circle_bitrev_index(uint32_t log_size, uint64_t coset_index) {
+    uint64_t n = 1ull << log_size;
+    uint64_t ci = (coset_index & 1) == 0 ? coset_index / 2 : (2 * n - coset_index) / 2;
+    return bitrev(ci, log_size);
+}
+void ora_xor_example_columns(uint32_t log_size, uint32_t log_step, uint64_t offset, uint32_t *is_first,
+                             uint32_t *is_step, uint32_t *main_col) {
+    uint64_t n = 1ull << log_size, step = 1ull << log_step;
+    memset(is_first, 0, 4 * n);
+    memset(is_step, 0, 4 * n);
+    is_first[0] = 1;
+    for (uint64_t i = offset % step; i < n; i += step) is_step[circle_bitrev_index(log_size, i)] = 1;
+    for (uint64_t i = 0; i < n; i++) main_col[circle_bitrev_index(log_size, i)] = (i & 1) == 0 ? 1 : 0;
+}
+
+/* ======================================================================
  * Proof assembly
  * ==================================================================== */
 static double g_timings[8];
@@ -1016,11 +1036,15 @@ static int cmp_u64(const void *a, const void *b) {
 /* The AIRs this proof core serves: the wide-Fibonacci component and the Poseidon
  * component (src/examples/poseidon.zig:397-504), whose composition is the constant
  * (log_n_instances, 158, 1264, 1) (poseidon.zig:562-569). */
-enum { AIR_WIDEFIB = 0, AIR_POSEIDON = 1 };
+enum { AIR_WIDEFIB = 0, AIR_POSEIDON = 1, AIR_XOR = 2 };
 
-static int prove_core(int air, const ora_widefib_params *prm, uint8_t *out, size_t cap, size_t *out_len) {
+/* air AIR_XOR: the periodic XOR example (src/examples/xor.zig:419-605): two preprocessed
+ * columns and one main column, statement (log_size, log_step, offset), constant
+ * composition (log_size, log_step, offset, 1); xs holds log_step and offset. */
+static int prove_core(int air, const ora_widefib_params *prm, const uint64_t *xs, uint8_t *out, size_t cap,
+                      size_t *out_len) {
     const uint32_t n = prm->log_n_rows, W = prm->sequence_len, B = prm->log_blowup_factor;
-    if (n < 1 || W < 3 || B < 1) return 1;
+    if (n < 1 || W < 1 || B < 1 || (air == AIR_WIDEFIB && W < 3)) return 1;
     double t0 = now_ms();
     memset(g_timings, 0, sizeof g_timings);
     init_gen_doubles();
@@ -1030,16 +1054,30 @@ static int prove_core(int air, const ora_widefib_params *prm, uint8_t *out, size
     qm31 cfg = {{prm->pow_bits, B}, {prm->n_queries, prm->log_last_layer_degree_bound}};
     ch_mix_felts(&ch, &cfg, 1);
 
-    /* Tree 0: preprocessed, no columns (examples/wide_fibonacci/trace.zig:184-186). */
+    /* Tree 0: preprocessed; no columns for wide-Fib and Poseidon (examples/wide_fibonacci/
+     * trace.zig:184-186), IsFirst and IsStepWithOffset for XOR (xor/input.zig:48-71). */
     ctree trees[3];
     memset(trees, 0, sizeof trees);
-    commit_coeffs(&trees[0], 0, NULL, NULL, B, &ch);
+    uint64_t N = 1ull << n;
+    uint32_t *trace = malloc(4ull * N * (W + 2));
+    if (air == AIR_XOR) {
+        ora_xor_example_columns(n, (uint32_t)xs[0], xs[1], trace + N * W, trace + N * (W + 1), trace);
+        uint32_t **pc = malloc(sizeof(uint32_t *) * 2);
+        uint32_t plogs[2] = {n, n};
+        get_twiddles(n > 2 ? n : 3);
+        for (int k = 0; k < 2; k++) {
+            pc[k] = malloc(4 * N);
+            memcpy(pc[k], trace + N * (W + k), 4 * N);
+            circle_interpolate(pc[k], n);
+        }
+        commit_coeffs(&trees[0], 2, pc, plogs, B, &ch);
+    } else {
+        commit_coeffs(&trees[0], 0, NULL, NULL, B, &ch);
+    }
 
     /* Tree 1: main trace: generate -> interpolate -> LDE -> Merkle (preparation.zig:113-210). */
-    uint64_t N = 1ull << n;
-    uint32_t *trace = malloc(4ull * N * W);
     if (air == AIR_POSEIDON) ora_poseidon_trace(n, trace);
-    else ora_widefib_trace(n, W, prm->trace_mode, prm->trace_seed, trace);
+    else if (air == AIR_WIDEFIB) ora_widefib_trace(n, W, prm->trace_mode, prm->trace_seed, trace);
     double t1 = now_ms();
     uint32_t **tc = malloc(sizeof(uint32_t *) * W);
     uint32_t *tlogs = malloc(4 * W);
@@ -1056,21 +1094,24 @@ static int prove_core(int air, const ora_widefib_params *prm, uint8_t *out, size
     free(tlogs);
     double t2 = now_ms();
 
-    /* mixStatement (wide_fibonacci.zig:496-498; poseidon.zig:571-573: log_n_instances). */
+    /* mixStatement (wide_fibonacci.zig:496-498; poseidon.zig:571-573: log_n_instances;
+     * xor.zig:599-605: log_size, log_step, then mixU64(offset)). */
     uint32_t st[2] = {n, W};
     if (air == AIR_POSEIDON) st[0] = n + 3;
+    if (air == AIR_XOR) st[1] = (uint32_t)xs[0];
     ch_mix_u32s(&ch, st, air == AIR_POSEIDON ? 1 : 2);
+    if (air == AIR_XOR) ch_mix_u64(&ch, xs[1]);
 
     /* prove.zig:209-318: composition. */
     qm31 alpha = ch_draw_secure_felt(&ch);
-    const uint32_t NC = air == AIR_POSEIDON ? 1 : W - 2, CL = n + 1; /* composition log size */
+    const uint32_t NC = air != AIR_WIDEFIB ? 1 : W - 2, CL = n + 1; /* composition log size */
     uint64_t CN = 1ull << CL;
     qm31 *pw = malloc(sizeof(qm31) * NC);
     pw[0] = Q_ONE;
     for (uint32_t k = 1; k < NC; k++) pw[k] = q_mul(pw[k - 1], alpha);
     /* Trace evaluations on CanonicCoset(n+1): the LDE itself when blowup is 1. */
     uint32_t **ev = malloc(sizeof(uint32_t *) * W);
-    int own_ev = (B != 1);
+    int own_ev = (B != 1) && air == AIR_WIDEFIB;
     for (uint32_t c = 0; c < W; c++) {
         if (!own_ev) ev[c] = trees[1].lde[c];
         else ev[c] = malloc(4 * CN);
@@ -1093,10 +1134,15 @@ static int prove_core(int air, const ora_widefib_params *prm, uint8_t *out, size
     }
     uint32_t *comp[4];
     for (int k = 0; k < 4; k++) comp[k] = malloc(4 * CN);
-    if (air == AIR_POSEIDON) {
+    if (air != AIR_WIDEFIB) {
         /* accumulateColumn of a constant column (accumulation.zig:142-159): the one
          * constraint's coefficient is pw[0] = 1. */
-        const uint32_t cv[4] = {n + 3, POS_COLS_PER_REP, POS_COLS, 1};
+        uint32_t cv[4] = {n + 3, POS_COLS_PER_REP, POS_COLS, 1};
+        if (air == AIR_XOR) {
+            cv[0] = n;
+            cv[1] = (uint32_t)xs[0];
+            cv[2] = (uint32_t)(xs[1] % P);
+        }
         for (int k = 0; k < 4; k++)
             for (uint64_t row = 0; row < CN; row++) comp[k][row] = cv[k];
     } else {
@@ -1148,12 +1194,12 @@ static int prove_core(int air, const ora_widefib_params *prm, uint8_t *out, size
     for (int tr = 0; tr < 3; tr++)
         for (uint32_t c = 0; c < trees[tr].n_cols; c++)
             if (trees[tr].log_sizes[c] > L) L = trees[tr].log_sizes[c];
-    uint32_t total_cols = trees[1].n_cols + trees[2].n_cols;
+    uint32_t total_cols = trees[0].n_cols + trees[1].n_cols + trees[2].n_cols;
     qm31 *sv = malloc(sizeof(qm31) * total_cols);
     qpt *spt = malloc(sizeof(qpt) * total_cols);
     {
         size_t at = 0;
-        for (int tr = 1; tr < 3; tr++)
+        for (int tr = 0; tr < 3; tr++)
             for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
                 qpt p = oods;
                 for (uint32_t f = trees[tr].log_sizes[c]; f < L; f++) { /* repeatedDouble */
@@ -1166,7 +1212,7 @@ static int prove_core(int air, const ora_widefib_params *prm, uint8_t *out, size
         /* All wide-Fib columns share one point and one coefficient log size. */
         qm31 *w = eval_weights(spt[0], n);
         at = 0;
-        for (int tr = 1; tr < 3; tr++) {
+        for (int tr = 0; tr < 3; tr++) {
             uint32_t nc = trees[tr].n_cols;
 #pragma omp parallel for schedule(dynamic, 1)
             for (int64_t c = 0; c < (int64_t)nc; c++) sv[at + c] = dot_weights(trees[tr].coeffs[c], w, N);
@@ -1200,7 +1246,7 @@ static int prove_core(int air, const ora_widefib_params *prm, uint8_t *out, size
     uint32_t *qlogs = malloc(4 * total_cols);
     {
         size_t at = 0;
-        for (int tr = 1; tr < 3; tr++)
+        for (int tr = 0; tr < 3; tr++)
             for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
                 qcols[at] = trees[tr].lde[c];
                 qlogs[at++] = trees[tr].log_sizes[c];
@@ -1335,11 +1381,11 @@ static int prove_core(int air, const ora_widefib_params *prm, uint8_t *out, size
         if (tr) by_str(&j, ",");
         json_hash(&j, r);
     }
-    by_str(&j, "],\"sampled_values\":[[]");
+    by_str(&j, "],\"sampled_values\":[");
     {
         size_t at = 0;
-        for (int tr = 1; tr < 3; tr++) {
-            by_str(&j, ",[");
+        for (int tr = 0; tr < 3; tr++) {
+            by_str(&j, tr ? ",[" : "[");
             for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
                 if (c) by_str(&j, ",");
                 by_str(&j, "[");
@@ -1435,7 +1481,19 @@ static int prove_core(int air, const ora_widefib_params *prm, uint8_t *out, size
 }
 
 int ora_prove_wide_fibonacci(const ora_widefib_params *prm, uint8_t *out, size_t cap, size_t *out_len) {
-    return prove_core(AIR_WIDEFIB, prm, out, cap, out_len);
+    return prove_core(AIR_WIDEFIB, prm, NULL, out, cap, out_len);
+}
+
+/* xor.prove (src/examples/xor.zig) for the statement (log_size, log_step, offset); prm
+ * supplies the PCS configuration only. */
+int ora_prove_xor(uint32_t log_size, uint32_t log_step, uint64_t offset, const ora_widefib_params *prm,
+                  uint8_t *out, size_t cap, size_t *out_len) {
+    if (log_size < 1 || log_size > 30 || log_step > log_size) return 1;
+    ora_widefib_params p = *prm;
+    p.log_n_rows = log_size;
+    p.sequence_len = 1;
+    const uint64_t xs[2] = {log_step, offset};
+    return prove_core(AIR_XOR, &p, xs, out, cap, out_len);
 }
 
 /* poseidon.prove (src/examples/poseidon.zig:67-73): 2^(log_n_instances - 3) rows of 1264
@@ -1446,7 +1504,7 @@ int ora_prove_poseidon(uint32_t log_n_instances, const ora_widefib_params *prm, 
     ora_widefib_params p = *prm;
     p.log_n_rows = log_n_instances - 3;
     p.sequence_len = POS_COLS;
-    return prove_core(AIR_POSEIDON, &p, out, cap, out_len);
+    return prove_core(AIR_POSEIDON, &p, NULL, out, cap, out_len);
 }
 
 /* ======================================================================

@@ -99,6 +99,14 @@ void ora_poseidon_trace(uint32_t log_n_rows, uint32_t *out);
 int ora_prove_poseidon(uint32_t log_n_instances, const ora_widefib_params *prm, uint8_t *out, size_t cap,
                        size_t *out_len);
 
+/* Periodic XOR example AIR columns (src/examples/xor/input.zig:31-107), 2^log_size words each. */
+void ora_xor_example_columns(uint32_t log_size, uint32_t log_step, uint64_t offset, uint32_t *is_first,
+                             uint32_t *is_step, uint32_t *main_col);
+
+/* Periodic XOR example AIR proof (src/examples/xor.zig); prm supplies the PCS configuration only. */
+int ora_prove_xor(uint32_t log_size, uint32_t log_step, uint64_t offset, const ora_widefib_params *prm,
+                  uint8_t *out, size_t cap, size_t *out_len);
+
 /* Stage timings of the last ora_prove_wide_fibonacci call (milliseconds). */
 void ora_last_timings(double out[8]);
 

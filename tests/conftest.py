@@ -39,17 +39,34 @@ def load_goldens():
     return out
 
 
-def load_poseidon_goldens():
-    """Poseidon AIR proofs of the reference CPU backend (benchmark rows log_n_instances 10, 13)."""
+def _example_goldens(example):
     import json
 
-    with open(os.path.join(GOLDEN_DIR, "poseidon_goldens.json")) as f:
+    with open(os.path.join(GOLDEN_DIR, example + "_goldens.json")) as f:
         idx = json.load(f)
     out = []
-    for name, e in sorted(idx.items(), key=lambda kv: kv[1]["statement"]["log_n_instances"]):
+    for name, e in sorted(idx.items()):
         with open(os.path.join(GOLDEN_DIR, e["file"]), "rb") as f:
             out.append(dict(e, name=name, proof=f.read()))
     return out
+
+
+def load_poseidon_goldens():
+    """Poseidon AIR proofs of the reference CPU backend (benchmark rows log_n_instances 10, 13)."""
+    return _example_goldens("poseidon")
+
+
+def load_xor_goldens():
+    """Periodic XOR example AIR proofs of the reference CPU backend (log_size 14, 16; step 4, offset 3)."""
+    return _example_goldens("xor")
+
+
+def xor_kwargs(e):
+    c, st = e["pcs_config"], e["statement"]
+    return dict(log_size=st["log_size"], log_step=st["log_step"], offset=st["offset"], pow_bits=c["pow_bits"],
+                log_blowup_factor=c["fri_config"]["log_blowup_factor"],
+                log_last_layer_degree_bound=c["fri_config"]["log_last_layer_degree_bound"],
+                n_queries=c["fri_config"]["n_queries"])
 
 
 def poseidon_kwargs(e):

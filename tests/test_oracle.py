@@ -2,8 +2,8 @@
 
 1. End to end: the oracle reproduces, byte for byte, every committed golden
    wide-Fibonacci proof of the reference (5x16 pow0 interop proof, 10x8, 14x32,
-   16x64, 18x100, 20x100 functional) and both Poseidon AIR proofs (log_n_instances
-   10 and 13, 1264 columns).  Each proof pins the trace LDE + Merkle
+   16x64, 18x100, 20x100 functional), both Poseidon AIR proofs (log_n_instances
+   10 and 13, 1264 columns) and both periodic XOR AIR proofs (log_size 14 and 16).  Each proof pins the trace LDE + Merkle
    root, the composition root, sampled values, quotients, all FRI layer roots
    and witnesses, the last layer, the PoW nonce and the decommitments.
 2. Primitives, against closed forms / independent restatements:
@@ -19,10 +19,12 @@ import pytest
 
 import mathref as mr
 import oracle
-from conftest import golden_kwargs, load_goldens, load_poseidon_goldens, poseidon_kwargs
+from conftest import (golden_kwargs, load_goldens, load_poseidon_goldens, load_xor_goldens, poseidon_kwargs,
+                      xor_kwargs)
 
 GOLDENS = load_goldens()
 POSEIDON_GOLDENS = load_poseidon_goldens()
+XOR_GOLDENS = load_xor_goldens()
 
 
 @pytest.mark.parametrize("g", GOLDENS, ids=[g["name"] for g in GOLDENS])
@@ -38,6 +40,33 @@ def test_oracle_reproduces_poseidon_golden_proof(g):
     proof = oracle.prove_poseidon(**poseidon_kwargs(g))
     assert hashlib.sha256(proof).hexdigest() == g["sha256"]
     assert proof == g["proof"]
+
+
+@pytest.mark.parametrize("g", XOR_GOLDENS, ids=[g["name"] for g in XOR_GOLDENS])
+def test_oracle_reproduces_xor_golden_proof(g):
+    # src/examples/xor.zig: two preprocessed indicator columns (sampled like every column),
+    # one main column, statement mixU32s(log_size, log_step) + mixU64(offset)
+    proof = oracle.prove_xor(**xor_kwargs(g))
+    assert hashlib.sha256(proof).hexdigest() == g["sha256"]
+    assert proof == g["proof"]
+
+
+def test_xor_example_columns_are_indicators():
+    # IsFirst: only storage index 0; IsStepWithOffset: one row in 2^log_step; main: every
+    # even coset row, each at its bit-reversed circle-domain position (a permutation).
+    log, step, off = 6, 2, 3
+    first, stepc, main = oracle.xor_example_columns(log, step, off)
+    n = 1 << log
+    assert first[0] == 1 and first.sum() == 1
+    assert stepc.sum() == n >> step and main.sum() == n // 2
+
+    def pos(i):  # coset index -> bit-reversed circle-domain index, from the definitions
+        ci = i // 2 if i % 2 == 0 else (2 * n - i) // 2
+        return int(format(ci, "0%db" % log)[::-1], 2)
+
+    assert sorted(pos(i) for i in range(n)) == list(range(n))
+    assert {pos(i) for i in range(off % (1 << step), n, 1 << step)} == set(np.nonzero(stepc)[0].tolist())
+    assert {pos(i) for i in range(0, n, 2)} == set(np.nonzero(main)[0].tolist())
 
 
 def test_poseidon_trace_inputs_and_first_round():

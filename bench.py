@@ -317,21 +317,24 @@ def bench_batched(stwo_rocm, args):
 
 
 def serving_requests(n=256, rate_per_s=1500.0, seed=2024):
-    """Seeded request stream: Poisson arrivals at `rate_per_s`; 3 of 5 requests are wide-Fibonacci
-    proofs (log_n_rows 14..20, sequence_len 16..100, seeded uniform trace), 1 of 5 the XOR-lookup
-    LogUp workload (log 16..18), 1 of 5 Poseidon AIR proofs (log_n_instances 12..16)."""
+    """Seeded request stream: Poisson arrivals at `rate_per_s`; half the requests are wide-Fibonacci
+    proofs (log_n_rows 14..20, sequence_len 16..100, seeded uniform trace), the rest periodic XOR
+    AIR proofs (log_size 12..18, log_step 0..4, offset < 2^20), Poseidon AIR proofs
+    (log_n_instances 12..16) and the XOR-lookup LogUp workload (log 16..18), in shares 2:1.5:1.5."""
     rng = np.random.default_rng(seed)
     t = 0.0
     jobs = []
     for i in range(n):
         t += rng.exponential(1000.0 / rate_per_s)
         u = rng.random()
-        if u < 0.6:
+        if u < 0.5:
             jobs.append((0, int(rng.integers(14, 21)), int(rng.integers(16, 101)), int(rng.integers(1, 1 << 40)), t))
-        elif u < 0.8:
-            jobs.append((1, int(rng.integers(16, 19)), 0, int(rng.integers(1, 1 << 40)), t))
-        else:
+        elif u < 0.7:
+            jobs.append((3, int(rng.integers(12, 19)), int(rng.integers(0, 5)), int(rng.integers(0, 1 << 20)), t))
+        elif u < 0.85:
             jobs.append((2, int(rng.integers(12, 17)), 0, 0, t))
+        else:
+            jobs.append((1, int(rng.integers(16, 19)), 0, int(rng.integers(1, 1 << 40)), t))
     return jobs
 
 
@@ -378,18 +381,20 @@ def bench_serving(stwo_rocm, args):
                 oracle.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=seed)
             elif kind == 2:
                 oracle.prove_poseidon(n)
+            elif kind == 3:
+                oracle.prove_xor(n, w, seed)
             else:
                 oracle.logup_xor_pipeline(n, seed)
         el = time.perf_counter() - t0
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         cpu = {"value": round(k / el, 2), "unit": "proofs/s", "cores": threads, "kind": "port",
                "sample": "the first %d requests of the same stream run back to back by the C oracle" % k}
-    line = {"metric": "serving proofs/s (256 Poisson requests, wide-Fib 14..20 x 16..100 + LogUp 16..18 + Poseidon 12..16) + p50/p99",
+    line = {"metric": "serving proofs/s (256 Poisson requests, wide-Fib 14..20 x 16..100 + XOR 12..18 + Poseidon 12..16 + LogUp 16..18) + p50/p99",
             "value": runs[best]["jobs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": len(jobs),
             "warmup": 8, "ms_per_step": round(runs[best]["makespan_ms"] / len(jobs), 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 (exact M31/QM31)",
             "data": "synthetic: seeded request stream and seeded traces",
-            "config": {"workload": "serving: 256 requests, Poisson 1500/s, 60% wide-Fib, 20% LogUp, 20% Poseidon",
+            "config": {"workload": "serving: 256 requests, Poisson 1500/s, 50% wide-Fib, 20% XOR AIR, 15% Poseidon, 15% LogUp",
                        "device": ident, "slots": best},
             "p50_ms": runs[best]["p50_ms"], "p99_ms": runs[best]["p99_ms"],
             "unbatched_one_slot": runs[1], "by_slots": runs,

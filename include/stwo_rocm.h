@@ -205,6 +205,24 @@ typedef struct {
 int stwo_rocm_prove_poseidon(stwo_rocm_ctx* ctx, const stwo_rocm_poseidon_params* params, uint8_t* proof_out,
                              uint64_t cap, uint64_t* proof_len, stwo_rocm_prove_stats* stats);
 
+/* Periodic XOR example AIR statement and PCS configuration (examples/xor/input.zig:12-16). */
+typedef struct {
+    uint32_t log_size; /* 1..26 */
+    uint32_t log_step; /* <= log_size */
+    uint64_t offset;
+    uint32_t pow_bits;
+    uint32_t log_blowup_factor;
+    uint32_t log_last_layer_degree_bound;
+    uint32_t n_queries;
+} stwo_rocm_xor_params;
+
+/* xor.prove (examples/xor.zig -> prover_transaction.zig -> prover/prove.zig): the IsFirst and
+ * IsStepWithOffset indicators (preprocessed tree) and the main column of xor/input.zig:31-107
+ * generated on the device, the constant composition of xor.zig:590-597, every column sampled
+ * at the OODS point.  Same output, buffer and stats conventions. */
+int stwo_rocm_prove_xor(stwo_rocm_ctx* ctx, const stwo_rocm_xor_params* params, uint8_t* proof_out, uint64_t cap,
+                        uint64_t* proof_len, stwo_rocm_prove_stats* stats);
+
 /* ------------------------------------------------------------------ LogUp / GKR */
 /* Layer kinds of src/prover/lookups/gkr_circuit.zig:79-92. */
 #define STWO_ROCM_GKR_GRAND_PRODUCT 0
@@ -312,7 +330,9 @@ int stwo_rocm_batched_destroy(stwo_rocm_ctx* ctx, stwo_rocm_batched* b);
  * wide-Fibonacci proof (functional protocol, seeded uniform trace: trace_mode 1 of
  * stwo_rocm_prove_wide_fibonacci), 1 XOR-lookup LogUp workload (stwo_rocm_logup_xor_pipeline,
  * log_n 16..24), 2 Poseidon AIR proof (stwo_rocm_prove_poseidon, log_n = log_n_instances 4..24,
- * functional protocol).  Each result carries the Blake2s-256 digest of its output bytes. */
+ * functional protocol), 3 periodic XOR AIR proof (stwo_rocm_prove_xor, log_n = log_size 2..24,
+ * seq_len = log_step, seed = offset, functional protocol).  Each result carries the Blake2s-256
+ * digest of its output bytes. */
 typedef struct {
     uint32_t kind, log_n, seq_len, reserved;
     uint64_t seed;

@@ -603,6 +603,28 @@ int stwo_rocm_prove_poseidon(stwo_rocm_ctx* ctx, const stwo_rocm_poseidon_params
     });
 }
 
+int stwo_rocm_prove_xor(stwo_rocm_ctx* ctx, const stwo_rocm_xor_params* params, uint8_t* proof_out, uint64_t cap,
+                        uint64_t* proof_len, stwo_rocm_prove_stats* stats) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(params && proof_len, "null argument");
+        SR_CHECK(params->log_size >= 1 && params->log_size <= 26, "log_size must be 1..26");
+        SR_CHECK(params->log_step <= params->log_size, "log_step must not exceed log_size");
+        ProveParams p;
+        p.log_n_rows = params->log_size;
+        p.sequence_len = 1;
+        p.pow_bits = params->pow_bits;
+        p.log_blowup_factor = params->log_blowup_factor;
+        p.log_last_layer_degree_bound = params->log_last_layer_degree_bound;
+        p.n_queries = params->n_queries;
+        p.trace_mode = 0;
+        p.trace_seed = 0;
+        p.air = AIR_XOR;
+        p.xor_log_step = params->log_step;
+        p.xor_offset = params->offset;
+        prove_out(c, p, nullptr, proof_out, cap, proof_len, stats);
+    });
+}
+
 // ---------------------------------------------------------------- LogUp / GKR
 int stwo_rocm_xor_trace(stwo_rocm_ctx* ctx, uint32_t log_size, uint64_t seed, uint32_t* d_a, uint32_t* d_b,
                         uint32_t* d_c, uint32_t* d_mult) {

@@ -330,6 +330,32 @@ void launch_poseidon_trace(Ctx& c, uint32_t log_n, uint32_t* d_out, uint64_t col
     SR_HIP(hipGetLastError());
 }
 
+// Periodic XOR example AIR columns (examples/xor/input.zig:31-107; core/air/utils.zig:23-65):
+// thread i takes coset row i to its bit-reversed circle-domain position (a permutation, so
+// every word of main and is_step is written once) and writes storage word i of is_first.
+__global__ __launch_bounds__(256) void xor_example_columns_kernel(uint32_t log_size, uint32_t step_mask,
+                                                                  uint32_t offset_mod, uint32_t* __restrict__ main_col,
+                                                                  uint32_t* __restrict__ is_first,
+                                                                  uint32_t* __restrict__ is_step) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = 1u << log_size;
+    if (i >= n) return;
+    const uint32_t ci = (i & 1) == 0 ? i / 2 : (2 * n - i) / 2;
+    const uint32_t pos = bitrev32(ci, log_size);
+    main_col[pos] = (i & 1) == 0 ? 1u : 0u;
+    is_step[pos] = (i & step_mask) == offset_mod ? 1u : 0u;
+    is_first[i] = i == 0 ? 1u : 0u;
+}
+void launch_xor_example_columns(Ctx& c, uint32_t log_size, uint32_t log_step, uint64_t offset, uint32_t* d_main,
+                                uint32_t* d_is_first, uint32_t* d_is_step) {
+    SR_CHECK(log_size >= 1 && log_size <= 30 && log_step <= log_size, "xor columns: log sizes");
+    const uint32_t n = 1u << log_size;
+    const uint32_t mask = (uint32_t)((1ull << log_step) - 1);
+    hipLaunchKernelGGL(xor_example_columns_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, log_size, mask,
+                       (uint32_t)(offset & mask), d_main, d_is_first, d_is_step);
+    SR_HIP(hipGetLastError());
+}
+
 // A constant composition evaluation (accumulation.zig:142-159 with one constraint, whose
 // coefficient is 1): every row of the four coordinate columns holds v.
 __global__ __launch_bounds__(256) void fill_qm31_kernel(uint32_t n, uint4 v, uint32_t* __restrict__ o0,

@@ -262,11 +262,12 @@ void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack
 void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trace_in, std::string& proof,
                           ProveStats* stats) {
     const uint32_t n = prm.log_n_rows, W = prm.sequence_len, B = prm.log_blowup_factor;
-    const bool poseidon = prm.air == AIR_POSEIDON;
-    SR_CHECK(prm.air == AIR_WIDE_FIBONACCI || poseidon, "unknown AIR");
+    const bool poseidon = prm.air == AIR_POSEIDON, xr = prm.air == AIR_XOR;
+    SR_CHECK(prm.air == AIR_WIDE_FIBONACCI || poseidon || xr, "unknown AIR");
     SR_CHECK(n >= 1 && n <= 26, "log_n_rows out of range");
-    SR_CHECK(W >= 3, "sequence_len must be >= 3");
+    SR_CHECK(xr || W >= 3, "sequence_len must be >= 3");
     SR_CHECK(!poseidon || W == POSEIDON_COLUMNS, "the Poseidon AIR has 1264 columns");
+    SR_CHECK(!xr || (W == 1 && prm.xor_log_step <= n), "XOR AIR: one main column, log_step <= log_size");
     SR_CHECK(B >= 1 && B <= 4, "log_blowup_factor out of range");
     SR_CHECK(prm.n_queries >= 1 && prm.n_queries <= 1024, "n_queries out of range");
     SR_CHECK(prm.log_last_layer_degree_bound <= 10, "log_last_layer_degree_bound out of range");
@@ -296,15 +297,16 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     qm31 cfg = {{prm.pow_bits, B}, {prm.n_queries, prm.log_last_layer_degree_bound}};
     ch.mix_felts(&cfg, 1);
 
-    // Tree 0: preprocessed, empty (examples/wide_fibonacci/trace.zig:184-186): a single leaf
-    // Blake2s(prefix) (leaves.zig:30-34), of which only the root is used.
+    // Tree 0: preprocessed.  Empty for wide-Fib and Poseidon (examples/wide_fibonacci/
+    // trace.zig:184-186): a single leaf Blake2s(prefix) (leaves.zig:30-34), of which only the
+    // root is used; for XOR the two indicator columns, committed and mixed on the device.
     Committed t0;
-    t0.tree = owner.make(0);
-    {
+    if (!xr) {
+        t0.tree = owner.make(0);
         uint8_t prefix[64] = {'l', 'e', 'a', 'f'};
         b2_hash(prefix, 64, t0.tree->root);
+        ch.mix_root(t0.tree->root);
     }
-    ch.mix_root(t0.tree->root);
 
     // From here to the FRI readback the transcript runs on a device copy of the channel:
     // the launches that need a drawn value read it from HBM, and the host reads the
@@ -315,10 +317,12 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     // of all of it goes up in one copy before the first launch.
     const uint32_t last_log = prm.log_last_layer_degree_bound + B;
     const uint32_t n_inner = L - 1 - last_log;  // inner FRI layers log L-1 .. last_log+1
-    const uint32_t NS = W + 8;                  // sampled columns: trace (W) and composition (8)
+    const uint32_t NP = xr ? 2 : 0;             // preprocessed columns
+    const uint32_t NS = NP + W + 8;             // sampled columns: preprocessed, trace (W), composition (8)
     const uint64_t lsz = 1ull << last_log;
     const size_t fbytes = CH_BYTES + 32 * (1 + (size_t)n_inner);
-    const size_t at_line = fbytes, at_troots = at_line + 16 * lsz, at_sv = at_troots + 64;
+    // roots of trees 1, 2 and (XOR) 0
+    const size_t at_line = fbytes, at_troots = at_line + 16 * lsz, at_sv = at_troots + 96;
     const size_t rb_bytes = at_sv + 16 * (size_t)NS;
     HostPack pack;
     {
@@ -349,15 +353,32 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     }
     Committed t1;
     CommitPlan cp1;
+    CommitPlan cp0;
+    DBuf pre_vals, pre_coeffs;
+    size_t at_pre_src = 0, at_pre_dst = 0, at_stmt = 0;
+    if (xr) {
+        pre_vals = DBuf(&c, 2 * N * 4);
+        pre_coeffs = DBuf(&c, 2 * N * 4);
+        const uint32_t* ps[2] = {pre_vals.as<uint32_t>(), pre_vals.as<uint32_t>() + N};
+        std::vector<const uint32_t*> pc = {pre_coeffs.as<uint32_t>(), pre_coeffs.as<uint32_t>() + N};
+        at_pre_src = pack.add(ps, 2);
+        at_pre_dst = pack.add(pc.data(), 2);
+        plan_commit(c, owner, t0, pc, n, B, pack, cp0);
+        // mixStatement (xor.zig:599-605): mixU32s(log_size, log_step), then mixU64(offset)
+        const uint32_t stw[4] = {n, prm.xor_log_step, (uint32_t)prm.xor_offset, (uint32_t)(prm.xor_offset >> 32)};
+        at_stmt = pack.add(stw, 4);
+    }
     plan_commit(c, owner, t1, cf, n, B, pack, cp1);
 
-    // constraints: W - 2 recurrence steps; the Poseidon component has one (poseidon.zig:414-416)
-    const uint32_t NC = poseidon ? 1 : W - 2, CL = n + 1;
+    // constraints: W - 2 recurrence steps; the Poseidon and XOR components have one
+    // (poseidon.zig:414-416, xor.zig:439-441)
+    const uint32_t NC = (poseidon || xr) ? 1 : W - 2, CL = n + 1;
+    SR_CHECK(NC >= 1 && NC < W + 2, "constraint count");
     const uint64_t CN = 1ull << CL;
     DBuf ev_own;
     std::vector<const uint32_t*> ev(W);
     size_t at_ev_src = 0, at_ev_dst = 0;
-    if (poseidon) {
+    if (poseidon || xr) {
         // the composition evaluation reads no trace values
         for (uint32_t i = 0; i < W; i++) ev[i] = t1.lde[i];
     } else if (B == 1) {
@@ -395,7 +416,7 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     std::vector<const uint32_t*> all_coeffs;
     std::vector<uint32_t*> all_lde;
     std::vector<uint32_t> all_lde_logs;
-    for (int tr = 1; tr < 3; tr++)
+    for (int tr = 0; tr < 3; tr++)
         for (size_t i = 0; i < trees[tr]->coeffs.size(); i++) {
             all_coeffs.push_back(trees[tr]->coeffs[i]);
             all_lde.push_back(trees[tr]->lde[i]);
@@ -428,10 +449,20 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     uint32_t* d_oods = (uint32_t*)(d_pack + at_trs);
     uint32_t* d_factors = d_oods + 4 * 2;
 
+    // XOR: tree 0 from the indicator columns, mixRoot on the device.
+    if (xr) {
+        SR_CHECK(!d_trace_in, "XOR AIR: the trace is generated on the device");
+        launch_xor_example_columns(c, n, prm.xor_log_step, prm.xor_offset, trace_own.as<uint32_t>(),
+                                   pre_vals.as<uint32_t>(), pre_vals.as<uint32_t>() + N);
+        launch_circle_fft(c, true, (const uint32_t* const*)(d_pack + at_pre_src),
+                          (uint32_t* const*)(d_pack + at_pre_dst), 2, n, n);
+        launch_commit(c, t0, cp0, d_pack, d_troots + 64);
+        launch_tr_mix_draws(c, d_ch, (const uint32_t*)(d_troots + 64), 8, 0, nullptr);
+    }
     // Tree 1: main trace -> coefficients -> LDE -> Merkle.
     if (!d_trace_in) {
         if (poseidon) launch_poseidon_trace(c, n, trace_own.as<uint32_t>(), N);
-        else launch_widefib_trace(c, n, W, prm.trace_mode, prm.trace_seed, trace_own.as<uint32_t>(), N);
+        else if (!xr) launch_widefib_trace(c, n, W, prm.trace_mode, prm.trace_seed, trace_own.as<uint32_t>(), N);
     }
     launch_circle_fft(c, true, (const uint32_t* const*)(d_pack + at_ifft_src), (uint32_t* const*)(d_pack + at_ifft_dst),
                       W, n, n);
@@ -443,7 +474,12 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     // poseidon.zig:571-573: log_n_instances), composition alpha and its powers
     // (prove.zig:237-318; wide_fibonacci/component.zig:146-228), on the device.
     DBuf d_pw(&c, 16ull * NC);
-    {
+    if (xr) {
+        // the drawn coefficient is unused: the one constraint's power is 1
+        launch_tr_mix_draws(c, d_ch, (const uint32_t*)t1.tree->layers, 8, 0, nullptr);
+        launch_tr_mix_draws(c, d_ch, (const uint32_t*)(d_pack + at_stmt), 2, 0, nullptr);
+        launch_tr_mix_draws(c, d_ch, (const uint32_t*)(d_pack + at_stmt) + 2, 2, 1, d_pw.as<uint32_t>());
+    } else {
         const uint32_t st[2] = {poseidon ? n + 3 : n, W};
         launch_tr_commit_draw(c, d_ch, t1.tree->layers, st, poseidon ? 1 : 2, d_pw.as<uint32_t>(), NC);
     }
@@ -458,10 +494,16 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         for (uint32_t i = 1; i < n; i++) x = mdouble_x(x);
         den_inv[r] = minv(x);
     }
-    if (poseidon) {
-        // compositionEval (poseidon.zig:562-569): (log_n_instances, 158, 1264, 1) times the
-        // constraint's coefficient 1 (accumulation.zig:142-159)
-        const uint32_t cv[4] = {n + 3, POSEIDON_COLUMNS_PER_REP, POSEIDON_COLUMNS, 1};
+    if (poseidon || xr) {
+        // compositionEval (poseidon.zig:562-569: (log_n_instances, 158, 1264, 1); xor.zig:590-597:
+        // (log_size, log_step, offset, 1)) times the constraint's coefficient 1
+        // (accumulation.zig:142-159)
+        uint32_t cv[4] = {n + 3, POSEIDON_COLUMNS_PER_REP, POSEIDON_COLUMNS, 1};
+        if (xr) {
+            cv[0] = n;
+            cv[1] = prm.xor_log_step;
+            cv[2] = (uint32_t)(prm.xor_offset % P);
+        }
         launch_constant_composition(c, CL, cv, comp_c);
     } else {
         if (B != 1)
@@ -565,6 +607,7 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         c.sync();
         memcpy(t1.tree->root, h + at_troots, 32);
         memcpy(t2.tree->root, h + at_troots + 32, 32);
+        if (xr) memcpy(t0.tree->root, h + at_troots + 64, 32);
         memcpy(sv.data(), h + at_sv, 16 * (size_t)NS);
         DevChannel hc;
         memcpy(&hc, h, sizeof(hc));
@@ -652,12 +695,14 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
 
     // Decommitment plan: everything read from HBM in one gather.
     Gather g;
-    // trees 1, 2: queried values (scheme_decommit.zig; decommit.zig:77-89) and witnesses.
+    // queried values (scheme_decommit.zig; decommit.zig:77-89) and witnesses of every tree
+    // with columns.
     size_t qv_at[3] = {0, 0, 0};
     std::vector<std::pair<uint32_t, uint64_t>> tw_pos[3];
     std::vector<size_t> tw_at[3];
-    for (int tr = 1; tr < 3; tr++) {
+    for (int tr = 0; tr < 3; tr++) {
         const Committed& T = *trees[tr];
+        if (T.lde.empty()) continue;
         qv_at[tr] = g.addr.size();
         for (size_t col = 0; col < T.lde.size(); col++)
             for (size_t q = 0; q < nq; q++) {
@@ -739,11 +784,11 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         if (tr) j.raw(",");
         j.hash(trees[tr]->tree->root);
     }
-    j.raw("],\"sampled_values\":[[]");
+    j.raw("],\"sampled_values\":[");
     {
         size_t at = 0;
-        for (int tr = 1; tr < 3; tr++) {
-            j.raw(",[");
+        for (int tr = 0; tr < 3; tr++) {
+            j.raw(tr ? ",[" : "[");
             for (size_t col = 0; col < trees[tr]->lde.size(); col++) {
                 if (col) j.raw(",");
                 j.raw("[");
@@ -753,18 +798,18 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
             j.raw("]");
         }
     }
-    j.raw("],\"decommitments\":[{\"hash_witness\":[]}");
-    for (int tr = 1; tr < 3; tr++) {
-        j.raw(",{\"hash_witness\":[");
+    j.raw("],\"decommitments\":[");
+    for (int tr = 0; tr < 3; tr++) {
+        j.raw(tr ? ",{\"hash_witness\":[" : "{\"hash_witness\":[");
         for (size_t k = 0; k < tw_at[tr].size(); k++) {
             if (k) j.raw(",");
             j.hash((const uint8_t*)&gw[tw_at[tr][k]]);
         }
         j.raw("]}");
     }
-    j.raw("],\"queried_values\":[[]");
-    for (int tr = 1; tr < 3; tr++) {
-        j.raw(",[");
+    j.raw("],\"queried_values\":[");
+    for (int tr = 0; tr < 3; tr++) {
+        j.raw(tr ? ",[" : "[");
         size_t at = qv_at[tr];
         for (size_t col = 0; col < trees[tr]->lde.size(); col++) {
             if (col) j.raw(",");

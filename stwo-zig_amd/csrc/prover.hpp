@@ -92,7 +92,9 @@ void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out, const std::ve
 
 // Components served by the one-trace-tree proof core: the wide-Fibonacci AIR and the Poseidon
 // AIR (examples/poseidon.zig:397-504: 1264 columns, a constant composition evaluation).
-enum : uint32_t { AIR_WIDE_FIBONACCI = 0, AIR_POSEIDON = 1 };
+// The periodic XOR example AIR (examples/xor.zig:419-605) adds a preprocessed tree of two
+// indicator columns to one main column.
+enum : uint32_t { AIR_WIDE_FIBONACCI = 0, AIR_POSEIDON = 1, AIR_XOR = 2 };
 constexpr uint32_t POSEIDON_COLUMNS = 1264, POSEIDON_COLUMNS_PER_REP = 158;
 
 struct ProveParams {
@@ -105,6 +107,8 @@ struct ProveParams {
     uint32_t trace_mode;
     uint64_t trace_seed;
     uint32_t air = AIR_WIDE_FIBONACCI;  // the component proven (sequence_len is its column count)
+    uint32_t xor_log_step = 0;          // XOR statement (log_size is log_n_rows)
+    uint64_t xor_offset = 0;
 };
 
 constexpr int STAGES = 8;

@@ -344,6 +344,9 @@ void launch_widefib_composition(Ctx& c, const uint32_t* const* d_cols, uint32_t 
                                 uint32_t den_inv1, uint32_t* const* out_coords /* host array of 4 */);
 // Poseidon AIR trace: 1264 columns of 2^log_n rows at d_out + column * col_stride.
 void launch_poseidon_trace(Ctx& c, uint32_t log_n, uint32_t* d_out, uint64_t col_stride);
+// Periodic XOR example AIR: main column and the IsFirst / IsStepWithOffset indicators.
+void launch_xor_example_columns(Ctx& c, uint32_t log_size, uint32_t log_step, uint64_t offset, uint32_t* d_main,
+                                uint32_t* d_is_first, uint32_t* d_is_step);
 // Composition evaluation that is one QM31 constant on 2^log_eval rows.
 void launch_constant_composition(Ctx& c, uint32_t log_eval, const uint32_t v[4], uint32_t* const* out_coords);
 // PoW: lowest nonce in [base, base+count) or UINT64_MAX.

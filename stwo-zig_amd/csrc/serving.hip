@@ -35,6 +35,7 @@ uint64_t serve_job_bytes(const ServeJob& j) {
     }
     const uint64_t N = 1ull << j.log_n;
     if (j.kind == 0) return 16ull * N * j.seq_len + 1024ull * N + (64ull << 20);
+    if (j.kind == 3) return 48ull * N + 1024ull * N + (64ull << 20);  // 3 columns
     return 1024ull * N + (64ull << 20);
 }
 
@@ -65,7 +66,8 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
     uint32_t max_log = 0;
     for (size_t i = 0; i < jobs.size(); i++) {
         const ServeJob& j = jobs[i];
-        SR_CHECK(j.kind <= 2, "serve: unknown job kind");
+        SR_CHECK(j.kind <= 3, "serve: unknown job kind");
+        SR_CHECK(j.kind != 3 || (j.log_n >= 2 && j.log_n <= 24 && j.seq_len <= j.log_n), "serve: XOR shape");
         SR_CHECK(j.kind != 2 || (j.log_n >= 4 && j.log_n <= 24), "serve: Poseidon shape");
         SR_CHECK(j.kind != 0 || (j.log_n >= 5 && j.log_n <= 24 && j.seq_len >= 3 && j.seq_len <= 256),
                  "serve: wide-Fibonacci shape");
@@ -140,6 +142,21 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
                     prm.n_queries = 3;
                     prm.trace_mode = 1;
                     prm.trace_seed = j.seed;
+                    std::string proof;
+                    prove_wide_fibonacci(c, prm, nullptr, proof, nullptr);
+                    b2_hash(proof.data(), proof.size(), o.digest);
+                    o.output_bytes = proof.size();
+                } else if (j.kind == 3) {
+                    ProveParams prm{};
+                    prm.log_n_rows = j.log_n;
+                    prm.sequence_len = 1;
+                    prm.pow_bits = 10;
+                    prm.log_blowup_factor = 1;
+                    prm.log_last_layer_degree_bound = 0;
+                    prm.n_queries = 3;
+                    prm.air = AIR_XOR;
+                    prm.xor_log_step = j.seq_len;
+                    prm.xor_offset = j.seed;
                     std::string proof;
                     prove_wide_fibonacci(c, prm, nullptr, proof, nullptr);
                     b2_hash(proof.data(), proof.size(), o.digest);

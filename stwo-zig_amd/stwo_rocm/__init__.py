@@ -53,6 +53,18 @@ class PoseidonParams(ctypes.Structure):
     ]
 
 
+class XorParams(ctypes.Structure):
+    _fields_ = [
+        ("log_size", ctypes.c_uint32),
+        ("log_step", ctypes.c_uint32),
+        ("offset", ctypes.c_uint64),
+        ("pow_bits", ctypes.c_uint32),
+        ("log_blowup_factor", ctypes.c_uint32),
+        ("log_last_layer_degree_bound", ctypes.c_uint32),
+        ("n_queries", ctypes.c_uint32),
+    ]
+
+
 class ProveStats(ctypes.Structure):
     _fields_ = [
         ("stage_gpu_ms", ctypes.c_double * 8),
@@ -121,6 +133,8 @@ _SIGNATURES = [
     ("stwo_rocm_widefib_composition", ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32,
                                                       _u32p, ctypes.POINTER(_vp)]),
     ("stwo_rocm_pow_grind", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
+    ("stwo_rocm_prove_xor", ctypes.c_int, [_vp, ctypes.POINTER(XorParams), ctypes.c_char_p, ctypes.c_uint64,
+                                            ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ProveStats)]),
     ("stwo_rocm_prove_poseidon", ctypes.c_int, [_vp, ctypes.POINTER(PoseidonParams), ctypes.c_char_p,
                                                  ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
                                                  ctypes.POINTER(ProveStats)]),
@@ -462,6 +476,24 @@ class Context:
             buf = ctypes.create_string_buffer(cap)
             rc = self.lib.stwo_rocm_prove_poseidon(self.h, ctypes.byref(prm), buf, cap, ctypes.byref(n),
                                                    ctypes.byref(st) if st is not None else None)
+            if rc == 1 and n.value > cap:
+                cap = n.value
+                continue
+            self._check(rc)
+            return buf.raw[: n.value], (st.as_dict() if st is not None else None)
+
+    def prove_xor(self, log_size, log_step, offset, pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0,
+                  n_queries=3, stats=True):
+        """Periodic XOR example AIR proof (examples/xor.zig): (proof bytes, stage timings or None)."""
+        prm = XorParams(log_size, log_step, offset, pow_bits, log_blowup_factor, log_last_layer_degree_bound,
+                        n_queries)
+        st = ProveStats() if stats else None
+        cap = 1 << 20
+        n = ctypes.c_uint64()
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = self.lib.stwo_rocm_prove_xor(self.h, ctypes.byref(prm), buf, cap, ctypes.byref(n),
+                                              ctypes.byref(st) if st is not None else None)
             if rc == 1 and n.value > cap:
                 cap = n.value
                 continue

@@ -1,6 +1,6 @@
 """Continuous-batching service on one GPU: every job's result is the same alone and in the mix,
 wide-Fibonacci proofs in the mix equal the oracle's proofs byte for byte (Blake2s digest of the
-proof bytes), Poseidon AIR proofs equal the oracle's, LogUp jobs equal the oracle's LogUp
+proof bytes), Poseidon and periodic XOR AIR proofs equal the oracle's, LogUp jobs equal the oracle's LogUp
 workload words, and the HBM budget bounds the
 number of jobs admitted together."""
 import hashlib
@@ -22,7 +22,8 @@ def _say(*a):
 
 JOBS = [  # (kind, log_n, seq_len, seed, arrival_ms)
     (0, 10, 16, 101, 0.0), (1, 16, 0, 7, 0.5), (0, 8, 8, 102, 1.0), (2, 12, 0, 0, 1.2), (0, 12, 32, 103, 1.5),
-    (0, 9, 64, 104, 2.0), (1, 16, 0, 8, 3.0), (2, 14, 0, 0, 3.2), (0, 11, 24, 105, 3.5), (0, 10, 100, 106, 6.0),
+    (0, 9, 64, 104, 2.0), (1, 16, 0, 8, 3.0), (2, 14, 0, 0, 3.2), (3, 14, 2, 3, 3.3), (0, 11, 24, 105, 3.5),
+    (3, 12, 4, 9, 4.0), (0, 10, 100, 106, 6.0),
 ]
 
 
@@ -33,6 +34,8 @@ def _ref_digest(job):
         return hashlib.blake2s(proof).hexdigest()
     if kind == 2:
         return hashlib.blake2s(oracle.prove_poseidon(n)).hexdigest()
+    if kind == 3:
+        return hashlib.blake2s(oracle.prove_xor(n, w, seed)).hexdigest()
     words = oracle.logup_xor_pipeline(n, seed)
     return hashlib.blake2s(words.astype(np.uint32).tobytes()).hexdigest()
 

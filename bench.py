@@ -103,6 +103,7 @@ def main():
 
     # Timed proofs: events only around the dominant kernel's launches.
     ctx.profile_select(dom)
+    ctx.profile_sample(_sample_every(prof[dom]["launches"] / prof_steps))
     ctx.profile_enable(True)
     ctx.synchronize()
     times, shas = [], set()
@@ -190,6 +191,17 @@ def _dominant(prof, steps):
     return dom, dv["ms"] / dv["launches"], dv["bytes"] / dv["launches"], kernels
 
 
+def _sample_every(launches_per_step):
+    """Stride for the timed run's events on the dominant class: about four bracketed launches
+    per step (each event costs a few microseconds of GPU time), the stride chosen not to divide
+    the launches of a step so that the sampled launches rotate over the step's launches."""
+    n = int(round(launches_per_step))
+    k = max(1, round(n / 4))
+    if k > 1 and n % k == 0:
+        k += 1
+    return k
+
+
 def _roofline(dom, avg_ms, bytes_per_launch, traffic):
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     return {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -214,6 +226,7 @@ def bench_logup(stwo_rocm, args):
     ctx.profile_enable(False)
     dom, _, _, kernels = _dominant(prof, prof_steps)
     ctx.profile_select(dom)
+    ctx.profile_sample(_sample_every(kernels[dom]["launches_per_step"]))
     ctx.profile_enable(True)
     times = []
     shas = set()

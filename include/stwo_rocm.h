@@ -360,9 +360,11 @@ int stwo_rocm_serve(stwo_rocm_ctx* ctx, const stwo_rocm_serve_job* jobs, uint32_
  * {class: {launches, ms, max_ms, bytes}} (bytes = algorithmic HBM bytes of the launches)
  * and clears the records.  profile_select restricts the records to one kernel class
  * (NULL or "" = every class), so a timed run can carry events around its dominant
- * kernel only. */
+ * kernel only; profile_sample records only every k-th matching launch (k >= 1, default 1),
+ * for a class launched many times per run. */
 int stwo_rocm_profile_enable(stwo_rocm_ctx* ctx, int enable);
 int stwo_rocm_profile_select(stwo_rocm_ctx* ctx, const char* kernel_class);
+int stwo_rocm_profile_sample(stwo_rocm_ctx* ctx, uint32_t every);
 int stwo_rocm_profile_read(stwo_rocm_ctx* ctx, char* json_out, size_t cap, size_t* len);
 
 #ifdef __cplusplus

@@ -870,6 +870,14 @@ int stwo_rocm_profile_select(stwo_rocm_ctx* ctx, const char* kernel_class) {
     return guarded(ctx, [&](Ctx& c) { c.profile_only = kernel_class ? kernel_class : ""; });
 }
 
+int stwo_rocm_profile_sample(stwo_rocm_ctx* ctx, uint32_t every) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(every >= 1, "profile_sample: every must be >= 1");
+        c.profile_every = every;
+        c.profile_seen = 0;
+    });
+}
+
 int stwo_rocm_profile_read(stwo_rocm_ctx* ctx, char* json_out, size_t cap, size_t* len) {
     return guarded(ctx, [&](Ctx& c) {
         SR_CHECK(len, "null argument");

@@ -888,8 +888,19 @@ GkrRun* gkr_launch_on_channel(Ctx& c, DevChannel* d_ch, const std::vector<GkrIns
     const GdInst* I = run->d_inst.as<GdInst>();
     uint32_t* eq = run->d_eq.as<uint32_t>();
     uint32_t* rec = run->d_rec.as<uint32_t>();
+    // algorithmic bytes of one sum-check round: 96 B per term (four values read, two fixed
+    // values written, one eq entry) for every instance summed in it
+    auto round_bytes = [&](uint32_t L, uint32_t rho) {
+        uint32_t n_part = 0;
+        for (uint32_t k = 0; k < n_inst; k++)
+            n_part += inst[k].n_vars + L >= n_layers && inst[k].n_vars + L - n_layers >= L - rho;
+        return 96.0 * (double)(1ull << (L - rho - 1)) * n_part;
+    };
     {
-        KTimer kt(c, "gkr_small_layers", 0.0);
+        double bytes = 0;
+        for (uint32_t L = 1; L < std::min(n_layers, GD_SMALL_LOG); L++)
+            for (uint32_t rho = 0; rho < L; rho++) bytes += round_bytes(L, rho);
+        KTimer kt(c, "gkr_small_layers", bytes);
         hipLaunchKernelGGL(gkr_fused_kernel, dim3(1), dim3(GD_FT), 0, c.stream, S, I, n_inst, n_layers, 0u,
                            std::min(n_layers, GD_SMALL_LOG), 0u, eq, rec);
         SR_HIP(hipGetLastError());
@@ -909,13 +920,15 @@ GkrRun* gkr_launch_on_channel(Ctx& c, DevChannel* d_ch, const std::vector<GkrIns
             uint32_t n_part = 0;  // instances summed in this round: active with onv >= L - rho
             for (uint32_t k = 0; k < n_inst; k++)
                 n_part += inst[k].n_vars + L >= n_layers && inst[k].n_vars + L - n_layers >= L - rho;
-            KTimer kt(c, "gkr_round", 96.0 * (double)nt * n_part);
+            KTimer kt(c, "gkr_round", round_bytes(L, rho));
             hipLaunchKernelGGL(gkr_round_kernel, dim3((uint32_t)bx, n_part), dim3(GD_RT), 0, c.stream, S, I, n_inst,
                                n_layers, L, rho, (const uint32_t*)eq, rec, run->d_part.as<unsigned long long>(),
                                run->d_cnt.as<uint32_t>());
             SR_HIP(hipGetLastError());
         }
-        KTimer kt(c, "gkr_small_rounds", 0.0);
+        double bytes = 0;
+        for (uint32_t r2 = rho; r2 < L; r2++) bytes += round_bytes(L, r2);
+        KTimer kt(c, "gkr_small_rounds", bytes);
         hipLaunchKernelGGL(gkr_fused_kernel, dim3(1), dim3(GD_FT), 0, c.stream, S, I, n_inst, n_layers, L, L + 1, rho,
                            eq, rec);
         SR_HIP(hipGetLastError());

@@ -101,6 +101,8 @@ struct Ctx {
     uint32_t merkle_levels = 10;  // Merkle parent levels per launch, 2..10
     bool profiling = false;
     std::string profile_only;  // non-empty: record only launches of this class
+    uint32_t profile_every = 1;  // record every k-th launch that matches
+    uint64_t profile_seen = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<KRec> krecs;
@@ -132,7 +134,8 @@ struct KTimer {
     double bytes;
     hipEvent_t a = nullptr;
     KTimer(Ctx& cc, std::string k, double b) : c(cc), cls(std::move(k)), bytes(b) {
-        if (c.profiling && (c.profile_only.empty() || c.profile_only == cls)) {
+        if (c.profiling && (c.profile_only.empty() || c.profile_only == cls) &&
+            c.profile_seen++ % c.profile_every == 0) {
             a = c.pool_event();
             (void)hipEventRecord(a, c.stream);
         }

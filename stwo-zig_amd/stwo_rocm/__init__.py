@@ -170,6 +170,7 @@ _SIGNATURES += [
                                         ctypes.c_void_p, ctypes.c_void_p]),
     ("stwo_rocm_profile_enable", ctypes.c_int, [_vp, ctypes.c_int]),
     ("stwo_rocm_profile_select", ctypes.c_int, [_vp, ctypes.c_char_p]),
+    ("stwo_rocm_profile_sample", ctypes.c_int, [_vp, ctypes.c_uint32]),
     ("stwo_rocm_profile_read", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t,
                                                ctypes.POINTER(ctypes.c_size_t)]),
 ]
@@ -324,6 +325,10 @@ class Context:
     def profile_select(self, kernel_class=None):
         """Record only launches of `kernel_class` (None: every class)."""
         self._check(self.lib.stwo_rocm_profile_select(self.h, kernel_class.encode() if kernel_class else None))
+
+    def profile_sample(self, every=1):
+        """Record only every `every`-th launch of the selected class."""
+        self._check(self.lib.stwo_rocm_profile_sample(self.h, every))
 
     def profile_read(self):
         import json

@@ -51,7 +51,9 @@ const char* stwo_rocm_last_error(stwo_rocm_ctx* ctx);
 /* runtimePlatformIdentityAlloc (metal/commit_backend.zig): "name;arch;CUs;HBM bytes". */
 int stwo_rocm_ctx_identity(stwo_rocm_ctx* ctx, char* out, size_t out_len);
 int stwo_rocm_synchronize(stwo_rocm_ctx* ctx);
-/* telemetrySnapshot (metal/telemetry.zig): arena bytes live / cached / peak. */
+/* telemetrySnapshot (metal/telemetry.zig): arena bytes live / cached / peak.  release_cached
+ * frees the cached arena blocks and the serving slots that stwo_rocm_serve keeps resident
+ * between calls (the next serve call rebuilds them). */
 int stwo_rocm_memory_info(stwo_rocm_ctx* ctx, uint64_t* live, uint64_t* cached, uint64_t* peak);
 int stwo_rocm_release_cached(stwo_rocm_ctx* ctx);
 

@@ -106,7 +106,12 @@ int stwo_rocm_memory_info(stwo_rocm_ctx* ctx, uint64_t* live, uint64_t* cached, 
 }
 
 int stwo_rocm_release_cached(stwo_rocm_ctx* ctx) {
-    return guarded(ctx, [&](Ctx& c) { c.trim(); });
+    return guarded(ctx, [&](Ctx& c) {
+        c.trim();
+        // the serving slots (their streams, arenas and towers) are rebuilt by the next serve call
+        serve_pool_free(ctx->pool);
+        ctx->pool = nullptr;
+    });
 }
 
 int stwo_rocm_buffer_alloc(stwo_rocm_ctx* ctx, uint64_t bytes, void** dptr) {

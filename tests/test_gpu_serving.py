@@ -63,3 +63,15 @@ def test_serving_budget_limits_concurrency(ctx):
 def test_serving_rejects_job_over_budget(ctx):
     with pytest.raises(stwo_rocm.RocmError):
         stwo_rocm.serve(ctx, [(0, 12, 32, 1, 0.0)], n_slots=1, hbm_budget_bytes=1 << 20)
+
+
+def test_serving_slots_persist_and_rebuild_after_release(ctx):
+    # slot contexts stay resident between calls; release_cached frees them and the next call
+    # rebuilds them, with the same results
+    jobs = [(0, 11, 24, 301, 0.0), (2, 12, 0, 0, 0.0), (3, 13, 3, 5, 0.0), (0, 9, 40, 302, 0.0)]
+    a, _ = stwo_rocm.serve(ctx, jobs, n_slots=4)
+    b, _ = stwo_rocm.serve(ctx, jobs, n_slots=2)
+    ctx.release_cached()
+    c, _ = stwo_rocm.serve(ctx, jobs, n_slots=4)
+    assert [r["digest"] for r in a] == [r["digest"] for r in b] == [r["digest"] for r in c]
+    assert a[0]["digest"] == _ref_digest(jobs[0])

@@ -55,7 +55,7 @@ def main():
                          "serving: 256 Poisson requests (wide-Fib and LogUp mix), 1 slot vs continuous batching")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
+        print("bench.py measures one GPU: the backend has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "serving":

@@ -1510,7 +1510,7 @@ int ora_prove_poseidon(uint32_t log_n_instances, const ora_widefib_params *prm, 
 /* ======================================================================
  * LogUp / GKR (src/core/fraction.zig:9-73; src/frontends/riscv/air/logup.zig:74-105;
  * src/prover/lookups/{gkr_circuit,gkr_prover,sumcheck,mle,utils}.zig) and the
- * XOR-lookup AIR of the LogUp workload (defined by this build: a, b uniform
+ * XOR-lookup AIR of the LogUp workload (defined by this backend's LogUp benchmark: a, b uniform
  * 8-bit from splitmix64, c = a ^ b; table of all 2^16 (x, y, x ^ y) with
  * multiplicities; denominators z - (a + b*alpha + c*alpha^2)).
  * ==================================================================== */

@@ -49,10 +49,12 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain streaming kernel of known size (HBM counter calibration)")
-    ap.add_argument("--workload", choices=["widefib", "logup", "batched", "serving"], default="widefib",
+    ap.add_argument("--workload", choices=["widefib", "logup", "batched", "serving", "poseidon", "xor"],
+                    default="widefib",
                     help="widefib: the 2^20x100 proof (default); logup: XOR-lookup LogUp workload at 2^18; "
                          "batched: 64 uneven columns LDE + Merkle + 3 FRI folds as one hipGraph; "
-                         "serving: 256 Poisson requests (wide-Fib and LogUp mix), 1 slot vs continuous batching")
+                         "serving: 256 Poisson requests (wide-Fib, XOR, Poseidon, LogUp), 1 slot vs continuous batching; "
+                         "poseidon, xor: the reference's example AIRs at the size of their golden proofs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the backend has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -76,6 +78,9 @@ def main():
         return
     if args.workload == "serving":
         bench_serving(stwo_rocm, args)
+        return
+    if args.workload in ("poseidon", "xor"):
+        bench_example_air(stwo_rocm, args)
         return
     n, W = args.log_n_rows, args.sequence_len
     proto = dict(stwo_rocm.PROTOCOLS[args.protocol])
@@ -325,6 +330,84 @@ def bench_batched(stwo_rocm, args):
             "cpu_baseline": cpu, "bit_exact_vs_oracle": ok, "kernels": kernels}
     if cpu:
         line["speedup_vs_cpu_baseline"] = round(cpu["value"] / med, 1)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def bench_example_air(stwo_rocm, args):
+    """The reference's Poseidon (2^13 instances) or periodic XOR (2^16 rows, step 4, offset 3)
+    example AIR, functional protocol, at the size of its committed golden proof: every timed
+    proof must equal the golden bytes."""
+
+    poseidon = args.workload == "poseidon"
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden",
+                                       "poseidon_goldens.json" if poseidon else "xor_goldens.json")))
+    g = gold["poseidon_log13_pow10" if poseidon else "xor_log16_step2_offset3_pow10"]
+    ctx = stwo_rocm.Context(0)
+    ident = ctx.identity()
+
+    def prove(stats=False):
+        if poseidon:
+            return ctx.prove_poseidon(13, stats=stats)
+        return ctx.prove_xor(16, 2, 3, stats=stats)
+
+    for _ in range(max(args.warmup, 1)):
+        prove()
+    prof_steps = 3
+    ctx.profile_select(None)
+    ctx.profile_enable(True)
+    for _ in range(prof_steps):
+        prove()
+    prof = ctx.profile_read()
+    ctx.profile_enable(False)
+    dom, _, _, kernels = _dominant(prof, prof_steps)
+    ctx.profile_select(dom)
+    ctx.profile_sample(_sample_every(kernels[dom]["launches_per_step"]))
+    ctx.profile_enable(True)
+    ctx.synchronize()
+    times, shas = [], set()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        proof, _ = prove()
+        times.append((time.perf_counter() - t0) * 1e3)
+        shas.add(hashlib.sha256(proof).hexdigest())
+    wall_ms = (time.perf_counter() - t_start) * 1e3
+    dprof = ctx.profile_read()
+    ctx.profile_enable(False)
+    ctx.profile_select(None)
+    ctx.profile_sample(1)
+    if shas != {g["sha256"]}:
+        raise SystemExit("example AIR proof differs from the reference golden proof")
+    dv = dprof[dom]
+    median_ms = statistics.median(times)
+    cpu = None
+    if args.cpu_baseline == "on":
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        k = 5
+        t0 = time.perf_counter()
+        for _ in range(k):
+            ref = oracle.prove_poseidon(13) if poseidon else oracle.prove_xor(16, 2, 3)
+        cpu_ms = (time.perf_counter() - t0) * 1e3 / k
+        if hashlib.sha256(ref).hexdigest() != g["sha256"]:
+            raise SystemExit("oracle proof differs from the reference golden proof")
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+        cpu = {"value": round(cpu_ms, 2), "unit": "ms", "cores": threads, "kind": "port",
+               "sample": "%d full proofs of the same statement by the C oracle (mean)" % k}
+    name = "Poseidon AIR log_n_instances=13 (1264 columns x 2^10 rows)" if poseidon else \
+        "periodic XOR AIR log_size=16 log_step=2 offset=3"
+    line = {"metric": "%s prove ms" % name, "value": round(median_ms, 3), "unit": "ms", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall_ms / args.steps, 3),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u32 (exact M31/CM31/QM31)",
+            "data": "the reference example's own deterministic trace, generated on device",
+            "config": {"workload": name + ", functional protocol (pow 10, 3 queries, blowup 2)", "device": ident},
+            "roofline": _roofline(dom, dv["ms"] / dv["launches"], dv["bytes"] / dv["launches"], profiled_traffic()),
+            "cpu_baseline": cpu, "proof_sha256": g["sha256"], "bit_exact_vs_reference_golden": True,
+            "prove_ms": {"median": round(median_ms, 3), "min": round(min(times), 3), "max": round(max(times), 3)},
+            "kernels": kernels}
+    if cpu:
+        line["speedup_vs_cpu_baseline"] = round(cpu["value"] / median_ms, 1)
     print(json.dumps(line), flush=True)
     ctx.close()
 

@@ -95,6 +95,101 @@ __global__ __launch_bounds__(256) void quotients_kernel(const QEntryDev* __restr
     }
 }
 
+// Few rows and many entries (a wide trace on a small domain): one thread per row would loop
+// over every entry with a handful of loads in flight.  Blocks instead take (row block, chunk
+// of QS_CHUNK entries) and store their canonical partial sums; a finishing kernel adds a
+// row's partials and applies the batch's subtraction, division and leaf hash.  One batch.
+constexpr uint32_t QS_CHUNK = 64;
+__global__ __launch_bounds__(256) void quotients_split_kernel(const QEntryDev* __restrict__ entries,
+                                                              const QuotientBatchDev* __restrict__ batches,
+                                                              uint32_t log_size, uint32_t* __restrict__ part) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = 1u << log_size;
+    if (row >= n) return;
+    const QuotientBatchDev bt = batches[0];
+    const uint32_t e0 = bt.first + blockIdx.y * QS_CHUNK;
+    const uint32_t end = min(e0 + QS_CHUNK, bt.first + bt.count);
+    constexpr uint32_t QB = 8;
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    uint32_t e = e0;
+    for (; e + QB <= end; e += QB) {
+        uint32_t v[QB];
+#pragma unroll
+        for (uint32_t k = 0; k < QB; k++) {
+            const QEntryDev& en = entries[e + k];
+            v[k] = ((gcu32*)en.col)[lift_row(row, en.shift)];
+        }
+#pragma unroll
+        for (uint32_t h = 0; h < QB; h += 4) {
+            uint64_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+            for (uint32_t k = h; k < h + 4; k++) {
+                const QEntryDev& en = entries[e + k];
+                t0 += (uint64_t)en.coeff[0] * v[k];
+                t1 += (uint64_t)en.coeff[1] * v[k];
+                t2 += (uint64_t)en.coeff[2] * v[k];
+                t3 += (uint64_t)en.coeff[3] * v[k];
+            }
+            s0 += mfold(t0);
+            s1 += mfold(t1);
+            s2 += mfold(t2);
+            s3 += mfold(t3);
+        }
+    }
+    for (; e < end; e++) {
+        const QEntryDev& en = entries[e];
+        const uint32_t v = ((gcu32*)en.col)[lift_row(row, en.shift)];
+        s0 += mfold((uint64_t)en.coeff[0] * v);
+        s1 += mfold((uint64_t)en.coeff[1] * v);
+        s2 += mfold((uint64_t)en.coeff[2] * v);
+        s3 += mfold((uint64_t)en.coeff[3] * v);
+    }
+    uint32_t* o = part + (uint64_t)blockIdx.y * 4 * n + row;
+    o[0] = mred64(s0);
+    o[n] = mred64(s1);
+    o[2 * (uint64_t)n] = mred64(s2);
+    o[3 * (uint64_t)n] = mred64(s3);
+}
+__global__ __launch_bounds__(256) void quotients_finish_kernel(const QuotientBatchDev* __restrict__ batches,
+                                                               uint32_t n_chunks, uint32_t log_size,
+                                                               const uint32_t* __restrict__ domain_xy,
+                                                               const uint32_t* __restrict__ part,
+                                                               uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
+                                                               uint32_t* __restrict__ o2, uint32_t* __restrict__ o3,
+                                                               uint8_t* __restrict__ leaves, B2State seed) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = 1u << log_size;
+    if (row >= n) return;
+    const uint32_t x = domain_xy[row], y = domain_xy[n + row];
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // n_chunks canonical words: far below 2^64
+    for (uint32_t k = 0; k < n_chunks; k++) {
+        const uint32_t* p = part + (uint64_t)k * 4 * n + row;
+        s0 += p[0];
+        s1 += p[n];
+        s2 += p[2 * (uint64_t)n];
+        s3 += p[3 * (uint64_t)n];
+    }
+    const QuotientBatchDev bt = batches[0];
+    qm31 num = {{mred64(s0), mred64(s1)}, {mred64(s2), mred64(s3)}};
+    num = qsub(num, qadd(qmulm(bt.sum_a, y), bt.sum_b));
+    const cm31 den = cadd(csub(bt.det, cmulm(bt.piy, x)), cmulm(bt.pix, y));
+    const qm31 acc = qmulc(num, cinv(den));
+    o0[row] = acc.c0.a;
+    o1[row] = acc.c0.b;
+    o2[row] = acc.c1.a;
+    o3[row] = acc.c1.b;
+    if (leaves) {
+        uint32_t m[16] = {acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t h[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) h[k] = seed.h[k];
+        b2_compress(h, m, 80, true);
+        uint4* o = (uint4*)(leaves + (uint64_t)row * 32);
+        o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+        o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+    }
+}
+
 void launch_quotients(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_col_logs, const uint32_t* h_entry_cols,
                       const uint32_t* h_entry_coeffs, uint32_t n_entries, const QuotientBatchDev* h_batches,
                       uint32_t n_batches, uint32_t log_size, uint32_t* const* out, uint8_t* d_leaf_out) {
@@ -124,8 +219,19 @@ void launch_quotients_dev(Ctx& c, const QEntryDev* d_ent, uint32_t n_entries, co
     const uint32_t* xy = c.domain_points(log_size);
     uint32_t n = 1u << log_size;
     KTimer kt(c, "quotients", in_bytes + (8.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n);
-    hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_ent, d_bat, n_batches,
-                       log_size, xy, out[0], out[1], out[2], out[3], d_leaf_out, c.leaf_seed);
+    // the split form pays off when one thread per row leaves most of the device idle
+    if (n_batches == 1 && n_entries > 2 * QS_CHUNK && log_size <= 16) {
+        const uint32_t n_chunks = (n_entries + QS_CHUNK - 1) / QS_CHUNK;  // the batch spans every entry
+        DBuf part(&c, 16ull * n_chunks * n);
+        hipLaunchKernelGGL(quotients_split_kernel, dim3((n + 255) / 256, n_chunks), dim3(256), 0, c.stream, d_ent,
+                           d_bat, log_size, part.as<uint32_t>());
+        hipLaunchKernelGGL(quotients_finish_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_bat, n_chunks,
+                           log_size, xy, (const uint32_t*)part.as<uint32_t>(), out[0], out[1], out[2], out[3],
+                           d_leaf_out, c.leaf_seed);
+    } else {
+        hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_ent, d_bat, n_batches,
+                           log_size, xy, out[0], out[1], out[2], out[3], d_leaf_out, c.leaf_seed);
+    }
     SR_HIP(hipGetLastError());
 }
 

@@ -85,6 +85,48 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_leaves_kernel(const LeafCol
     store_hash(out, p, h);
 }
 
+// The same with one leaf per quad of lanes (one compression spread over four lanes): for few
+// leaves with many message blocks (a wide trace on a small domain) the per-leaf chain of
+// compressions, not the lane count, sets the time.  Every lane loads the whole block.
+__global__ __launch_bounds__(MK_THREADS) void merkle_leaves_quad_kernel(const LeafColDev* __restrict__ cols,
+                                                                        uint32_t blk0, uint32_t nblk, uint32_t n_cols,
+                                                                        uint32_t level, const uint32_t* __restrict__ in,
+                                                                        uint32_t in_shift, B2State seed,
+                                                                        uint8_t* __restrict__ out) {
+    const uint64_t t64 = (uint64_t)blockIdx.x * MK_THREADS + threadIdx.x;
+    if (t64 >= (4ull << level)) return;  // whole quads: the thread count is a multiple of 4
+    const uint32_t p = (uint32_t)(t64 >> 2), q = (uint32_t)(t64 & 3);
+    const QuadLane L(q);
+    uint32_t hlo, hhi;
+    if (in) {
+        const uint32_t* s = in + 8ull * (((p >> in_shift) << 1) + (p & 1));
+        hlo = s[q];
+        hhi = s[4 + q];
+    } else {
+        hlo = L.sel(seed.h[0], seed.h[1], seed.h[2], seed.h[3]);
+        hhi = L.sel(seed.h[4], seed.h[5], seed.h[6], seed.h[7]);
+    }
+    const uint32_t nblocks_total = (n_cols + 15) >> 4;
+    const uint32_t blk_end = blk0 + nblk;
+    uint32_t cur[16];
+    leaf_block_load(cols + 16 * blk0, p, cur);
+    for (uint32_t blk = blk0; blk < blk_end; blk++) {
+        uint32_t nxt[16];
+        const bool more = blk + 1 < blk_end;
+        if (more) leaf_block_load(cols + 16 * (blk + 1), p, nxt);
+        const bool last = blk + 1 == nblocks_total;
+        const uint32_t t = 64 + (last ? 4 * n_cols : 64 * (blk + 1));
+        b2_compress_quad(L, hlo, hhi, cur, t, last);
+        if (more) {
+#pragma unroll
+            for (int w = 0; w < 16; w++) cur[w] = nxt[w];
+        }
+    }
+    uint32_t* o = (uint32_t*)(out + (uint64_t)p * 32);
+    o[q] = hlo;
+    o[4 + q] = hhi;
+}
+
 // Leaves of a QM31 column in AoS form: 4 coordinate words per leaf (FRI layers).
 __global__ __launch_bounds__(MK_THREADS) void merkle_leaves_qm31_kernel(const uint4* __restrict__ vals,
                                                                         uint32_t log_size, B2State seed,
@@ -280,9 +322,15 @@ void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, const std::vector
                               (S > 1 ? " blocks=" + std::to_string(sg.nblk) : "")
                         : "merkle_leaves_prefix L=" + std::to_string(sg.level) + " blocks=" + std::to_string(sg.nblk),
                   bytes);
-        hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)), dim3(MK_THREADS),
-                           0, c.stream, d_tab, sg.blk0, sg.nblk, n_cols, sg.level, prev,
-                           prev ? sg.level - prev_level + 1 : 0u, c.leaf_seed, out);
+        if (n <= (1u << 14) && sg.nblk >= 8) {
+            hipLaunchKernelGGL(merkle_leaves_quad_kernel, dim3((uint32_t)((4 * n + MK_THREADS - 1) / MK_THREADS)),
+                               dim3(MK_THREADS), 0, c.stream, d_tab, sg.blk0, sg.nblk, n_cols, sg.level, prev,
+                               prev ? sg.level - prev_level + 1 : 0u, c.leaf_seed, out);
+        } else {
+            hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)),
+                               dim3(MK_THREADS), 0, c.stream, d_tab, sg.blk0, sg.nblk, n_cols, sg.level, prev,
+                               prev ? sg.level - prev_level + 1 : 0u, c.leaf_seed, out);
+        }
         SR_HIP(hipGetLastError());
         prev = (const uint32_t*)out;
         prev_level = sg.level;

@@ -198,11 +198,13 @@ def _dominant(prof, steps):
 
 def _sample_every(launches_per_step):
     """Stride for the timed run's events on the dominant class: about four bracketed launches
-    per step (each event costs a few microseconds of GPU time), the stride chosen not to divide
-    the launches of a step so that the sampled launches rotate over the step's launches."""
+    per step (each event costs a few microseconds of GPU time), the stride coprime with the
+    launches of a step so that over k steps every launch position is sampled once."""
+    import math
+
     n = int(round(launches_per_step))
-    k = max(1, round(n / 4))
-    if k > 1 and n % k == 0:
+    k = max(1, n // 4)
+    while k > 1 and math.gcd(n, k) != 1:
         k += 1
     return k
 

@@ -134,7 +134,7 @@ namespace {
 
 constexpr uint32_t GD_MAX_INST = 8;
 constexpr uint32_t GD_MAX_VARS = 30;
-constexpr uint32_t GD_SMALL_LOG = 10;  // rounds on layers of <= 2^10 values run in one workgroup (measured best; 9 and 11 slower)
+constexpr uint32_t GD_SMALL_LOG = 9;  // rounds on layers of <= 2^9 values run in one workgroup (measured best with 10 within 2%)
 constexpr uint32_t GD_FT = 512;       // one-workgroup kernel: 7 summing waves + 1 for the pending update
 constexpr uint32_t GD_RT = 256;       // threads per workgroup of the round and begin kernels
 constexpr uint32_t GD_REC_ROUND = 17;  // [len, 4 QM31] per sum-check round

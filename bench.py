@@ -34,6 +34,9 @@ sys.path.insert(0, os.path.join(ROOT, "stwo-zig_amd"))
 
 METRIC = "wide-Fib 2^20x100 prove ms; batched proofs/s + p99; LDE/Merkle HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+# wave64 VALU instructions per second of the whole MI355X: 256 CUs x 2.4 GHz, one per CU-cycle
+# (four SIMDs per CU, each issuing a wave64 instruction over four cycles)
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9
 PUBLISHED_CPU_MS = 900.664792  # conformance/evidence/metrics-v2/w1-log20x100/cpu-report.json (M5 Max, 18 threads)
 GOLDEN_SHA = {(20, 100, "functional"): "e6609d0564a47192212bec7973e2660c2eea88bef90c573c3df09569cc3c7e86"}
 
@@ -41,26 +44,31 @@ GOLDEN_SHA = {(20, 100, "functional"): "e6609d0564a47192212bec7973e2660c2eea88be
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--log-n-rows", type=int, default=20)
     ap.add_argument("--sequence-len", type=int, default=100)
     ap.add_argument("--protocol", choices=["functional", "secure"], default="functional")
     ap.add_argument("--cpu-baseline", choices=["on", "off"], default="on")
+    ap.add_argument("--batched", choices=["on", "off"], default="on",
+                    help="also run the batched primitive pipeline (64 columns, logs 14..22) into the line")
+    ap.add_argument("--serving", choices=["on", "off"], default="on",
+                    help="also run the serving comparison (256 requests, 1 slot vs 16 slots at equal p99)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain streaming kernel of known size (HBM counter calibration)")
     ap.add_argument("--workload", choices=["widefib", "logup", "batched", "serving", "poseidon", "xor"],
                     default="widefib",
-                    help="widefib: the 2^20x100 proof (default); logup: XOR-lookup LogUp workload at 2^18; "
-                         "batched: 64 uneven columns LDE + Merkle + 3 FRI folds as one hipGraph; "
-                         "serving: 256 Poisson requests (wide-Fib, XOR, Poseidon, LogUp), 1 slot vs continuous batching; "
-                         "poseidon, xor: the reference's example AIRs at the size of their golden proofs")
+                    help="widefib: the 2^20x100 proof (default; with the batched and serving parts); logup: "
+                         "XOR-lookup LogUp workload at 2^18; batched: 64 uneven columns LDE + Merkle + 3 FRI folds "
+                         "as one hipGraph; serving: 256 Poisson requests (wide-Fib, XOR, Poseidon, LogUp), 1 slot "
+                         "vs continuous batching; poseidon, xor: the reference's example AIRs at the size of their "
+                         "golden proofs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the backend has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "serving":
+    if args.workload == "serving" or (args.workload == "widefib" and args.serving == "on" and not args.copy_probe):
         # Every serving slot owns a HIP stream; HIP spreads a process's streams over
         # GPU_MAX_HW_QUEUES hardware queues (4 by default), so with more slots than queues
         # streams share a queue and their proofs serialize.  Set before the runtime starts.
@@ -82,11 +90,53 @@ def main():
     if args.workload in ("poseidon", "xor"):
         bench_example_air(stwo_rocm, args)
         return
+    bench_widefib(stwo_rocm, args)
+
+
+def lde_unit_classes(prof, n, W, B):
+    """Kernel classes of the trace's interpolation and LDE (every launch over the W trace columns
+    on the 2^n trace domain and the 2^(n+B) evaluation domain)."""
+    out = []
+    for k in prof:
+        if not k.endswith(" cols=%d" % W):
+            continue
+        if k.startswith("fft_ifft_pass m=%d " % n) or k.startswith("fft_lde_pass m=%d " % (n + B)) or \
+                k.startswith("fft_ilde_pass m=%d " % n):
+            out.append(k)
+    return sorted(out)
+
+
+def unit_figures(name, classes, alg_bytes, per_proof_ms, counters, share=None):
+    """Roofline of a unit of work done by several launches: algorithmic bytes over the summed
+    device time of its launches, the HBM bytes that rocprof counted for them, and the VALU
+    bound (wave64 VALU instructions / (CUs x clock), one per CU-cycle)."""
+    share = share or {}
+    t_ms = sum(per_proof_ms[k] * share.get(k, 1.0) for k in classes)
+    achieved = alg_bytes / (t_ms * 1e-3) / 1e9
+    traffic = [counters["traffic"].get(k) for k in classes]
+    traffic = sum(t * share.get(k, 1.0) for k, t in zip(classes, traffic)) if all(t is not None for t in traffic) \
+        else None
+    valu = [counters["valu"].get(k) for k in classes]
+    valu = sum(v * share.get(k, 1.0) for k, v in zip(classes, valu)) if all(v is not None for v in valu) else None
+    out = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": int(traffic) if traffic else None,
+           "traffic_ratio": round(traffic / alg_bytes, 3) if traffic else None,
+           "algorithmic_bytes": int(alg_bytes), "device_ms": round(t_ms, 4),
+           "launches": {k: round(per_proof_ms[k] * share.get(k, 1.0), 4) for k in classes}}
+    if valu:
+        bound_ms = valu / VALU_PEAK_WAVE_INSTR_PER_S * 1e3
+        out["valu"] = {"wave_instructions": int(valu), "peak_wave_instructions_per_s": VALU_PEAK_WAVE_INSTR_PER_S,
+                       "bound_ms": round(bound_ms, 4), "frac": round(bound_ms / t_ms, 4)}
+    return out
+
+
+def bench_widefib(stwo_rocm, args):
     n, W = args.log_n_rows, args.sequence_len
     proto = dict(stwo_rocm.PROTOCOLS[args.protocol])
+    B = proto["log_blowup_factor"]
     ctx = stwo_rocm.Context(0)
     ident = ctx.identity()
-    ctx.prepare_twiddles(n + proto["log_blowup_factor"] + 1)
+    ctx.prepare_twiddles(n + B + 1)
     d_trace = ctx.alloc(4 * W << n)
     ctx.fill_quadratic_recurrence_trace(n, W, d_trace.ptr, mode=0)
     kw = dict(log_n_rows=n, sequence_len=W, trace_mode=0, d_trace=d_trace, **proto)
@@ -95,8 +145,7 @@ def main():
         ctx.prove_wide_fibonacci(**kw)
 
     golden = GOLDEN_SHA.get((n, W, args.protocol))
-    # Per-kernel breakdown from separate profiled proofs (every launch bracketed by
-    # events); it picks the dominant kernel class.
+    # Per-kernel breakdown from separate profiled proofs (every launch bracketed by events).
     prof_steps = 3
     ctx.profile_select(None)
     ctx.profile_enable(True)
@@ -104,11 +153,14 @@ def main():
         ctx.prove_wide_fibonacci(**kw)
     prof = ctx.profile_read()
     ctx.profile_enable(False)
-    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])[0]
+    lde = lde_unit_classes(prof, n, W, B)
+    if not lde:
+        raise SystemExit("no interpolation/LDE launches found in the profile")
 
-    # Timed proofs: events only around the dominant kernel's launches.
-    ctx.profile_select(dom)
-    ctx.profile_sample(_sample_every(prof[dom]["launches"] / prof_steps))
+    # Timed proofs: events only around the launches of the LDE unit, sampled (each event
+    # costs GPU time), the stride coprime with the unit's launches per proof.
+    ctx.profile_select(lde)
+    ctx.profile_sample(_sample_every(sum(prof[k]["launches"] for k in lde) / prof_steps, target=1))
     ctx.profile_enable(True)
     ctx.synchronize()
     times, shas = [], set()
@@ -123,6 +175,7 @@ def main():
     dprof = ctx.profile_read()
     ctx.profile_enable(False)
     ctx.profile_select(None)
+    ctx.profile_sample(1)
     # per-stage device times from untimed proofs (stage events are kept out of the timed ones)
     stats_all = []
     for _ in range(3):
@@ -134,21 +187,41 @@ def main():
     sha = shas.pop()
     if golden is not None and sha != golden:
         raise SystemExit("proof differs from the reference golden proof: %s" % sha)
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import stark_verifier
+        stark_verifier.verify_proof(proof, "wide_fibonacci", dict(log_n_rows=n, sequence_len=W), proto)
+        verified = True
+    except ImportError:
+        verified = None
 
     median_ms = statistics.median(times)
-    traffic = profiled_traffic()
-    dv = dprof[dom]
-    avg_ms = dv["ms"] / dv["launches"]
-    bytes_per_launch = dv["bytes"] / dv["launches"]
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    kernels = {k: {"launches_per_proof": v["launches"] / prof_steps, "ms_per_proof": v["ms"] / prof_steps,
-                   "GBs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else None}
+    counters = profiled_counters()
+    # LDE unit (live, timed proofs): per launch average of each class x its launches per proof
+    live_ms = {k: dprof[k]["ms"] / dprof[k]["launches"] * prof[k]["launches"] / prof_steps for k in lde if k in dprof}
+    if set(live_ms) != set(lde):
+        raise SystemExit("a launch of the LDE unit was never sampled in the timed proofs")
+    lde_bytes = 4.0 * W * ((1 << n) + (1 << n) + (1 << (n + B)))  # trace in, coefficients out, LDE out
+    roofline = unit_figures("LDE unit: trace interpolation + LDE (%d launches)" % len(lde), lde, lde_bytes, live_ms,
+                            counters)
+    # Merkle unit of the main tree (profiled proofs): its leaf launch, one parent launch and one
+    # tree-top launch (the trace, composition and first FRI trees have the same shape and share
+    # the parent class; the trace and composition trees share the top class)
+    per_proof = {k: v["ms"] / prof_steps for k, v in prof.items()}
+    L = n + B
+    leaf_cls = [k for k in prof if k == "merkle_leaves L=%d cols=%d" % (L, W)]
+    par_cls = [k for k in prof if k.startswith("merkle_parents L=%d " % L)]
+    top_cls = [k for k in prof if k.startswith("merkle_top L=") and "+channel" not in k]
+    # one launch's worth of each class (the classes are shared by trees of the same shape)
+    share = {k: prof_steps / prof[k]["launches"] for k in par_cls + top_cls}
+    merkle_bytes = 4.0 * W * (1 << L) + 32.0 * ((2 << L) - 1)
+    merkle = unit_figures("Merkle unit: main-tree leaves + parents + top", leaf_cls + par_cls + top_cls,
+                          merkle_bytes, per_proof, counters, share) if leaf_cls else None
+
+    kernels = {k: {"launches_per_proof": v["launches"] / prof_steps, "ms_per_proof": round(v["ms"] / prof_steps, 4),
+                   "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
     stage_gpu = {k: statistics.median(s["stage_gpu_ms"][k] for s in stats_all) for k in stats_all[0]["stage_gpu_ms"]}
-
-    cpu = None
-    if args.cpu_baseline == "on":
-        cpu = cpu_baseline(n, W, proto)
 
     line = {
         "metric": METRIC,
@@ -168,12 +241,12 @@ def main():
                                " log_blowup=%d), one proof" % (n, W, args.protocol, proto["pow_bits"],
                                                                proto["n_queries"], proto["log_blowup_factor"]),
                    "log_n_rows": n, "sequence_len": W, "protocol": args.protocol, "device": ident},
-        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
-                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)},
-        "cpu_baseline": cpu,
+        "roofline": roofline,
+        "merkle_unit": merkle,
+        "cpu_baseline": None,
         "proof_sha256": sha,
         "bit_exact_vs_reference_golden": golden is not None and sha == golden,
+        "verifier_accepts": verified,
         "prove_ms": {"median": round(median_ms, 3), "min": round(min(times), 3), "max": round(max(times), 3)},
         "device_ms_per_proof": round(statistics.median(s["total_gpu_ms"] for s in stats_all), 3),
         "stage_device_ms": {k: round(v, 3) for k, v in stage_gpu.items()},
@@ -181,10 +254,22 @@ def main():
         "published_cpu_ms_m5max_18t": PUBLISHED_CPU_MS,
         "speedup_vs_published_cpu": round(PUBLISHED_CPU_MS / median_ms, 1),
     }
-    if cpu:
-        line["speedup_vs_cpu_baseline"] = round(cpu["value"] / median_ms, 1)
-    print(json.dumps(line), flush=True)
+    d_trace.free()
+    if args.batched == "on":
+        line["batched"] = batched_part(stwo_rocm, ctx, steps=10, warmup=2, check_oracle=args.cpu_baseline == "on")
+        print(json.dumps({"batched": line["batched"]}), file=sys.stderr, flush=True)
+    if args.serving == "on":
+        ctx.release_cached()
+        line["serving"] = serving_part(stwo_rocm, ctx, check_oracle=args.cpu_baseline == "on")
+        print(json.dumps({"serving": {k: v for k, v in line["serving"].items() if k != "sweep"}}), file=sys.stderr,
+              flush=True)
     ctx.close()
+    if args.cpu_baseline == "on":
+        line["cpu_baseline"] = cpu_baseline(n, W, proto, sha)
+        line["speedup_vs_cpu_baseline"] = round(line["cpu_baseline"]["value"] / median_ms, 1)
+        if line["cpu_baseline"].get("one_thread_ms"):
+            line["speedup_vs_cpu_one_thread"] = round(line["cpu_baseline"]["one_thread_ms"] / median_ms, 1)
+    print(json.dumps(line), flush=True)
 
 
 def _dominant(prof, steps):
@@ -196,14 +281,14 @@ def _dominant(prof, steps):
     return dom, dv["ms"] / dv["launches"], dv["bytes"] / dv["launches"], kernels
 
 
-def _sample_every(launches_per_step):
-    """Stride for the timed run's events on the dominant class: about four bracketed launches
-    per step (each event costs a few microseconds of GPU time), the stride coprime with the
-    launches of a step so that over k steps every launch position is sampled once."""
+def _sample_every(launches_per_step, target=4):
+    """Stride for the timed run's events on the selected classes: about `target` bracketed
+    launches per step (each event costs a few microseconds of GPU time), the stride coprime with
+    the launches of a step so that over k steps every launch position is sampled once."""
     import math
 
     n = int(round(launches_per_step))
-    k = max(1, n // 4)
+    k = max(1, n // target)
     while k > 1 and math.gcd(n, k) != 1:
         k += 1
     return k
@@ -278,34 +363,54 @@ def bench_logup(stwo_rocm, args):
     ctx.close()
 
 
-def bench_batched(stwo_rocm, args):
-    """64 seeded columns, log sizes 14..22, LDE at blowup 4, one lifted Merkle tree, 3 FRI fold_line
-    rounds on a log-22 line; autotuned launch shapes; one hipGraph replay per step."""
-    spec = (64, 14, 22, 2, 22, 3, 1)
-    ctx = stwo_rocm.Context(0)
-    ident = ctx.identity()
+BATCHED_SPEC = (64, 14, 22, 2, 22, 3, 1)  # columns, log min, log max, log blowup, FRI line log, folds, seed
+
+
+def _splitmix64(x):
+    M = (1 << 64) - 1
+    z = (x + 0x9E3779B97F4A7C15) & M
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+    return z ^ (z >> 31)
+
+
+def batched_column_logs(spec):
+    """Column log sizes of the batched pipeline (the rule of batched_pipeline.hip / the oracle)."""
+    n, lo, hi, _, _, _, seed = spec
+    M = (1 << 64) - 1
+    return [lo + _splitmix64(seed ^ 0xA5A5A5A5 ^ ((i * 0x9E3779B97F4A7C15) & M)) % (hi - lo + 1) for i in range(n)]
+
+
+def batched_part(stwo_rocm, ctx, steps, warmup, check_oracle, spec=BATCHED_SPEC):
+    """Config C on `ctx`: autotuned launch shapes, graph replays timed (device ms per replay),
+    eager launches profiled for the LDE and Merkle units, result checked against the oracle."""
+    logs = batched_column_logs(spec)
+    B = spec[3]
     bp = ctx.batched(*spec)
-    tile, levels, table = bp.autotune(reps=2)
-    bp.run(max(args.warmup, 1), graph=True)
-    eager_words, eager_ms = bp.run(3, graph=False)
-    # per-kernel profile of eager runs (a graph carries no events)
-    prof_steps = 2
-    ctx.profile_select(None)
-    ctx.profile_enable(True)
-    bp.run(prof_steps, graph=False)
-    prof = ctx.profile_read()
-    ctx.profile_enable(False)
-    dom, avg_ms, bpl, kernels = _dominant(prof, prof_steps)
-    times = []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        words, ms = bp.run(1, graph=True)
-        times.append(ms)
-    wall_ms = (time.perf_counter() - t_start) * 1e3
+    try:
+        tile, levels, table = bp.autotune(reps=2)
+        bp.run(max(warmup, 1), graph=True)
+        eager_words, eager_ms = bp.run(3, graph=False)
+        # per-kernel profile of eager runs (a graph replay carries no events)
+        prof_steps = 2
+        ctx.profile_select(None)
+        ctx.profile_enable(True)
+        bp.run(prof_steps, graph=False)
+        prof = ctx.profile_read()
+        ctx.profile_enable(False)
+        times = []
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            words, ms = bp.run(1, graph=True)
+            times.append(ms)
+        wall_ms = (time.perf_counter() - t_start) * 1e3
+    finally:
+        bp.set_tuning(13, 10)
+        bp.close()
     if not np.array_equal(words, eager_words):
-        raise SystemExit("graph and eager results differ")
-    cpu, ok = None, None
-    if args.cpu_baseline == "on":
+        raise SystemExit("batched pipeline: graph and eager results differ")
+    ok, cpu = None, None
+    if check_oracle:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         t0 = time.perf_counter()
@@ -317,23 +422,49 @@ def bench_batched(stwo_rocm, args):
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         cpu = {"value": round(cpu_ms, 1), "unit": "ms", "cores": threads, "kind": "port",
                "sample": "1 full run of the same pipeline (64 columns, logs 14..22) by the C oracle"}
-    bp.close()
+    per_run = {k: v["ms"] / prof_steps for k, v in prof.items()}
+    counters = profiled_counters()
+    fft = sorted(k for k in prof if k.startswith("fft_"))
+    lde_bytes = 4.0 * sum((1 << l) + (1 << l) + (1 << (l + B)) for l in logs)
+    L = max(logs) + B
+    mk = sorted(k for k in prof if k.startswith("merkle_leaves") or k.startswith("merkle_parents L=%d " % L) or
+                (k.startswith("merkle_top") and "+channel" not in k))
+    merkle_bytes = 4.0 * sum(1 << (l + B) for l in logs) + 32.0 * ((2 << L) - 1)
     med = statistics.median(times)
+    return {"value": round(med, 3), "unit": "ms per graph replay", "steps": steps, "warmup": warmup,
+            "ms_per_step": round(wall_ms / steps, 3),
+            "workload": "64 columns (log sizes 14..22, seeded), interpolate + LDE at blowup 4, one lifted Blake2s "
+                        "Merkle tree, 3 FRI fold_line rounds on a log-22 QM31 line with a device channel; one hipGraph",
+            "tuning": {"fft_tile_log": tile, "merkle_levels": levels}, "autotune_table": table,
+            "eager_ms": round(eager_ms, 3), "graph_speedup_vs_eager": round(eager_ms / med, 3),
+            "roofline": unit_figures("LDE unit: interpolation + LDE of 64 columns (eager launches)", fft, lde_bytes,
+                                     per_run, counters),
+            "merkle_unit": unit_figures("Merkle unit: lifted tree of 64 LDE columns (eager launches)", mk,
+                                        merkle_bytes, per_run, counters),
+            "bit_exact_vs_oracle": ok, "cpu_baseline": cpu,
+            "kernels": {k: round(v, 4) for k, v in sorted(per_run.items(), key=lambda kv: -kv[1])}}
+
+
+def bench_batched(stwo_rocm, args):
+    """64 seeded columns, log sizes 14..22, LDE at blowup 4, one lifted Merkle tree, 3 FRI fold_line
+    rounds on a log-22 line; autotuned launch shapes; one hipGraph replay per step."""
+    ctx = stwo_rocm.Context(0)
+    ident = ctx.identity()
+    part = batched_part(stwo_rocm, ctx, args.steps, args.warmup, args.cpu_baseline == "on")
+    ctx.close()
     line = {"metric": "batched primitive pipeline ms (64 cols log 14..22, LDE x4, lifted Merkle, 3 FRI folds)",
-            "value": round(med, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": round(wall_ms / args.steps, 3), "higher_is_better": False, "scaling": "weak",
+            "value": part["value"], "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": part["ms_per_step"], "higher_is_better": False, "scaling": "weak",
             "vs_baseline": None, "dtype": "u32 (exact M31/QM31)",
             "data": "synthetic: seeded uniform M31 columns and QM31 line (splitmix64)",
             "config": {"workload": "batched: 64 columns, log sizes 14..22, log_blowup 2, fri_log 22, 3 folds",
-                       "device": ident, "tuning": {"fft_tile_log": tile, "merkle_levels": levels},
-                       "autotune_table": table},
-            "eager_ms": round(eager_ms, 3), "graph_speedup_vs_eager": round(eager_ms / med, 3),
-            "roofline": _roofline(dom, avg_ms, bpl, profiled_traffic()),
-            "cpu_baseline": cpu, "bit_exact_vs_oracle": ok, "kernels": kernels}
-    if cpu:
-        line["speedup_vs_cpu_baseline"] = round(cpu["value"] / med, 1)
+                       "device": ident, "tuning": part["tuning"]},
+            "roofline": part["roofline"], "merkle_unit": part["merkle_unit"], "cpu_baseline": part["cpu_baseline"],
+            "bit_exact_vs_oracle": part["bit_exact_vs_oracle"], "eager_ms": part["eager_ms"],
+            "autotune_table": part["autotune_table"], "kernels": part["kernels"]}
+    if part["cpu_baseline"]:
+        line["speedup_vs_cpu_baseline"] = round(part["cpu_baseline"]["value"] / part["value"], 1)
     print(json.dumps(line), flush=True)
-    ctx.close()
 
 
 def bench_example_air(stwo_rocm, args):
@@ -436,107 +567,136 @@ def serving_requests(n=256, rate_per_s=1500.0, seed=2024):
     return jobs
 
 
-def bench_serving(stwo_rocm, args):
-    """256 requests with Poisson arrivals: the one-slot service (the unbatched GPU path: one proof
-    at a time) against continuous batching over several slots; proofs/s (value), p50/p99 latency."""
+def _serve_ref_digest(oracle, job):
+    kind, n, w, seed, _ = job
+    if kind == 0:
+        out = oracle.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=seed)
+    elif kind == 2:
+        out = oracle.prove_poseidon(n)
+    elif kind == 3:
+        out = oracle.prove_xor(n, w, seed)
+    else:
+        out = oracle.logup_xor_pipeline(n, seed).astype(np.uint32).tobytes()
+    return hashlib.blake2s(out).hexdigest()
+
+
+def serving_part(stwo_rocm, ctx, check_oracle, slot_counts=(1, 16)):
+    """Config E on `ctx`: the 256-request stream served by one slot (the unbatched GPU path, one
+    proof at a time) and by 16 slots (continuous batching); proofs/s and p50/p99 at 1500/s, and
+    the equal-p99 comparison: the offered Poisson rate swept for both, keeping for each the highest
+    achieved proofs/s whose p99 stays within 50 ms.  Every run's results must be identical."""
     jobs = serving_requests()
     burst = serving_requests(rate_per_s=1e9)  # the same requests, all arriving at once
-    ctx = stwo_rocm.Context(0)
-    ident = ctx.identity()
-    # Warm-up: the largest configuration proves the whole stream once, so the
-    # resident slot contexts hold warm arenas and towers for every size in the mix.
+    # warm-up: the widest service proves the whole stream once, so the resident slot contexts
+    # hold warm arenas and towers for every size in the mix
     stwo_rocm.serve(ctx, burst, n_slots=16)
-    runs, capacity = {}, {}
     digests = None
 
     def rounded(st):
         return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()}
 
-    for slots in (1, 2, 4, 8, 16):
-        res, st = stwo_rocm.serve(ctx, jobs, n_slots=slots)
+    def check(res, what):
+        nonlocal digests
         d = [r["digest"] for r in res]
         if digests is None:
             digests = d
         elif d != digests:
-            raise SystemExit("serving results differ between slot counts")
+            raise SystemExit("serving results differ (%s)" % what)
+
+    runs, capacity = {}, {}
+    for slots in slot_counts:
+        res, st = stwo_rocm.serve(ctx, jobs, n_slots=slots)
+        check(res, "slots %d" % slots)
         runs[slots] = rounded(st)
         print(json.dumps({"serving_slots": slots, "rate_per_s": 1500, **runs[slots]}), file=sys.stderr, flush=True)
     for slots in (1, 16):
         res, st = stwo_rocm.serve(ctx, burst, n_slots=slots)
-        if [r["digest"] for r in res] != digests:
-            raise SystemExit("serving results differ between arrival patterns")
+        check(res, "burst, slots %d" % slots)
         capacity[slots] = rounded(st)
-        print(json.dumps({"serving_slots": slots, "rate_per_s": "burst", **capacity[slots]}), file=sys.stderr, flush=True)
-    # Equal-p99 comparison: offered Poisson rates swept for the one-slot path and for 16 slots;
-    # for each, the highest achieved proofs/s among the rates whose p99 stays within 50 ms.
     p99_bound_ms = 50.0
     sweep = {1: [], 16: []}
     for slots, rates in ((1, (200, 300, 400, 500, 600)), (16, (1000, 1500, 2000, 2500, 3000))):
         for rate in rates:
             res, st = stwo_rocm.serve(ctx, serving_requests(rate_per_s=float(rate)), n_slots=slots)
+            check(res, "rate %d, slots %d" % (rate, slots))
             sweep[slots].append({"rate_per_s": rate, "jobs_per_s": round(st["jobs_per_s"], 1),
                                  "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2)})
-            print(json.dumps({"serving_slots": slots, **sweep[slots][-1]}), file=sys.stderr, flush=True)
 
     def sustained(rows):
-        ok = [r["jobs_per_s"] for r in rows if r["p99_ms"] <= p99_bound_ms]
-        return max(ok) if ok else None
+        ok = [r for r in rows if r["p99_ms"] <= p99_bound_ms]
+        return max(ok, key=lambda r: r["jobs_per_s"]) if ok else None
 
-    equal_p99 = {"p99_bound_ms": p99_bound_ms, "one_slot": sustained(sweep[1]), "slots_16": sustained(sweep[16]),
-                 "sweep": sweep}
-    if equal_p99["one_slot"] and equal_p99["slots_16"]:
-        equal_p99["speedup"] = round(equal_p99["slots_16"] / equal_p99["one_slot"], 3)
-    best = max((s for s in runs if s > 1), key=lambda s: runs[s]["jobs_per_s"])
+    one, many = sustained(sweep[1]), sustained(sweep[16])
+    equal_p99 = {"p99_bound_ms": p99_bound_ms, "one_slot": one, "slots_16": many, "sweep": sweep}
+    if one and many:
+        equal_p99["speedup"] = round(many["jobs_per_s"] / one["jobs_per_s"], 3)
+    ok = None
+    if check_oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        subset = sorted({next(i for i, j in enumerate(jobs) if j[0] == k) for k in range(4)} | {0, 64, 128, 192})
+        ok = all(_serve_ref_digest(oracle, jobs[i]) == digests[i] for i in subset)
+        if not ok:
+            raise SystemExit("served results differ from the oracle")
+    return {"value": many["jobs_per_s"] if many else None, "unit": "proofs/s at p99 <= 50 ms (16 slots)",
+            "p50_ms": many["p50_ms"] if many else None, "p99_ms": many["p99_ms"] if many else None,
+            "unbatched_one_slot": one, "speedup_vs_unbatched_at_equal_p99": equal_p99.get("speedup"),
+            "workload": "256 requests, Poisson arrivals (seeded): 50% wide-Fibonacci (log 14..20 x 16..100, "
+                        "seeded-uniform traces), 20% periodic XOR AIR (log 12..18), 15% Poseidon AIR (log 12..16), "
+                        "15% XOR-lookup LogUp (log 16..18); functional protocol",
+            "at_1500_per_s": runs, "capacity_burst": capacity, "equal_p99": equal_p99,
+            "results_equal_across_runs": True, "oracle_subset_equal": ok}
+
+
+def bench_serving(stwo_rocm, args):
+    """256 requests with Poisson arrivals: the one-slot service (the unbatched GPU path: one proof
+    at a time) against continuous batching over several slots; proofs/s and p50/p99 latency."""
+    ctx = stwo_rocm.Context(0)
+    ident = ctx.identity()
+    part = serving_part(stwo_rocm, ctx, args.cpu_baseline == "on", slot_counts=(1, 2, 4, 8, 16))
+    ctx.close()
     cpu = None
     if args.cpu_baseline == "on":
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
+        jobs = serving_requests()
         k = 12
         t0 = time.perf_counter()
-        for kind, n, w, seed, _ in jobs[:k]:
-            if kind == 0:
-                oracle.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=seed)
-            elif kind == 2:
-                oracle.prove_poseidon(n)
-            elif kind == 3:
-                oracle.prove_xor(n, w, seed)
-            else:
-                oracle.logup_xor_pipeline(n, seed)
+        for job in jobs[:k]:
+            _serve_ref_digest(oracle, job)
         el = time.perf_counter() - t0
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
         cpu = {"value": round(k / el, 2), "unit": "proofs/s", "cores": threads, "kind": "port",
                "sample": "the first %d requests of the same stream run back to back by the C oracle" % k}
-    line = {"metric": "serving proofs/s (256 Poisson requests, wide-Fib 14..20 x 16..100 + XOR 12..18 + Poseidon 12..16 + LogUp 16..18) + p50/p99",
-            "value": runs[best]["jobs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": len(jobs),
-            "warmup": 8, "ms_per_step": round(runs[best]["makespan_ms"] / len(jobs), 4),
+    line = {"metric": "serving proofs/s at equal p99 (256 Poisson requests, wide-Fib 14..20 x 16..100 + XOR 12..18 "
+                      "+ Poseidon 12..16 + LogUp 16..18) + p50/p99",
+            "value": part["value"], "unit": "proofs/s", "n_gpus": 1, "steps": 256, "warmup": 256,
+            "ms_per_step": round(1e3 / part["value"], 4) if part["value"] else None,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32 (exact M31/QM31)",
             "data": "synthetic: seeded request stream and seeded traces",
-            "config": {"workload": "serving: 256 requests, Poisson 1500/s, 50% wide-Fib, 20% XOR AIR, 15% Poseidon, 15% LogUp",
-                       "device": ident, "slots": best},
-            "p50_ms": runs[best]["p50_ms"], "p99_ms": runs[best]["p99_ms"],
-            "unbatched_one_slot": runs[1], "by_slots": runs,
-            "speedup_vs_one_slot": round(runs[best]["jobs_per_s"] / runs[1]["jobs_per_s"], 3),
-            "capacity_burst": capacity,
-            "equal_p99": equal_p99,
-            "capacity_speedup_16_vs_1": round(capacity[16]["jobs_per_s"] / capacity[1]["jobs_per_s"], 3),
-            "cpu_baseline": cpu, "results_equal_across_slot_counts": True}
+            "config": {"workload": part["workload"], "device": ident, "slots": 16},
+            "cpu_baseline": cpu, **{k: v for k, v in part.items() if k not in ("value", "unit", "workload")}}
     print(json.dumps(line), flush=True)
-    ctx.close()
 
 
-def profiled_traffic():
-    """HBM bytes per launch per kernel class, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summaries (profiles/*.md lines `traffic_bytes_per_launch[<class>]: <bytes>`)."""
+def profiled_counters():
+    """Per-launch rocprof figures per kernel class from the committed summaries in profiles/:
+    lines `traffic_bytes_per_launch[<class>]: <bytes>` (FETCH_SIZE + WRITE_SIZE, calibrated on a
+    copy kernel) and `valu_wave_instructions_per_launch[<class>]: <count>` (SQ_INSTS_VALU)."""
     import glob
     import re
 
-    out = {}
-    for path in glob.glob(os.path.join(ROOT, "profiles", "*.md")):
+    out = {"traffic": {}, "valu": {}}
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
-            m = re.match(r"traffic_bytes_per_launch\[(.+)\]: (\d+)", line.strip())
+            m = re.match(r"(traffic_bytes|valu_wave_instructions)_per_launch\[(.+)\]: ([0-9.e+]+)$", line.strip())
             if m:
-                out[m.group(1)] = int(m.group(2))
+                out["traffic" if m.group(1) == "traffic_bytes" else "valu"][m.group(2)] = float(m.group(3))
     return out
+
+
+def profiled_traffic():
+    return profiled_counters()["traffic"]
 
 
 def copy_probe(stwo_rocm):
@@ -551,24 +711,41 @@ def copy_probe(stwo_rocm):
     ctx.close()
 
 
-def cpu_baseline(n, W, proto):
-    """The oracle (C + OpenMP restatement of the reference CPU backend) on this host."""
+def cpu_baseline(n, W, proto, gpu_sha=None, samples=10, warmups=2):
+    """The oracle (C + OpenMP restatement of the reference CPU backend) on this host: `warmups`
+    excluded proofs, then `samples` timed proofs at all host threads (median and MAD of the
+    prove time, trace generation excluded as in the reference's prove_seconds), then one proof
+    on one thread.  Every sample's proof must equal the GPU's."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     try:
         import oracle
     except Exception as e:  # oracle not built
         return {"error": str(e)}
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    oracle.set_threads(threads)
+    shas = set()
+    for _ in range(warmups):
+        shas.add(hashlib.sha256(oracle.prove_wide_fibonacci(n, W, **proto)).hexdigest())
     runs = []
-    proof = None
-    for _ in range(2):
-        t0 = time.perf_counter()
+    for _ in range(samples):
         proof = oracle.prove_wide_fibonacci(n, W, **proto)
         runs.append(oracle.last_timings()["prove_total"])
-    return {"value": round(min(runs), 1), "unit": "ms", "cores": threads, "kind": "port",
-            "sample": "2 full proofs of the same workload (wide_fibonacci %dx%d); best prove time (trace generation"
-                      " excluded, as the reference's prove_seconds); proof sha256 %s"
-                      % (n, W, hashlib.sha256(proof).hexdigest()[:16])}
+        shas.add(hashlib.sha256(proof).hexdigest())
+    oracle.set_threads(1)
+    proof = oracle.prove_wide_fibonacci(n, W, **proto)
+    one = oracle.last_timings()["prove_total"]
+    shas.add(hashlib.sha256(proof).hexdigest())
+    oracle.set_threads(threads)
+    med = statistics.median(runs)
+    mad = statistics.median(abs(r - med) for r in runs)
+    return {"value": round(med, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "mad_ms": round(mad, 1), "min_ms": round(min(runs), 1), "samples": samples, "warmups": warmups,
+            "one_thread_ms": round(one, 1),
+            "proofs_equal_gpu": (shas == {gpu_sha}) if gpu_sha else None,
+            "sample": "%d warmups, then %d full proofs of the same workload (wide_fibonacci %dx%d) at %d threads "
+                      "(median, MAD) and 1 proof at 1 thread; prove time excludes trace generation, as the "
+                      "reference's prove_seconds; every proof sha256 %s"
+                      % (warmups, samples, n, W, threads, (gpu_sha or shas.pop())[:16])}
 
 
 if __name__ == "__main__":

@@ -360,9 +360,9 @@ int stwo_rocm_serve(stwo_rocm_ctx* ctx, const stwo_rocm_serve_job* jobs, uint32_
  * on the host).  While enabled, every kernel launch of the context is bracketed by HIP
  * events on the context stream.  profile_read synchronises, returns a JSON object
  * {class: {launches, ms, max_ms, bytes}} (bytes = algorithmic HBM bytes of the launches)
- * and clears the records.  profile_select restricts the records to one kernel class
- * (NULL or "" = every class), so a timed run can carry events around its dominant
- * kernel only; profile_sample records only every k-th matching launch (k >= 1, default 1),
+ * and clears the records.  profile_select restricts the records to the kernel classes named
+ * in kernel_class, separated by '|' (NULL or "" = every class), so a timed run can carry
+ * events around its dominant kernels only; profile_sample records only every k-th matching launch (k >= 1, default 1),
  * for a class launched many times per run. */
 int stwo_rocm_profile_enable(stwo_rocm_ctx* ctx, int enable);
 int stwo_rocm_profile_select(stwo_rocm_ctx* ctx, const char* kernel_class);

@@ -74,6 +74,8 @@ def lib():
         L.ora_prove_wide_fibonacci.argtypes = [ctypes.POINTER(WideFibParams), ctypes.c_char_p, ctypes.c_size_t,
                                                ctypes.POINTER(ctypes.c_size_t)]
         L.ora_last_timings.argtypes = [ctypes.POINTER(ctypes.c_double)]
+        L.ora_set_threads.argtypes = [ctypes.c_int]
+        L.ora_get_threads.restype = ctypes.c_int
         L.ora_xor_trace.argtypes = [ctypes.c_uint32, ctypes.c_uint64, u32p, u32p, u32p, u32p]
         L.ora_logup_denominators.argtypes = [u32p, u32p, u32p, ctypes.c_uint64, u32p, u32p, u32p]
         L.ora_logup_cumulative.restype = ctypes.c_int
@@ -282,6 +284,15 @@ def prove_xor(log_size, log_step, offset, pow_bits=10, log_blowup_factor=1, log_
         if rc != 0:
             raise RuntimeError("oracle prove failed with code %d" % rc)
         return buf.raw[: n.value]
+
+
+def set_threads(n):
+    """OpenMP thread count of the following oracle calls."""
+    lib().ora_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().ora_get_threads())
 
 
 def last_timings():

@@ -892,6 +892,9 @@ void ora_xor_example_columns(uint32_t log_size, uint32_t log_step, uint64_t offs
  * ==================================================================== */
 static double g_timings[8];
 void ora_last_timings(double out[8]) { memcpy(out, g_timings, sizeof g_timings); }
+/* Thread count of the OpenMP regions (the CPU baseline is timed at all host threads and at 1). */
+void ora_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int ora_get_threads(void) { return omp_get_max_threads(); }
 static double now_ms(void) {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);

@@ -109,6 +109,8 @@ int ora_prove_xor(uint32_t log_size, uint32_t log_step, uint64_t offset, const o
 
 /* Stage timings of the last ora_prove_wide_fibonacci call (milliseconds). */
 void ora_last_timings(double out[8]);
+void ora_set_threads(int n);
+int ora_get_threads(void);
 
 /* LogUp / GKR (see stwo_oracle.c). */
 void ora_xor_trace(uint32_t log_size, uint64_t seed, uint32_t *a, uint32_t *b, uint32_t *c, uint32_t *mult);

@@ -872,7 +872,17 @@ int stwo_rocm_profile_enable(stwo_rocm_ctx* ctx, int enable) {
 }
 
 int stwo_rocm_profile_select(stwo_rocm_ctx* ctx, const char* kernel_class) {
-    return guarded(ctx, [&](Ctx& c) { c.profile_only = kernel_class ? kernel_class : ""; });
+    return guarded(ctx, [&](Ctx& c) {
+        c.profile_only.clear();
+        std::string all = kernel_class ? kernel_class : "";
+        size_t at = 0;
+        while (at < all.size()) {
+            size_t bar = all.find('|', at);
+            if (bar == std::string::npos) bar = all.size();
+            if (bar > at) c.profile_only.push_back(all.substr(at, bar - at));
+            at = bar + 1;
+        }
+    });
 }
 
 int stwo_rocm_profile_sample(stwo_rocm_ctx* ctx, uint32_t every) {

@@ -524,19 +524,29 @@ __global__ __launch_bounds__(256) void eval_weights_kernel(const uint4* __restri
     const qm31 v = qmul(qm31{{a.x, a.y}, {a.z, a.w}}, qm31{{b.x, b.y}, {b.z, b.w}});
     w[i] = make_uint4(v.c0.a, v.c0.b, v.c1.a, v.c1.b);
 }
-void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w) {
+uint64_t eval_table_bytes(uint32_t log_size) {
+    const uint32_t h = log_size / 2;
+    return 16ull * ((1ull << h) + (1ull << (log_size - h)));
+}
+
+void launch_eval_weights_into(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w, uint32_t* d_tab) {
     SR_CHECK(log_size <= 30, "eval weights: log size too large");
     Factors fs;
     for (uint32_t j = 0; j < log_size; j++) fs.f[j] = factors[j];
     const uint32_t n = 1u << log_size, h = log_size / 2;
     const uint32_t nt = (1u << h) + (1u << (log_size - h));
-    DBuf tab(&c, 16ull * nt);
     KTimer kt(c, "eval_weights", 16.0 * n);
     hipLaunchKernelGGL(eval_weight_tables_kernel, dim3((nt + 255) / 256), dim3(256), 0, c.stream, fs, log_size, h,
-                       tab.as<uint4>());
-    hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, (const uint4*)tab.as<uint4>(),
+                       (uint4*)d_tab);
+    hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, (const uint4*)d_tab,
                        log_size, h, (uint4*)d_w);
     SR_HIP(hipGetLastError());
+}
+
+void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w) {
+    SR_CHECK(log_size <= 30, "eval weights: log size too large");
+    DBuf tab(&c, eval_table_bytes(log_size));
+    launch_eval_weights_into(c, factors, log_size, d_w, tab.as<uint32_t>());
 }
 
 void launch_eval_weights_dev(Ctx& c, const uint32_t* d_factors, uint32_t log_size, uint32_t* d_w) {
@@ -657,21 +667,35 @@ __global__ __launch_bounds__(256) void eval_reduce_kernel(const unsigned long lo
     x = ev_wave_sum(x);
     if (l == 0) out[k] = mred64(x);
 }
-void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
-                         const uint32_t* d_w, uint32_t* d_out) {
+static uint32_t eval_chunks(uint32_t log_size) {
+    return ((1u << log_size) + 256 * EV_PER_THREAD - 1) / (256 * EV_PER_THREAD);
+}
+
+uint64_t eval_partial_bytes(uint32_t n_cols, uint32_t log_size) {
+    return 8ull * 4 * n_cols * eval_chunks(log_size);
+}
+
+void launch_eval_columns_into(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
+                              const uint32_t* d_w, uint32_t* d_out, unsigned long long* d_part) {
     if (n_cols == 0) return;
     const uint32_t n = 1u << log_size;
-    const uint32_t chunks = (n + 256 * EV_PER_THREAD - 1) / (256 * EV_PER_THREAD);
+    const uint32_t chunks = eval_chunks(log_size);
     const uint32_t groups = (n_cols + EV_CB - 1) / EV_CB;
     const uint32_t nw = n_cols * 4;
-    DBuf part(&c, 8ull * nw * chunks);
     KTimer kt(c, "eval_columns", (4.0 * n_cols + 16.0) * n);
     const uint32_t blocks = (chunks + EV_XCDS - 1) / EV_XCDS * EV_XCDS * groups;
     hipLaunchKernelGGL(eval_columns_kernel, dim3(blocks), dim3(256), 0, c.stream, d_cols, n_cols, log_size,
-                       (const uint4*)d_w, chunks, groups, part.as<unsigned long long>());
+                       (const uint4*)d_w, chunks, groups, d_part);
     hipLaunchKernelGGL(eval_reduce_kernel, dim3((nw + 3) / 4), dim3(256), 0, c.stream,
-                       (const unsigned long long*)part.as<unsigned long long>(), chunks, nw, d_out);
+                       (const unsigned long long*)d_part, chunks, nw, d_out);
     SR_HIP(hipGetLastError());
+}
+
+void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
+                         const uint32_t* d_w, uint32_t* d_out) {
+    if (n_cols == 0) return;
+    DBuf part(&c, eval_partial_bytes(n_cols, log_size));
+    launch_eval_columns_into(c, d_cols, n_cols, log_size, d_w, d_out, part.as<unsigned long long>());
 }
 
 }  // namespace sr

@@ -100,7 +100,7 @@ struct Ctx {
     uint32_t fft_tile_log = 13;   // FFT LDS tile, 2^11..2^13 words
     uint32_t merkle_levels = 10;  // Merkle parent levels per launch, 2..10
     bool profiling = false;
-    std::string profile_only;  // non-empty: record only launches of this class
+    std::vector<std::string> profile_only;  // non-empty: record only launches of these classes
     uint32_t profile_every = 1;  // record every k-th launch that matches
     uint64_t profile_seen = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -134,11 +134,16 @@ struct KTimer {
     double bytes;
     hipEvent_t a = nullptr;
     KTimer(Ctx& cc, std::string k, double b) : c(cc), cls(std::move(k)), bytes(b) {
-        if (c.profiling && (c.profile_only.empty() || c.profile_only == cls) &&
+        if (c.profiling && (c.profile_only.empty() || selected()) &&
             c.profile_seen++ % c.profile_every == 0) {
             a = c.pool_event();
             (void)hipEventRecord(a, c.stream);
         }
+    }
+    bool selected() const {
+        for (const auto& k : c.profile_only)
+            if (k == cls) return true;
+        return false;
     }
     ~KTimer() {
         if (a) {
@@ -339,6 +344,13 @@ void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint
 void launch_eval_weights_dev(Ctx& c, const uint32_t* d_factors, uint32_t log_size, uint32_t* d_w);
 void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
                          const uint32_t* d_w, uint32_t* d_out /* n_cols * 4 words */);
+// The same two steps with caller-provided scratch: the factor tables (eval_table_bytes) and
+// the per-chunk partial sums (eval_partial_bytes); nothing is allocated.
+uint64_t eval_table_bytes(uint32_t log_size);
+uint64_t eval_partial_bytes(uint32_t n_cols, uint32_t log_size);
+void launch_eval_weights_into(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w, uint32_t* d_tab);
+void launch_eval_columns_into(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
+                              const uint32_t* d_w, uint32_t* d_out, unsigned long long* d_part);
 // Wide Fibonacci.
 void launch_widefib_trace(Ctx& c, uint32_t log_n, uint32_t seq_len, uint32_t mode, uint64_t seed, uint32_t* d_out,
                           uint64_t col_stride);

@@ -323,7 +323,9 @@ class Context:
         self._check(self.lib.stwo_rocm_profile_enable(self.h, 1 if enable else 0))
 
     def profile_select(self, kernel_class=None):
-        """Record only launches of `kernel_class` (None: every class)."""
+        """Record only launches of `kernel_class` (a class name or a list of names; None: every class)."""
+        if isinstance(kernel_class, (list, tuple)):
+            kernel_class = "|".join(kernel_class)
         self._check(self.lib.stwo_rocm_profile_select(self.h, kernel_class.encode() if kernel_class else None))
 
     def profile_sample(self, every=1):

@@ -37,3 +37,41 @@ def test_sample_stride_rotates_over_the_launches_of_a_run():
             seen = {(r * n + i) % n for r in range(k) for i in range(n) if (r * n + i) % k == 0}
             assert seen == set(range(n))
         assert n / k < 8  # a handful of bracketed launches per run (about four when n >= 8)
+
+
+def test_sample_stride_for_one_bracketed_launch_per_proof():
+    # the LDE unit has 4 launches per proof; about one of them carries events per proof
+    k = bench._sample_every(4, target=1)
+    assert k > 1 and 4 % k != 0 and k % 4 != 0
+
+
+def test_batched_column_logs_follow_the_seeded_rule():
+    # SplitMix64 of state 0 (Vigna's splitmix64.c) pins the mixer; logs are in range and seeded
+    assert bench._splitmix64(0) == 0xE220A8397B1DCDAF
+    logs = bench.batched_column_logs(bench.BATCHED_SPEC)
+    assert len(logs) == 64 and min(logs) >= 14 and max(logs) <= 22 and len(set(logs)) > 4
+    assert logs == bench.batched_column_logs(bench.BATCHED_SPEC)
+    M = (1 << 64) - 1
+    assert logs[5] == 14 + bench._splitmix64(1 ^ 0xA5A5A5A5 ^ ((5 * 0x9E3779B97F4A7C15) & M)) % 9
+
+
+def test_lde_unit_classes_and_unit_figures():
+    prof = {"fft_ifft_pass m=20 R=13 C=1 cols=100": {}, "fft_ifft_pass m=20 R=7 C=64 cols=100": {},
+            "fft_lde_pass m=21 R=8 C=32 cols=100": {}, "fft_lde_pass m=21 R=13 C=1 cols=100": {},
+            "fft_lde_pass m=21 R=13 C=1 cols=8": {}, "fft_ifft_pass m=21 R=13 C=1 cols=4": {}, "quotients": {}}
+    cls = bench.lde_unit_classes(prof, 20, 100, 1)
+    assert len(cls) == 4 and all(k.endswith("cols=100") for k in cls)
+    ms = {k: 0.25 for k in cls}
+    counters = {"traffic": {k: 1.0e9 for k in cls}, "valu": {k: 1.0e8 for k in cls}}
+    u = bench.unit_figures("x", cls, 1.6e9, ms, counters)
+    assert abs(u["achieved"] - 1600.0) < 1e-6 and abs(u["frac"] - 0.2) < 1e-9
+    assert u["traffic"] == 4e9 and abs(u["traffic_ratio"] - 2.5) < 1e-9
+    assert abs(u["valu"]["bound_ms"] - 4e8 / (256 * 2.4e9) * 1e3) < 1e-3
+    half = bench.unit_figures("x", cls[:1], 1.0e9, ms, counters, share={cls[0]: 0.5})
+    assert abs(half["device_ms"] - 0.125) < 1e-9 and half["traffic"] == 5e8
+
+
+def test_profiled_counters_parse_the_committed_summaries():
+    c = bench.profiled_counters()
+    assert set(c) == {"traffic", "valu"}
+    assert all(v > 0 for v in c["traffic"].values()) and all(v > 0 for v in c["valu"].values())

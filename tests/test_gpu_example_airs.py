@@ -6,7 +6,8 @@ proofs on the GPU through the C-ABI.
   functional protocol).
 - Byte equality with the oracle on configurations no golden covers: the serving sizes,
   the smallest traces, other steps and offsets (including offsets above p and a step of
-  the whole trace), blowups, last-layer bounds, query counts and PoW difficulties.
+  the whole trace), blowups, last-layer bounds, query counts and PoW difficulties; each
+  such proof is also accepted by the verifier restatement (tests/stark_verifier.py).
 - Rejected statements leave the context usable.
 """
 import hashlib
@@ -14,11 +15,18 @@ import hashlib
 import pytest
 
 import oracle
+import stark_verifier as sv
 import stwo_rocm
 from conftest import load_poseidon_goldens, load_xor_goldens, poseidon_kwargs, xor_kwargs
 
 pytestmark = pytest.mark.gpu
 GOLDENS = load_poseidon_goldens()
+
+
+def _pcs(case):
+    return dict(pow_bits=case.get("pow_bits", 10), log_blowup_factor=case.get("log_blowup_factor", 1),
+                log_last_layer_degree_bound=case.get("log_last_layer_degree_bound", 0),
+                n_queries=case.get("n_queries", 3))
 XOR_GOLDENS = load_xor_goldens()
 
 
@@ -47,6 +55,7 @@ def test_gpu_poseidon_proof_equals_oracle(ctx, case):
     exp = oracle.prove_poseidon(**case)
     got, _ = ctx.prove_poseidon(stats=False, **case)
     assert got == exp
+    sv.verify_proof(got, "poseidon", dict(log_n_instances=case["log_n_instances"]), _pcs(case))
 
 
 def test_poseidon_invalid_statement_and_context_survives(ctx):
@@ -97,6 +106,8 @@ def test_gpu_xor_proof_equals_oracle(ctx, case):
     exp = oracle.prove_xor(**case)
     got, _ = ctx.prove_xor(stats=False, **case)
     assert got == exp
+    sv.verify_proof(got, "xor", dict(log_size=case["log_size"], log_step=case["log_step"], offset=case["offset"]),
+                    _pcs(case))
 
 
 def test_xor_invalid_statement_and_context_survives(ctx):

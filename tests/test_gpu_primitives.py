@@ -92,6 +92,10 @@ STAGED = {
     "same_level_blocks": [(32, 6), (3, 7)],
     "full_last_block": [(16, 4), (16, 9), (16, 10)],
     "five_segments_wide": [(3, 8), (13, 9), (16, 10), (16, 12), (16, 14), (2, 16)],
+    # segments of >= 8 message blocks on <= 2^14 leaves take the quad-lane leaf kernel: one that
+    # starts from a previous segment's lifted state and is not the last one, and one that ends
+    "quad_prefix_not_last": [(128, 6), (130, 9), (4, 12)],
+    "quad_after_small_prefix": [(16, 3), (200, 10)],
 }
 
 

@@ -4,7 +4,8 @@
   protocol, reference trace seeds), including 2^20 x 100 (sha256 e6609d05...).
 - Byte equality with the oracle on configurations no golden covers: seeded
   uniform traces, larger query counts, other blowups / last-layer bounds, and a
-  higher PoW difficulty.
+  higher PoW difficulty; each such proof is also accepted by the verifier
+  restatement (tests/stark_verifier.py).
 - Determinism across repeated proofs and between a resident (caller-provided)
   trace and one generated inside the call.
 - Error paths leave the context usable.
@@ -15,6 +16,7 @@ import sys
 import pytest
 
 import oracle
+import stark_verifier as sv
 import stwo_rocm
 from conftest import golden_kwargs, load_goldens
 
@@ -49,6 +51,9 @@ def test_gpu_proof_equals_oracle(ctx, case):
     exp = oracle.prove_wide_fibonacci(**case)
     got, _ = ctx.prove_wide_fibonacci(**case)
     assert got == exp
+    cfg = dict(pow_bits=case["pow_bits"], log_blowup_factor=case.get("log_blowup_factor", 1),
+               log_last_layer_degree_bound=case.get("log_last_layer_degree_bound", 0), n_queries=case["n_queries"])
+    sv.verify_proof(got, "wide_fibonacci", dict(log_n_rows=case["log_n_rows"], sequence_len=case["sequence_len"]), cfg)
 
 
 def test_resident_trace_and_determinism(ctx):

@@ -660,7 +660,7 @@ def serve(ctx, jobs, n_slots, hbm_budget_bytes=0):
     ctx._check(ctx.lib.stwo_rocm_serve(ctx.h, ctypes.addressof(arr), len(jobs), n_slots, hbm_budget_bytes,
                                        ctypes.addressof(res), ctypes.addressof(st)))
     out = [{"digest": bytes(r.digest).hex(), "arrival_ms": r.arrival_ms, "start_ms": r.start_ms,
-            "end_ms": r.end_ms, "slot": r.slot, "output_bytes": r.output_bytes} for r in res]
+            "end_ms": r.end_ms, "slot": r.slot, "status": r.status, "output_bytes": r.output_bytes} for r in res]
     stats = {k: getattr(st, k) for k, _ in ServeStats._fields_}
     return out, stats
 

@@ -141,6 +141,85 @@ int stwo_rocm_compute_quotients(stwo_rocm_ctx* ctx, uint32_t lifting_log_size, c
                                 const uint32_t random_coeff[4], uint32_t* const* out_coords,
                                 stwo_rocm_tree** tree_out);
 
+/* ------------------------------------------------------------------ resident hooks */
+/* The optional hooks the generic prover probes with @hasDecl(B, ...) (src/prover/fri.zig:341-586,
+ * src/prover/pcs/commit_dispatch.zig:34-90), in the shape of the Metal backend's
+ * (src/backends/metal/commit_backend.zig:120-800).  Every hook leaves its results in HBM: buffers
+ * the caller allocated with stwo_rocm_buffer_alloc (the ResidentStorage of
+ * src/backend/resident_storage.zig:1-8) and resident trees.  A Zig RocmBackend wraps the device
+ * pointers as SecureColumnByCoords / LineEvaluation with a ResidentStorage destroyFn that calls
+ * stwo_rocm_buffer_free. */
+
+/* Device-resident Blake2s Fiat-Shamir channel (Blake2sChannel, src/core/channel/blake2s.zig:33-134;
+ * MerkleChannel.mixRoot, src/core/vcs_lifted/blake2_merkle.zig:237-249): digest and draw counter in
+ * HBM, every mix and draw run on the device.  The FRI cascade hook advances it without a host
+ * round trip, as the Metal cascade does with its channel_state words
+ * (metal/commit_backend.zig:700-760). */
+typedef struct stwo_rocm_channel stwo_rocm_channel;
+/* digest NULL: the all-zero initial digest. */
+int stwo_rocm_channel_create(stwo_rocm_ctx* ctx, const uint8_t digest[32], uint32_t n_draws,
+                             stwo_rocm_channel** out);
+int stwo_rocm_channel_destroy(stwo_rocm_ctx* ctx, stwo_rocm_channel* ch);
+/* digest and n_draws as the host channel would hold them (either may be NULL). */
+int stwo_rocm_channel_state(stwo_rocm_ctx* ctx, stwo_rocm_channel* ch, uint8_t digest[32], uint32_t* n_draws);
+/* mixU32s / mixU64 / mixFelts (blake2s.zig:52-88): digest <- Blake2s(digest || LE words), n_draws <- 0.
+ * felts: n QM31 of 4 canonical words each. */
+int stwo_rocm_channel_mix_u32s(stwo_rocm_ctx* ctx, stwo_rocm_channel* ch, const uint32_t* words, uint64_t n);
+int stwo_rocm_channel_mix_u64(stwo_rocm_ctx* ctx, stwo_rocm_channel* ch, uint64_t value);
+int stwo_rocm_channel_mix_felts(stwo_rocm_ctx* ctx, stwo_rocm_channel* ch, const uint32_t* felts, uint64_t n);
+/* MerkleChannel.mixRoot (blake2_merkle.zig:237-249). */
+int stwo_rocm_channel_mix_root(stwo_rocm_ctx* ctx, stwo_rocm_channel* ch, const uint8_t root[32]);
+/* drawSecureFelt (blake2s.zig:109-112 with drawBaseFelts' rejection of words >= 2p): 4 words. */
+int stwo_rocm_channel_draw_secure_felt(stwo_rocm_ctx* ctx, stwo_rocm_channel* ch, uint32_t out[4]);
+/* n_blocks calls of drawU32s (blake2s.zig:90-107): 8 words per block into out. */
+int stwo_rocm_channel_draw_u32s(stwo_rocm_ctx* ctx, stwo_rocm_channel* ch, uint32_t n_blocks, uint32_t* out);
+
+/* prepareAndCommitOwned (commit_dispatch.zig:34-71 -> MetalCommitBackend.prepareAndCommitOwned;
+ * prover/pcs/columns/preparation.zig:113-210): evaluation columns (device, 2^log_sizes[i] words,
+ * bit-reversed circle-domain order) are interpolated in place into their coefficients (retained,
+ * scheme.zig:97 .always), extended to CanonicCoset(log + log_blowup) into ext_out[i] and
+ * committed as one lifted Blake2s Merkle tree (columns of any sizes).  root_out may be NULL. */
+int stwo_rocm_prepare_and_commit_owned(stwo_rocm_ctx* ctx, uint32_t* const* cols, const uint32_t* log_sizes,
+                                       uint32_t n_cols, uint32_t log_blowup, uint32_t* const* ext_out,
+                                       stwo_rocm_tree** tree_out, uint8_t* root_out, double* gpu_ms);
+/* prepareAndCommitPolys (commit_dispatch.zig:74-90; commit_polys.zig:10-67): the same from
+ * coefficient columns (left unchanged). */
+int stwo_rocm_prepare_and_commit_polys(stwo_rocm_ctx* ctx, const uint32_t* const* coeffs, const uint32_t* log_sizes,
+                                       uint32_t n_cols, uint32_t log_blowup, uint32_t* const* ext_out,
+                                       stwo_rocm_tree** tree_out, uint8_t* root_out, double* gpu_ms);
+/* secureColumnFromLine / secureColumnForMerkle (metal/commit_backend.zig:164-197; the Metal
+ * qm31ToCoordinates kernel): n QM31 values (AoS) -> 4 coordinate columns (SoA). */
+int stwo_rocm_secure_column_from_line(stwo_rocm_ctx* ctx, const uint32_t* d_line, uint64_t n,
+                                      uint32_t* const* coords);
+/* commitMerkle over the 4 coordinate columns of a line evaluation (fri.zig:497-517 with
+ * secureColumnForMerkle), hashed straight from the AoS line: 2^log_size QM31. */
+int stwo_rocm_commit_line(stwo_rocm_ctx* ctx, const uint32_t* d_line, uint32_t log_size, stwo_rocm_tree** tree_out,
+                          uint8_t* root_out);
+/* foldLineAndCommitNext (fri.zig:536-551; metal/commit_backend.zig:525-634): n_folds fold_line
+ * steps of the line of log_src (alpha, then alpha^2, ...) into d_dst (2^(log_src - n_folds) QM31),
+ * and the commitment of d_dst (its leaves hashed by the last fold). */
+int stwo_rocm_fri_fold_line_and_commit(stwo_rocm_ctx* ctx, const uint32_t* d_src, uint32_t log_src,
+                                       const uint32_t alpha[4], uint32_t n_folds, uint32_t* d_dst,
+                                       stwo_rocm_tree** tree_out, uint8_t* root_out);
+/* commitFriLayers / commitFriLineCascade (fri.zig:480-490; metal/commit_backend.zig:636-800),
+ * fold_step 1: every inner layer from the line d_line (log_size) down to the last layer
+ * (log_last_size >= 1), with the transcript on the device channel ch.  Per layer of log
+ * l = log_size .. log_last_size+1: commit the line, mixRoot, draw the fold alpha, fold.
+ * trees_out[k] (log_size - log_last_size handles) is the tree of the line of log log_size - k;
+ * d_lines[k] (caller buffers, 2^(log_size-1-k) QM31) receives the line of log log_size-1-k for
+ * k < log_size - log_last_size - 1; d_last_line (2^log_last_size QM31) the last layer's line. */
+int stwo_rocm_fri_commit_layers(stwo_rocm_ctx* ctx, stwo_rocm_channel* ch, const uint32_t* d_line,
+                                uint32_t log_size, uint32_t log_last_size, uint32_t* const* d_lines,
+                                stwo_rocm_tree** trees_out, uint32_t* d_last_line);
+/* Query-value gather of the decommitment (prover/vcs_lifted/decommit.zig:56-237; the Metal
+ * decommit kernels, shaders/core/decommit.metal:11-334): out_host[(i * n_positions + q) *
+ * words_per_value + w] = word w of column i at the lifted row of positions[q] (a tree of max log
+ * lifting_log; lifted row ((p >> (lifting_log - log + 1)) << 1) + (p & 1), leaves.zig:105-111).
+ * words_per_value 1: M31 columns; 4: QM31 lines (AoS). */
+int stwo_rocm_gather_values(stwo_rocm_ctx* ctx, const uint32_t* const* cols, const uint32_t* log_sizes,
+                            uint32_t n_cols, uint32_t words_per_value, uint32_t lifting_log,
+                            const uint64_t* positions, uint32_t n_positions, uint32_t* out_host);
+
 /* ------------------------------------------------------------------ wide Fibonacci */
 /* fillQuadraticRecurrenceTrace (examples/wide_fibonacci/trace.zig:63-72); generate_wide_fibonacci_trace
  * (cuda/ffi.zig:263).  d_out: seq_len columns of 2^log_n_rows, column-major.

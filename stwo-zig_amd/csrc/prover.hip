@@ -259,6 +259,44 @@ void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack
     launch_merkle_parents(c, *out.tree, nullptr, d_root_out);
 }
 
+void fri_fold_commit_down(Ctx& c, TreeOwner& owner, const uint32_t* d_top, uint32_t top_log, uint32_t last_log,
+                          DevChannel* d_ch, uint32_t* const* dst_lines, uint32_t* d_last, uint8_t* d_roots,
+                          std::vector<DeviceTree*>& trees) {
+    SR_CHECK(last_log >= 1 && top_log > last_log && top_log <= 29, "FRI cascade: layer sizes");
+    auto line_at = [&](uint32_t l) -> uint32_t* {
+        if (l == top_log) return (uint32_t*)d_top;
+        if (l == last_log) return d_last;
+        return dst_lines[top_log - 1 - l];
+    };
+    size_t k = 0;
+    // layers top_log-1 .. tail_top: one launch each (fold, commit, channel step); below, the
+    // tail kernel, or a plain fold when the last layer is larger than the tail
+    const uint32_t tail_top = std::max(last_log + 1, std::min(top_log, FRI_TAIL_MAX_LOG + 1));
+    for (uint32_t l = top_log - 1; l >= tail_top; l--) {
+        DeviceTree* t = owner.make(l);
+        launch_fri_commit_line(c, line_at(l + 1), l + 1, line_at(l), *t, d_ch, d_roots + 32 * k++);
+        trees.push_back(t);
+    }
+    if (tail_top <= FRI_TAIL_MAX_LOG + 1) {
+        FriTailArgs ta{};
+        ta.top_log = tail_top;
+        ta.last_log = last_log;
+        ta.ch = d_ch;
+        ta.lines[tail_top] = (uint4*)line_at(tail_top);
+        for (uint32_t l = tail_top - 1;; l--) {
+            ta.lines[l] = (uint4*)line_at(l);
+            if (l == last_log) break;
+            DeviceTree* t = owner.make(l);
+            ta.trees[l] = t->layers;
+            ta.roots[l] = (uint32_t*)(d_roots + 32 * k++);
+            trees.push_back(t);
+        }
+        launch_fri_tail(c, ta);
+    } else {
+        launch_fold_line(c, line_at(last_log + 1), last_log + 1, qzero(), d_last, nullptr, d_ch);
+    }
+}
+
 void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trace_in, std::string& proof,
                           ProveStats* stats) {
     const uint32_t n = prm.log_n_rows, W = prm.sequence_len, B = prm.log_blowup_factor;
@@ -558,42 +596,15 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     if (L - 1 == last_log) {
         launch_fold_circle_into_line(c, cc, L, qzero(), d_last_line, false, nullptr, d_ch);
     } else {
-        // layers L-1 .. tail_top: one launch each (fold, commit, channel step); below, the
-        // tail kernel, or a plain fold when the last layer is larger than the tail
-        const uint32_t tail_top = std::max(last_log + 1, std::min(L - 1, FRI_TAIL_MAX_LOG + 1));
-        for (uint32_t l = L - 1; l >= tail_top; l--) {
-            lines.emplace_back(&c, (1ull << l) * 16);
-            DeviceTree* t = owner.make(l);
-            uint8_t* root_out = d_roots + 32 * (1 + ftrees.size());
-            if (l == L - 1)
-                launch_fri_commit_circle(c, cc, L, lines.back().as<uint32_t>(), *t, d_ch, root_out);
-            else
-                launch_fri_commit_line(c, lines[lines.size() - 2].as<uint32_t>(), l + 1, lines.back().as<uint32_t>(),
-                                       *t, d_ch, root_out);
-            ftrees.push_back(t);
-        }
-        if (tail_top <= FRI_TAIL_MAX_LOG + 1) {
-            FriTailArgs ta{};
-            ta.top_log = tail_top;
-            ta.last_log = last_log;
-            ta.ch = d_ch;
-            ta.lines[tail_top] = lines.back().as<uint4>();
-            for (uint32_t l = tail_top - 1;; l--) {
-                if (l == last_log) {
-                    ta.lines[l] = (uint4*)d_last_line;
-                    break;
-                }
-                lines.emplace_back(&c, (1ull << l) * 16);
-                ta.lines[l] = lines.back().as<uint4>();
-                DeviceTree* t = owner.make(l);
-                ta.trees[l] = t->layers;
-                ta.roots[l] = (uint32_t*)(d_roots + 32 * (1 + ftrees.size()));
-                ftrees.push_back(t);
-            }
-            launch_fri_tail(c, ta);
-        } else {
-            launch_fold_line(c, lines.back().as<uint32_t>(), last_log + 1, qzero(), d_last_line, nullptr, d_ch);
-        }
+        // layer L-1: circle fold, commit and channel step in one launch; then the cascade
+        for (uint32_t k = 0; k < n_inner; k++) lines.emplace_back(&c, (1ull << (L - 1 - k)) * 16);
+        DeviceTree* t = owner.make(L - 1);
+        launch_fri_commit_circle(c, cc, L, lines[0].as<uint32_t>(), *t, d_ch, d_roots + 32);
+        ftrees.push_back(t);
+        std::vector<uint32_t*> below;
+        for (uint32_t k = 1; k < n_inner; k++) below.push_back(lines[k].as<uint32_t>());
+        fri_fold_commit_down(c, owner, lines[0].as<uint32_t>(), L - 1, last_log, d_ch, below.data(), d_last_line,
+                             d_roots + 64, ftrees);
     }
     const uint32_t llog = last_log;
     SR_CHECK(ftrees.size() == n_inner, "FRI layer count");

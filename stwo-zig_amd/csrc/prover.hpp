@@ -90,6 +90,17 @@ void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack
 void commit_coefficients(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
                          uint32_t coeff_log, uint32_t blowup);
 
+// FRI inner layers below a committed line (prover/fri.zig:497-586; the whole-cascade hook
+// commitFriLayers, metal/commit_backend.zig:636-800).  `d_top` (log top_log) is committed and
+// its fold alpha is in d_ch.  Each layer of log top_log-1 .. last_log+1 is folded from the one
+// above, committed into a tree made by `owner` (appended to `trees`, its root copied to
+// d_roots + 32 k for the k-th of them) and its root mixed into d_ch with the next alpha
+// drawn; the line of log last_log is folded into d_last.  dst_lines[k] receives the line of
+// log top_log-1-k (k < top_log-1-last_log).
+void fri_fold_commit_down(Ctx& c, TreeOwner& owner, const uint32_t* d_top, uint32_t top_log, uint32_t last_log,
+                          DevChannel* d_ch, uint32_t* const* dst_lines, uint32_t* d_last, uint8_t* d_roots,
+                          std::vector<DeviceTree*>& trees);
+
 // Components served by the one-trace-tree proof core: the wide-Fibonacci AIR and the Poseidon
 // AIR (examples/poseidon.zig:397-504: 1264 columns, a constant composition evaluation).
 // The periodic XOR example AIR (examples/xor.zig:419-605) adds a preprocessed tree of two
@@ -194,11 +205,13 @@ class BatchedPipeline {
 // mixRoot(root) [+ mixU32s(words[0..n_words))], alpha = drawSecureFelt, powers[k] = alpha^k.
 void launch_tr_commit_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, const uint32_t* words, uint32_t n_words,
                            uint32_t* d_powers, uint32_t n_pow);
-// digest <- H(digest || words[0..n_words)) if n_words > 0 (words in device memory), then
-// n_draws secure felts into d_out (4 words each).
+// digest <- H(digest || words[0..n_words)) if n_words > 0 or mix_empty (words in device
+// memory), then n_draws secure felts into d_out (4 words each).
 void launch_tr_mix_draws(Ctx& c, DevChannel* d_ch, const uint32_t* d_words, uint32_t n_words, uint32_t n_draws,
-                         uint32_t* d_out);
+                         uint32_t* d_out, bool mix_empty = false);
 // [mixRoot(root) if d_root], t = drawSecureFelt, OODS point (x, y) and the point-evaluation factors.
+// n_blocks draws of 8 raw words each (drawU32s) into d_out (8 n_blocks words).
+void launch_tr_draw_words(Ctx& c, DevChannel* d_ch, uint32_t n_blocks, uint32_t* d_out);
 void launch_tr_oods(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, uint32_t n_factors, uint32_t* d_oods,
                     uint32_t* d_factors);
 // mixFelts(sampled values), q = drawSecureFelt, coefficients of the entries (column order)

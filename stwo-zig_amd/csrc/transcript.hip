@@ -130,17 +130,17 @@ __global__ void tr_oods_kernel(DevChannel* __restrict__ ch, const uint32_t* __re
     }
 }
 
-// digest <- Blake2s(digest || words[0..n_words)) when n_words > 0 (mixRoot, mixFelts and
-// mixU32s are all this on little-endian words), then n_draws secure felts into out[0..n_draws).
+// digest <- Blake2s(digest || words[0..n_words)) when `mix` (mixRoot, mixFelts and mixU32s are
+// all this on little-endian words), then n_draws secure felts into out[0..n_draws).
 __global__ void tr_mix_draws_kernel(DevChannel* __restrict__ ch, const uint32_t* __restrict__ words,
-                                    uint32_t n_words, uint32_t n_draws, uint4* __restrict__ out) {
+                                    uint32_t n_words, uint32_t n_draws, uint4* __restrict__ out, uint32_t mix) {
     const uint32_t t = threadIdx.x;
     if (t >= 4) return;
     const QuadLane Lq(t);
     uint32_t dg[8];
     tr_load_dg(ch, dg);
     uint32_t nd = ch->n_draws, st = 0;
-    if (n_words) {
+    if (mix) {
         qchan_mix_long(Lq, dg, words, n_words);
         nd = 0;
     }
@@ -149,6 +149,27 @@ __global__ void tr_mix_draws_kernel(DevChannel* __restrict__ ch, const uint32_t*
         if (t == 0) out[k] = make_uint4(a.c0.a, a.c0.b, a.c1.a, a.c1.b);
     }
     if (t == 0) tr_store(ch, dg, nd, st);
+}
+
+// n_blocks draws of 8 raw words (Blake2sChannel.drawU32s, src/core/channel/blake2s.zig:90-107):
+// block k = Blake2s(digest || LE32(n_draws) || 0x00), n_draws += 1 per block, no rejection.
+__global__ void tr_draw_words_kernel(DevChannel* __restrict__ ch, uint32_t n_blocks, uint32_t* __restrict__ out) {
+    const uint32_t t = threadIdx.x;
+    if (t >= 4) return;
+    const QuadLane Lq(t);
+    const uint32_t ilo = Lq.sel(SR_B2_IV0 ^ 0x01010020u, SR_B2_IV1, SR_B2_IV2, SR_B2_IV3);
+    const uint32_t ihi = Lq.sel(SR_B2_IV4, SR_B2_IV5, SR_B2_IV6, SR_B2_IV7);
+    uint32_t w[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    tr_load_dg(ch, w);
+    uint32_t nd = ch->n_draws;
+    for (uint32_t k = 0; k < n_blocks; k++) {
+        w[8] = nd++;
+        uint32_t olo = ilo, ohi = ihi;
+        b2_compress_quad(Lq, olo, ohi, w, 37, true);
+        out[8 * k + t] = olo;
+        out[8 * k + 4 + t] = ohi;
+    }
+    if (t == 0) ch->n_draws = nd;
 }
 
 // One workgroup: mix the sampled values, draw the quotient coefficient q and fill the
@@ -232,12 +253,19 @@ void launch_tr_commit_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, cons
 }
 
 void launch_tr_mix_draws(Ctx& c, DevChannel* d_ch, const uint32_t* d_words, uint32_t n_words, uint32_t n_draws,
-                         uint32_t* d_out) {
+                         uint32_t* d_out, bool mix_empty) {
     SR_CHECK(n_words == 0 || d_words, "transcript: words");
     SR_CHECK(n_draws == 0 || d_out, "transcript: draw output");
     KTimer kt(c, "transcript", 4.0 * n_words);
     hipLaunchKernelGGL(tr_mix_draws_kernel, dim3(1), dim3(64), 0, c.stream, d_ch, d_words, n_words, n_draws,
-                       (uint4*)d_out);
+                       (uint4*)d_out, (n_words || mix_empty) ? 1u : 0u);
+    SR_HIP(hipGetLastError());
+}
+
+void launch_tr_draw_words(Ctx& c, DevChannel* d_ch, uint32_t n_blocks, uint32_t* d_out) {
+    SR_CHECK(n_blocks == 0 || d_out, "transcript: draw output");
+    KTimer kt(c, "transcript", 32.0 * n_blocks);
+    hipLaunchKernelGGL(tr_draw_words_kernel, dim3(1), dim3(64), 0, c.stream, d_ch, n_blocks, d_out);
     SR_HIP(hipGetLastError());
 }
 

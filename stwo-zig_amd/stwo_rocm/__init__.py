@@ -168,6 +168,31 @@ _SIGNATURES += [
     ("stwo_rocm_batched_destroy", ctypes.c_int, [_vp, _vp]),
     ("stwo_rocm_serve", ctypes.c_int, [_vp, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
                                         ctypes.c_void_p, ctypes.c_void_p]),
+    # resident hook surface
+    ("stwo_rocm_channel_create", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    ("stwo_rocm_channel_destroy", ctypes.c_int, [_vp, _vp]),
+    ("stwo_rocm_channel_state", ctypes.c_int, [_vp, _vp, ctypes.c_char_p, _u32p]),
+    ("stwo_rocm_channel_mix_u32s", ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint64]),
+    ("stwo_rocm_channel_mix_u64", ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
+    ("stwo_rocm_channel_mix_felts", ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint64]),
+    ("stwo_rocm_channel_mix_root", ctypes.c_int, [_vp, _vp, ctypes.c_char_p]),
+    ("stwo_rocm_channel_draw_secure_felt", ctypes.c_int, [_vp, _vp, _u32p]),
+    ("stwo_rocm_channel_draw_u32s", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p]),
+    ("stwo_rocm_prepare_and_commit_owned", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_uint32,
+                                                           ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                           ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]),
+    ("stwo_rocm_prepare_and_commit_polys", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_uint32,
+                                                           ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                           ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]),
+    ("stwo_rocm_secure_column_from_line", ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    ("stwo_rocm_commit_line", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.c_char_p]),
+    ("stwo_rocm_fri_fold_line_and_commit", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, ctypes.c_uint32, _vp,
+                                                           ctypes.POINTER(_vp), ctypes.c_char_p]),
+    ("stwo_rocm_fri_commit_layers", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
+                                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    ("stwo_rocm_gather_values", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                                ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32,
+                                                _u32p]),
     ("stwo_rocm_profile_enable", ctypes.c_int, [_vp, ctypes.c_int]),
     ("stwo_rocm_profile_select", ctypes.c_int, [_vp, ctypes.c_char_p]),
     ("stwo_rocm_profile_sample", ctypes.c_int, [_vp, ctypes.c_uint32]),
@@ -271,6 +296,54 @@ class Tree:
     def free(self):
         if self.h:
             self.ctx._check(self.ctx.lib.stwo_rocm_tree_free(self.ctx.h, self.h))
+            self.h = None
+
+
+class DeviceChannel:
+    """A Blake2s Fiat-Shamir channel resident in HBM (Blake2sChannel: mix*/draw* run on the device)."""
+
+    def __init__(self, ctx, digest=None, n_draws=0):
+        self.ctx = ctx
+        h = _vp()
+        ctx._check(ctx.lib.stwo_rocm_channel_create(ctx.h, bytes(digest) if digest is not None else None, n_draws,
+                                                    ctypes.byref(h)))
+        self.h = h.value
+
+    def state(self):
+        d = ctypes.create_string_buffer(32)
+        n = ctypes.c_uint32()
+        self.ctx._check(self.ctx.lib.stwo_rocm_channel_state(self.ctx.h, self.h, d, ctypes.byref(n)))
+        return d.raw, n.value
+
+    def mix_u32s(self, words):
+        w, pw = _u32(words if len(words) else [0])
+        self.ctx._check(self.ctx.lib.stwo_rocm_channel_mix_u32s(self.ctx.h, self.h, pw, len(words)))
+
+    def mix_u64(self, value):
+        self.ctx._check(self.ctx.lib.stwo_rocm_channel_mix_u64(self.ctx.h, self.h, int(value)))
+
+    def mix_felts(self, felts):
+        f = np.ascontiguousarray(np.asarray(felts, dtype=np.uint32).reshape(-1))
+        n = len(f) // 4
+        f, pf = _u32(f if len(f) else [0])
+        self.ctx._check(self.ctx.lib.stwo_rocm_channel_mix_felts(self.ctx.h, self.h, pf, n))
+
+    def mix_root(self, root):
+        self.ctx._check(self.ctx.lib.stwo_rocm_channel_mix_root(self.ctx.h, self.h, bytes(root)))
+
+    def draw_secure_felt(self):
+        o, po = _u32(np.zeros(4))
+        self.ctx._check(self.ctx.lib.stwo_rocm_channel_draw_secure_felt(self.ctx.h, self.h, po))
+        return [int(x) for x in o]
+
+    def draw_u32s(self, n_blocks=1):
+        o, po = _u32(np.zeros(8 * max(n_blocks, 1)))
+        self.ctx._check(self.ctx.lib.stwo_rocm_channel_draw_u32s(self.ctx.h, self.h, n_blocks, po))
+        return [int(x) for x in o[: 8 * n_blocks]]
+
+    def close(self):
+        if self.h:
+            self.ctx._check(self.ctx.lib.stwo_rocm_channel_destroy(self.ctx.h, self.h))
             self.h = None
 
 
@@ -414,6 +487,73 @@ class Context:
         a, pa = _u32(alpha)
         self._check(self.lib.stwo_rocm_fri_fold_circle_into_line(self.h, _ptr_array(coords), log_src, pa,
                                                                  int(d_dst)))
+
+    # ---- resident hooks (MetalCommitBackend hook set)
+    def channel(self, digest=None, n_draws=0):
+        return DeviceChannel(self, digest, n_draws)
+
+    def _prepare_and_commit(self, fn, cols, log_sizes, log_blowup, ext_out):
+        logs, plogs = _u32(log_sizes if len(log_sizes) else [0])
+        h = _vp()
+        root = ctypes.create_string_buffer(32)
+        ms = ctypes.c_double()
+        self._check(fn(self.h, _ptr_array(cols), plogs, len(cols), log_blowup, _ptr_array(ext_out), ctypes.byref(h),
+                       root, ctypes.byref(ms)))
+        return Tree(self, h.value, root.raw)
+
+    def prepare_and_commit_owned(self, cols, log_sizes, log_blowup, ext_out):
+        """prepareAndCommitOwned: evaluations -> coefficients (in place) -> LDE (ext_out) -> tree."""
+        return self._prepare_and_commit(self.lib.stwo_rocm_prepare_and_commit_owned, cols, log_sizes, log_blowup,
+                                        ext_out)
+
+    def prepare_and_commit_polys(self, coeffs, log_sizes, log_blowup, ext_out):
+        """prepareAndCommitPolys: coefficients -> LDE (ext_out) -> tree."""
+        return self._prepare_and_commit(self.lib.stwo_rocm_prepare_and_commit_polys, coeffs, log_sizes, log_blowup,
+                                        ext_out)
+
+    def secure_column_from_line(self, d_line, n, coords):
+        self._check(self.lib.stwo_rocm_secure_column_from_line(self.h, int(d_line), n, _ptr_array(coords)))
+
+    def commit_line(self, d_line, log_size):
+        h = _vp()
+        root = ctypes.create_string_buffer(32)
+        self._check(self.lib.stwo_rocm_commit_line(self.h, int(d_line), log_size, ctypes.byref(h), root))
+        return Tree(self, h.value, root.raw)
+
+    def fold_line_and_commit_next(self, d_src, log_src, alpha, n_folds, d_dst):
+        a, pa = _u32(alpha)
+        h = _vp()
+        root = ctypes.create_string_buffer(32)
+        self._check(self.lib.stwo_rocm_fri_fold_line_and_commit(self.h, int(d_src), log_src, pa, n_folds, int(d_dst),
+                                                                ctypes.byref(h), root))
+        return Tree(self, h.value, root.raw)
+
+    def commit_fri_layers(self, channel, d_line, log_size, log_last_size, d_lines, d_last_line):
+        """commitFriLayers: trees of the lines of log log_size .. log_last_size+1 (channel on the device)."""
+        n = log_size - log_last_size
+        trees = (_vp * n)()
+        self._check(self.lib.stwo_rocm_fri_commit_layers(self.h, channel.h, int(d_line), log_size, log_last_size,
+                                                         _ptr_array(d_lines), trees, int(d_last_line)))
+        out = []
+        for k in range(n):
+            t = Tree(self, trees[k], None)
+            t.root = self.tree_root(t)
+            out.append(t)
+        return out
+
+    def tree_root(self, tree):
+        root = ctypes.create_string_buffer(32)
+        self._check(self.lib.stwo_rocm_tree_root(self.h, tree.h, root))
+        return root.raw
+
+    def gather_values(self, cols, log_sizes, lifting_log, positions, words_per_value=1):
+        pos = np.ascontiguousarray(positions, dtype=np.uint64)
+        logs, plogs = _u32(log_sizes if len(log_sizes) else [0])
+        o, po = _u32(np.zeros(max(len(cols) * len(pos) * words_per_value, 1)))
+        self._check(self.lib.stwo_rocm_gather_values(self.h, _ptr_array(cols), plogs, len(cols), words_per_value,
+                                                     lifting_log, pos.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                                     len(pos), po))
+        return o[: len(cols) * len(pos) * words_per_value].reshape(len(cols), len(pos), words_per_value)
 
     # ---- quotients (computeLazyQuotients / commitLazyMerkle)
     def compute_quotients(self, lifting_log_size, columns, column_log_sizes, sample_columns, sample_points,

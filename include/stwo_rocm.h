@@ -387,7 +387,10 @@ typedef struct {
 } stwo_rocm_batched_spec;
 int stwo_rocm_batched_create(stwo_rocm_ctx* ctx, const stwo_rocm_batched_spec* spec, stwo_rocm_batched** out);
 /* Launch-shape tuning (results are bit-identical for every setting): FFT LDS tile 2^11..2^13 words,
- * Merkle parent levels per launch 2..10.  Applies to the context and recaptures the graph. */
+ * Merkle parent levels per launch 2..10.  Applies to this pipeline only (the context's other work
+ * keeps its shapes) and recaptures the graph.  A replay recaptures by itself when the context's
+ * twiddle tower has moved since the capture (a larger transform on the context, twiddles_prepare,
+ * release_cached). */
 int stwo_rocm_batched_set_tuning(stwo_rocm_ctx* ctx, stwo_rocm_batched* b, uint32_t fft_tile_log,
                                  uint32_t merkle_levels);
 /* Times every (fft_tile_log in 11..13) x (merkle_levels in {6, 8, 10}) graph over reps replays, checks

@@ -17,7 +17,8 @@
 // Tuning: fft_tile_log (LDS tile of the FFT passes) and merkle_levels (parent levels per
 // launch) change only launch shapes; the autotuner times each candidate graph on the
 // actual shapes, checks that every candidate produces the same result words, and keeps
-// the fastest.
+// the fastest.  The shapes belong to the pipeline: the context's own settings are
+// restored after every record.
 #include <algorithm>
 #include <vector>
 
@@ -58,6 +59,8 @@ uint32_t batched_column_log(const BatchedSpec& s, uint32_t i) {
 }
 
 BatchedPipeline::BatchedPipeline(Ctx& cc, const BatchedSpec& s) : c(cc), sp(s), owner(cc) {
+    tile_log = c.fft_tile_log;
+    levels = c.merkle_levels;
     SR_CHECK(sp.n_columns >= 1 && sp.n_columns <= 4096, "batched: n_columns");
     SR_CHECK(sp.log_min >= 1 && sp.log_min <= sp.log_max && sp.log_max + sp.log_blowup <= 28, "batched: log range");
     SR_CHECK(sp.log_blowup >= 1 && sp.log_blowup <= 4, "batched: log_blowup");
@@ -143,7 +146,22 @@ BatchedPipeline::~BatchedPipeline() {
     if (graph) (void)hipGraphDestroy(graph);
 }
 
+// Applies a pipeline's launch shapes to the context for the scope of one record.
+struct TuningScope {
+    Ctx& c;
+    uint32_t tile, levels;
+    TuningScope(Ctx& cc, uint32_t t, uint32_t l) : c(cc), tile(cc.fft_tile_log), levels(cc.merkle_levels) {
+        c.fft_tile_log = t;
+        c.merkle_levels = l;
+    }
+    ~TuningScope() {
+        c.fft_tile_log = tile;
+        c.merkle_levels = levels;
+    }
+};
+
 void BatchedPipeline::record() {
+    TuningScope ts(c, tile_log, levels);
     const uint32_t B = sp.log_blowup;
     for (auto& sc : classes)
         launch_circle_fft(c, true, sc.d_eval.as<const uint32_t* const>(), sc.d_coeff.as<uint32_t* const>(), sc.n,
@@ -184,10 +202,27 @@ void BatchedPipeline::capture() {
     SR_HIP(hipStreamEndCapture(c.stream, &graph));
     c.profiling = prof;
     SR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    cap_tower[0] = c.d_tw;
+    cap_tower[1] = c.d_itw;
+    cap_tower[2] = c.d_tw2;
+    cap_tower[3] = c.d_itw2;
+    cap_tower_log = c.tower_log;
+}
+
+bool BatchedPipeline::tower_moved() const {
+    return cap_tower[0] != c.d_tw || cap_tower[1] != c.d_itw || cap_tower[2] != c.d_tw2 || cap_tower[3] != c.d_itw2 ||
+           cap_tower_log != c.tower_log;
+}
+
+void BatchedPipeline::set_tuning(uint32_t t, uint32_t l) {
+    tile_log = t;
+    levels = l;
+    capture();
 }
 
 double BatchedPipeline::replay(uint32_t n) {
     SR_CHECK(exec, "batched: graph not captured");
+    if (tower_moved()) capture();
     hipEvent_t a, b;
     SR_HIP(hipEventCreate(&a));
     SR_HIP(hipEventCreate(&b));
@@ -225,15 +260,15 @@ std::vector<uint32_t> BatchedPipeline::result() {
 }
 
 void BatchedPipeline::autotune(uint32_t reps) {
-    const uint32_t tiles[3] = {11, 12, 13}, levels[3] = {6, 8, 10};
+    const uint32_t tiles[3] = {11, 12, 13}, levels_set[3] = {6, 8, 10};
     double best = 1e30;
-    uint32_t bt = c.fft_tile_log, bl = c.merkle_levels;
+    uint32_t bt = tile_log, bl = levels;
     std::vector<uint32_t> ref;
     tuning.clear();
     for (uint32_t t : tiles)
-        for (uint32_t l : levels) {
-            c.fft_tile_log = t;
-            c.merkle_levels = l;
+        for (uint32_t l : levels_set) {
+            tile_log = t;
+            levels = l;
             capture();
             replay(1);  // warm
             const double ms = replay(reps);
@@ -247,9 +282,7 @@ void BatchedPipeline::autotune(uint32_t reps) {
                 bl = l;
             }
         }
-    c.fft_tile_log = bt;
-    c.merkle_levels = bl;
-    capture();
+    set_tuning(bt, bl);
 }
 
 }  // namespace sr

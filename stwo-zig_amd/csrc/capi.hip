@@ -711,9 +711,8 @@ int stwo_rocm_batched_set_tuning(stwo_rocm_ctx* ctx, stwo_rocm_batched* b, uint3
         SR_CHECK(b && b->p, "null argument");
         SR_CHECK(fft_tile_log >= 11 && fft_tile_log <= 13, "fft_tile_log must be 11..13");
         SR_CHECK(merkle_levels >= 2 && merkle_levels <= 10, "merkle_levels must be 2..10");
-        c.fft_tile_log = fft_tile_log;
-        c.merkle_levels = merkle_levels;
-        b->p->capture();
+        (void)c;
+        b->p->set_tuning(fft_tile_log, merkle_levels);
     });
 }
 
@@ -721,9 +720,10 @@ int stwo_rocm_batched_autotune(stwo_rocm_ctx* ctx, stwo_rocm_batched* b, uint32_
                                uint32_t* merkle_levels, char* json_out, size_t cap) {
     return guarded(ctx, [&](Ctx& c) {
         SR_CHECK(b && b->p && fft_tile_log && merkle_levels, "null argument");
+        (void)c;
         b->p->autotune(reps ? reps : 3);
-        *fft_tile_log = c.fft_tile_log;
-        *merkle_levels = c.merkle_levels;
+        *fft_tile_log = b->p->tile_log;
+        *merkle_levels = b->p->levels;
         if (json_out && cap) {
             std::string js = "[";
             for (size_t i = 0; i < b->p->tuning.size(); i++) {

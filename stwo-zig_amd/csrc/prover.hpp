@@ -173,15 +173,23 @@ class BatchedPipeline {
     };
     BatchedPipeline(Ctx& c, const BatchedSpec& s);
     ~BatchedPipeline();
-    void record();                  // the launch sequence (eager, or under capture)
-    void capture();                 // capture record() into the graph with the current tuning
-    double replay(uint32_t n);      // n graph launches; device ms per launch
+    void record();                  // the launch sequence (eager, or under capture), with this pipeline's tuning
+    void capture();                 // capture record() into the graph
+    double replay(uint32_t n);      // n graph launches (recaptured first if the tower moved); ms per launch
     std::vector<uint32_t> result();  // roots, final channel digest and draw counter
     void autotune(uint32_t reps);    // pick fft_tile_log x merkle_levels, recapture
+    void set_tuning(uint32_t fft_tile_log, uint32_t merkle_levels);  // recaptures
     std::vector<Tuning> tuning;
+    uint32_t tile_log = 13, levels = 10;  // this pipeline's launch shapes (the context's are left alone)
     uint64_t input_words() const;
 
    private:
+    // The twiddle tower the graph was captured against: a later ensure_tower on the context
+    // (a larger proof, twiddles_prepare) moves it and returns the old arrays to the arena
+    // cache, so replay recaptures when these differ from the context's.
+    const uint32_t* cap_tower[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t cap_tower_log = 0;
+    bool tower_moved() const;
     struct SizeClass {
         uint32_t log = 0, n = 0;
         DBuf d_eval, d_coeff, d_lde;

@@ -74,3 +74,37 @@ def test_batched_autotune_picks_a_candidate_and_keeps_results(ctx):
     finally:
         bp.set_tuning(13, 10)
         bp.close()
+
+
+def test_batched_replay_after_the_tower_moves_and_tuning_stays_in_the_pipeline():
+    """A captured graph reads the context's twiddle tower.  Growing the tower (a larger
+    transform on the context) returns the old arrays to the arena cache and release_cached
+    frees them; the next replay must recapture against the new tower, not read freed memory.
+    The pipeline's launch shapes must not leak into the context's other work."""
+    import stwo_rocm
+
+    spec = SPECS["small"]
+    ref = oracle.batched_pipeline(*spec)
+    with stwo_rocm.Context(0) as c2:  # a fresh context: its tower covers only this pipeline
+        bp = c2.batched(*spec)
+        try:
+            bp.set_tuning(11, 6)
+            words, _ = bp.run(1, graph=True)
+            assert np.array_equal(words, ref)
+            c2.prepare_twiddles(24)  # the tower grows: the old arrays go to the cache
+            c2.release_cached()      # ... and are freed
+            junk = [c2.to_device(np.full(1 << 20, 0x5A5A5A5A, dtype=np.uint32)) for _ in range(8)]
+            for _ in range(2):
+                words, _ = bp.run(1, graph=True)
+                assert np.array_equal(words, ref)
+            for j in junk:
+                j.free()
+            # the context kept its own shapes: a proof's wide Merkle levels run 10 per launch
+            c2.profile_enable(True)
+            c2.prove_wide_fibonacci(18, 8, stats=False)
+            classes = c2.profile_read()
+            c2.profile_enable(False)
+            assert any(k.startswith("merkle_parents L=19 levels=10") for k in classes), sorted(classes)
+            assert not any("levels=6" in k for k in classes)
+        finally:
+            bp.close()

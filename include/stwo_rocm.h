@@ -327,6 +327,15 @@ int stwo_rocm_logup_denominators(stwo_rocm_ctx* ctx, const uint32_t* d_a, const 
  * denominator returns STWO_ROCM_EINVAL. */
 int stwo_rocm_logup_cumulative(stwo_rocm_ctx* ctx, int num_kind, const uint32_t* d_num, const uint32_t* d_den,
                                uint64_t n, uint32_t* d_sums, uint32_t claimed[4]);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same result as
+ * stwo_rocm_logup_cumulative, with no allocation: the scan's block partial sums, the total and
+ * the zero-denominator flag live in a fixed scratch array of the library on each device
+ * (1 MiB + 256 B, held until the process ends).  n <= 2^27 rows, else STWO_ROCM_EINVAL.  Every
+ * call on a device shares that device's array: calls that overlap on one GPU (from several
+ * contexts or threads) race. */
+int stwo_rocm_logup_cumulative_noalloc(stwo_rocm_ctx* ctx, int num_kind, const uint32_t* d_num,
+                                       const uint32_t* d_den, uint64_t n, uint32_t* d_sums, uint32_t claimed[4]);
 /* GkrOps.genEqEvals (src/backend/gkr_ops.zig:1-23; gkr_circuit.zig:651-674): d_out[i] (2^n QM31) =
  * scale * prod_j (bit_{n-1-j}(i) ? y_j : 1 - y_j); y is a host array of n QM31. */
 int stwo_rocm_gkr_gen_eq_evals(stwo_rocm_ctx* ctx, const uint32_t* y, uint32_t n, const uint32_t scale[4],

@@ -606,6 +606,14 @@ int stwo_rocm_logup_cumulative(stwo_rocm_ctx* ctx, int num_kind, const uint32_t*
     });
 }
 
+int stwo_rocm_logup_cumulative_noalloc(stwo_rocm_ctx* ctx, int num_kind, const uint32_t* d_num,
+                                       const uint32_t* d_den, uint64_t n, uint32_t* d_sums, uint32_t claimed[4]) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(d_den && d_sums && claimed, "null argument");
+        qstore(logup_cumulative_noalloc(c, num_kind, d_num, d_den, n, d_sums), claimed);
+    });
+}
+
 int stwo_rocm_gkr_gen_eq_evals(stwo_rocm_ctx* ctx, const uint32_t* y, uint32_t n, const uint32_t scale[4],
                                uint32_t* d_out) {
     return guarded(ctx, [&](Ctx& c) {

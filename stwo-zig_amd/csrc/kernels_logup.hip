@@ -12,6 +12,8 @@
 // words: sums are taken per coordinate in 64-bit integers (atomic adds across
 // blocks), the prefix scan adds QM31 values in any tree order, and inverses are
 // per element (the inverse is unique).
+#include <mutex>
+
 #include "runtime.hpp"
 
 namespace sr {
@@ -306,14 +308,15 @@ void launch_logup_denominators_dev(Ctx& c, const uint32_t* a, const uint32_t* b,
     SR_HIP(hipGetLastError());
 }
 
+static uint64_t scan_blocks(uint64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+// The three scan launches; d_bs holds scan_blocks(n) block sums (16 B each).
 static void launch_cumulative(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n,
-                              uint32_t* sums, DBuf& bs, uint4* d_total, uint32_t* d_flag) {
+                              uint32_t* sums, uint4* d_bs, uint4* d_total, uint32_t* d_flag) {
     SR_CHECK(n > 0, "cumulative column: empty");
     SR_CHECK(num_kind >= 0 && num_kind <= 2 && (num_kind == 2 || num), "cumulative column: numerators");
-    const uint64_t n_blocks = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const uint64_t n_blocks = scan_blocks(n);
     SR_CHECK(n_blocks < (1ull << 31), "cumulative column: too long");
-    bs = DBuf(&c, 16 * n_blocks + 16);
-    uint4* d_bs = bs.as<uint4>();
     launch_fill_u32(c, d_flag, 1, 0);
     {
         KTimer kt(c, "logup_scan", (num_kind == 0 ? 16.0 : num_kind == 1 ? 4.0 : 0.0) * n + 32.0 * n);
@@ -329,9 +332,10 @@ static void launch_cumulative(Ctx& c, int num_kind, const uint32_t* num, const u
 }
 
 qm31 logup_cumulative(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n, uint32_t* sums) {
-    DBuf tot(&c, 32), bs;
+    SR_CHECK(n > 0, "cumulative column: empty");
+    DBuf tot(&c, 32), bs(&c, 16 * scan_blocks(n) + 16);
     uint4* d_total = tot.as<uint4>();
-    launch_cumulative(c, num_kind, num, den, n, sums, bs, d_total, (uint32_t*)(d_total + 1));
+    launch_cumulative(c, num_kind, num, den, n, sums, bs.as<uint4>(), d_total, (uint32_t*)(d_total + 1));
     uint32_t* h = (uint32_t*)c.pinned(32);
     SR_HIP(hipMemcpyAsync(h, d_total, 20, hipMemcpyDeviceToHost, c.stream));
     c.sync();
@@ -342,8 +346,52 @@ qm31 logup_cumulative(Ctx& c, int num_kind, const uint32_t* num, const uint32_t*
 void logup_cumulative_dev(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n, uint32_t* sums,
                           uint32_t* d_total, uint32_t* d_flag) {
     SR_CHECK(((uintptr_t)d_total & 15) == 0 && d_flag, "cumulative column: total must be 16-byte aligned");
-    DBuf bs;
-    launch_cumulative(c, num_kind, num, den, n, sums, bs, (uint4*)d_total, d_flag);
+    SR_CHECK(n > 0, "cumulative column: empty");
+    DBuf bs(&c, 16 * scan_blocks(n) + 16);
+    launch_cumulative(c, num_kind, num, den, n, sums, bs.as<uint4>(), (uint4*)d_total, d_flag);
+}
+
+// ---- the cumulative column without allocation ----
+// A fixed per-device scratch array holds what the arena version allocates per call: the total
+// (16 B) and the zero-denominator flag at offset 0, the block sums from offset 256.  Every call
+// on a device shares that device's copy (the flag is cleared by a fill kernel each call; the
+// total and the block sums are written before they are read), so calls that overlap on one GPU
+// race.  The host never names the array itself: each device's address comes from
+// hipGetSymbolAddress at context set-up.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 256 + 16 * (LOGUP_NOALLOC_MAX_ROWS / SCAN_TILE);
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static uint8_t* g_scratch_ptr[MAX_DEVICES] = {};
+
+void logup_scratch_init(int device) {
+    SR_CHECK(device >= 0 && device < MAX_DEVICES, "scratch: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    SR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the calling thread has `device` current
+    SR_CHECK(p, "scratch: no device address");
+    g_scratch_ptr[device] = (uint8_t*)p;
+}
+
+qm31 logup_cumulative_noalloc(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n,
+                              uint32_t* sums) {
+    SR_CHECK(n > 0, "cumulative column: empty");
+    SR_CHECK(n <= LOGUP_NOALLOC_MAX_ROWS, "cumulative column without allocation: at most 2^27 rows (the scratch "
+                                          "array's size)");
+    uint8_t* base;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = c.device >= 0 && c.device < MAX_DEVICES ? g_scratch_ptr[c.device] : nullptr;
+    }
+    SR_CHECK(base, "scratch: not set up for this device");
+    uint4* d_total = (uint4*)base;
+    launch_cumulative(c, num_kind, num, den, n, sums, (uint4*)(base + 256), d_total, (uint32_t*)(base + 16));
+    uint32_t h[5];
+    SR_HIP(hipMemcpyAsync(h, d_total, sizeof h, hipMemcpyDeviceToHost, c.stream));
+    SR_HIP(hipStreamSynchronize(c.stream));
+    SR_CHECK(h[4] == 0, "LogUp: zero denominator");
+    return qload(h);
 }
 
 void launch_gkr_eq_evals(Ctx& c, const uint32_t* d_y, uint32_t n, qm31 scale, uint32_t* d_out) {

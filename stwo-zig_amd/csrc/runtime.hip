@@ -73,6 +73,7 @@ void ctx_init(Ctx& c, int device, uint64_t hbm_budget) {
     c.leaf_seed = b2_prefix_seed("leaf");
     c.node_seed = b2_prefix_seed("node");
     c.gen = make_gen_doubles();
+    logup_scratch_init(device);
     const size_t cnt_bytes = 4ull * (TREE_XR_OFFSET + 8 * TREE_MAX_BLOCKS);
     c.d_tree_cnt = (uint32_t*)c.alloc(cnt_bytes);
     SR_HIP(hipMemsetAsync(c.d_tree_cnt, 0, cnt_bytes, c.stream));

@@ -296,6 +296,15 @@ qm31 logup_cumulative(Ctx& c, int num_kind, const uint32_t* num, const uint32_t*
 // zero-denominator flag to *d_flag.
 void logup_cumulative_dev(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n, uint32_t* sums,
                           uint32_t* d_total, uint32_t* d_flag);
+// The same as logup_cumulative with no allocation: the block sums, the total and the flag live
+// in a fixed per-device scratch array (n <= LOGUP_NOALLOC_MAX_ROWS); calls that overlap on one
+// device race on it.  logup_scratch_init (once per device, that device current; called by
+// ctx_init) looks up the array's device address.
+constexpr int MAX_DEVICES = 64;
+constexpr uint64_t LOGUP_NOALLOC_MAX_ROWS = 1ull << 27;
+void logup_scratch_init(int device);
+qm31 logup_cumulative_noalloc(Ctx& c, int num_kind, const uint32_t* num, const uint32_t* den, uint64_t n,
+                              uint32_t* sums);
 // Denominators with z = d_z_alpha[0..4) and alpha = d_z_alpha[4..8) read on the device.
 void launch_logup_denominators_dev(Ctx& c, const uint32_t* a, const uint32_t* b, const uint32_t* cc, uint64_t n,
                                    const uint32_t* d_z_alpha, uint32_t* den);

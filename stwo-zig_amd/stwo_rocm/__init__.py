@@ -147,6 +147,8 @@ _SIGNATURES += [
     ("stwo_rocm_xor_trace", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, _vp, _vp]),
     ("stwo_rocm_logup_denominators", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_uint64, _u32p, _u32p, _vp]),
     ("stwo_rocm_logup_cumulative", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64, _vp, _u32p]),
+    ("stwo_rocm_logup_cumulative_noalloc", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64, _vp,
+                                                           _u32p]),
     ("stwo_rocm_gkr_gen_eq_evals", ctypes.c_int, [_vp, _u32p, ctypes.c_uint32, _u32p, _vp]),
     ("stwo_rocm_gkr_next_layer", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
     ("stwo_rocm_gkr_sum_as_poly_in_first_variable", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_uint32,
@@ -662,6 +664,14 @@ class Context:
         cl, pcl = _u32(np.zeros(4))
         self._check(self.lib.stwo_rocm_logup_cumulative(self.h, num_kind, int(d_num) if d_num is not None else None,
                                                         int(d_den), n, int(d_sums), pcl))
+        return cl
+
+    def logup_cumulative_noalloc(self, num_kind, d_num, d_den, n, d_sums):
+        """logup_cumulative with no allocation (fixed per-device scratch array; n <= 2^27)."""
+        cl, pcl = _u32(np.zeros(4))
+        self._check(self.lib.stwo_rocm_logup_cumulative_noalloc(self.h, num_kind,
+                                                                int(d_num) if d_num is not None else None,
+                                                                int(d_den), n, int(d_sums), pcl))
         return cl
 
     def gkr_gen_eq_evals(self, y, scale, d_out):

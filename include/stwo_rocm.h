@@ -418,8 +418,12 @@ int stwo_rocm_batched_destroy(stwo_rocm_ctx* ctx, stwo_rocm_batched* b);
  * admission (src/prover/native/resource_admission.zig) and its prover session loop
  * (src/tools/metal_prover_session/app.zig): n_slots proof slots (1..16), each with its own device
  * context and HIP stream; requests become eligible at arrival_ms (relative to the call, sorted) and
- * the FIFO head is admitted when a slot is idle and its estimated HBM footprint fits
- * hbm_budget_bytes (0 = unlimited).  Jobs on different slots overlap on the device.  Job kinds: 0
+ * the FIFO head is admitted when a slot is idle and it fits hbm_budget_bytes (0 = unlimited): the
+ * device memory every slot holds while idle (its tower, tables and cached arena blocks), plus the
+ * footprint estimates of the jobs in flight and of the head job, must not exceed the budget; an
+ * idle slot trims its cached blocks, and running slots trim theirs when their jobs end, until the
+ * head fits.  stats->peak_device_bytes reports the device memory the slots really held.  Jobs on
+ * different slots overlap on the device.  Job kinds: 0
  * wide-Fibonacci proof (functional protocol, seeded uniform trace: trace_mode 1 of
  * stwo_rocm_prove_wide_fibonacci), 1 XOR-lookup LogUp workload (stwo_rocm_logup_xor_pipeline,
  * log_n 16..24), 2 Poseidon AIR proof (stwo_rocm_prove_poseidon, log_n = log_n_instances 4..24,
@@ -440,6 +444,7 @@ typedef struct {
 typedef struct {
     double makespan_ms, p50_ms, p99_ms, mean_ms, jobs_per_s;
     uint64_t peak_admitted_bytes;
+    uint64_t peak_device_bytes; /* device memory held by all slots (live + cached arena blocks), peak */
     uint32_t max_in_flight, n_slots;
 } stwo_rocm_serve_stats;
 int stwo_rocm_serve(stwo_rocm_ctx* ctx, const stwo_rocm_serve_job* jobs, uint32_t n_jobs, uint32_t n_slots,

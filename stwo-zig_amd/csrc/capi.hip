@@ -813,6 +813,7 @@ int stwo_rocm_serve(stwo_rocm_ctx* ctx, const stwo_rocm_serve_job* jobs, uint32_
         stats->mean_ms = st.mean_ms;
         stats->jobs_per_s = st.jobs_per_s;
         stats->peak_admitted_bytes = st.peak_admitted_bytes;
+        stats->peak_device_bytes = st.peak_device_bytes;
         stats->max_in_flight = st.max_in_flight;
         stats->n_slots = st.n_slots;
     });

@@ -243,6 +243,7 @@ struct ServeOutcome {
 struct ServeStats {
     double makespan_ms, p50_ms, p99_ms, mean_ms, jobs_per_s;
     uint64_t peak_admitted_bytes;
+    uint64_t peak_device_bytes;  // device bytes held by all slots (live + cached), peak of the call
     uint32_t max_in_flight, n_slots;
 };
 uint64_t serve_job_bytes(const ServeJob& j);  // admission estimate of a job's HBM footprint

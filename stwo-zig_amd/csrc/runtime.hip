@@ -47,6 +47,7 @@ void* Ctx::alloc(size_t bytes) {
     live_blocks[p] = sz;
     bytes_live += sz;
     if (bytes_live > peak_live) peak_live = bytes_live;
+    if (ledger) ledger->add(sz);
     return p;
 }
 
@@ -89,6 +90,8 @@ void ctx_release(Ctx& c) {
     for (auto& kv : c.free_blocks) (void)hipFree(kv.second);
     c.live_blocks.clear();
     c.free_blocks.clear();
+    if (c.ledger) c.ledger->sub(c.bytes_live + c.bytes_cached);
+    c.bytes_live = c.bytes_cached = 0;
     if (c.h_pinned) (void)hipHostFree(c.h_pinned);
     if (c.h_ring) (void)hipHostFree(c.h_ring);
     c.h_pinned = nullptr;
@@ -107,6 +110,7 @@ void Ctx::trim() {
     (void)hipStreamSynchronize(stream);
     for (auto& kv : free_blocks) (void)hipFree(kv.second);
     free_blocks.clear();
+    if (ledger) ledger->sub(bytes_cached);
     bytes_cached = 0;
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -53,6 +54,20 @@ struct DevChannel {
     uint32_t alpha2[4];  // its square
 };
 
+// Device bytes held (live + cached arena blocks) by a set of contexts that share one HBM
+// budget (the serving slots), with the peak since the last reset.
+struct MemLedger {
+    std::atomic<uint64_t> held{0}, peak{0};
+    void add(uint64_t b) {
+        const uint64_t h = held.fetch_add(b) + b;
+        uint64_t p = peak.load();
+        while (h > p && !peak.compare_exchange_weak(p, h)) {
+        }
+    }
+    void sub(uint64_t b) { held.fetch_sub(b); }
+    void reset_peak() { peak.store(held.load()); }
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -62,6 +77,7 @@ struct Ctx {
     uint64_t bytes_live = 0, bytes_cached = 0, peak_live = 0;
     std::multimap<size_t, void*> free_blocks;
     std::unordered_map<void*, size_t> live_blocks;
+    MemLedger* ledger = nullptr;  // shared count of device bytes held (set before ctx_init), or none
 
     // Twiddle tower of CanonicCoset(tower_log) (twiddle_tower.zig:29-115).
     uint32_t tower_log = 0;

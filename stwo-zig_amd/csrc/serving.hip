@@ -54,6 +54,7 @@ struct Slot {
 }  // namespace
 
 struct ServePool {
+    MemLedger ledger;  // device bytes held by all slots (declared first: outlives the slots)
     std::vector<std::unique_ptr<Slot>> slots;
 };
 ServePool* serve_pool_new() { return new ServePool(); }
@@ -81,12 +82,16 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
     std::vector<std::unique_ptr<Slot>>& slots = pool.slots;
     while (slots.size() < n_slots) {
         std::unique_ptr<Slot> sl(new Slot());
+        sl->c.ledger = &pool.ledger;
         ctx_init(sl->c, device, 0);
         uint8_t z = 0;
         sl->c.stage(&z, 1);
         (void)sl->c.pinned(1u << 20);
         slots.push_back(std::move(sl));
     }
+    // Under a budget, slots left from an earlier call with more slots would hold memory that
+    // this call does not count: release them.
+    if (hbm_budget) slots.resize(n_slots);
     for (uint32_t s = 0; s < n_slots; s++) {
         slots[s]->c.ensure_tower(max_log);
         slots[s]->c.sync();
@@ -99,6 +104,20 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
     uint32_t in_flight = 0, max_in_flight = 0;
     bool done_arriving = false;
     std::string first_error;
+    // Device bytes each slot holds when idle (live: tower, counters, domain tables; cached arena
+    // blocks), published at its idle points.  The budget covers what the slots really hold: a
+    // job is admitted only if the idle holdings of every slot, the estimates of the jobs in
+    // flight (each job can allocate at most its estimate beyond its slot's holdings) and its own
+    // estimate fit; otherwise the admitting slot trims its cache, and running slots trim theirs
+    // when their jobs end, until it fits.
+    std::vector<uint64_t> idle_held(n_slots), idle_cached(n_slots);
+    for (uint32_t s = 0; s < n_slots; s++) {
+        idle_held[s] = slots[s]->c.bytes_live + slots[s]->c.bytes_cached;
+        idle_cached[s] = slots[s]->c.bytes_cached;
+    }
+    bool trim_wanted = false;
+    uint64_t version = 0;  // bumped on every change the admission depends on
+    pool.ledger.reset_peak();
     const double t0 = now_ms();
 
     auto worker = [&](uint32_t sid) {
@@ -113,19 +132,43 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
             size_t ji;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv.wait(lk, [&] {
-                    if (!first_error.empty()) return true;
-                    if (ready.empty()) return done_arriving;
+                for (;;) {
+                    cv.wait(lk, [&] { return !first_error.empty() || !ready.empty() || done_arriving; });
+                    if (!first_error.empty() || ready.empty()) return;
                     const uint64_t need = serve_job_bytes(jobs[ready.front()]);
-                    return hbm_budget == 0 || used + need <= hbm_budget;
-                });
-                if (!first_error.empty() || ready.empty()) return;
+                    uint64_t idle = 0, cached = 0;
+                    for (uint32_t t = 0; t < n_slots; t++) {
+                        idle += idle_held[t];
+                        cached += idle_cached[t];
+                    }
+                    if (hbm_budget == 0 || idle + used + need <= hbm_budget) break;
+                    if (c.bytes_cached > 0) {
+                        // this slot is idle: give its cached blocks back to the device
+                        lk.unlock();
+                        c.trim();
+                        lk.lock();
+                        idle_held[sid] = c.bytes_live;
+                        idle_cached[sid] = 0;
+                        version++;
+                        continue;
+                    }
+                    if (in_flight == 0 && cached == 0) {
+                        first_error = "serve: a job does not fit the HBM budget beside the slots' resident memory";
+                        cv.notify_all();
+                        return;
+                    }
+                    trim_wanted = true;
+                    const uint64_t seen = version;
+                    cv.wait(lk, [&] { return !first_error.empty() || version != seen; });
+                }
+                trim_wanted = false;
                 ji = ready.front();
                 ready.pop_front();
                 used += serve_job_bytes(jobs[ji]);
                 peak = std::max(peak, used);
                 in_flight++;
                 max_in_flight = std::max(max_in_flight, in_flight);
+                version++;
             }
             ServeOutcome& o = out[ji];
             o.slot = sid;
@@ -187,9 +230,17 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
             }
             o.end_ms = now_ms() - t0;
             {
-                std::lock_guard<std::mutex> lk(mu);
+                std::unique_lock<std::mutex> lk(mu);
+                if (trim_wanted && c.bytes_cached > 0) {
+                    lk.unlock();
+                    c.trim();
+                    lk.lock();
+                }
                 used -= serve_job_bytes(jobs[ji]);
                 in_flight--;
+                idle_held[sid] = c.bytes_live + c.bytes_cached;
+                idle_cached[sid] = c.bytes_cached;
+                version++;
             }
             cv.notify_all();
         }
@@ -205,6 +256,7 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
             out[i].arrival_ms = jobs[i].arrival_ms;
             ready.push_back(i);
             if (i + 1 == jobs.size()) done_arriving = true;
+            version++;
         }
         cv.notify_all();
     }
@@ -228,6 +280,7 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
     st.mean_ms = sum / (double)jobs.size();
     st.jobs_per_s = (double)jobs.size() / (st.makespan_ms * 1e-3);
     st.peak_admitted_bytes = peak;
+    st.peak_device_bytes = pool.ledger.peak.load();
     st.max_in_flight = max_in_flight;
     st.n_slots = n_slots;
 }

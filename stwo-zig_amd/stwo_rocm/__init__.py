@@ -798,7 +798,8 @@ class ServeResult(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [("makespan_ms", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("jobs_per_s", ctypes.c_double),
-                ("peak_admitted_bytes", ctypes.c_uint64), ("max_in_flight", ctypes.c_uint32),
+                ("peak_admitted_bytes", ctypes.c_uint64), ("peak_device_bytes", ctypes.c_uint64),
+                ("max_in_flight", ctypes.c_uint32),
                 ("n_slots", ctypes.c_uint32)]
 
 

@@ -75,3 +75,22 @@ def test_serving_slots_persist_and_rebuild_after_release(ctx):
     c, _ = stwo_rocm.serve(ctx, jobs, n_slots=4)
     assert [r["digest"] for r in a] == [r["digest"] for r in b] == [r["digest"] for r in c]
     assert a[0]["digest"] == _ref_digest(jobs[0])
+
+
+def test_serving_budget_bounds_the_device_memory_the_slots_hold(ctx):
+    """The budget covers what the slots really hold (live and cached arena blocks of every slot,
+    towers included), not only the admission estimates: the peak of the device memory held by
+    the slots stays within the budget, while the same jobs give the same results as without one."""
+    kinds = [(0, 16, 64, 400 + i) for i in range(8)] + [(2, 14, 0, 0), (1, 16, 0, 9), (3, 15, 2, 7), (0, 17, 20, 99)]
+    jobs = [k + (0.1 * i,) for i, k in enumerate(kinds)]
+    budget = 1 << 30
+    _say("serving under a 1 GiB budget")
+    res, st = stwo_rocm.serve(ctx, jobs, n_slots=4, hbm_budget_bytes=budget)
+    assert all(r["status"] == 0 for r in res)
+    assert 0 < st["peak_device_bytes"] <= budget, st
+    assert st["peak_admitted_bytes"] <= budget
+    ref, st1 = stwo_rocm.serve(ctx, jobs, n_slots=1)
+    assert [r["digest"] for r in res] == [r["digest"] for r in ref]
+    # a budget below what one slot must hold beside the job is refused, not overrun
+    with pytest.raises(stwo_rocm.RocmError):
+        stwo_rocm.serve(ctx, [(0, 16, 64, 1, 0.0)], n_slots=1, hbm_budget_bytes=64 << 20)

@@ -284,6 +284,10 @@ __global__ __launch_bounds__(TL_THREADS) void tree_layer_kernel(FriLayerArgs a) 
         }
         __syncthreads();
         if (!last) return;
+        // Acquire: the loads of the other workgroups' roots below are ordered after the counter
+        // update that observed their arrival (each publisher's roots were acknowledged, s_waitcnt,
+        // before its count).  One fence per launch, in the last workgroup only.
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         if (t <= TREE_CNT_GROUPS) atomicExch(&a.cnt[TREE_CNT_STRIDE * t], 0u);
         if (a.ch && t < 4) {
 #pragma unroll

@@ -48,6 +48,12 @@ void prepare_and_commit(Ctx& c, uint32_t* const* cols, const uint32_t* log_sizes
             tables.push_back(upload(c, src.data(), src.size()));
             tables.push_back(upload(c, dst.data(), dst.size()));
             const uint32_t* const* ds = tables[tables.size() - 2].as<const uint32_t* const>();
+            if (interpolate && log_blowup == 1) {
+                // interpolation in place and the extension, fused where the pass plans allow
+                launch_circle_ifft_lde(c, ds, tables[tables.size() - 2].as<uint32_t* const>(),
+                                       tables.back().as<uint32_t* const>(), (uint32_t)src.size(), lg);
+                continue;
+            }
             if (interpolate)
                 launch_circle_fft(c, true, ds, tables[tables.size() - 2].as<uint32_t* const>(), (uint32_t)src.size(),
                                   lg, lg);

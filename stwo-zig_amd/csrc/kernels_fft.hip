@@ -253,7 +253,9 @@ struct RxGroupShape {
 
 // w holds this group's twiddles (loaded before the previous exchange); the next group's
 // twiddles are loaded before this group's exchange so their latency overlaps it.
-template <bool INV, int R, int LOGC, int GI>
+// FROM_REGS: the first group takes its 16 values from v instead of loading them (the fused
+// interpolate + extend kernel hands the coefficients over in registers).
+template <bool INV, int R, int LOGC, int GI, bool FROM_REGS = false>
 __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4][8], uint32_t* lds,
                                        gcu32* __restrict__ src, gu32* __restrict__ dst,
                                        const uint32_t* __restrict__ tower, const RxPass& ps, uint32_t hi, uint32_t r,
@@ -263,7 +265,9 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4]
         using S = RxGroupShape<INV, R, GI>;
         constexpr int B = S::B;
         const uint32_t row0 = (r & ((1u << B) - 1)) | ((r >> B) << (B + RX_LOG));
-        if (GI == 0) {
+        if (GI == 0 && FROM_REGS) {
+            // values already in v (row bits [B, B+4) of this thread)
+        } else if (GI == 0) {
             const uint32_t g0 = base | (row0 << ps.s_lo) | lane;
             const uint32_t src_n = 1u << ps.src_log;
             if (B == 0 && LOGC == 0 && g0 + RX <= src_n) {
@@ -297,7 +301,7 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4]
             if (GI > 0) __syncthreads();
 #pragma unroll
             for (int k = 0; k < RX; k++) lds[rx_lds<LOGC>(row0 + ((uint32_t)k << B), lane)] = v[k];
-            rx_run<INV, R, LOGC, GI + 1>(v, wn, lds, src, dst, tower, ps, hi, r, lane, base);
+            rx_run<INV, R, LOGC, GI + 1, FROM_REGS>(v, wn, lds, src, dst, tower, ps, hi, r, lane, base);
         } else {
             const uint32_t g0 = base | (row0 << ps.s_lo) | lane;
             if (ps.scale != 1) {
@@ -337,6 +341,57 @@ __global__ __launch_bounds__(512) void fft_rx_kernel(const uint32_t* const* __re
     using S0 = RxGroupShape<INV, R, 0>;
     rx_group_tw<INV, R, LOGC, S0::B, S0::LO, S0::HI>(w, tower, ps, hi, r, base);
     rx_run<INV, R, LOGC, 0>(v, w, lds, (gcu32*)src_cols[col], (gu32*)dst_cols[col], tower, ps, hi, r, lane, base);
+}
+
+// Interpolation of log n fused with the extension to log n + 1 (blowup 2), over the top
+// strided pass of both transforms (transforms.zig:249-360 then 149-246).  The inverse pass
+// (stages [s_lo, n) of the evaluation domain, scaled by 2^-n) leaves each thread's 16
+// coefficients in registers; they are stored (the retained coefficients) and extended: the
+// top forward stage n pairs every coefficient with a zero, so both halves of the extended
+// column start from the same coefficients, and each half runs the forward stages
+// [s_lo, n) of CanonicCoset(n + 1) with its own twiddles (hi = the half).  The inverse's last
+// row group and the forward's first own the same row bits, so no exchange separates them.
+// Saves one HBM read of the coefficients and one launch per column set.
+template <int R, int LOGC>
+__global__ __launch_bounds__(512) void fft_ifft_lde_kernel(const uint32_t* const* __restrict__ src_cols,
+                                                           uint32_t* const* __restrict__ coef_cols,
+                                                           uint32_t* const* __restrict__ lde_cols,
+                                                           const uint32_t* __restrict__ itower,
+                                                           const uint32_t* __restrict__ tower, RxPass pi, RxPass pf) {
+    extern __shared__ uint32_t lds[];
+    constexpr int TILE_LOG = R + LOGC;
+    static_assert(RxGroupShape<true, R, (R + RX_LOG - 1) / RX_LOG - 1>::B == RxGroupShape<false, R, 0>::B,
+                  "the inverse's last group and the forward's first must own the same row bits");
+    const uint32_t per_col_log = pi.m - TILE_LOG;  // pi.s_lo + R == pi.m: every block bit is a lane block
+    const uint32_t col = blockIdx.x >> per_col_log;
+    const uint32_t lo_blk = blockIdx.x & ((1u << per_col_log) - 1);
+    const uint32_t base = lo_blk << LOGC;
+    const uint32_t t = threadIdx.x;
+    const uint32_t lane = t & ((1u << LOGC) - 1);
+    const uint32_t r = t >> LOGC;
+    uint32_t v[RX];
+    uint32_t w[4][8];
+    {
+        using S0 = RxGroupShape<true, R, 0>;
+        rx_group_tw<true, R, LOGC, S0::B, S0::LO, S0::HI>(w, itower, pi, 0, r, base);
+        rx_run<true, R, LOGC, 0>(v, w, lds, (gcu32*)src_cols[col], (gu32*)coef_cols[col], itower, pi, 0, r, lane,
+                                 base);
+    }
+    uint32_t cf[RX];
+#pragma unroll
+    for (int k = 0; k < RX; k++) cf[k] = v[k];
+    gu32* lde = (gu32*)lde_cols[col];
+    using F0 = RxGroupShape<false, R, 0>;
+#pragma unroll
+    for (uint32_t h = 0; h < 2; h++) {
+        if (h) {
+#pragma unroll
+            for (int k = 0; k < RX; k++) v[k] = cf[k];
+        }
+        rx_group_tw<false, R, LOGC, F0::B, F0::LO, F0::HI>(w, tower, pf, h, r, base | (h << (pf.s_lo + R)));
+        __syncthreads();  // the previous transform's last exchange has been read
+        rx_run<false, R, LOGC, 0, true>(v, w, lds, nullptr, lde, tower, pf, h, r, lane, base | (h << (pf.s_lo + R)));
+    }
 }
 
 // m <= 2: closed forms (transforms.zig:155-179,256-292), one thread per column.
@@ -440,6 +495,129 @@ static void run_rx_pass(Ctx& c, bool inverse, const uint32_t* const* src, uint32
     SR_HIP(hipGetLastError());
 }
 
+// Pass plan of a transform of log m >= 6: a contiguous pass of R0 <= TILE_LOG stages, then
+// strided passes of 4..8 stages over 2^TILE_LOG-element tiles (>= 8 consecutive words per row).
+static std::vector<std::pair<uint32_t, uint32_t>> rx_plan(uint32_t m, uint32_t TILE_LOG) {
+    std::vector<std::pair<uint32_t, uint32_t>> passes;  // (s_lo, R)
+    if (m <= TILE_LOG) {
+        passes.push_back({0, m});
+    } else {
+        uint32_t n_high = (m - TILE_LOG + 7) / 8;
+        uint32_t r0 = std::min<uint32_t>(TILE_LOG, m - 4 * n_high);
+        passes.push_back({0, r0});
+        uint32_t rem = m - r0, s = r0;
+        for (uint32_t i = 0; i < n_high; i++) {
+            uint32_t left = n_high - i;
+            uint32_t rr = (rem + left - 1) / left;
+            passes.push_back({s, rr});
+            s += rr;
+            rem -= rr;
+        }
+    }
+    return passes;
+}
+static uint32_t rx_logc(std::pair<uint32_t, uint32_t> p, uint32_t TILE_LOG) {
+    return p.first == 0 ? 0u : std::min<uint32_t>(TILE_LOG - p.second, p.first);
+}
+
+static bool fused_dispatch(uint32_t R, uint32_t logc, dim3 grid, dim3 block, size_t lds, hipStream_t st,
+                           const uint32_t* const* src, uint32_t* const* coef, uint32_t* const* lde,
+                           const uint32_t* itower, const uint32_t* tower, const RxPass& pi, const RxPass& pf) {
+#define SR_FX(RR, CC)                                                                                          \
+    if (R == RR && logc == CC) {                                                                               \
+        hipLaunchKernelGGL((fft_ifft_lde_kernel<RR, CC>), grid, block, lds, st, src, coef, lde, itower, tower, \
+                           pi, pf);                                                                            \
+        return true;                                                                                           \
+    }
+    SR_FX(4, 9) SR_FX(5, 8) SR_FX(6, 7) SR_FX(7, 6) SR_FX(8, 5)  // 8192-word tiles
+    SR_FX(4, 8) SR_FX(5, 7) SR_FX(6, 6) SR_FX(7, 5) SR_FX(8, 4)  // 4096-word tiles
+    SR_FX(4, 7) SR_FX(5, 6) SR_FX(6, 5) SR_FX(7, 4) SR_FX(8, 3)  // 2048-word tiles
+#undef SR_FX
+    return false;
+}
+
+void launch_circle_ifft_lde(Ctx& c, const uint32_t* const* d_eval, uint32_t* const* d_coef, uint32_t* const* d_lde,
+                            uint32_t n_cols, uint32_t n) {
+    if (n_cols == 0) return;
+    const uint32_t TILE_LOG = c.fft_tile_log;
+    const uint32_t m = n + 1;
+    bool fuse = n > TILE_LOG && n >= 6;
+    std::vector<std::pair<uint32_t, uint32_t>> pi_plan, pf_plan;
+    if (fuse) {
+        pi_plan = rx_plan(n, TILE_LOG);
+        pf_plan = rx_plan(m, TILE_LOG);
+        // the inverse's top pass and the extension's top pass cover the same stages [S, n) (the
+        // extension adds stage n, its zero-padded duplication) on the same tile shape
+        const auto a = pi_plan.back(), b = pf_plan.back();
+        fuse = a.first == b.first && a.second + 1 == b.second && pi_plan.size() == pf_plan.size() &&
+               a.first + a.second == n;
+        for (size_t i = 0; fuse && i + 1 < pi_plan.size(); i++) fuse = pi_plan[i] == pf_plan[i];
+    }
+    if (!fuse) {
+        launch_circle_fft(c, true, d_eval, d_coef, n_cols, n, n);
+        launch_circle_fft(c, false, (const uint32_t* const*)d_coef, d_lde, n_cols, m, n);
+        return;
+    }
+    c.ensure_tower(m);
+    const uint32_t n_inv = minv((uint32_t)((1ull << n) % P));
+    const size_t np = pi_plan.size();
+    // inverse passes below the top one (evaluations -> d_coef, in place afterwards)
+    for (size_t i = 0; i + 1 < np; i++) {
+        RxPass ps{};
+        ps.m = n;
+        ps.src_log = n;
+        ps.s_lo = pi_plan[i].first;
+        ps.tower_log = c.tower_log;
+        ps.scale = 1u;
+        ps.dup_top = 0;
+        run_rx_pass(c, true, i == 0 ? d_eval : (const uint32_t* const*)d_coef, d_coef, n_cols, pi_plan[i].second,
+                    rx_logc(pi_plan[i], TILE_LOG), ps);
+    }
+    {
+        const auto top = pi_plan.back();
+        const uint32_t R = top.second, logc = rx_logc(top, TILE_LOG);
+        RxPass pi{}, pf{};
+        pi.m = n;
+        pi.src_log = n;
+        pi.s_lo = top.first;
+        pi.tower_log = c.tower_log;
+        pi.scale = n_inv;
+        pf.m = m;
+        pf.src_log = m;
+        pf.s_lo = top.first;
+        pf.tower_log = c.tower_log;
+        pf.scale = 1u;
+        const uint32_t tile_log = R + logc;
+        SR_CHECK(R >= (uint32_t)RX_LOG && tile_log <= n && logc <= top.first, "fused fft pass shape");
+        const uint32_t nt = 1u << (tile_log - RX_LOG);
+        const size_t lds = (size_t)4 << tile_log;
+        const uint64_t blocks = (uint64_t)n_cols << (n - tile_log);
+        SR_CHECK(blocks < (1ull << 31), "fft grid too large");
+        // in the middle of a plan with more than one inverse pass, the top pass reads d_coef
+        const uint32_t* const* src = np > 1 ? (const uint32_t* const*)d_coef : d_eval;
+        KTimer kt(c,
+                  "fft_ilde_pass m=" + std::to_string(n) + " R=" + std::to_string(R) + " C=" +
+                      std::to_string(1u << logc) + " cols=" + std::to_string(n_cols),
+                  4.0 * n_cols * (2.0 * (double)(1ull << n) + (double)(1ull << m)));
+        SR_CHECK(fused_dispatch(R, logc, dim3((uint32_t)blocks), dim3(nt), lds, c.stream, src, d_coef, d_lde,
+                                c.d_itw2, c.d_tw2, pi, pf),
+                 "fft: no fused kernel for the pass shape");
+        SR_HIP(hipGetLastError());
+    }
+    // the extension's passes below the top one, in place on d_lde
+    for (size_t i = 1; i < np; i++) {
+        const auto pp = pf_plan[np - 1 - i];
+        RxPass ps{};
+        ps.m = m;
+        ps.src_log = m;
+        ps.s_lo = pp.first;
+        ps.tower_log = c.tower_log;
+        ps.scale = 1u;
+        ps.dup_top = 0;
+        run_rx_pass(c, false, (const uint32_t* const*)d_lde, d_lde, n_cols, pp.second, rx_logc(pp, TILE_LOG), ps);
+    }
+}
+
 void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint32_t* const* d_dst, uint32_t n_cols,
                        uint32_t m, uint32_t src_log) {
     if (n_cols == 0) return;
@@ -471,28 +649,9 @@ void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint3
         run_pass(c, inverse, d_src, d_dst, n_cols, ps);
         return;
     }
-    // Pass plan: a contiguous pass of R0 <= 13 stages, then strided passes of 4..8
-    // stages over 8192-element tiles (>= 32 consecutive words per row).
     const uint32_t TILE_LOG = c.fft_tile_log;  // 11..13 (tuning parameter, results independent of it)
-    std::vector<std::pair<uint32_t, uint32_t>> passes;  // (s_lo, R)
-    if (m <= TILE_LOG) {
-        passes.push_back({0, m});
-    } else {
-        uint32_t n_high = (m - TILE_LOG + 7) / 8;
-        uint32_t r0 = std::min<uint32_t>(TILE_LOG, m - 4 * n_high);
-        passes.push_back({0, r0});
-        uint32_t rem = m - r0, s = r0;
-        for (uint32_t i = 0; i < n_high; i++) {
-            uint32_t left = n_high - i;
-            uint32_t rr = (rem + left - 1) / left;
-            passes.push_back({s, rr});
-            s += rr;
-            rem -= rr;
-        }
-    }
-    auto logc_of = [&](std::pair<uint32_t, uint32_t> p) {
-        return p.first == 0 ? 0u : std::min<uint32_t>(TILE_LOG - p.second, p.first);
-    };
+    const std::vector<std::pair<uint32_t, uint32_t>> passes = rx_plan(m, TILE_LOG);  // (s_lo, R)
+    auto logc_of = [&](std::pair<uint32_t, uint32_t> p) { return rx_logc(p, TILE_LOG); };
     auto mk = [&](std::pair<uint32_t, uint32_t> p, bool last) {
         RxPass ps;
         ps.m = m;

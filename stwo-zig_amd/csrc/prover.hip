@@ -259,6 +259,21 @@ void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack
     launch_merkle_parents(c, *out.tree, nullptr, d_root_out);
 }
 
+void launch_commit_from_evals(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack, size_t at_evals,
+                              uint8_t* d_root_out) {
+    const uint32_t* const* ev = (const uint32_t* const*)(d_pack + at_evals);
+    uint32_t* const* cf = (uint32_t* const*)(d_pack + cp.at_src);
+    uint32_t* const* ld = (uint32_t* const*)(d_pack + cp.at_dst);
+    if (cp.m == cp.coeff_log + 1) {
+        launch_circle_ifft_lde(c, ev, cf, ld, cp.n, cp.coeff_log);
+    } else {
+        launch_circle_fft(c, true, ev, cf, cp.n, cp.coeff_log, cp.coeff_log);
+        launch_circle_fft(c, false, (const uint32_t* const*)cf, ld, cp.n, cp.m, cp.coeff_log);
+    }
+    launch_merkle_leaves_dev(c, (const LeafColDev*)(d_pack + cp.at_tab), cp.segs, cp.n, *out.tree);
+    launch_merkle_parents(c, *out.tree, nullptr, d_root_out);
+}
+
 void fri_fold_commit_down(Ctx& c, TreeOwner& owner, const uint32_t* d_top, uint32_t top_log, uint32_t last_log,
                           DevChannel* d_ch, uint32_t* const* dst_lines, uint32_t* d_last, uint8_t* d_roots,
                           std::vector<DeviceTree*>& trees) {
@@ -379,7 +394,7 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     if (!d_trace) d_trace = trace_own.as<uint32_t>();
     DBuf coeff1(&c, (size_t)W * N * 4);
     std::vector<const uint32_t*> cf(W);
-    size_t at_ifft_src, at_ifft_dst;
+    size_t at_ifft_src;
     {
         std::vector<const uint32_t*> src(W);
         for (uint32_t i = 0; i < W; i++) {
@@ -387,7 +402,6 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
             cf[i] = coeff1.as<uint32_t>() + i * N;
         }
         at_ifft_src = pack.add(src.data(), W);
-        at_ifft_dst = pack.add(cf.data(), W);
     }
     Committed t1;
     CommitPlan cp1;
@@ -502,9 +516,7 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         if (poseidon) launch_poseidon_trace(c, n, trace_own.as<uint32_t>(), N);
         else if (!xr) launch_widefib_trace(c, n, W, prm.trace_mode, prm.trace_seed, trace_own.as<uint32_t>(), N);
     }
-    launch_circle_fft(c, true, (const uint32_t* const*)(d_pack + at_ifft_src), (uint32_t* const*)(d_pack + at_ifft_dst),
-                      W, n, n);
-    launch_commit(c, t1, cp1, d_pack, d_troots);
+    launch_commit_from_evals(c, t1, cp1, d_pack, at_ifft_src, d_troots);
     trace_own.reset();
     mark();  // main commit
 

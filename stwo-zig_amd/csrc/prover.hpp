@@ -84,6 +84,10 @@ struct CommitPlan {
 void plan_commit(Ctx& c, TreeOwner& owner, Committed& out, const std::vector<const uint32_t*>& coeffs,
                  uint32_t coeff_log, uint32_t blowup, HostPack& pack, CommitPlan& cp);
 void launch_commit(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack, uint8_t* d_root_out);
+// The same from evaluations (the pack holds their pointer list at at_evals): interpolated into
+// the plan's coefficient columns and extended, fused into one launch sequence at blowup 2.
+void launch_commit_from_evals(Ctx& c, Committed& out, const CommitPlan& cp, uint8_t* d_pack, size_t at_evals,
+                              uint8_t* d_root_out);
 
 // LDE of coefficient columns (pointer list) by `blowup` and Merkle commit
 // (commit_polys.zig:10-67; preparation.zig:113-210).

@@ -205,6 +205,12 @@ struct DBuf {
 // interpolates evaluations of CanonicCoset(m) (src_log == m) and scales by 2^-m.
 void launch_circle_fft(Ctx& c, bool inverse, const uint32_t* const* d_src, uint32_t* const* d_dst, uint32_t n_cols,
                        uint32_t m, uint32_t src_log);
+// Interpolation of evaluations on CanonicCoset(n) into d_coef (the retained coefficients) and
+// their extension to CanonicCoset(n + 1) into d_lde (device pointer arrays, one entry per
+// column): the top strided passes of the two transforms run as one launch when the pass plans
+// allow it, otherwise the two transforms run one after the other.  Same results either way.
+void launch_circle_ifft_lde(Ctx& c, const uint32_t* const* d_eval, uint32_t* const* d_coef, uint32_t* const* d_lde,
+                            uint32_t n_cols, uint32_t n);
 // Lifted Merkle: columns sorted by log size ascending (d_cols/d_logs device arrays).
 // Column descriptor of the leaf kernel; tables are padded to a multiple of 16 entries with
 // mask 0 (padding reads column 0 and contributes a zero word).

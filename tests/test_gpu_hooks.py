@@ -237,3 +237,30 @@ def test_hook_assembled_proof_equals_monolithic(ctx, kw):
                log_last_layer_degree_bound=kw.get("log_last_layer_degree_bound", 0),
                n_queries=kw.get("n_queries", 3))
     sv.verify_proof(got, "wide_fibonacci", dict(log_n_rows=kw["log_n_rows"], sequence_len=kw["sequence_len"]), cfg)
+
+
+@pytest.mark.parametrize("log", [8, 13, 14, 16, 17, 18, 19, 20, 21, 22])
+def test_interpolate_and_extend_fused_and_plain(ctx, log):
+    """prepareAndCommitOwned at blowup 2: the interpolation's top pass and the extension's top
+    pass run as one launch where the pass plans line up (logs 17..20 and 22 with 2^13 tiles) and
+    as two transforms elsewhere; coefficients, extension and root equal the oracle's for both."""
+    rng = np.random.default_rng(900 + log)
+    vals = [_m31(rng, 1 << log) for _ in range(3)]
+    bufs = [ctx.to_device(v) for v in vals]
+    ext = [ctx.alloc(4 << (log + 1)) for _ in vals]
+    ctx.profile_enable(True)
+    tree = ctx.prepare_and_commit_owned([int(b) for b in bufs], [log] * 3, 1, [int(e) for e in ext])
+    classes = ctx.profile_read()
+    ctx.profile_enable(False)
+    _say("log", log, "fused" if any(k.startswith("fft_ilde_pass") for k in classes) else "plain")
+    coeffs = [oracle.circle_interpolate(v, log) for v in vals]
+    ldes = [oracle.circle_lde(c, log, 1) for c in coeffs]
+    for b, c in zip(bufs, coeffs):
+        assert np.array_equal(b.download(), c)
+    for e, l_ in zip(ext, ldes):
+        assert np.array_equal(e.download(), l_)
+    assert tree.root == oracle.merkle_commit(ldes)[0]
+    if log in (17, 18, 19, 20, 22):
+        assert any(k.startswith("fft_ilde_pass m=%d " % log) for k in classes), sorted(classes)
+    for x in [tree] + bufs + ext:
+        x.free()

@@ -1044,6 +1044,399 @@ enum { AIR_WIDEFIB = 0, AIR_POSEIDON = 1, AIR_XOR = 2 };
 /* air AIR_XOR: the periodic XOR example (src/examples/xor.zig:419-605): two preprocessed
  * columns and one main column, statement (log_size, log_step, offset), constant
  * composition (log_size, log_step, offset, 1); xs holds log_step and offset. */
+/* randomSecureFieldPoint (circle.zig:159-167). */
+static qpt draw_oods_point(channel *ch) {
+    qm31 tt = ch_draw_secure_felt(ch);
+    qm31 t2_ = q_square(tt);
+    qm31 inv1 = q_inv(q_add(t2_, Q_ONE));
+    return (qpt){q_mul(q_sub(Q_ONE, t2_), inv1), q_mul(q_add(tt, tt), inv1)};
+}
+static inline int qp_eq(qpt a, qpt b) { return q_eq(a.x, b.x) && q_eq(a.y, b.y); }
+/* QM31 circle point plus an M31 circle point. */
+static inline qpt qp_add_m(qpt a, pt b) {
+    qm31 bx = q_from_m(b.x), by = q_from_m(b.y);
+    return (qpt){q_sub(q_mul(a.x, bx), q_mul(a.y, by)), q_add(q_mul(a.x, by), q_mul(a.y, bx))};
+}
+
+/* The rest of the proof from the committed trees (the composition tree last): sampled values
+ * at the mask points (sampled_values.zig), their transcript, the quotient column with one batch
+ * per distinct point (core/pcs/quotients/samples.zig:245-312 with the periodicity sample of
+ * two-point columns, row_evaluation.zig:402-521), FRI (prover/fri.zig:163-621), PoW
+ * (scheme.zig:733-735), queries (core/queries.zig:8-60), decommitments and the proof wire
+ * (interop/proof_wire.zig:19-58).  ns[col] (1 or 2) and pts[2*col + s]: the mask points of every
+ * column in tree/column order. */
+static int prove_tail(channel *chp, ctree *trees, int ntr, const uint32_t *ns, const qpt *pts,
+                      const ora_widefib_params *prm, const double tm[3], double t3, uint8_t *out, size_t cap,
+                      size_t *out_len) {
+    channel ch = *chp;
+    const uint32_t B = prm->log_blowup_factor;
+    uint32_t L = 0, total_cols = 0;
+    for (int tr = 0; tr < ntr; tr++) {
+        total_cols += trees[tr].n_cols;
+        for (uint32_t c = 0; c < trees[tr].n_cols; c++)
+            if (trees[tr].log_sizes[c] > L) L = trees[tr].log_sizes[c];
+    }
+    /* flat column tables */
+    const uint32_t **fl_lde = malloc(sizeof(uint32_t *) * total_cols);
+    const uint32_t **fl_coef = malloc(sizeof(uint32_t *) * total_cols);
+    uint32_t *fl_log = malloc(4 * total_cols), *fl_clog = malloc(4 * total_cols), *first_s = malloc(4 * (total_cols + 1));
+    uint32_t n_samples = 0;
+    {
+        uint32_t at = 0;
+        for (int tr = 0; tr < ntr; tr++)
+            for (uint32_t c = 0; c < trees[tr].n_cols; c++, at++) {
+                fl_lde[at] = trees[tr].lde[c];
+                fl_coef[at] = trees[tr].coeffs[c];
+                fl_log[at] = trees[tr].log_sizes[c];
+                fl_clog[at] = trees[tr].coeff_logs[c];
+                first_s[at] = n_samples;
+                n_samples += ns[at];
+            }
+        first_s[total_cols] = n_samples;
+    }
+    /* Sampled values: each column at its mask points folded (L - log) times (the lifted PCS). */
+    qm31 *sv = malloc(sizeof(qm31) * (n_samples ? n_samples : 1));
+    qpt *spt = malloc(sizeof(qpt) * (n_samples ? n_samples : 1));
+    for (uint32_t i = 0; i < total_cols; i++)
+        for (uint32_t k = 0; k < ns[i]; k++) {
+            qpt p = pts[2 * i + k];
+            for (uint32_t f = fl_log[i]; f < L; f++) { /* repeatedDouble */
+                qm31 xx = q_square(p.x), xy = q_mul(p.x, p.y);
+                p.x = q_sub(q_add(xx, xx), Q_ONE);
+                p.y = q_add(xy, xy);
+            }
+            spt[first_s[i] + k] = p;
+        }
+    {
+        /* one weight vector per distinct (point, coefficient log) */
+        enum { MAXW = 16 };
+        qpt wpt[MAXW];
+        uint32_t wlog[MAXW];
+        qm31 *wv[MAXW];
+        int nw = 0;
+        for (uint32_t i = 0; i < total_cols; i++)
+            for (uint32_t k = 0; k < ns[i]; k++) {
+                qpt p = spt[first_s[i] + k];
+                int w = 0;
+                for (; w < nw; w++)
+                    if (wlog[w] == fl_clog[i] && qp_eq(wpt[w], p)) break;
+                if (w == nw) {
+                    if (nw == MAXW) return 4;
+                    wpt[nw] = p;
+                    wlog[nw] = fl_clog[i];
+                    wv[nw] = eval_weights(p, fl_clog[i]);
+                    nw++;
+                }
+            }
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t i = 0; i < (int64_t)total_cols; i++)
+            for (uint32_t k = 0; k < ns[i]; k++) {
+                qpt p = spt[first_s[i] + k];
+                int w = 0;
+                while (!(wlog[w] == fl_clog[i] && qp_eq(wpt[w], p))) w++;
+                sv[first_s[i] + k] = fl_clog[i] == 0 ? q_from_m(fl_coef[i][0])
+                                                      : dot_weights(fl_coef[i], wv[w], 1ull << fl_clog[i]);
+            }
+        for (int w = 0; w < nw; w++) free(wv[w]);
+    }
+    /* sampled_value_transcript.mixIntoChannel: tree, column, sample order. */
+    ch_mix_felts(&ch, sv, n_samples);
+    double t4 = now_ms();
+
+    /* Quotients (core/pcs/quotients/{samples,row_evaluation}.zig; constraints.zig:87-97): one batch
+     * per distinct point in first-seen order; a two-point column first adds its second sample at
+     * that point plus the lifting domain's step doubled log times (samples.zig:270-285); random
+     * coefficients 1, q, q^2, ... in that order. */
+    qm31 qalpha = ch_draw_secure_felt(&ch);
+    uint64_t LN = 1ull << L;
+    uint32_t max_e = n_samples + total_cols;
+    qpt *bpt = malloc(sizeof(qpt) * max_e);
+    qm31 *bsa = malloc(sizeof(qm31) * max_e), *bsb = malloc(sizeof(qm31) * max_e);
+    uint32_t *ecol = malloc(4 * max_e), *ebat = malloc(4 * max_e);
+    qm31 *ecoef = malloc(sizeof(qm31) * max_e);
+    uint32_t nb = 0, ne = 0;
+    {
+        qm31 rp = Q_ONE;
+        const uint32_t lift_gen = subgroup_gen(L);
+        for (uint32_t i = 0; i < total_cols; i++) {
+            for (int32_t k = ns[i] == 2 ? -1 : 0; k < (int32_t)ns[i]; k++) {
+                qpt p;
+                qm31 v;
+                if (k < 0) {
+                    p = qp_add_m(spt[first_s[i] + 1],
+                                 idx_to_point((uint32_t)(((uint64_t)lift_gen << fl_log[i]) & ORDER_MASK)));
+                    v = sv[first_s[i] + 1];
+                } else {
+                    p = spt[first_s[i] + k];
+                    v = sv[first_s[i] + k];
+                }
+                uint32_t b = 0;
+                for (; b < nb; b++)
+                    if (qp_eq(bpt[b], p)) break;
+                if (b == nb) {
+                    bpt[nb] = p;
+                    bsa[nb] = Q_ZERO;
+                    bsb[nb] = Q_ZERO;
+                    nb++;
+                }
+                qm31 a = q_sub(q_conj(v), v);
+                qm31 c = q_sub(q_conj(p.y), p.y);
+                qm31 bb = q_sub(q_mul(v, c), q_mul(a, p.y));
+                ecol[ne] = i;
+                ebat[ne] = b;
+                ecoef[ne] = q_mul(rp, c);
+                ne++;
+                bsa[b] = q_add(bsa[b], q_mul(rp, a));
+                bsb[b] = q_add(bsb[b], q_mul(rp, bb));
+                rp = q_mul(rp, qalpha);
+            }
+        }
+    }
+    qm31 *quot = malloc(sizeof(qm31) * LN);
+#pragma omp parallel for schedule(static)
+    for (int64_t row = 0; row < (int64_t)LN; row++) {
+        pt dp = domain_at(L, bitrev((uint64_t)row, L));
+        qm31 acc = Q_ZERO;
+        for (uint32_t b = 0; b < nb; b++) {
+            qm31 num = Q_ZERO;
+            for (uint32_t e = 0; e < ne; e++) {
+                if (ebat[e] != b) continue;
+                uint32_t i = ecol[e];
+                uint32_t v = fl_lde[i][lift_index((uint64_t)row, L, fl_log[i])];
+                num = q_add(num, q_mul_m(ecoef[e], v));
+            }
+            num = q_sub(num, q_add(q_mul_m(bsa[b], dp.y), bsb[b]));
+            cm31 prx = bpt[b].x.c0, pry = bpt[b].y.c0, pix = bpt[b].x.c1, piy = bpt[b].y.c1;
+            cm31 det = c_sub(c_mul(prx, piy), c_mul(pry, pix));
+            cm31 den = c_add(c_sub(det, c_mul_m(piy, dp.x)), c_mul_m(pix, dp.y));
+            acc = q_add(acc, q_mul_c(num, c_inv(den)));
+        }
+        quot[row] = acc;
+    }
+    free(bpt);
+    free(bsa);
+    free(bsb);
+    free(ecol);
+    free(ebat);
+    free(ecoef);
+    double t5 = now_ms();
+
+    /* FRI commit (prover/fri.zig:163-621). */
+    uint32_t nlayers_max = L + 1;
+    ora_tree **ftrees = calloc(nlayers_max, sizeof(ora_tree *));
+    qm31 **fcols = calloc(nlayers_max, sizeof(qm31 *));
+    uint32_t *flogs = calloc(nlayers_max, 4);
+    uint32_t *first_coords[4];
+    ftrees[0] = commit_qm31_column(quot, L, first_coords);
+    fcols[0] = quot;
+    flogs[0] = L;
+    {
+        uint8_t r[32];
+        ora_tree_root(ftrees[0], r);
+        ch_mix_root(&ch, r);
+    }
+    qm31 fa = ch_draw_secure_felt(&ch);
+    uint64_t ln = LN / 2;
+    qm31 *line = calloc(ln, sizeof(qm31));
+    ora_fold_circle_into_line((const uint32_t *const *)first_coords, L, (const uint32_t *)&fa, (uint32_t *)line);
+    for (int k = 0; k < 4; k++) free(first_coords[k]);
+    coset ldom = coset_half_odds(L - 1);
+    uint32_t llog = L - 1;
+    const uint32_t last_log = prm->log_last_layer_degree_bound + B;
+    uint32_t ninner = 0;
+    while (llog > last_log) {
+        ftrees[1 + ninner] = commit_qm31_column(line, llog, NULL);
+        fcols[1 + ninner] = line;
+        flogs[1 + ninner] = llog;
+        uint8_t r[32];
+        ora_tree_root(ftrees[1 + ninner], r);
+        ch_mix_root(&ch, r);
+        qm31 la = ch_draw_secure_felt(&ch);
+        qm31 *next = malloc(sizeof(qm31) * (1ull << (llog - 1)));
+        ora_fold_line((const uint32_t *)line, llog, ldom.initial_index, (const uint32_t *)&la, (uint32_t *)next);
+        ldom = coset_double(ldom);
+        llog--;
+        line = next;
+        ninner++;
+    }
+    /* Last layer (fri.zig:594-621; line_evaluation.zig:97-151; line.zig:134-142). */
+    uint64_t lsz = 1ull << llog;
+    bitrev_q(line, lsz);
+    {
+        coset cd = ldom;
+        while ((1ull << cd.log_size) > 1) {
+            uint64_t csz = 1ull << cd.log_size, half = csz / 2;
+            for (uint64_t st0 = 0; st0 < lsz; st0 += csz) {
+                pt cur = idx_to_point(cd.initial_index), step = idx_to_point(cd.step_size);
+                for (uint64_t i = 0; i < half; i++) {
+                    uint32_t ix = m_inv(cur.x);
+                    qm31 v0 = line[st0 + i], v1 = line[st0 + half + i];
+                    line[st0 + i] = q_add(v0, v1);
+                    line[st0 + half + i] = q_mul_m(q_sub(v0, v1), ix);
+                    cur = p_add(cur, step);
+                }
+            }
+            cd = coset_double(cd);
+        }
+        uint32_t li = m_inv((uint32_t)(lsz % P));
+        for (uint64_t i = 0; i < lsz; i++) line[i] = q_mul_m(line[i], li);
+    }
+    bitrev_q(line, lsz); /* ordered coefficients */
+    uint64_t bound = 1ull << prm->log_last_layer_degree_bound;
+    for (uint64_t i = bound; i < lsz; i++)
+        if (!q_is_zero(line[i])) return 3; /* InvalidLastLayerDegree */
+    qm31 *lastp = malloc(sizeof(qm31) * bound);
+    memcpy(lastp, line, sizeof(qm31) * bound);
+    bitrev_q(lastp, bound); /* fromOrderedCoefficients */
+    free(line);
+    ch_mix_felts(&ch, lastp, bound);
+    double t6 = now_ms();
+
+    /* PoW (scheme.zig:733-735). */
+    uint64_t nonce = ora_pow_grind(ch.digest, prm->pow_bits);
+    ch_mix_u64(&ch, nonce);
+    double t7 = now_ms();
+
+    /* Queries (core/queries.zig:8-60). */
+    uint32_t nq = prm->n_queries;
+    uint64_t *qs = malloc(sizeof(uint64_t) * (nq + 8));
+    {
+        uint32_t got = 0;
+        while (got < nq) {
+            uint32_t w[8];
+            ch_draw_u32s(&ch, w);
+            for (int i = 0; i < 8 && got < nq; i++) qs[got++] = w[i] & ((1u << L) - 1);
+        }
+    }
+    qsort(qs, nq, sizeof(uint64_t), cmp_u64);
+    size_t nuq = 0;
+    for (uint32_t i = 0; i < nq; i++)
+        if (nuq == 0 || qs[nuq - 1] != qs[i]) qs[nuq++] = qs[i];
+
+    /* Assemble the proof wire (interop/proof_wire.zig:19-58). */
+    bytes j = {0};
+    by_str(&j, "{\"config\":{\"pow_bits\":");
+    by_u64(&j, prm->pow_bits);
+    by_str(&j, ",\"fri_config\":{\"log_blowup_factor\":");
+    by_u64(&j, B);
+    by_str(&j, ",\"log_last_layer_degree_bound\":");
+    by_u64(&j, prm->log_last_layer_degree_bound);
+    by_str(&j, ",\"n_queries\":");
+    by_u64(&j, nq);
+    by_str(&j, ",\"fold_step\":1},\"lifting_log_size\":null},\"commitments\":[");
+    for (int tr = 0; tr < ntr; tr++) {
+        uint8_t r[32];
+        ora_tree_root(trees[tr].tree, r);
+        if (tr) by_str(&j, ",");
+        json_hash(&j, r);
+    }
+    by_str(&j, "],\"sampled_values\":[");
+    {
+        size_t at = 0;
+        for (int tr = 0; tr < ntr; tr++) {
+            by_str(&j, tr ? ",[" : "[");
+            for (uint32_t c = 0; c < trees[tr].n_cols; c++, at++) {
+                if (c) by_str(&j, ",");
+                by_str(&j, "[");
+                for (uint32_t k = 0; k < ns[at]; k++) {
+                    if (k) by_str(&j, ",");
+                    json_qm31(&j, sv[first_s[at] + k]);
+                }
+                by_str(&j, "]");
+            }
+            by_str(&j, "]");
+        }
+    }
+    by_str(&j, "],\"decommitments\":[");
+    for (int tr = 0; tr < ntr; tr++) {
+        bytes hw = {0};
+        if (trees[tr].n_cols) merkle_decommit_witness(trees[tr].tree, qs, nuq, &hw);
+        if (tr) by_str(&j, ",");
+        by_str(&j, "{\"hash_witness\":");
+        json_hash_list(&j, &hw);
+        by_str(&j, "}");
+        free(hw.d);
+    }
+    by_str(&j, "],\"queried_values\":[");
+    for (int tr = 0; tr < ntr; tr++) {
+        if (tr) by_str(&j, ",");
+        by_str(&j, "[");
+        for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
+            if (c) by_str(&j, ",");
+            by_str(&j, "[");
+            for (size_t q = 0; q < nuq; q++) {
+                if (q) by_str(&j, ",");
+                by_u64(&j, trees[tr].lde[c][lift_index(qs[q], trees[tr].tree->max_log, trees[tr].log_sizes[c])]);
+            }
+            by_str(&j, "]");
+        }
+        by_str(&j, "]");
+    }
+    by_str(&j, "],\"proof_of_work\":");
+    by_u64(&j, nonce);
+    by_str(&j, ",\"fri_proof\":{\"first_layer\":");
+    /* First layer column is the quotient (AoS view). */
+    fri_layer_json(&j, ftrees[0], fcols[0], qs, nuq);
+    by_str(&j, ",\"inner_layers\":[");
+    uint64_t *lq = malloc(sizeof(uint64_t) * (nuq + 1));
+    size_t nlq = 0;
+    for (size_t i = 0; i < nuq; i++) {
+        uint64_t v = qs[i] >> 1;
+        if (nlq == 0 || lq[nlq - 1] != v) lq[nlq++] = v;
+    }
+    for (uint32_t k = 0; k < ninner; k++) {
+        if (k) by_str(&j, ",");
+        fri_layer_json(&j, ftrees[1 + k], fcols[1 + k], lq, nlq);
+        size_t m2 = 0;
+        for (size_t i = 0; i < nlq; i++) {
+            uint64_t v = lq[i] >> 1;
+            if (m2 == 0 || lq[m2 - 1] != v) lq[m2++] = v;
+        }
+        nlq = m2;
+    }
+    by_str(&j, "],\"last_layer_poly\":[");
+    for (uint64_t i = 0; i < bound; i++) {
+        if (i) by_str(&j, ",");
+        json_qm31(&j, lastp[i]);
+    }
+    by_str(&j, "]}}");
+    double t8 = now_ms();
+
+    g_timings[0] = tm[1] - tm[0]; /* trace generation */
+    g_timings[1] = tm[2] - tm[1]; /* main commit */
+    g_timings[2] = t3 - tm[2];    /* composition + commit */
+    g_timings[3] = t4 - t3;       /* sampled values */
+    g_timings[4] = t5 - t4;       /* quotients */
+    g_timings[5] = t6 - t5;       /* FRI commit */
+    g_timings[6] = t7 - t6;       /* PoW */
+    g_timings[7] = t8 - tm[1];    /* prove total (excludes trace generation) */
+
+    int rc = 0;
+    *out_len = j.n;
+    if (j.n > cap) rc = 2;
+    else memcpy(out, j.d, j.n);
+    free(j.d);
+    free(lq);
+    free(qs);
+    free(lastp);
+    for (uint32_t k = 0; k <= ninner; k++) {
+        ora_tree_free(ftrees[k]);
+        free(fcols[k]);
+    }
+    free(ftrees);
+    free(fcols);
+    free(flogs);
+    free(sv);
+    free(spt);
+    free(fl_lde);
+    free(fl_coef);
+    free(fl_log);
+    free(fl_clog);
+    free(first_s);
+    *chp = ch;
+    return rc;
+}
+
 static int prove_core(int air, const ora_widefib_params *prm, const uint64_t *xs, uint8_t *out, size_t cap,
                       size_t *out_len) {
     const uint32_t n = prm->log_n_rows, W = prm->sequence_len, B = prm->log_blowup_factor;
@@ -1185,300 +1578,19 @@ static int prove_core(int air, const ora_widefib_params *prm, const uint64_t *xs
     commit_coeffs(&trees[2], 8, cc, clogs, B, &ch);
     double t3 = now_ms();
 
-    /* OODS point (circle.zig:159-167). */
-    qm31 tt = ch_draw_secure_felt(&ch);
-    qm31 t2_ = q_square(tt);
-    qm31 inv1 = q_inv(q_add(t2_, Q_ONE));
-    qpt oods = {q_mul(q_sub(Q_ONE, t2_), inv1), q_mul(q_add(tt, tt), inv1)};
-
-    /* Sampled values (sampled_values.zig): every column sampled once at the OODS
-     * point folded (lifting - log) times; here all trees share one log size. */
-    uint32_t L = 0;
-    for (int tr = 0; tr < 3; tr++)
-        for (uint32_t c = 0; c < trees[tr].n_cols; c++)
-            if (trees[tr].log_sizes[c] > L) L = trees[tr].log_sizes[c];
+    /* OODS point (circle.zig:159-167); every column is sampled once at it. */
+    qpt oods = draw_oods_point(&ch);
     uint32_t total_cols = trees[0].n_cols + trees[1].n_cols + trees[2].n_cols;
-    qm31 *sv = malloc(sizeof(qm31) * total_cols);
-    qpt *spt = malloc(sizeof(qpt) * total_cols);
-    {
-        size_t at = 0;
-        for (int tr = 0; tr < 3; tr++)
-            for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
-                qpt p = oods;
-                for (uint32_t f = trees[tr].log_sizes[c]; f < L; f++) { /* repeatedDouble */
-                    qm31 xx = q_square(p.x), xy = q_mul(p.x, p.y);
-                    p.x = q_sub(q_add(xx, xx), Q_ONE);
-                    p.y = q_add(xy, xy);
-                }
-                spt[at++] = p;
-            }
-        /* All wide-Fib columns share one point and one coefficient log size. */
-        qm31 *w = eval_weights(spt[0], n);
-        at = 0;
-        for (int tr = 0; tr < 3; tr++) {
-            uint32_t nc = trees[tr].n_cols;
-#pragma omp parallel for schedule(dynamic, 1)
-            for (int64_t c = 0; c < (int64_t)nc; c++) sv[at + c] = dot_weights(trees[tr].coeffs[c], w, N);
-            at += nc;
-        }
-        free(w);
+    uint32_t *ns = malloc(4 * total_cols);
+    qpt *pts = malloc(sizeof(qpt) * 2 * total_cols);
+    for (uint32_t i = 0; i < total_cols; i++) {
+        ns[i] = 1;
+        pts[2 * i] = oods;
     }
-    /* sampled_value_transcript.mixIntoChannel: tree, column, sample order. */
-    ch_mix_felts(&ch, sv, total_cols);
-    double t4 = now_ms();
-
-    /* Quotients (core/pcs/quotients/{samples,row_evaluation}.zig; constraints.zig:87-97). */
-    qm31 qalpha = ch_draw_secure_felt(&ch);
-    uint64_t LN = 1ull << L;
-    qm31 *lc_c = malloc(sizeof(qm31) * total_cols);
-    qm31 sum_a = Q_ZERO, sum_b = Q_ZERO;
-    {
-        qm31 rp = Q_ONE;
-        for (uint32_t i = 0; i < total_cols; i++) {
-            qm31 v = sv[i];
-            qm31 a = q_sub(q_conj(v), v);
-            qm31 c = q_sub(q_conj(spt[i].y), spt[i].y);
-            qm31 b = q_sub(q_mul(v, c), q_mul(a, spt[i].y));
-            lc_c[i] = q_mul(rp, c);
-            sum_a = q_add(sum_a, q_mul(rp, a));
-            sum_b = q_add(sum_b, q_mul(rp, b));
-            rp = q_mul(rp, qalpha);
-        }
-    }
-    const uint32_t **qcols = malloc(sizeof(uint32_t *) * total_cols);
-    uint32_t *qlogs = malloc(4 * total_cols);
-    {
-        size_t at = 0;
-        for (int tr = 0; tr < 3; tr++)
-            for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
-                qcols[at] = trees[tr].lde[c];
-                qlogs[at++] = trees[tr].log_sizes[c];
-            }
-    }
-    cm31 prx = oods.x.c0, pry = oods.y.c0, pix = oods.x.c1, piy = oods.y.c1;
-    cm31 det = c_sub(c_mul(prx, piy), c_mul(pry, pix));
-    qm31 *quot = malloc(sizeof(qm31) * LN);
-#pragma omp parallel for schedule(static)
-    for (int64_t row = 0; row < (int64_t)LN; row++) {
-        pt dp = domain_at(L, bitrev((uint64_t)row, L));
-        qm31 num = Q_ZERO;
-        for (uint32_t i = 0; i < total_cols; i++) {
-            uint32_t v = qcols[i][lift_index((uint64_t)row, L, qlogs[i])];
-            num = q_add(num, q_mul_m(lc_c[i], v));
-        }
-        num = q_sub(num, q_add(q_mul_m(sum_a, dp.y), sum_b));
-        cm31 den = c_add(c_sub(det, c_mul_m(piy, dp.x)), c_mul_m(pix, dp.y));
-        quot[row] = q_mul_c(num, c_inv(den));
-    }
-    free(qcols);
-    free(qlogs);
-    free(lc_c);
-    double t5 = now_ms();
-
-    /* FRI commit (prover/fri.zig:163-621). */
-    uint32_t nlayers_max = L + 1;
-    ora_tree **ftrees = calloc(nlayers_max, sizeof(ora_tree *));
-    qm31 **fcols = calloc(nlayers_max, sizeof(qm31 *));
-    uint32_t *flogs = calloc(nlayers_max, 4);
-    uint32_t *first_coords[4];
-    ftrees[0] = commit_qm31_column(quot, L, first_coords);
-    fcols[0] = quot;
-    flogs[0] = L;
-    {
-        uint8_t r[32];
-        ora_tree_root(ftrees[0], r);
-        ch_mix_root(&ch, r);
-    }
-    qm31 fa = ch_draw_secure_felt(&ch);
-    uint64_t ln = LN / 2;
-    qm31 *line = calloc(ln, sizeof(qm31));
-    ora_fold_circle_into_line((const uint32_t *const *)first_coords, L, (const uint32_t *)&fa, (uint32_t *)line);
-    for (int k = 0; k < 4; k++) free(first_coords[k]);
-    coset ldom = coset_half_odds(L - 1);
-    uint32_t llog = L - 1;
-    const uint32_t last_log = prm->log_last_layer_degree_bound + B;
-    uint32_t ninner = 0;
-    while (llog > last_log) {
-        ftrees[1 + ninner] = commit_qm31_column(line, llog, NULL);
-        fcols[1 + ninner] = line;
-        flogs[1 + ninner] = llog;
-        uint8_t r[32];
-        ora_tree_root(ftrees[1 + ninner], r);
-        ch_mix_root(&ch, r);
-        qm31 la = ch_draw_secure_felt(&ch);
-        qm31 *next = malloc(sizeof(qm31) * (1ull << (llog - 1)));
-        ora_fold_line((const uint32_t *)line, llog, ldom.initial_index, (const uint32_t *)&la, (uint32_t *)next);
-        ldom = coset_double(ldom);
-        llog--;
-        line = next;
-        ninner++;
-    }
-    /* Last layer (fri.zig:594-621; line_evaluation.zig:97-151; line.zig:134-142). */
-    uint64_t lsz = 1ull << llog;
-    bitrev_q(line, lsz);
-    {
-        coset cd = ldom;
-        while ((1ull << cd.log_size) > 1) {
-            uint64_t csz = 1ull << cd.log_size, half = csz / 2;
-            for (uint64_t st0 = 0; st0 < lsz; st0 += csz) {
-                pt cur = idx_to_point(cd.initial_index), step = idx_to_point(cd.step_size);
-                for (uint64_t i = 0; i < half; i++) {
-                    uint32_t ix = m_inv(cur.x);
-                    qm31 v0 = line[st0 + i], v1 = line[st0 + half + i];
-                    line[st0 + i] = q_add(v0, v1);
-                    line[st0 + half + i] = q_mul_m(q_sub(v0, v1), ix);
-                    cur = p_add(cur, step);
-                }
-            }
-            cd = coset_double(cd);
-        }
-        uint32_t li = m_inv((uint32_t)(lsz % P));
-        for (uint64_t i = 0; i < lsz; i++) line[i] = q_mul_m(line[i], li);
-    }
-    bitrev_q(line, lsz); /* ordered coefficients */
-    uint64_t bound = 1ull << prm->log_last_layer_degree_bound;
-    for (uint64_t i = bound; i < lsz; i++)
-        if (!q_is_zero(line[i])) return 3; /* InvalidLastLayerDegree */
-    qm31 *lastp = malloc(sizeof(qm31) * bound);
-    memcpy(lastp, line, sizeof(qm31) * bound);
-    bitrev_q(lastp, bound); /* fromOrderedCoefficients */
-    free(line);
-    ch_mix_felts(&ch, lastp, bound);
-    double t6 = now_ms();
-
-    /* PoW (scheme.zig:733-735). */
-    uint64_t nonce = ora_pow_grind(ch.digest, prm->pow_bits);
-    ch_mix_u64(&ch, nonce);
-    double t7 = now_ms();
-
-    /* Queries (core/queries.zig:8-60). */
-    uint32_t nq = prm->n_queries;
-    uint64_t *qs = malloc(sizeof(uint64_t) * (nq + 8));
-    {
-        uint32_t got = 0;
-        while (got < nq) {
-            uint32_t w[8];
-            ch_draw_u32s(&ch, w);
-            for (int i = 0; i < 8 && got < nq; i++) qs[got++] = w[i] & ((1u << L) - 1);
-        }
-    }
-    qsort(qs, nq, sizeof(uint64_t), cmp_u64);
-    size_t nuq = 0;
-    for (uint32_t i = 0; i < nq; i++)
-        if (nuq == 0 || qs[nuq - 1] != qs[i]) qs[nuq++] = qs[i];
-
-    /* Assemble the proof wire (interop/proof_wire.zig:19-58). */
-    bytes j = {0};
-    by_str(&j, "{\"config\":{\"pow_bits\":");
-    by_u64(&j, prm->pow_bits);
-    by_str(&j, ",\"fri_config\":{\"log_blowup_factor\":");
-    by_u64(&j, B);
-    by_str(&j, ",\"log_last_layer_degree_bound\":");
-    by_u64(&j, prm->log_last_layer_degree_bound);
-    by_str(&j, ",\"n_queries\":");
-    by_u64(&j, nq);
-    by_str(&j, ",\"fold_step\":1},\"lifting_log_size\":null},\"commitments\":[");
-    for (int tr = 0; tr < 3; tr++) {
-        uint8_t r[32];
-        ora_tree_root(trees[tr].tree, r);
-        if (tr) by_str(&j, ",");
-        json_hash(&j, r);
-    }
-    by_str(&j, "],\"sampled_values\":[");
-    {
-        size_t at = 0;
-        for (int tr = 0; tr < 3; tr++) {
-            by_str(&j, tr ? ",[" : "[");
-            for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
-                if (c) by_str(&j, ",");
-                by_str(&j, "[");
-                json_qm31(&j, sv[at++]);
-                by_str(&j, "]");
-            }
-            by_str(&j, "]");
-        }
-    }
-    by_str(&j, "],\"decommitments\":[");
-    for (int tr = 0; tr < 3; tr++) {
-        bytes hw = {0};
-        if (trees[tr].n_cols) merkle_decommit_witness(trees[tr].tree, qs, nuq, &hw);
-        if (tr) by_str(&j, ",");
-        by_str(&j, "{\"hash_witness\":");
-        json_hash_list(&j, &hw);
-        by_str(&j, "}");
-        free(hw.d);
-    }
-    by_str(&j, "],\"queried_values\":[");
-    for (int tr = 0; tr < 3; tr++) {
-        if (tr) by_str(&j, ",");
-        by_str(&j, "[");
-        for (uint32_t c = 0; c < trees[tr].n_cols; c++) {
-            if (c) by_str(&j, ",");
-            by_str(&j, "[");
-            for (size_t q = 0; q < nuq; q++) {
-                if (q) by_str(&j, ",");
-                by_u64(&j, trees[tr].lde[c][lift_index(qs[q], trees[tr].tree->max_log, trees[tr].log_sizes[c])]);
-            }
-            by_str(&j, "]");
-        }
-        by_str(&j, "]");
-    }
-    by_str(&j, "],\"proof_of_work\":");
-    by_u64(&j, nonce);
-    by_str(&j, ",\"fri_proof\":{\"first_layer\":");
-    /* First layer column is the quotient (AoS view). */
-    fri_layer_json(&j, ftrees[0], fcols[0], qs, nuq);
-    by_str(&j, ",\"inner_layers\":[");
-    uint64_t *lq = malloc(sizeof(uint64_t) * (nuq + 1));
-    size_t nlq = 0;
-    for (size_t i = 0; i < nuq; i++) {
-        uint64_t v = qs[i] >> 1;
-        if (nlq == 0 || lq[nlq - 1] != v) lq[nlq++] = v;
-    }
-    for (uint32_t k = 0; k < ninner; k++) {
-        if (k) by_str(&j, ",");
-        fri_layer_json(&j, ftrees[1 + k], fcols[1 + k], lq, nlq);
-        size_t m2 = 0;
-        for (size_t i = 0; i < nlq; i++) {
-            uint64_t v = lq[i] >> 1;
-            if (m2 == 0 || lq[m2 - 1] != v) lq[m2++] = v;
-        }
-        nlq = m2;
-    }
-    by_str(&j, "],\"last_layer_poly\":[");
-    for (uint64_t i = 0; i < bound; i++) {
-        if (i) by_str(&j, ",");
-        json_qm31(&j, lastp[i]);
-    }
-    by_str(&j, "]}}");
-    double t8 = now_ms();
-
-    g_timings[0] = t1 - t0; /* trace generation */
-    g_timings[1] = t2 - t1; /* main commit */
-    g_timings[2] = t3 - t2; /* composition + commit */
-    g_timings[3] = t4 - t3; /* sampled values */
-    g_timings[4] = t5 - t4; /* quotients */
-    g_timings[5] = t6 - t5; /* FRI commit */
-    g_timings[6] = t7 - t6; /* PoW */
-    g_timings[7] = t8 - t1; /* prove total (excludes trace generation) */
-
-    int rc = 0;
-    *out_len = j.n;
-    if (j.n > cap) rc = 2;
-    else memcpy(out, j.d, j.n);
-    free(j.d);
-    free(lq);
-    free(qs);
-    free(lastp);
-    for (uint32_t k = 0; k <= ninner; k++) {
-        ora_tree_free(ftrees[k]);
-        free(fcols[k]);
-    }
-    free(ftrees);
-    free(fcols);
-    free(flogs);
-    free(sv);
-    free(spt);
+    double tm[3] = {t0, t1, t2};
+    int rc = prove_tail(&ch, trees, 3, ns, pts, prm, tm, t3, out, cap, out_len);
+    free(ns);
+    free(pts);
     for (int tr = 0; tr < 3; tr++) ctree_free(&trees[tr]);
     return rc;
 }

@@ -67,6 +67,9 @@ def lib():
         L.ora_prove_xor.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(WideFibParams),
                                     ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.ora_xor_example_columns.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, u32p, u32p, u32p]
+        L.ora_prove_logup_xor.restype = ctypes.c_int
+        L.ora_prove_logup_xor.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(WideFibParams),
+                                          ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.ora_prove_poseidon.restype = ctypes.c_int
         L.ora_prove_poseidon.argtypes = [ctypes.c_uint32, ctypes.POINTER(WideFibParams), ctypes.c_char_p,
                                          ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
@@ -278,6 +281,22 @@ def prove_xor(log_size, log_step, offset, pow_bits=10, log_blowup_factor=1, log_
         buf = ctypes.create_string_buffer(cap)
         n = ctypes.c_size_t(0)
         rc = lib().ora_prove_xor(log_size, log_step, offset, ctypes.byref(prm), buf, cap, ctypes.byref(n))
+        if rc == 2:
+            cap = n.value + 1
+            continue
+        if rc != 0:
+            raise RuntimeError("oracle prove failed with code %d" % rc)
+        return buf.raw[: n.value]
+
+
+def prove_logup_xor(log_n, seed, pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3):
+    """Whole proof of the XOR-lookup AIR (see ora_prove_logup_xor)."""
+    prm = WideFibParams(0, 0, pow_bits, log_blowup_factor, log_last_layer_degree_bound, n_queries, 0, 0)
+    cap = 1 << 20
+    while True:
+        buf = ctypes.create_string_buffer(cap)
+        n = ctypes.c_size_t(0)
+        rc = lib().ora_prove_logup_xor(log_n, seed, ctypes.byref(prm), buf, cap, ctypes.byref(n))
         if rc == 2:
             cap = n.value + 1
             continue

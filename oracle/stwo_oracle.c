@@ -1688,6 +1688,188 @@ int ora_logup_cumulative(int num_kind, const uint32_t *num, const uint32_t *den,
     return 0;
 }
 
+/* ======================================================================
+ * XOR-lookup AIR as a whole STARK proof (defined by this backend; the LogUp machinery follows
+ * src/frontends/riscv/air/logup.zig:74-105 and the lookup-table component of
+ * src/frontends/riscv/air/lookups/tables/component.zig):
+ *   tree 0 (preprocessed): is_first, the table tuple (tx, ty, tz) = ((i >> 8) & 255, i & 255, xor)
+ *   tree 1 (main): a, b, c = a ^ b (uniform 8-bit lookups) and the table multiplicities m
+ *   tree 2 (interaction): the cumulative column S of the row pairs (1 / d1) + (-m / d2),
+ *           d1 = z - (a + alpha b + alpha^2 c), d2 = z - (tx + alpha ty + alpha^2 tz)
+ *   tree 3: the composition of the pair constraint (logup.zig:93-105)
+ *           (S(x) - S(x - g) + is_first(x) * claimed) * d1 * d2 - (d2 - m * d1),
+ *           g the step of the trace coset, so S(x - g) is the previous row.
+ * Rows are in coset order, row i stored at its bit-reversed circle-domain position; the
+ * lookups balance the table, so the claimed sum is zero (the statement).  Transcript: config,
+ * root 0, root 1, mixU32s([log_n]), z, alpha, root 2, mixFelts([claimed]), composition
+ * coefficient, root 3, OODS point; masks: trees 0, 1 and 3 at the OODS point, the interaction
+ * columns at the OODS point and at it minus g.
+ * ==================================================================== */
+static uint64_t prev_row_position(uint32_t lg, uint64_t r) {
+    /* position of x - g (g = CanonicCoset(lg - 1).step) for the point x at bit-reversed
+     * position r of CanonicCoset(lg).circleDomain() */
+    uint64_t half = 1ull << (lg - 1), i = bitrev(r, lg), j;
+    if (i < half) j = (i + half - 1) & (half - 1);
+    else j = half + ((i - half + 1) & (half - 1));
+    return bitrev(j, lg);
+}
+
+int ora_prove_logup_xor(uint32_t log_n, uint64_t seed, const ora_widefib_params *prm, uint8_t *out, size_t cap,
+                        size_t *out_len) {
+    const uint32_t n = log_n, B = prm->log_blowup_factor;
+    if (n < 16 || n > 26 || B < 1) return 1;
+    double t0 = now_ms();
+    memset(g_timings, 0, sizeof g_timings);
+    init_gen_doubles();
+    channel ch;
+    memset(&ch, 0, sizeof ch);
+    qm31 cfg = {{prm->pow_bits, B}, {prm->n_queries, prm->log_last_layer_degree_bound}};
+    ch_mix_felts(&ch, &cfg, 1);
+    const uint64_t N = 1ull << n;
+    uint32_t *ar = malloc(4 * N), *br = malloc(4 * N), *cr = malloc(4 * N), *mult = malloc(4u << 16);
+    ora_xor_trace(n, seed, ar, br, cr, mult);
+    uint32_t **pc = malloc(sizeof(uint32_t *) * 4), **mc = malloc(sizeof(uint32_t *) * 4);
+    for (int k = 0; k < 4; k++) {
+        pc[k] = malloc(4 * N);
+        mc[k] = malloc(4 * N);
+    }
+    for (uint64_t i = 0; i < N; i++) {
+        uint64_t p = circle_bitrev_index(n, i);
+        uint32_t tx = (uint32_t)((i >> 8) & 0xff), ty = (uint32_t)(i & 0xff);
+        pc[0][p] = i == 0;
+        pc[1][p] = tx;
+        pc[2][p] = ty;
+        pc[3][p] = tx ^ ty;
+        mc[0][p] = ar[i];
+        mc[1][p] = br[i];
+        mc[2][p] = cr[i];
+        mc[3][p] = i < (1u << 16) ? mult[i] : 0;
+    }
+    double t1 = now_ms();
+    ctree trees[4];
+    memset(trees, 0, sizeof trees);
+    uint32_t logs4[4] = {n, n, n, n};
+    get_twiddles(n + B > n + 1 ? n + B : n + 1);
+#pragma omp parallel for
+    for (int k = 0; k < 8; k++) circle_interpolate(k < 4 ? pc[k] : mc[k - 4], n);
+    commit_coeffs(&trees[0], 4, pc, logs4, B, &ch);
+    commit_coeffs(&trees[1], 4, mc, logs4, B, &ch);
+    double t2 = now_ms();
+    ch_mix_u32s(&ch, &n, 1);
+    qm31 z = ch_draw_secure_felt(&ch), al = ch_draw_secure_felt(&ch), al2 = q_square(al);
+    /* the interaction column in coset order */
+    qm31 *d1 = malloc(sizeof(qm31) * N), *d2 = malloc(sizeof(qm31) * N);
+    uint32_t **sc = malloc(sizeof(uint32_t *) * 4);
+    for (int k = 0; k < 4; k++) sc[k] = malloc(4 * N);
+    qm31 acc = Q_ZERO;
+    int zero_den = 0;
+    for (uint64_t i = 0; i < N; i++) {
+        uint32_t tx = (uint32_t)((i >> 8) & 0xff), ty = (uint32_t)(i & 0xff);
+        d1[i] = q_sub(z, logup_combine(ar[i], br[i], cr[i], al, al2));
+        d2[i] = q_sub(z, logup_combine(tx, ty, tx ^ ty, al, al2));
+        qm31 den = q_mul(d1[i], d2[i]);
+        if (q_is_zero(den)) zero_den = 1;
+        uint32_t m = i < (1u << 16) ? mult[i] : 0;
+        qm31 num = q_sub(d2[i], q_mul_m(d1[i], m));
+        if (!zero_den) acc = q_add(acc, q_mul(num, q_inv(den)));
+        uint64_t p = circle_bitrev_index(n, i);
+        sc[0][p] = acc.c0.a;
+        sc[1][p] = acc.c0.b;
+        sc[2][p] = acc.c1.a;
+        sc[3][p] = acc.c1.b;
+    }
+    free(d1);
+    free(d2);
+    free(ar);
+    free(br);
+    free(cr);
+    free(mult);
+    const qm31 claimed = acc;
+    if (zero_den || !q_is_zero(claimed)) {
+        for (int k = 0; k < 4; k++) free(sc[k]);
+        free(sc);
+        for (int tr = 0; tr < 2; tr++) ctree_free(&trees[tr]);
+        return 5; /* the lookups do not balance the table */
+    }
+#pragma omp parallel for
+    for (int k = 0; k < 4; k++) circle_interpolate(sc[k], n);
+    commit_coeffs(&trees[2], 4, sc, logs4, B, &ch);
+    ch_mix_felts(&ch, &claimed, 1);
+    (void)ch_draw_secure_felt(&ch); /* composition coefficient: one constraint, power 1 */
+
+    /* composition on CanonicCoset(n + 1) */
+    const uint32_t CL = n + 1;
+    const uint64_t CN = 1ull << CL;
+    const uint32_t *ev[12];
+    uint32_t *own[12] = {0};
+    for (int k = 0; k < 12; k++) {
+        const ctree *t = &trees[k / 4];
+        if (B == 1) {
+            ev[k] = t->lde[k % 4];
+        } else {
+            own[k] = malloc(4 * CN);
+            ora_circle_lde(t->coeffs[k % 4], n, 1, own[k]);
+            ev[k] = own[k];
+        }
+    }
+    uint32_t den_inv[2];
+    for (int r = 0; r < 2; r++) {
+        coset tc_ = coset_odds(n);
+        pt dp = domain_at(CL, (uint64_t)r);
+        pt shifted = p_add(p_add(dp, p_conj(idx_to_point(tc_.initial_index))), idx_to_point(tc_.step_size / 2));
+        uint32_t x = shifted.x;
+        for (uint32_t i = 1; i < n; i++) x = m_double_x(x);
+        den_inv[r] = m_inv(x);
+    }
+    uint32_t *comp[4];
+    for (int k = 0; k < 4; k++) comp[k] = malloc(4 * CN);
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < (int64_t)CN; r++) {
+        uint64_t q = prev_row_position(CL, (uint64_t)r);
+        qm31 S = {{ev[8][r], ev[9][r]}, {ev[10][r], ev[11][r]}};
+        qm31 Sp = {{ev[8][q], ev[9][q]}, {ev[10][q], ev[11][q]}};
+        qm31 e1 = q_sub(z, logup_combine(ev[4][r], ev[5][r], ev[6][r], al, al2));
+        qm31 e2 = q_sub(z, logup_combine(ev[1][r], ev[2][r], ev[3][r], al, al2));
+        qm31 delta = q_add(q_sub(S, Sp), q_mul_m(claimed, ev[0][r]));
+        qm31 cons = q_sub(q_mul(q_mul(delta, e1), e2), q_sub(e2, q_mul_m(e1, ev[7][r])));
+        cons = q_mul_m(cons, den_inv[(uint64_t)r >> n]);
+        comp[0][r] = cons.c0.a;
+        comp[1][r] = cons.c0.b;
+        comp[2][r] = cons.c1.a;
+        comp[3][r] = cons.c1.b;
+    }
+    for (int k = 0; k < 12; k++) free(own[k]);
+    get_twiddles(CL);
+#pragma omp parallel for
+    for (int k = 0; k < 4; k++) circle_interpolate(comp[k], CL);
+    uint32_t **cc = malloc(sizeof(uint32_t *) * 8);
+    uint32_t clogs[8];
+    for (int k = 0; k < 4; k++) {
+        cc[k] = malloc(4 * N);
+        cc[4 + k] = malloc(4 * N);
+        memcpy(cc[k], comp[k], 4 * N);
+        memcpy(cc[4 + k], comp[k] + N, 4 * N);
+        clogs[k] = clogs[4 + k] = n;
+        free(comp[k]);
+    }
+    commit_coeffs(&trees[3], 8, cc, clogs, B, &ch);
+    double t3 = now_ms();
+
+    qpt oods = draw_oods_point(&ch);
+    qpt prev = qp_add_m(oods, idx_to_point(idx_neg(subgroup_gen(n))));
+    uint32_t ns[20];
+    qpt pts[40];
+    for (int i = 0; i < 20; i++) {
+        ns[i] = (i >= 8 && i < 12) ? 2 : 1;
+        pts[2 * i] = oods;
+        pts[2 * i + 1] = prev;
+    }
+    double tm[3] = {t0, t1, t2};
+    int rc = prove_tail(&ch, trees, 4, ns, pts, prm, tm, t3, out, cap, out_len);
+    for (int tr = 0; tr < 4; tr++) ctree_free(&trees[tr]);
+    return rc;
+}
+
 /* EqEvals over the hypercube: out[i] = scale * prod_j (bit_{n-1-j}(i) ? y_j : 1 - y_j)
  * (gkr_circuit.zig:651-674: y[0] selects the upper half). */
 void ora_gkr_gen_eq_evals(const uint32_t *y, uint32_t n, const uint32_t scale[4], uint32_t *out) {

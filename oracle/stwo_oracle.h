@@ -127,6 +127,12 @@ size_t ora_gkr_prove(uint32_t n_inst, const int *kinds, const uint32_t *const *n
 size_t ora_logup_xor_pipeline(uint32_t log_n, uint64_t seed, uint32_t *out, size_t cap);
 size_t ora_batched_pipeline(uint32_t n_columns, uint32_t log_min, uint32_t log_max, uint32_t log_blowup,
                             uint32_t fri_log, uint32_t n_folds, uint64_t seed, uint32_t *out, size_t cap);
+/* XOR-lookup AIR as a whole proof (tree 0 is_first + table tuple, tree 1 a, b, c, multiplicities,
+ * tree 2 the pair-batched cumulative column, tree 3 the composition of the pair constraint);
+ * prm supplies the PCS configuration.  Returns 5 if the lookups do not balance. */
+int ora_prove_logup_xor(uint32_t log_n, uint64_t seed, const ora_widefib_params *prm, uint8_t *out, size_t cap,
+                        size_t *out_len);
+
 
 #ifdef __cplusplus
 }

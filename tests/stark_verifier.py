@@ -1,5 +1,6 @@
 """Independent pure-Python restatement of the reference STARK *verifier* for the proofs
-this backend produces (wide-Fibonacci, Poseidon and periodic XOR example AIRs), used as
+this backend produces (wide-Fibonacci, Poseidon and periodic XOR example AIRs, and this
+backend's XOR-lookup AIR with its LogUp interaction tree), used as
 an acceptance oracle: a proof that no committed golden covers is checked here, not
 only against the C oracle (test infrastructure; never on the product path).
 
@@ -294,7 +295,7 @@ def _air_shape(air, st):
                 a, b = b, c
             return acc
 
-        return [], [n] * W, n + 1, [], mix, comp
+        return [], [n] * W, n + 1, [], mix, comp, None
     if air == "poseidon":
         li = st["log_n_instances"]
         n = li - 3
@@ -306,7 +307,7 @@ def _air_shape(air, st):
         def comp(point, sv, alpha):
             return (li % P, POSEIDON_COLUMNS_PER_REP, POSEIDON_COLUMNS, 1)
 
-        return [], [n] * POSEIDON_COLUMNS, n + 1, [], mix, comp
+        return [], [n] * POSEIDON_COLUMNS, n + 1, [], mix, comp, None
     if air == "xor":
         n, ls, off = st["log_size"], st["log_step"], st["offset"]
         _check(n >= 1 and ls <= n, "statement out of range")
@@ -318,7 +319,53 @@ def _air_shape(air, st):
         def comp(point, sv, alpha):
             return (n, ls, off % P, 1)
 
-        return [n, n], [n], n + 1, [0, 1], mix, comp
+        return [n, n], [n], n + 1, [0, 1], mix, comp, None
+    if air == "xor_logup":
+        # the XOR-lookup AIR of this backend (the pair-batched LogUp constraint of
+        # frontends/riscv/air/logup.zig:93-105 over a preprocessed table component)
+        n = st["log_n"]
+        _check(16 <= n < 31, "statement out of range")
+        claimed = ZERO  # the lookups close the bus: the claimed sum is zero
+
+        def mix(ch):
+            ch.mix_u32s([n])
+
+        def draw(ch):
+            z = ch.draw_secure_felt()
+            return {"z": z, "alpha": ch.draw_secure_felt()}
+
+        def after(ch, state):
+            ch.mix_felts([claimed])
+
+        g = point_at((-subgroup_gen(n)) % ORDER)  # minus the trace coset's step
+
+        def prev(point):
+            return qp_add(point, (qm(g[0]), qm(g[1])))
+
+        def secure(vals):
+            return qadd(qadd(vals[0], qmul(vals[1], (0, 1, 0, 0))), qadd(qmul(vals[2], (0, 0, 1, 0)),
+                                                                      qmul(vals[3], (0, 0, 0, 1))))
+
+        def comp(point, sv, alpha, state):
+            pre, main, inter = sv[0], sv[1], sv[2]
+            _check(len(pre) == 4 and len(main) == 4 and len(inter) == 4, "sampled value shape")
+            z, al = state["z"], state["alpha"]
+            al2 = qmul(al, al)
+
+            def comb(x, y, w):
+                return qsub(z, qadd(qadd(x, qmul(al, y)), qmul(al2, w)))
+
+            isf, tx, ty, tz = (c[0] for c in pre)
+            a, b, c, m = (col[0] for col in main)
+            s_cur = secure([col[0] for col in inter])
+            s_prev = secure([col[1] for col in inter])
+            d1, d2 = comb(a, b, c), comb(tx, ty, tz)
+            delta = qadd(qsub(s_cur, s_prev), qmul(isf, claimed))
+            cons = qsub(qmul(qmul(delta, d1), d2), qsub(d2, qmul(m, d1)))
+            return qmul(cons, qinv(coset_vanishing_qm(n, point)))
+
+        inter = {"logs": [n] * 4, "npts": [2] * 4, "draw": draw, "after": after, "prev": prev}
+        return [n] * 4, [n] * 4, n + 1, [0, 1, 2, 3], mix, comp, inter
     raise VerificationError("unknown AIR " + air)
 
 
@@ -415,7 +462,8 @@ def verify_proof(proof_bytes, air, statement, pcs_config):
            pc["lifting_log_size"] == cfg["lifting_log_size"], "proof config differs from the PCS config")
     B, n_queries, pow_bits = cfg["log_blowup_factor"], cfg["n_queries"], cfg["pow_bits"]
     llb = cfg["log_last_layer_degree_bound"]
-    pre_logs, main_logs, comp_log, pre_sampled, mix_statement, comp_eval = _air_shape(air, statement)
+    pre_logs, main_logs, comp_log, pre_sampled, mix_statement, comp_eval, inter = _air_shape(air, statement)
+    n_trees = 4 if inter else 3
 
     ch = Channel()
     ch.mix_felts([(pow_bits, B, n_queries, llb)])
@@ -424,29 +472,41 @@ def verify_proof(proof_bytes, air, statement, pcs_config):
     _check(cfg["fold_step"] == 1, "only fold_step 1 is supported")
 
     commits = pr["commitments"]
-    _check(len(commits) == 3, "commitment count")
+    _check(len(commits) == n_trees, "commitment count")
     for c in commits:
         _check(len(c) == 32 and all(0 <= b < 256 for b in c), "commitment shape")
     tree_logs = [[l + B for l in pre_logs], [l + B for l in main_logs]]
     ch.mix_root(commits[0])
     ch.mix_root(commits[1])
     mix_statement(ch)
+    state = None
+    if inter:
+        # relation challenges, the interaction tree and its claimed sums (the interaction phase)
+        state = inter["draw"](ch)
+        ch.mix_root(commits[2])
+        inter["after"](ch, state)
+        tree_logs.append([l + B for l in inter["logs"]])
     _check(comp_log > 1, "composition log size")
     alpha = ch.draw_secure_felt()
-    ch.mix_root(commits[2])
+    ch.mix_root(commits[-1])
     tree_logs.append([comp_log - 1 + B] * 8)
     oods = random_secure_point(ch)
 
     sv = [[[_felt(v) for v in col] for col in tree] for tree in pr["sampled_values"]]
-    _check(len(sv) == 3, "sampled value trees")
-    # mask points: the preprocessed columns the component samples, every main column and the
-    # composition columns at the OODS point (components.zig:90-142, verifier.zig:109-133)
-    npts = [[1 if i in pre_sampled else 0 for i in range(len(pre_logs))], [1] * len(main_logs), [1] * 8]
-    for t in range(3):
+    _check(len(sv) == n_trees, "sampled value trees")
+    # mask points: the preprocessed columns the component samples, every main column, the
+    # interaction columns at the OODS point and the previous row, and the composition columns
+    # at the OODS point (components.zig:90-142, verifier.zig:109-133)
+    npts = [[1 if i in pre_sampled else 0 for i in range(len(pre_logs))], [1] * len(main_logs)]
+    if inter:
+        npts.append(inter["npts"])
+    npts.append([1] * 8)
+    for t in range(n_trees):
         _check(len(sv[t]) == len(npts[t]), "sampled value columns of tree %d" % t)
         for col, k in zip(sv[t], npts[t]):
             _check(len(col) == k, "sampled values per column")
-    comp_cols = [col[0] for col in sv[2]]
+    mask_points = [oods, inter["prev"](oods) if inter else None]
+    comp_cols = [col[0] for col in sv[-1]]
     left = qadd(qadd(comp_cols[0], qmul(comp_cols[1], (0, 1, 0, 0))),
                 qadd(qmul(comp_cols[2], (0, 0, 1, 0)), qmul(comp_cols[3], (0, 0, 0, 1))))
     right = qadd(qadd(comp_cols[4], qmul(comp_cols[5], (0, 1, 0, 0))),
@@ -454,12 +514,13 @@ def verify_proof(proof_bytes, air, statement, pcs_config):
     x = oods[0]
     for _ in range(comp_log - 2):
         x = qp_double_x(x)
-    _check(qadd(left, qmul(x, right)) == comp_eval(oods, sv, alpha), "OODS values do not match the AIR")
+    expect = comp_eval(oods, sv, alpha, state) if inter else comp_eval(oods, sv, alpha)
+    _check(qadd(left, qmul(x, right)) == expect, "OODS values do not match the AIR")
 
     # verifyValues (pcs/verifier.zig:83-178)
     ch.mix_felts([v for tree in sv for col in tree for v in col])
     q_coeff = ch.draw_secure_felt()
-    sampled_logs = [tree_logs[t][i] for t in range(3) for i, k in enumerate(npts[t]) if k]
+    sampled_logs = [tree_logs[t][i] for t in range(n_trees) for i, k in enumerate(npts[t]) if k]
     _check(sampled_logs, "empty sampled set")
     L = max(sampled_logs)
     _check(L >= B, "lifting log size")
@@ -493,9 +554,9 @@ def verify_proof(proof_bytes, air, statement, pcs_config):
     # trace trees (the preprocessed tree at its own size, pcs/utils.zig:292-314)
     qv = pr["queried_values"]
     dc = pr["decommitments"]
-    _check(len(qv) == 3 and len(dc) == 3, "decommitment trees")
+    _check(len(qv) == n_trees and len(dc) == n_trees, "decommitment trees")
     pp_max = max(tree_logs[0]) if tree_logs[0] else 0
-    for t in range(3):
+    for t in range(n_trees):
         logs = tree_logs[t]
         if t == 0:
             if pp_max == 0:
@@ -515,15 +576,21 @@ def verify_proof(proof_bytes, air, statement, pcs_config):
         except VerificationError as e:
             raise VerificationError("tree %d: %s" % (t, e))
 
-    # quotient answers at the queries (fri_answers.zig:107-173): one batch per distinct point
+    # quotient answers at the queries (fri_answers.zig:107-173): one batch per distinct point; a
+    # column sampled at two points first adds its second sample at that point plus the lifting
+    # domain's step doubled log-size times (samples.zig:270-285)
     batches = {}
     order = []
     pw = ONE
     flat = 0
-    for t in range(3):
+    lift_gen = subgroup_gen(L)
+    for t in range(n_trees):
         for i, col in enumerate(sv[t]):
-            for v in col:
-                key = oods
+            samples = [(mask_points[k], v) for k, v in enumerate(col)]
+            if len(col) == 2:
+                g = point_at((lift_gen << tree_logs[t][i]) % ORDER)
+                samples.insert(0, (qp_add(mask_points[1], (qm(g[0]), qm(g[1]))), col[1]))
+            for key, v in samples:
                 if key not in batches:
                     batches[key] = []
                     order.append(key)

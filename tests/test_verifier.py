@@ -206,3 +206,46 @@ def test_seeded_uniform_trace_rule(log_n, seed):
         assert (int(t[0][s]), int(t[1][s])) == (a, b)
         assert int(t[2][s]) == (a * a + b * b) % P
         assert int(t[3][s]) == (b * b + int(t[2][s]) ** 2) % P
+
+
+# ---------------------------------------------------------------- XOR-lookup AIR (LogUp proof)
+LOGUP_CASES = [
+    (16, 7, dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3)),
+    (16, 99, dict(pow_bits=4, log_blowup_factor=2, log_last_layer_degree_bound=1, n_queries=20)),
+    (17, 3, dict(pow_bits=0, log_blowup_factor=1, log_last_layer_degree_bound=2, n_queries=40)),
+]
+
+
+@pytest.mark.parametrize("log_n,seed,cfg", LOGUP_CASES, ids=["16-functional", "16-blowup2", "17-last2"])
+def test_verifier_accepts_logup_xor_proof(log_n, seed, cfg):
+    """The XOR-lookup AIR's whole proof (is_first and table tuple preprocessed; a, b, c and the
+    multiplicities committed; the pair-batched cumulative column as the interaction tree,
+    sampled at the OODS point and the previous row): the composition check at the OODS point
+    ties the committed composition to the pair constraint, so acceptance pins the oracle's
+    interaction column, its previous-row mask and the two-point quotient batches."""
+    proof = oracle.prove_logup_xor(log_n, seed, **cfg)
+    sv.verify_proof(proof, "xor_logup", {"log_n": log_n}, cfg)
+    assert len(json.loads(proof)["commitments"]) == 4
+
+
+def test_verifier_rejects_logup_xor_mutations():
+    cfg = LOGUP_CASES[0][2]
+    base = json.loads(oracle.prove_logup_xor(16, 7, **cfg))
+
+    def rejected(mutate, statement=None):
+        pr = copy.deepcopy(base)
+        mutate(pr)
+        with pytest.raises(sv.VerificationError):
+            sv.verify_proof(json.dumps(pr, separators=(",", ":")).encode(), "xor_logup",
+                            statement or {"log_n": 16}, cfg)
+
+    def bump(v):
+        v[0] = (v[0] + 1) % P
+
+    rejected(lambda pr: None, {"log_n": 17})                                   # statement
+    rejected(lambda pr: bump(pr["sampled_values"][2][1][1]))                   # previous-row sample
+    rejected(lambda pr: bump(pr["sampled_values"][2][0][0]))                   # current interaction sample
+    rejected(lambda pr: bump(pr["sampled_values"][1][3][0]))                   # multiplicity sample
+    rejected(lambda pr: pr["commitments"][2].__setitem__(0, pr["commitments"][2][0] ^ 1))  # interaction root
+    rejected(lambda pr: pr["queried_values"][2][1].__setitem__(0, (pr["queried_values"][2][1][0] + 1) % P))
+    rejected(lambda pr: pr["sampled_values"][2][0].pop())                      # mask shape

@@ -60,7 +60,7 @@ def main():
                     default="widefib",
                     help="widefib: the 2^20x100 proof (default; with the batched and serving parts); logup: "
                          "XOR-lookup LogUp workload at 2^18; batched: 64 uneven columns LDE + Merkle + 3 FRI folds "
-                         "as one hipGraph; serving: 256 Poisson requests (wide-Fib, XOR, Poseidon, LogUp), 1 slot "
+                         "as one hipGraph; serving: 256 Poisson requests (wide-Fib, XOR, Poseidon, XOR-lookup AIR), 1 slot "
                          "vs continuous batching; poseidon, xor: the reference's example AIRs at the size of their "
                          "golden proofs")
     args = ap.parse_args()
@@ -549,7 +549,8 @@ def serving_requests(n=256, rate_per_s=1500.0, seed=2024):
     """Seeded request stream: Poisson arrivals at `rate_per_s`; half the requests are wide-Fibonacci
     proofs (log_n_rows 14..20, sequence_len 16..100, seeded uniform trace), the rest periodic XOR
     AIR proofs (log_size 12..18, log_step 0..4, offset < 2^20), Poseidon AIR proofs
-    (log_n_instances 12..16) and the XOR-lookup LogUp workload (log 16..18), in shares 2:1.5:1.5."""
+    (log_n_instances 12..16) and XOR-lookup AIR proofs (the LogUp AIR, log 16..18), in shares
+    2:1.5:1.5."""
     rng = np.random.default_rng(seed)
     t = 0.0
     jobs = []
@@ -563,7 +564,7 @@ def serving_requests(n=256, rate_per_s=1500.0, seed=2024):
         elif u < 0.85:
             jobs.append((2, int(rng.integers(12, 17)), 0, 0, t))
         else:
-            jobs.append((1, int(rng.integers(16, 19)), 0, int(rng.integers(1, 1 << 40)), t))
+            jobs.append((4, int(rng.integers(16, 19)), 0, int(rng.integers(1, 1 << 40)), t))
     return jobs
 
 
@@ -575,6 +576,8 @@ def _serve_ref_digest(oracle, job):
         out = oracle.prove_poseidon(n)
     elif kind == 3:
         out = oracle.prove_xor(n, w, seed)
+    elif kind == 4:
+        out = oracle.prove_logup_xor(n, seed)
     else:
         out = oracle.logup_xor_pipeline(n, seed).astype(np.uint32).tobytes()
     return hashlib.blake2s(out).hexdigest()

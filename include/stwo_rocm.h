@@ -304,6 +304,30 @@ typedef struct {
 int stwo_rocm_prove_xor(stwo_rocm_ctx* ctx, const stwo_rocm_xor_params* params, uint8_t* proof_out, uint64_t cap,
                         uint64_t* proof_len, stwo_rocm_prove_stats* stats);
 
+/* XOR-lookup AIR statement and PCS configuration (the AIR of this backend's LogUp benchmark; see
+ * stwo_rocm_prove_logup_xor). */
+typedef struct {
+    uint32_t log_n; /* 16..26 */
+    uint32_t pow_bits;
+    uint64_t seed;
+    uint32_t log_blowup_factor;
+    uint32_t log_last_layer_degree_bound;
+    uint32_t n_queries;
+    uint32_t reserved;
+} stwo_rocm_logup_params;
+
+/* No single reference counterpart: the XOR-lookup AIR of the LogUp benchmark as a whole proof
+ * through the generic transaction (prover_transaction.zig:222-339 -> prove.zig:209-389 ->
+ * scheme.zig:571-800), with the LogUp machinery of src/frontends/riscv/air/logup.zig:74-105: tree 0
+ * is_first and the table tuple (x, y, x ^ y); tree 1 the lookups a, b (uniform 8-bit from the
+ * seed), c = a ^ b and the table multiplicities; tree 2 the pair-batched cumulative column of
+ * 1 / (z - (a + alpha b + alpha^2 c)) - m / (z - (x + alpha y + alpha^2 (x ^ y))), sampled at the
+ * OODS point and at the previous row; tree 3 the composition of the pair constraint.  The claimed
+ * sum is zero (the lookups balance the table).  Same output and buffer conventions as
+ * stwo_rocm_prove_wide_fibonacci (no stage timings). */
+int stwo_rocm_prove_logup_xor(stwo_rocm_ctx* ctx, const stwo_rocm_logup_params* params, uint8_t* proof_out,
+                              uint64_t cap, uint64_t* proof_len);
+
 /* ------------------------------------------------------------------ LogUp / GKR */
 /* Layer kinds of src/prover/lookups/gkr_circuit.zig:79-92. */
 #define STWO_ROCM_GKR_GRAND_PRODUCT 0
@@ -428,8 +452,9 @@ int stwo_rocm_batched_destroy(stwo_rocm_ctx* ctx, stwo_rocm_batched* b);
  * stwo_rocm_prove_wide_fibonacci), 1 XOR-lookup LogUp workload (stwo_rocm_logup_xor_pipeline,
  * log_n 16..24), 2 Poseidon AIR proof (stwo_rocm_prove_poseidon, log_n = log_n_instances 4..24,
  * functional protocol), 3 periodic XOR AIR proof (stwo_rocm_prove_xor, log_n = log_size 2..24,
- * seq_len = log_step, seed = offset, functional protocol).  Each result carries the Blake2s-256
- * digest of its output bytes. */
+ * seq_len = log_step, seed = offset, functional protocol), 4 XOR-lookup AIR proof
+ * (stwo_rocm_prove_logup_xor, log_n 16..24, functional protocol).  Each result carries the
+ * Blake2s-256 digest of its output bytes. */
 typedef struct {
     uint32_t kind, log_n, seq_len, reserved;
     uint64_t seed;

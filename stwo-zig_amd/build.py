@@ -17,7 +17,7 @@ LIB = os.path.join(LIB_DIR, "libstwo_rocm.so")
 SOURCES = ["runtime.hip", "kernels_fft.hip", "kernels_merkle.hip", "kernels_fri.hip", "kernels_prove.hip",
            "kernels_misc.hip", "kernels_chan.hip", "kernels_logup.hip", "gkr.hip", "transcript.hip", "logup_pipeline.hip",
            "batched_pipeline.hip", "serving.hip",
-           "prover.hip", "hooks.hip",
+           "prover.hip", "hooks.hip", "logup_proof.hip",
            "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("STWO_ROCM_ARCH", "gfx950")

@@ -346,74 +346,18 @@ int stwo_rocm_compute_quotients(stwo_rocm_ctx* ctx, uint32_t L, const uint32_t* 
         for (uint32_t i = 0; i < n_columns; i++)
             SR_CHECK(column_log_sizes[i] >= 1 && column_log_sizes[i] <= L, "column log size out of range");
         // Samples per column in input order (tree/column/sample order).
-        std::vector<std::vector<uint32_t>> per_col(n_columns);
+        std::vector<std::vector<ColumnSample>> per_col(n_columns);
         for (uint32_t s = 0; s < n_samples; s++) {
             SR_CHECK(sample_columns[s] < n_columns, "sample column out of range");
             SR_CHECK(s == 0 || sample_columns[s - 1] <= sample_columns[s], "samples must be in column order");
-            per_col[sample_columns[s]].push_back(s);
+            per_col[sample_columns[s]].push_back(
+                {{qin(sample_points + 8 * s), qin(sample_points + 8 * s + 4)}, qin(sample_values + 4 * s)});
         }
-        struct Pt8 {
-            qm31 x, y;
-        };
-        std::vector<Pt8> bpts;
-        std::vector<std::vector<std::pair<uint32_t, qm31>>> bent;
-        std::vector<qm31> sa, sb;
-        qm31 alpha = qin(random_coeff), rp = qone();
-        uint32_t lift_gen = subgroup_gen(L);
-        auto add_sample = [&](uint32_t col, Pt8 p, qm31 v) {
-            size_t bi = 0;
-            for (; bi < bpts.size(); bi++)
-                if (qeq(bpts[bi].x, p.x) && qeq(bpts[bi].y, p.y)) break;
-            if (bi == bpts.size()) {
-                bpts.push_back(p);
-                bent.emplace_back();
-                sa.push_back(qzero());
-                sb.push_back(qzero());
-            }
-            // complexConjugateLineCoeffs (core/constraints.zig:87-97)
-            SR_CHECK(!qeq(p.y, qconj(p.y)), "degenerate sample point (y in the base field)");
-            qm31 a = qsub(qconj(v), v);
-            qm31 cc = qsub(qconj(p.y), p.y);
-            qm31 b = qsub(qmul(v, cc), qmul(a, p.y));
-            bent[bi].push_back({col, qmul(rp, cc)});
-            sa[bi] = qadd(sa[bi], qmul(rp, a));
-            sb[bi] = qadd(sb[bi], qmul(rp, b));
-            rp = qmul(rp, alpha);
-        };
-        for (uint32_t col = 0; col < n_columns; col++) {
-            auto& ss = per_col[col];
-            if (ss.size() == 2) {
-                // Periodicity sample (core/pcs/quotients/samples.zig:270-285).
-                uint32_t s = ss[1];
-                pt g = idx_to_point(c.gen, (uint32_t)(((uint64_t)lift_gen << column_log_sizes[col]) & ORDER_MASK));
-                Pt8 p = {qin(sample_points + 8 * s), qin(sample_points + 8 * s + 4)};
-                Pt8 q = {qsub(qmul(p.x, qfromm(g.x)), qmul(p.y, qfromm(g.y))),
-                         qadd(qmul(p.x, qfromm(g.y)), qmul(p.y, qfromm(g.x)))};
-                add_sample(col, q, qin(sample_values + 4 * s));
-            }
-            for (uint32_t s : ss) add_sample(col, {qin(sample_points + 8 * s), qin(sample_points + 8 * s + 4)},
-                                             qin(sample_values + 4 * s));
-        }
-        std::vector<QuotientBatchDev> batches;
-        std::vector<uint32_t> ecols, ecf;
-        for (size_t bi = 0; bi < bpts.size(); bi++) {
-            QuotientBatchDev bd;
-            cm31 prx = bpts[bi].x.c0, pry = bpts[bi].y.c0, pix = bpts[bi].x.c1, piy = bpts[bi].y.c1;
-            bd.det = csub(cmul(prx, piy), cmul(pry, pix));
-            bd.pix = pix;
-            bd.piy = piy;
-            bd.sum_a = sa[bi];
-            bd.sum_b = sb[bi];
-            bd.first = (uint32_t)ecols.size();
-            bd.count = (uint32_t)bent[bi].size();
-            for (auto& e : bent[bi]) {
-                ecols.push_back(e.first);
-                uint32_t w4[4];
-                qstore(e.second, w4);
-                ecf.insert(ecf.end(), w4, w4 + 4);
-            }
-            batches.push_back(bd);
-        }
+        std::vector<uint32_t> logs(column_log_sizes, column_log_sizes + n_columns);
+        QuotientPlan qp = plan_quotients(c, L, logs, per_col, qin(random_coeff));
+        std::vector<uint32_t>& ecols = qp.entry_cols;
+        std::vector<uint32_t>& ecf = qp.entry_coeffs;
+        std::vector<QuotientBatchDev>& batches = qp.batches;
         DeviceTree* t = nullptr;
         try {
             if (tree_out) {
@@ -579,6 +523,23 @@ int stwo_rocm_prove_xor(stwo_rocm_ctx* ctx, const stwo_rocm_xor_params* params, 
 }
 
 // ---------------------------------------------------------------- LogUp / GKR
+int stwo_rocm_prove_logup_xor(stwo_rocm_ctx* ctx, const stwo_rocm_logup_params* params, uint8_t* proof_out,
+                              uint64_t cap, uint64_t* proof_len) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(params && proof_len, "null argument");
+        ProveParams p{};
+        p.pow_bits = params->pow_bits;
+        p.log_blowup_factor = params->log_blowup_factor;
+        p.log_last_layer_degree_bound = params->log_last_layer_degree_bound;
+        p.n_queries = params->n_queries;
+        std::string proof;
+        prove_logup_xor(c, params->log_n, params->seed, p, proof);
+        *proof_len = proof.size();
+        SR_CHECK(proof_out && cap >= proof.size(), "proof buffer too small");
+        memcpy(proof_out, proof.data(), proof.size());
+    });
+}
+
 int stwo_rocm_xor_trace(stwo_rocm_ctx* ctx, uint32_t log_size, uint64_t seed, uint32_t* d_a, uint32_t* d_b,
                         uint32_t* d_c, uint32_t* d_mult) {
     return guarded(ctx, [&](Ctx& c) {

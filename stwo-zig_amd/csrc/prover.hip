@@ -206,6 +206,71 @@ struct JsonOut {
     }
 };
 
+// ---------------------------------------------------------------- quotient batches
+QuotientPlan plan_quotients(Ctx& c, uint32_t L, const std::vector<uint32_t>& col_logs,
+                            const std::vector<std::vector<ColumnSample>>& samples, qm31 alpha) {
+    SR_CHECK(col_logs.size() == samples.size(), "quotients: one sample list per column");
+    std::vector<SamplePoint> bpts;
+    std::vector<std::vector<std::pair<uint32_t, qm31>>> bent;
+    std::vector<qm31> sa, sb;
+    qm31 rp = qone();
+    const uint32_t lift_gen = subgroup_gen(L);
+    auto add_sample = [&](uint32_t col, SamplePoint p, qm31 v) {
+        size_t bi = 0;
+        for (; bi < bpts.size(); bi++)
+            if (qeq(bpts[bi].x, p.x) && qeq(bpts[bi].y, p.y)) break;
+        if (bi == bpts.size()) {
+            bpts.push_back(p);
+            bent.emplace_back();
+            sa.push_back(qzero());
+            sb.push_back(qzero());
+        }
+        // complexConjugateLineCoeffs (core/constraints.zig:87-97)
+        SR_CHECK(!qeq(p.y, qconj(p.y)), "degenerate sample point (y in the base field)");
+        qm31 a = qsub(qconj(v), v);
+        qm31 cc = qsub(qconj(p.y), p.y);
+        qm31 b = qsub(qmul(v, cc), qmul(a, p.y));
+        bent[bi].push_back({col, qmul(rp, cc)});
+        sa[bi] = qadd(sa[bi], qmul(rp, a));
+        sb[bi] = qadd(sb[bi], qmul(rp, b));
+        rp = qmul(rp, alpha);
+    };
+    for (uint32_t col = 0; col < col_logs.size(); col++) {
+        const auto& ss = samples[col];
+        SR_CHECK(ss.size() <= 2, "quotients: at most two samples per column");
+        if (ss.size() == 2) {
+            // Periodicity sample (core/pcs/quotients/samples.zig:270-285): the second point plus
+            // the lifting domain's step doubled log-size times.
+            const pt g = idx_to_point(c.gen, (uint32_t)(((uint64_t)lift_gen << col_logs[col]) & ORDER_MASK));
+            const SamplePoint p = ss[1].p;
+            const SamplePoint q = {qsub(qmul(p.x, qfromm(g.x)), qmul(p.y, qfromm(g.y))),
+                                   qadd(qmul(p.x, qfromm(g.y)), qmul(p.y, qfromm(g.x)))};
+            add_sample(col, q, ss[1].v);
+        }
+        for (const auto& smp : ss) add_sample(col, smp.p, smp.v);
+    }
+    QuotientPlan out;
+    for (size_t bi = 0; bi < bpts.size(); bi++) {
+        QuotientBatchDev bd;
+        cm31 prx = bpts[bi].x.c0, pry = bpts[bi].y.c0, pix = bpts[bi].x.c1, piy = bpts[bi].y.c1;
+        bd.det = csub(cmul(prx, piy), cmul(pry, pix));
+        bd.pix = pix;
+        bd.piy = piy;
+        bd.sum_a = sa[bi];
+        bd.sum_b = sb[bi];
+        bd.first = (uint32_t)out.entry_cols.size();
+        bd.count = (uint32_t)bent[bi].size();
+        for (auto& e : bent[bi]) {
+            out.entry_cols.push_back(e.first);
+            uint32_t w4[4];
+            qstore(e.second, w4);
+            out.entry_coeffs.insert(out.entry_coeffs.end(), w4, w4 + 4);
+        }
+        out.batches.push_back(bd);
+    }
+    return out;
+}
+
 // ---------------------------------------------------------------- prove
 // LDE of coefficient columns (pointer list) by `blowup` and Merkle commit
 // (commit_polys.zig:10-67; preparation.zig:113-210).
@@ -310,6 +375,255 @@ void fri_fold_commit_down(Ctx& c, TreeOwner& owner, const uint32_t* d_top, uint3
     } else {
         launch_fold_line(c, line_at(last_log + 1), last_log + 1, qzero(), d_last, nullptr, d_ch);
     }
+}
+
+void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std::string& proof,
+                  const std::function<void()>& mark) {
+    auto bitrev_vec = [](std::vector<qm31>& v) {
+        uint32_t lg = 0;
+        while ((1ull << lg) < v.size()) lg++;
+        for (uint64_t i = 0; i < v.size(); i++) {
+            uint64_t j = bitrev32((uint32_t)i, lg);
+            if (j > i) std::swap(v[i], v[j]);
+        }
+    };
+    const uint32_t L = t.L, llog = t.last_log, B = prm.log_blowup_factor;
+    std::vector<qm31>& lv = t.last_values;
+    const uint64_t lsz = lv.size();
+    SR_CHECK(lsz == (1ull << llog), "last layer size");
+    bitrev_vec(lv);
+    {
+        // line IFFT over LineDomain(halfOdds(llog)) and its doublings
+        uint32_t init = half_odds_initial(llog), step = subgroup_gen(llog), clog = llog;
+        while (clog > 0) {
+            uint64_t csz = 1ull << clog, half = csz / 2;
+            for (uint64_t s0 = 0; s0 < lsz; s0 += csz) {
+                pt cur = idx_to_point(c.gen, init), stp = idx_to_point(c.gen, step);
+                for (uint64_t i = 0; i < half; i++) {
+                    uint32_t ix = minv(cur.x);
+                    qm31 v0 = lv[s0 + i], v1 = lv[s0 + half + i];
+                    lv[s0 + i] = qadd(v0, v1);
+                    lv[s0 + half + i] = qmulm(qsub(v0, v1), ix);
+                    cur = padd(cur, stp);
+                }
+            }
+            init = (init * 2) & ORDER_MASK;
+            step = (step * 2) & ORDER_MASK;
+            clog--;
+        }
+        uint32_t li = minv((uint32_t)(lsz % P));
+        for (auto& v : lv) v = qmulm(v, li);
+    }
+    bitrev_vec(lv);  // ordered coefficients
+    const uint64_t bound = 1ull << prm.log_last_layer_degree_bound;
+    for (uint64_t i = bound; i < lsz; i++)
+        SR_CHECK(qiszero(lv[i]), "FRI last layer exceeds the degree bound (constraints not satisfied)");
+    std::vector<qm31> lastp(lv.begin(), lv.begin() + bound);
+    bitrev_vec(lastp);  // LinePoly.fromOrderedCoefficients
+    ch.mix_felts(lastp.data(), lastp.size());
+    mark();  // FRI commit
+
+    // Proof of work (scheme.zig:733-735; lowest nonce, channel/blake2s.zig:163-167).
+    uint64_t nonce = 0;
+    if (prm.pow_bits > 0) {
+        uint8_t prefix[32];
+        ch.pow_prefix(prm.pow_bits, prefix);
+        // chunks are searched in order and each returns its lowest nonce, so the size only sets
+        // how much is ground past the first solution: ~16x the expected work, 2^12..2^26
+        const uint32_t cl = std::min(26u, std::max(12u, prm.pow_bits + 4));
+        const uint64_t chunk = 1ull << cl;
+        for (uint64_t base = 0;; base += chunk) {
+            uint64_t r = launch_pow_grind(c, prefix, prm.pow_bits, base, chunk);
+            if (r != UINT64_MAX) {
+                nonce = r;
+                break;
+            }
+            SR_CHECK(base < (1ull << 50), "pow: no nonce found");
+        }
+    }
+    ch.mix_u64(nonce);
+    mark();  // pow
+
+    // Queries (core/queries.zig:8-60).
+    std::vector<uint64_t> qs;
+    {
+        while (qs.size() < prm.n_queries) {
+            uint32_t w[8];
+            ch.draw_u32s(w);
+            for (int i = 0; i < 8 && qs.size() < prm.n_queries; i++) qs.push_back(w[i] & ((1u << L) - 1));
+        }
+        std::sort(qs.begin(), qs.end());
+        qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
+    }
+    const size_t nq = qs.size();
+
+    // Decommitment plan: everything read from HBM in one gather.
+    Gather g;
+    // queried values (scheme_decommit.zig; decommit.zig:77-89) and witnesses of every tree
+    // with columns.
+    const size_t NT = t.trees.size();
+    std::vector<size_t> qv_at(NT, 0);
+    std::vector<std::vector<std::pair<uint32_t, uint64_t>>> tw_pos(NT);
+    std::vector<std::vector<size_t>> tw_at(NT);
+    for (size_t tr = 0; tr < NT; tr++) {
+        const Committed& T = *t.trees[tr];
+        if (T.lde.empty()) continue;
+        qv_at[tr] = g.addr.size();
+        for (size_t col = 0; col < T.lde.size(); col++)
+            for (size_t q = 0; q < nq; q++) {
+                uint32_t sh = T.tree->max_log - T.lde_logs[col] + 1;
+                uint64_t idx = ((qs[q] >> sh) << 1) + (qs[q] & 1);
+                g.add_words(T.lde[col] + idx, 1);
+            }
+        merkle_witness_positions(T.tree->max_log, qs, tw_pos[tr]);
+        for (auto& lp : tw_pos[tr]) tw_at[tr].push_back(g.add_hash(*T.tree, lp.first, lp.second));
+    }
+    // FRI layers (fri.zig:626-740): subsets {2k, 2k+1}, witness = non-queried partner.
+    struct LayerPlan {
+        std::vector<size_t> wit_at;
+        std::vector<size_t> hw_at;
+    };
+    std::vector<LayerPlan> lplans;
+    auto plan_layer = [&](const DeviceTree& t, const std::vector<uint64_t>& q,
+                          std::function<size_t(uint64_t)> add_val) {
+        LayerPlan lp;
+        std::vector<uint64_t> dpos;
+        size_t i = 0;
+        while (i < q.size()) {
+            uint64_t key = q[i] >> 1;
+            size_t j = i + 1;
+            while (j < q.size() && (q[j] >> 1) == key) j++;
+            size_t at = i;
+            for (uint64_t p = key << 1; p < (key << 1) + 2; p++) {
+                dpos.push_back(p);
+                if (at < j && q[at] == p) at++;
+                else lp.wit_at.push_back(add_val(p));
+            }
+            i = j;
+        }
+        std::vector<std::pair<uint32_t, uint64_t>> hw;
+        merkle_witness_positions(t.max_log, dpos, hw);
+        for (auto& h : hw) lp.hw_at.push_back(g.add_hash(t, h.first, h.second));
+        lplans.push_back(lp);
+    };
+    plan_layer(*t.ftree0, qs, [&](uint64_t p) {
+        size_t at = g.addr.size();
+        for (int k = 0; k < 4; k++) g.add_words(t.qc[k] + p, 1);
+        return at;
+    });
+    std::vector<uint64_t> lq;
+    for (uint64_t q : qs)
+        if (lq.empty() || lq.back() != (q >> 1)) lq.push_back(q >> 1);
+    for (size_t k = 0; k < t.ftrees.size(); k++) {
+        const uint32_t* base = t.lines[k];
+        plan_layer(*t.ftrees[k], lq, [&](uint64_t p) { return g.add_words(base + 4 * p, 4); });
+        std::vector<uint64_t> nx;
+        for (uint64_t q : lq)
+            if (nx.empty() || nx.back() != (q >> 1)) nx.push_back(q >> 1);
+        lq.swap(nx);
+    }
+    std::vector<uint32_t> gw(g.addr.size());
+    if (!g.addr.empty()) {
+        // the gather reads its address list from pinned host memory and writes the words
+        // there too: no copy operations on the stream
+        uint32_t* h = (uint32_t*)c.pinned(g.addr.size() * 4);
+        const uint64_t* h_addr = (const uint64_t*)c.stage(g.addr.data(), g.addr.size() * 8);
+        launch_gather(c, h_addr, g.addr.size(), h);
+        c.sync();
+        memcpy(gw.data(), h, g.addr.size() * 4);
+    }
+    mark();  // decommit gather
+
+    // Proof wire (interop/proof_wire.zig:19-76).
+    JsonOut j;
+    j.raw("{\"config\":{\"pow_bits\":");
+    j.u64(prm.pow_bits);
+    j.raw(",\"fri_config\":{\"log_blowup_factor\":");
+    j.u64(B);
+    j.raw(",\"log_last_layer_degree_bound\":");
+    j.u64(prm.log_last_layer_degree_bound);
+    j.raw(",\"n_queries\":");
+    j.u64(prm.n_queries);
+    j.raw(",\"fold_step\":1},\"lifting_log_size\":null},\"commitments\":[");
+    for (size_t tr = 0; tr < NT; tr++) {
+        if (tr) j.raw(",");
+        j.hash(t.trees[tr]->tree->root);
+    }
+    j.raw("],\"sampled_values\":[");
+    {
+        size_t at = 0, col_at = 0;
+        for (size_t tr = 0; tr < NT; tr++) {
+            j.raw(tr ? ",[" : "[");
+            for (size_t col = 0; col < t.trees[tr]->lde.size(); col++, col_at++) {
+                if (col) j.raw(",");
+                j.raw("[");
+                for (uint32_t k = 0; k < t.n_samples[col_at]; k++) {
+                    if (k) j.raw(",");
+                    j.qm(&t.sv[4 * at++]);
+                }
+                j.raw("]");
+            }
+            j.raw("]");
+        }
+    }
+    j.raw("],\"decommitments\":[");
+    for (size_t tr = 0; tr < NT; tr++) {
+        j.raw(tr ? ",{\"hash_witness\":[" : "{\"hash_witness\":[");
+        for (size_t k = 0; k < tw_at[tr].size(); k++) {
+            if (k) j.raw(",");
+            j.hash((const uint8_t*)&gw[tw_at[tr][k]]);
+        }
+        j.raw("]}");
+    }
+    j.raw("],\"queried_values\":[");
+    for (size_t tr = 0; tr < NT; tr++) {
+        j.raw(tr ? ",[" : "[");
+        size_t at = qv_at[tr];
+        for (size_t col = 0; col < t.trees[tr]->lde.size(); col++) {
+            if (col) j.raw(",");
+            j.raw("[");
+            for (size_t q = 0; q < nq; q++) {
+                if (q) j.raw(",");
+                j.u64(gw[at++]);
+            }
+            j.raw("]");
+        }
+        j.raw("]");
+    }
+    j.raw("],\"proof_of_work\":");
+    j.u64(nonce);
+    j.raw(",\"fri_proof\":{\"first_layer\":");
+    auto emit_layer = [&](const LayerPlan& lp, const DeviceTree& lt) {
+        j.raw("{\"fri_witness\":[");
+        for (size_t k = 0; k < lp.wit_at.size(); k++) {
+            if (k) j.raw(",");
+            j.qm(&gw[lp.wit_at[k]]);
+        }
+        j.raw("],\"decommitment\":{\"hash_witness\":[");
+        for (size_t k = 0; k < lp.hw_at.size(); k++) {
+            if (k) j.raw(",");
+            j.hash((const uint8_t*)&gw[lp.hw_at[k]]);
+        }
+        j.raw("]},\"commitment\":");
+        j.hash(lt.root);
+        j.raw("}");
+    };
+    emit_layer(lplans[0], *t.ftree0);
+    j.raw(",\"inner_layers\":[");
+    for (size_t k = 0; k < t.ftrees.size(); k++) {
+        if (k) j.raw(",");
+        emit_layer(lplans[1 + k], *t.ftrees[k]);
+    }
+    j.raw("],\"last_layer_poly\":[");
+    for (size_t k = 0; k < lastp.size(); k++) {
+        if (k) j.raw(",");
+        uint32_t w4[4];
+        qstore(lastp[k], w4);
+        j.qm(w4);
+    }
+    j.raw("]}}");
+    proof = j.finish();
+    mark();  // encode
 }
 
 void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trace_in, std::string& proof,
@@ -642,243 +956,20 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         const uint32_t* w = (const uint32_t*)(h + at_line);
         for (uint64_t i = 0; i < lsz; i++) lv[i] = qload(w + 4 * i);
     }
-    auto bitrev_vec = [](std::vector<qm31>& v) {
-        uint32_t lg = 0;
-        while ((1ull << lg) < v.size()) lg++;
-        for (uint64_t i = 0; i < v.size(); i++) {
-            uint64_t j = bitrev32((uint32_t)i, lg);
-            if (j > i) std::swap(v[i], v[j]);
-        }
-    };
-    bitrev_vec(lv);
     {
-        // line IFFT over LineDomain(halfOdds(llog)) and its doublings
-        uint32_t init = half_odds_initial(llog), step = subgroup_gen(llog), clog = llog;
-        while (clog > 0) {
-            uint64_t csz = 1ull << clog, half = csz / 2;
-            for (uint64_t s0 = 0; s0 < lsz; s0 += csz) {
-                pt cur = idx_to_point(c.gen, init), stp = idx_to_point(c.gen, step);
-                for (uint64_t i = 0; i < half; i++) {
-                    uint32_t ix = minv(cur.x);
-                    qm31 v0 = lv[s0 + i], v1 = lv[s0 + half + i];
-                    lv[s0 + i] = qadd(v0, v1);
-                    lv[s0 + half + i] = qmulm(qsub(v0, v1), ix);
-                    cur = padd(cur, stp);
-                }
-            }
-            init = (init * 2) & ORDER_MASK;
-            step = (step * 2) & ORDER_MASK;
-            clog--;
-        }
-        uint32_t li = minv((uint32_t)(lsz % P));
-        for (auto& v : lv) v = qmulm(v, li);
+        ProofTail tail;
+        tail.L = L;
+        tail.last_log = llog;
+        tail.trees = {&t0, &t1, &t2};
+        tail.n_samples.assign(NS, 1);
+        tail.sv = std::move(sv);
+        tail.ftree0 = ftree0;
+        for (int k = 0; k < 4; k++) tail.qc[k] = qc[k];
+        tail.ftrees = ftrees;
+        for (auto& ln : lines) tail.lines.push_back(ln.as<uint32_t>());
+        tail.last_values = std::move(lv);
+        finish_proof(c, ch, prm, tail, proof, mark);
     }
-    bitrev_vec(lv);  // ordered coefficients
-    const uint64_t bound = 1ull << prm.log_last_layer_degree_bound;
-    for (uint64_t i = bound; i < lsz; i++)
-        SR_CHECK(qiszero(lv[i]), "FRI last layer exceeds the degree bound (constraints not satisfied)");
-    std::vector<qm31> lastp(lv.begin(), lv.begin() + bound);
-    bitrev_vec(lastp);  // LinePoly.fromOrderedCoefficients
-    ch.mix_felts(lastp.data(), lastp.size());
-    mark();  // FRI commit
-
-    // Proof of work (scheme.zig:733-735; lowest nonce, channel/blake2s.zig:163-167).
-    uint64_t nonce = 0;
-    if (prm.pow_bits > 0) {
-        uint8_t prefix[32];
-        ch.pow_prefix(prm.pow_bits, prefix);
-        // chunks are searched in order and each returns its lowest nonce, so the size only sets
-        // how much is ground past the first solution: ~16x the expected work, 2^12..2^26
-        const uint32_t cl = std::min(26u, std::max(12u, prm.pow_bits + 4));
-        const uint64_t chunk = 1ull << cl;
-        for (uint64_t base = 0;; base += chunk) {
-            uint64_t r = launch_pow_grind(c, prefix, prm.pow_bits, base, chunk);
-            if (r != UINT64_MAX) {
-                nonce = r;
-                break;
-            }
-            SR_CHECK(base < (1ull << 50), "pow: no nonce found");
-        }
-    }
-    ch.mix_u64(nonce);
-    mark();  // pow
-
-    // Queries (core/queries.zig:8-60).
-    std::vector<uint64_t> qs;
-    {
-        while (qs.size() < prm.n_queries) {
-            uint32_t w[8];
-            ch.draw_u32s(w);
-            for (int i = 0; i < 8 && qs.size() < prm.n_queries; i++) qs.push_back(w[i] & ((1u << L) - 1));
-        }
-        std::sort(qs.begin(), qs.end());
-        qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
-    }
-    const size_t nq = qs.size();
-
-    // Decommitment plan: everything read from HBM in one gather.
-    Gather g;
-    // queried values (scheme_decommit.zig; decommit.zig:77-89) and witnesses of every tree
-    // with columns.
-    size_t qv_at[3] = {0, 0, 0};
-    std::vector<std::pair<uint32_t, uint64_t>> tw_pos[3];
-    std::vector<size_t> tw_at[3];
-    for (int tr = 0; tr < 3; tr++) {
-        const Committed& T = *trees[tr];
-        if (T.lde.empty()) continue;
-        qv_at[tr] = g.addr.size();
-        for (size_t col = 0; col < T.lde.size(); col++)
-            for (size_t q = 0; q < nq; q++) {
-                uint32_t sh = T.tree->max_log - T.lde_logs[col] + 1;
-                uint64_t idx = ((qs[q] >> sh) << 1) + (qs[q] & 1);
-                g.add_words(T.lde[col] + idx, 1);
-            }
-        merkle_witness_positions(T.tree->max_log, qs, tw_pos[tr]);
-        for (auto& lp : tw_pos[tr]) tw_at[tr].push_back(g.add_hash(*T.tree, lp.first, lp.second));
-    }
-    // FRI layers (fri.zig:626-740): subsets {2k, 2k+1}, witness = non-queried partner.
-    struct LayerPlan {
-        std::vector<size_t> wit_at;
-        std::vector<size_t> hw_at;
-    };
-    std::vector<LayerPlan> lplans;
-    auto plan_layer = [&](const DeviceTree& t, const std::vector<uint64_t>& q,
-                          std::function<size_t(uint64_t)> add_val) {
-        LayerPlan lp;
-        std::vector<uint64_t> dpos;
-        size_t i = 0;
-        while (i < q.size()) {
-            uint64_t key = q[i] >> 1;
-            size_t j = i + 1;
-            while (j < q.size() && (q[j] >> 1) == key) j++;
-            size_t at = i;
-            for (uint64_t p = key << 1; p < (key << 1) + 2; p++) {
-                dpos.push_back(p);
-                if (at < j && q[at] == p) at++;
-                else lp.wit_at.push_back(add_val(p));
-            }
-            i = j;
-        }
-        std::vector<std::pair<uint32_t, uint64_t>> hw;
-        merkle_witness_positions(t.max_log, dpos, hw);
-        for (auto& h : hw) lp.hw_at.push_back(g.add_hash(t, h.first, h.second));
-        lplans.push_back(lp);
-    };
-    plan_layer(*ftree0, qs, [&](uint64_t p) {
-        size_t at = g.addr.size();
-        for (int k = 0; k < 4; k++) g.add_words(qc[k] + p, 1);
-        return at;
-    });
-    std::vector<uint64_t> lq;
-    for (uint64_t q : qs)
-        if (lq.empty() || lq.back() != (q >> 1)) lq.push_back(q >> 1);
-    for (size_t k = 0; k < ftrees.size(); k++) {
-        const uint32_t* base = lines[k].as<uint32_t>();
-        plan_layer(*ftrees[k], lq, [&](uint64_t p) { return g.add_words(base + 4 * p, 4); });
-        std::vector<uint64_t> nx;
-        for (uint64_t q : lq)
-            if (nx.empty() || nx.back() != (q >> 1)) nx.push_back(q >> 1);
-        lq.swap(nx);
-    }
-    std::vector<uint32_t> gw(g.addr.size());
-    if (!g.addr.empty()) {
-        // the gather reads its address list from pinned host memory and writes the words
-        // there too: no copy operations on the stream
-        uint32_t* h = (uint32_t*)c.pinned(g.addr.size() * 4);
-        const uint64_t* h_addr = (const uint64_t*)c.stage(g.addr.data(), g.addr.size() * 8);
-        launch_gather(c, h_addr, g.addr.size(), h);
-        c.sync();
-        memcpy(gw.data(), h, g.addr.size() * 4);
-    }
-    mark();  // decommit gather
-
-    // Proof wire (interop/proof_wire.zig:19-76).
-    JsonOut j;
-    j.raw("{\"config\":{\"pow_bits\":");
-    j.u64(prm.pow_bits);
-    j.raw(",\"fri_config\":{\"log_blowup_factor\":");
-    j.u64(B);
-    j.raw(",\"log_last_layer_degree_bound\":");
-    j.u64(prm.log_last_layer_degree_bound);
-    j.raw(",\"n_queries\":");
-    j.u64(prm.n_queries);
-    j.raw(",\"fold_step\":1},\"lifting_log_size\":null},\"commitments\":[");
-    for (int tr = 0; tr < 3; tr++) {
-        if (tr) j.raw(",");
-        j.hash(trees[tr]->tree->root);
-    }
-    j.raw("],\"sampled_values\":[");
-    {
-        size_t at = 0;
-        for (int tr = 0; tr < 3; tr++) {
-            j.raw(tr ? ",[" : "[");
-            for (size_t col = 0; col < trees[tr]->lde.size(); col++) {
-                if (col) j.raw(",");
-                j.raw("[");
-                j.qm(&sv[4 * at++]);
-                j.raw("]");
-            }
-            j.raw("]");
-        }
-    }
-    j.raw("],\"decommitments\":[");
-    for (int tr = 0; tr < 3; tr++) {
-        j.raw(tr ? ",{\"hash_witness\":[" : "{\"hash_witness\":[");
-        for (size_t k = 0; k < tw_at[tr].size(); k++) {
-            if (k) j.raw(",");
-            j.hash((const uint8_t*)&gw[tw_at[tr][k]]);
-        }
-        j.raw("]}");
-    }
-    j.raw("],\"queried_values\":[");
-    for (int tr = 0; tr < 3; tr++) {
-        j.raw(tr ? ",[" : "[");
-        size_t at = qv_at[tr];
-        for (size_t col = 0; col < trees[tr]->lde.size(); col++) {
-            if (col) j.raw(",");
-            j.raw("[");
-            for (size_t q = 0; q < nq; q++) {
-                if (q) j.raw(",");
-                j.u64(gw[at++]);
-            }
-            j.raw("]");
-        }
-        j.raw("]");
-    }
-    j.raw("],\"proof_of_work\":");
-    j.u64(nonce);
-    j.raw(",\"fri_proof\":{\"first_layer\":");
-    auto emit_layer = [&](const LayerPlan& lp, const DeviceTree& t) {
-        j.raw("{\"fri_witness\":[");
-        for (size_t k = 0; k < lp.wit_at.size(); k++) {
-            if (k) j.raw(",");
-            j.qm(&gw[lp.wit_at[k]]);
-        }
-        j.raw("],\"decommitment\":{\"hash_witness\":[");
-        for (size_t k = 0; k < lp.hw_at.size(); k++) {
-            if (k) j.raw(",");
-            j.hash((const uint8_t*)&gw[lp.hw_at[k]]);
-        }
-        j.raw("]},\"commitment\":");
-        j.hash(t.root);
-        j.raw("}");
-    };
-    emit_layer(lplans[0], *ftree0);
-    j.raw(",\"inner_layers\":[");
-    for (size_t k = 0; k < ftrees.size(); k++) {
-        if (k) j.raw(",");
-        emit_layer(lplans[1 + k], *ftrees[k]);
-    }
-    j.raw("],\"last_layer_poly\":[");
-    for (size_t k = 0; k < lastp.size(); k++) {
-        if (k) j.raw(",");
-        uint32_t w4[4];
-        qstore(lastp[k], w4);
-        j.qm(w4);
-    }
-    j.raw("]}}");
-    proof = j.finish();
-    mark();  // encode
 
     if (stats) {
         SR_HIP(hipEventSynchronize(evs.back()));

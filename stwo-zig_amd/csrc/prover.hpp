@@ -105,6 +105,46 @@ void fri_fold_commit_down(Ctx& c, TreeOwner& owner, const uint32_t* d_top, uint3
                           DevChannel* d_ch, uint32_t* const* dst_lines, uint32_t* d_last, uint8_t* d_roots,
                           std::vector<DeviceTree*>& trees);
 
+// Quotient batches (core/pcs/quotients/samples.zig:245-312): one batch per distinct sample point
+// in first-seen order over columns in tree/column/sample order, a two-point column adding its
+// periodicity sample first; random coefficients 1, alpha, alpha^2, ... in that order.
+struct SamplePoint {
+    qm31 x, y;
+};
+struct ColumnSample {
+    SamplePoint p;
+    qm31 v;
+};
+struct QuotientPlan {
+    std::vector<uint32_t> entry_cols, entry_coeffs;  // per entry: column index, 4 coefficient words
+    std::vector<QuotientBatchDev> batches;
+};
+QuotientPlan plan_quotients(Ctx& c, uint32_t L, const std::vector<uint32_t>& col_logs,
+                            const std::vector<std::vector<ColumnSample>>& samples, qm31 alpha);
+
+// The end of a proof once the FRI layers are committed (fri.zig:594-621, scheme.zig:733-800,
+// interop/proof_wire.zig:19-76): the last layer polynomial from the last line (values as read
+// back, bit-reversed), PoW, the queries, one gather of every queried value and witness, and
+// the proof wire.  trees: the committed trees in commitment order (composition last);
+// n_samples: mask points per column (tree/column order); sv: the sampled values (4 words each,
+// tree/column/sample order); the first FRI layer's tree and coordinate columns, the inner
+// layers' trees and lines (log L-1 first).  `mark` is called after the FRI, PoW, decommitment
+// and encoding steps.
+struct ProofTail {
+    uint32_t L = 0, last_log = 0;
+    std::vector<const Committed*> trees;
+    std::vector<uint32_t> n_samples;
+    std::vector<uint32_t> sv;
+    DeviceTree* ftree0 = nullptr;
+    const uint32_t* qc[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<DeviceTree*> ftrees;
+    std::vector<const uint32_t*> lines;
+    std::vector<qm31> last_values;
+};
+struct ProveParams;
+void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std::string& proof,
+                  const std::function<void()>& mark);
+
 // Components served by the one-trace-tree proof core: the wide-Fibonacci AIR and the Poseidon
 // AIR (examples/poseidon.zig:397-504: 1264 columns, a constant composition evaluation).
 // The periodic XOR example AIR (examples/xor.zig:419-605) adds a preprocessed tree of two
@@ -140,6 +180,10 @@ struct ProveStats {
 // with ProveParams.trace_mode / trace_seed inside the proof).
 void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trace, std::string& proof,
                           ProveStats* stats);
+
+// The XOR-lookup AIR as a whole proof (logup_proof.hip): trace from (log_n, seed), PCS
+// configuration from prm (pow_bits, log_blowup_factor, log_last_layer_degree_bound, n_queries).
+void prove_logup_xor(Ctx& c, uint32_t log_n, uint64_t seed, const ProveParams& prm, std::string& proof);
 
 // GKR batch proof of the given input layers on `ch` (device-resident layers);
 // returns the proof word stream described in gkr.hip.

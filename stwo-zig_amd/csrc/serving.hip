@@ -13,7 +13,8 @@
 //
 // Job kinds: 0 = wide-Fibonacci proof (functional protocol, seeded uniform trace,
 // examples/wide_fibonacci.zig:181-343), 1 = XOR-lookup LogUp workload
-// (logup_pipeline.hip).
+// (logup_pipeline.hip), 2 = Poseidon AIR proof, 3 = periodic XOR AIR proof, 4 = XOR-lookup AIR
+// proof (logup_proof.hip).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -35,6 +36,7 @@ uint64_t serve_job_bytes(const ServeJob& j) {
     }
     const uint64_t N = 1ull << j.log_n;
     if (j.kind == 0) return 16ull * N * j.seq_len + 1024ull * N + (64ull << 20);
+    if (j.kind == 4) return 1536ull * N + (64ull << 20);  // 4 trees of 4..8 columns, pair fractions
     if (j.kind == 3) return 48ull * N + 1024ull * N + (64ull << 20);  // 3 columns
     return 1024ull * N + (64ull << 20);
 }
@@ -67,7 +69,8 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
     uint32_t max_log = 0;
     for (size_t i = 0; i < jobs.size(); i++) {
         const ServeJob& j = jobs[i];
-        SR_CHECK(j.kind <= 3, "serve: unknown job kind");
+        SR_CHECK(j.kind <= 4, "serve: unknown job kind");
+        SR_CHECK(j.kind != 4 || (j.log_n >= 16 && j.log_n <= 24), "serve: XOR-lookup AIR shape");
         SR_CHECK(j.kind != 3 || (j.log_n >= 2 && j.log_n <= 24 && j.seq_len <= j.log_n), "serve: XOR shape");
         SR_CHECK(j.kind != 2 || (j.log_n >= 4 && j.log_n <= 24), "serve: Poseidon shape");
         SR_CHECK(j.kind != 0 || (j.log_n >= 5 && j.log_n <= 24 && j.seq_len >= 3 && j.seq_len <= 256),
@@ -215,6 +218,16 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
                     prm.air = AIR_POSEIDON;
                     std::string proof;
                     prove_wide_fibonacci(c, prm, nullptr, proof, nullptr);
+                    b2_hash(proof.data(), proof.size(), o.digest);
+                    o.output_bytes = proof.size();
+                } else if (j.kind == 4) {
+                    ProveParams prm{};
+                    prm.pow_bits = 10;
+                    prm.log_blowup_factor = 1;
+                    prm.log_last_layer_degree_bound = 0;
+                    prm.n_queries = 3;
+                    std::string proof;
+                    prove_logup_xor(c, j.log_n, j.seed, prm, proof);
                     b2_hash(proof.data(), proof.size(), o.digest);
                     o.output_bytes = proof.size();
                 } else {

@@ -65,6 +65,18 @@ class XorParams(ctypes.Structure):
     ]
 
 
+class LogupParams(ctypes.Structure):
+    _fields_ = [
+        ("log_n", ctypes.c_uint32),
+        ("pow_bits", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+        ("log_blowup_factor", ctypes.c_uint32),
+        ("log_last_layer_degree_bound", ctypes.c_uint32),
+        ("n_queries", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
 class ProveStats(ctypes.Structure):
     _fields_ = [
         ("stage_gpu_ms", ctypes.c_double * 8),
@@ -144,6 +156,8 @@ _SIGNATURES = [
 ]
 
 _SIGNATURES += [
+    ("stwo_rocm_prove_logup_xor", ctypes.c_int, [_vp, ctypes.POINTER(LogupParams), ctypes.c_char_p, ctypes.c_uint64,
+                                                  ctypes.POINTER(ctypes.c_uint64)]),
     ("stwo_rocm_xor_trace", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint64, _vp, _vp, _vp, _vp]),
     ("stwo_rocm_logup_denominators", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_uint64, _u32p, _u32p, _vp]),
     ("stwo_rocm_logup_cumulative", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint64, _vp, _u32p]),
@@ -630,6 +644,21 @@ class Context:
                 continue
             self._check(rc)
             return buf.raw[: n.value], (st.as_dict() if st is not None else None)
+
+    def prove_logup_xor(self, log_n, seed, pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0,
+                        n_queries=3):
+        """XOR-lookup AIR proof (the LogUp benchmark's AIR as a whole proof): proof bytes."""
+        prm = LogupParams(log_n, pow_bits, seed, log_blowup_factor, log_last_layer_degree_bound, n_queries, 0)
+        cap = 1 << 20
+        n = ctypes.c_uint64()
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = self.lib.stwo_rocm_prove_logup_xor(self.h, ctypes.byref(prm), buf, cap, ctypes.byref(n))
+            if rc == 1 and n.value > cap:
+                cap = n.value
+                continue
+            self._check(rc)
+            return buf.raw[: n.value]
 
     def prove_xor(self, log_size, log_step, offset, pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0,
                   n_queries=3, stats=True):

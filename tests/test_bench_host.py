@@ -20,8 +20,8 @@ def test_serving_stream_is_seeded_and_rate_only_scales_arrivals():
     assert np.allclose([j[4] for j in a], [2 * j[4] for j in c])
     assert all(a[i][4] <= a[i + 1][4] for i in range(255))
     kinds = [j[0] for j in a]
-    assert set(kinds) == {0, 1, 2, 3}  # wide-Fib, LogUp, Poseidon, XOR AIR
-    for kind, lo, hi in ((0, 14, 20), (1, 16, 18), (2, 12, 16), (3, 12, 18)):
+    assert set(kinds) == {0, 2, 3, 4}  # wide-Fib, Poseidon, XOR AIR, XOR-lookup AIR
+    for kind, lo, hi in ((0, 14, 20), (4, 16, 18), (2, 12, 16), (3, 12, 18)):
         logs = [j[1] for j in a if j[0] == kind]
         assert min(logs) >= lo and max(logs) <= hi
     assert all(j[2] <= j[1] for j in a if j[0] == 3)  # XOR log_step <= log_size

@@ -9,8 +9,8 @@ B  wide-Fibonacci 2^20 x 100, seeded-uniform pairs (the target run), functional 
 C  batched primitive pipeline, 64 columns of log 14..22, blowup 4, 3 folds of a log-22 line:
    graph replay, eager launches and the autotuned shape all equal the oracle.
 E  the 256-request serving stream at its size ranges (wide-Fibonacci 14..20 x 16..100, XOR AIR
-   12..18, Poseidon 12..16, LogUp 16..18): every result is the same alone and in the mix, and a
-   seeded subset equals the oracle.
+   12..18, Poseidon 12..16, XOR-lookup AIR 16..18): every result is the same alone and in the
+   mix, and a seeded subset equals the oracle.
 """
 import hashlib
 import sys
@@ -86,6 +86,8 @@ def _ref_digest(job):
         return hashlib.blake2s(oracle.prove_poseidon(n)).hexdigest()
     if kind == 3:
         return hashlib.blake2s(oracle.prove_xor(n, w, seed)).hexdigest()
+    if kind == 4:
+        return hashlib.blake2s(oracle.prove_logup_xor(n, seed)).hexdigest()
     return hashlib.blake2s(oracle.logup_xor_pipeline(n, seed).astype(np.uint32).tobytes()).hexdigest()
 
 
@@ -93,7 +95,7 @@ def test_config_e_serving_stream_alone_mix_oracle(ctx):
     jobs = bench.serving_requests()
     assert len(jobs) == 256
     kinds = {j[0] for j in jobs}
-    assert kinds == {0, 1, 2, 3}
+    assert kinds == {0, 2, 3, 4}
     wf = [j for j in jobs if j[0] == 0]
     assert min(j[1] for j in wf) == 14 and max(j[1] for j in wf) == 20
     _say("config E: mix of 256 requests over 16 slots")
@@ -107,7 +109,7 @@ def test_config_e_serving_stream_alone_mix_oracle(ctx):
         assert alone[0]["digest"] == mix[i]["digest"], (i, job)
     # a seeded subset (every 16th request, one of each kind, and the largest proof) against the oracle
     subset = set(range(0, 256, 16))
-    for k in range(4):
+    for k in (0, 2, 3, 4):
         subset.add(next(i for i, j in enumerate(jobs) if j[0] == k))
     subset.add(max((i for i, j in enumerate(jobs) if j[0] == 0), key=lambda i: (jobs[i][1], jobs[i][2])))
     for i in sorted(subset):

@@ -47,6 +47,10 @@ typedef struct stwo_rocm_tree stwo_rocm_tree;
  * unlimited.  On failure *out is NULL and err (if non-NULL) receives the message. */
 int stwo_rocm_ctx_create(int device, uint64_t hbm_budget_bytes, stwo_rocm_ctx** out, char* err, size_t err_len);
 int stwo_rocm_ctx_destroy(stwo_rocm_ctx* ctx);
+/* Changes the context's device budget (0 = unlimited) for later allocations, like the
+ * reference's per-request budget_gib (src/tools/metal_session/protocol.zig:51); when the bytes
+ * live and cached exceed it the cached arena blocks are released. */
+int stwo_rocm_set_budget(stwo_rocm_ctx* ctx, uint64_t hbm_budget_bytes);
 const char* stwo_rocm_last_error(stwo_rocm_ctx* ctx);
 /* runtimePlatformIdentityAlloc (metal/commit_backend.zig): "name;arch;CUs;HBM bytes". */
 int stwo_rocm_ctx_identity(stwo_rocm_ctx* ctx, char* out, size_t out_len);
@@ -56,6 +60,17 @@ int stwo_rocm_synchronize(stwo_rocm_ctx* ctx);
  * between calls (the next serve call rebuilds them). */
 int stwo_rocm_memory_info(stwo_rocm_ctx* ctx, uint64_t* live, uint64_t* cached, uint64_t* peak);
 int stwo_rocm_release_cached(stwo_rocm_ctx* ctx);
+/* Session fields of the reference report (src/prover/native/report.zig:73-78: max circle log,
+ * retained twiddle bytes, tower build count) for the device context, with the arena counters.
+ * reset_peak != 0 restarts the peak at the bytes live now (after reading it). */
+typedef struct {
+    uint32_t tower_log; /* circle log of the resident twiddle tower; 0 before the first build */
+    uint32_t reserved;
+    uint64_t tower_bytes;  /* device bytes of the tower (twiddles, inverses and their doubled copies) */
+    uint64_t tower_builds; /* builds of the tower since the context was created */
+    uint64_t bytes_live, bytes_cached, peak_live;
+} stwo_rocm_runtime_info;
+int stwo_rocm_get_runtime_info(stwo_rocm_ctx* ctx, int reset_peak, stwo_rocm_runtime_info* out);
 
 /* ------------------------------------------------------------------ buffers */
 /* cuda_malloc_uint32_t / cuda_free_memory / copy_uint32_t_vec_* (cuda/ffi.zig:27-37,82). */
@@ -474,6 +489,11 @@ typedef struct {
 } stwo_rocm_serve_stats;
 int stwo_rocm_serve(stwo_rocm_ctx* ctx, const stwo_rocm_serve_job* jobs, uint32_t n_jobs, uint32_t n_slots,
                     uint64_t hbm_budget_bytes, stwo_rocm_serve_result* results, stwo_rocm_serve_stats* stats);
+/* The device-byte footprint estimate that stwo_rocm_serve admits a job against (kind, log_n and
+ * seq_len as above; the other fields are ignored).  It takes the place of the reference's
+ * host-byte accounting per committed cell (src/prover/native/resource_admission.zig:5-109) for a
+ * device backend; it needs no device and no context. */
+int stwo_rocm_job_device_bytes(const stwo_rocm_serve_job* job, uint64_t* bytes);
 
 /* ------------------------------------------------------------------ measurement */
 /* No counterpart in the reference: a measurement hook for callers that need per-kernel

@@ -76,6 +76,34 @@ int stwo_rocm_release_cached(stwo_rocm_ctx* ctx) {
     });
 }
 
+int stwo_rocm_set_budget(stwo_rocm_ctx* ctx, uint64_t hbm_budget_bytes) {
+    return guarded(ctx, [&](Ctx& c) {
+        c.hbm_budget = hbm_budget_bytes;
+        if (hbm_budget_bytes && c.bytes_live + c.bytes_cached > hbm_budget_bytes) c.trim();
+    });
+}
+
+int stwo_rocm_get_runtime_info(stwo_rocm_ctx* ctx, int reset_peak, stwo_rocm_runtime_info* out) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(out, "null output");
+        memset(out, 0, sizeof(*out));
+        out->tower_log = c.tower_log;
+        out->tower_bytes = c.tower_log ? 4ull * 4ull * (1ull << (c.tower_log - 1)) : 0;
+        out->tower_builds = c.tower_builds;
+        out->bytes_live = c.bytes_live;
+        out->bytes_cached = c.bytes_cached;
+        out->peak_live = c.peak_live;
+        if (reset_peak) c.peak_live = c.bytes_live;
+    });
+}
+
+int stwo_rocm_job_device_bytes(const stwo_rocm_serve_job* job, uint64_t* bytes) {
+    if (!job || !bytes) return STWO_ROCM_EINVAL;
+    if (job->kind > 4 || job->log_n < 1 || job->log_n > 30 || (job->kind == 2 && job->log_n < 4)) return STWO_ROCM_EINVAL;
+    *bytes = serve_job_bytes(ServeJob{job->kind, job->log_n, job->seq_len, 0, job->seed, 0.0});
+    return STWO_ROCM_OK;
+}
+
 int stwo_rocm_buffer_alloc(stwo_rocm_ctx* ctx, uint64_t bytes, void** dptr) {
     return guarded(ctx, [&](Ctx& c) {
         SR_CHECK(dptr, "null output");

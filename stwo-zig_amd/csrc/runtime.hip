@@ -186,6 +186,7 @@ void Ctx::ensure_tower(uint32_t log) {
     d_tw2 = tw2;
     d_itw2 = itw2;
     tower_log = log;
+    tower_builds++;
 }
 
 const uint32_t* Ctx::twiddle_layer(bool inverse, uint32_t m, uint32_t s) const {

@@ -81,6 +81,7 @@ struct Ctx {
 
     // Twiddle tower of CanonicCoset(tower_log) (twiddle_tower.zig:29-115).
     uint32_t tower_log = 0;
+    uint64_t tower_builds = 0;  // builds of the tower since ctx_init
     uint32_t* d_tw = nullptr;
     uint32_t* d_itw = nullptr;
     // The same towers doubled (2t as a plain u32 word, < 2^32): the register FFT multiplies by

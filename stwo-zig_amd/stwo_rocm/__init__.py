@@ -109,6 +109,9 @@ _SIGNATURES = [
     ("stwo_rocm_memory_info", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                               ctypes.POINTER(ctypes.c_uint64)]),
     ("stwo_rocm_release_cached", ctypes.c_int, [_vp]),
+    ("stwo_rocm_set_budget", ctypes.c_int, [_vp, ctypes.c_uint64]),
+    ("stwo_rocm_get_runtime_info", ctypes.c_int, [_vp, ctypes.c_int, _vp]),
+    ("stwo_rocm_job_device_bytes", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     ("stwo_rocm_buffer_alloc", ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     ("stwo_rocm_buffer_free", ctypes.c_int, [_vp, _vp]),
     ("stwo_rocm_buffer_upload", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint64]),
@@ -406,6 +409,16 @@ class Context:
 
     def release_cached(self):
         self._check(self.lib.stwo_rocm_release_cached(self.h))
+
+    def set_budget(self, hbm_budget_bytes):
+        """Device budget for later allocations (0 = unlimited); cached blocks beyond it are released."""
+        self._check(self.lib.stwo_rocm_set_budget(self.h, int(hbm_budget_bytes)))
+
+    def runtime_info(self, reset_peak=False):
+        """Twiddle tower (circle log, bytes, builds) and arena counters; reset_peak restarts the peak."""
+        info = RuntimeInfo()
+        self._check(self.lib.stwo_rocm_get_runtime_info(self.h, 1 if reset_peak else 0, ctypes.byref(info)))
+        return {k: getattr(info, k) for k, _ in RuntimeInfo._fields_ if k != "reserved"}
 
     # ---- per-kernel device-time profile (HIP events on the context stream)
     def profile_enable(self, enable=True):
@@ -811,6 +824,22 @@ class BatchedPipeline:
         if self.h:
             self.ctx._check(self.ctx.lib.stwo_rocm_batched_destroy(self.ctx.h, self.h))
             self.h = None
+
+
+class RuntimeInfo(ctypes.Structure):
+    _fields_ = [("tower_log", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("tower_bytes", ctypes.c_uint64),
+                ("tower_builds", ctypes.c_uint64), ("bytes_live", ctypes.c_uint64), ("bytes_cached", ctypes.c_uint64),
+                ("peak_live", ctypes.c_uint64)]
+
+
+def job_device_bytes(kind, log_n, seq_len=0):
+    """Device-byte estimate stwo_rocm_serve admits a job of this kind and shape against (no device needed)."""
+    lib = load_library()
+    out = ctypes.c_uint64()
+    rc = lib.stwo_rocm_job_device_bytes(ctypes.byref(ServeJob(kind, log_n, seq_len, 0, 0, 0.0)), ctypes.byref(out))
+    if rc != 0:
+        raise RocmError("job_device_bytes: invalid job shape")
+    return out.value
 
 
 class ServeJob(ctypes.Structure):

@@ -284,6 +284,26 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
                                    const uint32_t* d_trace, uint8_t* proof_out, uint64_t cap, uint64_t* proof_len,
                                    stwo_rocm_prove_stats* stats);
 
+/* Many wide-Fibonacci proofs in single launches.  No single reference counterpart: the
+ * reference proves one statement per ProverEngine.prove call (examples/wide_fibonacci.zig:181-343);
+ * this entry point proves n_proofs statements of one shape and PCS configuration (params) at
+ * once, every stage one launch over all of them, and each proof is byte-identical to what
+ * stwo_rocm_prove_wide_fibonacci returns for the same parameters.  Proof k uses trace_seeds[k]
+ * (trace_seeds NULL: params->trace_seed for all).  Limits: log_n_rows + log_blowup_factor <= 17,
+ * log_last_layer_degree_bound + log_blowup_factor <= 10, n_proofs 1..65535, device memory
+ * n_proofs * stats->slice_bytes.  The proofs are written back to back to proofs_out, proof k at
+ * [offsets[k], offsets[k + 1]) (offsets has n_proofs + 1 entries); when cap is too small the call
+ * returns STWO_ROCM_EINVAL with offsets[n_proofs] set to the bytes needed.  stats is optional. */
+typedef struct {
+    double device_ms;     /* first launch to the readback of every proof's FRI state */
+    double tail_ms;       /* last layers, proof of work, decommitment gather and wire encoding */
+    double total_ms;      /* the whole call */
+    uint64_t slice_bytes; /* device bytes per proof */
+} stwo_rocm_batch_stats;
+int stwo_rocm_prove_wide_fibonacci_batch(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params,
+                                         uint32_t n_proofs, const uint64_t* trace_seeds, uint8_t* proofs_out,
+                                         uint64_t cap, uint64_t* offsets, stwo_rocm_batch_stats* stats);
+
 /* Poseidon AIR statement and PCS configuration (examples/poseidon/input.zig:20-22;
  * core/pcs/mod.zig PcsConfig). */
 typedef struct {

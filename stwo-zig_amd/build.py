@@ -20,7 +20,7 @@ SESSION = os.path.join(LIB_DIR, "stwo_rocm_session")
 SOURCES = ["runtime.hip", "kernels_fft.hip", "kernels_merkle.hip", "kernels_fri.hip", "kernels_prove.hip",
            "kernels_misc.hip", "kernels_chan.hip", "kernels_logup.hip", "gkr.hip", "transcript.hip", "logup_pipeline.hip",
            "batched_pipeline.hip", "serving.hip",
-           "prover.hip", "hooks.hip", "logup_proof.hip",
+           "prover.hip", "hooks.hip", "logup_proof.hip", "batch_prover.hip",
            "capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("STWO_ROCM_ARCH", "gfx950")

@@ -509,6 +509,41 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
     });
 }
 
+int stwo_rocm_prove_wide_fibonacci_batch(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params,
+                                         uint32_t n_proofs, const uint64_t* trace_seeds, uint8_t* proofs_out,
+                                         uint64_t cap, uint64_t* offsets, stwo_rocm_batch_stats* stats) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(params && offsets, "null argument");
+        ProveParams p;
+        p.log_n_rows = params->log_n_rows;
+        p.sequence_len = params->sequence_len;
+        p.pow_bits = params->pow_bits;
+        p.log_blowup_factor = params->log_blowup_factor;
+        p.log_last_layer_degree_bound = params->log_last_layer_degree_bound;
+        p.n_queries = params->n_queries;
+        p.trace_mode = params->trace_mode;
+        p.trace_seed = params->trace_seed;
+        SR_CHECK(p.trace_mode <= 1, "trace_mode must be 0 or 1");
+        std::vector<std::string> proofs;
+        BatchStats st;
+        prove_wide_fibonacci_batch(c, p, n_proofs, trace_seeds, proofs, &st);
+        uint64_t total = 0;
+        for (uint32_t k = 0; k < n_proofs; k++) {
+            offsets[k] = total;
+            total += proofs[k].size();
+        }
+        offsets[n_proofs] = total;
+        if (stats) {
+            stats->device_ms = st.device_ms;
+            stats->tail_ms = st.tail_ms;
+            stats->total_ms = st.total_ms;
+            stats->slice_bytes = st.slice_bytes;
+        }
+        SR_CHECK(proofs_out && total <= cap, "proof buffer too small");
+        for (uint32_t k = 0; k < n_proofs; k++) memcpy(proofs_out + offsets[k], proofs[k].data(), proofs[k].size());
+    });
+}
+
 int stwo_rocm_prove_poseidon(stwo_rocm_ctx* ctx, const stwo_rocm_poseidon_params* params, uint8_t* proof_out,
                              uint64_t cap, uint64_t* proof_len, stwo_rocm_prove_stats* stats) {
     return guarded(ctx, [&](Ctx& c) {

@@ -52,9 +52,19 @@ __global__ __launch_bounds__(256) void fold_circle_kernel(const uint32_t* __rest
                                                           const uint32_t* __restrict__ inv_y, qm31 alpha, qm31 alpha2,
                                                           const DevChannel* __restrict__ dch, bool accumulate,
                                                           uint4* __restrict__ dst, uint8_t* __restrict__ leaves,
-                                                          B2State seed) {
+                                                          B2State seed, uint64_t bstride) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (1u << (log_src - 1))) return;
+    if (blockIdx.y) {  // proof blockIdx.y of a batch: its buffers bstride bytes further on
+        const uint64_t boff = (uint64_t)blockIdx.y * bstride;
+        c0 = (const uint32_t*)((const uint8_t*)c0 + boff);
+        c1 = (const uint32_t*)((const uint8_t*)c1 + boff);
+        c2 = (const uint32_t*)((const uint8_t*)c2 + boff);
+        c3 = (const uint32_t*)((const uint8_t*)c3 + boff);
+        if (dch) dch = (const DevChannel*)((const uint8_t*)dch + boff);
+        dst = (uint4*)((uint8_t*)dst + boff);
+        if (leaves) leaves += boff;
+    }
     if (dch) {
         alpha = qload(dch->alpha);
         alpha2 = qload(dch->alpha2);
@@ -213,7 +223,22 @@ __device__ __forceinline__ qm31 tr_chan_step(const QuadLane& L, uint32_t q, uint
 // reduces the 2^(k0-s) subtree roots to the root and, with `ch`, runs the channel step on
 // it.  The counters are left at zero.
 template <int SRC>
-__global__ __launch_bounds__(TL_THREADS) void tree_layer_kernel(FriLayerArgs a) {
+__global__ __launch_bounds__(TL_THREADS) void tree_layer_kernel(FriLayerArgs a0) {
+    FriLayerArgs a = a0;
+    if (blockIdx.y) {
+        // tree blockIdx.y of a batch: its line, tree, channel and root copy bstride bytes further
+        // on, its counters and root exchange TREE_CNT_WORDS words further on
+        const uint64_t boff = (uint64_t)blockIdx.y * a.bstride;
+        if (a.src_line) a.src_line = (const uint4*)((const uint8_t*)a.src_line + boff);
+        for (int k = 0; k < 4; k++)
+            if (a.coord[k]) a.coord[k] = (const uint32_t*)((const uint8_t*)a.coord[k] + boff);
+        if (a.dst_line) a.dst_line = (uint4*)((uint8_t*)a.dst_line + boff);
+        a.tree += boff;
+        if (a.ch) a.ch = (DevChannel*)((uint8_t*)a.ch + boff);
+        if (a.root_out) a.root_out = (uint32_t*)((uint8_t*)a.root_out + boff);
+        a.cnt += (uint64_t)blockIdx.y * TREE_CNT_WORDS;
+        a.xr += (uint64_t)blockIdx.y * TREE_CNT_WORDS;
+    }
     __shared__ __align__(16) uint32_t hA[1024 * 8];
     __shared__ __align__(16) uint32_t hB[512 * 8];
     __shared__ uint32_t last;
@@ -327,7 +352,18 @@ __global__ __launch_bounds__(TL_THREADS) void tree_layer_kernel(FriLayerArgs a) 
 // the parent levels (layers.zig:99-140) and the channel step on the root, separated by
 // workgroup barriers instead of kernel boundaries (prover/fri.zig:377-621 runs the same
 // sequence per layer).  The last fold (into last_log) stores the line only.
-__global__ __launch_bounds__(TAIL_THREADS) void fri_tail_kernel(FriTailArgs a) {
+__global__ __launch_bounds__(TAIL_THREADS) void fri_tail_kernel(FriTailArgs a0) {
+    FriTailArgs a = a0;
+    if (blockIdx.x) {
+        // proof blockIdx.x of a batch: its lines, trees, root copies and channel bstride bytes on
+        const uint64_t boff = (uint64_t)blockIdx.x * a.bstride;
+        for (uint32_t l = 0; l < FRI_TAIL_MAX_LOG + 2; l++) {
+            if (a.lines[l]) a.lines[l] = (uint4*)((uint8_t*)a.lines[l] + boff);
+            if (a.trees[l]) a.trees[l] += boff;
+            if (a.roots[l]) a.roots[l] = (uint32_t*)((uint8_t*)a.roots[l] + boff);
+        }
+        a.ch = (DevChannel*)((uint8_t*)a.ch + boff);
+    }
     __shared__ __align__(16) uint32_t hA[TAIL_THREADS * 8];
     __shared__ __align__(16) uint32_t hB[TAIL_THREADS / 2 * 8];
     __shared__ uint4 vs[TAIL_THREADS];  // current line
@@ -402,22 +438,29 @@ __global__ __launch_bounds__(TAIL_THREADS) void fri_tail_kernel(FriTailArgs a) {
 static uint32_t tree_chunk_log(uint32_t k0) { return k0 <= 8 ? k0 : std::min(10u, std::max(k0 / 2, k0 - 8)); }
 
 template <int SRC>
-static void launch_tree_layer(Ctx& c, FriLayerArgs& a, const std::string& cls, double bytes) {
+static void launch_tree_layer(Ctx& c, FriLayerArgs& a, const std::string& cls, double bytes, uint32_t batch,
+                              uint64_t bstride, uint32_t* d_cnt) {
     SR_CHECK(a.k0 >= 1 && a.k0 <= TREE_LAYER_MAX_LOG, "tree layer: level out of range");
+    SR_CHECK(batch >= 1 && batch <= 65535, "tree layer: batch size");
     a.s = tree_chunk_log(a.k0);
     const uint32_t nb = 1u << (a.k0 - a.s);
     SR_CHECK(nb <= TREE_MAX_BLOCKS && a.s >= 1 && a.s <= 10, "tree layer: chunking");
-    SR_CHECK(c.d_tree_cnt, "tree layer: context without completion counters");
-    a.cnt = c.d_tree_cnt;
-    a.xr = c.d_tree_cnt + TREE_XR_OFFSET;
+    // one tree: the context's counters; a batch: the caller's, one region per tree
+    SR_CHECK(batch == 1 || d_cnt, "tree layer: a batch needs its own completion counters");
+    uint32_t* cnt = batch > 1 ? d_cnt : c.d_tree_cnt;
+    SR_CHECK(cnt, "tree layer: context without completion counters");
+    a.cnt = cnt;
+    a.xr = cnt + TREE_XR_OFFSET;
     a.leaf_seed = c.leaf_seed;
     a.node_seed = c.node_seed;
-    KTimer kt(c, cls, bytes);
-    hipLaunchKernelGGL(tree_layer_kernel<SRC>, dim3(nb), dim3(TL_THREADS), 0, c.stream, a);
+    a.bstride = bstride;
+    KTimer kt(c, cls, batch * bytes);
+    hipLaunchKernelGGL(tree_layer_kernel<SRC>, dim3(nb, batch), dim3(TL_THREADS), 0, c.stream, a);
     SR_HIP(hipGetLastError());
 }
 
-void launch_tree_top(Ctx& c, DeviceTree& t, uint32_t k0, DevChannel* d_ch, uint8_t* d_root_out) {
+void launch_tree_top(Ctx& c, DeviceTree& t, uint32_t k0, DevChannel* d_ch, uint8_t* d_root_out, uint32_t batch,
+                     uint64_t bstride, uint32_t* d_cnt) {
     SR_CHECK(k0 <= t.max_log, "tree top: level above the leaves");
     FriLayerArgs a{};
     a.tree = t.layers;
@@ -425,14 +468,15 @@ void launch_tree_top(Ctx& c, DeviceTree& t, uint32_t k0, DevChannel* d_ch, uint8
     a.ch = d_ch;
     a.root_out = (uint32_t*)d_root_out;
     launch_tree_layer<FRI_SRC_HASHES>(c, a, "merkle_top L=" + std::to_string(k0) + (d_ch ? " +channel" : ""),
-                                      32.0 * (double)((2ull << k0) - 1));
+                                      32.0 * (double)((2ull << k0) - 1), batch, bstride, d_cnt);
 }
 
 void launch_fri_commit_line(Ctx& c, const uint32_t* d_src, uint32_t log_src, uint32_t* d_dst, DeviceTree& t,
-                            DevChannel* d_ch, uint8_t* d_root_out) {
+                            DevChannel* d_ch, uint8_t* d_root_out, uint32_t batch, uint64_t bstride, uint32_t* d_cnt) {
     const uint32_t l = log_src - 1;
     SR_CHECK(log_src >= 1 && t.max_log == l && d_ch, "fri commit: layer shape");
     if (l == 0 || l > TREE_LAYER_MAX_LOG) {
+        SR_CHECK(batch == 1, "fri commit: batches of layers above 2^16");
         launch_fold_line(c, d_src, log_src, qzero(), d_dst, t.layer(l), d_ch);
         launch_merkle_parents(c, t, d_ch, d_root_out);
         return;
@@ -447,14 +491,17 @@ void launch_fri_commit_line(Ctx& c, const uint32_t* d_src, uint32_t log_src, uin
     a.ch = d_ch;
     a.root_out = (uint32_t*)d_root_out;
     const double n = (double)(1ull << l);
-    launch_tree_layer<FRI_SRC_LINE>(c, a, "fri_layer", (32.0 + 4.0 + 16.0) * n + 32.0 * (2.0 * n - 1.0));
+    launch_tree_layer<FRI_SRC_LINE>(c, a, "fri_layer", (32.0 + 4.0 + 16.0) * n + 32.0 * (2.0 * n - 1.0), batch,
+                                    bstride, d_cnt);
 }
 
 void launch_fri_commit_circle(Ctx& c, const uint32_t* const* coords, uint32_t log_src, uint32_t* d_dst,
-                              DeviceTree& t, DevChannel* d_ch, uint8_t* d_root_out) {
+                              DeviceTree& t, DevChannel* d_ch, uint8_t* d_root_out, uint32_t batch, uint64_t bstride,
+                              uint32_t* d_cnt) {
     const uint32_t l = log_src - 1;
     SR_CHECK(log_src >= 1 && t.max_log == l && d_ch, "fri commit: layer shape");
     if (l == 0 || l > TREE_LAYER_MAX_LOG) {
+        SR_CHECK(batch == 1, "fri commit: batches of layers above 2^16");
         launch_fold_circle_into_line(c, coords, log_src, qzero(), d_dst, false, t.layer(l), d_ch);
         launch_merkle_parents(c, t, d_ch, d_root_out);
         return;
@@ -468,18 +515,20 @@ void launch_fri_commit_circle(Ctx& c, const uint32_t* const* coords, uint32_t lo
     a.ch = d_ch;
     a.root_out = (uint32_t*)d_root_out;
     const double n = (double)(1ull << l);
-    launch_tree_layer<FRI_SRC_CIRCLE>(c, a, "fri_layer_circle", (32.0 + 4.0 + 16.0) * n + 32.0 * (2.0 * n - 1.0));
+    launch_tree_layer<FRI_SRC_CIRCLE>(c, a, "fri_layer_circle", (32.0 + 4.0 + 16.0) * n + 32.0 * (2.0 * n - 1.0),
+                                      batch, bstride, d_cnt);
 }
 
-void launch_fri_tail(Ctx& c, FriTailArgs& a) {
+void launch_fri_tail(Ctx& c, FriTailArgs& a, uint32_t batch) {
     SR_CHECK(a.top_log <= FRI_TAIL_MAX_LOG + 1 && a.top_log > a.last_log, "fri_tail: layer range");
+    SR_CHECK(batch >= 1 && batch <= 65535, "fri_tail: batch size");
     c.ensure_tower(a.top_log + 1);
     // inverse twiddles of halfOdds(l) live at [2^(M-1) - 2^l, 2^(M-1) - 2^(l-1)) of the inverse tower
     a.itw_half = c.d_itw + (1u << (c.tower_log - 1));
     a.leaf_seed = c.leaf_seed;
     a.node_seed = c.node_seed;
     KTimer kt(c, "fri_tail", 0.0);
-    hipLaunchKernelGGL(fri_tail_kernel, dim3(1), dim3(TAIL_THREADS), 0, c.stream, a);
+    hipLaunchKernelGGL(fri_tail_kernel, dim3(batch), dim3(TAIL_THREADS), 0, c.stream, a);
     SR_HIP(hipGetLastError());
 }
 
@@ -497,14 +546,16 @@ void launch_fold_line(Ctx& c, const uint32_t* d_src, uint32_t log_src, qm31 alph
 }
 
 void launch_fold_circle_into_line(Ctx& c, const uint32_t* const* coords, uint32_t log_src, qm31 alpha,
-                                  uint32_t* d_dst, bool accumulate, uint8_t* d_leaf_out, const DevChannel* d_alpha) {
+                                  uint32_t* d_dst, bool accumulate, uint8_t* d_leaf_out, const DevChannel* d_alpha,
+                                  uint32_t batch, uint64_t bstride) {
     SR_CHECK(log_src >= 1, "fold_circle: source too short");
+    SR_CHECK(batch >= 1 && batch <= 65535, "fold_circle: batch size");
     const uint32_t* inv_y = c.inv_circle_y(log_src);
     uint32_t n = 1u << (log_src - 1);
-    KTimer kt(c, "fri_fold_circle", (32.0 + 4.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n);
-    hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, coords[0], coords[1],
+    KTimer kt(c, "fri_fold_circle", batch * (32.0 + 4.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n);
+    hipLaunchKernelGGL(fold_circle_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, c.stream, coords[0], coords[1],
                        coords[2], coords[3], log_src, inv_y, alpha, qsqr(alpha), d_alpha, accumulate,
-                       (uint4*)d_dst, d_leaf_out, c.leaf_seed);
+                       (uint4*)d_dst, d_leaf_out, c.leaf_seed, bstride);
     SR_HIP(hipGetLastError());
 }
 

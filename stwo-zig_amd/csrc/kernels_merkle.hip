@@ -37,11 +37,13 @@ __device__ __forceinline__ void store_hash(uint8_t* __restrict__ out, uint64_t i
     o[1] = make_uint4(h[4], h[5], h[6], h[7]);
 }
 
-__device__ __forceinline__ void leaf_block_load(const LeafColDev* __restrict__ cd, uint32_t p, uint32_t m[16]) {
+// boff: byte offset of the proof's columns in a batch (0 for one tree).
+__device__ __forceinline__ void leaf_block_load(const LeafColDev* __restrict__ cd, uint32_t p, uint32_t m[16],
+                                                uint64_t boff) {
 #pragma unroll
     for (int w = 0; w < 16; w++) {
         const LeafColDev d = cd[w];
-        m[w] = ((gcu32*)d.col)[((p >> d.shift) << 1) + (p & 1)] & d.mask;
+        m[w] = ((gcu32*)((const uint8_t*)d.col + boff))[((p >> d.shift) << 1) + (p & 1)] & d.mask;
     }
 }
 
@@ -53,10 +55,14 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_leaves_kernel(const LeafCol
                                                                    uint32_t blk0, uint32_t nblk, uint32_t n_cols,
                                                                    uint32_t level, const uint32_t* __restrict__ in,
                                                                    uint32_t in_shift, B2State seed,
-                                                                   uint8_t* __restrict__ out) {
+                                                                   uint8_t* __restrict__ out, uint64_t bstride) {
     const uint64_t p64 = (uint64_t)blockIdx.x * MK_THREADS + threadIdx.x;
     if (p64 >= (1ull << level)) return;
     const uint32_t p = (uint32_t)p64;
+    // tree blockIdx.y of a batch: its columns, states and hashes bstride bytes further on
+    const uint64_t boff = (uint64_t)blockIdx.y * bstride;
+    out += boff;
+    if (in) in = (const uint32_t*)((const uint8_t*)in + boff);
     uint32_t h[8];
     if (in) {
         const SR_GLOBAL u32x4* s4 = (const SR_GLOBAL u32x4*)((gcu32*)in + 8ull * (((p >> in_shift) << 1) + (p & 1)));
@@ -69,11 +75,11 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_leaves_kernel(const LeafCol
     const uint32_t nblocks_total = (n_cols + 15) >> 4;
     const uint32_t blk_end = blk0 + nblk;
     uint32_t cur[16];
-    leaf_block_load(cols + 16 * blk0, p, cur);
+    leaf_block_load(cols + 16 * blk0, p, cur, boff);
     for (uint32_t blk = blk0; blk < blk_end; blk++) {
         uint32_t nxt[16];
         const bool more = blk + 1 < blk_end;
-        if (more) leaf_block_load(cols + 16 * (blk + 1), p, nxt);
+        if (more) leaf_block_load(cols + 16 * (blk + 1), p, nxt, boff);
         const bool last = blk + 1 == nblocks_total;
         const uint32_t t = 64 + (last ? 4 * n_cols : 64 * (blk + 1));
         b2_compress(h, cur, t, last);
@@ -92,10 +98,13 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_leaves_quad_kernel(const Le
                                                                         uint32_t blk0, uint32_t nblk, uint32_t n_cols,
                                                                         uint32_t level, const uint32_t* __restrict__ in,
                                                                         uint32_t in_shift, B2State seed,
-                                                                        uint8_t* __restrict__ out) {
+                                                                        uint8_t* __restrict__ out, uint64_t bstride) {
     const uint64_t t64 = (uint64_t)blockIdx.x * MK_THREADS + threadIdx.x;
     if (t64 >= (4ull << level)) return;  // whole quads: the thread count is a multiple of 4
     const uint32_t p = (uint32_t)(t64 >> 2), q = (uint32_t)(t64 & 3);
+    const uint64_t boff = (uint64_t)blockIdx.y * bstride;
+    out += boff;
+    if (in) in = (const uint32_t*)((const uint8_t*)in + boff);
     const QuadLane L(q);
     uint32_t hlo, hhi;
     if (in) {
@@ -109,11 +118,11 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_leaves_quad_kernel(const Le
     const uint32_t nblocks_total = (n_cols + 15) >> 4;
     const uint32_t blk_end = blk0 + nblk;
     uint32_t cur[16];
-    leaf_block_load(cols + 16 * blk0, p, cur);
+    leaf_block_load(cols + 16 * blk0, p, cur, boff);
     for (uint32_t blk = blk0; blk < blk_end; blk++) {
         uint32_t nxt[16];
         const bool more = blk + 1 < blk_end;
-        if (more) leaf_block_load(cols + 16 * (blk + 1), p, nxt);
+        if (more) leaf_block_load(cols + 16 * (blk + 1), p, nxt, boff);
         const bool last = blk + 1 == nblocks_total;
         const uint32_t t = 64 + (last ? 4 * n_cols : 64 * (blk + 1));
         b2_compress_quad(L, hlo, hhi, cur, t, last);
@@ -163,9 +172,10 @@ __device__ __forceinline__ void node_hash(const B2State& seed, const uint32_t* l
 
 // Levels k-1 .. k-levels of a tree: block b reduces its 2^levels hashes of level k.
 __global__ __launch_bounds__(MP_THREADS) void merkle_parents_kernel(uint8_t* __restrict__ tree, uint32_t k,
-                                                                    uint32_t levels, B2State seed) {
+                                                                    uint32_t levels, B2State seed, uint64_t bstride) {
     __shared__ uint32_t sh[MP_THREADS * 8];
     const uint32_t t = threadIdx.x;
+    tree += (uint64_t)blockIdx.y * bstride;  // tree blockIdx.y of a batch
     const uint8_t* child = tree + (((uint64_t)1 << k) - 1) * 32;
     if (levels == 1) {
         uint64_t i = (uint64_t)blockIdx.x * MP_THREADS + t;
@@ -300,8 +310,9 @@ LeafPlan merkle_leaf_plan(const uint32_t* const* h_cols, const uint32_t* h_logs,
 }
 
 void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, const std::vector<LeafSeg>& segs, uint32_t n_cols,
-                              DeviceTree& t) {
+                              DeviceTree& t, uint32_t batch, uint64_t bstride) {
     const uint32_t L = t.max_log;
+    SR_CHECK(batch >= 1 && batch <= 65535, "merkle leaves: batch size");
     const size_t S = segs.size();
     SR_CHECK(S >= 1 && segs.back().level == L && segs.back().blk0 + segs.back().nblk == (n_cols + 15) / 16,
              "merkle leaves: segment plan does not match the tree");
@@ -316,20 +327,22 @@ void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, const std::vector
         // launches overwrite both afterwards
         uint8_t* out = final ? t.layer(L) : t.layer(((S - 1 - s) & 1) ? L - 1 : L - 2);
         const uint64_t n = 1ull << sg.level;
-        double bytes = 4.0 * (double)sg.col_words + 32.0 * (double)n + (prev ? 32.0 * (double)(1ull << prev_level) : 0.0);
+        double bytes = batch * (4.0 * (double)sg.col_words + 32.0 * (double)n +
+                                (prev ? 32.0 * (double)(1ull << prev_level) : 0.0));
         KTimer kt(c,
                   final ? "merkle_leaves L=" + std::to_string(L) + " cols=" + std::to_string(n_cols) +
                               (S > 1 ? " blocks=" + std::to_string(sg.nblk) : "")
                         : "merkle_leaves_prefix L=" + std::to_string(sg.level) + " blocks=" + std::to_string(sg.nblk),
                   bytes);
         if (n <= (1u << 14) && sg.nblk >= 8) {
-            hipLaunchKernelGGL(merkle_leaves_quad_kernel, dim3((uint32_t)((4 * n + MK_THREADS - 1) / MK_THREADS)),
-                               dim3(MK_THREADS), 0, c.stream, d_tab, sg.blk0, sg.nblk, n_cols, sg.level, prev,
-                               prev ? sg.level - prev_level + 1 : 0u, c.leaf_seed, out);
+            hipLaunchKernelGGL(merkle_leaves_quad_kernel,
+                               dim3((uint32_t)((4 * n + MK_THREADS - 1) / MK_THREADS), batch), dim3(MK_THREADS), 0,
+                               c.stream, d_tab, sg.blk0, sg.nblk, n_cols, sg.level, prev,
+                               prev ? sg.level - prev_level + 1 : 0u, c.leaf_seed, out, bstride);
         } else {
-            hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS)),
+            hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS), batch),
                                dim3(MK_THREADS), 0, c.stream, d_tab, sg.blk0, sg.nblk, n_cols, sg.level, prev,
-                               prev ? sg.level - prev_level + 1 : 0u, c.leaf_seed, out);
+                               prev ? sg.level - prev_level + 1 : 0u, c.leaf_seed, out, bstride);
         }
         SR_HIP(hipGetLastError());
         prev = (const uint32_t*)out;
@@ -355,9 +368,12 @@ void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size
 // Wide parent levels (throughput-bound) with merkle_parents_kernel while more than
 // TREE_LAYER_MAX_LOG levels remain, then the top levels and the channel step in one
 // tree-layer launch (latency-bound: one quad of lanes per compression on the narrow levels).
-void launch_merkle_parents(Ctx& c, DeviceTree& t, DevChannel* d_ch, uint8_t* d_root_out) {
+void launch_merkle_parents(Ctx& c, DeviceTree& t, DevChannel* d_ch, uint8_t* d_root_out, uint32_t batch,
+                           uint64_t bstride, uint32_t* d_cnt) {
     uint32_t k = t.max_log;
+    SR_CHECK(batch >= 1 && batch <= 65535, "merkle parents: batch size");
     if (k == 0) {
+        SR_CHECK(batch == 1, "merkle parents: a batch of one-leaf trees");
         if (d_ch) launch_chan_mix_root_draw(c, d_ch, t.layers, d_root_out);
         else if (d_root_out) SR_HIP(hipMemcpyAsync(d_root_out, t.layers, 32, hipMemcpyDeviceToDevice, c.stream));
         return;
@@ -370,13 +386,13 @@ void launch_merkle_parents(Ctx& c, DeviceTree& t, DevChannel* d_ch, uint8_t* d_r
         // levels >= 2: 2^(levels-2) threads carry work; at least one wave per block
         const uint32_t threads = levels == 1 ? MP_THREADS : std::max(64u, 1u << (levels - 2));
         KTimer kt(c, "merkle_parents L=" + std::to_string(k) + " levels=" + std::to_string(levels),
-                  32.0 * (double)((1ull << k) + (1ull << k) - (1ull << (k - levels))));
-        hipLaunchKernelGGL(merkle_parents_kernel, dim3(blocks), dim3(threads), 0, c.stream, t.layers, k, levels,
-                           c.node_seed);
+                  batch * 32.0 * (double)((1ull << k) + (1ull << k) - (1ull << (k - levels))));
+        hipLaunchKernelGGL(merkle_parents_kernel, dim3(blocks, batch), dim3(threads), 0, c.stream, t.layers, k, levels,
+                           c.node_seed, bstride);
         SR_HIP(hipGetLastError());
         k -= levels;
     }
-    launch_tree_top(c, t, k, d_ch, d_root_out);
+    launch_tree_top(c, t, k, d_ch, d_root_out, batch, bstride, d_cnt);
 }
 
 static DeviceTree* new_tree(Ctx& c, uint32_t max_log) {

@@ -8,11 +8,8 @@ namespace sr {
 // one 40-byte final block; valid when the first 16 bytes, read as a
 // little-endian 128-bit integer, have >= bits trailing zeros.  The lowest valid
 // nonce of the launch range is kept with a 64-bit atomic min (vector atomic).
-__global__ __launch_bounds__(256) void pow_kernel(B2State init, uint4 p0, uint4 p1, uint32_t bits, uint64_t base,
-                                                  uint64_t count, unsigned long long* __restrict__ best) {
-    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= count) return;
-    uint64_t nonce = base + i;
+__device__ __forceinline__ void pow_try(const B2State& init, uint4 p0, uint4 p1, uint32_t bits, uint64_t nonce,
+                                        unsigned long long* __restrict__ best) {
     uint32_t m[16] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w,
                       (uint32_t)nonce, (uint32_t)(nonce >> 32), 0, 0, 0, 0, 0, 0};
     uint32_t h[8];
@@ -26,6 +23,33 @@ __global__ __launch_bounds__(256) void pow_kernel(B2State init, uint4 p0, uint4 
     else if (h[3]) tz = 96 + __builtin_ctz(h[3]);
     else tz = 128;
     if (tz >= bits) atomicMin(best, (unsigned long long)nonce);
+}
+__global__ __launch_bounds__(256) void pow_kernel(B2State init, uint4 p0, uint4 p1, uint32_t bits, uint64_t base,
+                                                  uint64_t count, unsigned long long* __restrict__ best) {
+    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    pow_try(init, p0, p1, bits, base + i, best);
+}
+// Prefix blockIdx.y of a batch: its 8 words at prefix + 8 y, its best nonce at best[y].
+__global__ __launch_bounds__(256) void pow_batch_kernel(B2State init, const uint4* __restrict__ prefix, uint32_t bits,
+                                                        uint64_t base, uint64_t count,
+                                                        unsigned long long* __restrict__ best) {
+    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    const uint4 p0 = prefix[2 * blockIdx.y], p1 = prefix[2 * blockIdx.y + 1];
+    pow_try(init, p0, p1, bits, base + i, best + blockIdx.y);
+}
+void launch_pow_grind_batch(Ctx& c, const uint32_t* d_prefix, uint32_t batch, uint32_t bits, uint64_t base,
+                            uint64_t count, unsigned long long* d_best) {
+    SR_CHECK(batch >= 1 && batch <= 65535, "pow: batch size");
+    B2State st;
+    b2_init(st.h);
+    const uint64_t blocks = (count + 255) / 256;
+    SR_CHECK(blocks < (1ull << 31), "pow: launch range too large");
+    KTimer kt(c, "pow_grind", 0.0);
+    hipLaunchKernelGGL(pow_batch_kernel, dim3((uint32_t)blocks, batch), dim3(256), 0, c.stream, st,
+                       (const uint4*)d_prefix, bits, base, count, d_best);
+    SR_HIP(hipGetLastError());
 }
 
 uint64_t launch_pow_grind(Ctx& c, const uint8_t prefix[32], uint32_t bits, uint64_t base, uint64_t count) {

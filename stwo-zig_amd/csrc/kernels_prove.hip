@@ -24,16 +24,35 @@ __device__ __forceinline__ uint32_t lift_row(uint32_t row, uint32_t shift) {
     return ((row >> shift) << 1) + (row & 1);
 }
 
+// Word `i` of an entry's column for the proof at byte offset boff of a batch.
+__device__ __forceinline__ uint32_t ent_word(const QEntryDev& en, uint64_t boff, uint32_t i) {
+    return ((gcu32*)((const uint8_t*)en.col + boff))[i];
+}
+
+template <class T>
+__device__ __forceinline__ T* boffp(T* p, uint64_t boff) {
+    return (T*)((uint8_t*)p + boff);
+}
+template <class T>
+__device__ __forceinline__ const T* boffp(const T* p, uint64_t boff) {
+    return (const T*)((const uint8_t*)p + boff);
+}
+
 __global__ __launch_bounds__(256) void quotients_kernel(const QEntryDev* __restrict__ entries,
                                                         const QuotientBatchDev* __restrict__ batches,
                                                         uint32_t n_batches, uint32_t log_size,
                                                         const uint32_t* __restrict__ domain_xy,
                                                         uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
                                                         uint32_t* __restrict__ o2, uint32_t* __restrict__ o3,
-                                                        uint8_t* __restrict__ leaves, B2State seed) {
+                                                        uint8_t* __restrict__ leaves, B2State seed, uint64_t bstride) {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = 1u << log_size;
     if (row >= n) return;
+    // proof blockIdx.y of a batch: every per-proof buffer (and every column an entry names) sits
+    // bstride bytes after the previous proof's
+    const uint64_t boff = (uint64_t)blockIdx.y * bstride;
+    entries = (const QEntryDev*)((const uint8_t*)entries + boff);
+    batches = (const QuotientBatchDev*)((const uint8_t*)batches + boff);
     const uint32_t x = domain_xy[row], y = domain_xy[n + row];
     constexpr uint32_t QB = 8;
     qm31 acc = qzero();
@@ -47,7 +66,7 @@ __global__ __launch_bounds__(256) void quotients_kernel(const QEntryDev* __restr
 #pragma unroll
             for (uint32_t k = 0; k < QB; k++) {
                 const QEntryDev& en = entries[e + k];
-                v[k] = ((gcu32*)en.col)[lift_row(row, en.shift)];
+                v[k] = ent_word(en, boff, lift_row(row, en.shift));
             }
 #pragma unroll
             for (uint32_t h = 0; h < QB; h += 4) {
@@ -68,7 +87,7 @@ __global__ __launch_bounds__(256) void quotients_kernel(const QEntryDev* __restr
         }
         for (; e < end; e++) {
             const QEntryDev& en = entries[e];
-            const uint32_t v = ((gcu32*)en.col)[lift_row(row, en.shift)];
+            const uint32_t v = ent_word(en, boff, lift_row(row, en.shift));
             s0 += mfold((uint64_t)en.coeff[0] * v);
             s1 += mfold((uint64_t)en.coeff[1] * v);
             s2 += mfold((uint64_t)en.coeff[2] * v);
@@ -79,11 +98,12 @@ __global__ __launch_bounds__(256) void quotients_kernel(const QEntryDev* __restr
         const cm31 den = cadd(csub(bt.det, cmulm(bt.piy, x)), cmulm(bt.pix, y));
         acc = qadd(acc, qmulc(num, cinv(den)));
     }
-    o0[row] = acc.c0.a;
-    o1[row] = acc.c0.b;
-    o2[row] = acc.c1.a;
-    o3[row] = acc.c1.b;
+    boffp(o0, boff)[row] = acc.c0.a;
+    boffp(o1, boff)[row] = acc.c0.b;
+    boffp(o2, boff)[row] = acc.c1.a;
+    boffp(o3, boff)[row] = acc.c1.b;
     if (leaves) {
+        leaves += boff;
         uint32_t m[16] = {acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         uint32_t h[8];
 #pragma unroll
@@ -102,11 +122,16 @@ __global__ __launch_bounds__(256) void quotients_kernel(const QEntryDev* __restr
 constexpr uint32_t QS_CHUNK = 64;
 __global__ __launch_bounds__(256) void quotients_split_kernel(const QEntryDev* __restrict__ entries,
                                                               const QuotientBatchDev* __restrict__ batches,
-                                                              uint32_t log_size, uint32_t* __restrict__ part) {
+                                                              uint32_t log_size, uint32_t* __restrict__ part,
+                                                              uint64_t bstride, uint64_t pstride) {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = 1u << log_size;
     if (row >= n) return;
-    const QuotientBatchDev bt = batches[0];
+    // proof blockIdx.z of a batch (blockIdx.y is the entry chunk); partials pstride apart
+    const uint64_t boff = (uint64_t)blockIdx.z * bstride;
+    entries = boffp(entries, boff);
+    part = boffp(part, (uint64_t)blockIdx.z * pstride);
+    const QuotientBatchDev bt = boffp(batches, boff)[0];
     const uint32_t e0 = bt.first + blockIdx.y * QS_CHUNK;
     const uint32_t end = min(e0 + QS_CHUNK, bt.first + bt.count);
     constexpr uint32_t QB = 8;
@@ -117,7 +142,7 @@ __global__ __launch_bounds__(256) void quotients_split_kernel(const QEntryDev* _
 #pragma unroll
         for (uint32_t k = 0; k < QB; k++) {
             const QEntryDev& en = entries[e + k];
-            v[k] = ((gcu32*)en.col)[lift_row(row, en.shift)];
+            v[k] = ent_word(en, boff, lift_row(row, en.shift));
         }
 #pragma unroll
         for (uint32_t h = 0; h < QB; h += 4) {
@@ -138,7 +163,7 @@ __global__ __launch_bounds__(256) void quotients_split_kernel(const QEntryDev* _
     }
     for (; e < end; e++) {
         const QEntryDev& en = entries[e];
-        const uint32_t v = ((gcu32*)en.col)[lift_row(row, en.shift)];
+        const uint32_t v = ent_word(en, boff, lift_row(row, en.shift));
         s0 += mfold((uint64_t)en.coeff[0] * v);
         s1 += mfold((uint64_t)en.coeff[1] * v);
         s2 += mfold((uint64_t)en.coeff[2] * v);
@@ -156,10 +181,14 @@ __global__ __launch_bounds__(256) void quotients_finish_kernel(const QuotientBat
                                                                const uint32_t* __restrict__ part,
                                                                uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
                                                                uint32_t* __restrict__ o2, uint32_t* __restrict__ o3,
-                                                               uint8_t* __restrict__ leaves, B2State seed) {
+                                                               uint8_t* __restrict__ leaves, B2State seed,
+                                                               uint64_t bstride, uint64_t pstride) {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = 1u << log_size;
     if (row >= n) return;
+    const uint64_t boff = (uint64_t)blockIdx.y * bstride;
+    part = boffp(part, (uint64_t)blockIdx.y * pstride);
+    batches = boffp(batches, boff);
     const uint32_t x = domain_xy[row], y = domain_xy[n + row];
     uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;  // n_chunks canonical words: far below 2^64
     for (uint32_t k = 0; k < n_chunks; k++) {
@@ -174,11 +203,12 @@ __global__ __launch_bounds__(256) void quotients_finish_kernel(const QuotientBat
     num = qsub(num, qadd(qmulm(bt.sum_a, y), bt.sum_b));
     const cm31 den = cadd(csub(bt.det, cmulm(bt.piy, x)), cmulm(bt.pix, y));
     const qm31 acc = qmulc(num, cinv(den));
-    o0[row] = acc.c0.a;
-    o1[row] = acc.c0.b;
-    o2[row] = acc.c1.a;
-    o3[row] = acc.c1.b;
+    boffp(o0, boff)[row] = acc.c0.a;
+    boffp(o1, boff)[row] = acc.c0.b;
+    boffp(o2, boff)[row] = acc.c1.a;
+    boffp(o3, boff)[row] = acc.c1.b;
     if (leaves) {
+        leaves += boff;
         uint32_t m[16] = {acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         uint32_t h[8];
 #pragma unroll
@@ -214,23 +244,25 @@ void launch_quotients(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_c
 
 void launch_quotients_dev(Ctx& c, const QEntryDev* d_ent, uint32_t n_entries, const QuotientBatchDev* d_bat,
                           uint32_t n_batches, double in_bytes, uint32_t log_size, uint32_t* const* out,
-                          uint8_t* d_leaf_out) {
+                          uint8_t* d_leaf_out, uint32_t batch, uint64_t bstride) {
     SR_CHECK(n_entries >= 1 && n_batches >= 1, "quotients: no samples");
+    SR_CHECK(batch >= 1 && batch <= 65535, "quotients: batch size");
     const uint32_t* xy = c.domain_points(log_size);
     uint32_t n = 1u << log_size;
-    KTimer kt(c, "quotients", in_bytes + (8.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n);
+    KTimer kt(c, "quotients", batch * (in_bytes + (8.0 + 16.0 + (d_leaf_out ? 32.0 : 0.0)) * n));
     // the split form pays off when one thread per row leaves most of the device idle
     if (n_batches == 1 && n_entries > 2 * QS_CHUNK && log_size <= 16) {
         const uint32_t n_chunks = (n_entries + QS_CHUNK - 1) / QS_CHUNK;  // the batch spans every entry
-        DBuf part(&c, 16ull * n_chunks * n);
-        hipLaunchKernelGGL(quotients_split_kernel, dim3((n + 255) / 256, n_chunks), dim3(256), 0, c.stream, d_ent,
-                           d_bat, log_size, part.as<uint32_t>());
-        hipLaunchKernelGGL(quotients_finish_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_bat, n_chunks,
-                           log_size, xy, (const uint32_t*)part.as<uint32_t>(), out[0], out[1], out[2], out[3],
-                           d_leaf_out, c.leaf_seed);
+        const uint64_t pstride = 16ull * n_chunks * n;
+        DBuf part(&c, pstride * batch);
+        hipLaunchKernelGGL(quotients_split_kernel, dim3((n + 255) / 256, n_chunks, batch), dim3(256), 0, c.stream,
+                           d_ent, d_bat, log_size, part.as<uint32_t>(), bstride, pstride);
+        hipLaunchKernelGGL(quotients_finish_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, c.stream, d_bat,
+                           n_chunks, log_size, xy, (const uint32_t*)part.as<uint32_t>(), out[0], out[1], out[2],
+                           out[3], d_leaf_out, c.leaf_seed, bstride, pstride);
     } else {
-        hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_ent, d_bat, n_batches,
-                           log_size, xy, out[0], out[1], out[2], out[3], d_leaf_out, c.leaf_seed);
+        hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, c.stream, d_ent, d_bat,
+                           n_batches, log_size, xy, out[0], out[1], out[2], out[3], d_leaf_out, c.leaf_seed, bstride);
     }
     SR_HIP(hipGetLastError());
 }
@@ -244,10 +276,13 @@ __global__ __launch_bounds__(256) void widefib_composition_kernel(const uint32_t
                                                                   uint32_t den0, uint32_t den1,
                                                                   uint32_t* __restrict__ o0, uint32_t* __restrict__ o1,
                                                                   uint32_t* __restrict__ o2,
-                                                                  uint32_t* __restrict__ o3) {
+                                                                  uint32_t* __restrict__ o3, uint64_t bstride) {
     uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= (1u << log_eval)) return;
-    uint32_t a = ((gcu32*)cols[0])[row], b = ((gcu32*)cols[1])[row];
+    const uint64_t boff = (uint64_t)blockIdx.y * bstride;  // proof blockIdx.y of a batch
+    powers = boffp(powers, boff);
+#define WF_COL(i) ((gcu32*)((const uint8_t*)cols[i] + boff))
+    uint32_t a = WF_COL(0)[row], b = WF_COL(1)[row];
     // sa, sb: squares of the two previous columns (each column is squared once).
     uint32_t sa = msqr(a), sb = msqr(b);
     uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -260,7 +295,7 @@ __global__ __launch_bounds__(256) void widefib_composition_kernel(const uint32_t
     for (; i + CB <= nc; i += CB) {
         uint32_t v[CB];
 #pragma unroll
-        for (uint32_t k = 0; k < CB; k++) v[k] = ((gcu32*)cols[i + 2 + k])[row];
+        for (uint32_t k = 0; k < CB; k++) v[k] = WF_COL(i + 2 + k)[row];
 #pragma unroll
         for (uint32_t h = 0; h < CB; h += 4) {
             uint64_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
@@ -282,7 +317,7 @@ __global__ __launch_bounds__(256) void widefib_composition_kernel(const uint32_t
         }
     }
     for (; i < nc; i++) {
-        const uint32_t cv = ((gcu32*)cols[i + 2])[row];
+        const uint32_t cv = WF_COL(i + 2)[row];
         const uint32_t cons = msub(cv, madd(sa, sb));
         const uint4 pw = powers[nc - 1 - i];
         s0 += mfold((uint64_t)pw.x * cons);
@@ -292,21 +327,24 @@ __global__ __launch_bounds__(256) void widefib_composition_kernel(const uint32_t
         sa = sb;
         sb = msqr(cv);
     }
+#undef WF_COL
     uint32_t d = (row >> log_n) ? den1 : den0;
-    o0[row] = mmul(mred64(s0), d);
-    o1[row] = mmul(mred64(s1), d);
-    o2[row] = mmul(mred64(s2), d);
-    o3[row] = mmul(mred64(s3), d);
+    boffp(o0, boff)[row] = mmul(mred64(s0), d);
+    boffp(o1, boff)[row] = mmul(mred64(s1), d);
+    boffp(o2, boff)[row] = mmul(mred64(s2), d);
+    boffp(o3, boff)[row] = mmul(mred64(s3), d);
 }
 
 void launch_widefib_composition(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_eval,
                                 uint32_t log_n, const uint32_t* d_powers, uint32_t den0, uint32_t den1,
-                                uint32_t* const* out) {
+                                uint32_t* const* out, uint32_t batch, uint64_t bstride) {
     SR_CHECK(n_cols >= 3, "composition needs at least 3 columns");
+    SR_CHECK(batch >= 1 && batch <= 65535, "composition: batch size");
     uint32_t n = 1u << log_eval;
-    KTimer kt(c, "composition", (4.0 * n_cols + 16.0) * n);
-    hipLaunchKernelGGL(widefib_composition_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, d_cols, n_cols,
-                       log_eval, log_n, (const uint4*)d_powers, den0, den1, out[0], out[1], out[2], out[3]);
+    KTimer kt(c, "composition", batch * (4.0 * n_cols + 16.0) * n);
+    hipLaunchKernelGGL(widefib_composition_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, c.stream, d_cols,
+                       n_cols, log_eval, log_n, (const uint4*)d_powers, den0, den1, out[0], out[1], out[2], out[3],
+                       bstride);
     SR_HIP(hipGetLastError());
 }
 
@@ -321,10 +359,14 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
 }
 __global__ __launch_bounds__(256) void widefib_trace_kernel(uint32_t log_n, uint32_t seq_len, uint32_t mode,
                                                             uint64_t seed, uint32_t* __restrict__ out,
-                                                            uint64_t stride) {
+                                                            uint64_t stride, const uint64_t* __restrict__ seeds,
+                                                            uint64_t bstride) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = 1u << log_n;
     if (s >= n) return;
+    // proof blockIdx.y of a batch: its own seed and trace bstride bytes further on
+    if (seeds) seed = seeds[blockIdx.y];
+    out = boffp(out, (uint64_t)blockIdx.y * bstride);
     uint32_t a, b;
     if (mode == 0) {
         uint32_t ci = bitrev32(s, log_n);
@@ -345,11 +387,12 @@ __global__ __launch_bounds__(256) void widefib_trace_kernel(uint32_t log_n, uint
     }
 }
 void launch_widefib_trace(Ctx& c, uint32_t log_n, uint32_t seq_len, uint32_t mode, uint64_t seed, uint32_t* d_out,
-                          uint64_t col_stride) {
+                          uint64_t col_stride, uint32_t batch, const uint64_t* d_seeds, uint64_t bstride) {
+    SR_CHECK(batch >= 1 && batch <= 65535, "widefib trace: batch size");
     uint32_t n = 1u << log_n;
-    KTimer kt(c, "widefib_trace", 4.0 * seq_len * n);
-    hipLaunchKernelGGL(widefib_trace_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, log_n, seq_len, mode,
-                       seed, d_out, col_stride);
+    KTimer kt(c, "widefib_trace", batch * 4.0 * seq_len * n);
+    hipLaunchKernelGGL(widefib_trace_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, c.stream, log_n, seq_len,
+                       mode, seed, d_out, col_stride, d_seeds, bstride);
     SR_HIP(hipGetLastError());
 }
 
@@ -502,8 +545,12 @@ __global__ __launch_bounds__(256) void eval_weight_tables_kernel(Factors fs, uin
     tab[k] = make_uint4(acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b);
 }
 __global__ __launch_bounds__(256) void eval_weight_tables_dev_kernel(const uint4* __restrict__ fs, uint32_t log_size,
-                                                                     uint32_t h, uint4* __restrict__ tab) {
+                                                                     uint32_t h, uint4* __restrict__ tab,
+                                                                     uint64_t bstride) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t boff = (uint64_t)blockIdx.y * bstride;
+    fs = boffp(fs, boff);
+    tab = boffp(tab, boff);
     const uint32_t n_lo = 1u << h, n_hi = 1u << (log_size - h);
     if (k >= n_lo + n_hi) return;
     const bool lo = k < n_lo;
@@ -517,9 +564,12 @@ __global__ __launch_bounds__(256) void eval_weight_tables_dev_kernel(const uint4
     tab[k] = make_uint4(acc.c0.a, acc.c0.b, acc.c1.a, acc.c1.b);
 }
 __global__ __launch_bounds__(256) void eval_weights_kernel(const uint4* __restrict__ tab, uint32_t log_size, uint32_t h,
-                                                           uint4* __restrict__ w) {
+                                                           uint4* __restrict__ w, uint64_t bstride) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (1u << log_size)) return;
+    const uint64_t boff = (uint64_t)blockIdx.y * bstride;
+    tab = boffp(tab, boff);
+    w = boffp(w, boff);
     const uint4 a = tab[i & ((1u << h) - 1)], b = tab[(1u << h) + (i >> h)];
     const qm31 v = qmul(qm31{{a.x, a.y}, {a.z, a.w}}, qm31{{b.x, b.y}, {b.z, b.w}});
     w[i] = make_uint4(v.c0.a, v.c0.b, v.c1.a, v.c1.b);
@@ -539,7 +589,7 @@ void launch_eval_weights_into(Ctx& c, const qm31 factors[32], uint32_t log_size,
     hipLaunchKernelGGL(eval_weight_tables_kernel, dim3((nt + 255) / 256), dim3(256), 0, c.stream, fs, log_size, h,
                        (uint4*)d_tab);
     hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, (const uint4*)d_tab,
-                       log_size, h, (uint4*)d_w);
+                       log_size, h, (uint4*)d_w, (uint64_t)0);
     SR_HIP(hipGetLastError());
 }
 
@@ -549,16 +599,22 @@ void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint
     launch_eval_weights_into(c, factors, log_size, d_w, tab.as<uint32_t>());
 }
 
-void launch_eval_weights_dev(Ctx& c, const uint32_t* d_factors, uint32_t log_size, uint32_t* d_w) {
+void launch_eval_weights_dev(Ctx& c, const uint32_t* d_factors, uint32_t log_size, uint32_t* d_w, uint32_t batch,
+                             uint64_t bstride, uint32_t* d_tab) {
     SR_CHECK(log_size <= 30, "eval weights: log size too large");
+    SR_CHECK(batch >= 1 && batch <= 65535 && (batch == 1 || d_tab), "eval weights: batch");
     const uint32_t n = 1u << log_size, h = log_size / 2;
     const uint32_t nt = (1u << h) + (1u << (log_size - h));
-    DBuf tab(&c, 16ull * nt);
-    KTimer kt(c, "eval_weights", 16.0 * n);
-    hipLaunchKernelGGL(eval_weight_tables_dev_kernel, dim3((nt + 255) / 256), dim3(256), 0, c.stream,
-                       (const uint4*)d_factors, log_size, h, tab.as<uint4>());
-    hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, c.stream, (const uint4*)tab.as<uint4>(),
-                       log_size, h, (uint4*)d_w);
+    DBuf tab_own;
+    if (!d_tab) {
+        tab_own = DBuf(&c, 16ull * nt);
+        d_tab = tab_own.as<uint32_t>();
+    }
+    KTimer kt(c, "eval_weights", batch * 16.0 * n);
+    hipLaunchKernelGGL(eval_weight_tables_dev_kernel, dim3((nt + 255) / 256, batch), dim3(256), 0, c.stream,
+                       (const uint4*)d_factors, log_size, h, (uint4*)d_tab, bstride);
+    hipLaunchKernelGGL(eval_weights_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, c.stream, (const uint4*)d_tab,
+                       log_size, h, (uint4*)d_w, bstride);
     SR_HIP(hipGetLastError());
 }
 
@@ -583,9 +639,12 @@ __device__ __forceinline__ uint64_t ev_wave_sum(uint64_t v) {
 __global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
                                                            uint32_t log_size, const uint4* __restrict__ w,
                                                            uint32_t chunks, uint32_t groups,
-                                                           unsigned long long* __restrict__ part) {
+                                                           unsigned long long* __restrict__ part, uint64_t bstride) {
     __shared__ unsigned long long red[4][EV_CB * 4];
     const uint32_t t = threadIdx.x;
+    const uint64_t boff = (uint64_t)blockIdx.y * bstride;  // proof blockIdx.y of a batch
+    w = boffp(w, boff);
+    part = boffp(part, boff);
     const uint32_t k = blockIdx.x / EV_XCDS;
     const uint32_t chunk = (k / groups) * EV_XCDS + blockIdx.x % EV_XCDS;
     if (chunk >= chunks) return;
@@ -610,7 +669,7 @@ __global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const
             u32x4 cv[EV_CB];
 #pragma unroll
             for (uint32_t c = 0; c < EV_CB; c++)
-                cv[c] = *(const SR_GLOBAL u32x4*)((gcu32*)cols[c0 + (c < nc ? c : nc - 1)] + i0);
+                cv[c] = *(const SR_GLOBAL u32x4*)((gcu32*)boffp(cols[c0 + (c < nc ? c : nc - 1)], boff) + i0);
 #pragma unroll
             for (uint32_t c = 0; c < EV_CB; c++) {
                 if (c < nc) {
@@ -636,7 +695,7 @@ __global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const
 #pragma unroll
         for (uint32_t c = 0; c < EV_CB; c++) {
             if (c < nc) {
-                const uint64_t v = cols[c0 + c][t];
+                const uint64_t v = boffp(cols[c0 + c], boff)[t];
                 s[c][0] += mfold(v * ww.x);
                 s[c][1] += mfold(v * ww.y);
                 s[c][2] += mfold(v * ww.z);
@@ -659,9 +718,12 @@ __global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const
 }
 // out[k] = sum over chunks of part[k][chunk] mod p: one wave per output word.
 __global__ __launch_bounds__(256) void eval_reduce_kernel(const unsigned long long* __restrict__ part, uint32_t chunks,
-                                                          uint32_t n_out, uint32_t* __restrict__ out) {
+                                                          uint32_t n_out, uint32_t* __restrict__ out, uint64_t bstride) {
     const uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
     if (k >= n_out) return;
+    const uint64_t boff = (uint64_t)blockIdx.y * bstride;
+    part = boffp(part, boff);
+    out = boffp(out, boff);
     uint64_t x = 0;
     for (uint32_t j = l; j < chunks; j += 64) x += part[(uint64_t)k * chunks + j];
     x = ev_wave_sum(x);
@@ -676,18 +738,20 @@ uint64_t eval_partial_bytes(uint32_t n_cols, uint32_t log_size) {
 }
 
 void launch_eval_columns_into(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
-                              const uint32_t* d_w, uint32_t* d_out, unsigned long long* d_part) {
+                              const uint32_t* d_w, uint32_t* d_out, unsigned long long* d_part, uint32_t batch,
+                              uint64_t bstride) {
     if (n_cols == 0) return;
+    SR_CHECK(batch >= 1 && batch <= 65535, "eval columns: batch size");
     const uint32_t n = 1u << log_size;
     const uint32_t chunks = eval_chunks(log_size);
     const uint32_t groups = (n_cols + EV_CB - 1) / EV_CB;
     const uint32_t nw = n_cols * 4;
-    KTimer kt(c, "eval_columns", (4.0 * n_cols + 16.0) * n);
+    KTimer kt(c, "eval_columns", batch * (4.0 * n_cols + 16.0) * n);
     const uint32_t blocks = (chunks + EV_XCDS - 1) / EV_XCDS * EV_XCDS * groups;
-    hipLaunchKernelGGL(eval_columns_kernel, dim3(blocks), dim3(256), 0, c.stream, d_cols, n_cols, log_size,
-                       (const uint4*)d_w, chunks, groups, d_part);
-    hipLaunchKernelGGL(eval_reduce_kernel, dim3((nw + 3) / 4), dim3(256), 0, c.stream,
-                       (const unsigned long long*)d_part, chunks, nw, d_out);
+    hipLaunchKernelGGL(eval_columns_kernel, dim3(blocks, batch), dim3(256), 0, c.stream, d_cols, n_cols, log_size,
+                       (const uint4*)d_w, chunks, groups, d_part, bstride);
+    hipLaunchKernelGGL(eval_reduce_kernel, dim3((nw + 3) / 4, batch), dim3(256), 0, c.stream,
+                       (const unsigned long long*)d_part, chunks, nw, d_out, bstride);
     SR_HIP(hipGetLastError());
 }
 

@@ -116,18 +116,6 @@ static void merkle_witness_positions(uint32_t max_log, const std::vector<uint64_
     }
 }
 
-// Gather plan: absolute device word addresses, downloaded with one kernel.
-struct Gather {
-    std::vector<uint64_t> addr;
-    size_t add_words(const uint32_t* p, uint64_t n) {
-        size_t at = addr.size();
-        for (uint64_t i = 0; i < n; i++) addr.push_back((uint64_t)(uintptr_t)(p + i));
-        return at;
-    }
-    size_t add_hash(const DeviceTree& t, uint32_t layer, uint64_t idx) {
-        return add_words((const uint32_t*)(t.layer(layer) + idx * 32), 8);
-    }
-};
 
 // Compact JSON writer into a growing char buffer: byte values come from a table of their
 // decimal forms, wider integers are formatted two digits at a time.
@@ -377,8 +365,10 @@ void fri_fold_commit_down(Ctx& c, TreeOwner& owner, const uint32_t* d_top, uint3
     }
 }
 
-void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std::string& proof,
-                  const std::function<void()>& mark) {
+// The proof tail in three phases, so that a batch of proofs can share the proof-of-work launch and
+// the decommitment gather: the last layer (its interpolation, degree check and mix), then the
+// nonce mix, the queries and the gather plan, then the wire encoding.
+void finish_last_layer(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, FinishPlan& fp) {
     auto bitrev_vec = [](std::vector<qm31>& v) {
         uint32_t lg = 0;
         while ((1ull << lg) < v.size()) lg++;
@@ -387,7 +377,7 @@ void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std
             if (j > i) std::swap(v[i], v[j]);
         }
     };
-    const uint32_t L = t.L, llog = t.last_log, B = prm.log_blowup_factor;
+    const uint32_t llog = t.last_log;
     std::vector<qm31>& lv = t.last_values;
     const uint64_t lsz = lv.size();
     SR_CHECK(lsz == (1ull << llog), "last layer size");
@@ -418,57 +408,54 @@ void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std
     const uint64_t bound = 1ull << prm.log_last_layer_degree_bound;
     for (uint64_t i = bound; i < lsz; i++)
         SR_CHECK(qiszero(lv[i]), "FRI last layer exceeds the degree bound (constraints not satisfied)");
-    std::vector<qm31> lastp(lv.begin(), lv.begin() + bound);
-    bitrev_vec(lastp);  // LinePoly.fromOrderedCoefficients
-    ch.mix_felts(lastp.data(), lastp.size());
-    mark();  // FRI commit
+    fp.lastp.assign(lv.begin(), lv.begin() + bound);
+    bitrev_vec(fp.lastp);  // LinePoly.fromOrderedCoefficients
+    ch.mix_felts(fp.lastp.data(), fp.lastp.size());
+}
 
-    // Proof of work (scheme.zig:733-735; lowest nonce, channel/blake2s.zig:163-167).
-    uint64_t nonce = 0;
-    if (prm.pow_bits > 0) {
-        uint8_t prefix[32];
-        ch.pow_prefix(prm.pow_bits, prefix);
-        // chunks are searched in order and each returns its lowest nonce, so the size only sets
-        // how much is ground past the first solution: ~16x the expected work, 2^12..2^26
-        const uint32_t cl = std::min(26u, std::max(12u, prm.pow_bits + 4));
-        const uint64_t chunk = 1ull << cl;
-        for (uint64_t base = 0;; base += chunk) {
-            uint64_t r = launch_pow_grind(c, prefix, prm.pow_bits, base, chunk);
-            if (r != UINT64_MAX) {
-                nonce = r;
-                break;
-            }
-            SR_CHECK(base < (1ull << 50), "pow: no nonce found");
-        }
+// Proof of work for one proof (scheme.zig:733-735; lowest nonce, channel/blake2s.zig:163-167).
+static uint64_t grind_one(Ctx& c, const Channel& ch, uint32_t pow_bits) {
+    if (pow_bits == 0) return 0;
+    uint8_t prefix[32];
+    ch.pow_prefix(pow_bits, prefix);
+    // chunks are searched in order and each returns its lowest nonce, so the size only sets
+    // how much is ground past the first solution: ~16x the expected work, 2^12..2^26
+    const uint32_t cl = std::min(26u, std::max(12u, pow_bits + 4));
+    const uint64_t chunk = 1ull << cl;
+    for (uint64_t base = 0;; base += chunk) {
+        uint64_t r = launch_pow_grind(c, prefix, pow_bits, base, chunk);
+        if (r != UINT64_MAX) return r;
+        SR_CHECK(base < (1ull << 50), "pow: no nonce found");
     }
-    ch.mix_u64(nonce);
-    mark();  // pow
+}
 
+void finish_queries(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, FinishPlan& fp, Gather& g) {
+    (void)c;
+    const uint32_t L = t.L;
+    ch.mix_u64(fp.nonce);
     // Queries (core/queries.zig:8-60).
-    std::vector<uint64_t> qs;
-    {
-        while (qs.size() < prm.n_queries) {
-            uint32_t w[8];
-            ch.draw_u32s(w);
-            for (int i = 0; i < 8 && qs.size() < prm.n_queries; i++) qs.push_back(w[i] & ((1u << L) - 1));
-        }
-        std::sort(qs.begin(), qs.end());
-        qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
+    std::vector<uint64_t>& qs = fp.qs;
+    qs.clear();
+    while (qs.size() < prm.n_queries) {
+        uint32_t w[8];
+        ch.draw_u32s(w);
+        for (int i = 0; i < 8 && qs.size() < prm.n_queries; i++) qs.push_back(w[i] & ((1u << L) - 1));
     }
+    std::sort(qs.begin(), qs.end());
+    qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
     const size_t nq = qs.size();
 
     // Decommitment plan: everything read from HBM in one gather.
-    Gather g;
     // queried values (scheme_decommit.zig; decommit.zig:77-89) and witnesses of every tree
     // with columns.
     const size_t NT = t.trees.size();
-    std::vector<size_t> qv_at(NT, 0);
+    fp.qv_at.assign(NT, 0);
+    fp.tw_at.assign(NT, {});
     std::vector<std::vector<std::pair<uint32_t, uint64_t>>> tw_pos(NT);
-    std::vector<std::vector<size_t>> tw_at(NT);
     for (size_t tr = 0; tr < NT; tr++) {
         const Committed& T = *t.trees[tr];
         if (T.lde.empty()) continue;
-        qv_at[tr] = g.addr.size();
+        fp.qv_at[tr] = g.addr.size();
         for (size_t col = 0; col < T.lde.size(); col++)
             for (size_t q = 0; q < nq; q++) {
                 uint32_t sh = T.tree->max_log - T.lde_logs[col] + 1;
@@ -476,17 +463,13 @@ void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std
                 g.add_words(T.lde[col] + idx, 1);
             }
         merkle_witness_positions(T.tree->max_log, qs, tw_pos[tr]);
-        for (auto& lp : tw_pos[tr]) tw_at[tr].push_back(g.add_hash(*T.tree, lp.first, lp.second));
+        for (auto& lp : tw_pos[tr]) fp.tw_at[tr].push_back(g.add_hash(*T.tree, lp.first, lp.second));
     }
     // FRI layers (fri.zig:626-740): subsets {2k, 2k+1}, witness = non-queried partner.
-    struct LayerPlan {
-        std::vector<size_t> wit_at;
-        std::vector<size_t> hw_at;
-    };
-    std::vector<LayerPlan> lplans;
-    auto plan_layer = [&](const DeviceTree& t, const std::vector<uint64_t>& q,
+    fp.lplans.clear();
+    auto plan_layer = [&](const DeviceTree& lt, const std::vector<uint64_t>& q,
                           std::function<size_t(uint64_t)> add_val) {
-        LayerPlan lp;
+        FinishPlan::LayerPlan lp;
         std::vector<uint64_t> dpos;
         size_t i = 0;
         while (i < q.size()) {
@@ -502,9 +485,9 @@ void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std
             i = j;
         }
         std::vector<std::pair<uint32_t, uint64_t>> hw;
-        merkle_witness_positions(t.max_log, dpos, hw);
-        for (auto& h : hw) lp.hw_at.push_back(g.add_hash(t, h.first, h.second));
-        lplans.push_back(lp);
+        merkle_witness_positions(lt.max_log, dpos, hw);
+        for (auto& h : hw) lp.hw_at.push_back(g.add_hash(lt, h.first, h.second));
+        fp.lplans.push_back(lp);
     };
     plan_layer(*t.ftree0, qs, [&](uint64_t p) {
         size_t at = g.addr.size();
@@ -522,18 +505,25 @@ void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std
             if (nx.empty() || nx.back() != (q >> 1)) nx.push_back(q >> 1);
         lq.swap(nx);
     }
-    std::vector<uint32_t> gw(g.addr.size());
-    if (!g.addr.empty()) {
-        // the gather reads its address list from pinned host memory and writes the words
-        // there too: no copy operations on the stream
-        uint32_t* h = (uint32_t*)c.pinned(g.addr.size() * 4);
-        const uint64_t* h_addr = (const uint64_t*)c.stage(g.addr.data(), g.addr.size() * 8);
-        launch_gather(c, h_addr, g.addr.size(), h);
-        c.sync();
-        memcpy(gw.data(), h, g.addr.size() * 4);
-    }
-    mark();  // decommit gather
+}
 
+// Words of a gather plan, read from HBM in one launch through pinned host memory.
+void gather_words(Ctx& c, const Gather& g, std::vector<uint32_t>& gw) {
+    gw.assign(g.addr.size(), 0);
+    if (g.addr.empty()) return;
+    // the gather reads its address list from pinned host memory and writes the words
+    // there too: no copy operations on the stream
+    uint32_t* h = (uint32_t*)c.pinned(g.addr.size() * 4);
+    const uint64_t* h_addr = (const uint64_t*)c.stage(g.addr.data(), g.addr.size() * 8);
+    launch_gather(c, h_addr, g.addr.size(), h);
+    c.sync();
+    memcpy(gw.data(), h, g.addr.size() * 4);
+}
+
+std::string finish_encode(const ProveParams& prm, const ProofTail& t, const FinishPlan& fp, const uint32_t* gw) {
+    const uint32_t B = prm.log_blowup_factor;
+    const size_t NT = t.trees.size();
+    const size_t nq = fp.qs.size();
     // Proof wire (interop/proof_wire.zig:19-76).
     JsonOut j;
     j.raw("{\"config\":{\"pow_bits\":");
@@ -569,16 +559,16 @@ void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std
     j.raw("],\"decommitments\":[");
     for (size_t tr = 0; tr < NT; tr++) {
         j.raw(tr ? ",{\"hash_witness\":[" : "{\"hash_witness\":[");
-        for (size_t k = 0; k < tw_at[tr].size(); k++) {
+        for (size_t k = 0; k < fp.tw_at[tr].size(); k++) {
             if (k) j.raw(",");
-            j.hash((const uint8_t*)&gw[tw_at[tr][k]]);
+            j.hash((const uint8_t*)&gw[fp.tw_at[tr][k]]);
         }
         j.raw("]}");
     }
     j.raw("],\"queried_values\":[");
     for (size_t tr = 0; tr < NT; tr++) {
         j.raw(tr ? ",[" : "[");
-        size_t at = qv_at[tr];
+        size_t at = fp.qv_at[tr];
         for (size_t col = 0; col < t.trees[tr]->lde.size(); col++) {
             if (col) j.raw(",");
             j.raw("[");
@@ -591,9 +581,9 @@ void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std
         j.raw("]");
     }
     j.raw("],\"proof_of_work\":");
-    j.u64(nonce);
+    j.u64(fp.nonce);
     j.raw(",\"fri_proof\":{\"first_layer\":");
-    auto emit_layer = [&](const LayerPlan& lp, const DeviceTree& lt) {
+    auto emit_layer = [&](const FinishPlan::LayerPlan& lp, const DeviceTree& lt) {
         j.raw("{\"fri_witness\":[");
         for (size_t k = 0; k < lp.wit_at.size(); k++) {
             if (k) j.raw(",");
@@ -608,22 +598,52 @@ void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std
         j.hash(lt.root);
         j.raw("}");
     };
-    emit_layer(lplans[0], *t.ftree0);
+    emit_layer(fp.lplans[0], *t.ftree0);
     j.raw(",\"inner_layers\":[");
     for (size_t k = 0; k < t.ftrees.size(); k++) {
         if (k) j.raw(",");
-        emit_layer(lplans[1 + k], *t.ftrees[k]);
+        emit_layer(fp.lplans[1 + k], *t.ftrees[k]);
     }
     j.raw("],\"last_layer_poly\":[");
-    for (size_t k = 0; k < lastp.size(); k++) {
+    for (size_t k = 0; k < fp.lastp.size(); k++) {
         if (k) j.raw(",");
         uint32_t w4[4];
-        qstore(lastp[k], w4);
+        qstore(fp.lastp[k], w4);
         j.qm(w4);
     }
     j.raw("]}}");
-    proof = j.finish();
+    return j.finish();
+}
+
+void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std::string& proof,
+                  const std::function<void()>& mark) {
+    FinishPlan fp;
+    finish_last_layer(c, ch, prm, t, fp);
+    mark();  // FRI commit
+    fp.nonce = grind_one(c, ch, prm.pow_bits);
+    mark();  // pow
+    Gather g;
+    finish_queries(c, ch, prm, t, fp, g);
+    std::vector<uint32_t> gw;
+    gather_words(c, g, gw);
+    mark();  // decommit gather
+    proof = finish_encode(prm, t, fp, gw.data());
     mark();  // encode
+}
+
+// cosetVanishing(CanonicCoset(n).coset(), domain.at(0/1)) (constraints.zig:16-27): the inverse
+// vanishing value at the two halves of the composition domain CanonicCoset(n + 1).
+void trace_vanishing_inverses(Ctx& c, uint32_t n, uint32_t den_inv[2]) {
+    const uint32_t CL = n + 1;
+    for (int r = 0; r < 2; r++) {
+        pt dp = idx_to_point(c.gen, domain_index_at(CL, (uint64_t)r));
+        pt init = idx_to_point(c.gen, subgroup_gen(n + 1));     // odds(n).initial
+        pt half_step = idx_to_point(c.gen, subgroup_gen(n + 1));  // step_size(n) / 2
+        pt s = padd(padd(dp, pconj(init)), half_step);
+        uint32_t x = s.x;
+        for (uint32_t i = 1; i < n; i++) x = mdouble_x(x);
+        den_inv[r] = minv(x);
+    }
 }
 
 void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trace_in, std::string& proof,
@@ -847,17 +867,8 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         const uint32_t st[2] = {poseidon ? n + 3 : n, W};
         launch_tr_commit_draw(c, d_ch, t1.tree->layers, st, poseidon ? 1 : 2, d_pw.as<uint32_t>(), NC);
     }
-    // cosetVanishing(CanonicCoset(n).coset(), domain.at(0/1)) (constraints.zig:16-27).
     uint32_t den_inv[2];
-    for (int r = 0; r < 2; r++) {
-        pt dp = idx_to_point(c.gen, domain_index_at(CL, (uint64_t)r));
-        pt init = idx_to_point(c.gen, subgroup_gen(n + 1));     // odds(n).initial
-        pt half_step = idx_to_point(c.gen, subgroup_gen(n + 1));  // step_size(n) / 2
-        pt s = padd(padd(dp, pconj(init)), half_step);
-        uint32_t x = s.x;
-        for (uint32_t i = 1; i < n; i++) x = mdouble_x(x);
-        den_inv[r] = minv(x);
-    }
+    trace_vanishing_inverses(c, n, den_inv);
     if (poseidon || xr) {
         // compositionEval (poseidon.zig:562-569: (log_n_instances, 158, 1264, 1); xor.zig:590-597:
         // (log_size, log_step, offset, 1)) times the constraint's coefficient 1

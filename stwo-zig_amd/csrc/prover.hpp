@@ -145,6 +145,36 @@ struct ProveParams;
 void finish_proof(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, std::string& proof,
                   const std::function<void()>& mark);
 
+// Gather plan: absolute device word addresses, downloaded with one kernel.
+struct Gather {
+    std::vector<uint64_t> addr;
+    size_t add_words(const uint32_t* p, uint64_t n) {
+        size_t at = addr.size();
+        for (uint64_t i = 0; i < n; i++) addr.push_back((uint64_t)(uintptr_t)(p + i));
+        return at;
+    }
+    size_t add_hash(const DeviceTree& t, uint32_t layer, uint64_t idx) {
+        return add_words((const uint32_t*)(t.layer(layer) + idx * 32), 8);
+    }
+};
+// The phases of finish_proof: the last layer and its mix; with fp.nonce set, the nonce mix, the
+// queries and the gather plan (appended to g); the gather; the wire encoding from the words.
+struct FinishPlan {
+    struct LayerPlan {
+        std::vector<size_t> wit_at, hw_at;
+    };
+    std::vector<qm31> lastp;
+    uint64_t nonce = 0;
+    std::vector<uint64_t> qs;
+    std::vector<size_t> qv_at;
+    std::vector<std::vector<size_t>> tw_at;
+    std::vector<LayerPlan> lplans;
+};
+void finish_last_layer(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, FinishPlan& fp);
+void finish_queries(Ctx& c, Channel& ch, const ProveParams& prm, ProofTail& t, FinishPlan& fp, Gather& g);
+void gather_words(Ctx& c, const Gather& g, std::vector<uint32_t>& gw);
+std::string finish_encode(const ProveParams& prm, const ProofTail& t, const FinishPlan& fp, const uint32_t* gw);
+
 // Components served by the one-trace-tree proof core: the wide-Fibonacci AIR and the Poseidon
 // AIR (examples/poseidon.zig:397-504: 1264 columns, a constant composition evaluation).
 // The periodic XOR example AIR (examples/xor.zig:419-605) adds a preprocessed tree of two
@@ -165,6 +195,21 @@ struct ProveParams {
     uint32_t xor_log_step = 0;          // XOR statement (log_size is log_n_rows)
     uint64_t xor_offset = 0;
 };
+
+void trace_vanishing_inverses(Ctx& c, uint32_t n, uint32_t den_inv[2]);
+
+// Many wide-Fibonacci proofs of one shape and PCS configuration in single launches
+// (batch_prover.hip): proof k uses prm with trace_seed = seeds[k] (seeds may be NULL: every
+// proof uses prm.trace_seed).  Requires n + log_blowup <= TREE_LAYER_MAX_LOG + 1 and
+// log_last_layer_degree_bound + log_blowup <= FRI_TAIL_MAX_LOG.
+struct BatchStats {
+    double device_ms = 0;  // first launch to the readback of the FRI state
+    double tail_ms = 0;    // last layers, proof of work, gather and encoding of every proof
+    double total_ms = 0;
+    uint64_t slice_bytes = 0;  // device bytes per proof
+};
+void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t n_proofs, const uint64_t* seeds,
+                                std::vector<std::string>& proofs, BatchStats* stats);
 
 constexpr int STAGES = 8;
 struct ProveStats {
@@ -259,8 +304,9 @@ class BatchedPipeline {
 
 // Transcript steps on the device channel between commitments (transcript.hip).
 // mixRoot(root) [+ mixU32s(words[0..n_words))], alpha = drawSecureFelt, powers[k] = alpha^k.
+// (batch, bstride: one workgroup per proof, its buffers bstride bytes apart; see launch_quotients_dev)
 void launch_tr_commit_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, const uint32_t* words, uint32_t n_words,
-                           uint32_t* d_powers, uint32_t n_pow);
+                           uint32_t* d_powers, uint32_t n_pow, uint32_t batch = 1, uint64_t bstride = 0);
 // digest <- H(digest || words[0..n_words)) if n_words > 0 or mix_empty (words in device
 // memory), then n_draws secure felts into d_out (4 words each).
 void launch_tr_mix_draws(Ctx& c, DevChannel* d_ch, const uint32_t* d_words, uint32_t n_words, uint32_t n_draws,
@@ -269,11 +315,11 @@ void launch_tr_mix_draws(Ctx& c, DevChannel* d_ch, const uint32_t* d_words, uint
 // n_blocks draws of 8 raw words each (drawU32s) into d_out (8 n_blocks words).
 void launch_tr_draw_words(Ctx& c, DevChannel* d_ch, uint32_t n_blocks, uint32_t* d_out);
 void launch_tr_oods(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, uint32_t n_factors, uint32_t* d_oods,
-                    uint32_t* d_factors);
+                    uint32_t* d_factors, uint32_t batch = 1, uint64_t bstride = 0);
 // mixFelts(sampled values), q = drawSecureFelt, coefficients of the entries (column order)
 // and the single batch at the OODS point (entries' col/shift set by the caller).
 void launch_tr_quotient(Ctx& c, DevChannel* d_ch, const uint32_t* d_sv, uint32_t ns, const uint32_t* d_oods,
-                        QEntryDev* d_ents, QuotientBatchDev* d_bat);
+                        QEntryDev* d_ents, QuotientBatchDev* d_bat, uint32_t batch = 1, uint64_t bstride = 0);
 
 // Continuous-batching proof service (serving.hip).
 struct ServeJob {

@@ -237,14 +237,16 @@ LeafPlan merkle_leaf_plan(const uint32_t* const* h_cols, const uint32_t* h_logs,
 // Intermediate segment states live in t's two parent layers below the leaves until the
 // parents are built.
 void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, const std::vector<LeafSeg>& segs, uint32_t n_cols,
-                              DeviceTree& t);
+                              DeviceTree& t, uint32_t batch = 1, uint64_t bstride = 0);
 // Host arrays in: column device pointers and log sizes, sorted by log size; max log = t.max_log.
 void launch_merkle_leaves(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_logs, uint32_t n_cols,
                           DeviceTree& t);
 void launch_merkle_leaves_qm31(Ctx& c, const uint32_t* d_qm31, uint32_t log_size, uint8_t* d_leaves);
 // All parent layers above t's leaves.  With d_ch, the launch reaching the root
 // also mixes it into the device channel and draws the next alpha (root copied to d_root_out).
-void launch_merkle_parents(Ctx& c, DeviceTree& t, DevChannel* d_ch = nullptr, uint8_t* d_root_out = nullptr);
+// Batches (batch > 1) bring their own completion counters: d_cnt, TREE_CNT_WORDS per proof, zeroed.
+void launch_merkle_parents(Ctx& c, DeviceTree& t, DevChannel* d_ch = nullptr, uint8_t* d_root_out = nullptr,
+                           uint32_t batch = 1, uint64_t bstride = 0, uint32_t* d_cnt = nullptr);
 void launch_merkle_parent_layer(Ctx& c, const uint8_t* d_children, uint32_t log_size, uint8_t* d_parents);
 // FRI folds (folding.zig): line domain halfOdds(log_src) / circle domain CanonicCoset(log_src).
 // d_alpha (optional): read alpha (and alpha^2) from a DevChannel in HBM instead of `alpha`.
@@ -253,7 +255,7 @@ void launch_fold_line(Ctx& c, const uint32_t* d_src, uint32_t log_src, qm31 alph
 // `coords` is a host array of 4 device pointers.
 void launch_fold_circle_into_line(Ctx& c, const uint32_t* const* coords, uint32_t log_src, qm31 alpha,
                                   uint32_t* d_dst, bool accumulate, uint8_t* d_leaf_out,
-                                  const DevChannel* d_alpha = nullptr);
+                                  const DevChannel* d_alpha = nullptr, uint32_t batch = 1, uint64_t bstride = 0);
 // digest <- H(digest || root); alpha <- drawSecureFelt; root copied to d_root_out (optional).
 void launch_chan_mix_root_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, uint8_t* d_root_out);
 // One tree reduction per launch from level k0 (tree_layer_kernel, kernels_fri.hip), for
@@ -265,6 +267,7 @@ constexpr uint32_t TREE_CNT_GROUPS = 16;  // completion sub-counters, then the t
 constexpr uint32_t TREE_CNT_STRIDE = 32;  // words between counters (a cache line each)
 constexpr uint32_t TREE_XR_OFFSET = TREE_CNT_STRIDE * (TREE_CNT_GROUPS + 1);  // root exchange after them
 constexpr uint32_t TREE_MAX_BLOCKS = 256;
+constexpr uint32_t TREE_CNT_WORDS = TREE_XR_OFFSET + 8 * TREE_MAX_BLOCKS;  // counters + exchange of one tree launch
 enum FriSrc { FRI_SRC_HASHES = 0, FRI_SRC_LINE = 1, FRI_SRC_CIRCLE = 2 };
 struct FriLayerArgs {
     const uint4* src_line;      // FRI_SRC_LINE: the previous line (2^(k0+1) QM31 values)
@@ -278,16 +281,21 @@ struct FriLayerArgs {
     uint32_t* cnt;              // completion counters (set by the launcher)
     uint32_t* xr;               // subtree root exchange, 8 words per workgroup (set by the launcher)
     B2State leaf_seed, node_seed;  // set by the launcher
+    uint64_t bstride;           // batch: proof blockIdx.y's buffers bstride bytes after proof 0's,
+                                // its counters and exchange TREE_CNT_WORDS words after
 };
 // Levels k0-1 .. 0 of `t` from its hashes at level k0; with d_ch, the channel step on the root.
-void launch_tree_top(Ctx& c, DeviceTree& t, uint32_t k0, DevChannel* d_ch, uint8_t* d_root_out);
+void launch_tree_top(Ctx& c, DeviceTree& t, uint32_t k0, DevChannel* d_ch, uint8_t* d_root_out, uint32_t batch = 1,
+                     uint64_t bstride = 0, uint32_t* d_cnt = nullptr);
 // One FRI inner layer: fold the line of log_src (or the 4 circle coordinates of log_src) with
 // the alpha in d_ch into d_dst, commit it into `t` (max_log = log_src - 1) and run the channel
 // step on its root (root copied to d_root_out).
 void launch_fri_commit_line(Ctx& c, const uint32_t* d_src, uint32_t log_src, uint32_t* d_dst, DeviceTree& t,
-                            DevChannel* d_ch, uint8_t* d_root_out);
+                            DevChannel* d_ch, uint8_t* d_root_out, uint32_t batch = 1, uint64_t bstride = 0,
+                            uint32_t* d_cnt = nullptr);
 void launch_fri_commit_circle(Ctx& c, const uint32_t* const* coords, uint32_t log_src, uint32_t* d_dst,
-                              DeviceTree& t, DevChannel* d_ch, uint8_t* d_root_out);
+                              DeviceTree& t, DevChannel* d_ch, uint8_t* d_root_out, uint32_t batch = 1,
+                              uint64_t bstride = 0, uint32_t* d_cnt = nullptr);
 // FRI inner layers below the committed line top_log (<= FRI_TAIL_MAX_LOG + 1, its alpha in
 // ch) in one workgroup.  Out: lines top_log-1 .. last_log; for each log l in
 // (last_log, top_log) the tree (every layer), its root copy (roots[l]) and the channel step.
@@ -300,8 +308,9 @@ struct FriTailArgs {
     DevChannel* ch;
     const uint32_t* itw_half;  // inverse tower + 2^(M-1) (set by the launcher)
     B2State leaf_seed, node_seed;  // set by the launcher
+    uint64_t bstride;              // batch: proof blockIdx.x's buffers bstride bytes after proof 0's
 };
-void launch_fri_tail(Ctx& c, FriTailArgs& a);
+void launch_fri_tail(Ctx& c, FriTailArgs& a, uint32_t batch = 1);
 
 // LogUp / GKR (kernels_logup.hip).  Layer kinds of gkr_circuit.zig:79-92.
 enum GkrKind { GKR_GRAND_PRODUCT = 0, GKR_LOGUP_GENERIC = 1, GKR_LOGUP_MULTIPLICITIES = 2, GKR_LOGUP_SINGLES = 3 };
@@ -367,13 +376,20 @@ void launch_quotients(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_c
                       uint8_t* d_leaf_out);
 // The same with the entries and batches already in device memory (in_bytes: algorithmic
 // bytes read, for the profile only).
+//
+// Batches of proofs (the launchers' `batch`, `bstride` arguments): `batch` proofs of one shape in
+// one launch (grid dimension y, or z where y is taken), proof k's buffers -- every per-proof
+// pointer argument and every column pointer in a table -- at k * bstride bytes past proof 0's.
+// Shared read-only data (twiddles, domain tables, seeds) is not offset.
 void launch_quotients_dev(Ctx& c, const QEntryDev* d_ent, uint32_t n_entries, const QuotientBatchDev* d_bat,
                           uint32_t n_batches, double in_bytes, uint32_t log_size, uint32_t* const* out_coords,
-                          uint8_t* d_leaf_out);
+                          uint8_t* d_leaf_out, uint32_t batch = 1, uint64_t bstride = 0);
 // Point evaluation: sum_i coeff_i * w_i with weights of `point` (2^log).
 void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w);
 // The same with the log_size factors (QM31) in device memory.
-void launch_eval_weights_dev(Ctx& c, const uint32_t* d_factors, uint32_t log_size, uint32_t* d_w);
+// d_tab: the factor tables (eval_table_bytes, per proof of a batch), or NULL for a temporary.
+void launch_eval_weights_dev(Ctx& c, const uint32_t* d_factors, uint32_t log_size, uint32_t* d_w, uint32_t batch = 1,
+                             uint64_t bstride = 0, uint32_t* d_tab = nullptr);
 void launch_eval_columns(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
                          const uint32_t* d_w, uint32_t* d_out /* n_cols * 4 words */);
 // The same two steps with caller-provided scratch: the factor tables (eval_table_bytes) and
@@ -382,13 +398,17 @@ uint64_t eval_table_bytes(uint32_t log_size);
 uint64_t eval_partial_bytes(uint32_t n_cols, uint32_t log_size);
 void launch_eval_weights_into(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w, uint32_t* d_tab);
 void launch_eval_columns_into(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_size,
-                              const uint32_t* d_w, uint32_t* d_out, unsigned long long* d_part);
+                              const uint32_t* d_w, uint32_t* d_out, unsigned long long* d_part, uint32_t batch = 1,
+                              uint64_t bstride = 0);
 // Wide Fibonacci.
+// d_seeds (device, one per proof of a batch) replaces `seed` when non-NULL.
 void launch_widefib_trace(Ctx& c, uint32_t log_n, uint32_t seq_len, uint32_t mode, uint64_t seed, uint32_t* d_out,
-                          uint64_t col_stride);
+                          uint64_t col_stride, uint32_t batch = 1, const uint64_t* d_seeds = nullptr,
+                          uint64_t bstride = 0);
 void launch_widefib_composition(Ctx& c, const uint32_t* const* d_cols, uint32_t n_cols, uint32_t log_rows_eval,
                                 uint32_t log_n_rows, const uint32_t* d_powers, uint32_t den_inv0,
-                                uint32_t den_inv1, uint32_t* const* out_coords /* host array of 4 */);
+                                uint32_t den_inv1, uint32_t* const* out_coords /* host array of 4 */,
+                                uint32_t batch = 1, uint64_t bstride = 0);
 // Poseidon AIR trace: 1264 columns of 2^log_n rows at d_out + column * col_stride.
 void launch_poseidon_trace(Ctx& c, uint32_t log_n, uint32_t* d_out, uint64_t col_stride);
 // Periodic XOR example AIR: main column and the IsFirst / IsStepWithOffset indicators.
@@ -398,6 +418,10 @@ void launch_xor_example_columns(Ctx& c, uint32_t log_size, uint32_t log_step, ui
 void launch_constant_composition(Ctx& c, uint32_t log_eval, const uint32_t v[4], uint32_t* const* out_coords);
 // PoW: lowest nonce in [base, base+count) or UINT64_MAX.
 uint64_t launch_pow_grind(Ctx& c, const uint8_t prefix[32], uint32_t bits, uint64_t base, uint64_t count);
+// The same for `batch` independent prefixes (d_prefix: 8 words each) over one nonce range:
+// d_best[k] (initialised by the caller to UINT64_MAX) receives prefix k's lowest valid nonce.
+void launch_pow_grind_batch(Ctx& c, const uint32_t* d_prefix, uint32_t batch, uint32_t bits, uint64_t base,
+                            uint64_t count, unsigned long long* d_best);
 // Gather words: out[i] = word at absolute device address d_addr[i].
 void launch_gather(Ctx& c, const uint64_t* d_addr, uint64_t n, uint32_t* d_out);
 void launch_tower(Ctx& c, uint32_t log, uint32_t* d_tw, uint32_t* d_itw, uint32_t* d_tw2, uint32_t* d_itw2);

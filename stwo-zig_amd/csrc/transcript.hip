@@ -53,6 +53,16 @@ __device__ __forceinline__ qm31 tr_pow(qm31 x, uint32_t e) {
     return r;
 }
 
+// Proof blockIdx.y of a batch: its buffers bstride bytes after proof 0's.
+template <class T>
+__device__ __forceinline__ T* tr_boff(T* p, uint64_t bstride) {
+    return p ? (T*)((uint8_t*)p + (uint64_t)blockIdx.y * bstride) : p;
+}
+template <class T>
+__device__ __forceinline__ const T* tr_boff(const T* p, uint64_t bstride) {
+    return p ? (const T*)((const uint8_t*)p + (uint64_t)blockIdx.y * bstride) : p;
+}
+
 __device__ __forceinline__ void tr_load_dg(const DevChannel* ch, uint32_t dg[8]) {
 #pragma unroll
     for (int k = 0; k < 8; k++) dg[k] = ch->digest[k];
@@ -66,9 +76,13 @@ __device__ __forceinline__ void tr_store(DevChannel* ch, const uint32_t dg[8], u
 
 __global__ __launch_bounds__(256) void tr_commit_draw_kernel(DevChannel* __restrict__ ch, const uint32_t* __restrict__ root,
                                                              uint32_t w0, uint32_t w1, uint32_t n_words,
-                                                             uint4* __restrict__ powers, uint32_t n_pow) {
+                                                             uint4* __restrict__ powers, uint32_t n_pow,
+                                                             uint64_t bstride) {
     __shared__ uint32_t a_sh[4];
     const uint32_t t = threadIdx.x;
+    ch = tr_boff(ch, bstride);
+    root = tr_boff(root, bstride);
+    powers = tr_boff(powers, bstride);
     if (t < 4) {
         const QuadLane Lq(t);
         uint32_t dg[8], pay[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -98,9 +112,13 @@ __global__ __launch_bounds__(256) void tr_commit_draw_kernel(DevChannel* __restr
 
 // root == nullptr: the point is drawn without a mix first (the draw counter continues).
 __global__ void tr_oods_kernel(DevChannel* __restrict__ ch, const uint32_t* __restrict__ root, uint32_t n_factors,
-                               uint4* __restrict__ oods, uint4* __restrict__ factors) {
+                               uint4* __restrict__ oods, uint4* __restrict__ factors, uint64_t bstride) {
     const uint32_t t = threadIdx.x;
     if (t >= 4) return;
+    ch = tr_boff(ch, bstride);
+    root = tr_boff(root, bstride);
+    oods = tr_boff(oods, bstride);
+    factors = tr_boff(factors, bstride);
     const QuadLane Lq(t);
     uint32_t dg[8], pay[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     tr_load_dg(ch, dg);
@@ -182,11 +200,17 @@ __global__ __launch_bounds__(TRQ_THREADS) void tr_quotient_kernel(DevChannel* __
                                                                   const uint32_t* __restrict__ sv, uint32_t ns,
                                                                   const uint4* __restrict__ oods,
                                                                   QEntryDev* __restrict__ ents,
-                                                                  QuotientBatchDev* __restrict__ bat) {
+                                                                  QuotientBatchDev* __restrict__ bat,
+                                                                  uint64_t bstride) {
     __shared__ uint32_t q_sh[4];
     __shared__ qm31 ra[TRQ_THREADS], rb[TRQ_THREADS];
     __shared__ uint32_t svs[4 * TRQ_MAX_NS];
     const uint32_t t = threadIdx.x;
+    ch = tr_boff(ch, bstride);
+    sv = tr_boff(sv, bstride);
+    oods = tr_boff(oods, bstride);
+    ents = tr_boff(ents, bstride);
+    bat = tr_boff(bat, bstride);
     // the hash reads its message from LDS: one load round trip instead of one per block
     for (uint32_t k = t; k < 4 * ns; k += TRQ_THREADS) svs[k] = sv[k];
     __syncthreads();
@@ -244,11 +268,13 @@ __global__ __launch_bounds__(TRQ_THREADS) void tr_quotient_kernel(DevChannel* __
 }  // namespace
 
 void launch_tr_commit_draw(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, const uint32_t* words, uint32_t n_words,
-                           uint32_t* d_powers, uint32_t n_pow) {
+                           uint32_t* d_powers, uint32_t n_pow, uint32_t batch, uint64_t bstride) {
     SR_CHECK(n_words <= 2, "transcript: at most 2 statement words");
+    SR_CHECK(batch >= 1 && batch <= 65535, "transcript: batch size");
     KTimer kt(c, "transcript", 0.0);
-    hipLaunchKernelGGL(tr_commit_draw_kernel, dim3(1), dim3(256), 0, c.stream, d_ch, (const uint32_t*)d_root,
-                       n_words > 0 ? words[0] : 0u, n_words > 1 ? words[1] : 0u, n_words, (uint4*)d_powers, n_pow);
+    hipLaunchKernelGGL(tr_commit_draw_kernel, dim3(1, batch), dim3(256), 0, c.stream, d_ch, (const uint32_t*)d_root,
+                       n_words > 0 ? words[0] : 0u, n_words > 1 ? words[1] : 0u, n_words, (uint4*)d_powers, n_pow,
+                       bstride);
     SR_HIP(hipGetLastError());
 }
 
@@ -270,20 +296,22 @@ void launch_tr_draw_words(Ctx& c, DevChannel* d_ch, uint32_t n_blocks, uint32_t*
 }
 
 void launch_tr_oods(Ctx& c, DevChannel* d_ch, const uint8_t* d_root, uint32_t n_factors, uint32_t* d_oods,
-                    uint32_t* d_factors) {
+                    uint32_t* d_factors, uint32_t batch, uint64_t bstride) {
     SR_CHECK(n_factors <= 30, "transcript: too many evaluation factors");
+    SR_CHECK(batch >= 1 && batch <= 65535, "transcript: batch size");
     KTimer kt(c, "transcript", 0.0);
-    hipLaunchKernelGGL(tr_oods_kernel, dim3(1), dim3(64), 0, c.stream, d_ch, (const uint32_t*)d_root, n_factors,
-                       (uint4*)d_oods, (uint4*)d_factors);
+    hipLaunchKernelGGL(tr_oods_kernel, dim3(1, batch), dim3(64), 0, c.stream, d_ch, (const uint32_t*)d_root, n_factors,
+                       (uint4*)d_oods, (uint4*)d_factors, bstride);
     SR_HIP(hipGetLastError());
 }
 
 void launch_tr_quotient(Ctx& c, DevChannel* d_ch, const uint32_t* d_sv, uint32_t ns, const uint32_t* d_oods,
-                        QEntryDev* d_ents, QuotientBatchDev* d_bat) {
+                        QEntryDev* d_ents, QuotientBatchDev* d_bat, uint32_t batch, uint64_t bstride) {
     SR_CHECK(ns >= 1 && ns <= TRQ_MAX_NS, "transcript: sampled value count");
-    KTimer kt(c, "transcript", 16.0 * ns);
-    hipLaunchKernelGGL(tr_quotient_kernel, dim3(1), dim3(TRQ_THREADS), 0, c.stream, d_ch, d_sv, ns,
-                       (const uint4*)d_oods, d_ents, d_bat);
+    SR_CHECK(batch >= 1 && batch <= 65535, "transcript: batch size");
+    KTimer kt(c, "transcript", batch * 16.0 * ns);
+    hipLaunchKernelGGL(tr_quotient_kernel, dim3(1, batch), dim3(TRQ_THREADS), 0, c.stream, d_ch, d_sv, ns,
+                       (const uint4*)d_oods, d_ents, d_bat, bstride);
     SR_HIP(hipGetLastError());
 }
 

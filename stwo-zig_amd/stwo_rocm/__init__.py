@@ -150,6 +150,8 @@ _SIGNATURES = [
     ("stwo_rocm_pow_grind", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]),
     ("stwo_rocm_prove_xor", ctypes.c_int, [_vp, ctypes.POINTER(XorParams), ctypes.c_char_p, ctypes.c_uint64,
                                             ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ProveStats)]),
+    ("stwo_rocm_prove_wide_fibonacci_batch", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint64,
+                                                             _vp, _vp]),
     ("stwo_rocm_prove_poseidon", ctypes.c_int, [_vp, ctypes.POINTER(PoseidonParams), ctypes.c_char_p,
                                                  ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
                                                  ctypes.POINTER(ProveStats)]),
@@ -641,6 +643,35 @@ class Context:
             return buf.raw[: n.value], (st.as_dict() if st is not None else None)
 
 
+    def prove_wide_fibonacci_batch(self, log_n_rows, sequence_len, seeds=None, n_proofs=None, pow_bits=10,
+                                   log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3, trace_mode=1,
+                                   trace_seed=0):
+        """Many wide-Fibonacci proofs of one shape in single launches: proof k uses seeds[k] (or
+        trace_seed for all n_proofs when seeds is None).  Returns (list of proof bytes, stats dict)."""
+        if seeds is not None:
+            seeds = [int(x) for x in seeds]
+            n_proofs = len(seeds)
+            arr = (ctypes.c_uint64 * n_proofs)(*seeds)
+        else:
+            arr = None
+        prm = WideFibParams(log_n_rows, sequence_len, pow_bits, log_blowup_factor, log_last_layer_degree_bound,
+                            n_queries, trace_mode, trace_seed)
+        offs = (ctypes.c_uint64 * (n_proofs + 1))()
+        st = BatchStats()
+        cap = n_proofs * (64 << 10)
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = self.lib.stwo_rocm_prove_wide_fibonacci_batch(self.h, ctypes.byref(prm), n_proofs, arr, buf, cap,
+                                                               offs, ctypes.byref(st))
+            if rc == 1 and offs[n_proofs] > cap:
+                cap = offs[n_proofs]
+                continue
+            self._check(rc)
+            raw = buf.raw
+            proofs = [raw[offs[k]:offs[k + 1]] for k in range(n_proofs)]
+            return proofs, {"device_ms": st.device_ms, "tail_ms": st.tail_ms, "total_ms": st.total_ms,
+                            "slice_bytes": st.slice_bytes}
+
     def prove_poseidon(self, log_n_instances, pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0,
                        n_queries=3, stats=True):
         """Poseidon AIR proof (examples/poseidon.zig:67-73): (proof bytes, stage timings or None)."""
@@ -824,6 +855,11 @@ class BatchedPipeline:
         if self.h:
             self.ctx._check(self.ctx.lib.stwo_rocm_batched_destroy(self.ctx.h, self.h))
             self.h = None
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("device_ms", ctypes.c_double), ("tail_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("slice_bytes", ctypes.c_uint64)]
 
 
 class RuntimeInfo(ctypes.Structure):

@@ -1,0 +1,95 @@
+"""Many small wide-Fibonacci proofs in single launches (stwo_rocm_prove_wide_fibonacci_batch).
+
+Every proof of a batch must be byte-identical to the one-proof path's proof of the same
+statement, trace seed and PCS configuration (itself pinned to the reference's golden proofs and
+to the oracle), and a sample of them to the oracle directly.  The cases cover the batch's
+branches: the fused interpolation + extension (blowup 1) and the two-launch form (blowup 2),
+trees that start with wide parent launches (n + blowup = 17), a FRI with no inner layer, the
+one-workgroup tail, the quotient split for more than 128 columns, proof of work from 0 to 26
+bits, and the reference's own (1, i) trace against its golden proof."""
+import hashlib
+import json
+import os
+
+import pytest
+
+import oracle
+import stark_verifier as sv
+import stwo_rocm
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# (log_n_rows, sequence_len, seeds, pcs) -- pcs: pow_bits, log_blowup_factor, last-layer bound, queries
+CASES = [
+    (6, 4, [1, 2, 3, 4, 5], dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3)),
+    (8, 12, list(range(100, 164)), dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0,
+                                         n_queries=3)),
+    (7, 6, [9, 10, 11], dict(pow_bits=0, log_blowup_factor=2, log_last_layer_degree_bound=1, n_queries=20)),
+    (2, 3, [5, 6], dict(pow_bits=4, log_blowup_factor=1, log_last_layer_degree_bound=1, n_queries=4)),
+    (16, 3, [21, 22], dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3)),
+    (9, 130, [31, 32, 33], dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=2, n_queries=8)),
+    (11, 16, [41, 42], dict(pow_bits=26, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=70)),
+]
+IDS = ["6x4-5", "8x12-64", "7x6-blowup2", "2x3-no-inner", "16x3-wide-tree", "9x130-split", "11x16-secure"]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    with stwo_rocm.Context(0) as c:
+        yield c
+
+
+@pytest.mark.parametrize("n,w,seeds,pcs", CASES, ids=IDS)
+def test_batch_equals_one_proof_path(ctx, n, w, seeds, pcs):
+    print("batch %dx%d K=%d %s" % (n, w, len(seeds), pcs), flush=True)
+    proofs, st = ctx.prove_wide_fibonacci_batch(n, w, seeds=seeds, trace_mode=1, **pcs)
+    assert len(proofs) == len(seeds) and st["slice_bytes"] > 0
+    for k, s in enumerate(seeds):
+        one, _ = ctx.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=s, stats=False, **pcs)
+        assert proofs[k] == one, "proof %d (seed %d) differs from the one-proof path" % (k, s)
+    # distinct seeds give distinct proofs
+    assert len({hashlib.sha256(p).hexdigest() for p in proofs}) == len(seeds)
+    # the first and last against the oracle and the verifier
+    for k in {0, len(seeds) - 1}:
+        if n <= 11 and pcs["pow_bits"] <= 12:
+            assert proofs[k] == oracle.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=seeds[k], **pcs)
+        sv.verify_proof(proofs[k], "wide_fibonacci", {"log_n_rows": n, "sequence_len": w}, pcs)
+
+
+def test_batch_reference_trace_is_the_golden_proof(ctx):
+    idx = json.load(open(os.path.join(GOLD, "wide_fibonacci_goldens.json")))
+    e = idx["wide_fibonacci_log10_seq8_pow10"]
+    golden = open(os.path.join(GOLD, e["file"]), "rb").read()
+    proofs, _ = ctx.prove_wide_fibonacci_batch(10, 8, n_proofs=4, trace_mode=0, pow_bits=10, log_blowup_factor=1,
+                                               log_last_layer_degree_bound=0, n_queries=3)
+    assert all(p == golden for p in proofs)
+
+
+def test_batch_runs_back_to_back(ctx):
+    """Two batches of different shapes in a row on one context, then the first again: the
+    completion counters and cached blocks of one batch leave the next unaffected."""
+    pcs = dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3)
+    a1, _ = ctx.prove_wide_fibonacci_batch(9, 8, seeds=[1, 2, 3], **pcs)
+    b, _ = ctx.prove_wide_fibonacci_batch(12, 5, seeds=[7] * 9, **pcs)
+    a2, _ = ctx.prove_wide_fibonacci_batch(9, 8, seeds=[1, 2, 3], **pcs)
+    assert a1 == a2
+    assert len(set(b)) == 1
+    one, _ = ctx.prove_wide_fibonacci(12, 5, trace_mode=1, trace_seed=7, stats=False, **pcs)
+    assert b[0] == one
+
+
+def test_batch_rejects_unsupported_shapes(ctx):
+    pcs = dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3)
+    with pytest.raises(stwo_rocm.RocmError):
+        ctx.prove_wide_fibonacci_batch(17, 8, seeds=[1], **pcs)  # n + blowup = 18
+    with pytest.raises(stwo_rocm.RocmError):
+        ctx.prove_wide_fibonacci_batch(12, 8, seeds=[1], pow_bits=10, log_blowup_factor=1,
+                                       log_last_layer_degree_bound=10, n_queries=3)  # last layer 2^11
+    with pytest.raises(stwo_rocm.RocmError):
+        ctx.prove_wide_fibonacci_batch(8, 2, seeds=[1], **pcs)  # sequence_len < 3
+    # the context stays usable
+    p, _ = ctx.prove_wide_fibonacci_batch(8, 4, seeds=[3], **pcs)
+    one, _ = ctx.prove_wide_fibonacci(8, 4, trace_mode=1, trace_seed=3, stats=False, **pcs)
+    assert p[0] == one

@@ -56,13 +56,14 @@ def main():
                     help="also run the serving comparison (256 requests, 1 slot vs 16 slots at equal p99)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain streaming kernel of known size (HBM counter calibration)")
-    ap.add_argument("--workload", choices=["widefib", "logup", "batched", "serving", "poseidon", "xor"],
+    ap.add_argument("--workload", choices=["widefib", "logup", "batched", "serving", "poseidon", "xor", "many"],
                     default="widefib",
                     help="widefib: the 2^20x100 proof (default; with the batched and serving parts); logup: "
                          "XOR-lookup LogUp workload at 2^18; batched: 64 uneven columns LDE + Merkle + 3 FRI folds "
                          "as one hipGraph; serving: 256 Poisson requests (wide-Fib, XOR, Poseidon, XOR-lookup AIR), 1 slot "
                          "vs continuous batching; poseidon, xor: the reference's example AIRs at the size of their "
-                         "golden proofs")
+                         "golden proofs; many: batches of small wide-Fib proofs in single launches vs one proof "
+                         "per call")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the backend has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -89,6 +90,9 @@ def main():
         return
     if args.workload in ("poseidon", "xor"):
         bench_example_air(stwo_rocm, args)
+        return
+    if args.workload == "many":
+        bench_many(stwo_rocm, args)
         return
     bench_widefib(stwo_rocm, args)
 
@@ -258,6 +262,9 @@ def bench_widefib(stwo_rocm, args):
     if args.batched == "on":
         line["batched"] = batched_part(stwo_rocm, ctx, steps=10, warmup=2, check_oracle=args.cpu_baseline == "on")
         print(json.dumps({"batched": line["batched"]}), file=sys.stderr, flush=True)
+    if args.batched == "on":
+        line["many_small_proofs"] = many_proofs_part(stwo_rocm, ctx, check_oracle=args.cpu_baseline == "on")
+        ctx.release_cached()
     if args.serving == "on":
         ctx.release_cached()
         line["serving"] = serving_part(stwo_rocm, ctx, check_oracle=args.cpu_baseline == "on")
@@ -543,6 +550,71 @@ def bench_example_air(stwo_rocm, args):
         line["speedup_vs_cpu_baseline"] = round(cpu["value"] / median_ms, 1)
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+MANY_SHAPES = ((10, 8, 256), (14, 32, 64))
+
+
+def many_proofs_part(stwo_rocm, ctx, check_oracle, shapes=MANY_SHAPES, reps=3):
+    """Many small proofs in single launches against the one-at-a-time GPU path: for each shape, K
+    seeded-uniform wide-Fibonacci proofs (functional protocol) proven one call per proof and as one
+    batch (stwo_rocm_prove_wide_fibonacci_batch); proofs/s of both by wall clock, the batch's
+    device and host-tail times, and the check that the batch's proofs are the one-at-a-time proofs
+    (with check_oracle, the first and last also the oracle's)."""
+    pcs = dict(stwo_rocm.PROTOCOLS["functional"])
+    out = {}
+    for n, w, k in shapes:
+        seeds = [1000 * n + w + i for i in range(k)]
+        ctx.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=seeds[0], stats=False, **pcs)
+        ctx.prove_wide_fibonacci_batch(n, w, seeds=seeds, trace_mode=1, **pcs)
+        one_s, ones = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ones = [ctx.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=sd, stats=False, **pcs)[0] for sd in seeds]
+            one_s.append(time.perf_counter() - t0)
+        bat_s, stats_all, proofs = [], [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            proofs, st = ctx.prove_wide_fibonacci_batch(n, w, seeds=seeds, trace_mode=1, **pcs)
+            bat_s.append(time.perf_counter() - t0)
+            stats_all.append(st)
+        if proofs != ones:
+            raise SystemExit("batched proofs differ from the one-at-a-time proofs (%dx%d)" % (n, w))
+        ok = None
+        if check_oracle:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle
+            ok = all(proofs[i] == oracle.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=seeds[i], **pcs)
+                     for i in (0, k - 1))
+            if not ok:
+                raise SystemExit("batched proofs differ from the oracle (%dx%d)" % (n, w))
+        one_ps, bat_ps = k / statistics.median(one_s), k / statistics.median(bat_s)
+        out["%dx%d" % (n, w)] = {
+            "proofs": k, "one_at_a_time_proofs_per_s": round(one_ps, 1), "batched_proofs_per_s": round(bat_ps, 1),
+            "speedup": round(bat_ps / one_ps, 2), "batch_wall_ms": round(statistics.median(bat_s) * 1e3, 3),
+            "batch_device_ms": round(statistics.median(x["device_ms"] for x in stats_all), 3),
+            "batch_host_tail_ms": round(statistics.median(x["tail_ms"] for x in stats_all), 3),
+            "device_bytes_per_proof": stats_all[0]["slice_bytes"], "equal_to_one_at_a_time": True,
+            "oracle_first_last_equal": ok}
+        print(json.dumps({"many_small_proofs": {"%dx%d" % (n, w): out["%dx%d" % (n, w)]}}), file=sys.stderr,
+              flush=True)
+    return out
+
+
+def bench_many(stwo_rocm, args):
+    ctx = stwo_rocm.Context(0)
+    ident = ctx.identity()
+    part = many_proofs_part(stwo_rocm, ctx, args.cpu_baseline == "on")
+    ctx.close()
+    head = part["%dx%d" % MANY_SHAPES[0][:2]]
+    line = {"metric": "many small proofs in single launches: wide-Fib 2^10x8 proofs/s (batch of 256)",
+            "value": head["batched_proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": 3, "warmup": 1,
+            "ms_per_step": head["batch_wall_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u32 (exact M31/CM31/QM31)", "data": "synthetic: seeded-uniform traces",
+            "config": {"workload": "wide_fibonacci seeded-uniform, functional protocol; shapes %s" %
+                                   ", ".join("2^%dx%d (K=%d)" % s for s in MANY_SHAPES), "device": ident},
+            "cpu_baseline": None, "shapes": part}
+    print(json.dumps(line), flush=True)
 
 
 def serving_requests(n=256, rate_per_s=1500.0, seed=2024):

@@ -15,7 +15,11 @@
 // one the one-proof path produces from the same parameters.
 #include <string.h>
 
+#include <atomic>
 #include <chrono>
+#include <exception>
+#include <mutex>
+#include <thread>
 
 #include "prover.hpp"
 
@@ -26,6 +30,39 @@ namespace {
 double now_host_ms() {
     using namespace std::chrono;
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// f(0) .. f(n-1) on up to 16 host threads (the proofs' host phases are independent); the first
+// exception is rethrown after every thread has joined.
+template <class F>
+void parallel_for(uint32_t n, F&& f) {
+    const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const uint32_t T = std::min(std::min(hw, 16u), n);
+    if (T <= 1) {
+        for (uint32_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<uint32_t> next{0};
+    std::exception_ptr err;
+    std::mutex mu;
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < T; t++)
+        th.emplace_back([&] {
+            for (;;) {
+                const uint32_t i = next.fetch_add(1);
+                if (i >= n) return;
+                try {
+                    f(i);
+                } catch (...) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (!err) err = std::current_exception();
+                    next.store(n);
+                    return;
+                }
+            }
+        });
+    for (auto& x : th) x.join();
+    if (err) std::rethrow_exception(err);
 }
 
 struct SliceLayout {
@@ -321,7 +358,7 @@ void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, cons
         FinishPlan fp;
     };
     std::vector<PerProof> pp(K);
-    for (uint32_t k = 0; k < K; k++) {
+    parallel_for(K, [&](uint32_t k) {
         PerProof& P = pp[k];
         const uint8_t* r = rb.data() + (size_t)k * rb_bytes;
         DevChannel hc;
@@ -370,7 +407,7 @@ void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, cons
         const uint32_t* w = (const uint32_t*)(r + at_line);
         for (uint64_t i = 0; i < lsz; i++) T.last_values[i] = qload(w + 4 * i);
         finish_last_layer(c, P.ch, prm, T, P.fp);
-    }
+    });
 
     // ---- proof of work: every prefix over one nonce range per launch, in order
     if (prm.pow_bits > 0) {
@@ -399,12 +436,19 @@ void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, cons
     }
 
     // ---- queries and one gather of every decommitment
+    // (the plans on host threads, each into its own list; the lists concatenated for one launch)
+    std::vector<Gather> gk(K);
+    parallel_for(K, [&](uint32_t k) { finish_queries(c, pp[k].ch, prm, pp[k].tail, pp[k].fp, gk[k]); });
     Gather g;
-    for (uint32_t k = 0; k < K; k++) finish_queries(c, pp[k].ch, prm, pp[k].tail, pp[k].fp, g);
+    std::vector<size_t> gbase(K);
+    for (uint32_t k = 0; k < K; k++) {
+        gbase[k] = g.addr.size();
+        g.addr.insert(g.addr.end(), gk[k].addr.begin(), gk[k].addr.end());
+    }
     std::vector<uint32_t> gw;
     gather_words(c, g, gw);
     proofs.resize(K);
-    for (uint32_t k = 0; k < K; k++) proofs[k] = finish_encode(prm, pp[k].tail, pp[k].fp, gw.data());
+    parallel_for(K, [&](uint32_t k) { proofs[k] = finish_encode(prm, pp[k].tail, pp[k].fp, gw.data() + gbase[k]); });
     if (stats) {
         stats->device_ms = dev_ms;
         stats->tail_ms = now_host_ms() - t_dev;

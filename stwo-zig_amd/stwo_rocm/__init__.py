@@ -658,17 +658,26 @@ class Context:
                             n_queries, trace_mode, trace_seed)
         offs = (ctypes.c_uint64 * (n_proofs + 1))()
         st = BatchStats()
-        cap = n_proofs * (64 << 10)
+        # an upper bound of one proof's wire size (sampled values, commitments, hash witnesses
+        # of every tree and FRI layer, queried values, the last layer), kept in one buffer reused
+        # across calls
+        L = log_n_rows + log_blowup_factor
+        per = (8192 + 48 * (sequence_len + 8) + 12 * n_queries * (sequence_len + 8) +
+               n_queries * (L + 1) * (4 * 132 + 48) + 48 * (1 << log_last_layer_degree_bound))
+        cap = max(n_proofs * per, getattr(self, "_batch_cap", 0))
         while True:
-            buf = ctypes.create_string_buffer(cap)
+            if getattr(self, "_batch_cap", 0) < cap:
+                self._batch_buf = ctypes.create_string_buffer(cap)
+                self._batch_cap = cap
+            buf, cap = self._batch_buf, self._batch_cap
             rc = self.lib.stwo_rocm_prove_wide_fibonacci_batch(self.h, ctypes.byref(prm), n_proofs, arr, buf, cap,
                                                                offs, ctypes.byref(st))
             if rc == 1 and offs[n_proofs] > cap:
                 cap = offs[n_proofs]
                 continue
             self._check(rc)
-            raw = buf.raw
-            proofs = [raw[offs[k]:offs[k + 1]] for k in range(n_proofs)]
+            addr = ctypes.addressof(buf)
+            proofs = [ctypes.string_at(addr + offs[k], offs[k + 1] - offs[k]) for k in range(n_proofs)]
             return proofs, {"device_ms": st.device_ms, "tail_ms": st.tail_ms, "total_ms": st.total_ms,
                             "slice_bytes": st.slice_bytes}
 

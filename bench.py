@@ -555,7 +555,7 @@ def bench_example_air(stwo_rocm, args):
 MANY_SHAPES = ((10, 8, 256), (14, 32, 64))
 
 
-def many_proofs_part(stwo_rocm, ctx, check_oracle, shapes=MANY_SHAPES, reps=3):
+def many_proofs_part(stwo_rocm, ctx, check_oracle, shapes=MANY_SHAPES, reps=5):
     """Many small proofs in single launches against the one-at-a-time GPU path: for each shape, K
     seeded-uniform wide-Fibonacci proofs (functional protocol) proven one call per proof and as one
     batch (stwo_rocm_prove_wide_fibonacci_batch); proofs/s of both by wall clock, the batch's
@@ -566,7 +566,8 @@ def many_proofs_part(stwo_rocm, ctx, check_oracle, shapes=MANY_SHAPES, reps=3):
     for n, w, k in shapes:
         seeds = [1000 * n + w + i for i in range(k)]
         ctx.prove_wide_fibonacci(n, w, trace_mode=1, trace_seed=seeds[0], stats=False, **pcs)
-        ctx.prove_wide_fibonacci_batch(n, w, seeds=seeds, trace_mode=1, **pcs)
+        for _ in range(2):
+            ctx.prove_wide_fibonacci_batch(n, w, seeds=seeds, trace_mode=1, **pcs)
         one_s, ones = [], []
         for _ in range(reps):
             t0 = time.perf_counter()
@@ -594,6 +595,8 @@ def many_proofs_part(stwo_rocm, ctx, check_oracle, shapes=MANY_SHAPES, reps=3):
             "speedup": round(bat_ps / one_ps, 2), "batch_wall_ms": round(statistics.median(bat_s) * 1e3, 3),
             "batch_device_ms": round(statistics.median(x["device_ms"] for x in stats_all), 3),
             "batch_host_tail_ms": round(statistics.median(x["tail_ms"] for x in stats_all), 3),
+            "batch_host_phase_ms": {ph: round(statistics.median(x["phase_ms"][ph] for x in stats_all), 3)
+                                    for ph in stats_all[0]["phase_ms"]},
             "device_bytes_per_proof": stats_all[0]["slice_bytes"], "equal_to_one_at_a_time": True,
             "oracle_first_last_equal": ok}
         print(json.dumps({"many_small_proofs": {"%dx%d" % (n, w): out["%dx%d" % (n, w)]}}), file=sys.stderr,
@@ -608,7 +611,7 @@ def bench_many(stwo_rocm, args):
     ctx.close()
     head = part["%dx%d" % MANY_SHAPES[0][:2]]
     line = {"metric": "many small proofs in single launches: wide-Fib 2^10x8 proofs/s (batch of 256)",
-            "value": head["batched_proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": 3, "warmup": 1,
+            "value": head["batched_proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": 5, "warmup": 2,
             "ms_per_step": head["batch_wall_ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "u32 (exact M31/CM31/QM31)", "data": "synthetic: seeded-uniform traces",
             "config": {"workload": "wide_fibonacci seeded-uniform, functional protocol; shapes %s" %

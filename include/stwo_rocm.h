@@ -299,6 +299,9 @@ typedef struct {
     double tail_ms;       /* last layers, proof of work, decommitment gather and wire encoding */
     double total_ms;      /* the whole call */
     uint64_t slice_bytes; /* device bytes per proof */
+    /* host wall time of the phases: set-up (tables, uploads, launches), wait for the device,
+     * last layers, proof of work, query plans and gather, wire encoding */
+    double phase_ms[6];
 } stwo_rocm_batch_stats;
 int stwo_rocm_prove_wide_fibonacci_batch(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params,
                                          uint32_t n_proofs, const uint64_t* trace_seeds, uint8_t* proofs_out,

@@ -15,11 +15,7 @@
 // one the one-proof path produces from the same parameters.
 #include <string.h>
 
-#include <atomic>
 #include <chrono>
-#include <exception>
-#include <mutex>
-#include <thread>
 
 #include "prover.hpp"
 
@@ -30,39 +26,6 @@ namespace {
 double now_host_ms() {
     using namespace std::chrono;
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
-}
-
-// f(0) .. f(n-1) on up to 16 host threads (the proofs' host phases are independent); the first
-// exception is rethrown after every thread has joined.
-template <class F>
-void parallel_for(uint32_t n, F&& f) {
-    const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
-    const uint32_t T = std::min(std::min(hw, 16u), n);
-    if (T <= 1) {
-        for (uint32_t i = 0; i < n; i++) f(i);
-        return;
-    }
-    std::atomic<uint32_t> next{0};
-    std::exception_ptr err;
-    std::mutex mu;
-    std::vector<std::thread> th;
-    for (uint32_t t = 0; t < T; t++)
-        th.emplace_back([&] {
-            for (;;) {
-                const uint32_t i = next.fetch_add(1);
-                if (i >= n) return;
-                try {
-                    f(i);
-                } catch (...) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (!err) err = std::current_exception();
-                    next.store(n);
-                    return;
-                }
-            }
-        });
-    for (auto& x : th) x.join();
-    if (err) std::rethrow_exception(err);
 }
 
 struct SliceLayout {
@@ -339,6 +302,7 @@ void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, cons
     }
 
     // ---- one readback of every proof's FRI state
+    const double t_enq = now_host_ms();
     uint8_t* h = c.pinned(rb_bytes * K);
     SR_HIP(hipMemcpy2DAsync(h, rb_bytes, base, S, rb_bytes, K, hipMemcpyDeviceToHost, c.stream));
     SR_HIP(hipEventRecord(ev1e, c.stream));
@@ -409,6 +373,7 @@ void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, cons
         finish_last_layer(c, P.ch, prm, T, P.fp);
     });
 
+    const double t_ll = now_host_ms();
     // ---- proof of work: every prefix over one nonce range per launch, in order
     if (prm.pow_bits > 0) {
         std::vector<uint32_t> pre(8ull * K);
@@ -435,25 +400,37 @@ void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, cons
         for (uint32_t k = 0; k < K; k++) pp[k].fp.nonce = best[k];
     }
 
+    const double t_pow = now_host_ms();
     // ---- queries and one gather of every decommitment
     // (the plans on host threads, each into its own list; the lists concatenated for one launch)
     std::vector<Gather> gk(K);
     parallel_for(K, [&](uint32_t k) { finish_queries(c, pp[k].ch, prm, pp[k].tail, pp[k].fp, gk[k]); });
-    Gather g;
-    std::vector<size_t> gbase(K);
-    for (uint32_t k = 0; k < K; k++) {
-        gbase[k] = g.addr.size();
-        g.addr.insert(g.addr.end(), gk[k].addr.begin(), gk[k].addr.end());
+    // one pinned block: every proof's address list (written by the threads in place) and, after it,
+    // the gathered words, which the encoders read where the gather left them
+    std::vector<size_t> gbase(K + 1, 0);
+    for (uint32_t k = 0; k < K; k++) gbase[k + 1] = gbase[k] + gk[k].addr.size();
+    const size_t n_words = gbase[K];
+    uint8_t* hg = c.pinned(12 * n_words + 16);
+    uint64_t* h_addr = (uint64_t*)hg;
+    uint32_t* h_words = (uint32_t*)(hg + 8 * n_words);
+    parallel_for(K, [&](uint32_t k) {
+        if (!gk[k].addr.empty()) memcpy(h_addr + gbase[k], gk[k].addr.data(), 8 * gk[k].addr.size());
+    });
+    if (n_words) {
+        launch_gather(c, h_addr, n_words, h_words);
+        c.sync();
     }
-    std::vector<uint32_t> gw;
-    gather_words(c, g, gw);
+    const double t_gat = now_host_ms();
     proofs.resize(K);
-    parallel_for(K, [&](uint32_t k) { proofs[k] = finish_encode(prm, pp[k].tail, pp[k].fp, gw.data() + gbase[k]); });
+    parallel_for(K, [&](uint32_t k) { proofs[k] = finish_encode(prm, pp[k].tail, pp[k].fp, h_words + gbase[k]); });
     if (stats) {
         stats->device_ms = dev_ms;
         stats->tail_ms = now_host_ms() - t_dev;
         stats->total_ms = now_host_ms() - t_start;
         stats->slice_bytes = S;
+        const double t_end = now_host_ms();
+        const double marks[7] = {t_start, t_enq, t_dev, t_ll, t_pow, t_gat, t_end};
+        for (int i = 0; i < 6; i++) stats->phase_ms[i] = marks[i + 1] - marks[i];
     }
 }
 

@@ -538,9 +538,10 @@ int stwo_rocm_prove_wide_fibonacci_batch(stwo_rocm_ctx* ctx, const stwo_rocm_wid
             stats->tail_ms = st.tail_ms;
             stats->total_ms = st.total_ms;
             stats->slice_bytes = st.slice_bytes;
+            for (int i = 0; i < 6; i++) stats->phase_ms[i] = st.phase_ms[i];
         }
         SR_CHECK(proofs_out && total <= cap, "proof buffer too small");
-        for (uint32_t k = 0; k < n_proofs; k++) memcpy(proofs_out + offsets[k], proofs[k].data(), proofs[k].size());
+        parallel_for(n_proofs, [&](uint32_t k) { memcpy(proofs_out + offsets[k], proofs[k].data(), proofs[k].size()); });
     });
 }
 

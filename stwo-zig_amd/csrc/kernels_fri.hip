@@ -442,7 +442,9 @@ static void launch_tree_layer(Ctx& c, FriLayerArgs& a, const std::string& cls, d
                               uint64_t bstride, uint32_t* d_cnt) {
     SR_CHECK(a.k0 >= 1 && a.k0 <= TREE_LAYER_MAX_LOG, "tree layer: level out of range");
     SR_CHECK(batch >= 1 && batch <= 65535, "tree layer: batch size");
-    a.s = tree_chunk_log(a.k0);
+    // one tree: spread over the CUs; a batch fills the device with its trees, so each workgroup
+    // takes as many hashes as it can hold (fewer workgroups, shorter top reductions)
+    a.s = batch > 1 ? std::min(a.k0, 10u) : tree_chunk_log(a.k0);
     const uint32_t nb = 1u << (a.k0 - a.s);
     SR_CHECK(nb <= TREE_MAX_BLOCKS && a.s >= 1 && a.s <= 10, "tree layer: chunking");
     // one tree: the context's counters; a batch: the caller's, one region per tree

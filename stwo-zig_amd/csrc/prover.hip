@@ -122,7 +122,9 @@ static void merkle_witness_positions(uint32_t max_log, const std::vector<uint64_
 struct JsonOut {
     std::string s;
     size_t n = 0;
-    JsonOut() { s.resize(1 << 17); }
+    // 32 KiB to start (grown by doubling): blocks of 128 KiB and more come from mmap, whose
+    // unmapping on free costs more than the encoding of a small proof
+    JsonOut() { s.resize(1 << 15); }
     char* room(size_t k) {
         if (n + k > s.size()) s.resize(std::max(2 * s.size(), n + k));
         return &s[n];

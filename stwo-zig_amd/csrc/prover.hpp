@@ -1,6 +1,11 @@
 // Host-side proving driver of the rocm backend (internal; the C-ABI in
 // include/stwo_rocm.h exposes it as stwo_rocm_prove_wide_fibonacci).
 #pragma once
+#include <algorithm>
+#include <atomic>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <functional>
 #include <string>
 #include <vector>
@@ -198,6 +203,40 @@ struct ProveParams {
 
 void trace_vanishing_inverses(Ctx& c, uint32_t n, uint32_t den_inv[2]);
 
+// f(0) .. f(n-1) on up to 16 host threads (the proofs' host phases are independent); the first
+// exception is rethrown after every thread has joined.
+template <class F>
+void parallel_for(uint32_t n, F&& f) {
+    const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const uint32_t T = std::min(std::min(hw, 16u), n);
+    if (T <= 1) {
+        for (uint32_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<uint32_t> next{0};
+    std::exception_ptr err;
+    std::mutex mu;
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < T; t++)
+        th.emplace_back([&] {
+            for (;;) {
+                const uint32_t i = next.fetch_add(1);
+                if (i >= n) return;
+                try {
+                    f(i);
+                } catch (...) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (!err) err = std::current_exception();
+                    next.store(n);
+                    return;
+                }
+            }
+        });
+    for (auto& x : th) x.join();
+    if (err) std::rethrow_exception(err);
+}
+
+
 // Many wide-Fibonacci proofs of one shape and PCS configuration in single launches
 // (batch_prover.hip): proof k uses prm with trace_seed = seeds[k] (seeds may be NULL: every
 // proof uses prm.trace_seed).  Requires n + log_blowup <= TREE_LAYER_MAX_LOG + 1 and
@@ -207,6 +246,9 @@ struct BatchStats {
     double tail_ms = 0;    // last layers, proof of work, gather and encoding of every proof
     double total_ms = 0;
     uint64_t slice_bytes = 0;  // device bytes per proof
+    // host wall time of the phases: set-up (tables, uploads, launches), wait for the device,
+    // last layers, proof of work, query plans and gather, wire encoding
+    double phase_ms[6] = {0, 0, 0, 0, 0, 0};
 };
 void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t n_proofs, const uint64_t* seeds,
                                 std::vector<std::string>& proofs, BatchStats* stats);

@@ -678,8 +678,10 @@ class Context:
             self._check(rc)
             addr = ctypes.addressof(buf)
             proofs = [ctypes.string_at(addr + offs[k], offs[k + 1] - offs[k]) for k in range(n_proofs)]
+            phases = ("setup", "device_wait", "last_layers", "pow", "queries_gather", "encode")
             return proofs, {"device_ms": st.device_ms, "tail_ms": st.tail_ms, "total_ms": st.total_ms,
-                            "slice_bytes": st.slice_bytes}
+                            "slice_bytes": st.slice_bytes,
+                            "phase_ms": {k: round(st.phase_ms[i], 3) for i, k in enumerate(phases)}}
 
     def prove_poseidon(self, log_n_instances, pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0,
                        n_queries=3, stats=True):
@@ -868,7 +870,7 @@ class BatchedPipeline:
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("device_ms", ctypes.c_double), ("tail_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("slice_bytes", ctypes.c_uint64)]
+                ("slice_bytes", ctypes.c_uint64), ("phase_ms", ctypes.c_double * 6)]
 
 
 class RuntimeInfo(ctypes.Structure):

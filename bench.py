@@ -557,10 +557,11 @@ MANY_SHAPES = ((10, 8, 256), (14, 32, 64))
 
 def many_proofs_part(stwo_rocm, ctx, check_oracle, shapes=MANY_SHAPES, reps=5):
     """Many small proofs in single launches against the one-at-a-time GPU path: for each shape, K
-    seeded-uniform wide-Fibonacci proofs (functional protocol) proven one call per proof and as one
-    batch (stwo_rocm_prove_wide_fibonacci_batch); proofs/s of both by wall clock, the batch's
-    device and host-tail times, and the check that the batch's proofs are the one-at-a-time proofs
-    (with check_oracle, the first and last also the oracle's)."""
+    seeded-uniform wide-Fibonacci proofs (functional protocol) proven one call per proof, as one
+    batch call (stwo_rocm_prove_wide_fibonacci_batch), and from a batch plan whose device part is
+    replayed from a hipGraph (the headline figure); proofs/s by wall clock, the device and host
+    phase times, and the check that every batched proof is the one-at-a-time proof (with
+    check_oracle, the first and last also the oracle's)."""
     pcs = dict(stwo_rocm.PROTOCOLS["functional"])
     out = {}
     for n, w, k in shapes:
@@ -581,6 +582,17 @@ def many_proofs_part(stwo_rocm, ctx, check_oracle, shapes=MANY_SHAPES, reps=5):
             stats_all.append(st)
         if proofs != ones:
             raise SystemExit("batched proofs differ from the one-at-a-time proofs (%dx%d)" % (n, w))
+        # the same batches from a plan set up once, its device part replayed from a hipGraph
+        plan_s, plan_stats = [], []
+        with ctx.batch_prover(n, w, k, trace_mode=1, **pcs) as bp:
+            bp.run(seeds, graph=True)  # records the graph
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                pproofs, st = bp.run(seeds, graph=True)
+                plan_s.append(time.perf_counter() - t0)
+                plan_stats.append(st)
+        if pproofs != ones:
+            raise SystemExit("graph-replayed batch proofs differ from the one-at-a-time proofs (%dx%d)" % (n, w))
         ok = None
         if check_oracle:
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -590,9 +602,15 @@ def many_proofs_part(stwo_rocm, ctx, check_oracle, shapes=MANY_SHAPES, reps=5):
             if not ok:
                 raise SystemExit("batched proofs differ from the oracle (%dx%d)" % (n, w))
         one_ps, bat_ps = k / statistics.median(one_s), k / statistics.median(bat_s)
+        plan_ps = k / statistics.median(plan_s)
         out["%dx%d" % (n, w)] = {
-            "proofs": k, "one_at_a_time_proofs_per_s": round(one_ps, 1), "batched_proofs_per_s": round(bat_ps, 1),
-            "speedup": round(bat_ps / one_ps, 2), "batch_wall_ms": round(statistics.median(bat_s) * 1e3, 3),
+            "proofs": k, "one_at_a_time_proofs_per_s": round(one_ps, 1), "batched_proofs_per_s": round(plan_ps, 1),
+            "speedup": round(plan_ps / one_ps, 2),
+            "graph_plan": {"proofs_per_s": round(plan_ps, 1), "wall_ms": round(statistics.median(plan_s) * 1e3, 3),
+                           "device_ms": round(statistics.median(x["device_ms"] for x in plan_stats), 3),
+                           "host_phase_ms": {ph: round(statistics.median(x["phase_ms"][ph] for x in plan_stats), 3)
+                                             for ph in plan_stats[0]["phase_ms"]}},
+            "one_call_proofs_per_s": round(bat_ps, 1), "batch_wall_ms": round(statistics.median(bat_s) * 1e3, 3),
             "batch_device_ms": round(statistics.median(x["device_ms"] for x in stats_all), 3),
             "batch_host_tail_ms": round(statistics.median(x["tail_ms"] for x in stats_all), 3),
             "batch_host_phase_ms": {ph: round(statistics.median(x["phase_ms"][ph] for x in stats_all), 3)

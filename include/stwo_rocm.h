@@ -306,6 +306,21 @@ typedef struct {
 int stwo_rocm_prove_wide_fibonacci_batch(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params,
                                          uint32_t n_proofs, const uint64_t* trace_seeds, uint8_t* proofs_out,
                                          uint64_t cap, uint64_t* offsets, stwo_rocm_batch_stats* stats);
+/* The same in two steps, for repeated batches of one shape (no reference counterpart; for
+ * serving loops that prove many batches): create sets up the device allocation, tables and
+ * pinned readback of n_proofs proofs of params' shape and PCS configuration; each run proves
+ * one batch (trace_seeds as above; params->trace_seed when NULL) with the same output
+ * conventions.  use_graph != 0 records the device part of the first run into a hipGraph and
+ * replays it in later runs (runs eagerly when the context profiles kernels or the plan's
+ * quotients need a temporary buffer: more than 120 columns on at most 2^16 rows).  A plan
+ * belongs to its context; destroy it before the context. */
+typedef struct stwo_rocm_batch_prover stwo_rocm_batch_prover;
+int stwo_rocm_batch_prover_create(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params, uint32_t n_proofs,
+                                  stwo_rocm_batch_prover** out);
+int stwo_rocm_batch_prover_run(stwo_rocm_ctx* ctx, stwo_rocm_batch_prover* bp, const uint64_t* trace_seeds,
+                               int use_graph, uint8_t* proofs_out, uint64_t cap, uint64_t* offsets,
+                               stwo_rocm_batch_stats* stats);
+int stwo_rocm_batch_prover_destroy(stwo_rocm_ctx* ctx, stwo_rocm_batch_prover* bp);
 
 /* Poseidon AIR statement and PCS configuration (examples/poseidon/input.zig:20-22;
  * core/pcs/mod.zig PcsConfig). */

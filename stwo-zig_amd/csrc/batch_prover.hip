@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <memory>
 
 #include "prover.hpp"
 
@@ -40,9 +41,43 @@ struct SliceLayout {
 
 }  // namespace
 
-void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, const uint64_t* seeds,
-                                std::vector<std::string>& proofs, BatchStats* stats) {
-    const double t_start = now_host_ms();
+// Everything of a batch that does not depend on the traces: the slice layout, the device
+// allocation, the pointer tables and leaf plans, the per-proof header image, the pinned
+// readback block and (after the first graph run) the recorded launch graph.
+struct BatchPlan {
+    ProveParams prm{};
+    uint32_t K = 0, n = 0, W = 0, B = 0, L = 0, last_log = 0, NS = 0, CL = 0, NC = 0, n_inner = 0;
+    uint64_t N = 0, LN = 0, CN = 0, lsz = 0, S = 0;
+    size_t at_ch = 0, at_roots = 0, at_line = 0, at_troots = 0, at_sv = 0, rb_bytes = 0, at_oods = 0, at_ents = 0,
+           at_bat = 0, hdr_bytes = 0, at_pw = 0, at_trace = 0, at_coef = 0, at_lde1 = 0, at_ev = 0, at_comp = 0,
+           at_lde2 = 0, at_tree1 = 0, at_tree2 = 0, at_ftree0 = 0, at_w = 0, at_etab = 0, at_epart = 0, at_quot = 0;
+    std::vector<size_t> at_lines, at_itrees;
+    size_t tb_trace = 0, tb_coef1 = 0, tb_lde1 = 0, tb_coefev = 0, tb_ev = 0, tb_comp = 0, tb_cp2 = 0, tb_lde2 = 0,
+           tb_ev0 = 0, tb_leaf1 = 0, tb_leaf2 = 0, tb_cols0 = 0;
+    Channel ch0;
+    uint8_t pre_root[32] = {0};
+    uint32_t den_inv[2] = {0, 0};
+    LeafPlan lp1, lp2;
+    DBuf slab, counters, tabs, hdr_img, seeds;
+    uint8_t* h_rb = nullptr;  // pinned readback of every proof's FRI state
+    bool graph_ok = false;    // no temporary buffer inside the device sequence
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    Ctx* ctx = nullptr;
+    ~BatchPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (h_rb) (void)hipHostFree(h_rb);
+    }
+    uint8_t* P0(size_t at) const { return slab.as<uint8_t>() + at; }
+    uint8_t* PK(uint32_t k, size_t at) const { return slab.as<uint8_t>() + (uint64_t)k * S + at; }
+    uint32_t* col(uint32_t k, size_t at, uint64_t len, uint32_t i) const {
+        return (uint32_t*)PK(k, at) + (uint64_t)i * len;
+    }
+    uint8_t* TB(size_t at) const { return tabs.as<uint8_t>() + at; }
+};
+
+BatchPlan* batch_plan_create(Ctx& c, const ProveParams& prm, uint32_t K) {
     const uint32_t n = prm.log_n_rows, W = prm.sequence_len, B = prm.log_blowup_factor;
     SR_CHECK(prm.air == AIR_WIDE_FIBONACCI, "proof batches: wide-Fibonacci AIR only");
     SR_CHECK(K >= 1 && K <= 65535, "proof batches: 1..65535 proofs");
@@ -58,225 +93,226 @@ void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, cons
     SR_CHECK(last_log <= FRI_TAIL_MAX_LOG, "proof batches: last layer above 2^10");
     const uint32_t NS = W + 8;  // sampled columns: the trace and the 8 composition halves
     SR_CHECK(NS <= 2048, "proof batches: too many columns");
-    const uint64_t N = 1ull << n, LN = 1ull << L;
-    const uint32_t CL = n + 1, NC = W - 2;
-    const uint64_t CN = 1ull << CL;
-    const uint32_t n_inner = L - 1 - last_log;
-    const uint64_t lsz = 1ull << last_log;
+    std::unique_ptr<BatchPlan> bp(new BatchPlan());
+    BatchPlan& P = *bp;
+    P.ctx = &c;
+    P.prm = prm;
+    P.K = K, P.n = n, P.W = W, P.B = B, P.L = L, P.last_log = last_log, P.NS = NS;
+    P.N = 1ull << n, P.LN = 1ull << L;
+    P.CL = n + 1, P.NC = W - 2;
+    P.CN = 1ull << P.CL;
+    P.n_inner = L - 1 - last_log;
+    P.lsz = 1ull << last_log;
+    const uint64_t N = P.N, LN = P.LN, CN = P.CN;
+    // the tables a launch of the sequence reads are built here, not in the sequence
     c.ensure_tower(L > n + 1 ? L : n + 1);
+    (void)c.domain_points(L);
+    (void)c.inv_circle_y(L);
+    trace_vanishing_inverses(c, n, P.den_inv);
+    // quotients over more than 128 columns on at most 2^16 rows use a temporary (the split form)
+    P.graph_ok = !(NS > 128 && L <= 16);
 
     // Transcript up to the main commitment: the PCS configuration and the empty preprocessed
     // tree's root (examples/wide_fibonacci/trace.zig:184-186), the same for every proof.
-    Channel ch0;
     qm31 cfg = {{prm.pow_bits, B}, {prm.n_queries, prm.log_last_layer_degree_bound}};
-    ch0.mix_felts(&cfg, 1);
-    uint8_t pre_root[32];
+    P.ch0.mix_felts(&cfg, 1);
     {
         uint8_t prefix[64] = {'l', 'e', 'a', 'f'};
-        b2_hash(prefix, 64, pre_root);
+        b2_hash(prefix, 64, P.pre_root);
     }
-    ch0.mix_root(pre_root);
+    P.ch0.mix_root(P.pre_root);
 
     // ---- slice layout.  The readback region comes first: [channel | FRI roots | last line |
-    // roots of trees 1 and 2 | sampled values]; then the rest of the header (uploaded once).
+    // roots of trees 1 and 2 | sampled values]; then the rest of the header.
     SliceLayout sl;
-    const size_t at_ch = sl.add(CH_BYTES);
-    const size_t at_roots = at_ch + CH_BYTES;  // first FRI layer, then the inner layers
-    sl.size = at_roots + 32 * (1 + (size_t)n_inner);
-    const size_t at_line = sl.add(16 * lsz);
-    const size_t at_troots = sl.add(64);
-    const size_t at_sv = sl.add(16 * (size_t)NS);
-    const size_t rb_bytes = (sl.size + 15) & ~(size_t)15;
-    const size_t at_oods = sl.add(16 * (2 + 32));
-    const size_t at_ents = sl.add(sizeof(QEntryDev) * NS);
-    const size_t at_bat = sl.add(sizeof(QuotientBatchDev));
-    const size_t hdr_bytes = sl.size;
-    const size_t at_pw = sl.add(16 * (size_t)NC);
-    const size_t at_trace = sl.add(4 * (size_t)W * N);
-    const size_t at_coef = sl.add(4 * (size_t)W * N);
-    const size_t at_lde1 = sl.add(4 * (size_t)W * LN);
-    const size_t at_ev = B != 1 ? sl.add(4 * (size_t)W * CN) : 0;
-    const size_t at_comp = sl.add(16 * CN);
-    const size_t at_lde2 = sl.add(32 * LN);
+    P.at_ch = sl.add(CH_BYTES);
+    P.at_roots = P.at_ch + CH_BYTES;  // first FRI layer, then the inner layers
+    sl.size = P.at_roots + 32 * (1 + (size_t)P.n_inner);
+    P.at_line = sl.add(16 * P.lsz);
+    P.at_troots = sl.add(64);
+    P.at_sv = sl.add(16 * (size_t)NS);
+    P.rb_bytes = (sl.size + 15) & ~(size_t)15;
+    P.at_oods = sl.add(16 * (2 + 32));
+    P.at_ents = sl.add(sizeof(QEntryDev) * NS);
+    P.at_bat = sl.add(sizeof(QuotientBatchDev));
+    P.hdr_bytes = sl.size;
+    P.at_pw = sl.add(16 * (size_t)P.NC);
+    P.at_trace = sl.add(4 * (size_t)W * N);
+    P.at_coef = sl.add(4 * (size_t)W * N);
+    P.at_lde1 = sl.add(4 * (size_t)W * LN);
+    P.at_ev = B != 1 ? sl.add(4 * (size_t)W * CN) : 0;
+    P.at_comp = sl.add(16 * CN);
+    P.at_lde2 = sl.add(32 * LN);
     const size_t tree_bytes = ((2ull << L) - 1) * 32;
-    const size_t at_tree1 = sl.add(tree_bytes), at_tree2 = sl.add(tree_bytes), at_ftree0 = sl.add(tree_bytes);
-    const size_t at_w = sl.add(16 * N);
-    const size_t at_etab = sl.add(eval_table_bytes(n));
-    const size_t at_epart = sl.add(eval_partial_bytes(NS, n));
-    const size_t at_quot = sl.add(16 * LN);
-    std::vector<size_t> at_lines(n_inner), at_itrees(n_inner);
-    for (uint32_t k = 0; k < n_inner; k++) {
+    P.at_tree1 = sl.add(tree_bytes);
+    P.at_tree2 = sl.add(tree_bytes);
+    P.at_ftree0 = sl.add(tree_bytes);
+    P.at_w = sl.add(16 * N);
+    P.at_etab = sl.add(eval_table_bytes(n));
+    P.at_epart = sl.add(eval_partial_bytes(NS, n));
+    P.at_quot = sl.add(16 * LN);
+    P.at_lines.resize(P.n_inner);
+    P.at_itrees.resize(P.n_inner);
+    for (uint32_t k = 0; k < P.n_inner; k++) {
         const uint32_t l = L - 1 - k;  // layer of lines[k] and of inner tree k
-        at_lines[k] = sl.add(16ull << l);
-        at_itrees[k] = sl.add(((2ull << l) - 1) * 32);
+        P.at_lines[k] = sl.add(16ull << l);
+        P.at_itrees[k] = sl.add(((2ull << l) - 1) * 32);
     }
-    const uint64_t S = ((uint64_t)sl.size + 4095) & ~(uint64_t)4095;
-    SR_CHECK(S * K <= (1ull << 40), "proof batches: slab too large");
+    P.S = ((uint64_t)sl.size + 4095) & ~(uint64_t)4095;
+    SR_CHECK(P.S * K <= (1ull << 40), "proof batches: slab too large");
 
-    DBuf slab(&c, S * K);
-    DBuf counters(&c, 4ull * TREE_CNT_WORDS * K);
-    launch_fill_u32(c, counters.as<uint32_t>(), (uint64_t)TREE_CNT_WORDS * K, 0u);
-    uint8_t* base = slab.as<uint8_t>();
-    uint32_t* d_cnt = counters.as<uint32_t>();
-    auto P0 = [&](size_t at) { return base + at; };                // proof 0's buffer
-    auto PK = [&](uint32_t k, size_t at) { return base + (uint64_t)k * S + at; };
-    auto col = [&](uint32_t k, size_t at, uint64_t len, uint32_t i) { return (uint32_t*)PK(k, at) + (uint64_t)i * len; };
+    P.slab = DBuf(&c, P.S * K);
+    P.counters = DBuf(&c, 4ull * TREE_CNT_WORDS * K);
+    launch_fill_u32(c, P.counters.as<uint32_t>(), (uint64_t)TREE_CNT_WORDS * K, 0u);
+    P.seeds = DBuf(&c, 8ull * K);
 
-    // ---- host tables: pointer lists (over every proof for the FFTs, proof 0's for the rest),
-    // leaf plans, the quotient entries and the initial channel
+    // ---- tables: pointer lists (over every proof for the FFTs, proof 0's for the rest) and
+    // leaf plans
     HostPack pack;
-    std::vector<const uint32_t*> t_trace;
-    std::vector<uint32_t*> t_coef1w, t_lde1;
-    std::vector<const uint32_t*> t_coefev;
-    std::vector<uint32_t*> t_ev;
-    std::vector<uint32_t*> t_comp;
-    std::vector<const uint32_t*> t_cp2;
-    std::vector<uint32_t*> t_lde2;
+    std::vector<const uint32_t*> t_trace, t_coefev, t_cp2;
+    std::vector<uint32_t*> t_coef1w, t_lde1, t_ev, t_comp, t_lde2;
     for (uint32_t k = 0; k < K; k++) {
         for (uint32_t i = 0; i < W; i++) {
-            t_trace.push_back(col(k, at_trace, N, i));
-            t_coef1w.push_back(col(k, at_coef, N, i));
-            t_lde1.push_back(col(k, at_lde1, LN, i));
+            t_trace.push_back(P.col(k, P.at_trace, N, i));
+            t_coef1w.push_back(P.col(k, P.at_coef, N, i));
+            t_lde1.push_back(P.col(k, P.at_lde1, LN, i));
             if (B != 1) {
-                t_coefev.push_back(col(k, at_coef, N, i));
-                t_ev.push_back(col(k, at_ev, CN, i));
+                t_coefev.push_back(P.col(k, P.at_coef, N, i));
+                t_ev.push_back(P.col(k, P.at_ev, CN, i));
             }
         }
-        for (uint32_t q = 0; q < 4; q++) t_comp.push_back(col(k, at_comp, CN, q));
+        for (uint32_t q = 0; q < 4; q++) t_comp.push_back(P.col(k, P.at_comp, CN, q));
         // splitAtMid -> [left0..3, right0..3] (poly.zig:302-318; prove.zig:423-445)
         for (uint32_t h = 0; h < 2; h++)
-            for (uint32_t q = 0; q < 4; q++) t_cp2.push_back(col(k, at_comp, CN, q) + h * N);
-        for (uint32_t i = 0; i < 8; i++) t_lde2.push_back(col(k, at_lde2, LN, i));
+            for (uint32_t q = 0; q < 4; q++) t_cp2.push_back(P.col(k, P.at_comp, CN, q) + h * N);
+        for (uint32_t i = 0; i < 8; i++) t_lde2.push_back(P.col(k, P.at_lde2, LN, i));
     }
-    const size_t tb_trace = pack.add(t_trace.data(), t_trace.size());
-    const size_t tb_coef1 = pack.add(t_coef1w.data(), t_coef1w.size());
-    const size_t tb_lde1 = pack.add(t_lde1.data(), t_lde1.size());
-    const size_t tb_coefev = B != 1 ? pack.add(t_coefev.data(), t_coefev.size()) : 0;
-    const size_t tb_ev = B != 1 ? pack.add(t_ev.data(), t_ev.size()) : 0;
-    const size_t tb_comp = pack.add(t_comp.data(), t_comp.size());
-    const size_t tb_cp2 = pack.add(t_cp2.data(), t_cp2.size());
-    const size_t tb_lde2 = pack.add(t_lde2.data(), t_lde2.size());
+    P.tb_trace = pack.add(t_trace.data(), t_trace.size());
+    P.tb_coef1 = pack.add(t_coef1w.data(), t_coef1w.size());
+    P.tb_lde1 = pack.add(t_lde1.data(), t_lde1.size());
+    P.tb_coefev = B != 1 ? pack.add(t_coefev.data(), t_coefev.size()) : 0;
+    P.tb_ev = B != 1 ? pack.add(t_ev.data(), t_ev.size()) : 0;
+    P.tb_comp = pack.add(t_comp.data(), t_comp.size());
+    P.tb_cp2 = pack.add(t_cp2.data(), t_cp2.size());
+    P.tb_lde2 = pack.add(t_lde2.data(), t_lde2.size());
     // proof 0's composition inputs (the LDE, or the evaluation on CanonicCoset(n + 1))
     std::vector<const uint32_t*> ev0(W);
-    for (uint32_t i = 0; i < W; i++) ev0[i] = B == 1 ? col(0, at_lde1, LN, i) : col(0, at_ev, CN, i);
-    const size_t tb_ev0 = pack.add(ev0.data(), W);
+    for (uint32_t i = 0; i < W; i++) ev0[i] = B == 1 ? P.col(0, P.at_lde1, LN, i) : P.col(0, P.at_ev, CN, i);
+    P.tb_ev0 = pack.add(ev0.data(), W);
     // leaf plans of proof 0's trees 1 and 2 (one log size each: column order kept)
     std::vector<const uint32_t*> l1(W), l2(8);
     std::vector<uint32_t> l1_logs(W, L), l2_logs(8, L);
-    for (uint32_t i = 0; i < W; i++) l1[i] = col(0, at_lde1, LN, i);
-    for (uint32_t i = 0; i < 8; i++) l2[i] = col(0, at_lde2, LN, i);
-    const LeafPlan lp1 = merkle_leaf_plan(l1.data(), l1_logs.data(), W);
-    const LeafPlan lp2 = merkle_leaf_plan(l2.data(), l2_logs.data(), 8);
-    const size_t tb_leaf1 = pack.add(lp1.tab.data(), lp1.tab.size());
-    const size_t tb_leaf2 = pack.add(lp2.tab.data(), lp2.tab.size());
+    for (uint32_t i = 0; i < W; i++) l1[i] = P.col(0, P.at_lde1, LN, i);
+    for (uint32_t i = 0; i < 8; i++) l2[i] = P.col(0, P.at_lde2, LN, i);
+    P.lp1 = merkle_leaf_plan(l1.data(), l1_logs.data(), W);
+    P.lp2 = merkle_leaf_plan(l2.data(), l2_logs.data(), 8);
+    P.tb_leaf1 = pack.add(P.lp1.tab.data(), P.lp1.tab.size());
+    P.tb_leaf2 = pack.add(P.lp2.tab.data(), P.lp2.tab.size());
     // sampled columns of proof 0: the W trace coefficients, then the 8 composition halves
     std::vector<const uint32_t*> cols0;
-    for (uint32_t i = 0; i < W; i++) cols0.push_back(col(0, at_coef, N, i));
+    for (uint32_t i = 0; i < W; i++) cols0.push_back(P.col(0, P.at_coef, N, i));
     for (uint32_t h = 0; h < 2; h++)
-        for (uint32_t q = 0; q < 4; q++) cols0.push_back(col(0, at_comp, CN, q) + h * N);
-    const size_t tb_cols0 = pack.add(cols0.data(), NS);
-    const uint64_t* seed_src = seeds;
-    std::vector<uint64_t> seed_all;
-    if (!seed_src) {
-        seed_all.assign(K, prm.trace_seed);
-        seed_src = seed_all.data();
-    }
-    const size_t tb_seeds = pack.add(seed_src, K);
-    DBuf tabs = upload(c, pack.buf.data(), pack.buf.size());
-    uint8_t* d_tabs = tabs.as<uint8_t>();
-    auto TB = [&](size_t at) { return d_tabs + at; };
+        for (uint32_t q = 0; q < 4; q++) cols0.push_back(P.col(0, P.at_comp, CN, q) + h * N);
+    P.tb_cols0 = pack.add(cols0.data(), NS);
+    P.tabs = upload(c, pack.buf.data(), pack.buf.size());
 
-    // per-proof header: the initial channel, the quotient entries (proof 0's column pointers:
-    // the kernels add the slice offset; coefficients written on the device) and a zero batch
-    std::vector<uint8_t> hdr(hdr_bytes * K, 0);
+    // per-proof header image, copied into every slice at the start of each run: the initial
+    // channel, the quotient entries (proof 0's column pointers: the kernels add the slice
+    // offset; coefficients written on the device) and a zero batch
     {
-        std::vector<uint8_t> h(hdr_bytes, 0);
+        std::vector<uint8_t> h(P.hdr_bytes, 0);
         DevChannel hc{};
-        memcpy(hc.digest, ch0.digest, 32);
-        hc.n_draws = ch0.n_draws;
-        memcpy(h.data() + at_ch, &hc, sizeof(hc));
+        memcpy(hc.digest, P.ch0.digest, 32);
+        hc.n_draws = P.ch0.n_draws;
+        memcpy(h.data() + P.at_ch, &hc, sizeof(hc));
         // the quotient columns are the LDEs of trees 1 and 2 (lifting log L for every column:
         // shift 1), in the order of the sampled values
-        QEntryDev* ents = (QEntryDev*)(h.data() + at_ents);
-        for (uint32_t i = 0; i < W; i++) ents[i] = QEntryDev{col(0, at_lde1, LN, i), 1u, 0u, {0, 0, 0, 0}};
-        for (uint32_t i = 0; i < 8; i++) ents[W + i] = QEntryDev{col(0, at_lde2, LN, i), 1u, 0u, {0, 0, 0, 0}};
-        for (uint32_t k = 0; k < K; k++) memcpy(hdr.data() + (size_t)k * hdr_bytes, h.data(), hdr_bytes);
+        QEntryDev* ents = (QEntryDev*)(h.data() + P.at_ents);
+        for (uint32_t i = 0; i < W; i++) ents[i] = QEntryDev{P.col(0, P.at_lde1, LN, i), 1u, 0u, {0, 0, 0, 0}};
+        for (uint32_t i = 0; i < 8; i++) ents[W + i] = QEntryDev{P.col(0, P.at_lde2, LN, i), 1u, 0u, {0, 0, 0, 0}};
+        P.hdr_img = upload(c, h.data(), h.size());
     }
-    SR_HIP(hipMemcpy2DAsync(base, S, c.stage(hdr.data(), hdr.size()), hdr_bytes, hdr_bytes, K, hipMemcpyHostToDevice,
-                            c.stream));
-    DevChannel* d_ch = (DevChannel*)P0(at_ch);
-    uint8_t* d_roots = P0(at_roots);
+    SR_HIP(hipHostMalloc((void**)&P.h_rb, P.rb_bytes * K, hipHostMallocDefault));
+    c.sync();
+    return bp.release();
+}
 
-    struct Events {
-        hipEvent_t a = nullptr, b = nullptr;
-        ~Events() {
-            if (a) (void)hipEventDestroy(a);
-            if (b) (void)hipEventDestroy(b);
-        }
-    } evs;
-    SR_HIP(hipEventCreate(&evs.a));
-    SR_HIP(hipEventCreate(&evs.b));
-    hipEvent_t ev0e = evs.a, ev1e = evs.b;
-    SR_HIP(hipEventRecord(ev0e, c.stream));
+// The device part of a run: header copies, every stage of every proof, and the readback into
+// the plan's pinned block.  Host-side work only enqueues, so the sequence can be recorded.
+static void batch_enqueue(Ctx& c, BatchPlan& P) {
+    const uint32_t K = P.K, n = P.n, W = P.W, B = P.B, L = P.L, last_log = P.last_log, NS = P.NS, CL = P.CL;
+    const uint64_t N = P.N, LN = P.LN, CN = P.CN, S = P.S;
+    uint32_t* d_cnt = P.counters.as<uint32_t>();
+    SR_HIP(hipMemcpyAsync(P.P0(0), P.hdr_img.p, P.hdr_bytes, hipMemcpyDeviceToDevice, c.stream));
+    for (uint32_t k = 1; k < K; k *= 2) {
+        // doubling copies of the header into slices [k, 2k)
+        const uint32_t cnt = std::min(k, K - k);
+        SR_HIP(hipMemcpy2DAsync(P.PK(k, 0), S, P.P0(0), S, P.hdr_bytes, cnt, hipMemcpyDeviceToDevice, c.stream));
+    }
+    DevChannel* d_ch = (DevChannel*)P.P0(P.at_ch);
+    uint8_t* d_roots = P.P0(P.at_roots);
 
     // ---- tree 1: trace -> coefficients -> LDE -> Merkle
-    launch_widefib_trace(c, n, W, prm.trace_mode, prm.trace_seed, (uint32_t*)P0(at_trace), N, K,
-                         (const uint64_t*)TB(tb_seeds), S);
+    launch_widefib_trace(c, n, W, P.prm.trace_mode, P.prm.trace_seed, (uint32_t*)P.P0(P.at_trace), N, K,
+                         P.seeds.as<const uint64_t>(), S);
     if (B == 1) {
-        launch_circle_ifft_lde(c, (const uint32_t* const*)TB(tb_trace), (uint32_t* const*)TB(tb_coef1),
-                               (uint32_t* const*)TB(tb_lde1), K * W, n);
+        launch_circle_ifft_lde(c, (const uint32_t* const*)P.TB(P.tb_trace), (uint32_t* const*)P.TB(P.tb_coef1),
+                               (uint32_t* const*)P.TB(P.tb_lde1), K * W, n);
     } else {
-        launch_circle_fft(c, true, (const uint32_t* const*)TB(tb_trace), (uint32_t* const*)TB(tb_coef1), K * W, n, n);
-        launch_circle_fft(c, false, (const uint32_t* const*)TB(tb_coef1), (uint32_t* const*)TB(tb_lde1), K * W, L, n);
+        launch_circle_fft(c, true, (const uint32_t* const*)P.TB(P.tb_trace), (uint32_t* const*)P.TB(P.tb_coef1), K * W,
+                          n, n);
+        launch_circle_fft(c, false, (const uint32_t* const*)P.TB(P.tb_coef1), (uint32_t* const*)P.TB(P.tb_lde1), K * W,
+                          L, n);
     }
-    DeviceTree tree1{L, P0(at_tree1), {}}, tree2{L, P0(at_tree2), {}}, ftree0{L, P0(at_ftree0), {}};
-    launch_merkle_leaves_dev(c, (const LeafColDev*)TB(tb_leaf1), lp1.segs, W, tree1, K, S);
-    launch_merkle_parents(c, tree1, nullptr, P0(at_troots), K, S, d_cnt);
+    DeviceTree tree1{L, P.P0(P.at_tree1), {}}, tree2{L, P.P0(P.at_tree2), {}}, ftree0{L, P.P0(P.at_ftree0), {}};
+    launch_merkle_leaves_dev(c, (const LeafColDev*)P.TB(P.tb_leaf1), P.lp1.segs, W, tree1, K, S);
+    launch_merkle_parents(c, tree1, nullptr, P.P0(P.at_troots), K, S, d_cnt);
 
     // mixRoot(main), mixStatement (wide_fibonacci.zig:496-498), composition alpha and its powers
     const uint32_t stw[2] = {n, W};
-    launch_tr_commit_draw(c, d_ch, P0(at_tree1), stw, 2, (uint32_t*)P0(at_pw), NC, K, S);
-    uint32_t den_inv[2];
-    trace_vanishing_inverses(c, n, den_inv);
+    launch_tr_commit_draw(c, d_ch, P.P0(P.at_tree1), stw, 2, (uint32_t*)P.P0(P.at_pw), P.NC, K, S);
     if (B != 1)
-        launch_circle_fft(c, false, (const uint32_t* const*)TB(tb_coefev), (uint32_t* const*)TB(tb_ev), K * W, CL, n);
+        launch_circle_fft(c, false, (const uint32_t* const*)P.TB(P.tb_coefev), (uint32_t* const*)P.TB(P.tb_ev), K * W,
+                          CL, n);
     uint32_t* comp0[4];
-    for (uint32_t q = 0; q < 4; q++) comp0[q] = col(0, at_comp, CN, q);
-    launch_widefib_composition(c, (const uint32_t* const*)TB(tb_ev0), W, CL, n, (const uint32_t*)P0(at_pw), den_inv[0],
-                               den_inv[1], comp0, K, S);
+    for (uint32_t q = 0; q < 4; q++) comp0[q] = P.col(0, P.at_comp, CN, q);
+    launch_widefib_composition(c, (const uint32_t* const*)P.TB(P.tb_ev0), W, CL, n, (const uint32_t*)P.P0(P.at_pw),
+                               P.den_inv[0], P.den_inv[1], comp0, K, S);
     // interpolate (secure_poly.zig:164-215) in place, then tree 2
-    launch_circle_fft(c, true, (const uint32_t* const*)TB(tb_comp), (uint32_t* const*)TB(tb_comp), 4 * K, CL, CL);
-    launch_circle_fft(c, false, (const uint32_t* const*)TB(tb_cp2), (uint32_t* const*)TB(tb_lde2), 8 * K, L, n);
-    launch_merkle_leaves_dev(c, (const LeafColDev*)TB(tb_leaf2), lp2.segs, 8, tree2, K, S);
-    launch_merkle_parents(c, tree2, nullptr, P0(at_troots) + 32, K, S, d_cnt);
+    launch_circle_fft(c, true, (const uint32_t* const*)P.TB(P.tb_comp), (uint32_t* const*)P.TB(P.tb_comp), 4 * K, CL,
+                      CL);
+    launch_circle_fft(c, false, (const uint32_t* const*)P.TB(P.tb_cp2), (uint32_t* const*)P.TB(P.tb_lde2), 8 * K, L,
+                      n);
+    launch_merkle_leaves_dev(c, (const LeafColDev*)P.TB(P.tb_leaf2), P.lp2.segs, 8, tree2, K, S);
+    launch_merkle_parents(c, tree2, nullptr, P.P0(P.at_troots) + 32, K, S, d_cnt);
 
     // mixRoot(composition), OODS point, evaluation factors; sampled values
-    uint32_t* d_oods = (uint32_t*)P0(at_oods);
-    launch_tr_oods(c, d_ch, P0(at_tree2), n, d_oods, d_oods + 8, K, S);
-    launch_eval_weights_dev(c, d_oods + 8, n, (uint32_t*)P0(at_w), K, S, (uint32_t*)P0(at_etab));
-    launch_eval_columns_into(c, (const uint32_t* const*)TB(tb_cols0), NS, n, (const uint32_t*)P0(at_w),
-                             (uint32_t*)P0(at_sv), (unsigned long long*)P0(at_epart), K, S);
+    uint32_t* d_oods = (uint32_t*)P.P0(P.at_oods);
+    launch_tr_oods(c, d_ch, P.P0(P.at_tree2), n, d_oods, d_oods + 8, K, S);
+    launch_eval_weights_dev(c, d_oods + 8, n, (uint32_t*)P.P0(P.at_w), K, S, (uint32_t*)P.P0(P.at_etab));
+    launch_eval_columns_into(c, (const uint32_t* const*)P.TB(P.tb_cols0), NS, n, (const uint32_t*)P.P0(P.at_w),
+                             (uint32_t*)P.P0(P.at_sv), (unsigned long long*)P.P0(P.at_epart), K, S);
 
     // quotients (one batch at the OODS point) with the first FRI layer's leaves, its tree and
     // channel step
-    launch_tr_quotient(c, d_ch, (const uint32_t*)P0(at_sv), NS, d_oods, (QEntryDev*)P0(at_ents),
-                       (QuotientBatchDev*)P0(at_bat), K, S);
+    launch_tr_quotient(c, d_ch, (const uint32_t*)P.P0(P.at_sv), NS, d_oods, (QEntryDev*)P.P0(P.at_ents),
+                       (QuotientBatchDev*)P.P0(P.at_bat), K, S);
     uint32_t* qc0[4];
-    for (uint32_t q = 0; q < 4; q++) qc0[q] = col(0, at_quot, LN, q);
-    launch_quotients_dev(c, (const QEntryDev*)P0(at_ents), NS, (const QuotientBatchDev*)P0(at_bat), 1,
+    for (uint32_t q = 0; q < 4; q++) qc0[q] = P.col(0, P.at_quot, LN, q);
+    launch_quotients_dev(c, (const QEntryDev*)P.P0(P.at_ents), NS, (const QuotientBatchDev*)P.P0(P.at_bat), 1,
                          4.0 * NS * (double)LN, L, qc0, ftree0.layer(L), K, S);
     launch_merkle_parents(c, ftree0, d_ch, d_roots, K, S, d_cnt);
 
     // FRI inner layers
-    uint32_t* d_last = (uint32_t*)P0(at_line);
-    std::vector<DeviceTree> itrees(n_inner);
-    for (uint32_t k = 0; k < n_inner; k++) itrees[k] = DeviceTree{L - 1 - k, P0(at_itrees[k]), {}};
+    uint32_t* d_last = (uint32_t*)P.P0(P.at_line);
+    std::vector<DeviceTree> itrees(P.n_inner);
+    for (uint32_t k = 0; k < P.n_inner; k++) itrees[k] = DeviceTree{L - 1 - k, P.P0(P.at_itrees[k]), {}};
     const uint32_t* cc0[4] = {qc0[0], qc0[1], qc0[2], qc0[3]};
-    if (n_inner == 0) {
+    if (P.n_inner == 0) {
         launch_fold_circle_into_line(c, cc0, L, qzero(), d_last, false, nullptr, d_ch, K, S);
     } else {
         auto line_at = [&](uint32_t l) -> uint32_t* {
-            return l == last_log ? d_last : (uint32_t*)P0(at_lines[L - 1 - l]);
+            return l == last_log ? d_last : (uint32_t*)P.P0(P.at_lines[L - 1 - l]);
         };
         launch_fri_commit_circle(c, cc0, L, line_at(L - 1), itrees[0], d_ch, d_roots + 32, K, S, d_cnt);
         const uint32_t top_log = L - 1;
@@ -298,20 +334,59 @@ void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, cons
             ta.roots[l] = (uint32_t*)(d_roots + 32 * kr++);
         }
         launch_fri_tail(c, ta, K);
-        SR_CHECK(kr == 1 + (size_t)n_inner, "FRI root count");
+        SR_CHECK(kr == 1 + (size_t)P.n_inner, "FRI root count");
     }
-
     // ---- one readback of every proof's FRI state
-    const double t_enq = now_host_ms();
-    uint8_t* h = c.pinned(rb_bytes * K);
-    SR_HIP(hipMemcpy2DAsync(h, rb_bytes, base, S, rb_bytes, K, hipMemcpyDeviceToHost, c.stream));
-    SR_HIP(hipEventRecord(ev1e, c.stream));
-    c.sync();
-    std::vector<uint8_t> rb(h, h + rb_bytes * K);
-    float dev_ms = 0;
-    SR_HIP(hipEventElapsedTime(&dev_ms, ev0e, ev1e));
-    const double t_dev = now_host_ms();
+    SR_HIP(hipMemcpy2DAsync(P.h_rb, P.rb_bytes, P.P0(0), S, P.rb_bytes, K, hipMemcpyDeviceToHost, c.stream));
+}
 
+void batch_plan_run(Ctx& c, BatchPlan& P, const uint64_t* seeds, bool use_graph, std::vector<std::string>& proofs,
+                    BatchStats* stats) {
+    const double t_start = now_host_ms();
+    SR_CHECK(P.ctx == &c, "proof batches: the plan belongs to another context");
+    const uint32_t K = P.K, W = P.W, L = P.L, last_log = P.last_log, NS = P.NS, n_inner = P.n_inner;
+    const uint64_t LN = P.LN, lsz = P.lsz;
+    const ProveParams& prm = P.prm;
+    {
+        std::vector<uint64_t> sd(K, prm.trace_seed);
+        if (seeds) std::copy(seeds, seeds + K, sd.begin());
+        SR_HIP(hipMemcpyAsync(P.seeds.p, c.stage(sd.data(), 8ull * K), 8ull * K, hipMemcpyHostToDevice, c.stream));
+    }
+    struct Events {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Events() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } evs;
+    SR_HIP(hipEventCreate(&evs.a));
+    SR_HIP(hipEventCreate(&evs.b));
+    SR_HIP(hipEventRecord(evs.a, c.stream));
+    // a graph replays the recorded launches; the stage profile needs its own events, so a
+    // profiled context runs the sequence eagerly
+    const bool graph = use_graph && P.graph_ok && !c.profiling;
+    if (graph && !P.exec) {
+        SR_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+        try {
+            batch_enqueue(c, P);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        SR_HIP(hipStreamEndCapture(c.stream, &P.graph));
+        SR_HIP(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+    }
+    if (graph) SR_HIP(hipGraphLaunch(P.exec, c.stream));
+    else batch_enqueue(c, P);
+    SR_HIP(hipEventRecord(evs.b, c.stream));
+    const double t_enq = now_host_ms();
+    c.sync();
+    std::vector<uint8_t> rb(P.h_rb, P.h_rb + P.rb_bytes * K);
+    float dev_ms = 0;
+    SR_HIP(hipEventElapsedTime(&dev_ms, evs.a, evs.b));
+    const double t_dev = now_host_ms();
     // ---- host objects of every proof: its trees and buffers at its slice
     struct PerProof {
         Committed t0, t1, t2;
@@ -323,54 +398,54 @@ void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, cons
     };
     std::vector<PerProof> pp(K);
     parallel_for(K, [&](uint32_t k) {
-        PerProof& P = pp[k];
-        const uint8_t* r = rb.data() + (size_t)k * rb_bytes;
+        PerProof& Q = pp[k];
+        const uint8_t* r = rb.data() + (size_t)k * P.rb_bytes;
         DevChannel hc;
-        memcpy(&hc, r + at_ch, sizeof(hc));
+        memcpy(&hc, r + P.at_ch, sizeof(hc));
         SR_CHECK(hc.status == 0, "FRI: channel draw rejected 64 times in a row");
-        memcpy(P.ch.digest, hc.digest, 32);
-        P.ch.n_draws = hc.n_draws;
-        P.tr0 = DeviceTree{0, nullptr, {}};
-        memcpy(P.tr0.root, pre_root, 32);
-        P.tr1 = DeviceTree{L, PK(k, at_tree1), {}};
-        memcpy(P.tr1.root, r + at_troots, 32);
-        P.tr2 = DeviceTree{L, PK(k, at_tree2), {}};
-        memcpy(P.tr2.root, r + at_troots + 32, 32);
-        P.f0 = DeviceTree{L, PK(k, at_ftree0), {}};
-        memcpy(P.f0.root, r + at_roots, 32);
-        P.fi.resize(n_inner);
+        memcpy(Q.ch.digest, hc.digest, 32);
+        Q.ch.n_draws = hc.n_draws;
+        Q.tr0 = DeviceTree{0, nullptr, {}};
+        memcpy(Q.tr0.root, P.pre_root, 32);
+        Q.tr1 = DeviceTree{L, P.PK(k, P.at_tree1), {}};
+        memcpy(Q.tr1.root, r + P.at_troots, 32);
+        Q.tr2 = DeviceTree{L, P.PK(k, P.at_tree2), {}};
+        memcpy(Q.tr2.root, r + P.at_troots + 32, 32);
+        Q.f0 = DeviceTree{L, P.PK(k, P.at_ftree0), {}};
+        memcpy(Q.f0.root, r + P.at_roots, 32);
+        Q.fi.resize(n_inner);
         for (uint32_t j = 0; j < n_inner; j++) {
-            P.fi[j] = DeviceTree{L - 1 - j, PK(k, at_itrees[j]), {}};
-            memcpy(P.fi[j].root, r + at_roots + 32 * (1 + j), 32);
+            Q.fi[j] = DeviceTree{L - 1 - j, P.PK(k, P.at_itrees[j]), {}};
+            memcpy(Q.fi[j].root, r + P.at_roots + 32 * (1 + j), 32);
         }
-        P.t0.tree = &P.tr0;
-        P.t1.tree = &P.tr1;
-        P.t2.tree = &P.tr2;
+        Q.t0.tree = &Q.tr0;
+        Q.t1.tree = &Q.tr1;
+        Q.t2.tree = &Q.tr2;
         for (uint32_t i = 0; i < W; i++) {
-            P.t1.lde.push_back(col(k, at_lde1, LN, i));
-            P.t1.lde_logs.push_back(L);
+            Q.t1.lde.push_back(P.col(k, P.at_lde1, LN, i));
+            Q.t1.lde_logs.push_back(L);
         }
         for (uint32_t i = 0; i < 8; i++) {
-            P.t2.lde.push_back(col(k, at_lde2, LN, i));
-            P.t2.lde_logs.push_back(L);
+            Q.t2.lde.push_back(P.col(k, P.at_lde2, LN, i));
+            Q.t2.lde_logs.push_back(L);
         }
-        ProofTail& T = P.tail;
+        ProofTail& T = Q.tail;
         T.L = L;
         T.last_log = last_log;
-        T.trees = {&P.t0, &P.t1, &P.t2};
+        T.trees = {&Q.t0, &Q.t1, &Q.t2};
         T.n_samples.assign(NS, 1);
         T.sv.resize(4 * (size_t)NS);
-        memcpy(T.sv.data(), r + at_sv, 16 * (size_t)NS);
-        T.ftree0 = &P.f0;
-        for (uint32_t q = 0; q < 4; q++) T.qc[q] = col(k, at_quot, LN, q);
+        memcpy(T.sv.data(), r + P.at_sv, 16 * (size_t)NS);
+        T.ftree0 = &Q.f0;
+        for (uint32_t q = 0; q < 4; q++) T.qc[q] = P.col(k, P.at_quot, LN, q);
         for (uint32_t j = 0; j < n_inner; j++) {
-            T.ftrees.push_back(&P.fi[j]);
-            T.lines.push_back((const uint32_t*)PK(k, at_lines[j]));
+            T.ftrees.push_back(&Q.fi[j]);
+            T.lines.push_back((const uint32_t*)P.PK(k, P.at_lines[j]));
         }
         T.last_values.resize(lsz);
-        const uint32_t* w = (const uint32_t*)(r + at_line);
+        const uint32_t* w = (const uint32_t*)(r + P.at_line);
         for (uint64_t i = 0; i < lsz; i++) T.last_values[i] = qload(w + 4 * i);
-        finish_last_layer(c, P.ch, prm, T, P.fp);
+        finish_last_layer(c, Q.ch, prm, T, Q.fp);
     });
 
     const double t_ll = now_host_ms();
@@ -427,11 +502,21 @@ void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, cons
         stats->device_ms = dev_ms;
         stats->tail_ms = now_host_ms() - t_dev;
         stats->total_ms = now_host_ms() - t_start;
-        stats->slice_bytes = S;
+        stats->slice_bytes = P.S;
         const double t_end = now_host_ms();
         const double marks[7] = {t_start, t_enq, t_dev, t_ll, t_pow, t_gat, t_end};
         for (int i = 0; i < 6; i++) stats->phase_ms[i] = marks[i + 1] - marks[i];
     }
+}
+
+void batch_plan_free(BatchPlan* P) { delete P; }
+
+uint64_t batch_plan_slice_bytes(const BatchPlan& P) { return P.S; }
+
+void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t K, const uint64_t* seeds,
+                                std::vector<std::string>& proofs, BatchStats* stats) {
+    std::unique_ptr<BatchPlan> P(batch_plan_create(c, prm, K));
+    batch_plan_run(c, *P, seeds, false, proofs, stats);
 }
 
 }  // namespace sr

@@ -509,40 +509,89 @@ int stwo_rocm_prove_wide_fibonacci(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_p
     });
 }
 
+static ProveParams batch_params(const stwo_rocm_widefib_params* params) {
+    SR_CHECK(params, "null argument");
+    ProveParams p;
+    p.log_n_rows = params->log_n_rows;
+    p.sequence_len = params->sequence_len;
+    p.pow_bits = params->pow_bits;
+    p.log_blowup_factor = params->log_blowup_factor;
+    p.log_last_layer_degree_bound = params->log_last_layer_degree_bound;
+    p.n_queries = params->n_queries;
+    p.trace_mode = params->trace_mode;
+    p.trace_seed = params->trace_seed;
+    SR_CHECK(p.trace_mode <= 1, "trace_mode must be 0 or 1");
+    return p;
+}
+
+// Offsets, stats and the proofs back to back (the copy on host threads).
+static void batch_out(const std::vector<std::string>& proofs, const BatchStats& st, uint8_t* proofs_out,
+                      uint64_t cap, uint64_t* offsets, stwo_rocm_batch_stats* stats) {
+    const uint32_t n = (uint32_t)proofs.size();
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        offsets[k] = total;
+        total += proofs[k].size();
+    }
+    offsets[n] = total;
+    if (stats) {
+        stats->device_ms = st.device_ms;
+        stats->tail_ms = st.tail_ms;
+        stats->total_ms = st.total_ms;
+        stats->slice_bytes = st.slice_bytes;
+        for (int i = 0; i < 6; i++) stats->phase_ms[i] = st.phase_ms[i];
+    }
+    SR_CHECK(proofs_out && total <= cap, "proof buffer too small");
+    parallel_for(n, [&](uint32_t k) { memcpy(proofs_out + offsets[k], proofs[k].data(), proofs[k].size()); });
+}
+
 int stwo_rocm_prove_wide_fibonacci_batch(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params,
                                          uint32_t n_proofs, const uint64_t* trace_seeds, uint8_t* proofs_out,
                                          uint64_t cap, uint64_t* offsets, stwo_rocm_batch_stats* stats) {
     return guarded(ctx, [&](Ctx& c) {
-        SR_CHECK(params && offsets, "null argument");
-        ProveParams p;
-        p.log_n_rows = params->log_n_rows;
-        p.sequence_len = params->sequence_len;
-        p.pow_bits = params->pow_bits;
-        p.log_blowup_factor = params->log_blowup_factor;
-        p.log_last_layer_degree_bound = params->log_last_layer_degree_bound;
-        p.n_queries = params->n_queries;
-        p.trace_mode = params->trace_mode;
-        p.trace_seed = params->trace_seed;
-        SR_CHECK(p.trace_mode <= 1, "trace_mode must be 0 or 1");
+        SR_CHECK(offsets, "null argument");
+        const ProveParams p = batch_params(params);
         std::vector<std::string> proofs;
         BatchStats st;
         prove_wide_fibonacci_batch(c, p, n_proofs, trace_seeds, proofs, &st);
-        uint64_t total = 0;
-        for (uint32_t k = 0; k < n_proofs; k++) {
-            offsets[k] = total;
-            total += proofs[k].size();
-        }
-        offsets[n_proofs] = total;
-        if (stats) {
-            stats->device_ms = st.device_ms;
-            stats->tail_ms = st.tail_ms;
-            stats->total_ms = st.total_ms;
-            stats->slice_bytes = st.slice_bytes;
-            for (int i = 0; i < 6; i++) stats->phase_ms[i] = st.phase_ms[i];
-        }
-        SR_CHECK(proofs_out && total <= cap, "proof buffer too small");
-        parallel_for(n_proofs, [&](uint32_t k) { memcpy(proofs_out + offsets[k], proofs[k].data(), proofs[k].size()); });
+        batch_out(proofs, st, proofs_out, cap, offsets, stats);
     });
+}
+
+int stwo_rocm_batch_prover_create(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params, uint32_t n_proofs,
+                                  stwo_rocm_batch_prover** out) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(out, "null argument");
+        *out = nullptr;
+        const ProveParams p = batch_params(params);
+        std::unique_ptr<stwo_rocm_batch_prover> h(new stwo_rocm_batch_prover());
+        h->p = batch_plan_create(c, p, n_proofs);
+        h->n = n_proofs;
+        *out = h.release();
+    });
+}
+
+int stwo_rocm_batch_prover_run(stwo_rocm_ctx* ctx, stwo_rocm_batch_prover* bp, const uint64_t* trace_seeds,
+                               int use_graph, uint8_t* proofs_out, uint64_t cap, uint64_t* offsets,
+                               stwo_rocm_batch_stats* stats) {
+    return guarded(ctx, [&](Ctx& c) {
+        SR_CHECK(bp && bp->p && offsets, "null argument");
+        std::vector<std::string> proofs;
+        BatchStats st;
+        batch_plan_run(c, *bp->p, trace_seeds, use_graph != 0, proofs, &st);
+        batch_out(proofs, st, proofs_out, cap, offsets, stats);
+    });
+}
+
+int stwo_rocm_batch_prover_destroy(stwo_rocm_ctx* ctx, stwo_rocm_batch_prover* bp) {
+    if (!bp) return STWO_ROCM_OK;
+    const int rc = guarded(ctx, [&](Ctx& c) {
+        c.sync();
+        batch_plan_free(bp->p);
+        bp->p = nullptr;
+    });
+    delete bp;
+    return rc;
 }
 
 int stwo_rocm_prove_poseidon(stwo_rocm_ctx* ctx, const stwo_rocm_poseidon_params* params, uint8_t* proof_out,

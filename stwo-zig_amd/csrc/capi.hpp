@@ -18,6 +18,10 @@ struct stwo_rocm_ctx {
 struct stwo_rocm_tree {
     DeviceTree* t;
 };
+struct stwo_rocm_batch_prover {
+    BatchPlan* p = nullptr;
+    uint32_t n = 0;
+};
 
 template <class F>
 static inline int guarded(stwo_rocm_ctx* ctx, F&& f) {

@@ -252,6 +252,17 @@ struct BatchStats {
 };
 void prove_wide_fibonacci_batch(Ctx& c, const ProveParams& prm, uint32_t n_proofs, const uint64_t* seeds,
                                 std::vector<std::string>& proofs, BatchStats* stats);
+// The same in two steps, for repeated batches of one shape: a plan holds the device allocation,
+// tables and pinned readback of n_proofs proofs; each run proves one batch of seeds.  With
+// use_graph the device part of the first run is recorded into a hipGraph and later runs
+// replay it (a plan whose quotients need a temporary buffer, or a profiled context, runs
+// eagerly).  Plans are tied to their context and freed before it.
+struct BatchPlan;
+BatchPlan* batch_plan_create(Ctx& c, const ProveParams& prm, uint32_t n_proofs);
+void batch_plan_run(Ctx& c, BatchPlan& plan, const uint64_t* seeds, bool use_graph, std::vector<std::string>& proofs,
+                    BatchStats* stats);
+void batch_plan_free(BatchPlan* plan);
+uint64_t batch_plan_slice_bytes(const BatchPlan& plan);
 
 constexpr int STAGES = 8;
 struct ProveStats {

@@ -152,6 +152,9 @@ _SIGNATURES = [
                                             ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ProveStats)]),
     ("stwo_rocm_prove_wide_fibonacci_batch", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint64,
                                                              _vp, _vp]),
+    ("stwo_rocm_batch_prover_create", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    ("stwo_rocm_batch_prover_run", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_uint64, _vp, _vp]),
+    ("stwo_rocm_batch_prover_destroy", ctypes.c_int, [_vp, _vp]),
     ("stwo_rocm_prove_poseidon", ctypes.c_int, [_vp, ctypes.POINTER(PoseidonParams), ctypes.c_char_p,
                                                  ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64),
                                                  ctypes.POINTER(ProveStats)]),
@@ -648,36 +651,42 @@ class Context:
                                    trace_seed=0):
         """Many wide-Fibonacci proofs of one shape in single launches: proof k uses seeds[k] (or
         trace_seed for all n_proofs when seeds is None).  Returns (list of proof bytes, stats dict)."""
-        if seeds is not None:
-            seeds = [int(x) for x in seeds]
-            n_proofs = len(seeds)
-            arr = (ctypes.c_uint64 * n_proofs)(*seeds)
-        else:
-            arr = None
         prm = WideFibParams(log_n_rows, sequence_len, pow_bits, log_blowup_factor, log_last_layer_degree_bound,
                             n_queries, trace_mode, trace_seed)
+        arr, n_proofs = _seed_array(seeds, n_proofs)
+        return self._batch_call(
+            lambda buf, cap, offs, st: self.lib.stwo_rocm_prove_wide_fibonacci_batch(
+                self.h, ctypes.byref(prm), n_proofs, arr, buf, cap, offs, ctypes.byref(st)),
+            n_proofs, log_n_rows + log_blowup_factor, sequence_len, n_queries, log_last_layer_degree_bound)
+
+    def batch_prover(self, log_n_rows, sequence_len, n_proofs, pow_bits=10, log_blowup_factor=1,
+                     log_last_layer_degree_bound=0, n_queries=3, trace_mode=1, trace_seed=0):
+        """A reusable plan for batches of n_proofs proofs of one shape (BatchProver.run)."""
+        return BatchProver(self, log_n_rows, sequence_len, n_proofs, pow_bits, log_blowup_factor,
+                           log_last_layer_degree_bound, n_queries, trace_mode, trace_seed)
+
+    def _batch_call(self, call, n_proofs, L, sequence_len, n_queries, last_bound):
         offs = (ctypes.c_uint64 * (n_proofs + 1))()
         st = BatchStats()
         # an upper bound of one proof's wire size (sampled values, commitments, hash witnesses
         # of every tree and FRI layer, queried values, the last layer), kept in one buffer reused
         # across calls
-        L = log_n_rows + log_blowup_factor
         per = (8192 + 48 * (sequence_len + 8) + 12 * n_queries * (sequence_len + 8) +
-               n_queries * (L + 1) * (4 * 132 + 48) + 48 * (1 << log_last_layer_degree_bound))
+               n_queries * (L + 1) * (4 * 132 + 48) + 48 * (1 << last_bound))
         cap = max(n_proofs * per, getattr(self, "_batch_cap", 0))
         while True:
             if getattr(self, "_batch_cap", 0) < cap:
                 self._batch_buf = ctypes.create_string_buffer(cap)
                 self._batch_cap = cap
             buf, cap = self._batch_buf, self._batch_cap
-            rc = self.lib.stwo_rocm_prove_wide_fibonacci_batch(self.h, ctypes.byref(prm), n_proofs, arr, buf, cap,
-                                                               offs, ctypes.byref(st))
+            rc = call(buf, cap, offs, st)
             if rc == 1 and offs[n_proofs] > cap:
                 cap = offs[n_proofs]
                 continue
             self._check(rc)
             addr = ctypes.addressof(buf)
-            proofs = [ctypes.string_at(addr + offs[k], offs[k + 1] - offs[k]) for k in range(n_proofs)]
+            o = list(offs)
+            proofs = [ctypes.string_at(addr + o[k], o[k + 1] - o[k]) for k in range(n_proofs)]
             phases = ("setup", "device_wait", "last_layers", "pow", "queries_gather", "encode")
             return proofs, {"device_ms": st.device_ms, "tail_ms": st.tail_ms, "total_ms": st.total_ms,
                             "slice_bytes": st.slice_bytes,
@@ -866,6 +875,50 @@ class BatchedPipeline:
         if self.h:
             self.ctx._check(self.ctx.lib.stwo_rocm_batched_destroy(self.ctx.h, self.h))
             self.h = None
+
+
+def _seed_array(seeds, n_proofs):
+    if seeds is None:
+        return None, int(n_proofs)
+    seeds = [int(x) for x in seeds]
+    return (ctypes.c_uint64 * len(seeds))(*seeds), len(seeds)
+
+
+class BatchProver:
+    """stwo_rocm_batch_prover_*: the device allocation, tables and pinned readback of a batch
+    shape, set up once; run(seeds) proves one batch (graph=True records the device part of the
+    first run into a hipGraph and replays it afterwards)."""
+
+    def __init__(self, ctx, log_n_rows, sequence_len, n_proofs, pow_bits, log_blowup_factor,
+                 log_last_layer_degree_bound, n_queries, trace_mode, trace_seed):
+        self.ctx, self.n = ctx, int(n_proofs)
+        self.shape = (log_n_rows + log_blowup_factor, sequence_len, n_queries, log_last_layer_degree_bound)
+        prm = WideFibParams(log_n_rows, sequence_len, pow_bits, log_blowup_factor, log_last_layer_degree_bound,
+                            n_queries, trace_mode, trace_seed)
+        h = _vp()
+        ctx._check(ctx.lib.stwo_rocm_batch_prover_create(ctx.h, ctypes.byref(prm), self.n, ctypes.byref(h)))
+        self.h = h
+
+    def run(self, seeds=None, graph=True):
+        arr, n = _seed_array(seeds, self.n)
+        if n != self.n:
+            raise ValueError("the plan proves %d proofs per run" % self.n)
+        L, W, q, lb = self.shape
+        return self.ctx._batch_call(
+            lambda buf, cap, offs, st: self.ctx.lib.stwo_rocm_batch_prover_run(
+                self.ctx.h, self.h, arr, 1 if graph else 0, buf, cap, offs, ctypes.byref(st)),
+            self.n, L, W, q, lb)
+
+    def close(self):
+        if self.h:
+            self.ctx._check(self.ctx.lib.stwo_rocm_batch_prover_destroy(self.ctx.h, self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class BatchStats(ctypes.Structure):

@@ -93,3 +93,35 @@ def test_batch_rejects_unsupported_shapes(ctx):
     p, _ = ctx.prove_wide_fibonacci_batch(8, 4, seeds=[3], **pcs)
     one, _ = ctx.prove_wide_fibonacci(8, 4, trace_mode=1, trace_seed=3, stats=False, **pcs)
     assert p[0] == one
+
+
+def test_batch_plan_graph_replays_equal_eager(ctx):
+    """A batch plan run eagerly, recorded into a hipGraph and replayed with new seeds, with the
+    program's own eager work and synchronisations between the replays: every replay's proofs
+    equal the one-proof path's for its seeds (the graph carries no memset node)."""
+    pcs = dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3)
+    with ctx.batch_prover(8, 6, 16, **pcs) as bp:
+        for rep, seeds in enumerate([list(range(1, 17)), list(range(100, 116)), list(range(1, 17)),
+                                     [7] * 16]):
+            print("plan run", rep, flush=True)
+            got, st = bp.run(seeds, graph=True)
+            eager, _ = bp.run(seeds, graph=False)
+            assert got == eager
+            one, _ = ctx.prove_wide_fibonacci(8, 6, trace_mode=1, trace_seed=seeds[3], stats=False, **pcs)
+            assert got[3] == one
+            ctx.synchronize()
+        other, _ = ctx.prove_wide_fibonacci_batch(8, 6, seeds=list(range(100, 116)), **pcs)
+        again, _ = bp.run(list(range(100, 116)), graph=True)
+        assert again == other
+
+
+def test_batch_plan_without_graph_form(ctx):
+    """More than 120 columns on at most 2^16 rows: the quotients use a temporary buffer, so the
+    plan runs eagerly even when a graph is asked for; results are unchanged."""
+    pcs = dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3)
+    with ctx.batch_prover(7, 125, 3, **pcs) as bp:
+        a, _ = bp.run([4, 5, 6], graph=True)
+        b, _ = bp.run([4, 5, 6], graph=True)
+    assert a == b
+    one, _ = ctx.prove_wide_fibonacci(7, 125, trace_mode=1, trace_seed=5, stats=False, **pcs)
+    assert a[1] == one

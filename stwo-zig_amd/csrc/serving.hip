@@ -83,6 +83,18 @@ void serve(ServePool& pool, int device, const std::vector<ServeJob>& jobs, uint3
     // Slots: resident contexts (kept in the pool between calls) with the twiddle tower and the
     // pinned staging ready, so no job pays set-up.
     std::vector<std::unique_ptr<Slot>>& slots = pool.slots;
+    if (hbm_budget) {
+        // Slots kept from earlier calls hold towers and domain tables of the sizes they served:
+        // when those (after trimming the cached blocks) leave no room for this call's largest
+        // job, the slots are rebuilt and hold only what this call needs.
+        uint64_t max_job = 0, held = 0;
+        for (const ServeJob& j : jobs) max_job = std::max(max_job, serve_job_bytes(j));
+        for (auto& sl : slots) {
+            sl->c.trim();
+            held += sl->c.bytes_live + sl->c.bytes_cached;
+        }
+        if (held + max_job > hbm_budget) slots.clear();
+    }
     while (slots.size() < n_slots) {
         std::unique_ptr<Slot> sl(new Slot());
         sl->c.ledger = &pool.ledger;

@@ -676,6 +676,13 @@ def _serve_ref_digest(oracle, job):
     return hashlib.blake2s(out).hexdigest()
 
 
+def serving_oracle_subset(jobs):
+    """Requests whose served result is recomputed by the oracle: the first request of every kind
+    in the stream, and four spread over it."""
+    first = {next(i for i, j in enumerate(jobs) if j[0] == k) for k in {j[0] for j in jobs}}
+    return sorted(first | {i for i in (0, 64, 128, 192) if i < len(jobs)})
+
+
 def serving_part(stwo_rocm, ctx, check_oracle, slot_counts=(1, 16)):
     """Config E on `ctx`: the 256-request stream served by one slot (the unbatched GPU path, one
     proof at a time) and by 16 slots (continuous batching); proofs/s and p50/p99 at 1500/s, and
@@ -730,7 +737,7 @@ def serving_part(stwo_rocm, ctx, check_oracle, slot_counts=(1, 16)):
     if check_oracle:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        subset = sorted({next(i for i, j in enumerate(jobs) if j[0] == k) for k in range(4)} | {0, 64, 128, 192})
+        subset = serving_oracle_subset(jobs)
         ok = all(_serve_ref_digest(oracle, jobs[i]) == digests[i] for i in subset)
         if not ok:
             raise SystemExit("served results differ from the oracle")

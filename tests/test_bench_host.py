@@ -75,3 +75,12 @@ def test_profiled_counters_parse_the_committed_summaries():
     c = bench.profiled_counters()
     assert set(c) == {"traffic", "valu"}
     assert all(v > 0 for v in c["traffic"].values()) and all(v > 0 for v in c["valu"].values())
+
+
+def test_serving_oracle_subset_covers_every_kind():
+    import bench
+
+    jobs = bench.serving_requests()
+    sub = bench.serving_oracle_subset(jobs)
+    assert {jobs[i][0] for i in sub} == {j[0] for j in jobs}
+    assert all(0 <= i < len(jobs) for i in sub) and sub == sorted(set(sub))

@@ -63,11 +63,24 @@ struct BatchPlan {
     bool graph_ok = false;    // no temporary buffer inside the device sequence
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // the context's twiddle tower the recorded launches read: growing the tower moves it (the
+    // old arrays go to the arena cache, where a trim frees them), so a replay re-records then
+    const uint32_t* cap_tower[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t cap_tower_log = 0;
     Ctx* ctx = nullptr;
     ~BatchPlan() {
+        drop_graph();
+        if (h_rb) (void)hipHostFree(h_rb);
+    }
+    void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        if (h_rb) (void)hipHostFree(h_rb);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    bool tower_moved(const Ctx& c) const {
+        return cap_tower[0] != c.d_tw || cap_tower[1] != c.d_itw || cap_tower[2] != c.d_tw2 ||
+               cap_tower[3] != c.d_itw2 || cap_tower_log != c.tower_log;
     }
     uint8_t* P0(size_t at) const { return slab.as<uint8_t>() + at; }
     uint8_t* PK(uint32_t k, size_t at) const { return slab.as<uint8_t>() + (uint64_t)k * S + at; }
@@ -365,6 +378,10 @@ void batch_plan_run(Ctx& c, BatchPlan& P, const uint64_t* seeds, bool use_graph,
     // a graph replays the recorded launches; the stage profile needs its own events, so a
     // profiled context runs the sequence eagerly
     const bool graph = use_graph && P.graph_ok && !c.profiling;
+    if (graph && P.exec && P.tower_moved(c)) {
+        SR_HIP(hipStreamSynchronize(c.stream));  // an earlier replay may still run
+        P.drop_graph();
+    }
     if (graph && !P.exec) {
         SR_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
         try {
@@ -377,6 +394,11 @@ void batch_plan_run(Ctx& c, BatchPlan& P, const uint64_t* seeds, bool use_graph,
         }
         SR_HIP(hipStreamEndCapture(c.stream, &P.graph));
         SR_HIP(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+        P.cap_tower[0] = c.d_tw;
+        P.cap_tower[1] = c.d_itw;
+        P.cap_tower[2] = c.d_tw2;
+        P.cap_tower[3] = c.d_itw2;
+        P.cap_tower_log = c.tower_log;
     }
     if (graph) SR_HIP(hipGraphLaunch(P.exec, c.stream));
     else batch_enqueue(c, P);

@@ -11,6 +11,7 @@ import hashlib
 import json
 import os
 
+import numpy as np
 import pytest
 
 import oracle
@@ -113,6 +114,32 @@ def test_batch_plan_graph_replays_equal_eager(ctx):
         other, _ = ctx.prove_wide_fibonacci_batch(8, 6, seeds=list(range(100, 116)), **pcs)
         again, _ = bp.run(list(range(100, 116)), graph=True)
         assert again == other
+
+
+def test_batch_plan_replay_after_the_tower_moves():
+    """A recorded plan reads the context's twiddle tower.  Growing the tower (a larger
+    transform on the context) returns the old arrays to the arena cache and release_cached
+    frees them; the next graph run must re-record against the new tower, not read freed
+    memory, and its proofs stay equal to the first run's."""
+    pcs = dict(pow_bits=10, log_blowup_factor=1, log_last_layer_degree_bound=0, n_queries=3)
+    seeds = list(range(50, 58))
+    with stwo_rocm.Context(0) as c2:  # a fresh context: its tower covers only this plan
+        with c2.batch_prover(9, 5, 8, **pcs) as bp:
+            first, _ = bp.run(seeds, graph=True)
+            builds = c2.runtime_info()["tower_builds"]
+            c2.prepare_twiddles(24)  # the tower grows: the old arrays go to the cache
+            c2.release_cached()      # ... and are freed
+            assert c2.runtime_info()["tower_builds"] == builds + 1
+            junk = [c2.to_device(np.full(1 << 20, 0x5A5A5A5A, dtype=np.uint32)) for _ in range(8)]
+            try:
+                for _ in range(2):
+                    again, _ = bp.run(seeds, graph=True)
+                    assert again == first
+            finally:
+                for j in junk:
+                    j.free()
+        one, _ = c2.prove_wide_fibonacci(9, 5, trace_mode=1, trace_seed=seeds[2], stats=False, **pcs)
+        assert first[2] == one
 
 
 def test_batch_plan_without_graph_form(ctx):

@@ -141,22 +141,61 @@ __device__ __forceinline__ uint32_t rx_lds(uint32_t j, uint32_t lane) {
     return LOGC == 0 ? (j + (j >> 4)) : ((j << LOGC) | lane);
 }
 
+// LDS words of a thread's 16 values with row bits [B, B+4) (row0 has those bits zero): for
+// interleaved columns, for B == 0 and for B >= 4 the skewed index of value k is the index of
+// value 0 plus k * STRIDE (the row bits of k never carry into the skew), so the 16 accesses
+// take one address and immediate offsets; B = 1..3 of a contiguous tile index each value.
+template <int LOGC, int B>
+struct RxLdsMap {
+    static constexpr bool AFFINE = LOGC != 0 || B == 0 || B >= RX_LOG;
+    static constexpr uint32_t STRIDE =
+        LOGC != 0 ? (1u << (B + LOGC)) : (B == 0 ? 1u : (1u << B) + (1u << (B >= RX_LOG ? B - RX_LOG : 0)));
+};
+
+template <int LOGC, int B>
+__device__ __forceinline__ void rx_lds_put(uint32_t* lds, const uint32_t (&v)[RX], uint32_t row0, uint32_t lane) {
+    using M = RxLdsMap<LOGC, B>;
+    if constexpr (M::AFFINE) {
+        uint32_t* p = lds + rx_lds<LOGC>(row0, lane);
+#pragma unroll
+        for (int k = 0; k < RX; k++) p[k * M::STRIDE] = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < RX; k++) lds[rx_lds<LOGC>(row0 + ((uint32_t)k << B), lane)] = v[k];
+    }
+}
+
+template <int LOGC, int B>
+__device__ __forceinline__ void rx_lds_get(uint32_t (&v)[RX], const uint32_t* lds, uint32_t row0, uint32_t lane) {
+    using M = RxLdsMap<LOGC, B>;
+    if constexpr (M::AFFINE) {
+        const uint32_t* p = lds + rx_lds<LOGC>(row0, lane);
+#pragma unroll
+        for (int k = 0; k < RX; k++) v[k] = p[k * M::STRIDE];
+    } else {
+#pragma unroll
+        for (int k = 0; k < RX; k++) v[k] = lds[rx_lds<LOGC>(row0 + ((uint32_t)k << B), lane)];
+    }
+}
+
 // Twiddles of one stage (doubled tower words): RX >> (Q+1) consecutive words of T_{s-1}
 // for a line stage; for the circle stage the two (x, y) pairs of T_0 the thread needs.
 template <bool INV, int R, int LOGC, int B, int Q>
 __device__ __forceinline__ void rx_tw_load(uint32_t (&w)[8], const uint32_t* __restrict__ tower, const RxPass& ps,
                                            uint32_t hi, uint32_t r, uint32_t base) {
     constexpr int LS = B + Q;
-    constexpr bool MAY_BE_CIRCLE = (LS == 0 && LOGC == 0);
+    // a contiguous pass (LOGC == 0) starts at stage 0 (the launcher checks it), so its local
+    // stage 0 is the circle stage; interleaved passes start above it
+    constexpr bool IS_CIRCLE = (LS == 0 && LOGC == 0);
     const uint32_t s = ps.s_lo + LS;
-    if (!MAY_BE_CIRCLE || s > 0) {
+    if constexpr (!IS_CIRCLE) {
         // h = (hi << (R-LS-1)) | (r_high << (3-Q)) | hk  with r_high = r >> B
         const uint32_t tb = ((1u << (ps.tower_log - 1)) - (1u << (ps.m - s))) + ((hi << (R - LS - 1)) |
                                                                                  ((r >> B) << (3 - Q)));
         const uint32_t* __restrict__ twp = tower + tb;  // constant offsets below
 #pragma unroll
         for (int hk = 0; hk < (RX >> (Q + 1)); hk++) w[hk] = twp[hk];
-    } else if constexpr (MAY_BE_CIRCLE) {
+    } else {
         const uint32_t* T0 = tower + ((1u << (ps.tower_log - 1)) - (1u << (ps.m - 1)));
         const uint32_t kk0 = ((base >> 1) | (r << 3)) >> 2;
 #pragma unroll
@@ -167,9 +206,8 @@ __device__ __forceinline__ void rx_tw_load(uint32_t (&w)[8], const uint32_t* __r
 template <bool INV, int R, int LOGC, int B, int Q>
 __device__ __forceinline__ void rx_stage(uint32_t (&v)[RX], const uint32_t (&w)[8], const RxPass& ps, bool dup) {
     constexpr int LS = B + Q;
-    constexpr bool MAY_BE_CIRCLE = (LS == 0 && LOGC == 0);
-    const uint32_t s = ps.s_lo + LS;
-    if (!MAY_BE_CIRCLE || s > 0) {
+    constexpr bool IS_CIRCLE = (LS == 0 && LOGC == 0);  // as in rx_tw_load
+    if constexpr (!IS_CIRCLE) {
 #pragma unroll
         for (int hk = 0; hk < (RX >> (Q + 1)); hk++) {
             const uint32_t tw = w[hk];
@@ -191,7 +229,7 @@ __device__ __forceinline__ void rx_stage(uint32_t (&v)[RX], const uint32_t (&w)[
                 }
             }
         }
-    } else if constexpr (MAY_BE_CIRCLE) {
+    } else {
         // Circle stage (B == 0, Q == 0, s_lo == 0): pairs (2h', 2h'+1), h' = hp0 + hk,
         // twiddles [y, -y, -x, x] from consecutive pairs (x, y) of T_0.
         const uint32_t x0 = w[0], y0 = w[1], x1 = w[2], y1 = w[3];
@@ -241,14 +279,17 @@ __device__ __forceinline__ void rx_group(uint32_t (&v)[RX], const uint32_t (&w)[
     }
 }
 
-// Group gi (compile time): forward top-down, inverse bottom-up; the outermost
-// group is aligned to the tile edge (B clamped) and may have fewer active stages.
+// Group gi (compile time): inverse bottom-up, forward top-down over the same row windows
+// [B, B+4) with B = 0, 4, 8, ... and the topmost window aligned to the tile edge (B = R - 4,
+// fewer active stages when 4 does not divide R).  Every B is then 0 or >= 4 once R >= 8,
+// which keeps the LDS exchanges affine (RxLdsMap).
 template <bool INV, int R, int GI>
 struct RxGroupShape {
-    static constexpr int TOP = R - GI * RX_LOG;
-    static constexpr int LO = INV ? GI * RX_LOG : (TOP >= RX_LOG ? TOP - RX_LOG : 0);
-    static constexpr int HI = INV ? (GI * RX_LOG + RX_LOG < R ? GI * RX_LOG + RX_LOG : R) : TOP;
-    static constexpr int B = INV ? (GI * RX_LOG + RX_LOG <= R ? GI * RX_LOG : R - RX_LOG) : LO;
+    static constexpr int NG = (R + RX_LOG - 1) / RX_LOG;
+    static constexpr int G = INV ? GI : NG - 1 - GI;  // window index from the bottom
+    static constexpr int LO = G * RX_LOG;
+    static constexpr int HI = G * RX_LOG + RX_LOG < R ? G * RX_LOG + RX_LOG : R;
+    static constexpr int B = G * RX_LOG + RX_LOG <= R ? G * RX_LOG : R - RX_LOG;
 };
 
 // w holds this group's twiddles (loaded before the previous exchange); the next group's
@@ -290,8 +331,7 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4]
             }
         } else {
             __syncthreads();
-#pragma unroll
-            for (int k = 0; k < RX; k++) v[k] = lds[rx_lds<LOGC>(row0 + ((uint32_t)k << B), lane)];
+            rx_lds_get<LOGC, B>(v, lds, row0, lane);
         }
         rx_group<INV, R, LOGC, B, S::LO, S::HI>(v, w, ps, GI == 0);
         if constexpr (GI + 1 < NG) {
@@ -299,8 +339,7 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4]
             uint32_t wn[4][8];
             rx_group_tw<INV, R, LOGC, SN::B, SN::LO, SN::HI>(wn, tower, ps, hi, r, base);
             if (GI > 0) __syncthreads();
-#pragma unroll
-            for (int k = 0; k < RX; k++) lds[rx_lds<LOGC>(row0 + ((uint32_t)k << B), lane)] = v[k];
+            rx_lds_put<LOGC, B>(lds, v, row0, lane);
             rx_run<INV, R, LOGC, GI + 1, FROM_REGS>(v, wn, lds, src, dst, tower, ps, hi, r, lane, base);
         } else {
             const uint32_t g0 = base | (row0 << ps.s_lo) | lane;

@@ -352,18 +352,15 @@ __global__ __launch_bounds__(TL_THREADS) void tree_layer_kernel(FriLayerArgs a0)
 // the parent levels (layers.zig:99-140) and the channel step on the root, separated by
 // workgroup barriers instead of kernel boundaries (prover/fri.zig:377-621 runs the same
 // sequence per layer).  The last fold (into last_log) stores the line only.
-__global__ __launch_bounds__(TAIL_THREADS) void fri_tail_kernel(FriTailArgs a0) {
-    FriTailArgs a = a0;
-    if (blockIdx.x) {
-        // proof blockIdx.x of a batch: its lines, trees, root copies and channel bstride bytes on
-        const uint64_t boff = (uint64_t)blockIdx.x * a.bstride;
-        for (uint32_t l = 0; l < FRI_TAIL_MAX_LOG + 2; l++) {
-            if (a.lines[l]) a.lines[l] = (uint4*)((uint8_t*)a.lines[l] + boff);
-            if (a.trees[l]) a.trees[l] += boff;
-            if (a.roots[l]) a.roots[l] = (uint32_t*)((uint8_t*)a.roots[l] + boff);
-        }
-        a.ch = (DevChannel*)((uint8_t*)a.ch + boff);
-    }
+__global__ __launch_bounds__(TAIL_THREADS) void fri_tail_kernel(const FriTailArgs a) {
+    // The per-log pointer arrays are read from the kernel arguments with the (uniform) log as
+    // index and proof blockIdx.x's offset added at each use: a local copy of the arguments
+    // indexed by a runtime log would live in scratch memory, and every scratch load's wait
+    // would also wait for the tree stores of the previous level.
+    const uint64_t boff = (uint64_t)blockIdx.x * a.bstride;
+    const auto at = [boff](auto* p) { return (decltype(p))((uint8_t*)p + boff); };
+    DevChannel* __restrict__ ch = at(a.ch);
+    const B2State leaf_seed = a.leaf_seed, node_seed = a.node_seed;
     __shared__ __align__(16) uint32_t hA[TAIL_THREADS * 8];
     __shared__ __align__(16) uint32_t hB[TAIL_THREADS / 2 * 8];
     __shared__ uint4 vs[TAIL_THREADS];  // current line
@@ -371,11 +368,11 @@ __global__ __launch_bounds__(TAIL_THREADS) void fri_tail_kernel(FriTailArgs a0) 
     const uint32_t t = threadIdx.x, q = t & 3;
     const QuadLane L(q);
     uint32_t l = a.top_log;
-    qm31 alpha = qload(a.ch->alpha);
+    qm31 alpha = qload(ch->alpha);
     uint32_t dg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (t < 4) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) dg[k] = a.ch->digest[k];
+        for (int k = 0; k < 8; k++) dg[k] = ch->digest[k];
     }
     bool first = true;
     for (;;) {
@@ -383,8 +380,9 @@ __global__ __launch_bounds__(TAIL_THREADS) void fri_tail_kernel(FriTailArgs a0) 
         const uint32_t* inv_x = a.itw_half - ((size_t)1 << l);
         qm31 r = qzero();
         if (t < n) {
-            const uint4 x = first ? a.lines[l][2 * t] : vs[2 * t];
-            const uint4 y = first ? a.lines[l][2 * t + 1] : vs[2 * t + 1];
+            const uint4* __restrict__ src = at(a.lines[l]);
+            const uint4 x = first ? src[2 * t] : vs[2 * t];
+            const uint4 y = first ? src[2 * t + 1] : vs[2 * t + 1];
             r = tr_fold(x, y, alpha, inv_x[t]);
         }
         first = false;
@@ -393,22 +391,22 @@ __global__ __launch_bounds__(TAIL_THREADS) void fri_tail_kernel(FriTailArgs a0) 
         const uint4 rv = make_uint4(r.c0.a, r.c0.b, r.c1.a, r.c1.b);
         if (t < n) {
             vs[t] = rv;
-            a.lines[l][t] = rv;
+            at(a.lines[l])[t] = rv;
         }
         if (l == a.last_log) break;
-        uint8_t* tree = a.trees[l];
+        uint8_t* tree = at(a.trees[l]);
         uint8_t* leaves = tree + (((size_t)1 << l) - 1) * 32;
         if (n <= TR_QUAD_MAX) {
             __syncthreads();
             for (uint32_t p = t >> 2; p < n; p += TAIL_THREADS / 4) {
                 const uint4 v = vs[p];
                 uint32_t hlo, hhi;
-                tr_leaf_quad(L, a.leaf_seed, qm31{{v.x, v.y}, {v.z, v.w}}, hlo, hhi);
+                tr_leaf_quad(L, leaf_seed, qm31{{v.x, v.y}, {v.z, v.w}}, hlo, hhi);
                 tr_put_quad(hA + 8 * p, leaves + (size_t)p * 32, q, hlo, hhi);
             }
         } else if (t < n) {
             uint32_t h[8];
-            tr_leaf_lane(a.leaf_seed, r, h);
+            tr_leaf_lane(leaf_seed, r, h);
             tr_put_lane(hA + 8 * t, leaves + (size_t)t * 32, h);
         }
         __syncthreads();
@@ -417,13 +415,13 @@ __global__ __launch_bounds__(TAIL_THREADS) void fri_tail_kernel(FriTailArgs a0) 
         uint32_t lev = l;
         for (uint32_t c = n >> 1; c >= 1; c >>= 1) {
             lev--;
-            tr_level<TAIL_THREADS>(in, out, c, tree + (((size_t)1 << lev) - 1) * 32, 0, a.node_seed, t, rlo, rhi);
+            tr_level<TAIL_THREADS>(in, out, c, tree + (((size_t)1 << lev) - 1) * 32, 0, node_seed, t, rlo, rhi);
             uint32_t* x = in;
             in = out;
             out = x;
         }
         if (t < 4) {
-            const qm31 na = tr_chan_step(L, q, dg, rlo, rhi, a.ch, a.roots[l]);
+            const qm31 na = tr_chan_step(L, q, dg, rlo, rhi, ch, a.roots[l] ? at(a.roots[l]) : nullptr);
             if (t == 0) qstore(na, sa);
         }
         __syncthreads();

@@ -265,9 +265,9 @@ __device__ __forceinline__ void rx_group_tw(uint32_t (&w)[4][8], const uint32_t*
 }
 
 template <bool INV, int R, int LOGC, int B, int LO, int HI>
-__device__ __forceinline__ void rx_group(uint32_t (&v)[RX], const uint32_t (&w)[4][8], const RxPass& ps, bool first) {
+__device__ __forceinline__ void rx_group(uint32_t (&v)[RX], const uint32_t (&w)[4][8], const RxPass& ps, bool dup) {
     if (!INV) {
-        if (B + 3 >= LO && B + 3 < HI) rx_stage<INV, R, LOGC, B, 3>(v, w[3], ps, first && ps.dup_top);
+        if (B + 3 >= LO && B + 3 < HI) rx_stage<INV, R, LOGC, B, 3>(v, w[3], ps, dup);
         if (B + 2 >= LO && B + 2 < HI) rx_stage<INV, R, LOGC, B, 2>(v, w[2], ps, false);
         if (B + 1 >= LO && B + 1 < HI) rx_stage<INV, R, LOGC, B, 1>(v, w[1], ps, false);
         if (B + 0 >= LO && B + 0 < HI) rx_stage<INV, R, LOGC, B, 0>(v, w[0], ps, false);
@@ -296,12 +296,17 @@ struct RxGroupShape {
 // twiddles are loaded before this group's exchange so their latency overlaps it.
 // FROM_REGS: the first group takes its 16 values from v instead of loading them (the fused
 // interpolate + extend kernel hands the coefficients over in registers).
-template <bool INV, int R, int LOGC, int GI, bool FROM_REGS = false>
+// SP (compile time): an inverse pass scales on store (ps.scale), a forward pass has a
+// zero-padded top stage (ps.dup_top): its values 8..15 are the zero partners, never read.
+// Strided HBM accesses address the uniform column pointer plus a uniform row offset with a
+// 32-bit per-lane byte offset (words < 2^30), which the SGPR-base global forms take as is.
+template <bool INV, int R, int LOGC, int GI, bool SP, bool FROM_REGS = false>
 __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4][8], uint32_t* lds,
                                        gcu32* __restrict__ src, gu32* __restrict__ dst,
                                        const uint32_t* __restrict__ tower, const RxPass& ps, uint32_t hi, uint32_t r,
                                        uint32_t lane, uint32_t base) {
     constexpr int NG = (R + RX_LOG - 1) / RX_LOG;
+    constexpr bool DUP = !INV && SP;
     if constexpr (GI < NG) {
         using S = RxGroupShape<INV, R, GI>;
         constexpr int B = S::B;
@@ -323,27 +328,32 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4]
                     v[4 * k + 3] = w.w;
                 }
             } else {
+                const uint32_t sh = B + ps.s_lo, voff = g0 << 2;
 #pragma unroll
                 for (int k = 0; k < RX; k++) {
-                    const uint32_t idx = g0 + ((uint32_t)k << (B + ps.s_lo));
-                    v[k] = idx < src_n ? src[idx] : 0u;
+                    if (DUP && k >= RX / 2) {
+                        v[k] = 0u;
+                        continue;
+                    }
+                    const SR_GLOBAL uint8_t* sk = (const SR_GLOBAL uint8_t*)src + ((uint64_t)k << (sh + 2));
+                    v[k] = g0 + ((uint32_t)k << sh) < src_n ? *(gcu32*)(sk + voff) : 0u;
                 }
             }
         } else {
             __syncthreads();
             rx_lds_get<LOGC, B>(v, lds, row0, lane);
         }
-        rx_group<INV, R, LOGC, B, S::LO, S::HI>(v, w, ps, GI == 0);
+        rx_group<INV, R, LOGC, B, S::LO, S::HI>(v, w, ps, DUP && GI == 0);
         if constexpr (GI + 1 < NG) {
             using SN = RxGroupShape<INV, R, GI + 1>;
             uint32_t wn[4][8];
             rx_group_tw<INV, R, LOGC, SN::B, SN::LO, SN::HI>(wn, tower, ps, hi, r, base);
             if (GI > 0) __syncthreads();
             rx_lds_put<LOGC, B>(lds, v, row0, lane);
-            rx_run<INV, R, LOGC, GI + 1, FROM_REGS>(v, wn, lds, src, dst, tower, ps, hi, r, lane, base);
+            rx_run<INV, R, LOGC, GI + 1, SP, FROM_REGS>(v, wn, lds, src, dst, tower, ps, hi, r, lane, base);
         } else {
             const uint32_t g0 = base | (row0 << ps.s_lo) | lane;
-            if (ps.scale != 1) {
+            if constexpr (INV && SP) {
 #pragma unroll
                 for (int k = 0; k < RX; k++) v[k] = mmul(v[k], ps.scale);
             }
@@ -352,14 +362,16 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4]
 #pragma unroll
                 for (int k = 0; k < RX / 4; k++) d4[k] = u32x4{v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]};
             } else {
+                const uint32_t sh = B + ps.s_lo, voff = g0 << 2;
 #pragma unroll
-                for (int k = 0; k < RX; k++) dst[g0 + ((uint32_t)k << (B + ps.s_lo))] = v[k];
+                for (int k = 0; k < RX; k++)
+                    *(gu32*)((SR_GLOBAL uint8_t*)dst + ((uint64_t)k << (sh + 2)) + voff) = v[k];
             }
         }
     }
 }
 
-template <bool INV, int R, int LOGC>
+template <bool INV, int R, int LOGC, bool SP>
 __global__ __launch_bounds__(512) void fft_rx_kernel(const uint32_t* const* __restrict__ src_cols,
                                                      uint32_t* const* __restrict__ dst_cols,
                                                      const uint32_t* __restrict__ tower, RxPass ps) {
@@ -379,7 +391,7 @@ __global__ __launch_bounds__(512) void fft_rx_kernel(const uint32_t* const* __re
     uint32_t w[4][8];
     using S0 = RxGroupShape<INV, R, 0>;
     rx_group_tw<INV, R, LOGC, S0::B, S0::LO, S0::HI>(w, tower, ps, hi, r, base);
-    rx_run<INV, R, LOGC, 0>(v, w, lds, (gcu32*)src_cols[col], (gu32*)dst_cols[col], tower, ps, hi, r, lane, base);
+    rx_run<INV, R, LOGC, 0, SP>(v, w, lds, (gcu32*)src_cols[col], (gu32*)dst_cols[col], tower, ps, hi, r, lane, base);
 }
 
 // Interpolation of log n fused with the extension to log n + 1 (blowup 2), over the top
@@ -413,7 +425,7 @@ __global__ __launch_bounds__(512) void fft_ifft_lde_kernel(const uint32_t* const
     {
         using S0 = RxGroupShape<true, R, 0>;
         rx_group_tw<true, R, LOGC, S0::B, S0::LO, S0::HI>(w, itower, pi, 0, r, base);
-        rx_run<true, R, LOGC, 0>(v, w, lds, (gcu32*)src_cols[col], (gu32*)coef_cols[col], itower, pi, 0, r, lane,
+        rx_run<true, R, LOGC, 0, true>(v, w, lds, (gcu32*)src_cols[col], (gu32*)coef_cols[col], itower, pi, 0, r, lane,
                                  base);
     }
     uint32_t cf[RX];
@@ -429,7 +441,8 @@ __global__ __launch_bounds__(512) void fft_ifft_lde_kernel(const uint32_t* const
         }
         rx_group_tw<false, R, LOGC, F0::B, F0::LO, F0::HI>(w, tower, pf, h, r, base | (h << (pf.s_lo + R)));
         __syncthreads();  // the previous transform's last exchange has been read
-        rx_run<false, R, LOGC, 0, true>(v, w, lds, nullptr, lde, tower, pf, h, r, lane, base | (h << (pf.s_lo + R)));
+        rx_run<false, R, LOGC, 0, false, true>(v, w, lds, nullptr, lde, tower, pf, h, r, lane,
+                                               base | (h << (pf.s_lo + R)));
     }
 }
 
@@ -500,9 +513,14 @@ static void run_pass(Ctx& c, bool inverse, const uint32_t* const* src, uint32_t*
 template <bool INV>
 static void rx_dispatch(uint32_t R, uint32_t logc, dim3 grid, dim3 block, size_t lds, hipStream_t st,
                         const uint32_t* const* src, uint32_t* const* dst, const uint32_t* tower, const RxPass& ps) {
+    // the inverse's scaled pass and the forward's zero-padded top pass run the SP form
+    const bool sp = INV ? ps.scale != 1u : ps.dup_top != 0u;
 #define SR_RX(RR, CC)                                                                                       \
     if (R == RR && logc == CC) {                                                                            \
-        hipLaunchKernelGGL((fft_rx_kernel<INV, RR, CC>), grid, block, lds, st, src, dst, tower, ps);        \
+        if (sp)                                                                                             \
+            hipLaunchKernelGGL((fft_rx_kernel<INV, RR, CC, true>), grid, block, lds, st, src, dst, tower, ps); \
+        else                                                                                                \
+            hipLaunchKernelGGL((fft_rx_kernel<INV, RR, CC, false>), grid, block, lds, st, src, dst, tower, ps); \
         return;                                                                                             \
     }
     SR_RX(6, 0) SR_RX(7, 0) SR_RX(8, 0) SR_RX(9, 0) SR_RX(10, 0) SR_RX(11, 0) SR_RX(12, 0) SR_RX(13, 0)

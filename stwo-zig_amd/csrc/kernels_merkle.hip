@@ -47,10 +47,26 @@ __device__ __forceinline__ void leaf_block_load(const LeafColDev* __restrict__ c
     }
 }
 
+// The same for a segment whose columns all have the segment's size (lift(p) = p): every
+// word is the column pointer (uniform) plus the lane's byte offset p * 4 (rows < 2^30), and
+// the padding mask applies only to a padded last block (MASKED).
+template <bool MASKED>
+__device__ __forceinline__ void leaf_block_load_ident(const LeafColDev* __restrict__ cd, uint32_t voff,
+                                                      uint32_t m[16], uint64_t boff) {
+#pragma unroll
+    for (int w = 0; w < 16; w++) {
+        const LeafColDev d = cd[w];
+        const uint32_t x = *(gcu32*)((const SR_GLOBAL uint8_t*)d.col + boff + voff);
+        m[w] = MASKED ? (x & d.mask) : x;
+    }
+}
+
 // One state per lane over blocks [blk0, blk0 + nblk) of a segment at `level`: the
 // lane starts from the seed (first segment) or from the previous segment's state
 // at the lifted row, loads the 16 words of the next block while the current one is
-// compressed, and stores the state (the leaf hash after the last block).
+// compressed, and stores the state (the leaf hash after the last block).  IDENT: every
+// column of the segment has its size (no lift; the blocks alternate between two buffers).
+template <bool IDENT>
 __global__ __launch_bounds__(MK_THREADS) void merkle_leaves_kernel(const LeafColDev* __restrict__ cols,
                                                                    uint32_t blk0, uint32_t nblk, uint32_t n_cols,
                                                                    uint32_t level, const uint32_t* __restrict__ in,
@@ -74,18 +90,41 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_leaves_kernel(const LeafCol
     }
     const uint32_t nblocks_total = (n_cols + 15) >> 4;
     const uint32_t blk_end = blk0 + nblk;
-    uint32_t cur[16];
-    leaf_block_load(cols + 16 * blk0, p, cur, boff);
-    for (uint32_t blk = blk0; blk < blk_end; blk++) {
-        uint32_t nxt[16];
-        const bool more = blk + 1 < blk_end;
-        if (more) leaf_block_load(cols + 16 * (blk + 1), p, nxt, boff);
-        const bool last = blk + 1 == nblocks_total;
-        const uint32_t t = 64 + (last ? 4 * n_cols : 64 * (blk + 1));
-        b2_compress(h, cur, t, last);
-        if (more) {
+    if constexpr (IDENT) {
+        // blocks of 16 real columns, then at most one padded block (the last of the leaf)
+        const uint32_t full_end = ((n_cols & 15) && blk_end == nblocks_total) ? blk_end - 1 : blk_end;
+        const uint32_t voff = p << 2;
+        uint32_t a[16], b[16];
+        uint32_t blk = blk0;
+        if (blk < full_end) leaf_block_load_ident<false>(cols + 16 * blk, voff, a, boff);
+        for (; blk + 1 < full_end; blk += 2) {
+            leaf_block_load_ident<false>(cols + 16 * (blk + 1), voff, b, boff);
+            b2_compress(h, a, 64 + 64 * (blk + 1), blk + 1 == nblocks_total);
+            if (blk + 2 < full_end) leaf_block_load_ident<false>(cols + 16 * (blk + 2), voff, a, boff);
+            b2_compress(h, b, 64 + 64 * (blk + 2), blk + 2 == nblocks_total);
+        }
+        if (blk < full_end) {
+            b2_compress(h, a, 64 + 64 * (blk + 1), blk + 1 == nblocks_total);
+            blk++;
+        }
+        if (blk < blk_end) {
+            leaf_block_load_ident<true>(cols + 16 * blk, voff, a, boff);
+            b2_compress(h, a, 64 + 4 * n_cols, true);
+        }
+    } else {
+        uint32_t cur[16];
+        leaf_block_load(cols + 16 * blk0, p, cur, boff);
+        for (uint32_t blk = blk0; blk < blk_end; blk++) {
+            uint32_t nxt[16];
+            const bool more = blk + 1 < blk_end;
+            if (more) leaf_block_load(cols + 16 * (blk + 1), p, nxt, boff);
+            const bool last = blk + 1 == nblocks_total;
+            const uint32_t t = 64 + (last ? 4 * n_cols : 64 * (blk + 1));
+            b2_compress(h, cur, t, last);
+            if (more) {
 #pragma unroll
-            for (int w = 0; w < 16; w++) cur[w] = nxt[w];
+                for (int w = 0; w < 16; w++) cur[w] = nxt[w];
+            }
         }
     }
     store_hash(out, p, h);
@@ -297,13 +336,16 @@ LeafPlan merkle_leaf_plan(const uint32_t* const* h_cols, const uint32_t* h_logs,
         sg.nblk++;
         for (uint32_t w = 0; w < 16; w++) {
             const uint32_t i = 16 * k + w;
+            // padding (only in the last block, of the final segment) reads the last column, whose
+            // log is the final level: in range for both load forms, then masked to zero
             const bool real = i < n_cols;
-            const uint32_t lg = real ? h_logs[i] : h_logs[0];
+            const uint32_t lg = real ? h_logs[i] : h_logs[n_cols - 1];
             LeafColDev& d = plan.tab[i];
-            d.col = real ? h_cols[i] : h_cols[0];
+            d.col = real ? h_cols[i] : h_cols[n_cols - 1];
             d.shift = level - lg + 1;
             d.mask = real ? 0xffffffffu : 0u;
             if (real) sg.col_words += 1ull << lg;
+            if (real && d.shift != 1) sg.ident = false;
         }
     }
     return plan;
@@ -340,9 +382,15 @@ void launch_merkle_leaves_dev(Ctx& c, const LeafColDev* d_tab, const std::vector
                                c.stream, d_tab, sg.blk0, sg.nblk, n_cols, sg.level, prev,
                                prev ? sg.level - prev_level + 1 : 0u, c.leaf_seed, out, bstride);
         } else {
-            hipLaunchKernelGGL(merkle_leaves_kernel, dim3((uint32_t)((n + MK_THREADS - 1) / MK_THREADS), batch),
-                               dim3(MK_THREADS), 0, c.stream, d_tab, sg.blk0, sg.nblk, n_cols, sg.level, prev,
-                               prev ? sg.level - prev_level + 1 : 0u, c.leaf_seed, out, bstride);
+            const dim3 grid((uint32_t)((n + MK_THREADS - 1) / MK_THREADS), batch);
+            if (sg.ident)
+                hipLaunchKernelGGL(merkle_leaves_kernel<true>, grid, dim3(MK_THREADS), 0, c.stream, d_tab, sg.blk0,
+                                   sg.nblk, n_cols, sg.level, prev, prev ? sg.level - prev_level + 1 : 0u,
+                                   c.leaf_seed, out, bstride);
+            else
+                hipLaunchKernelGGL(merkle_leaves_kernel<false>, grid, dim3(MK_THREADS), 0, c.stream, d_tab, sg.blk0,
+                                   sg.nblk, n_cols, sg.level, prev, prev ? sg.level - prev_level + 1 : 0u,
+                                   c.leaf_seed, out, bstride);
         }
         SR_HIP(hipGetLastError());
         prev = (const uint32_t*)out;

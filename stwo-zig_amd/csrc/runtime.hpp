@@ -225,6 +225,7 @@ static_assert(sizeof(LeafColDev) == 16, "leaf column layout");
 struct LeafSeg {
     uint32_t blk0, nblk, level;
     uint64_t col_words;  // sum of the segment's column sizes (words): algorithmic byte count only
+    bool ident = true;   // every real column of the segment has the segment's log (row p reads row p)
 };
 // Descriptor table (16 entries per message block) and the segment plan for columns sorted
 // by log size; every column's shift is relative to its segment's level.

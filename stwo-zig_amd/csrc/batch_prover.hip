@@ -313,7 +313,7 @@ static void batch_enqueue(Ctx& c, BatchPlan& P) {
     uint32_t* qc0[4];
     for (uint32_t q = 0; q < 4; q++) qc0[q] = P.col(0, P.at_quot, LN, q);
     launch_quotients_dev(c, (const QEntryDev*)P.P0(P.at_ents), NS, (const QuotientBatchDev*)P.P0(P.at_bat), 1,
-                         4.0 * NS * (double)LN, L, qc0, ftree0.layer(L), K, S);
+                         4.0 * NS * (double)LN, L, qc0, ftree0.layer(L), K, S, true);
     launch_merkle_parents(c, ftree0, d_ch, d_roots, K, S, d_cnt);
 
     // FRI inner layers

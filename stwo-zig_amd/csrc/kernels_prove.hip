@@ -38,6 +38,60 @@ __device__ __forceinline__ const T* boffp(const T* p, uint64_t boff) {
     return (const T*)((const uint8_t*)p + boff);
 }
 
+// The numerator of batch bt at `row`: sum over its entries of coeff * column value, minus the
+// batch's linear term.  IDENT: every entry's column has the domain's size (lift(row) = row), so
+// each word is the entry's column pointer (uniform) plus the lane's byte offset row * 4.
+template <bool IDENT>
+__device__ __forceinline__ qm31 q_batch_num(const QEntryDev* __restrict__ entries, const QuotientBatchDev& bt,
+                                            uint32_t row, uint32_t y, uint64_t boff) {
+    constexpr uint32_t QB = 8;
+    const uint32_t voff = row << 2;
+    auto word = [&](const QEntryDev& en) -> uint32_t {
+        if constexpr (IDENT) return *(gcu32*)((const SR_GLOBAL uint8_t*)en.col + boff + voff);
+        else return ent_word(en, boff, lift_row(row, en.shift));
+    };
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    uint32_t e = bt.first;
+    const uint32_t end = bt.first + bt.count;
+    for (; e + QB <= end; e += QB) {
+        uint32_t v[QB];
+#pragma unroll
+        for (uint32_t k = 0; k < QB; k++) v[k] = word(entries[e + k]);
+#pragma unroll
+        for (uint32_t h = 0; h < QB; h += 4) {
+            uint64_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+            for (uint32_t k = h; k < h + 4; k++) {
+                const QEntryDev& en = entries[e + k];
+                t0 += (uint64_t)en.coeff[0] * v[k];
+                t1 += (uint64_t)en.coeff[1] * v[k];
+                t2 += (uint64_t)en.coeff[2] * v[k];
+                t3 += (uint64_t)en.coeff[3] * v[k];
+            }
+            s0 += mfold(t0);
+            s1 += mfold(t1);
+            s2 += mfold(t2);
+            s3 += mfold(t3);
+        }
+    }
+    for (; e < end; e++) {
+        const QEntryDev& en = entries[e];
+        const uint32_t v = word(en);
+        s0 += mfold((uint64_t)en.coeff[0] * v);
+        s1 += mfold((uint64_t)en.coeff[1] * v);
+        s2 += mfold((uint64_t)en.coeff[2] * v);
+        s3 += mfold((uint64_t)en.coeff[3] * v);
+    }
+    const qm31 num = {{mred64(s0), mred64(s1)}, {mred64(s2), mred64(s3)}};
+    return qsub(num, qadd(qmulm(bt.sum_a, y), bt.sum_b));
+}
+
+// Quotient row of every batch: acc = sum_b num_b / den_b.  Up to Q_INV_BATCHES batches share
+// one CM31 inversion (Montgomery's trick over the row's denominators: prefix products, one
+// inverse, then each inverse from the running inverse and the prefix before it); more
+// batches invert each denominator.  Field arithmetic is exact, so both forms give the same row.
+constexpr uint32_t Q_INV_BATCHES = 4;
+template <bool IDENT>
 __global__ __launch_bounds__(256) void quotients_kernel(const QEntryDev* __restrict__ entries,
                                                         const QuotientBatchDev* __restrict__ batches,
                                                         uint32_t n_batches, uint32_t log_size,
@@ -54,49 +108,37 @@ __global__ __launch_bounds__(256) void quotients_kernel(const QEntryDev* __restr
     entries = (const QEntryDev*)((const uint8_t*)entries + boff);
     batches = (const QuotientBatchDev*)((const uint8_t*)batches + boff);
     const uint32_t x = domain_xy[row], y = domain_xy[n + row];
-    constexpr uint32_t QB = 8;
     qm31 acc = qzero();
-    for (uint32_t bi = 0; bi < n_batches; bi++) {
-        const QuotientBatchDev bt = batches[bi];
-        uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-        uint32_t e = bt.first;
-        const uint32_t end = bt.first + bt.count;
-        for (; e + QB <= end; e += QB) {
-            uint32_t v[QB];
+    if (n_batches <= Q_INV_BATCHES) {
+        qm31 num[Q_INV_BATCHES];
+        cm31 den[Q_INV_BATCHES], pre[Q_INV_BATCHES];
 #pragma unroll
-            for (uint32_t k = 0; k < QB; k++) {
-                const QEntryDev& en = entries[e + k];
-                v[k] = ent_word(en, boff, lift_row(row, en.shift));
-            }
-#pragma unroll
-            for (uint32_t h = 0; h < QB; h += 4) {
-                uint64_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
-#pragma unroll
-                for (uint32_t k = h; k < h + 4; k++) {
-                    const QEntryDev& en = entries[e + k];
-                    t0 += (uint64_t)en.coeff[0] * v[k];
-                    t1 += (uint64_t)en.coeff[1] * v[k];
-                    t2 += (uint64_t)en.coeff[2] * v[k];
-                    t3 += (uint64_t)en.coeff[3] * v[k];
-                }
-                s0 += mfold(t0);
-                s1 += mfold(t1);
-                s2 += mfold(t2);
-                s3 += mfold(t3);
+        for (uint32_t bi = 0; bi < Q_INV_BATCHES; bi++) {
+            if (bi < n_batches) {
+                const QuotientBatchDev& bt = batches[bi];
+                num[bi] = q_batch_num<IDENT>(entries, bt, row, y, boff);
+                den[bi] = cadd(csub(bt.det, cmulm(bt.piy, x)), cmulm(bt.pix, y));
+                pre[bi] = bi == 0 ? den[bi] : cmul(pre[bi - 1], den[bi]);
+            } else {
+                pre[bi] = pre[bi - 1];  // bi >= 1: n_batches >= 1
             }
         }
-        for (; e < end; e++) {
-            const QEntryDev& en = entries[e];
-            const uint32_t v = ent_word(en, boff, lift_row(row, en.shift));
-            s0 += mfold((uint64_t)en.coeff[0] * v);
-            s1 += mfold((uint64_t)en.coeff[1] * v);
-            s2 += mfold((uint64_t)en.coeff[2] * v);
-            s3 += mfold((uint64_t)en.coeff[3] * v);
+        cm31 inv = cinv(pre[Q_INV_BATCHES - 1]);  // the inverse of the product of all denominators
+#pragma unroll
+        for (int bi = (int)Q_INV_BATCHES - 1; bi >= 0; bi--) {
+            if ((uint32_t)bi < n_batches) {
+                const cm31 ib = bi == 0 ? inv : cmul(inv, pre[bi - 1]);
+                if (bi > 0) inv = cmul(inv, den[bi]);
+                acc = qadd(acc, qmulc(num[bi], ib));
+            }
         }
-        qm31 num = {{mred64(s0), mred64(s1)}, {mred64(s2), mred64(s3)}};
-        num = qsub(num, qadd(qmulm(bt.sum_a, y), bt.sum_b));
-        const cm31 den = cadd(csub(bt.det, cmulm(bt.piy, x)), cmulm(bt.pix, y));
-        acc = qadd(acc, qmulc(num, cinv(den)));
+    } else {
+        for (uint32_t bi = 0; bi < n_batches; bi++) {
+            const QuotientBatchDev bt = batches[bi];
+            const qm31 num = q_batch_num<IDENT>(entries, bt, row, y, boff);
+            const cm31 den = cadd(csub(bt.det, cmulm(bt.piy, x)), cmulm(bt.pix, y));
+            acc = qadd(acc, qmulc(num, cinv(den)));
+        }
     }
     boffp(o0, boff)[row] = acc.c0.a;
     boffp(o1, boff)[row] = acc.c0.b;
@@ -225,12 +267,14 @@ void launch_quotients(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_c
                       uint32_t n_batches, uint32_t log_size, uint32_t* const* out, uint8_t* d_leaf_out) {
     std::vector<QEntryDev> ents(n_entries);
     double in_bytes = 0;
+    bool ident = true;
     for (uint32_t e = 0; e < n_entries; e++) {
         const uint32_t col = h_entry_cols[e];
         SR_CHECK(h_col_logs[col] >= 1 && h_col_logs[col] <= log_size, "quotients: column larger than the domain");
         ents[e].col = h_cols[col];
         ents[e].shift = log_size - h_col_logs[col] + 1;
         ents[e].pad = 0;
+        ident = ident && ents[e].shift == 1;
         for (int k = 0; k < 4; k++) ents[e].coeff[k] = h_entry_coeffs[4 * e + k];
         in_bytes += 4.0 * (double)(1ull << h_col_logs[col]);
     }
@@ -239,12 +283,12 @@ void launch_quotients(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_c
     DBuf d_ent = upload(c, ents.data(), ents.size());
     DBuf d_bat = upload(c, h_batches, n_batches);
     launch_quotients_dev(c, d_ent.as<QEntryDev>(), n_entries, d_bat.as<QuotientBatchDev>(), n_batches, in_bytes,
-                         log_size, out, d_leaf_out);
+                         log_size, out, d_leaf_out, 1, 0, ident);
 }
 
 void launch_quotients_dev(Ctx& c, const QEntryDev* d_ent, uint32_t n_entries, const QuotientBatchDev* d_bat,
                           uint32_t n_batches, double in_bytes, uint32_t log_size, uint32_t* const* out,
-                          uint8_t* d_leaf_out, uint32_t batch, uint64_t bstride) {
+                          uint8_t* d_leaf_out, uint32_t batch, uint64_t bstride, bool ident) {
     SR_CHECK(n_entries >= 1 && n_batches >= 1, "quotients: no samples");
     SR_CHECK(batch >= 1 && batch <= 65535, "quotients: batch size");
     const uint32_t* xy = c.domain_points(log_size);
@@ -261,8 +305,14 @@ void launch_quotients_dev(Ctx& c, const QEntryDev* d_ent, uint32_t n_entries, co
                            n_chunks, log_size, xy, (const uint32_t*)part.as<uint32_t>(), out[0], out[1], out[2],
                            out[3], d_leaf_out, c.leaf_seed, bstride, pstride);
     } else {
-        hipLaunchKernelGGL(quotients_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, c.stream, d_ent, d_bat,
-                           n_batches, log_size, xy, out[0], out[1], out[2], out[3], d_leaf_out, c.leaf_seed, bstride);
+        if (ident)
+            hipLaunchKernelGGL(quotients_kernel<true>, dim3((n + 255) / 256, batch), dim3(256), 0, c.stream, d_ent,
+                               d_bat, n_batches, log_size, xy, out[0], out[1], out[2], out[3], d_leaf_out,
+                               c.leaf_seed, bstride);
+        else
+            hipLaunchKernelGGL(quotients_kernel<false>, dim3((n + 255) / 256, batch), dim3(256), 0, c.stream, d_ent,
+                               d_bat, n_batches, log_size, xy, out[0], out[1], out[2], out[3], d_leaf_out,
+                               c.leaf_seed, bstride);
     }
     SR_HIP(hipGetLastError());
 }

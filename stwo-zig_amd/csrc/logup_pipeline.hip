@@ -155,7 +155,7 @@ std::vector<uint32_t> logup_xor_pipeline(Ctx& c, uint32_t log_n, uint64_t seed, 
         DBuf d_bat(&c, sizeof(QuotientBatchDev));
         launch_tr_quotient(c, d_ch, d_sv, NS, d_oods, d_ents, d_bat.as<QuotientBatchDev>());
         launch_quotients_dev(c, d_ents, NS, d_bat.as<QuotientBatchDev>(), 1, 4.0 * NS * (double)LN, L, qc,
-                             qtree->layer(L));
+                             qtree->layer(L), 1, 0, true);
         launch_merkle_parents(c, *qtree, nullptr, d_roots + 64);
         launch_tr_mix_draws(c, d_ch, (const uint32_t*)(d_roots + 64), 8, 0, nullptr);
     } catch (...) {

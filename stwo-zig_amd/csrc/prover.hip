@@ -813,6 +813,7 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
     SR_CHECK(all_coeffs.size() == NS, "sampled column count");
     const size_t at_cols = pack.add(all_coeffs.data(), NS);
     double q_in_bytes = 0;
+    bool q_ident = true;  // every sampled column has the quotient domain's size
     size_t at_ents;
     {
         // the coefficients are written on the device by the quotient transcript step
@@ -822,6 +823,7 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
             ents[i].col = all_lde[i];
             ents[i].shift = L - all_lde_logs[i] + 1;
             ents[i].pad = 0;
+            q_ident = q_ident && ents[i].shift == 1;
             for (int k = 0; k < 4; k++) ents[i].coeff[k] = 0;
             q_in_bytes += 4.0 * (double)(1ull << all_lde_logs[i]);
         }
@@ -923,7 +925,8 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         // FRI (prover/fri.zig:377-621).  The commit cascade continues on the device channel: the
         // launch that produces each layer's root mixes it and draws the next alpha, which the
         // fold reads from HBM.
-        launch_quotients_dev(c, d_ents, NS, d_bat.as<QuotientBatchDev>(), 1, q_in_bytes, L, qc, ftree0->layer(L));
+        launch_quotients_dev(c, d_ents, NS, d_bat.as<QuotientBatchDev>(), 1, q_in_bytes, L, qc, ftree0->layer(L), 1, 0,
+                             q_ident);
         launch_merkle_parents(c, *ftree0, d_ch, d_roots);
     }
     mark();  // quotients + first layer commit

@@ -382,9 +382,10 @@ void launch_quotients(Ctx& c, const uint32_t* const* h_cols, const uint32_t* h_c
 // one launch (grid dimension y, or z where y is taken), proof k's buffers -- every per-proof
 // pointer argument and every column pointer in a table -- at k * bstride bytes past proof 0's.
 // Shared read-only data (twiddles, domain tables, seeds) is not offset.
+// ident: every entry's shift is 1 (each column has the domain's size).
 void launch_quotients_dev(Ctx& c, const QEntryDev* d_ent, uint32_t n_entries, const QuotientBatchDev* d_bat,
                           uint32_t n_batches, double in_bytes, uint32_t log_size, uint32_t* const* out_coords,
-                          uint8_t* d_leaf_out, uint32_t batch = 1, uint64_t bstride = 0);
+                          uint8_t* d_leaf_out, uint32_t batch = 1, uint64_t bstride = 0, bool ident = false);
 // Point evaluation: sum_i coeff_i * w_i with weights of `point` (2^log).
 void launch_eval_weights(Ctx& c, const qm31 factors[32], uint32_t log_size, uint32_t* d_w);
 // The same with the log_size factors (QM31) in device memory.

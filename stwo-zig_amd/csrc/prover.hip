@@ -10,6 +10,7 @@
 // src/prover/fri.zig:163-621).
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <functional>
 #include <memory>
 #include <string>
@@ -1003,6 +1004,94 @@ void prove_wide_fibonacci(Ctx& c, const ProveParams& prm, const uint32_t* d_trac
         stats->peak_hbm_bytes = c.peak_live;
     }
     for (auto e : evs) (void)hipEventDestroy(e);
+}
+
+// ---- host worker pool (parallel_for).  Workers are started on first use and live until the
+// process ends (detached, waiting on a condition variable between jobs).  One job at a time:
+// the caller that takes the pool publishes the job under the mutex, takes indices itself, and
+// waits until every worker that joined the job has left it.
+namespace {
+struct HostPool {
+    std::mutex mu;
+    std::condition_variable wake, left;
+    std::atomic<bool> busy{false};
+    uint64_t gen = 0;  // job generation, guarded by mu
+    const std::function<void(uint32_t)>* fn = nullptr;
+    uint32_t n = 0, want = 0;  // indices; workers wanted by the job
+    uint32_t joined = 0, inside = 0;
+    std::atomic<uint32_t> next{0};
+    std::exception_ptr err;
+    uint32_t started = 0;
+
+    void take(const std::function<void(uint32_t)>& f, uint32_t count) {
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= count) return;
+            try {
+                f(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!err) err = std::current_exception();
+                next.store(count);
+                return;
+            }
+        }
+    }
+    void worker() {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            wake.wait(lk, [&] { return gen != seen && joined < want; });
+            seen = gen;
+            joined++;
+            inside++;
+            const std::function<void(uint32_t)>* f = fn;
+            const uint32_t count = n;
+            lk.unlock();
+            take(*f, count);
+            lk.lock();
+            if (--inside == 0) left.notify_all();
+        }
+    }
+};
+
+HostPool& host_pool() {
+    static HostPool* p = new HostPool();  // never destroyed: its workers outlive static destructors
+    return *p;
+}
+}  // namespace
+
+bool host_pool_run(uint32_t n, uint32_t threads, const std::function<void(uint32_t)>& f, std::exception_ptr& err) {
+    HostPool& P = host_pool();
+    bool expected = false;
+    if (!P.busy.compare_exchange_strong(expected, true)) return false;
+    struct Release {
+        HostPool& P;
+        ~Release() { P.busy.store(false); }
+    } release{P};
+    const uint32_t workers = threads - 1;
+    {
+        std::lock_guard<std::mutex> lk(P.mu);
+        while (P.started < workers) {
+            std::thread(&HostPool::worker, &P).detach();
+            P.started++;
+        }
+        P.fn = &f;
+        P.n = n;
+        P.want = workers;
+        P.joined = 0;
+        P.err = nullptr;
+        P.next.store(0);
+        P.gen++;
+    }
+    P.wake.notify_all();
+    P.take(f, n);
+    std::unique_lock<std::mutex> lk(P.mu);
+    P.want = 0;  // workers that have not joined yet stay out of this job
+    P.left.wait(lk, [&] { return P.inside == 0; });
+    err = P.err;
+    P.fn = nullptr;
+    return true;
 }
 
 }  // namespace sr

@@ -203,37 +203,28 @@ struct ProveParams {
 
 void trace_vanishing_inverses(Ctx& c, uint32_t n, uint32_t den_inv[2]);
 
-// f(0) .. f(n-1) on up to 16 host threads (the proofs' host phases are independent); the first
-// exception is rethrown after every thread has joined.
+// Runs f(0) .. f(n-1) on the process's host worker pool (up to `threads` - 1 workers plus the
+// calling thread) and returns true, or returns false without running anything when the pool is
+// taken by another caller (a nested call or another thread), who then runs its loop itself.
+// The first exception is stored in err after every index has finished or been skipped.
+bool host_pool_run(uint32_t n, uint32_t threads, const std::function<void(uint32_t)>& f, std::exception_ptr& err);
+
+// f(0) .. f(n-1) on up to 16 host threads (the proofs' host phases are independent): the
+// workers live for the process, so a call costs a wake-up rather than thread start-ups; the
+// first exception is rethrown after every index has finished or been skipped.
 template <class F>
 void parallel_for(uint32_t n, F&& f) {
     const uint32_t hw = std::max(1u, std::thread::hardware_concurrency());
     const uint32_t T = std::min(std::min(hw, 16u), n);
-    if (T <= 1) {
-        for (uint32_t i = 0; i < n; i++) f(i);
-        return;
+    if (T > 1) {
+        std::exception_ptr err;
+        const std::function<void(uint32_t)> fn = [&f](uint32_t i) { f(i); };
+        if (host_pool_run(n, T, fn, err)) {
+            if (err) std::rethrow_exception(err);
+            return;
+        }
     }
-    std::atomic<uint32_t> next{0};
-    std::exception_ptr err;
-    std::mutex mu;
-    std::vector<std::thread> th;
-    for (uint32_t t = 0; t < T; t++)
-        th.emplace_back([&] {
-            for (;;) {
-                const uint32_t i = next.fetch_add(1);
-                if (i >= n) return;
-                try {
-                    f(i);
-                } catch (...) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (!err) err = std::current_exception();
-                    next.store(n);
-                    return;
-                }
-            }
-        });
-    for (auto& x : th) x.join();
-    if (err) std::rethrow_exception(err);
+    for (uint32_t i = 0; i < n; i++) f(i);
 }
 
 

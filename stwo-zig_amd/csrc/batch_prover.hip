@@ -519,7 +519,13 @@ void batch_plan_run(Ctx& c, BatchPlan& P, const uint64_t* seeds, bool use_graph,
     }
     const double t_gat = now_host_ms();
     proofs.resize(K);
-    parallel_for(K, [&](uint32_t k) { proofs[k] = finish_encode(prm, pp[k].tail, pp[k].fp, h_words + gbase[k]); });
+    parallel_for(K, [&](uint32_t k) {
+        proofs[k] = finish_encode(prm, pp[k].tail, pp[k].fp, h_words + gbase[k]);
+        // the proof's host objects (tens of small vectors) are freed here, on the pool, rather
+        // than one after another when the vectors go out of scope
+        pp[k] = PerProof();
+        gk[k] = Gather();
+    });
     if (stats) {
         stats->device_ms = dev_ms;
         stats->tail_ms = now_host_ms() - t_dev;

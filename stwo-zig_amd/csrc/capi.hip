@@ -525,7 +525,7 @@ static ProveParams batch_params(const stwo_rocm_widefib_params* params) {
 }
 
 // Offsets, stats and the proofs back to back (the copy on host threads).
-static void batch_out(const std::vector<std::string>& proofs, const BatchStats& st, uint8_t* proofs_out,
+static void batch_out(std::vector<std::string>& proofs, const BatchStats& st, uint8_t* proofs_out,
                       uint64_t cap, uint64_t* offsets, stwo_rocm_batch_stats* stats) {
     const uint32_t n = (uint32_t)proofs.size();
     uint64_t total = 0;
@@ -542,7 +542,10 @@ static void batch_out(const std::vector<std::string>& proofs, const BatchStats& 
         for (int i = 0; i < 6; i++) stats->phase_ms[i] = st.phase_ms[i];
     }
     SR_CHECK(proofs_out && total <= cap, "proof buffer too small");
-    parallel_for(n, [&](uint32_t k) { memcpy(proofs_out + offsets[k], proofs[k].data(), proofs[k].size()); });
+    parallel_for(n, [&](uint32_t k) {
+        memcpy(proofs_out + offsets[k], proofs[k].data(), proofs[k].size());
+        std::string().swap(proofs[k]);  // freed on the pool
+    });
 }
 
 int stwo_rocm_prove_wide_fibonacci_batch(stwo_rocm_ctx* ctx, const stwo_rocm_widefib_params* params,

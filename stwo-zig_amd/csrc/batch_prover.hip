@@ -478,7 +478,9 @@ void batch_plan_run(Ctx& c, BatchPlan& P, const uint64_t* seeds, bool use_graph,
         DBuf d_pre = upload(c, pre.data(), pre.size());
         std::vector<uint64_t> best(K, UINT64_MAX);
         DBuf d_best = upload(c, best.data(), K);
-        const uint32_t cl = std::min(26u, std::max(12u, prm.pow_bits + 4));
+        // 2^(bits+3) nonces per proof and launch: a proof misses its range with probability
+        // e^-8, and only the proofs still without a nonce grind the next range
+        const uint32_t cl = std::min(26u, std::max(12u, prm.pow_bits + 3));
         const uint64_t chunk = 1ull << cl;
         for (uint64_t b0 = 0;; b0 += chunk) {
             launch_pow_grind_batch(c, d_pre.as<uint32_t>(), K, prm.pow_bits, b0, chunk,

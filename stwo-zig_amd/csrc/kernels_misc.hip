@@ -30,12 +30,15 @@ __global__ __launch_bounds__(256) void pow_kernel(B2State init, uint4 p0, uint4 
     if (i >= count) return;
     pow_try(init, p0, p1, bits, base + i, best);
 }
-// Prefix blockIdx.y of a batch: its 8 words at prefix + 8 y, its best nonce at best[y].
+// Prefix blockIdx.y of a batch: its 8 words at prefix + 8 y, its best nonce at best[y].  A
+// prefix whose nonce an earlier range found (best < base) is skipped; a value this launch
+// wrote (>= base) does not skip, so the lowest valid nonce is found either way.
 __global__ __launch_bounds__(256) void pow_batch_kernel(B2State init, const uint4* __restrict__ prefix, uint32_t bits,
                                                         uint64_t base, uint64_t count,
                                                         unsigned long long* __restrict__ best) {
     uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
+    if (best[blockIdx.y] < base) return;
     const uint4 p0 = prefix[2 * blockIdx.y], p1 = prefix[2 * blockIdx.y + 1];
     pow_try(init, p0, p1, bits, base + i, best + blockIdx.y);
 }

@@ -695,8 +695,9 @@ __global__ __launch_bounds__(256) void eval_columns_kernel(const uint32_t* const
     const uint64_t boff = (uint64_t)blockIdx.y * bstride;  // proof blockIdx.y of a batch
     w = boffp(w, boff);
     part = boffp(part, boff);
-    const uint32_t k = blockIdx.x / EV_XCDS;
-    const uint32_t chunk = (k / groups) * EV_XCDS + blockIdx.x % EV_XCDS;
+    const uint32_t per = chunks < EV_XCDS ? chunks : EV_XCDS;  // fewer chunks than XCDs: no idle blocks
+    const uint32_t k = blockIdx.x / per;
+    const uint32_t chunk = (k / groups) * per + blockIdx.x % per;
     if (chunk >= chunks) return;
     const uint32_t c0 = (k % groups) * EV_CB;
     const uint32_t nc = n_cols - c0 < EV_CB ? n_cols - c0 : EV_CB;
@@ -797,7 +798,8 @@ void launch_eval_columns_into(Ctx& c, const uint32_t* const* d_cols, uint32_t n_
     const uint32_t groups = (n_cols + EV_CB - 1) / EV_CB;
     const uint32_t nw = n_cols * 4;
     KTimer kt(c, "eval_columns", batch * (4.0 * n_cols + 16.0) * n);
-    const uint32_t blocks = (chunks + EV_XCDS - 1) / EV_XCDS * EV_XCDS * groups;
+    const uint32_t per = chunks < EV_XCDS ? chunks : EV_XCDS;
+    const uint32_t blocks = (chunks + per - 1) / per * per * groups;
     hipLaunchKernelGGL(eval_columns_kernel, dim3(blocks, batch), dim3(256), 0, c.stream, d_cols, n_cols, log_size,
                        (const uint4*)d_w, chunks, groups, d_part, bstride);
     hipLaunchKernelGGL(eval_reduce_kernel, dim3((nw + 3) / 4, batch), dim3(256), 0, c.stream,

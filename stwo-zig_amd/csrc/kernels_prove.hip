@@ -676,7 +676,7 @@ void launch_eval_weights_dev(Ctx& c, const uint32_t* d_factors, uint32_t log_siz
 // take all column groups of one chunk in a row, and the chunk's weights are read from HBM
 // once and then served by that XCD's L2.
 constexpr uint32_t EV_PER_THREAD = 16;
-constexpr uint32_t EV_CB = 8;
+constexpr uint32_t EV_CB = 4;
 constexpr uint32_t EV_XCDS = 8;
 __device__ __forceinline__ uint64_t ev_wave_sum(uint64_t v) {
 #pragma unroll

@@ -170,6 +170,11 @@ struct GdState {
     uint32_t craw[GD_MAX_INST][4][4];
     uint32_t pend, pend_rho, pad1, pad2;
     uint32_t pend_flags[GD_MAX_INST];  // bit 0: active, bit 1: summed in that round
+    // the current round's coefficient c_q of instance i as A * s0 + C * s2 + B in the round
+    // sums s0, s2 (set at the round start from the claim, the correction and the round's
+    // basis rows), and the instance's flags in this round (bit 0: active, bit 1: summed)
+    uint32_t fin[GD_MAX_INST][4][3][4];
+    uint32_t rflags[GD_MAX_INST];
 };
 static_assert(sizeof(GdState) % 8 == 0, "state copy in 8-byte words");
 constexpr uint32_t GD_MAX_BX = 1024;   // workgroups per instance in a round launch
@@ -300,30 +305,67 @@ __device__ __forceinline__ GdSrc gd_src(const GdInst& in, uint32_t onv, uint32_t
     return s;
 }
 
-// The pending claim update of the previous round: sc_claim_i = rp_i(r) and, for the
-// instances summed in that round, corr_i *= eq(r, y_rho).  One full wave; lane t handles
-// instance t>>2, coefficient t&3.
-__device__ __forceinline__ void gd_apply_pending(GdState* S, uint32_t n_inst) {
-    if (!S->pend) return;
+// Round start: the previous round's pending claim update (sc_claim_i = rp_i(r) and, for the
+// instances summed in that round, corr_i *= eq(r, y_rho)), then the finishing coefficients
+// of this round.  With y0 = s0 corr F0, y2 = s2 corr F2 and
+// y1 = claim - y0, the round polynomial's coefficient q of an instance summed in the round is
+// y0 for q = 0 and b1 y0 + b2 y1 + b3 y2 = (b1 - b2) y0 + b2 claim + b3 y2 for q = 1..3 (basis
+// rows b of the round), so A, B, C depend only on values known before the sums.  One full
+// wave; lane t handles instance t>>2, coefficient t&3; the quad's lanes share the instance's
+// claim and correction in registers.
+__device__ __forceinline__ void gd_round_prep(GdState* S, const GdInst* I, uint32_t n_inst, uint32_t n_layers,
+                                              uint32_t L, uint32_t rho) {
     const uint32_t t = threadIdx.x & 63;
     const uint32_t i = t >> 2, q = t & 3;
-    const qm31 r = qload(S->r);
-    const qm31 craw = i < n_inst ? qload(S->craw[i][q]) : qzero();
-    qm31 pw = qone();
-    if (q >= 1) pw = r;
-    if (q >= 2) pw = qmul(pw, r);
-    if (q >= 3) pw = qmul(pw, r);
-    qm31 v = qmul(craw, pw);
-    v = qadd(v, gd_qperm<QP_ROT2>(v));
-    v = qadd(v, gd_qperm<QP_ROT1>(v));
-    if (i < n_inst && q == 0 && (S->pend_flags[i] & 1)) {
-        gd_qstore(S->sc_claim[i], v);
-        if (S->pend_flags[i] & 2) {
-            const qm31 y = qload(S->ood[S->pend_rho]);
-            const qm31 e = qadd(qmul(r, y), qmul(qsub(qone(), r), qsub(qone(), y)));
-            gd_qstore(S->corr[i], qmul(qload(S->corr[i]), e));
+    const bool in_range = i < n_inst;
+    qm31 sc = in_range ? qload(S->sc_claim[i]) : qzero();
+    qm31 corr = in_range ? qload(S->corr[i]) : qzero();
+    if (S->pend) {
+        const qm31 r = qload(S->r);
+        const qm31 craw = in_range ? qload(S->craw[i][q]) : qzero();
+        qm31 pw = qone();
+        if (q >= 1) pw = r;
+        if (q >= 2) pw = qmul(pw, r);
+        if (q >= 3) pw = qmul(pw, r);
+        qm31 v = qmul(craw, pw);
+        v = qadd(v, gd_qperm<QP_ROT2>(v));
+        v = qadd(v, gd_qperm<QP_ROT1>(v));  // every lane of the quad: rp_i(r)
+        const uint32_t fl = in_range ? S->pend_flags[i] : 0u;
+        if (fl & 1) {
+            sc = v;
+            if (fl & 2) {
+                const qm31 y = qload(S->ood[S->pend_rho]);
+                const qm31 e = qadd(qmul(r, y), qmul(qsub(qone(), r), qsub(qone(), y)));
+                corr = qmul(corr, e);
+            }
+            if (q == 0) {
+                gd_qstore(S->sc_claim[i], sc);
+                if (fl & 2) gd_qstore(S->corr[i], corr);
+            }
         }
     }
+    if (!in_range) return;
+    const GdView vw = gd_view(I[i], n_layers, L);
+    const bool act = vw.active, part = act && vw.onv >= L - rho;
+    if (q == 0) S->rflags[i] = (act ? 1u : 0u) | (part ? 2u : 0u);
+    if (!part) return;
+    const uint32_t(*pr)[4] = S->pre[rho];
+    const qm31 c0 = qmul(corr, qload(pr[0])), c2 = qmul(corr, qload(pr[1]));
+    qm31 A, B, C;
+    if (q == 0) {
+        A = c0;
+        B = qzero();
+        C = qzero();
+    } else {
+        const uint32_t b = 2 + 3 * (q - 1);
+        const qm31 b1 = qload(pr[b]), b2 = qload(pr[b + 1]), b3 = qload(pr[b + 2]);
+        A = qmul(qsub(b1, b2), c0);
+        B = qmul(b2, sc);
+        C = qmul(b3, c2);
+    }
+    gd_qstore(S->fin[i][q][0], A);
+    gd_qstore(S->fin[i][q][1], B);
+    gd_qstore(S->fin[i][q][2], C);
 }
 
 // Turns the round sums (sums[8 i + c]: r(0) then r(2) coordinates before the eq
@@ -335,35 +377,24 @@ __device__ __forceinline__ void gd_finish(GdState* S, const GdInst* I, uint32_t 
     const uint32_t t = threadIdx.x;
     const uint32_t i = t >> 2, q = t & 3;
     const uint32_t rem = L - rho;
+    (void)I;
+    (void)n_layers;
+    (void)rem;
     qm31 craw = qzero();
-    bool act = false, part = false;
     if (i < n_inst) {
-        const GdView v = gd_view(I[i], n_layers, L);
-        act = v.active;
-        part = act && v.onv >= rem;
-        if (act) {
-            const qm31 sc = qload(S->sc_claim[i]);
-            if (part) {
-                const qm31 corr = qload(S->corr[i]);
-                const unsigned long long* a = sums + 8 * i;
-                const qm31 s0 = {{mred64(a[0]), mred64(a[1])}, {mred64(a[2]), mred64(a[3])}};
-                const qm31 s2 = {{mred64(a[4]), mred64(a[5])}, {mred64(a[6]), mred64(a[7])}};
-                const uint32_t(*pr)[4] = S->pre[rho];
-                const qm31 y0 = qmul(qmul(s0, corr), qload(pr[0]));
-                const qm31 y2 = qmul(qmul(s2, corr), qload(pr[1]));
-                const qm31 y1 = qsub(sc, y0);
-                if (q == 0) {
-                    craw = y0;
-                } else {
-                    const uint32_t b = 2 + 3 * (q - 1);
-                    craw = qadd(qadd(qmul(qload(pr[b]), y0), qmul(qload(pr[b + 1]), y1)), qmul(qload(pr[b + 2]), y2));
-                }
-            } else if (q == 0) {
-                craw = qmulm(sc, (P + 1) / 2);
-            }
+        const uint32_t fl = S->rflags[i];  // set by gd_round_prep for this round
+        const bool act = fl & 1, part = fl & 2;
+        if (part) {
+            const unsigned long long* a = sums + 8 * i;
+            const qm31 s0 = {{mred64(a[0]), mred64(a[1])}, {mred64(a[2]), mred64(a[3])}};
+            const qm31 s2 = {{mred64(a[4]), mred64(a[5])}, {mred64(a[6]), mred64(a[7])}};
+            craw = qadd(qadd(qmul(qload(S->fin[i][q][0]), s0), qmul(qload(S->fin[i][q][2]), s2)),
+                        qload(S->fin[i][q][1]));
+        } else if (act && q == 0) {
+            craw = qmulm(qload(S->sc_claim[i]), (P + 1) / 2);
         }
         gd_qstore(S->craw[i][q], craw);
-        if (q == 0) S->pend_flags[i] = (act ? 1u : 0u) | (part ? 2u : 0u);
+        if (q == 0) S->pend_flags[i] = fl;
     }
     qm31 cv = i < n_inst ? qmul(craw, qload(S->alpha_pow[i])) : qzero();
 #pragma unroll
@@ -630,7 +661,7 @@ __global__ __launch_bounds__(GD_FT) void gkr_fused_kernel(GdState* __restrict__ 
             const uint64_t nt = 1ull << (rem - 1);
             const uint32_t b = rc & 1;
             if (wv == 7) {
-                gd_apply_pending(&S, n_inst);
+                gd_round_prep(&S, I, n_inst, n_layers, L, rho);
                 (&red[b ^ 1][0][0])[ln] = 0;
             } else {
                 // (instance, term) pairs over the 112 quads of waves 0..6; pair x = p * nt + k for
@@ -718,7 +749,7 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
             break;
         }
     }
-    if (blockIdx.x == 0 && blockIdx.y == 0 && t < 64) gd_apply_pending(S, n_inst);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && t < 64) gd_round_prep(S, I, n_inst, n_layers, L, rho);
     const GdView v = gd_view(I[i], n_layers, L);
     uint64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     {

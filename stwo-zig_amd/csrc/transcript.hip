@@ -26,17 +26,31 @@ __device__ void qchan_mix_long(const QuadLane& L, uint32_t dg[8], const uint32_t
     uint32_t hhi = L.sel(SR_B2_IV4, SR_B2_IV5, SR_B2_IV6, SR_B2_IV7);
     const uint32_t total = 8 + n;  // words
     const uint32_t nblk = (total + 15) / 16;
-    uint32_t m[16];
+    // Block b >= 1 holds words [16 b - 8, 16 b + 8) of `words`.  The next block's words are
+    // loaded while a block is compressed (the chain waits on no load), and every block before
+    // the last is full, so only the last one checks the bounds.
+    auto load = [&](uint32_t b, uint32_t(&dst)[16]) {
+        const uint32_t o = 16 * b - 8;
+        if (b + 1 < nblk) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) dst[i] = words[o + i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; i++) dst[i] = o + i < n ? words[o + i] : 0u;
+        }
+    };
+    uint32_t m[16], nx[16];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         m[i] = dg[i];
         m[8 + i] = (uint32_t)i < n ? words[i] : 0u;
     }
+    if (nblk > 1) load(1, nx);
     b2_compress_quad(L, hlo, hhi, m, nblk == 1 ? 4 * total : 64, nblk == 1);
     for (uint32_t b = 1; b < nblk; b++) {
-        const uint32_t o = 16 * b - 8;  // first word of block b in `words`
 #pragma unroll
-        for (int i = 0; i < 16; i++) m[i] = o + i < n ? words[o + i] : 0u;
+        for (int i = 0; i < 16; i++) m[i] = nx[i];
+        if (b + 1 < nblk) load(b + 1, nx);
         const bool last = b + 1 == nblk;
         b2_compress_quad(L, hlo, hhi, m, last ? 4 * total : 64 * (b + 1), last);
     }

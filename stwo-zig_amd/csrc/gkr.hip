@@ -727,8 +727,9 @@ __global__ __launch_bounds__(GD_RT) void gkr_begin_eq_kernel(GdState* __restrict
 // sums in its own slot of `part` and counts itself done on one of 16 sub-counters (separate
 // cache lines; the workgroup completing a sub-counter counts on the top counter), so no
 // address takes more than ~nb/16 atomics; the last workgroup adds the slots up and finishes
-// the round.  Workgroup (0, 0) applies the previous round's pending claim update first; the
-// release fence before its count (the only one) publishes it to the last workgroup.
+// the round.  Workgroup (x = last, y = 0) sums nothing: it applies the previous round's pending
+// claim update and sets this round's finishing coefficients (gd_round_prep) while the others
+// sum; the release fence before its count (the only one) publishes them to the last workgroup.
 __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ S, const GdInst* __restrict__ I,
                                                           uint32_t n_inst, uint32_t n_layers, uint32_t L, uint32_t rho,
                                                           const uint32_t* __restrict__ eq, uint32_t* __restrict__ rec,
@@ -749,14 +750,18 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
             break;
         }
     }
-    if (blockIdx.x == 0 && blockIdx.y == 0 && t < 64) gd_round_prep(S, I, n_inst, n_layers, L, rho);
+    // the last workgroup column runs the round-start prep (instance row 0) instead of sums, so
+    // the prep overlaps the sums rather than delaying one summing workgroup
+    const uint32_t nsb = gridDim.x - 1;
+    const bool prep_blk = blockIdx.x == nsb;
+    if (prep_blk && blockIdx.y == 0 && t < 64) gd_round_prep(S, I, n_inst, n_layers, L, rho);
     const GdView v = gd_view(I[i], n_layers, L);
     uint64_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    {
+    if (!prep_blk) {
         const GdSrc s = gd_src(I[i], v.onv, L, rho);
         const qm31 lambda = qload(S->lambda), r = qload(S->r);
         const uint32_t j = t & 3;
-        for (uint64_t k = (uint64_t)blockIdx.x * (GD_RT / 4) + (t >> 2); k < nt; k += (uint64_t)gridDim.x * (GD_RT / 4)) {
+        for (uint64_t k = (uint64_t)blockIdx.x * (GD_RT / 4) + (t >> 2); k < nt; k += (uint64_t)nsb * (GD_RT / 4)) {
             const qm31 y = gd_term_quad(s.fix, s.src_kind, s.num, s.den, I[i].scr_num, I[i].scr_den, r, nt, k, lambda,
                                         eq, j);
             if (j == 0) {
@@ -774,8 +779,8 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
     __syncthreads();
     // The slot is written with device-coherent stores (agent-scope atomic stores) by wave 0,
     // which waits for their acknowledgement before thread 0 counts the workgroup: no L2
-    // writeback per workgroup.  Workgroup (0, 0) also wrote the pending claim update into the
-    // state with plain stores and publishes it with one release; the last workgroup acquires.
+    // writeback per workgroup.  The prep workgroup wrote the state with plain stores and
+    // publishes it with one release; the last workgroup acquires.
     if (t < 8) {
         uint64_t w = 0;
         for (uint32_t k = 0; k < GD_RT / 64; k++) w += red[k][t];
@@ -785,7 +790,7 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
     if (t < 64) {
         __builtin_amdgcn_s_waitcnt(0);
         if (t == 0) {
-            if (blockIdx.x == 0 && blockIdx.y == 0) __threadfence();
+            if (prep_blk && blockIdx.y == 0) __threadfence();
             const uint32_t nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
             const uint32_t g = b % GD_CNT_GROUPS, ng = nb < GD_CNT_GROUPS ? nb : GD_CNT_GROUPS;
             const uint32_t gsize = (nb - g + GD_CNT_GROUPS - 1) / GD_CNT_GROUPS;
@@ -797,7 +802,9 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
+    // acquire: the slots came with device-coherent stores and the prep's state with its
+    // workgroup's release; this workgroup has nothing of its own to write back
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (t <= GD_CNT_GROUPS) atomicExch(&cnt[GD_CNT_STRIDE * t], 0u);
     for (uint32_t ii = 0, py = 0; ii < n_inst; ii++) {
         const GdView vi = gd_view(I[ii], n_layers, L);
@@ -947,7 +954,8 @@ GkrRun* gkr_launch_on_channel(Ctx& c, DevChannel* d_ch, const std::vector<GkrIns
         uint32_t rho = 0;
         for (; L - rho >= GD_SMALL_LOG; rho++) {
             const uint64_t nt = 1ull << (L - rho - 1);
-            const uint64_t bx = std::min<uint64_t>((nt + GD_RT / 4 - 1) / (GD_RT / 4), GD_MAX_BX);
+            // summing workgroups per instance, plus one column for the round-start prep
+            const uint64_t bx = std::min<uint64_t>((nt + GD_RT / 4 - 1) / (GD_RT / 4), GD_MAX_BX - 1) + 1;
             uint32_t n_part = 0;  // instances summed in this round: active with onv >= L - rho
             for (uint32_t k = 0; k < n_inst; k++)
                 n_part += inst[k].n_vars + L >= n_layers && inst[k].n_vars + L - n_layers >= L - rho;

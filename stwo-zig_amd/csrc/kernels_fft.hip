@@ -327,6 +327,15 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4]
                     v[4 * k + 2] = w.z;
                     v[4 * k + 3] = w.w;
                 }
+            } else if (INV || ps.src_log >= ps.m) {
+                // a full source (every inverse pass, every forward pass after the first):
+                // no row is past its end, so the loads take no bound check
+                const uint32_t sh = B + ps.s_lo, voff = g0 << 2;
+#pragma unroll
+                for (int k = 0; k < RX; k++) {
+                    const SR_GLOBAL uint8_t* sk = (const SR_GLOBAL uint8_t*)src + ((uint64_t)k << (sh + 2));
+                    v[k] = *(gcu32*)(sk + voff);
+                }
             } else {
                 const uint32_t sh = B + ps.s_lo, voff = g0 << 2;
 #pragma unroll

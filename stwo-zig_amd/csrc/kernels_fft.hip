@@ -294,12 +294,23 @@ struct RxGroupShape {
 
 // w holds this group's twiddles (loaded before the previous exchange); the next group's
 // twiddles are loaded before this group's exchange so their latency overlaps it.
+// A row's uniform address kept in SGPRs and hidden from reassociation (an empty asm, no
+// instruction).  Otherwise the compiler adds the lane offset to the column pointer once, in a
+// 64-bit VGPR base, and pays one 64-bit VALU add per access for the row offsets; this way
+// every strided access takes the SGPR-base + 32-bit lane offset form.
+template <class T>
+__device__ __forceinline__ T* rx_sbase(T* p) {
+    uint64_t v = (uint64_t)p;
+    asm("" : "+s"(v));
+    return (T*)v;
+}
+
 // FROM_REGS: the first group takes its 16 values from v instead of loading them (the fused
 // interpolate + extend kernel hands the coefficients over in registers).
 // SP (compile time): an inverse pass scales on store (ps.scale), a forward pass has a
 // zero-padded top stage (ps.dup_top): its values 8..15 are the zero partners, never read.
-// Strided HBM accesses address the uniform column pointer plus a uniform row offset with a
-// 32-bit per-lane byte offset (words < 2^30), which the SGPR-base global forms take as is.
+// Strided HBM accesses address the uniform column pointer plus a uniform row offset (rx_sbase)
+// with a 32-bit per-lane byte offset (words < 2^30), which the SGPR-base global forms take as is.
 template <bool INV, int R, int LOGC, int GI, bool SP, bool FROM_REGS = false>
 __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4][8], uint32_t* lds,
                                        gcu32* __restrict__ src, gu32* __restrict__ dst,
@@ -333,7 +344,7 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4]
                 const uint32_t sh = B + ps.s_lo, voff = g0 << 2;
 #pragma unroll
                 for (int k = 0; k < RX; k++) {
-                    const SR_GLOBAL uint8_t* sk = (const SR_GLOBAL uint8_t*)src + ((uint64_t)k << (sh + 2));
+                    const SR_GLOBAL uint8_t* sk = rx_sbase((const SR_GLOBAL uint8_t*)src + ((uint64_t)k << (sh + 2)));
                     v[k] = *(gcu32*)(sk + voff);
                 }
             } else {
@@ -374,7 +385,7 @@ __device__ __forceinline__ void rx_run(uint32_t (&v)[RX], const uint32_t (&w)[4]
                 const uint32_t sh = B + ps.s_lo, voff = g0 << 2;
 #pragma unroll
                 for (int k = 0; k < RX; k++)
-                    *(gu32*)((SR_GLOBAL uint8_t*)dst + ((uint64_t)k << (sh + 2)) + voff) = v[k];
+                    *(gu32*)(rx_sbase((SR_GLOBAL uint8_t*)dst + ((uint64_t)k << (sh + 2))) + voff) = v[k];
             }
         }
     }

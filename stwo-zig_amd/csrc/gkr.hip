@@ -211,12 +211,35 @@ template <int CTRL>
 __device__ __forceinline__ qm31 gd_qperm(qm31 v) {
     return {{quad_perm<CTRL>(v.c0.a), quad_perm<CTRL>(v.c0.b)}, {quad_perm<CTRL>(v.c1.a), quad_perm<CTRL>(v.c1.b)}};
 }
-__device__ __forceinline__ uint64_t gd_wave_sum(uint64_t v) {
+// The 8 sums of a wave by recursive halving: at each exchange a lane keeps half of its values
+// and adds its partner's copy of that half (lane bit 5 picks the half first, then bits 4 and
+// 3), then three full exchanges.  6 dependent rounds of word exchanges where 8 separate
+// reductions take 48; the rounds sit on the latency path of every round.  Afterwards lane l
+// holds the total of value (l >> 3) & 7.
+__device__ __forceinline__ uint64_t gd_xchg(uint64_t v, uint32_t addr) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute((int)addr, (int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute((int)addr, (int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t gd_wave_sum8(uint64_t (&x)[8], uint32_t lane) {
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m), hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m);
-        v += ((uint64_t)hi << 32) | lo;
+    for (int st = 0; st < 3; st++) {
+        const uint32_t h = 4u >> st, b = 32u >> st;
+        // the halves are picked with a lane mask, not a select of two elements (which the
+        // compiler turns into an indexed read of the whole array)
+        const uint64_t m = 0ull - (uint64_t)((lane & b) != 0);  // all ones: keep the upper half
+        const uint32_t addr = (lane ^ b) << 2;
+#pragma unroll
+        for (uint32_t i = 0; i < h; i++) {
+            const uint64_t lo = x[i], hi = x[i + h];
+            const uint64_t send = (lo & m) | (hi & ~m);
+            x[i] = (lo ^ hi ^ send) + gd_xchg(send, addr);
+        }
     }
+    uint64_t v = x[0];
+    v += gd_xchg(v, (lane ^ 4) << 2);
+    v += gd_xchg(v, (lane ^ 2) << 2);
+    v += gd_xchg(v, (lane ^ 1) << 2);
     return v;
 }
 __device__ __forceinline__ void gd_qstore(uint32_t* w, qm31 v) { qstore(v, w); }
@@ -771,10 +794,9 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
             }
         }
     }
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const uint64_t w = gd_wave_sum(a[c]);
-        if ((t & 63) == 0) red[t >> 6][c] = w;
+    {
+        const uint64_t w = gd_wave_sum8(a, t & 63);
+        if ((t & 7) == 0) red[t >> 6][(t & 63) >> 3] = w;
     }
     __syncthreads();
     // The slot is written with device-coherent stores (agent-scope atomic stores) by wave 0,
@@ -816,10 +838,9 @@ __global__ __launch_bounds__(GD_RT) void gkr_round_kernel(GdState* __restrict__ 
 #pragma unroll
             for (int c = 0; c < 8; c++) b8[c] += __hip_atomic_load(&pp[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const uint64_t w = gd_wave_sum(b8[c]);
-            if ((t & 63) == 0) red[t >> 6][c] = w;
+        {
+            const uint64_t w = gd_wave_sum8(b8, t & 63);
+            if ((t & 7) == 0) red[t >> 6][(t & 63) >> 3] = w;
         }
         __syncthreads();
         if (t < 8) {

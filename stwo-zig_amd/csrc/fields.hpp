@@ -85,10 +85,12 @@ struct cm31 {
 SR_HD cm31 cadd(cm31 x, cm31 y) { return {madd(x.a, y.a), madd(x.b, y.b)}; }
 SR_HD cm31 csub(cm31 x, cm31 y) { return {msub(x.a, y.a), msub(x.b, y.b)}; }
 SR_HD cm31 cneg(cm31 x) { return {mneg(x.a), mneg(x.b)}; }
+// (a + bi)(c + di) = (ac - bd) + (ad + bc)i: each coordinate is a sum of two raw 62-bit
+// products (the subtraction as b (p - d)) reduced once.
 SR_HD cm31 cmul(cm31 x, cm31 y) {
-    uint32_t ac = mmul(x.a, y.a), bd = mmul(x.b, y.b);
-    uint32_t cr = mmul(madd(x.a, x.b), madd(y.a, y.b));
-    return {msub(ac, bd), msub(msub(cr, ac), bd)};
+    const uint64_t re = (uint64_t)x.a * y.a + (uint64_t)x.b * (P - y.b);
+    const uint64_t im = (uint64_t)x.a * y.b + (uint64_t)x.b * y.a;
+    return {mred64(re), mred64(im)};
 }
 SR_HD cm31 cmulm(cm31 x, uint32_t m) { return {mmul(x.a, m), mmul(x.b, m)}; }
 SR_HD cm31 cinv(cm31 x) {
@@ -106,10 +108,27 @@ SR_HD qm31 qfromm(uint32_t m) { return {{m, 0}, {0, 0}}; }
 SR_HD qm31 qadd(qm31 x, qm31 y) { return {cadd(x.c0, y.c0), cadd(x.c1, y.c1)}; }
 SR_HD qm31 qsub(qm31 x, qm31 y) { return {csub(x.c0, y.c0), csub(x.c1, y.c1)}; }
 SR_HD qm31 qneg(qm31 x) { return {cneg(x.c0), cneg(x.c1)}; }
+// QM31 = CM31[u] / (u^2 - (2 + i)).  With x = (a0 + b0 i) + (a1 + b1 i) u and
+// y = (c0 + d0 i) + (c1 + d1 i) u, x y = x0 y0 + (2 + i) x1 y1 + (x0 y1 + x1 y0) u, whose four
+// coordinates, grouped by the words of x, are
+//   r0 = a0 c0 - b0 d0 + a1 (2 c1 - d1) - b1 (c1 + 2 d1)
+//   r1 = a0 d0 + b0 c0 + a1 (c1 + 2 d1) + b1 (2 c1 - d1)
+//   r2 = a0 c1 - b0 d1 + a1 c0 - b1 d0
+//   r3 = a0 d1 + b0 c1 + a1 d0 + b1 c0.
+// Negated factors are taken as p - v (at most p), so each coordinate is four raw products
+// (< 2^62 each, < 2^64 together) reduced once: 16 multiply-adds and 4 reductions instead of
+// 9 reduced products and 15 reduced additions of the Karatsuba form.
 SR_HD qm31 qmul(qm31 x, qm31 y) {
-    cm31 ac = cmul(x.c0, y.c0), bd = cmul(x.c1, y.c1);
-    cm31 cr = csub(csub(cmul(cadd(x.c0, x.c1), cadd(y.c0, y.c1)), ac), bd);
-    return {cadd(ac, cmul_r(bd)), cr};
+    const uint32_t a0 = x.c0.a, b0 = x.c0.b, a1 = x.c1.a, b1 = x.c1.b;
+    const uint32_t c0 = y.c0.a, d0 = y.c0.b, c1 = y.c1.a, d1 = y.c1.b;
+    const uint32_t nd0 = P - d0, nd1 = P - d1;
+    const uint32_t t = madd(madd(c1, c1), nd1);  // 2 c1 - d1
+    const uint32_t s = madd(c1, madd(d1, d1));   // c1 + 2 d1
+    const uint64_t r0 = (uint64_t)a0 * c0 + (uint64_t)b0 * nd0 + (uint64_t)a1 * t + (uint64_t)b1 * (P - s);
+    const uint64_t r1 = (uint64_t)a0 * d0 + (uint64_t)b0 * c0 + (uint64_t)a1 * s + (uint64_t)b1 * t;
+    const uint64_t r2 = (uint64_t)a0 * c1 + (uint64_t)b0 * nd1 + (uint64_t)a1 * c0 + (uint64_t)b1 * nd0;
+    const uint64_t r3 = (uint64_t)a0 * d1 + (uint64_t)b0 * c1 + (uint64_t)a1 * d0 + (uint64_t)b1 * c0;
+    return {{mred64(r0), mred64(r1)}, {mred64(r2), mred64(r3)}};
 }
 SR_HD qm31 qmulm(qm31 x, uint32_t m) { return {cmulm(x.c0, m), cmulm(x.c1, m)}; }
 SR_HD qm31 qmulc(qm31 x, cm31 c) { return {cmul(x.c0, c), cmul(x.c1, c)}; }

@@ -72,11 +72,22 @@ SR_HD uint32_t mpow(uint32_t a, uint32_t e) {
     return acc;
 }
 // a^(p-2); p-2 = 2^31 - 3.  Inverse is unique, so any exponentiation chain is exact.
+// Addition chain over the exponents 2^k - 1 (x_k = a^(2^k - 1), x_{j+k} = x_j^(2^k) x_k):
+// x2, x4, x8, x16, x24, x28, x29, then a^(2^31 - 3) = x29^4 a.  30 squarings and 8
+// multiplications.
+SR_HD uint32_t msqrn(uint32_t a, int n) {
+    for (int i = 0; i < n; i++) a = msqr(a);
+    return a;
+}
 SR_HD uint32_t minv(uint32_t a) {
-    // a^(2^k - 1) ladder: t = a^(2^29-1), result = t^4 * a  (2^31-3 = (2^29-1)*4 + 1)
-    uint32_t t = a;  // a^(2^1-1)
-    for (int i = 1; i < 29; i++) t = mmul(msqr(t), a);  // a^(2^(i+1)-1)
-    return mmul(msqr(msqr(t)), a);
+    const uint32_t x2 = mmul(msqr(a), a);
+    const uint32_t x4 = mmul(msqrn(x2, 2), x2);
+    const uint32_t x8 = mmul(msqrn(x4, 4), x4);
+    const uint32_t x16 = mmul(msqrn(x8, 8), x8);
+    const uint32_t x24 = mmul(msqrn(x16, 8), x8);
+    const uint32_t x28 = mmul(msqrn(x24, 4), x4);
+    const uint32_t x29 = mmul(msqr(x28), a);
+    return mmul(msqrn(x29, 2), a);
 }
 
 struct cm31 {

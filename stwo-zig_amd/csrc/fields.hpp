@@ -128,7 +128,8 @@ SR_HD qm31 qneg(qm31 x) { return {cneg(x.c0), cneg(x.c1)}; }
 //   r3 = a0 d1 + b0 c1 + a1 d0 + b1 c0.
 // Negated factors are taken as p - v (at most p), so each coordinate is four raw products
 // (< 2^62 each, < 2^64 together) reduced once: 16 multiply-adds and 4 reductions instead of
-// 9 reduced products and 15 reduced additions of the Karatsuba form.
+// 9 reduced products and 15 reduced additions of the Karatsuba form.  The combinations of
+// y's words (t, s, the negations) come first, so a uniform or loop-invariant factor goes in y.
 SR_HD qm31 qmul(qm31 x, qm31 y) {
     const uint32_t a0 = x.c0.a, b0 = x.c0.b, a1 = x.c1.a, b1 = x.c1.b;
     const uint32_t c0 = y.c0.a, d0 = y.c0.b, c1 = y.c1.a, d1 = y.c1.b;

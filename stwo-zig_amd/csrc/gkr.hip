@@ -276,8 +276,8 @@ __device__ __forceinline__ qm31 gd_term_quad(bool fix, uint32_t kind, const uint
     const qm31 db = fix ? gd_ld(den, jj + h) : qzero();
     const qm31 na = has_num ? gd_num(kind, num, jj) : qone();
     const qm31 nb = has_num && fix ? gd_num(kind, num, jj + h) : qone();
-    const qm31 d = fix ? qadd(qmul(r, qsub(db, da)), da) : da;
-    const qm31 n = has_num && fix ? qadd(qmul(r, qsub(nb, na)), na) : na;
+    const qm31 d = fix ? qadd(qmul(qsub(db, da), r), da) : da;
+    const qm31 n = has_num && fix ? qadd(qmul(qsub(nb, na), r), na) : na;
     if (fix) {
         gd_st(dden, jj, d);
         if (has_num) gd_st(dnum, jj, n);

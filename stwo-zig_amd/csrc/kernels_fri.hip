@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void fold_line_kernel(const uint4* __restrict_
     if (dch) alpha = qload(dch->alpha);
     uint4 a = src[2 * i], b = src[2 * i + 1];
     qm31 f0 = {{a.x, a.y}, {a.z, a.w}}, f1 = {{b.x, b.y}, {b.z, b.w}};
-    qm31 r = qadd(qadd(f0, f1), qmul(alpha, qmulm(qsub(f0, f1), inv_x[i])));
+    qm31 r = qadd(qadd(f0, f1), qmul(qmulm(qsub(f0, f1), inv_x[i]), alpha));
     dst[i] = make_uint4(r.c0.a, r.c0.b, r.c1.a, r.c1.b);
     if (leaves) leaf_qm31(leaves, i, seed, r);
 }
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void fold_circle_kernel(const uint32_t* __rest
     uint2 w0 = ((const uint2*)c0)[i], w1 = ((const uint2*)c1)[i], w2 = ((const uint2*)c2)[i],
           w3 = ((const uint2*)c3)[i];
     qm31 f0 = {{w0.x, w1.x}, {w2.x, w3.x}}, f1 = {{w0.y, w1.y}, {w2.y, w3.y}};
-    qm31 r = qadd(qadd(f0, f1), qmul(alpha, qmulm(qsub(f0, f1), inv_y[i])));
+    qm31 r = qadd(qadd(f0, f1), qmul(qmulm(qsub(f0, f1), inv_y[i]), alpha));
     if (accumulate) {
         uint4 d = dst[i];
         r = qadd(qmul(qm31{{d.x, d.y}, {d.z, d.w}}, alpha2), r);
@@ -172,7 +172,7 @@ __device__ __forceinline__ void tr_leaf_lane(const B2State& seed, qm31 v, uint32
 
 __device__ __forceinline__ qm31 tr_fold(uint4 a, uint4 b, qm31 alpha, uint32_t inv) {
     const qm31 f0 = {{a.x, a.y}, {a.z, a.w}}, f1 = {{b.x, b.y}, {b.z, b.w}};
-    return qadd(qadd(f0, f1), qmul(alpha, qmulm(qsub(f0, f1), inv)));
+    return qadd(qadd(f0, f1), qmul(qmulm(qsub(f0, f1), inv), alpha));
 }
 
 // Fold input i of a layer kernel: line (fold_line_kernel) or circle (fold_circle_kernel,

@@ -227,8 +227,8 @@ __global__ __launch_bounds__(LG_THREADS) void gkr_sum_kernel(int kind, const uin
             qm31 n00 = layer_num(kind, num, 2 * i), n01 = layer_num(kind, num, 2 * i + 1);
             qm31 n10 = layer_num(kind, num, 2 * (nt + i)), n11 = layer_num(kind, num, 2 * (nt + i) + 1);
             qm31 n20 = qsub(qadd(n10, n10), n00), n21 = qsub(qadd(n11, n11), n01);
-            t0 = qadd(qadd(qmul(d01, n00), qmul(d00, n01)), qmul(lambda, qmul(d00, d01)));
-            t2 = qadd(qadd(qmul(d21, n20), qmul(d20, n21)), qmul(lambda, qmul(d20, d21)));
+            t0 = qadd(qadd(qmul(d01, n00), qmul(d00, n01)), qmul(qmul(d00, d01), lambda));
+            t2 = qadd(qadd(qmul(d21, n20), qmul(d20, n21)), qmul(qmul(d20, d21), lambda));
         }
         qm31 e = ld4(eq, i);
         qm31 v0 = qmul(e, t0), v2 = qmul(e, t2);
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(LG_THREADS) void mle_fix_first_kernel(int base, con
     if (i >= half) return;
     qm31 e0 = base ? qfromm(in[i]) : ld4((const uint4*)in, i);
     qm31 e1 = base ? qfromm(in[i + half]) : ld4((const uint4*)in, i + half);
-    st4(out, i, qadd(qmul(r, qsub(e1, e0)), e0));
+    st4(out, i, qadd(qmul(qsub(e1, e0), r), e0));
 }
 
 // Table columns of the LogUp workload on n >= 2^16 rows: mult[i] for i < 2^16 (0 beyond) and the

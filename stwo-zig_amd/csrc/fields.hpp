@@ -2,9 +2,11 @@
 // host and device code.
 //
 // Field laws follow src/core/fields/m31.zig:49-171 (canonical [0,p) words,
-// one-fold product reduction), cm31.zig:97-129 (Karatsuba) and
-// qm31.zig:13,153-222,269-284 (u^2 = 2+i, mulByR = (2a-b, a+2b)).  The circle
-// group follows src/core/circle.zig:8-374.
+// one-fold product reduction), cm31.zig:97-129 and qm31.zig:13,153-222,269-284
+// (u^2 = 2+i, mulByR = (2a-b, a+2b)).  The reference multiplies CM31 by Karatsuba;
+// here the CM31 and QM31 products are written out as raw products reduced once per
+// coordinate (the same field elements).  The circle group follows
+// src/core/circle.zig:8-374.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
